@@ -1,0 +1,694 @@
+// dcm_advect.hip -- MPDATA advection (pamm/sfvt/advection_3d.F90, the copy
+// linked into the reference main.exe) on one GPU.
+//
+// Data flow of one mpdata_3d / mpdata_vec_3d call (kord=2, limiter on):
+//   K1 min        m = min(psi)  (whole array for mpdata_3d, interior and per
+//                 field for the vector variant), combined with lbc/ubc
+//   K2 upwind-1   psi1 = donor(psi-m, u,v,w) into scratch; the same kernel
+//                 sums psi-m over the interior for the zero-sum exit test
+//   K3 antidiff   U*,V*,W* from (u,v,w, psi1)           Smolarkiewicz 1984
+//   K4 beta       beta_up/beta_down from psi-m, psi1, U* Smolarkiewicz &
+//                 Grabowski 1990 eqs 19-20
+//   K5 upwind-2   donor(psi1) with the limited velocities (eq 18) evaluated on
+//                 the fly at the six faces, + m, written straight into psi
+// The reference's psi_store copy, the shift/unshift sweeps and the limited
+// velocity arrays are never materialised: psi itself is left untouched until
+// K5, so psi-m is recomputed wherever the reference reads psi_store.  Each
+// quantity is still evaluated by the reference's expression in the
+// reference's order, so the FP64 path reproduces the reference bit for bit
+// (built with -ffp-contract=off).  The zero-sum exit (:284-289) is decided on
+// the device; later kernels test the flag, so no host synchronisation occurs.
+//
+// The same kernels instantiated with R=float give the mixed-precision path:
+// psi1, U*, beta and the velocity shadows are FP32 (half the bytes), while
+// psi, the shift m and the output stay FP64.
+#include <cfloat>
+
+#include "dcm_device.h"
+
+namespace dcm {
+
+template <typename R> struct Small;
+template <> struct Small<double> { static constexpr double v = 1e-60; };  // :19
+// 1e-60 underflows in binary32; the FP32 path uses the smallest normal.
+template <> struct Small<float> { static constexpr float v = FLT_MIN; };
+
+struct MetPack {
+    Metrics m64;
+    Metrics32 m32;
+};
+
+template <typename R> struct M1;
+template <> struct M1<double> {
+    const double *dx, *dy, *dz, *dxn, *dyn, *dzn, *ra, *rn;
+    __host__ __device__ M1(const MetPack &p)
+        : dx(p.m64.dx), dy(p.m64.dy), dz(p.m64.dz), dxn(p.m64.dxn), dyn(p.m64.dyn),
+          dzn(p.m64.dzn), ra(p.m64.rhoa), rn(p.m64.rhoan) {}
+};
+template <> struct M1<float> {
+    const float *dx, *dy, *dz, *dxn, *dyn, *dzn, *ra, *rn;
+    __host__ __device__ M1(const MetPack &p)
+        : dx(p.m32.dx), dy(p.m32.dy), dz(p.m32.dz), dxn(p.m32.dxn), dyn(p.m32.dyn),
+          dzn(p.m32.dzn), ra(p.m32.rhoa), rn(p.m32.rhoan) {}
+};
+
+template <typename R> __device__ __forceinline__ R pos0(R x) { return x > R(0) ? x : R(0); }
+template <typename R> __device__ __forceinline__ R neg0(R x) { return x < R(0) ? x : R(0); }
+template <typename R> __device__ __forceinline__ R min3(R a, R b, R c) {
+    R m = a;
+    if (b < m) m = b;
+    if (c < m) m = c;
+    return m;
+}
+
+// first_order_upstream_3d arithmetic for one cell (advection_3d.F90:74-115)
+template <typename R>
+__device__ __forceinline__ R donor(R dt, R uc, R ul, R vc, R vl, R wc, R wl, R pc, R pe, R pw,
+                                   R pn, R ps, R pu, R pd, R dxn, R dyn, R dzn, R rak, R rakm,
+                                   R rnk, int k, int kp, int neumann) {
+    const R two = R(2);
+    R fxr = (((uc + fabs(uc)) * pc + (uc - fabs(uc)) * pe) * dt) / (two * dxn);
+    R fxl = (((ul + fabs(ul)) * pw + (ul - fabs(ul)) * pc) * dt) / (two * dxn);
+    R fyr = (((vc + fabs(vc)) * pc + (vc - fabs(vc)) * pn) * dt) / (two * dyn);
+    R fyl = (((vl + fabs(vl)) * ps + (vl - fabs(vl)) * pc) * dt) / (two * dyn);
+    R fzr = ((((wc + fabs(wc)) * rak) * pc + ((wc - fabs(wc)) * rak) * pu) * dt) /
+            ((two * dzn) * rnk);
+    R fzl = ((((wl + fabs(wl)) * rakm) * pd + ((wl - fabs(wl)) * rakm) * pc) * dt) /
+            ((two * dzn) * rnk);
+    if (neumann == 1) {  // :95-101
+        if (k == 1) fzl = fzr;
+        if (k == kp) fzr = fzl;
+    } else if (neumann == 2) {  // :102-110
+        if (k == 1) fzl = -fzr;
+        if (k == kp) fzr = -fzl;
+    }
+    return ((pc - (fxr - fxl)) - (fyr - fyl)) - (fzr - fzl);
+}
+
+// ---------------------------------------------------------------------------
+// K1: minimum (mpdata_3d: whole array :275; vector variant: interior :1307)
+// ---------------------------------------------------------------------------
+__global__ void k_min(const double *a, Geo g, int interior_only, double *partials) {
+    double v = INFINITY;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.ntot;
+         t += (long long)gridDim.x * blockDim.x) {
+        if (interior_only) {
+            const int k = (int)(t % g.nk);
+            const long long r = t / g.nk;
+            const int j = (int)(r % g.nj), i = (int)(r / g.nj);
+            if (k < 1 || k > g.kp || j < 1 || j > g.jp || i < 1 || i > g.ip) continue;
+        }
+        const double x = a[t];
+        v = x < v ? x : v;
+    }
+    block_min_store(v, partials);
+}
+
+// m = min(min(a), lbc, ubc); flag cleared
+__global__ void k_min_finish(const double *partials, int nb, double lbc, double ubc,
+                             double *m, int *flag) {
+    double r = reduce_min_1024(partials, nb);
+    if (threadIdx.x == 0) {
+        if (lbc < r) r = lbc;
+        if (ubc < r) r = ubc;
+        *m = r;
+        *flag = 0;
+    }
+}
+
+// zero-sum exit test (:282-289): flag = (sum(psi-m) < small)
+__global__ void k_exit_flag(const double *partials, int nb, int *flag) {
+    double s = reduce_slot_1024(partials, nb);
+    if (threadIdx.x == 0) *flag = (s < 1e-60) ? 1 : 0;
+}
+
+// Early-exit result: psi = (psi - m) + m over the whole array (:285).
+__global__ void k_exit_unshift(double *psi, long long n, const double *m, const int *flag) {
+    if (!*flag) return;
+    const double mm = *m;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        psi[t] = (psi[t] - mm) + mm;
+}
+
+// Velocity shadow for the FP32 path: whole padded arrays.
+__global__ void k_to_float3(const double *u, const double *v, const double *w, float *uf,
+                            float *vf, float *wf, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        uf[t] = (float)u[t];
+        vf[t] = (float)v[t];
+        wf[t] = (float)w[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: first donor pass on psi-m (also the plain upwind of the shifted field)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_upwind_first(
+    const double *psi, const double *mptr, const R *u, const R *v, const R *w, R *out, Geo g,
+    MetPack mp, R dt, int neumann, int do_sum, double *partials) {
+    const M1<R> M(mp);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    const bool act = k <= g.kp && j <= g.jp;
+    const double m = *mptr;
+    double s[1] = {0.0};
+    if (act) {
+        const long long c = pidx(g, k, j, i);
+        auto P = [&](long long q) { return (R)(psi[q] - m); };
+        const R pc = P(c);
+        if (do_sum) s[0] = psi[c] - m;
+        R res = donor<R>(dt, u[c], u[c - g.si], v[c], v[c - g.sj], w[c], w[c - 1], pc,
+                         P(c + g.si), P(c - g.si), P(c + g.sj), P(c - g.sj), P(c + 1), P(c - 1),
+                         M.dxn[i], M.dyn[j], M.dzn[k], M.ra[k], M.ra[k - 1], M.rn[k], k, g.kp,
+                         neumann);
+        out[c] = res;
+    }
+    if (do_sum) block_sum_store<1>(s, partials, 0);
+}
+
+// ---------------------------------------------------------------------------
+// plain donor pass on a filled scratch field; output to scratch or to psi+m
+// ---------------------------------------------------------------------------
+template <typename R, bool TO_PSI>
+__global__ __launch_bounds__(256) void k_upwind(const R *src, const R *u, const R *v,
+                                                const R *w, R *out, double *psi_out,
+                                                const double *mptr, const int *flag, Geo g,
+                                                MetPack mp, R dt, int neumann) {
+    if (flag && *flag) return;
+    const M1<R> M(mp);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    R res = donor<R>(dt, u[c], u[c - g.si], v[c], v[c - g.sj], w[c], w[c - 1], src[c],
+                     src[c + g.si], src[c - g.si], src[c + g.sj], src[c - g.sj], src[c + 1],
+                     src[c - 1], M.dxn[i], M.dyn[j], M.dzn[k], M.ra[k], M.ra[k - 1], M.rn[k], k,
+                     g.kp, neumann);
+    if (TO_PSI)
+        psi_out[c] = (double)res + *mptr;
+    else
+        out[c] = res;
+}
+
+// ---------------------------------------------------------------------------
+// K3: antidiffusive velocities (advection_3d.F90:318-506)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_antidiff(const R *ut, const R *vt, const R *wt,
+                                                  const R *po, R *us, R *vs, R *ws,
+                                                  const int *flag, Geo g, MetPack mp, R dt) {
+    if (flag && *flag) return;
+    const M1<R> M(mp);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long SI = g.si, SJ = g.sj;
+    const long long c = pidx(g, k, j, i);
+#define U(dk, dj, di) ut[c + (dk) + (dj)*SJ + (di)*SI]
+#define V(dk, dj, di) vt[c + (dk) + (dj)*SJ + (di)*SI]
+#define W(dk, dj, di) wt[c + (dk) + (dj)*SJ + (di)*SI]
+#define P(dk, dj, di) po[c + (dk) + (dj)*SJ + (di)*SI]
+    const R small = Small<R>::v;
+    const R h = R(0.5), q = R(0.25);
+    const R *ra = M.ra, *rn = M.rn;
+    const R g1 = rn[k], g2 = rn[k];
+    const R g3 = h * (rn[k] + rn[k + 1]);
+    const R dxim = M.dx[i - 1], dyjm = M.dy[j - 1], dzkm = M.dz[k - 1];
+    R u_div1 = (ra[k] * (W(0, 0, 0) + W(0, 0, 1)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 0, 1))) /
+                   (dzkm * g1) +
+               (rn[k] * (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1))) / (dyjm * g1);
+    R u_div2 = (ra[k] * (W(0, 0, 0) + W(0, 1, 0)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 1, 0))) /
+                   (dzkm * g2) +
+               (rn[k] * (((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1))) / (dxim * g2);
+    R u_div3 = (rn[k] * (U(0, 0, 0) - U(0, 0, -1)) + rn[k + 1] * (U(1, 0, 0) - U(1, 0, -1))) /
+                   (dxim * g3) +
+               (rn[k] * (V(0, 0, 0) - V(0, -1, 0)) + rn[k + 1] * (V(1, 0, 0) - V(1, -1, 0))) /
+                   (dyjm * g3);
+    R tz1 = ((((q * (((ra[k] * W(0, 0, 1) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 0, 1)) +
+                     ra[k - 1] * W(-1, 0, 0))) / g1) *
+              (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))) /
+             ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small)) /
+            (h * (M.dzn[k - 1] + M.dzn[k]));
+    R ty1 = ((((q * (((rn[k] * V(0, 0, 1) + rn[k] * V(0, 0, 0)) + rn[k] * V(0, -1, 1)) +
+                     rn[k] * V(0, -1, 0))) / g1) *
+              (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))) /
+             ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small)) /
+            (h * (M.dyn[j - 1] + M.dyn[j]));
+    R ujb1 = ((h * dt) * U(0, 0, 0)) * (tz1 + ty1);
+    R tz2 = ((((q * (((ra[k] * W(0, 1, 0) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 1, 0)) +
+                     ra[k - 1] * W(-1, 0, 0))) / g2) *
+              (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))) /
+             ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small)) /
+            (h * (M.dzn[k - 1] + M.dzn[k]));
+    R tx2 = ((((q * (((rn[k] * U(0, 1, 0) + rn[k] * U(0, 0, 0)) + rn[k] * U(0, 1, -1)) +
+                     rn[k] * U(0, 0, -1))) / g2) *
+              (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))) /
+             ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small)) /
+            (h * (M.dxn[i - 1] + M.dxn[i]));
+    R ujb2 = ((h * dt) * V(0, 0, 0)) * (tz2 + tx2);
+    R ty3 = ((((q * (((rn[k + 1] * V(1, 0, 0) + rn[k] * V(0, 0, 0)) + rn[k + 1] * V(1, -1, 0)) +
+                     rn[k] * V(0, -1, 0))) / g3) *
+              (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))) /
+             ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small)) /
+            (h * (M.dyn[j - 1] + M.dyn[j]));
+    R tx3 = ((((q * (((rn[k + 1] * U(1, 0, 0) + rn[k] * U(0, 0, 0)) + rn[k + 1] * U(1, 0, -1)) +
+                     rn[k] * U(0, 0, -1))) / g3) *
+              (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))) /
+             ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small)) /
+            (h * (M.dxn[i - 1] + M.dxn[i]));
+    R ujb3 = ((h * dt) * W(0, 0, 0)) * (ty3 + tx3);
+    const R uc = U(0, 0, 0), vc = V(0, 0, 0), wc = W(0, 0, 0);
+    R usv = ((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))) /
+             ((P(0, 0, 1) + P(0, 0, 0)) + small)) /
+                M.dxn[i] -
+            ujb1;
+    R vsv = ((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))) /
+             ((P(0, 1, 0) + P(0, 0, 0)) + small)) /
+                M.dyn[j] -
+            ujb2;
+    R wsv = ((((fabs(wc) * M.dz[k]) - (((dt * (ra[k] / g3)) * wc) * wc)) *
+              (P(1, 0, 0) - P(0, 0, 0))) /
+             ((P(1, 0, 0) + P(0, 0, 0)) + small)) /
+                M.dzn[k] -
+            ujb3;
+    usv = usv - ((q * dt) * uc) * (((U(0, 0, 1) - U(0, 0, -1)) / dxim) + u_div1);
+    vsv = vsv - ((q * dt) * vc) * (((V(0, 1, 0) - V(0, -1, 0)) / dyjm) + u_div2);
+    wsv = wsv - ((q * dt) * wc) *
+                    (((ra[k + 1] * W(1, 0, 0) - ra[k - 1] * W(-1, 0, 0)) / (dzkm * g3)) + u_div3);
+    us[c] = usv;
+    vs[c] = vsv;
+    ws[c] = wsv;
+#undef U
+#undef V
+#undef W
+#undef P
+}
+
+// ---------------------------------------------------------------------------
+// K4: beta_up / beta_down (advection_3d.F90:549-616).  psi_store is psi-m,
+// recomputed from the untouched psi.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_beta(const double *psi, const double *mptr,
+                                              const R *po, const R *ut, const R *vt,
+                                              const R *wt, R *bup, R *bdn, const int *flag,
+                                              Geo g, MetPack mp, R dt) {
+    if (flag && *flag) return;
+    const M1<R> M(mp);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long SI = g.si, SJ = g.sj;
+    const long long c = pidx(g, k, j, i);
+    const double m = *mptr;
+    const R small = Small<R>::v;
+    const R vals[14] = {(R)(psi[c] - m),      (R)(psi[c - SI] - m), (R)(psi[c + SI] - m),
+                        (R)(psi[c - SJ] - m), (R)(psi[c + SJ] - m), (R)(psi[c - 1] - m),
+                        (R)(psi[c + 1] - m),  po[c],                po[c - SI],
+                        po[c + SI],           po[c - SJ],           po[c + SJ],
+                        po[c - 1],            po[c + 1]};
+    R mx = vals[0], mn = vals[0];
+#pragma unroll
+    for (int q = 1; q < 14; ++q) {
+        if (vals[q] > mx) mx = vals[q];
+        if (vals[q] < mn) mn = vals[q];
+    }
+    const R *ra = M.ra, *rn = M.rn;
+    const R pc = po[c];
+    R d1 = (dt * ((((pos0(ut[c - SI]) * rn[k]) * po[c - SI] - (neg0(ut[c]) * rn[k]) * po[c + SI]) /
+                       M.dxn[i] +
+                   ((pos0(vt[c - SJ]) * rn[k]) * po[c - SJ] - (neg0(vt[c]) * rn[k]) * po[c + SJ]) /
+                       M.dyn[j]) +
+                  ((pos0(wt[c - 1]) * ra[k - 1]) * po[c - 1] - (neg0(wt[c]) * ra[k]) * po[c + 1]) /
+                      M.dzn[k])) +
+           small;
+    R d2 = dt * (((((pos0(ut[c]) * rn[k]) * pc - (neg0(ut[c - SI]) * rn[k]) * pc) / M.dxn[i] +
+                   ((pos0(vt[c]) * rn[k]) * pc - (neg0(vt[c - SJ]) * rn[k]) * pc) / M.dyn[j]) +
+                  ((pos0(wt[c]) * ra[k]) * pc - (neg0(wt[c - 1]) * ra[k - 1]) * pc) / M.dzn[k]) +
+                 small);
+    bup[c] = (rn[k] * (mx - pc)) / d1;
+    bdn[c] = (rn[k] * (pc - mn)) / d2;
+}
+
+// limited velocity of eq 18 at the face owned by cell c (stride s to the
+// neighbour across that face)
+template <typename R>
+__device__ __forceinline__ R limited(const R *us, const R *bup, const R *bdn, long long c,
+                                     long long s) {
+    const R x = us[c];
+    return min3<R>(R(1), bdn[c], bup[c + s]) * pos0(x) + min3<R>(R(1), bup[c], bdn[c + s]) * neg0(x);
+}
+
+// eq 18 materialised (only for kord > 2, where the next antidiffusive pass
+// needs the limited velocities as its input)
+template <typename R>
+__global__ __launch_bounds__(256) void k_limit(const R *us, const R *vs, const R *ws,
+                                               const R *bup, const R *bdn, R *ul, R *vl, R *wl,
+                                               const int *flag, Geo g) {
+    if (flag && *flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    ul[c] = limited<R>(us, bup, bdn, c, g.si);
+    vl[c] = limited<R>(vs, bup, bdn, c, g.sj);
+    wl[c] = limited<R>(ws, bup, bdn, c, 1);
+}
+
+// ---------------------------------------------------------------------------
+// K5: last donor pass with the limited velocities computed at the six faces.
+// The lower faces of cells i=1 / j=1 take the periodic copy of the face at
+// ip / jp (exchange_full of the limited velocities, :669-674); the face
+// below k=1 is the lbc=0 plane.
+// ---------------------------------------------------------------------------
+template <typename R, bool TO_PSI>
+__global__ __launch_bounds__(256) void k_upwind_limited(
+    const R *src, const R *us, const R *vs, const R *ws, const R *bup, const R *bdn, R *out,
+    double *psi_out, const double *mptr, const int *flag, Geo g, MetPack mp, R dt,
+    int neumann) {
+    if (flag && *flag) return;
+    const M1<R> M(mp);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const long long cw = pidx(g, k, j, i == 1 ? g.ip : i - 1);
+    const long long cs = pidx(g, k, j == 1 ? g.jp : j - 1, i);
+    const R uc = limited<R>(us, bup, bdn, c, g.si);
+    const R ul = limited<R>(us, bup, bdn, cw, g.si);
+    const R vc = limited<R>(vs, bup, bdn, c, g.sj);
+    const R vl = limited<R>(vs, bup, bdn, cs, g.sj);
+    const R wc = limited<R>(ws, bup, bdn, c, 1);
+    const R wl = k == 1 ? R(0) : limited<R>(ws, bup, bdn, c - 1, 1);
+    R res = donor<R>(dt, uc, ul, vc, vl, wc, wl, src[c], src[c + g.si], src[c - g.si],
+                     src[c + g.sj], src[c - g.sj], src[c + 1], src[c - 1], M.dxn[i], M.dyn[j],
+                     M.dzn[k], M.ra[k], M.ra[k - 1], M.rn[k], k, g.kp, neumann);
+    if (TO_PSI)
+        psi_out[c] = (double)res + *mptr;
+    else
+        out[c] = res;
+}
+
+// copy the interior of a scratch field into psi (standalone upwind API)
+__global__ void k_copy_interior(const double *src, double *dst, Geo g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    dst[c] = src[c];
+}
+
+// ---------------------------------------------------------------------------
+// adv_ref_state (advection_3d.F90:1148-1218): every (j,i) column including
+// the lateral halo, k = 1..kp
+// ---------------------------------------------------------------------------
+__global__ void k_adv_ref_state(const double *w, double *psi, const double *ref, Geo g,
+                                Metrics M, double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k > g.kp || j > g.jp + 1) return;
+    const long long c = pidx(g, k, j, i);
+    psi[c] = psi[c] - (dt * ((((0.5 / M.dz[k - 1]) * w[c - 1]) * M.rhoa[k - 1]) *
+                                 (ref[k] - ref[k - 1]) +
+                             (((0.5 / M.dz[k]) * w[c]) * M.rhoa[k]) * (ref[k + 1] - ref[k]))) /
+                          M.rhoan[k];
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+template <typename R>
+static int get_buf(dcm_ctx *c, int slot, R **out);
+template <>
+int get_buf<double>(dcm_ctx *c, int slot, double **out) { return work_field64(c, slot, out); }
+template <>
+int get_buf<float>(dcm_ctx *c, int slot, float **out) { return work_field32(c, slot, out); }
+
+// Work-slot map (separate pools for FP64 and FP32):
+//   0-2 antidiffusive U*,V*,W*   3-5 limited (kord>2)   6-7 beta up/down
+//   8.. psi1 per field, then psi2 per field (kord>2)
+static constexpr int kSlotPsi = 8;
+
+// psi(interior) = psi1 + m (kord = 1)
+template <typename R>
+__global__ void k_store_shifted(const R *src, double *psi, const double *mptr, const int *flag,
+                                Geo g) {
+    if (*flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    psi[c] = (double)src[c] + *mptr;
+}
+
+// Shared body of mpdata_3d and mpdata_vec_3d.  u,v,w are the advecting
+// velocities in precision R (for R=float, FP32 shadows).  psi: nf fields of
+// ntot FP64 each.  lbc/ubc are host values (unshifted); the shifts m[n] and
+// the exit flag stay on the device.
+template <typename R>
+int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *psi, int nf,
+                const double *lbc, const double *ubc, int kord, int monotone, int neumann,
+                int interior_min) {
+    const Geo &g = c->geo;
+    const Interior L = interior_launch(g);
+    MetPack mp{c->met, c->met32};
+    double *dm = c->dscal + 16;         // shifts of up to 32 fields
+    int *flag = (int *)(c->dscal + 8);  // zero-sum exit flag
+    if (nf < 1 || nf > 32) {
+        set_error("mpdata: a category must hold 1..32 fields");
+        return DCM_BAD_ARG;
+    }
+    const unsigned nbmin = 1024;
+    const unsigned nbint = L.grid.x * L.grid.y * L.grid.z;
+    DCM_CHECK(ensure_partials(c, (size_t)nbint + nbmin + 16));
+    // K1: shifts (mpdata_3d :275-280, mpdata_vec_3d :1306-1312)
+    for (int n = 0; n < nf; ++n) {
+        const double *f = psi + (size_t)n * g.ntot;
+        int tok = prof_begin(c, "mpdata_min");
+        hipLaunchKernelGGL(k_min, dim3(nbmin), dim3(kBlock), 0, c->stream, f, g, interior_min,
+                           c->partials);
+        DCM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_min_finish, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                           (int)nbmin, lbc[n], ubc[n], dm + n, flag);
+        DCM_LAUNCH_CHECK();
+        prof_end(c, tok);
+    }
+    // K2: first donor pass of every field with (u,v,w); zero-sum test on field 1
+    std::vector<R *> s1(nf), s2(nf);
+    for (int n = 0; n < nf; ++n) DCM_CHECK(get_buf<R>(c, kSlotPsi + n, &s1[n]));
+    for (int n = 0; n < nf; ++n) {
+        int tok = prof_begin(c, "upwind_first");
+        hipLaunchKernelGGL(k_upwind_first<R>, L.grid, L.block, 0, c->stream,
+                           psi + (size_t)n * g.ntot, dm + n, u, v, w, s1[n], g, mp, dt, neumann,
+                           n == 0 ? 1 : 0, c->partials);
+        DCM_LAUNCH_CHECK();
+        prof_end(c, tok);
+        if (n == 0) {
+            hipLaunchKernelGGL(k_exit_flag, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                               (int)nbint, flag);
+            DCM_LAUNCH_CHECK();
+        }
+    }
+    if (kord <= 1) {
+        for (int n = 0; n < nf; ++n) {
+            hipLaunchKernelGGL(k_store_shifted<R>, L.grid, L.block, 0, c->stream, s1[n],
+                               psi + (size_t)n * g.ntot, dm + n, flag, g);
+            DCM_LAUNCH_CHECK();
+        }
+    } else {
+        for (int n = 0; n < nf; ++n)
+            DCM_CHECK(fill<R>(c, s1[n], FILL_FULL, lbc[n], ubc[n], dm + n, 1, flag));
+        if (kord > 2)
+            for (int n = 0; n < nf; ++n) DCM_CHECK(get_buf<R>(c, kSlotPsi + nf + n, &s2[n]));
+        // velocity triples: X = (u,v,w); A, B scratch
+        R *A[3], *B[3], *bup = nullptr, *bdn = nullptr;
+        for (int d = 0; d < 3; ++d) DCM_CHECK(get_buf<R>(c, d, &A[d]));
+        if (kord > 2)
+            for (int d = 0; d < 3; ++d) DCM_CHECK(get_buf<R>(c, 3 + d, &B[d]));
+        if (monotone) {
+            DCM_CHECK(get_buf<R>(c, 6, &bup));
+            DCM_CHECK(get_buf<R>(c, 7, &bdn));
+        }
+        const R *in[3] = {u, v, w};
+        R **anti = A, **other = B;
+        std::vector<R *> cur = s1, nxt = s2;
+        for (int it = 2; it <= kord; ++it) {
+            const bool last = it == kord;
+            // antidiffusive velocities from the current advecting velocity
+            int tok = prof_begin(c, "antidiff");
+            hipLaunchKernelGGL(k_antidiff<R>, L.grid, L.block, 0, c->stream, in[0], in[1], in[2],
+                               cur[0], anti[0], anti[1], anti[2], flag, g, mp, dt);
+            DCM_LAUNCH_CHECK();
+            prof_end(c, tok);
+            for (int d = 0; d < 3; ++d) DCM_CHECK(fill<R>(c, anti[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
+            if (monotone) {
+                tok = prof_begin(c, "beta");
+                hipLaunchKernelGGL(k_beta<R>, L.grid, L.block, 0, c->stream, psi, dm, cur[0],
+                                   anti[0], anti[1], anti[2], bup, bdn, flag, g, mp, dt);
+                DCM_LAUNCH_CHECK();
+                prof_end(c, tok);
+                DCM_CHECK(fill<R>(c, bup, FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
+                DCM_CHECK(fill<R>(c, bdn, FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
+            }
+            if (last) {
+                for (int n = 0; n < nf; ++n) {
+                    tok = prof_begin(c, monotone ? "upwind_limited" : "upwind_last");
+                    if (monotone)
+                        hipLaunchKernelGGL((k_upwind_limited<R, true>), L.grid, L.block, 0,
+                                           c->stream, cur[n], anti[0], anti[1], anti[2], bup,
+                                           bdn, (R *)nullptr, psi + (size_t)n * g.ntot, dm + n,
+                                           flag, g, mp, dt, neumann);
+                    else
+                        hipLaunchKernelGGL((k_upwind<R, true>), L.grid, L.block, 0, c->stream,
+                                           cur[n], anti[0], anti[1], anti[2], (R *)nullptr,
+                                           psi + (size_t)n * g.ntot, dm + n, flag, g, mp, dt,
+                                           neumann);
+                    DCM_LAUNCH_CHECK();
+                    prof_end(c, tok);
+                }
+                break;
+            }
+            // an intermediate pass (kord > 2)
+            R **adv = anti;  // advecting velocity of this pass
+            if (monotone) {
+                // eq 18 into the other triple, which the next pass reads
+                hipLaunchKernelGGL(k_limit<R>, L.grid, L.block, 0, c->stream, anti[0], anti[1],
+                                   anti[2], bup, bdn, other[0], other[1], other[2], flag, g);
+                DCM_LAUNCH_CHECK();
+                for (int d = 0; d < 3; ++d)
+                    DCM_CHECK(fill<R>(c, other[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
+                adv = other;
+            }
+            for (int n = 0; n < nf; ++n) {
+                hipLaunchKernelGGL((k_upwind<R, false>), L.grid, L.block, 0, c->stream, cur[n],
+                                   adv[0], adv[1], adv[2], nxt[n], (double *)nullptr, dm + n,
+                                   flag, g, mp, dt, neumann);
+                DCM_LAUNCH_CHECK();
+                DCM_CHECK(fill<R>(c, nxt[n], FILL_FULL, lbc[n], ubc[n], dm + n, 1, flag));
+            }
+            for (int d = 0; d < 3; ++d) in[d] = adv[d];
+            if (!monotone) std::swap(anti, other);  // never overwrite the input
+            std::swap(cur, nxt);
+        }
+    }
+    // halo of the result: exchange_full(psi, lbc-m, ubc-m) then +m, i.e. the
+    // z planes hold (bc-m)+m and every other halo cell copies the interior
+    for (int n = 0; n < nf; ++n) {
+        double *f = psi + (size_t)n * g.ntot;
+        DCM_CHECK(fill<double>(c, f, FILL_FULL, lbc[n], ubc[n], dm + n, 2, flag));
+    }
+    // zero-sum exit: psi = (psi-m)+m over the whole array, nothing else
+    for (int n = 0; n < nf; ++n) {
+        hipLaunchKernelGGL(k_exit_unshift, dim3(1024), dim3(kBlock), 0, c->stream,
+                           psi + (size_t)n * g.ntot, g.ntot, dm + n, flag);
+        DCM_LAUNCH_CHECK();
+    }
+    return DCM_OK;
+}
+
+template int mpdata_core<double>(dcm_ctx *, double, const double *, const double *,
+                                 const double *, double *, int, const double *, const double *,
+                                 int, int, int, int);
+template int mpdata_core<float>(dcm_ctx *, float, const float *, const float *, const float *,
+                                double *, int, const double *, const double *, int, int, int,
+                                int);
+
+// adv_ref_state over the columns j = 0..jp+1, i = 0..ip+1 (ref on the device)
+int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref) {
+    const Geo &g = c->geo;
+    Interior L = interior_launch(g);
+    L.grid.y = (g.nj + L.block.y - 1) / L.block.y;
+    L.grid.z = g.ni;
+    int tok = prof_begin(c, "adv_ref_state");
+    hipLaunchKernelGGL(k_adv_ref_state, L.grid, L.block, 0, c->stream, w, psi, ref, g, c->met, dt);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+
+// Convert u,v,w (whole padded arrays) to the FP32 shadows used by the mixed path.
+int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
+                    float *vf, float *wf) {
+    hipLaunchKernelGGL(k_to_float3, dim3(2048), dim3(kBlock), 0, c->stream, u, v, w, uf, vf, wf,
+                       c->geo.ntot);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+}  // namespace dcm
+
+using namespace dcm;
+
+extern "C" {
+
+int dcm_upwind3d(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+                 double *psi, int neumann) {
+    if (!c || !u || !v || !w || !psi) return DCM_BAD_ARG;
+    const Geo &g = c->geo;
+    const Interior L = interior_launch(g);
+    MetPack mp{c->met, c->met32};
+    double *tmp;
+    DCM_CHECK(work_field64(c, kSlotPsi, &tmp));
+    // fluxes from the old field into scratch, then the in-place update
+    // (advection_3d.F90:74-115)
+    hipLaunchKernelGGL((k_upwind<double, false>), L.grid, L.block, 0, c->stream, psi, u, v, w,
+                       tmp, (double *)nullptr, (const double *)nullptr, (const int *)nullptr, g,
+                       mp, dt, neumann);
+    DCM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_interior, L.grid, L.block, 0, c->stream, tmp, psi, g);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+static int mpdata_api(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+                      double *psi, int nq, double *lbc, double *ubc, int kord, int monotone,
+                      int neumann, int *early_exit, int interior_min) {
+    if (!c || !u || !v || !w || !psi || !lbc || !ubc || nq < 1 || nq > 32) return DCM_BAD_ARG;
+    DCM_CHECK(mpdata_core<double>(c, dt, u, v, w, psi, nq, lbc, ubc, kord, monotone, neumann,
+                                  interior_min));
+    // lbc/ubc leave as (bc - m) + m, like the Fortran INTENT(INOUT) scalars
+    DCM_HIP(hipMemcpyAsync(c->hscal, c->dscal, 48 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < nq; ++n) {
+        const double m = c->hscal[16 + n];
+        lbc[n] = (lbc[n] - m) + m;
+        ubc[n] = (ubc[n] - m) + m;
+    }
+    if (early_exit) *early_exit = *(int *)(c->hscal + 8);
+    return DCM_OK;
+}
+
+int dcm_mpdata3d(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+                 double *psi, double *lbc, double *ubc, int kord, int monotone, int neumann,
+                 int *early_exit) {
+    return mpdata_api(c, dt, u, v, w, psi, 1, lbc, ubc, kord, monotone, neumann, early_exit, 0);
+}
+
+int dcm_mpdata_vec3d(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+                     double *psi, int nq, double *lbc, double *ubc, int kord, int monotone,
+                     int neumann, int *early_exit) {
+    return mpdata_api(c, dt, u, v, w, psi, nq, lbc, ubc, kord, monotone, neumann, early_exit, 1);
+}
+
+int dcm_adv_ref_state(dcm_ctx *c, double dt, const double *w, double *psi,
+                      const double *psi_ref) {
+    if (!c || !w || !psi || !psi_ref) return DCM_BAD_ARG;
+    const double *dref;
+    DCM_CHECK(upload_1d(c, psi_ref, (size_t)c->geo.kp + 2, 0, &dref));
+    return adv_ref_state_dev(c, dt, w, psi, dref);
+}
+
+}  // extern "C"

@@ -1,0 +1,694 @@
+// dcm_dynamics.hip -- momentum sources, horizontal means, the momentum and
+// scalar updates, and the model_driver time step on device-resident state.
+#include <cmath>
+#include <cstring>
+
+#include "dcm_device.h"
+
+namespace dcm {
+
+static constexpr double kGrav = 9.81;  // dynamics.f90:18
+
+// ---------------------------------------------------------------------------
+// sources, dynamics.f90:743-797 (interior k = 1..kp)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sources(const double *u, const double *v,
+                                                 const double *w, const double *th,
+                                                 const double *theta, const double *thbar,
+                                                 double *su, double *sv, double *sw, Geo g,
+                                                 Metrics M) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const long long SI = g.si, SJ = g.sj;
+    const double *dx = M.dx, *dy = M.dy, *dz = M.dz, *dxn = M.dxn, *dyn = M.dyn, *dzn = M.dzn;
+    {  // su: :743-757
+        const double X = (u[c - SI] * (u[c] + u[c - SI]) - u[c + SI] * (u[c] + u[c + SI])) /
+                         (2.0 * (dx[i] + dx[i - 1]));
+        const double Y = (u[c - SJ] * (v[c - SJ] + v[c - SJ + SI]) - u[c + SJ] * (v[c] + v[c + SI])) /
+                         (2.0 * (dyn[j] + dyn[j - 1]));
+        const double Z = (u[c - 1] * (w[c - 1] + w[c - 1 + SI]) - u[c + 1] * (w[c] + w[c + SI])) /
+                         (2.0 * (dzn[k] + dzn[k - 1]));
+        su[c] = M.rhoan[k] * ((X + Y) + Z);
+    }
+    {  // sv: :761-775
+        const double X = (v[c - SI] * (u[c - SI] + u[c + SJ - SI]) - v[c + SI] * (u[c] + u[c + SJ])) /
+                         (2.0 * (dxn[i] + dxn[i - 1]));
+        const double Y = (v[c - SJ] * (v[c] + v[c - SJ]) - v[c + SJ] * (v[c] + v[c + SJ])) /
+                         (2.0 * (dy[j] + dy[j - 1]));
+        const double Z = (v[c - 1] * (w[c - 1] + w[c - 1 + SJ]) - v[c + 1] * (w[c] + w[c + SJ])) /
+                         (2.0 * (dzn[k] + dzn[k - 1]));
+        sv[c] = M.rhoan[k] * ((X + Y) + Z);
+    }
+    {  // sw with buoyancy: :779-797
+        const double X = (w[c - SI] * (u[c - SI] + u[c + 1 - SI]) - w[c + SI] * (u[c] + u[c + 1])) /
+                         (2.0 * (dxn[i] + dxn[i - 1]));
+        const double Y = (w[c - SJ] * (v[c - SJ] + v[c + 1 - SJ]) - w[c + SJ] * (v[c] + v[c + 1])) /
+                         (2.0 * (dyn[j] + dyn[j - 1]));
+        const double Z = (w[c - 1] * (w[c] + w[c - 1]) - w[c + 1] * (w[c] + w[c + 1])) /
+                         (2.0 * (dz[k] + dz[k - 1]));
+        const double B = (kGrav * (th[c] + th[c + 1])) / (((2.0 * theta[k]) + thbar[k]) + thbar[k + 1]);
+        sw[c] = M.rhoa[k] * (((X + Y) + Z) + B);
+    }
+}
+
+// floor and lid rows of the sources (dynamics.f90:922-973), whole planes
+__global__ void k_sources_rows(double *su, double *sv, double *sw, Geo g) {
+    const long long P = (long long)g.nj * g.ni;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+        const long long c0 = pidx(g, 0, j, i);
+        su[c0] = -su[c0 + 1];
+        sv[c0] = -sv[c0 + 1];
+        sw[c0] = 0.0;
+        const long long ck = pidx(g, g.kp, j, i);
+        sw[ck] = 0.0;
+        sw[ck + 1] = 0.0;
+        su[ck] = 0.0;
+        su[ck + 1] = 0.0;
+        sv[ck] = 0.0;
+        sv[ck + 1] = 0.0;
+    }
+}
+
+// Poisson right-hand side (dynamics.f90:977-998), k = 0..kp+1
+__global__ __launch_bounds__(256) void k_rhs(const double *su, const double *sv,
+                                             const double *sw, double *rhs, Geo g, Metrics M) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp + 1 || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double X = (su[c] - su[c - g.si]) / M.dx[i - 1];
+    const double Y = (sv[c] - sv[c - g.sj]) / M.dy[j - 1];
+    if (k >= 1)
+        rhs[c] = (X + Y) + (sw[c] - sw[c - 1]) / M.dz[k - 1];
+    else
+        rhs[c] = (X + Y) + (sw[c] + sw[c + 1]) / M.dz[1];
+}
+
+// ---------------------------------------------------------------------------
+// horizontal means (diagnostics.f90:31-107): per level k = 0..kp+1, sum over
+// the interior columns in a fixed blocked order, divided by ip*jp.
+// ---------------------------------------------------------------------------
+constexpr int kHmCols = 256;  // columns per block
+__global__ void k_level_sums(const double *a, Geo g, double *partials) {
+    // block: 64 lanes along k x 4 column groups; grid: (k chunks, column chunks)
+    __shared__ double red[4][64];
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const long long ncol = (long long)g.ip * g.jp;
+    const long long c0 = (long long)blockIdx.y * kHmCols;
+    double acc = 0.0;
+    if (k < g.nk) {
+        for (long long col = c0 + threadIdx.y; col < c0 + kHmCols && col < ncol; col += 4) {
+            const int j = (int)(col % g.jp) + 1, i = (int)(col / g.jp) + 1;
+            acc += a[pidx(g, k, j, i)];
+        }
+    }
+    red[threadIdx.y][threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.y == 0 && k < g.nk)
+        partials[(long long)blockIdx.y * g.nk + k] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+__global__ void k_level_means(const double *partials, int nchunks, Geo g, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.nk) return;
+    double s = 0.0;
+    for (int q = 0; q < nchunks; ++q) s += partials[(long long)q * g.nk + k];
+    out[k] = s / (double)(g.ip * g.jp);
+}
+
+int level_means(dcm_ctx *c, const double *a, double *out_dev) {
+    const Geo &g = c->geo;
+    const long long ncol = (long long)g.ip * g.jp;
+    const int nchunks = (int)((ncol + kHmCols - 1) / kHmCols);
+    const size_t need = (size_t)nchunks * g.nk;
+    if (c->hm_cap < need) {
+        if (c->hm_partials) {
+            DCM_HIP(hipStreamSynchronize(c->stream));
+            DCM_HIP(hipFree(c->hm_partials));
+        }
+        void *p = nullptr;
+        DCM_HIP(hipMalloc(&p, need * sizeof(double)));
+        c->hm_partials = (double *)p;
+        c->hm_cap = need;
+    }
+    int tok = prof_begin(c, "hmeans");
+    hipLaunchKernelGGL(k_level_sums, dim3((g.nk + 63) / 64, nchunks), dim3(64, 4), 0, c->stream,
+                       a, g, c->hm_partials);
+    hipLaunchKernelGGL(k_level_means, dim3((g.nk + 255) / 256), dim3(256), 0, c->stream,
+                       c->hm_partials, nchunks, g, out_dev);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// advance_momentum (dynamics.f90:1053-1094): k = 0..kp, interior (j,i)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advance_momentum(
+    double *u, double *v, double *w, const double *zu, const double *zv, const double *zw,
+    const double *su, const double *sv, const double *sw, const double *p, Geo g, Metrics M,
+    double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double pc = p[c];
+    u[c] = zu[c] + ((su[c] - (p[c + g.si] - pc) / M.dxn[i]) * dt) / M.rhoan[k];
+    v[c] = zv[c] + ((sv[c] - (p[c + g.sj] - pc) / M.dyn[j]) * dt) / M.rhoan[k];
+    w[c] = zw[c] + ((sw[c] - (p[c + 1] - pc) / M.dzn[k]) * dt) / M.rhoa[k];
+}
+
+// advance_scalar_fields_3d (sgm/subgrid_3d.f90:601-621), whole arrays
+__global__ void k_axpy_dt(double *a, const double *s, double dt, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        a[t] = a[t] + s[t] * dt;
+}
+
+// time filter (driver_code.f90:618-626) on whole arrays.  The filtered
+// previous level is written only when WRITE_Z (the model drops it: that
+// buffer becomes the next future level, whose every cell is rewritten before
+// it is read).
+template <bool WRITE_Z>
+__global__ void k_filter(double *u, double *zu, double *v, double *zv, double *w, double *zw,
+                         long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const double a = u[t], b = v[t], cc = w[t];
+        const double za = zu[t], zb = zv[t], zc = zw[t];
+        u[t] = 0.99 * a + 0.01 * za;
+        v[t] = 0.99 * b + 0.01 * zb;
+        w[t] = 0.99 * cc + 0.01 * zc;
+        if (WRITE_Z) {
+            zu[t] = 0.01 * a + 0.99 * za;
+            zv[t] = 0.01 * b + 0.99 * zb;
+            zw[t] = 0.01 * cc + 0.99 * zc;
+        }
+    }
+}
+
+// floor / lid rows of the new current level (driver_code.f90:663-674)
+__global__ void k_momentum_rows(double *u, double *v, double *w, Geo g) {
+    const long long P = (long long)g.nj * g.ni;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+        const long long c0 = pidx(g, 0, j, i), ck = pidx(g, g.kp + 1, j, i);
+        w[c0] = 0.0;
+        u[c0] = -u[c0 + 1];
+        v[c0] = -v[c0 + 1];
+        w[ck] = 0.0;
+        u[ck] = u[ck - 1];
+        v[ck] = v[ck - 1];
+    }
+}
+
+// initial previous-level rows (driver_code.f90:287-295)
+__global__ void k_init_rows(double *zu, double *zv, double *zw, Geo g) {
+    const long long P = (long long)g.nj * g.ni;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+        zw[pidx(g, 0, j, i)] = -zw[pidx(g, 1, j, i)];
+        zu[pidx(g, 1, j, i)] = 0.0;
+        zv[pidx(g, 1, j, i)] = 0.0;
+    }
+}
+__global__ void k_init_rows_top(double *zu, double *zv, double *zw, Geo g) {
+    const long long P = (long long)g.nj * g.ni;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+        zw[pidx(g, g.kp, j, i)] = -zw[pidx(g, g.kp - 1, j, i)];
+        zw[pidx(g, g.kp + 1, j, i)] = 0.0;
+        zu[pidx(g, g.kp, j, i)] = 0.0;
+        zv[pidx(g, g.kp, j, i)] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// internal entry points used by the model and the C-ABI
+// ---------------------------------------------------------------------------
+int sources_dev(dcm_ctx *c, const double *u, const double *v, const double *w, const double *th,
+                const double *theta_d, const double *thbar_d, double *su, double *sv, double *sw,
+                double *sth, double *rhs) {
+    const Geo &g = c->geo;
+    Interior L = interior_launch(g);
+    int tok = prof_begin(c, "sources");
+    hipLaunchKernelGGL(k_sources, L.grid, L.block, 0, c->stream, u, v, w, th, theta_d, thbar_d,
+                       su, sv, sw, g, c->met);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    // inviscid branch: exchange_full(su,sv,sw,sth) (dynamics.f90:909-916)
+    DCM_CHECK(fill<double>(c, su, FILL_FULL, 0.0, 0.0));
+    DCM_CHECK(fill<double>(c, sv, FILL_FULL, 0.0, 0.0));
+    DCM_CHECK(fill<double>(c, sw, FILL_FULL, 0.0, 0.0));
+    if (sth) DCM_CHECK(fill<double>(c, sth, FILL_FULL, 0.0, 0.0));
+    hipLaunchKernelGGL(k_sources_rows, dim3(256), dim3(kBlock), 0, c->stream, su, sv, sw, g);
+    DCM_LAUNCH_CHECK();
+    Interior R = interior_launch(g, 2);
+    tok = prof_begin(c, "rhs");
+    hipLaunchKernelGGL(k_rhs, R.grid, R.block, 0, c->stream, su, sv, sw, rhs, g, c->met);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+
+int advance_momentum_dev(dcm_ctx *c, double dt, double *u, double *v, double *w,
+                         const double *zu, const double *zv, const double *zw, const double *su,
+                         const double *sv, const double *sw, const double *p) {
+    const Geo &g = c->geo;
+    Interior L = interior_launch(g, 1);
+    int tok = prof_begin(c, "advance_momentum");
+    hipLaunchKernelGGL(k_advance_momentum, L.grid, L.block, 0, c->stream, u, v, w, zu, zv, zw, su,
+                       sv, sw, p, g, c->met, dt);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+
+}  // namespace dcm
+
+// ---------------------------------------------------------------------------
+// model state
+// ---------------------------------------------------------------------------
+struct dcm_model {
+    dcm_ctx *c = nullptr;
+    int nq = 0, ncat = 0;
+    std::vector<int> c_s, c_e;
+    double dt = 1.0, ptol = 1e-10;
+    int kord = 2, monotone = 1, advection_scheme = 1;
+    int precision = DCM_FP64;
+    double *lev[3][3] = {};  // [component u,v,w][buffer ut,zut,tut]
+    double *th = nullptr, *p = nullptr, *psrc = nullptr;
+    double *su = nullptr, *sv = nullptr, *sw = nullptr, *sth = nullptr;
+    double *q = nullptr;
+    double *oned = nullptr;  // theta, thetan, thbar (kp+2 each)
+    float *shadow = nullptr; // FP32 u,v,w for the mixed advection
+    std::vector<double> xg, yg, zg;
+    int n = 0;               // completed steps (pointer rotation)
+    double t1 = 0.0, t2 = 0.0;
+    double inner_rtol = 1e-5;
+    std::vector<void *> allocs;
+
+    double *theta() { return oned; }
+    double *thetan() { return oned + (c->geo.kp + 2); }
+    double *thbar() { return oned + 2 * (c->geo.kp + 2); }
+    // driver_code.f90:634-646: after n steps
+    void levels(double **cur, double **prev, double **fut) {
+        static const int map[3][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}};  // [n%3][cur,prev,fut]
+        const int r = n % 3;
+        for (int d = 0; d < 3; ++d) {
+            cur[d] = lev[d][map[r][0]];
+            prev[d] = lev[d][map[r][1]];
+            fut[d] = lev[d][map[r][2]];
+        }
+    }
+};
+
+using namespace dcm;
+
+static int model_alloc(dcm_model *m, size_t bytes, void **out) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error("dcm_model: out of device memory");
+        return DCM_OUT_OF_MEMORY;
+    }
+    DCM_HIP(hipMemsetAsync(p, 0, bytes, m->c->stream));
+    m->allocs.push_back(p);
+    *out = p;
+    return DCM_OK;
+}
+
+extern "C" {
+
+int dcm_hmeans(dcm_ctx *c, const double *u, const double *v, const double *w, const double *th,
+               const double *q, int nq, double *ubar, double *vbar, double *wbar, double *thbar,
+               double *qbar) {
+    if (!c || !th) return DCM_BAD_ARG;
+    const Geo &g = c->geo;
+    double *d;
+    DCM_CHECK(work_field64(c, 0, &d));  // scratch for the 1-D results
+    const double *src[4] = {u, v, w, th};
+    double *dst[4] = {ubar, vbar, wbar, thbar};
+    for (int a = 0; a < 4; ++a) {
+        if (!src[a] || !dst[a]) continue;
+        DCM_CHECK(level_means(c, src[a], d));
+        DCM_HIP(hipMemcpyAsync(dst[a], d, g.nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        DCM_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (q && qbar)
+        for (int n = 0; n < nq; ++n) {
+            DCM_CHECK(level_means(c, q + (size_t)n * g.ntot, d));
+            DCM_HIP(hipMemcpyAsync(qbar + (size_t)n * g.nk, d, g.nk * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->stream));
+            DCM_HIP(hipStreamSynchronize(c->stream));
+        }
+    return DCM_OK;
+}
+
+int dcm_sources(dcm_ctx *c, const double *u, const double *v, const double *w, const double *th,
+                const double *theta, const double *thbar, double *su, double *sv, double *sw,
+                double *sth, double *rhs) {
+    if (!c || !u || !v || !w || !th || !theta || !thbar || !su || !sv || !sw || !rhs)
+        return DCM_BAD_ARG;
+    const double *dth, *dthbar;
+    DCM_CHECK(upload_1d(c, theta, (size_t)c->geo.kp + 2, 1, &dth));
+    DCM_CHECK(upload_1d(c, thbar, (size_t)c->geo.kp + 2, 2, &dthbar));
+    return sources_dev(c, u, v, w, th, dth, dthbar, su, sv, sw, sth, rhs);
+}
+
+int dcm_advance_momentum(dcm_ctx *c, double dt, double *u, double *v, double *w,
+                         const double *zu, const double *zv, const double *zw, const double *su,
+                         const double *sv, const double *sw, const double *p) {
+    if (!c || !u || !v || !w || !zu || !zv || !zw || !su || !sv || !sw || !p) return DCM_BAD_ARG;
+    return advance_momentum_dev(c, dt, u, v, w, zu, zv, zw, su, sv, sw, p);
+}
+
+int dcm_advance_scalars(dcm_ctx *c, double dt, double *q, const double *sq, double *th,
+                        const double *sth, int nq, int moisture) {
+    if (!c || !th || !sth) return DCM_BAD_ARG;
+    const long long n = c->geo.ntot;
+    hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, th, sth, dt, n);
+    DCM_LAUNCH_CHECK();
+    if (moisture && nq > 0) {
+        if (!q || !sq) return DCM_BAD_ARG;
+        hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, q, sq, dt,
+                           n * nq);
+        DCM_LAUNCH_CHECK();
+    }
+    return DCM_OK;
+}
+
+int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_e, double dt,
+                     int kord, int monotone, int advection_scheme, double ptol,
+                     const double *theta, const double *thetan, const double *xg,
+                     const double *yg, const double *zg, dcm_model **out) {
+    if (!c || !out || nq < 0 || ncat < 0 || (ncat > 0 && (!c_s || !c_e)) || !theta || !thetan ||
+        (advection_scheme != 0 && advection_scheme != 1) || kord < 1) {
+        set_error("dcm_model_create: bad arguments");
+        return DCM_BAD_ARG;
+    }
+    for (int q = 0; q < ncat; ++q)
+        if (c_s[q] < 1 || c_e[q] > nq || c_e[q] < c_s[q] || c_e[q] - c_s[q] + 1 > 32) {
+            set_error("dcm_model_create: bad category bounds");
+            return DCM_BAD_ARG;
+        }
+    dcm_model *m = new dcm_model();
+    m->c = c;
+    m->nq = nq;
+    m->ncat = ncat;
+    m->c_s.assign(c_s, c_s + ncat);
+    m->c_e.assign(c_e, c_e + ncat);
+    m->dt = dt;
+    m->kord = kord;
+    m->monotone = monotone;
+    m->advection_scheme = advection_scheme;
+    m->ptol = ptol;
+    const Geo &g = c->geo;
+    const size_t fb = (size_t)g.ntot * sizeof(double);
+    void *p;
+    int st = DCM_OK;
+    for (int d = 0; d < 3 && st == DCM_OK; ++d)
+        for (int l = 0; l < 3 && st == DCM_OK; ++l) {
+            st = model_alloc(m, fb, &p);
+            m->lev[d][l] = (double *)p;
+        }
+    double **singles[7] = {&m->th, &m->p, &m->psrc, &m->su, &m->sv, &m->sw, &m->sth};
+    for (int a = 0; a < 7 && st == DCM_OK; ++a) {
+        st = model_alloc(m, fb, &p);
+        *singles[a] = (double *)p;
+    }
+    if (st == DCM_OK && nq > 0) {
+        st = model_alloc(m, fb * nq, &p);
+        m->q = (double *)p;
+    }
+    if (st == DCM_OK) {
+        st = model_alloc(m, 3 * (g.kp + 2) * sizeof(double), &p);
+        m->oned = (double *)p;
+    }
+    if (st != DCM_OK) {
+        dcm_model_destroy(m);
+        return st;
+    }
+    hipMemcpyAsync(m->theta(), theta, (g.kp + 2) * sizeof(double), hipMemcpyHostToDevice,
+                   c->stream);
+    hipMemcpyAsync(m->thetan(), thetan, (g.kp + 2) * sizeof(double), hipMemcpyHostToDevice,
+                   c->stream);
+    if (xg) m->xg.assign(xg, xg + g.ip + 2);
+    if (yg) m->yg.assign(yg, yg + g.jp + 2);
+    if (zg) m->zg.assign(zg, zg + g.kp + 2);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("dcm_model_create: upload failed");
+        dcm_model_destroy(m);
+        return DCM_HIP_ERROR;
+    }
+    *out = m;
+    return DCM_OK;
+}
+
+int dcm_model_destroy(dcm_model *m) {
+    if (!m) return DCM_BAD_ARG;
+    hipStreamSynchronize(m->c->stream);
+    for (void *p : m->allocs) hipFree(p);
+    if (m->shadow) hipFree(m->shadow);
+    delete m;
+    return DCM_OK;
+}
+
+int dcm_model_set_precision(dcm_model *m, int precision) {
+    if (!m || (precision != DCM_FP64 && precision != DCM_MIXED)) return DCM_BAD_ARG;
+    m->precision = precision;
+    if (precision == DCM_MIXED && !m->shadow) {
+        void *p = nullptr;
+        if (hipMalloc(&p, 3 * (size_t)m->c->geo.ntot * sizeof(float)) != hipSuccess) {
+            set_error("dcm_model_set_precision: out of device memory");
+            return DCM_OUT_OF_MEMORY;
+        }
+        m->shadow = (float *)p;
+    }
+    return DCM_OK;
+}
+
+static int field_ptr(dcm_model *m, int field, int index, double **ptr, size_t *count) {
+    const Geo &g = m->c->geo;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
+    *count = (size_t)g.ntot;
+    switch (field) {
+        case DCM_F_U: *ptr = cur[0]; break;
+        case DCM_F_V: *ptr = cur[1]; break;
+        case DCM_F_W: *ptr = cur[2]; break;
+        case DCM_F_ZU: *ptr = prev[0]; break;
+        case DCM_F_ZV: *ptr = prev[1]; break;
+        case DCM_F_ZW: *ptr = prev[2]; break;
+        case DCM_F_TH: *ptr = m->th; break;
+        case DCM_F_P: *ptr = m->p; break;
+        case DCM_F_PSRC: *ptr = m->psrc; break;
+        case DCM_F_SU: *ptr = m->su; break;
+        case DCM_F_SV: *ptr = m->sv; break;
+        case DCM_F_SW: *ptr = m->sw; break;
+        case DCM_F_STH: *ptr = m->sth; break;
+        case DCM_F_Q:
+            if (index < 0 || index >= m->nq) return DCM_BAD_ARG;
+            *ptr = m->q + (size_t)index * g.ntot;
+            break;
+        case DCM_F_THBAR:
+            *ptr = m->thbar();
+            *count = (size_t)g.kp + 2;
+            break;
+        default:
+            return DCM_BAD_ARG;
+    }
+    return DCM_OK;
+}
+
+int dcm_model_set(dcm_model *m, int field, int index, const double *host) {
+    if (!m || !host) return DCM_BAD_ARG;
+    double *d;
+    size_t n;
+    DCM_CHECK(field_ptr(m, field, index, &d, &n));
+    DCM_HIP(hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, m->c->stream));
+    DCM_HIP(hipStreamSynchronize(m->c->stream));
+    return DCM_OK;
+}
+
+int dcm_model_get(dcm_model *m, int field, int index, double *host) {
+    if (!m || !host) return DCM_BAD_ARG;
+    double *d;
+    size_t n;
+    DCM_CHECK(field_ptr(m, field, index, &d, &n));
+    DCM_HIP(hipMemcpyAsync(host, d, n * sizeof(double), hipMemcpyDeviceToHost, m->c->stream));
+    DCM_HIP(hipStreamSynchronize(m->c->stream));
+    return DCM_OK;
+}
+
+int dcm_model_field(dcm_model *m, int field, int index, double **dptr) {
+    if (!m || !dptr) return DCM_BAD_ARG;
+    size_t n;
+    return field_ptr(m, field, index, dptr, &n);
+}
+
+int dcm_model_init(dcm_model *m) {
+    if (!m) return DCM_BAD_ARG;
+    dcm_ctx *c = m->c;
+    const Geo &g = c->geo;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
+    // initialisation.f90:861-870 (every q field here)
+    for (int d = 0; d < 3; ++d) DCM_CHECK(fill<double>(c, cur[d], FILL_FULL, 0.0, 0.0));
+    DCM_CHECK(fill<double>(c, m->th, FILL_FULL, 0.0, 0.0));
+    for (int n = 0; n < m->nq; ++n)
+        DCM_CHECK(fill<double>(c, m->q + (size_t)n * g.ntot, FILL_FULL, 0.0, 0.0));
+    // previous and future levels start as copies of the current one
+    for (int d = 0; d < 3; ++d) {
+        DCM_HIP(hipMemcpyAsync(prev[d], cur[d], g.ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+        DCM_HIP(hipMemcpyAsync(fut[d], cur[d], g.ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    hipLaunchKernelGGL(k_init_rows, dim3(256), dim3(kBlock), 0, c->stream, prev[0], prev[1],
+                       prev[2], g);
+    hipLaunchKernelGGL(k_init_rows_top, dim3(256), dim3(kBlock), 0, c->stream, prev[0], prev[1],
+                       prev[2], g);
+    DCM_LAUNCH_CHECK();
+    DCM_HIP(hipMemsetAsync(m->sth, 0, g.ntot * sizeof(double), c->stream));
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    return DCM_OK;
+}
+
+// one pass of the model_driver loop, driver_code.f90:304-676 (hot path)
+static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
+    dcm_ctx *c = m->c;
+    const Geo &g = c->geo;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
+    // sth = 0 (:340) stays zero: sources' inviscid branch only fills it, and
+    // the model never writes it; sq is likewise zero, so the scalar update of
+    // :601-603 (q += sq*dt, th += sth*dt) is an exact no-op and is skipped.
+    DCM_CHECK(level_means(c, m->th, m->thbar()));                  // :433-435
+    DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
+                          m->sv, m->sw, nullptr, m->psrc));          // :455-479
+    DCM_CHECK(fill<double>(c, m->psrc, FILL_ALONG_DIM_WO, 0.0, 0.0)); // :485-487
+    SolverOut so;
+    int st;
+    if (m->precision == DCM_MIXED)
+        st = bicgstab_mixed(c, m->sw, m->p, m->psrc, m->ptol, 9999, m->inner_rtol, &so);
+    else {
+        c->first_solve = false;
+        st = bicgstab64(c, m->sw, m->p, m->psrc, m->ptol, 9999, &so);  // :496-500
+    }
+    if (stats) {
+        stats->iters_total += so.iters;
+        if (so.iters > stats->iters_max) stats->iters_max = so.iters;
+        stats->outer_total += so.outer;
+        stats->last_err = so.err;
+    }
+    if (st != DCM_OK) return st;
+    // :506-507 exchange_along_dim_wo(p): the solver already left the faces
+    // periodic, so the fill is an exact no-op here.
+    if (m->advection_scheme == 0 || m->advection_scheme == 1) {
+        DCM_CHECK(adv_ref_state_dev(c, m->dt, cur[2], m->th, m->thetan()));  // :518-520
+        DCM_CHECK(fill<double>(c, m->th, FILL_FULL, 0.0, 0.0));            // :526-527
+    }
+    const bool mixed = m->precision == DCM_MIXED;
+    float *uf = nullptr, *vf = nullptr, *wf = nullptr;
+    if (mixed) {
+        uf = m->shadow;
+        vf = uf + g.ntot;
+        wf = vf + g.ntot;
+        DCM_CHECK(velocity_shadow(c, cur[0], cur[1], cur[2], uf, vf, wf));
+    }
+    const int kord = m->advection_scheme == 0 ? 1 : m->kord;
+    const int mono = m->advection_scheme == 0 ? 0 : m->monotone;
+    {
+        double lb = m->t1, ub = m->t2;
+        if (mixed)
+            st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, m->th, 1, &lb, &ub, kord, mono,
+                                    0, 0);
+        else
+            st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], m->th, 1, &lb, &ub, kord,
+                                     mono, 0, 0);  // :548-551
+        if (st != DCM_OK) return st;
+    }
+    if (m->nq > 0) {  // :571-580 with q1 = q2 = 0
+        for (int cat = 0; cat < m->ncat; ++cat) {
+            const int nf = m->c_e[cat] - m->c_s[cat] + 1;
+            std::vector<double> z(nf, 0.0);
+            double *f = m->q + (size_t)(m->c_s[cat] - 1) * g.ntot;
+            if (mixed)
+                st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, f, nf, z.data(), z.data(), m->kord,
+                                        m->monotone, 0, 1);
+            else
+                st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], f, nf, z.data(),
+                                         z.data(), m->kord, m->monotone, 0, 1);
+            if (st != DCM_OK) return st;
+        }
+    }
+    DCM_CHECK(advance_momentum_dev(c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
+                                   prev[2], m->su, m->sv, m->sw, m->p));  // :592-594
+    {
+        int tok = prof_begin(c, "filter");
+        hipLaunchKernelGGL(k_filter<false>, dim3(4096), dim3(kBlock), 0, c->stream, cur[0],
+                           prev[0], cur[1], prev[1], cur[2], prev[2], g.ntot);  // :618-626
+        DCM_LAUNCH_CHECK();
+        prof_end(c, tok);
+    }
+    m->n += 1;  // :634-646
+    m->levels(cur, prev, fut);
+    for (int d = 0; d < 3; ++d) DCM_CHECK(fill<double>(c, cur[d], FILL_FULL, 0.0, 0.0));
+    hipLaunchKernelGGL(k_momentum_rows, dim3(256), dim3(kBlock), 0, c->stream, cur[0], cur[1],
+                       cur[2], g);  // :663-674
+    DCM_LAUNCH_CHECK();
+    if (stats) stats->steps += 1;
+    return DCM_OK;
+}
+
+int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {
+    if (!m || nsteps < 0) return DCM_BAD_ARG;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    for (int s = 0; s < nsteps; ++s) DCM_CHECK(model_step_once(m, stats));
+    DCM_HIP(hipStreamSynchronize(m->c->stream));
+    DCM_CHECK(prof_collect(m->c));
+    return DCM_OK;
+}
+
+int dcm_model_profile(dcm_model *m, int enable) {
+    if (!m) return DCM_BAD_ARG;
+    DCM_CHECK(prof_collect(m->c));
+    m->c->profiling = enable != 0;
+    m->c->timers.clear();
+    return DCM_OK;
+}
+
+int dcm_model_profile_filter(dcm_model *m, const char *name) {
+    if (!m) return DCM_BAD_ARG;
+    m->c->prof_only = name ? name : "";
+    return DCM_OK;
+}
+
+int dcm_model_set_inner_rtol(dcm_model *m, double rtol) {
+    if (!m || !(rtol > 0.0 && rtol < 1.0)) return DCM_BAD_ARG;
+    m->inner_rtol = rtol;
+    return DCM_OK;
+}
+
+int dcm_model_kernel_time(dcm_model *m, const char *name, double *avg_ms, int *launches) {
+    if (!m || !name) return DCM_BAD_ARG;
+    DCM_CHECK(prof_collect(m->c));
+    auto it = m->c->timers.find(name);
+    if (it == m->c->timers.end()) return DCM_BAD_ARG;
+    if (avg_ms) *avg_ms = it->second.launches ? it->second.total_ms / it->second.launches : 0.0;
+    if (launches) *launches = it->second.launches;
+    return DCM_OK;
+}
+
+}  // extern "C"

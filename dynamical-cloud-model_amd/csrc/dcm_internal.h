@@ -1,0 +1,218 @@
+// dcm_internal.h -- shared declarations of the HIP implementation.
+//
+// Device layout: every 3-D field is the reference's padded Fortran array
+// (0:kp+1, 0:jp+1, 0:ip+1), k fastest.  A wavefront of 64 lanes walks 64
+// consecutive k of one (j,i) column, so every stencil load along k is one
+// coalesced 512-byte access and the j/i neighbours are the same addresses one
+// column (nk*8 bytes) or one plane (nj*nk*8 bytes) away.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dcm_hip.h"
+
+namespace dcm {
+
+void set_error(const std::string &msg);
+
+#define DCM_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                              \
+            ::dcm::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return DCM_HIP_ERROR;                                            \
+        }                                                                    \
+    } while (0)
+
+#define DCM_CHECK(expr)                                                      \
+    do {                                                                     \
+        int s_ = (expr);                                                     \
+        if (s_ != DCM_OK) return s_;                                         \
+    } while (0)
+
+#define DCM_LAUNCH_CHECK()                                                   \
+    do {                                                                     \
+        hipError_t e_ = hipGetLastError();                                   \
+        if (e_ != hipSuccess) {                                              \
+            ::dcm::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return DCM_HIP_ERROR;                                            \
+        }                                                                    \
+    } while (0)
+
+// Grid geometry passed by value to kernels.
+struct Geo {
+    int ip, jp, kp;
+    int nk, nj, ni;       // padded extents kp+2, jp+2, ip+2
+    long long sj, si;     // strides of j and i in elements (nk, nk*nj)
+    long long ntot;       // padded elements
+};
+
+__host__ __device__ inline long long pidx(const Geo &g, int k, int j, int i) {
+    return (long long)i * g.si + (long long)j * g.sj + k;
+}
+
+// Device-side 1-D metrics (length n+2, Fortran index).
+struct Metrics {
+    const double *dx, *dy, *dz, *dxn, *dyn, *dzn, *rhoa, *rhoan;
+    // 1/(dx(i-1)*dxn(i)) style Laplacian coefficients (set_mat_a,
+    // dynamics.f90:84-96), FP64, indexed by the cell index.
+    const double *ae, *aw, *an, *as, *au, *ad;
+};
+
+struct Metrics32 {
+    const float *dx, *dy, *dz, *dxn, *dyn, *dzn, *rhoa, *rhoan;
+    const float *ae, *aw, *an, *as, *au, *ad;
+};
+
+// Scalars of the BiCGSTAB recurrences, resident on the device.
+struct SolverScalars {
+    double rho, nrm, alf, omg, bet;
+    double sc_err, err, bb, rr;
+    double tt, ts;
+    double mean;       // mean removed from an inner right-hand side
+    int breakdown;     // |omega| < tiny
+    int done;          // converged (device-side early exit for chunks)
+    int nonfinite;
+    int pad;
+};
+
+// Per-kernel timing (profiling mode): accumulated over launches.
+struct KernelTimer {
+    double total_ms = 0.0;
+    int launches = 0;
+};
+
+struct Mirror {
+    void *dev = nullptr;
+    size_t bytes = 0;
+};
+
+// Result of one pressure solve.
+struct SolverOut {
+    int iters = 0;   // BiCGSTAB iterations (inner iterations for mixed)
+    int outer = 0;   // refinement sweeps (mixed)
+    double err = 0.0;
+};
+
+}  // namespace dcm
+
+struct dcm_ctx {
+    dcm::Geo geo;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // device metrics (fp64 and fp32 copies), one allocation
+    double *dmet = nullptr;
+    float *dmet32 = nullptr;
+    dcm::Metrics met;
+    dcm::Metrics32 met32;
+    std::vector<double> h_dx, h_dy, h_dz, h_dxn, h_dyn, h_dzn, h_rhoa, h_rhoan;
+    // reduction scratch
+    double *partials = nullptr;      // per-block partial sums (several slots)
+    size_t partials_cap = 0;         // doubles
+    double *dscal = nullptr;         // generic device scalars (64 doubles)
+    double *hscal = nullptr;         // pinned host mirror (64 doubles)
+    dcm::SolverScalars *solver = nullptr;   // device
+    dcm::SolverScalars *h_solver = nullptr; // pinned host
+    // lazily allocated work fields (FP64 padded) and compact solver vectors
+    std::vector<double *> work64;
+    std::vector<float *> work32;
+    double *sv64 = nullptr;          // solver vectors, fp64
+    size_t sv64_count = 0;
+    float *sv32 = nullptr;           // solver vectors, fp32
+    size_t sv32_count = 0;
+    double *hm_partials = nullptr;   // horizontal-mean partials
+    size_t hm_cap = 0;
+    double *d1d = nullptr;           // scratch for 1-D host arrays (theta...)
+    size_t d1d_cap = 0;
+    bool first_solve = true;         // bicgstab_first (dynamics.f90:16)
+    std::unordered_map<const void *, dcm::Mirror> mirrors;
+    // profiling
+    bool profiling = false;
+    std::string prof_only;           // when non-empty, time only this name
+    std::unordered_map<std::string, dcm::KernelTimer> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace dcm {
+
+// ---------------------------------------------------------------------------
+// helpers implemented in dcm_context.hip
+// ---------------------------------------------------------------------------
+int work_field64(dcm_ctx *c, int slot, double **out);   // padded fp64 scratch
+int work_field32(dcm_ctx *c, int slot, float **out);    // padded fp32 scratch
+int ensure_partials(dcm_ctx *c, size_t n);
+int ensure_solver_vectors64(dcm_ctx *c, size_t count);
+int ensure_solver_vectors32(dcm_ctx *c, size_t count);
+int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **out);
+// profiling bracket: returns an event pair index, or -1 when disabled
+int prof_begin(dcm_ctx *c, const char *name);
+void prof_end(dcm_ctx *c, int token);
+int prof_collect(dcm_ctx *c);
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+
+struct Interior {
+    dim3 grid, block;
+};
+inline Interior interior_launch(const Geo &g, int kfirst_extra = 0) {
+    // lanes along k, the rest of the 256 threads along j; one i per z block
+    int nkk = g.kp + kfirst_extra;
+    int bx = nkk >= 64 ? 64 : (nkk >= 32 ? 32 : 16);
+    int by = kBlock / bx;
+    Interior L;
+    L.block = dim3(bx, by, 1);
+    L.grid = dim3((nkk + bx - 1) / bx, (g.jp + by - 1) / by, g.ip);
+    return L;
+}
+inline unsigned blocks_for(long long n, int bs = kBlock) {
+    return (unsigned)((n + bs - 1) / bs);
+}
+
+// ---------------------------------------------------------------------------
+// fills (dcm_fill.hip)
+// ---------------------------------------------------------------------------
+enum FillKind { FILL_FULL = 0, FILL_ALONG_DIM, FILL_ALONG_DIM_WO, FILL_FULL_WO, FILL_ALONG_Z };
+// lbc/ubc either by value or, when bc_shift is non-null, (bc - *bc_shift)
+// bc_mode 1: z planes get bc - *bc_shift; 2: (bc - *bc_shift) + *bc_shift.
+// skip_if: when non-null and *skip_if != 0 on the device, the fill is a no-op.
+template <typename T>
+int fill(dcm_ctx *c, T *a, FillKind kind, double lbc, double ubc,
+         const double *bc_shift = nullptr, int bc_mode = 0, const int *skip_if = nullptr);
+// Neumann z rows for the matvec: x(0,:,:)=x(1,:,:), x(kp+1,:,:)=x(kp,:,:)
+int fill_neumann_z(dcm_ctx *c, double *x);
+
+// ---------------------------------------------------------------------------
+// reductions (dcm_reduce.hip)
+// ---------------------------------------------------------------------------
+// finalize a set of `nsums` partial-sum slots of `nblocks` blocks each into
+// dscal[dst + s] (deterministic order)
+int finalize_sums(dcm_ctx *c, const double *partials, int nblocks, int nsums,
+                  double *dst);
+int finalize_min(dcm_ctx *c, const double *partials, int nblocks, double *dst);
+
+// ---------------------------------------------------------------------------
+// cross-file entry points
+// ---------------------------------------------------------------------------
+template <typename R>
+int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *psi, int nf,
+                const double *lbc, const double *ubc, int kord, int monotone, int neumann,
+                int interior_min);
+int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
+                    float *vf, float *wf);
+int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref_dev);
+int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
+               SolverOut *out);
+int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
+                   double inner_rtol, SolverOut *out);
+
+}  // namespace dcm

@@ -1,0 +1,351 @@
+"""ctypes binding of the C-ABI library ``libdcm_hip.so`` (include/dcm_hip.h).
+
+This is the Python host of the HIP path: the tests and the benchmark drive
+the kernels through exactly the entry points a Fortran host binds.  Device
+memory is owned by the library (``dcm_malloc``), so no other GPU runtime is
+loaded into the process.  There is no CPU fallback: if the library cannot be
+loaded or no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libdcm_hip.so")
+
+STATUS = {0: "DCM_OK", 1: "DCM_NOT_CONVERGED", 2: "DCM_BREAKDOWN", 3: "DCM_NONFINITE",
+          4: "DCM_BAD_ARG", 5: "DCM_HIP_ERROR", 6: "DCM_NO_DEVICE", 7: "DCM_OUT_OF_MEMORY"}
+
+FP64, MIXED = 0, 1
+(F_U, F_V, F_W, F_ZU, F_ZV, F_ZW, F_TH, F_P, F_PSRC, F_SU, F_SV, F_SW, F_STH, F_Q,
+ F_THBAR) = range(15)
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_d = C.c_double
+_i = C.c_int
+
+
+class DcmError(RuntimeError):
+    def __init__(self, fn, status, msg=""):
+        self.status = status
+        super().__init__(f"{fn} failed: {STATUS.get(status, status)} {msg}")
+
+
+class StepStats(C.Structure):
+    _fields_ = [("steps", _i), ("iters_total", _i), ("iters_max", _i), ("outer_total", _i),
+                ("last_err", _d), ("max_true_resid", _d)]
+
+
+_SIGS = {
+    "dcm_last_error": ([], C.c_char_p),
+    "dcm_version": ([], C.c_char_p),
+    "dcm_context_create": ([_i, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                            C.POINTER(_vp)], _i),
+    "dcm_context_destroy": ([_vp], _i),
+    "dcm_field_elems": ([_vp], C.c_size_t),
+    "dcm_malloc": ([_vp, C.c_size_t, C.POINTER(_vp)], _i),
+    "dcm_free": ([_vp, _vp], _i),
+    "dcm_memcpy_htod": ([_vp, _vp, _vp, C.c_size_t], _i),
+    "dcm_memcpy_dtoh": ([_vp, _vp, _vp, C.c_size_t], _i),
+    "dcm_synchronize": ([_vp], _i),
+    "dcm_mirror": ([_vp, _vp, C.c_size_t, C.POINTER(_vp)], _i),
+    "dcm_mirror_upload": ([_vp, _vp], _i),
+    "dcm_mirror_download": ([_vp, _vp], _i),
+    "dcm_mirror_release_all": ([_vp], _i),
+    "dcm_fill_full": ([_vp, _vp, _d, _d], _i),
+    "dcm_fill_along_dim": ([_vp, _vp, _d, _d], _i),
+    "dcm_fill_along_dim_wo": ([_vp, _vp], _i),
+    "dcm_fill_full_wo": ([_vp, _vp], _i),
+    "dcm_fill_along_z": ([_vp, _vp, _d, _d], _i),
+    "dcm_upwind3d": ([_vp, _d, _vp, _vp, _vp, _vp, _i], _i),
+    "dcm_mpdata3d": ([_vp, _d, _vp, _vp, _vp, _vp, _dp, _dp, _i, _i, _i, _ip], _i),
+    "dcm_mpdata_vec3d": ([_vp, _d, _vp, _vp, _vp, _vp, _i, _dp, _dp, _i, _i, _i, _ip], _i),
+    "dcm_adv_ref_state": ([_vp, _d, _vp, _vp, _dp], _i),
+    "dcm_hmeans": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _dp, _dp, _dp, _dp], _i),
+    "dcm_sources": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _vp, _vp, _vp, _vp, _vp], _i),
+    "dcm_matvec": ([_vp, _vp, _vp], _i),
+    "dcm_precondition": ([_vp, _vp, _vp, _i], _i),
+    "dcm_inn_prod": ([_vp, _vp, _vp, _dp], _i),
+    "dcm_bicgstab": ([_vp, _dp, _dp, _dp, _vp, _vp, _vp, _d, _i, _i, _ip, _dp], _i),
+    "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
+    "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
+    "dcm_advance_momentum": ([_vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "dcm_advance_scalars": ([_vp, _d, _vp, _vp, _vp, _vp, _i, _i], _i),
+    "dcm_model_create": ([_vp, _i, _i, _ip, _ip, _d, _i, _i, _i, _d, _dp, _dp, _dp, _dp, _dp,
+                          C.POINTER(_vp)], _i),
+    "dcm_model_destroy": ([_vp], _i),
+    "dcm_model_set_precision": ([_vp, _i], _i),
+    "dcm_model_set": ([_vp, _i, _i, _dp], _i),
+    "dcm_model_get": ([_vp, _i, _i, _dp], _i),
+    "dcm_model_field": ([_vp, _i, _i, C.POINTER(_vp)], _i),
+    "dcm_model_init": ([_vp], _i),
+    "dcm_model_step": ([_vp, _i, C.POINTER(StepStats)], _i),
+    "dcm_model_profile": ([_vp, _i], _i),
+    "dcm_model_profile_filter": ([_vp, C.c_char_p], _i),
+    "dcm_model_set_inner_rtol": ([_vp, _d], _i),
+    "dcm_model_kernel_time": ([_vp, C.c_char_p, _dp, _ip], _i),
+}
+
+_lib = None
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def load():
+    """Load libdcm_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _call(name, *args):
+    st = getattr(load(), name)(*args)
+    if st != 0:
+        raise DcmError(name, st, load().dcm_last_error().decode())
+    return st
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class DeviceArray:
+    """A device buffer owned by the library, with a numpy shape."""
+
+    def __init__(self, ctx: "Context", shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        _call("dcm_malloc", ctx.h, max(self.nbytes, 8), C.byref(p))
+        self.ptr = p
+
+    def set(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _call("dcm_memcpy_htod", self.ctx.h, self.ptr, a.ctypes.data_as(_vp), self.nbytes)
+        return self
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _call("dcm_memcpy_dtoh", self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            _call("dcm_free", self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """A dcm_ctx for one workloads.Domain."""
+
+    def __init__(self, dom):
+        self.dom = dom
+        self._keep = [np.ascontiguousarray(getattr(dom, n), dtype=np.float64)
+                      for n in ("dx", "dy", "dz", "dxn", "dyn", "dzn", "rhoa", "rhoan")]
+        h = _vp()
+        _call("dcm_context_create", dom.ip, dom.jp, dom.kp, *[_dptr(a) for a in self._keep],
+              C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _call("dcm_context_destroy", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- memory ---------------------------------------------------------------
+    def array(self, a: np.ndarray) -> DeviceArray:
+        return DeviceArray(self, a.shape, a.dtype).set(a)
+
+    def zeros(self, shape=None) -> DeviceArray:
+        shape = self.dom.shape if shape is None else shape
+        return DeviceArray(self, shape).set(np.zeros(shape))
+
+    def sync(self):
+        _call("dcm_synchronize", self.h)
+
+    # -- a1 fills -------------------------------------------------------------
+    def fill_full(self, a, lbc=0.0, ubc=0.0):
+        _call("dcm_fill_full", self.h, a.ptr, lbc, ubc)
+
+    def fill_along_dim(self, a, lbc=0.0, ubc=0.0):
+        _call("dcm_fill_along_dim", self.h, a.ptr, lbc, ubc)
+
+    def fill_along_dim_wo(self, a):
+        _call("dcm_fill_along_dim_wo", self.h, a.ptr)
+
+    def fill_full_wo(self, a):
+        _call("dcm_fill_full_wo", self.h, a.ptr)
+
+    def fill_along_z(self, a, lbc=0.0, ubc=0.0):
+        _call("dcm_fill_along_z", self.h, a.ptr, lbc, ubc)
+
+    # -- a2-a5 advection ------------------------------------------------------
+    def upwind3d(self, dt, u, v, w, psi, neumann=0):
+        _call("dcm_upwind3d", self.h, dt, u.ptr, v.ptr, w.ptr, psi.ptr, neumann)
+
+    def mpdata3d(self, dt, u, v, w, psi, lbc=0.0, ubc=0.0, kord=2, monotone=True, neumann=0):
+        lb, ub, ex = _d(lbc), _d(ubc), _i(0)
+        _call("dcm_mpdata3d", self.h, dt, u.ptr, v.ptr, w.ptr, psi.ptr, C.byref(lb),
+              C.byref(ub), kord, int(monotone), neumann, C.byref(ex))
+        return ex.value, lb.value, ub.value
+
+    def mpdata_vec3d(self, dt, u, v, w, psi, nq, lbc=None, ubc=None, kord=2, monotone=True,
+                     neumann=0):
+        lb = np.zeros(nq) if lbc is None else np.array(lbc, dtype=np.float64)
+        ub = np.zeros(nq) if ubc is None else np.array(ubc, dtype=np.float64)
+        ex = _i(0)
+        _call("dcm_mpdata_vec3d", self.h, dt, u.ptr, v.ptr, w.ptr, psi.ptr, nq, _dptr(lb),
+              _dptr(ub), kord, int(monotone), neumann, C.byref(ex))
+        return ex.value, lb, ub
+
+    def adv_ref_state(self, dt, w, psi, ref):
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        _call("dcm_adv_ref_state", self.h, dt, w.ptr, psi.ptr, _dptr(ref))
+
+    # -- a6, a7 ---------------------------------------------------------------
+    def hmeans(self, u, v, w, th, q=None, nq=0):
+        n = self.dom.kp + 2
+        out = [np.zeros(n) for _ in range(4)]
+        qbar = np.zeros(max(nq, 1) * n)
+        _call("dcm_hmeans", self.h, u.ptr, v.ptr, w.ptr, th.ptr, q.ptr if q else None, nq,
+              *[_dptr(a) for a in out], _dptr(qbar))
+        return out + [qbar[: nq * n].reshape(nq, n)]
+
+    def sources(self, u, v, w, th, theta, thbar, su, sv, sw, sth, rhs):
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        thbar = np.ascontiguousarray(thbar, dtype=np.float64)
+        _call("dcm_sources", self.h, u.ptr, v.ptr, w.ptr, th.ptr, _dptr(theta), _dptr(thbar),
+              su.ptr, sv.ptr, sw.ptr, sth.ptr if sth else None, rhs.ptr)
+
+    # -- a8-a12 solver --------------------------------------------------------
+    def matvec(self, x, ax):
+        _call("dcm_matvec", self.h, x.ptr, ax.ptr)
+
+    def precondition(self, s, r, prit=3):
+        _call("dcm_precondition", self.h, s.ptr, r.ptr, prit)
+
+    def inn_prod(self, x, y) -> float:
+        out = _d()
+        _call("dcm_inn_prod", self.h, x.ptr, y.ptr, C.byref(out))
+        return out.value
+
+    def bicgstab(self, sw, x, b, tol=1e-10, test_solver=False, itmax=0, check=True):
+        d = self.dom
+        it, err = _i(0), _d(0.0)
+        keep = [np.ascontiguousarray(a) for a in (d.x, d.y, d.z)]
+        st = load().dcm_bicgstab(self.h, *[_dptr(a) for a in keep], sw.ptr, x.ptr, b.ptr, tol,
+                                 int(test_solver), itmax, C.byref(it), C.byref(err))
+        if check and st != 0:
+            raise DcmError("dcm_bicgstab", st, load().dcm_last_error().decode())
+        return st, it.value, err.value
+
+    def bicgstab_mixed(self, sw, x, b, tol=1e-10, itmax=0):
+        it, outer, err = _i(0), _i(0), _d(0.0)
+        _call("dcm_bicgstab_mixed", self.h, sw.ptr, x.ptr, b.ptr, tol, itmax, C.byref(it),
+              C.byref(outer), C.byref(err))
+        return it.value, outer.value, err.value
+
+    def true_residual(self, x, b) -> float:
+        out = _d()
+        _call("dcm_true_residual", self.h, x.ptr, b.ptr, C.byref(out))
+        return out.value
+
+    # -- a13, a14 -------------------------------------------------------------
+    def advance_momentum(self, dt, u, v, w, zu, zv, zw, su, sv, sw, p):
+        _call("dcm_advance_momentum", self.h, dt, u.ptr, v.ptr, w.ptr, zu.ptr, zv.ptr, zw.ptr,
+              su.ptr, sv.ptr, sw.ptr, p.ptr)
+
+    def advance_scalars(self, dt, q, sq, th, sth, nq, moisture):
+        _call("dcm_advance_scalars", self.h, dt, q.ptr if q else None, sq.ptr if sq else None,
+              th.ptr, sth.ptr, nq, int(moisture))
+
+
+class Model:
+    """dcm_model: the model_driver time step on device-resident state."""
+
+    def __init__(self, ctx: Context, case, precision=FP64):
+        self.ctx = ctx
+        self.case = case
+        d = case.dom
+        ncat = case.ncat
+        cs = (_i * max(ncat, 1))(*(case.c_s or (1,)))
+        ce = (_i * max(ncat, 1))(*(case.c_e or (1,)))
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64)
+                      for a in (d.theta, d.thetan, d.x, d.y, d.z)]
+        h = _vp()
+        _call("dcm_model_create", ctx.h, case.nq, ncat, cs, ce, case.dt, case.kord,
+              int(case.monotone), case.advection_scheme, case.ptol,
+              *[_dptr(a) for a in self._keep], C.byref(h))
+        self.h = h
+        self.set_precision(precision)
+        self.set(F_U, case.u)
+        self.set(F_V, case.v)
+        self.set(F_W, case.w)
+        self.set(F_TH, case.th)
+        for n in range(case.nq):
+            self.set(F_Q, case.q[n], n)
+        _call("dcm_model_init", h)
+
+    def set_precision(self, precision):
+        _call("dcm_model_set_precision", self.h, precision)
+
+    def set_inner_rtol(self, rtol):
+        _call("dcm_model_set_inner_rtol", self.h, rtol)
+
+    def set(self, field, a, index=0):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _call("dcm_model_set", self.h, field, index, _dptr(a))
+
+    def get(self, field, index=0) -> np.ndarray:
+        if field == F_THBAR:
+            out = np.empty(self.case.dom.kp + 2)
+        else:
+            out = np.empty(self.case.dom.shape)
+        _call("dcm_model_get", self.h, field, index, _dptr(out))
+        return out
+
+    def step(self, n=1) -> StepStats:
+        st = StepStats()
+        _call("dcm_model_step", self.h, n, C.byref(st))
+        return st
+
+    def profile(self, enable=True, only=None):
+        _call("dcm_model_profile", self.h, int(enable))
+        _call("dcm_model_profile_filter", self.h, only.encode() if only else None)
+
+    def kernel_time(self, name):
+        avg, n = _d(), _i()
+        st = load().dcm_model_kernel_time(self.h, name.encode(), C.byref(avg), C.byref(n))
+        if st != 0:
+            return None, 0
+        return avg.value, n.value
+
+    def close(self):
+        if self.h:
+            _call("dcm_model_destroy", self.h)
+            self.h = None

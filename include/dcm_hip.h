@@ -1,0 +1,230 @@
+/*
+ * dcm_hip.h -- C-ABI of the MI355X dynamical-core hot path (libdcm_hip.so).
+ *
+ * Every entry point replaces one routine of the reference Fortran model
+ * (UoM-maul1609/dynamical-cloud-model) for a domain that lives on one GPU.
+ * The Fortran host binds these with ISO_C_BINDING (module dcm_hip,
+ * fortran/dcm_hip.f90); Python binds them with ctypes (dcm_amd/hip.py).
+ *
+ * Conventions
+ *   - All 3-D field arguments are DEVICE pointers to FP64 arrays in the
+ *     reference's layout: Fortran (0:kp+1, 0:jp+1, 0:ip+1), column major,
+ *     k fastest, halo width 1.  The element (k,j,i) is at offset
+ *     (i*(jp+2) + j)*(kp+2) + k.
+ *   - 1-D arrays (metrics, base state) are HOST pointers of length n+2
+ *     (Fortran index 0..n+1) unless stated otherwise; the context keeps
+ *     device copies.
+ *   - Scalars passed by pointer (lbc, ubc, iteration counts) are HOST
+ *     pointers, updated like the Fortran INTENT(INOUT) arguments.
+ *   - Every call is enqueued on the context's stream and returns a dcm_status;
+ *     calls that return host-visible results synchronise that stream.
+ *   - One context serves one domain on the device that was current when it
+ *     was created; calls on one context are not re-entrant.
+ */
+#ifndef DCM_HIP_H
+#define DCM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DCM_OK = 0,
+    DCM_NOT_CONVERGED = 1, /* dynamics.f90:431-434 ("convergence failure")  */
+    DCM_BREAKDOWN = 2,     /* dynamics.f90:414-417 (|omega| < 1e-16)         */
+    DCM_NONFINITE = 3,     /* a NaN/Inf residual (the reference loops on)    */
+    DCM_BAD_ARG = 4,
+    DCM_HIP_ERROR = 5,
+    DCM_NO_DEVICE = 6,
+    DCM_OUT_OF_MEMORY = 7  /* the reference's "Not enough memory" stops      */
+} dcm_status;
+
+typedef enum { DCM_FP64 = 0, DCM_MIXED = 1 } dcm_precision;
+
+typedef struct dcm_ctx dcm_ctx;
+typedef struct dcm_model dcm_model;
+
+/* Text of the last error seen by this thread ("" if none). */
+const char *dcm_last_error(void);
+/* Version string of the library build. */
+const char *dcm_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Context: grid metrics and module-level state.                          */
+/* Replaces the module state of d_solver (dynamics.f90:15-17, the saved   */
+/* coefficient arrays and bicgstab_first) and the grid arrays that every  */
+/* routine receives (initialisation.f90:429-446, 539, 573).               */
+/* ---------------------------------------------------------------------- */
+int dcm_context_create(int ip, int jp, int kp, const double *dx,
+                       const double *dy, const double *dz, const double *dxn,
+                       const double *dyn, const double *dzn,
+                       const double *rhoa, const double *rhoan,
+                       dcm_ctx **out);
+int dcm_context_destroy(dcm_ctx *ctx);
+/* Number of padded elements (kp+2)*(jp+2)*(ip+2) of one 3-D field. */
+size_t dcm_field_elems(const dcm_ctx *ctx);
+
+/* Device memory and transfers on the context's stream. */
+int dcm_malloc(dcm_ctx *ctx, size_t bytes, void **dptr);
+int dcm_free(dcm_ctx *ctx, void *dptr);
+int dcm_memcpy_htod(dcm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int dcm_memcpy_dtoh(dcm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int dcm_synchronize(dcm_ctx *ctx);
+
+/* Host-array mirrors for the Fortran drop-in: returns (creating on first
+ * use) the device copy of a host array of `bytes` bytes. */
+int dcm_mirror(dcm_ctx *ctx, const void *host, size_t bytes, void **dptr);
+int dcm_mirror_upload(dcm_ctx *ctx, const void *host);
+int dcm_mirror_download(dcm_ctx *ctx, void *host);
+int dcm_mirror_release_all(dcm_ctx *ctx);
+
+/* ---------------------------------------------------------------------- */
+/* a1. Boundary fills (one rank, halo 1), replacing the MPI halo swaps.   */
+/* ---------------------------------------------------------------------- */
+/* exchange_full (mpi_module.f90:926-1271): faces periodic in x,y; z planes
+ * = lbc/ubc; edges and corners touching a z plane take z-wrapped values. */
+int dcm_fill_full(dcm_ctx *ctx, double *a, double lbc, double ubc);
+/* exchange_along_dim (mpi_module.f90:700-910): faces + z planes only. */
+int dcm_fill_along_dim(dcm_ctx *ctx, double *a, double lbc, double ubc);
+/* exchange_along_dim_wo (mpi_module.f90:1285-1494): x,y faces only. */
+int dcm_fill_along_dim_wo(dcm_ctx *ctx, double *a);
+/* exchange_full_wo (mpi_module.f90:1511-1855): as full, no z planes. */
+int dcm_fill_full_wo(dcm_ctx *ctx, double *a);
+/* exchange_along_z (mpi_module.f90:584-685): z planes only. */
+int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
+
+/* ---------------------------------------------------------------------- */
+/* a2-a5. Advection (pamm/sfvt/advection_3d.F90, module advection_s_3d).  */
+/* ---------------------------------------------------------------------- */
+/* first_order_upstream_3d (:39-116): donor cell, psi updated in place. */
+int dcm_upwind3d(dcm_ctx *ctx, double dt, const double *u, const double *v,
+                 const double *w, double *psi, int neumann);
+/* mpdata_3d (:219-714): positive-definite shift, kord passes, optional
+ * non-oscillatory limiter.  *lbc/*ubc are shifted and restored as in the
+ * reference.  *early_exit (may be NULL) reports the zero-sum exit. */
+int dcm_mpdata3d(dcm_ctx *ctx, double dt, const double *u, const double *v,
+                 const double *w, double *psi, double *lbc, double *ubc,
+                 int kord, int monotone, int neumann, int *early_exit);
+/* mpdata_vec_3d (:1250-1753): nq contiguous padded fields share field 1's
+ * antidiffusive and limited velocities. lbc/ubc are arrays of nq. */
+int dcm_mpdata_vec3d(dcm_ctx *ctx, double dt, const double *u,
+                     const double *v, const double *w, double *psi, int nq,
+                     double *lbc, double *ubc, int kord, int monotone,
+                     int neumann, int *early_exit);
+/* adv_ref_state (:1148-1218): psi_ref is a host array (kp+2). */
+int dcm_adv_ref_state(dcm_ctx *ctx, double dt, const double *w, double *psi,
+                      const double *psi_ref);
+
+/* ---------------------------------------------------------------------- */
+/* a6. horizontal_means (diagnostics.f90:31-107).  Outputs are HOST arrays */
+/* of length kp+2 (qbar: nq*(kp+2), field-major); any may be NULL.        */
+/* ---------------------------------------------------------------------- */
+int dcm_hmeans(dcm_ctx *ctx, const double *u, const double *v,
+               const double *w, const double *th, const double *q, int nq,
+               double *ubar, double *vbar, double *wbar, double *thbar,
+               double *qbar);
+
+/* ---------------------------------------------------------------------- */
+/* a7. sources (dynamics.f90:639-1000), inviscid branch with damping,     */
+/* forcing and subsidence off.  theta/thbar are HOST arrays (kp+2).       */
+/* ---------------------------------------------------------------------- */
+int dcm_sources(dcm_ctx *ctx, const double *u, const double *v,
+                const double *w, const double *th, const double *theta,
+                const double *thbar, double *su, double *sv, double *sw,
+                double *sth, double *rhs);
+
+/* ---------------------------------------------------------------------- */
+/* a8-a12. Pressure solver (dynamics.f90:24-482).                         */
+/* ---------------------------------------------------------------------- */
+/* mat_ax (:187-229): fills the halo of x like the reference, ax = A x. */
+int dcm_matvec(dcm_ctx *ctx, double *x, double *ax);
+/* precondition (:241-270): prit Jacobi sweeps, s = M r. */
+int dcm_precondition(dcm_ctx *ctx, double *s, const double *r, int prit);
+/* inn_prod (:458-482): interior dot product, result to *out (host). */
+int dcm_inn_prod(dcm_ctx *ctx, const double *x, const double *y, double *out);
+/* bicgstab (:293-444).  xg,yg,zg are host coordinate arrays used only for
+ * the test_solver right-hand side on the context's first call.  itmax <= 0
+ * means the reference's 9999.  *iters / *err (may be NULL) receive the
+ * iteration count and the final recursive residual. */
+int dcm_bicgstab(dcm_ctx *ctx, const double *xg, const double *yg,
+                 const double *zg, const double *sw, double *x, double *b,
+                 double tol, int test_solver, int itmax, int *iters,
+                 double *err);
+/* Mixed-precision variant: FP32 BiCGSTAB inner solves inside FP64
+ * iterative refinement on the true residual ||b-Ax||/max(||b||,0.01).
+ * *iters receives the total inner iterations, *outer the refinements. */
+int dcm_bicgstab_mixed(dcm_ctx *ctx, const double *sw, double *x, double *b,
+                       double tol, int itmax, int *iters, int *outer,
+                       double *err);
+/* True residual ||b-Ax||_2 / max(||b||_2, 0.01) of a padded x (host out). */
+int dcm_true_residual(dcm_ctx *ctx, const double *x, const double *b,
+                      double *out);
+
+/* a13. advance_momentum (dynamics.f90:1019-1099); dt is the leapfrog 2*dt. */
+int dcm_advance_momentum(dcm_ctx *ctx, double dt, double *u, double *v,
+                         double *w, const double *zu, const double *zv,
+                         const double *zw, const double *su, const double *sv,
+                         const double *sw, const double *p);
+/* a14. advance_scalar_fields_3d (sgm/subgrid_3d.f90:582-623). */
+int dcm_advance_scalars(dcm_ctx *ctx, double dt, double *q, const double *sq,
+                        double *th, const double *sth, int nq, int moisture);
+
+/* ---------------------------------------------------------------------- */
+/* a15. The model_driver time step (driver_code.f90:287-295, 304-676) on  */
+/* device-resident state.  The model owns every field; the host reads or  */
+/* writes them with dcm_model_get / dcm_model_set (host arrays).          */
+/* ---------------------------------------------------------------------- */
+enum {
+    DCM_F_U = 0, DCM_F_V, DCM_F_W,        /* current momentum level        */
+    DCM_F_ZU, DCM_F_ZV, DCM_F_ZW,         /* previous level                */
+    DCM_F_TH, DCM_F_P, DCM_F_PSRC,
+    DCM_F_SU, DCM_F_SV, DCM_F_SW, DCM_F_STH,
+    DCM_F_Q,                              /* q field index in `index`      */
+    DCM_F_THBAR,                          /* 1-D (kp+2)                    */
+    DCM_F_COUNT
+};
+
+typedef struct {
+    int steps;            /* steps completed by the call                    */
+    int iters_total;      /* BiCGSTAB iterations (inner, for mixed)         */
+    int iters_max;        /* largest per-step iteration count               */
+    int outer_total;      /* refinement sweeps (mixed only)                 */
+    double last_err;      /* last step's final residual                     */
+    double max_true_resid;/* largest FP64 true residual seen (0 if unchecked)*/
+} dcm_step_stats;
+
+/* theta/thetan/xg/yg/zg: host 1-D arrays; c_s/c_e: 1-based category bounds
+ * into q (driver_code.f90:571-580).  advection_scheme: 0 upwind, 1 mpdata. */
+int dcm_model_create(dcm_ctx *ctx, int nq, int ncat, const int *c_s,
+                     const int *c_e, double dt, int kord, int monotone,
+                     int advection_scheme, double ptol, const double *theta,
+                     const double *thetan, const double *xg, const double *yg,
+                     const double *zg, dcm_model **out);
+int dcm_model_destroy(dcm_model *m);
+int dcm_model_set_precision(dcm_model *m, int precision);
+/* host <-> device copies of one model field (index: q field, else 0) */
+int dcm_model_set(dcm_model *m, int field, int index, const double *host);
+int dcm_model_get(dcm_model *m, int field, int index, double *host);
+/* device pointer of a model field (for zero-copy callers) */
+int dcm_model_field(dcm_model *m, int field, int index, double **dptr);
+/* Initial halos (initialisation.f90:861-870) and the previous-level rows of
+ * driver_code.f90:287-295; call once after the fields are set. */
+int dcm_model_init(dcm_model *m);
+/* Advance nsteps time steps; stats may be NULL. */
+int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats);
+/* Average kernel time (ms) of the named kernel over the last timed region,
+ * and the number of launches it had; returns DCM_BAD_ARG if unknown. */
+int dcm_model_profile(dcm_model *m, int enable);
+/* Time only the kernel group `name` (NULL: every group). */
+int dcm_model_profile_filter(dcm_model *m, const char *name);
+/* Relative residual reduction of each FP32 inner solve (mixed; default 1e-5). */
+int dcm_model_set_inner_rtol(dcm_model *m, double rtol);
+int dcm_model_kernel_time(dcm_model *m, const char *name, double *avg_ms,
+                          int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,342 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Every routine is called on the same seeded inputs through libdcm_hip.so and
+through the oracle.  Routines without a reduction must agree bit for bit (the
+FP64 kernels evaluate the reference's expressions in its operand order with
+no FMA contraction); routines with a global sum (horizontal means, dot
+products, the BiCGSTAB trajectory, the model step) must agree to the stated
+relative tolerance, since only their summation order differs.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from dcm_amd import hip as H
+from dcm_amd import workloads as W
+from helpers import field, interior, lateral_edge_mask, rel_l2, small_domain
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 6, 5), (12, 10, 8), (5, 4, 16), (16, 16, 16)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle_built():
+    O.build()
+
+
+def _ctx(dom):
+    return H.Context(dom)
+
+
+def _vel(dom, seed, courant=0.3, dt=1.0):
+    """Velocities with |c| <= courant, filled like the reference's state."""
+    g = O.Grid(dom)
+    u = field(dom, seed, -1, 1) * courant * dom.dx[1] / dt
+    v = field(dom, seed + 1, -1, 1) * courant * dom.dy[1] / dt
+    w = field(dom, seed + 2, -1, 1) * courant * dom.dz[1] / dt
+    for a in (u, v, w):
+        O.fill_full(g, a, 0.0, 0.0)
+    return u, v, w
+
+
+# ---------------------------------------------------------------- a1 fills
+@pytest.mark.parametrize("shape", SHAPES + [(1, 3, 4), (3, 1, 2)])
+@pytest.mark.parametrize("kind", ["full", "along_dim", "along_dim_wo", "full_wo", "along_z"])
+def test_fills_bitwise(shape, kind):
+    dom = small_domain(*shape)
+    g = O.Grid(dom)
+    a = field(dom, 11)
+    with _ctx(dom) as c:
+        d = c.array(a)
+        if kind in ("full", "along_dim", "along_z"):
+            getattr(c, "fill_" + kind)(d, 0.25, -0.5)
+            getattr(O, "fill_" + kind)(g, a, 0.25, -0.5)
+        else:
+            getattr(c, "fill_" + kind)(d)
+            getattr(O, "fill_" + kind)(g, a)
+        np.testing.assert_array_equal(d.get(), a)
+
+
+# ---------------------------------------------------------------- a2 upwind
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("neumann", [0, 1, 2])
+def test_upwind_bitwise(shape, neumann):
+    dom = small_domain(*shape, stretched=True)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 21)
+    psi = field(dom, 24, 0.0, 2.0)
+    O.fill_full(g, psi, 0.1, 0.2)
+    with _ctx(dom) as c:
+        du, dv, dw, dpsi = c.array(u), c.array(v), c.array(w), c.array(psi)
+        c.upwind3d(1.0, du, dv, dw, dpsi, neumann)
+        O.upwind3d(g, 1.0, u, v, w, psi, neumann)
+        np.testing.assert_array_equal(dpsi.get(), psi)
+
+
+# ---------------------------------------------------------------- a3/a4 MPDATA
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kord,monotone", [(2, True), (2, False), (3, True), (3, False), (1, True)])
+def test_mpdata_bitwise(shape, kord, monotone):
+    dom = small_domain(*shape, stretched=True, varying_theta=True)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 31)
+    psi = field(dom, 34, -1.0, 3.0)
+    O.fill_full(g, psi, 0.0, 0.0)
+    with _ctx(dom) as c:
+        du, dv, dw, dpsi = c.array(u), c.array(v), c.array(w), c.array(psi)
+        ex, lb, ub = c.mpdata3d(1.0, du, dv, dw, dpsi, 0.0, 0.0, kord, monotone, 0)
+        r, olb, oub = O.mpdata3d(g, 1.0, u, v, w, psi, 0.0, 0.0, kord, monotone, 0)
+        assert ex == r == 0
+        assert (lb, ub) == (olb, oub)
+        np.testing.assert_array_equal(dpsi.get(), psi)
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_mpdata_nonzero_bc_and_neumann(shape):
+    dom = small_domain(*shape)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 41)
+    psi = field(dom, 44, 280.0, 300.0)
+    O.fill_full(g, psi, 290.0, 295.0)
+    with _ctx(dom) as c:
+        du, dv, dw, dpsi = c.array(u), c.array(v), c.array(w), c.array(psi)
+        ex, lb, ub = c.mpdata3d(0.7, du, dv, dw, dpsi, 290.0, 295.0, 2, True, 1)
+        r, olb, oub = O.mpdata3d(g, 0.7, u, v, w, psi, 290.0, 295.0, 2, True, 1)
+        assert (lb, ub) == (olb, oub)
+        np.testing.assert_array_equal(dpsi.get(), psi)
+
+
+def test_mpdata_zero_sum_exit():
+    """A constant field takes the early exit (advection_3d.F90:284-289)."""
+    dom = small_domain(8, 6, 5)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 51)
+    psi = np.full(dom.shape, 3.5)
+    with _ctx(dom) as c:
+        du, dv, dw, dpsi = c.array(u), c.array(v), c.array(w), c.array(psi)
+        ex, lb, ub = c.mpdata3d(1.0, du, dv, dw, dpsi, 3.5, 3.5)
+        r, olb, oub = O.mpdata3d(g, 1.0, u, v, w, psi, 3.5, 3.5)
+        assert ex == 1 and r == 1
+        np.testing.assert_array_equal(dpsi.get(), psi)
+
+
+@pytest.mark.parametrize("nq", [1, 3])
+@pytest.mark.parametrize("kord", [2, 3])
+def test_mpdata_vec_bitwise(nq, kord):
+    dom = small_domain(12, 10, 8, stretched=True)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 61)
+    q = np.stack([field(dom, 70 + n, 0.0, 1.0 + n) for n in range(nq)])
+    for n in range(nq):
+        O.fill_full(g, q[n], 0.0, 0.0)
+    with _ctx(dom) as c:
+        du, dv, dw, dq = c.array(u), c.array(v), c.array(w), c.array(q)
+        ex, lb, ub = c.mpdata_vec3d(1.0, du, dv, dw, dq, nq, kord=kord)
+        r, olb, oub = O.mpdata_vec3d(g, 1.0, u, v, w, q, kord=kord)
+        np.testing.assert_array_equal(lb, olb)
+        np.testing.assert_array_equal(dq.get(), q)
+
+
+def test_adv_ref_state_bitwise():
+    dom = small_domain(12, 10, 8, varying_theta=True)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 81)
+    psi = field(dom, 84)
+    with _ctx(dom) as c:
+        dw, dpsi = c.array(w), c.array(psi)
+        c.adv_ref_state(0.5, dw, dpsi, dom.thetan)
+        O.adv_ref_state(g, 0.5, w, psi, dom.thetan)
+        np.testing.assert_array_equal(dpsi.get(), psi)
+
+
+# ---------------------------------------------------------------- a6 means
+def test_hmeans():
+    dom = small_domain(12, 10, 8)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 91)
+    th = field(dom, 94)
+    q = np.stack([field(dom, 95 + n) for n in range(2)])
+    with _ctx(dom) as c:
+        got = c.hmeans(c.array(u), c.array(v), c.array(w), c.array(th), c.array(q), 2)
+        ref = O.hmeans(g, u, v, w, th, q)
+        for a, b in zip(got, ref):
+            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-15)
+
+
+# ---------------------------------------------------------------- a7 sources
+@pytest.mark.parametrize("shape", SHAPES)
+def test_sources_bitwise(shape):
+    dom = small_domain(*shape, stretched=True, varying_theta=True)
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, 101, courant=0.5)
+    th = field(dom, 104, -1.0, 2.0)
+    O.fill_full(g, th, 0.0, 0.0)
+    thbar = O.hmeans(g, u, v, w, th)[3]
+    sh = dom.shape
+    su, sv, sw, sth, rhs = (np.zeros(sh) for _ in range(5))
+    with _ctx(dom) as c:
+        d = [c.zeros() for _ in range(5)]
+        c.sources(c.array(u), c.array(v), c.array(w), c.array(th), dom.theta, thbar, *d)
+        O.sources(g, u, v, w, th, dom.theta, thbar, su, sv, sw, sth, rhs)
+        for a, b in zip(d, (su, sv, sw, sth, rhs)):
+            np.testing.assert_array_equal(a.get(), b)
+
+
+# ---------------------------------------------------------------- a9-a11
+@pytest.mark.parametrize("shape", SHAPES)
+def test_matvec_precondition_bitwise(shape):
+    dom = small_domain(*shape, stretched=True)
+    g = O.Grid(dom)
+    x = field(dom, 111)
+    ax = np.zeros(dom.shape)
+    r = field(dom, 114)
+    s = np.zeros(dom.shape)
+    with _ctx(dom) as c:
+        dx_, dax = c.array(x), c.zeros()
+        c.matvec(dx_, dax)
+        O.matvec(g, x, ax)
+        np.testing.assert_array_equal(dax.get(), ax)
+        np.testing.assert_array_equal(dx_.get(), x)  # mat_ax fills x's halo
+        dr, ds = c.array(r), c.zeros()
+        c.precondition(ds, dr, 3)
+        O.precondition(g, s, r, 3)
+        np.testing.assert_array_equal(ds.get(), s)
+        ip = c.inn_prod(dx_, dr)
+        assert abs(ip - O.inn_prod(g, x, r)) <= 1e-13 * abs(O.inn_prod(g, abs(x), abs(r)))
+
+
+# ---------------------------------------------------------------- a12 BiCGSTAB
+def _poisson_problem(dom, seed):
+    """A compatible right-hand side from a random source field (sum ~ 0)."""
+    g = O.Grid(dom)
+    u, v, w = _vel(dom, seed, courant=0.5)
+    th = field(dom, seed + 5, -1.0, 2.0)
+    O.fill_full(g, th, 0.0, 0.0)
+    thbar = O.hmeans(g, u, v, w, th)[3]
+    sh = dom.shape
+    su, sv, sw, sth, rhs = (np.zeros(sh) for _ in range(5))
+    O.sources(g, u, v, w, th, dom.theta, thbar, su, sv, sw, sth, rhs)
+    O.fill_along_dim_wo(g, rhs)
+    return sw, rhs
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_bicgstab_parity(shape):
+    dom = small_domain(*shape, varying_theta=True)
+    g = O.Grid(dom)
+    sw, b = _poisson_problem(dom, 121)
+    x0 = 0.01 * field(dom, 125)
+    x = x0.copy()
+    st, it, err = O.bicgstab(g, sw, x, b.copy(), 1e-10)
+    assert st == 0
+    with _ctx(dom) as c:
+        dx_ = c.array(x0)
+        gst, git, gerr = c.bicgstab(c.array(sw), dx_, c.array(b), 1e-10)
+        assert gst == 0
+        assert git == it
+        got = dx_.get()
+        mask = ~lateral_edge_mask(dom)
+        assert rel_l2(x[mask], got[mask]) < 1e-12
+        assert c.true_residual(dx_, c.array(b)) < 1e-9
+
+
+def test_bicgstab_early_return_and_not_converged():
+    dom = small_domain(8, 6, 5)
+    g = O.Grid(dom)
+    sw, b = _poisson_problem(dom, 131)
+    with _ctx(dom) as c:
+        # converged start: x solves the problem already -> zero iterations
+        x = np.zeros(dom.shape)
+        O.bicgstab(g, sw, x, b.copy(), 1e-10)
+        dx_ = c.array(x)
+        st, it, err = c.bicgstab(c.array(sw), dx_, c.array(b), 1e-6)
+        assert st == 0 and it == 0
+        # an unreachable tolerance with a small itmax reports not converged
+        dx2 = c.zeros()
+        st, it, err = c.bicgstab(c.array(sw), dx2, c.array(b), 1e-30, itmax=3, check=False)
+        assert st == 1 and it == 3
+
+
+def test_bicgstab_mixed_residual():
+    dom = small_domain(16, 16, 16)
+    sw, b = _poisson_problem(dom, 141)
+    g = O.Grid(dom)
+    xr = np.zeros(dom.shape)
+    O.bicgstab(g, sw, xr, b.copy(), 1e-10)
+    with _ctx(dom) as c:
+        dx_ = c.zeros()
+        it, outer, err = c.bicgstab_mixed(c.array(sw), dx_, c.array(b), 1e-10)
+        assert err < 1e-10
+        assert c.true_residual(dx_, c.array(b)) < 1e-10
+        got = interior(dx_.get())
+        ref = interior(xr)
+        assert rel_l2(ref - ref.mean(), got - got.mean()) < 1e-6
+
+
+# ---------------------------------------------------------------- a13/a14
+def test_advance_momentum_bitwise():
+    dom = small_domain(12, 10, 8, stretched=True)
+    g = O.Grid(dom)
+    arrs = [field(dom, 150 + n) for n in range(10)]
+    u, v, w, zu, zv, zw, su, sv, sw, p = arrs
+    with _ctx(dom) as c:
+        d = [c.array(a) for a in arrs]
+        c.advance_momentum(2.0, *d)
+        O.advance_momentum(g, 2.0, u, v, w, zu, zv, zw, su, sv, sw, p)
+        for k in range(3):
+            np.testing.assert_array_equal(d[k].get(), arrs[k])
+
+
+def test_advance_scalars_bitwise():
+    dom = small_domain(8, 6, 5)
+    g = O.Grid(dom)
+    th, sth = field(dom, 161), field(dom, 162)
+    q, sq = np.stack([field(dom, 163), field(dom, 164)]), np.stack([field(dom, 165), field(dom, 166)])
+    with _ctx(dom) as c:
+        dth, dsth, dq, dsq = c.array(th), c.array(sth), c.array(q), c.array(sq)
+        c.advance_scalars(0.5, dq, dsq, dth, dsth, 2, True)
+        O.advance_scalars(g, 0.5, q, sq, th, sth, True)
+        np.testing.assert_array_equal(dth.get(), th)
+        np.testing.assert_array_equal(dq.get(), q)
+
+
+# ---------------------------------------------------------------- a15 model step
+def _compare_models(case, steps, tol=1e-12):
+    om = O.Model(case)
+    with H.Context(case.dom) as c:
+        gm = H.Model(c, case)
+        try:
+            for s in range(steps):
+                st = gm.step(1)
+                om.step(1)
+                assert st.iters_total == om.iters[-1], (s, st.iters_total, om.iters[-1])
+            (u, v, w), (zu, zv, zw) = om.levels()
+            pairs = {"u": (H.F_U, u), "v": (H.F_V, v), "w": (H.F_W, w), "zu": (H.F_ZU, zu),
+                     "zv": (H.F_ZV, zv), "zw": (H.F_ZW, zw), "th": (H.F_TH, om.th)}
+            for name, (fid, ref) in pairs.items():
+                got = gm.get(fid)
+                assert rel_l2(ref, got) < tol, (name, rel_l2(ref, got))
+            mask = ~lateral_edge_mask(case.dom)
+            got = gm.get(H.F_P)
+            assert rel_l2(om.p[mask], got[mask]) < tol, ("p", rel_l2(om.p[mask], got[mask]))
+            for n in range(case.nq):
+                got = gm.get(H.F_Q, n)
+                assert rel_l2(om.q[n], got) < tol, ("q", n, rel_l2(om.q[n], got))
+        finally:
+            gm.close()
+
+
+def test_model_warm_bubble_parity():
+    case = W.warm_bubble(16, 16, 16, steps=6)
+    _compare_models(case, 6)
+
+
+def test_model_moist_bubble_parity():
+    case = W.moist_bubble(16, 12, 10, delta=50.0, dt=0.5, steps=4, ntracer=2)
+    _compare_models(case, 4)
+
+
+def test_model_kelvin_helmholtz_parity():
+    case = W.kelvin_helmholtz(24, 4, 16, steps=5)
+    _compare_models(case, 5)

@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the dynamical-core time step on one MI355X.
+
+Workload: the moist bubble with tracers, 256x256x128 cells at 50 m, dt=0.5 s,
+theta + qv + 8 passive tracers (F=10 advected fields, each its own SFVT
+category), MPDATA kord=2 with the limiter, BiCGSTAB to a 1e-10 relative
+residual.  Inputs are the seeded synthetic fields of
+``dcm_amd.workloads.moist_bubble``; they are resident in HBM before the timed
+region starts.
+
+A step is one pass of the model_driver time loop (driver_code.f90:304-676):
+horizontal means, sources, pressure solve, reference-state advection, MPDATA
+of every field, momentum update, time filter and halo fills.  Both paths run
+W warm-up steps, then K timed steps from the same state:
+
+* FP64  -- the reference arithmetic (parity path);
+* mixed -- FP32 MPDATA intermediates and FP32 inner BiCGSTAB inside FP64
+  iterative refinement, every solve ending at FP64 true residual < 1e-10.
+
+``value`` is the mixed path's cell-steps/s (interior cells x timed steps /
+wall seconds).  The ``roofline`` object describes the dominant kernel of that
+path, timed with HIP events on the model's stream over the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dynamical-cloud-model_amd"))
+
+METRIC = "cell-steps/s mixed vs FP64 at 1e-10 pressure resid; % HBM roofline"
+HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
+DOMINANT = "jac"         # Jacobi sweep kernel k_jac<R,false>: s' = s + (t - A s)/a_p
+JAC_WORDS = 3            # reads s and t, writes s'
+
+
+def canonical_bytes_fp64(F, iters):
+    """Canonical stage model, FP64 words per cell-step: 39 + 30F + 37*it."""
+    return 8.0 * (39 + 30 * F + 37 * iters)
+
+
+def canonical_bytes_mixed(F, inner_iters, outer):
+    """Same stages with each word at its storage precision (bytes/cell-step).
+
+    FP64: state (u,v,w,theta,q,p,b,x), sources, RHS, the true residual and
+    the correction of each refinement sweep.  FP32: MPDATA intermediates, the
+    velocity shadows and the inner solver vectors.
+    """
+    b = 8 + 56 + 32                      # thbar, sources, Poisson RHS
+    b += outer * (20 + 24 + 20)          # true residual, inner start, x += d
+    b += 148 * inner_iters               # 37 FP32 words per inner iteration
+    b += 24 + 36                         # adv_ref_state, FP32 velocity shadows
+    b += F * (8 + 24 + 28 + 32 + 32)     # MPDATA per field (min, 4 passes)
+    b += F * 24                          # scalar update (canonical; an exact no-op)
+    b += 80 + 72                         # advance_momentum, time filter
+    return float(b)
+
+
+def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rtol=None):
+    ctx = H.Context(case.dom)
+    model = H.Model(ctx, case, precision)
+    model.set_check_residual(True)
+    if inner_rtol:
+        model.set_inner_rtol(inner_rtol)
+    if warmup:
+        model.step(warmup)
+    model.profile(True, only=profile_only)
+    t0 = time.perf_counter()
+    st = model.step(steps)
+    t1 = time.perf_counter()
+    avg_ms, launches = model.kernel_time(profile_only)
+    model.profile(False)
+    wall = t1 - t0
+    cells = case.dom.cells
+    snap = {"th": model.get(H.F_TH), "u": model.get(H.F_U), "w": model.get(H.F_W),
+            "p": model.get(H.F_P), "q0": model.get(H.F_Q, 0), "q1": model.get(H.F_Q, 1)}
+    res = {
+        "value": cells * steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "iters_per_step": st.iters_total / steps,
+        "iters_max": st.iters_max,
+        "outer_per_step": st.outer_total / steps,
+        "max_true_resid": st.max_true_resid,
+        "kernel": {"name": profile_only, "avg_ms": avg_ms, "launches": launches},
+    }
+    return res, snap, (ctx, model)
+
+
+def rel_l2(a, b, center=False):
+    a = a[1:-1, 1:-1, 1:-1]
+    b = b[1:-1, 1:-1, 1:-1]
+    if center:
+        a = a - a.mean()
+        b = b - b.mean()
+    n = np.linalg.norm(a)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def cpu_baseline(W, ip, jp, steps):
+    """The CPU oracle (a single-threaded C restatement of the reference) on a
+    bounded sample: the same workload on an ip x jp x 128 sub-domain."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    case = W.moist_bubble(ip, jp, 128, steps=steps)
+    m = O.Model(case)
+    t0 = time.perf_counter()
+    m.step(steps)
+    wall = time.perf_counter() - t0
+    return {"value": case.dom.cells * steps / wall, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle model step, moist bubble {ip}x{jp}x128 (same dx, dt, fields), "
+                      f"{steps} step(s), {wall:.1f} s; BiCGSTAB iterations {m.iters}"}
+
+
+def stream_copy_gbps(H, dom):
+    n = 1 << 27  # 1 GiB per array
+    ctx = H.Context(dom)
+    a = H.DeviceArray(ctx, (n,))
+    b = H.DeviceArray(ctx, (n,))
+    ctx.stream_copy(b, a)
+    best = min(ctx.stream_copy(b, a) for _ in range(5))
+    a.free()
+    b.free()
+    ctx.close()
+    return 2.0 * n * 8 / (best * 1e-3) / 1e9
+
+
+def load_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{DOMINANT}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", default="256x256x128")
+    ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--inner-rtol", type=float, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", default="64x64:1")
+    ap.add_argument("--no-stream", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
+              "whole domain lives on one MI355X.", file=sys.stderr)
+        return 2
+
+    from dcm_amd import hip as H
+    from dcm_amd import workloads as W
+
+    ip, jp, kp = (int(x) for x in args.grid.split("x"))
+    case = W.moist_bubble(ip, jp, kp)
+    workload = f"moist_bubble_{ip}x{jp}x{kp}"
+    F = 1 + case.nq
+    paths = args.paths.split(",")
+    out = {}
+    snaps = {}
+    keep = []
+    for pth in paths:
+        prec = H.MIXED if pth == "mixed" else H.FP64
+        print(f"[bench] {pth}: warmup {args.warmup}, timed {args.steps} steps", file=sys.stderr,
+              flush=True)
+        res, snap, handles = run_path(H, case, prec, args.warmup, args.steps,
+                                      inner_rtol=args.inner_rtol if pth == "mixed" else None)
+        out[pth] = res
+        snaps[pth] = snap
+        handles[1].close()
+        handles[0].close()
+        print(f"[bench] {pth}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} "
+              f"ms/step, {res['iters_per_step']:.1f} it/step", file=sys.stderr, flush=True)
+
+    main_path = "mixed" if "mixed" in out else paths[0]
+    r = out[main_path]
+    cells = case.dom.cells
+    word = 4 if main_path == "mixed" else 8
+    k = r["kernel"]
+    achieved = None
+    if k["avg_ms"]:
+        achieved = JAC_WORDS * word * cells / (k["avg_ms"] * 1e-3) / 1e9
+    traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"))
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
+                "traffic": traffic,
+                "kernel": f"k_jac<{'float' if word == 4 else 'double'},false>",
+                "algorithmic_bytes_per_launch": JAC_WORDS * word * cells,
+                "avg_launch_ms": k["avg_ms"], "launches": k["launches"]}
+    step_roof = {}
+    for pth, res in out.items():
+        if pth == "mixed":
+            bpc = canonical_bytes_mixed(F, res["iters_per_step"], res["outer_per_step"])
+        else:
+            bpc = canonical_bytes_fp64(F, res["iters_per_step"])
+        step_roof[pth] = {"bytes_per_cell_step": bpc,
+                          "GBps": bpc * res["value"] / 1e9,
+                          "frac_of_8TBps": bpc * res["value"] / 8e12}
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_path == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "grid": args.grid, "delta_m": 50.0, "dt_s": 0.5,
+                   "fields": F, "kord": 2, "monotone": True, "ptol": 1e-10,
+                   "parallelism": "single GPU"},
+        "paths": out,
+        "roofline": roofline,
+        "step_roofline": step_roof,
+    }
+    if "fp64" in out and "mixed" in out:
+        line["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+        a, b = snaps["fp64"], snaps["mixed"]
+        line["mixed_error_vs_fp64"] = {
+            name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
+    if not args.no_stream:
+        line["stream_copy_GBps"] = stream_copy_gbps(H, case.dom)
+    if not args.no_cpu:
+        cip, steps = args.cpu_sample.split(":")
+        cx, cy = (int(x) for x in cip.split("x"))
+        line["cpu_baseline"] = cpu_baseline(W, cx, cy, int(steps))
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -143,14 +143,14 @@ int prof_begin(dcm_ctx *c, const char *name) {
     } else {
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
     }
-    hipEventRecord(a, c->stream);
+    (void)hipEventRecord(a, c->stream);
     c->pending.push_back({name, {a, b}});
     return (int)c->pending.size() - 1;
 }
 
 void prof_end(dcm_ctx *c, int token) {
     if (token < 0) return;
-    hipEventRecord(c->pending[token].second.second, c->stream);
+    (void)hipEventRecord(c->pending[token].second.second, c->stream);
 }
 
 int prof_collect(dcm_ctx *c) {
@@ -158,7 +158,7 @@ int prof_collect(dcm_ctx *c) {
     DCM_HIP(hipStreamSynchronize(c->stream));
     for (auto &p : c->pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
         auto &t = c->timers[p.first];
         t.total_ms += ms;
         t.launches += 1;
@@ -200,7 +200,7 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
     g.sj = g.nk;
     g.si = (long long)g.nk * g.nj;
     g.ntot = (long long)g.nk * g.nj * g.ni;
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         set_error("hipStreamCreate failed");
@@ -252,8 +252,8 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
     }
     c->dmet = (double *)p;
     c->dmet32 = (float *)p32;
-    hipMemcpy(c->dmet, h.data(), total * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(c->dmet32, h32.data(), total * sizeof(float), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->dmet, h.data(), total * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->dmet32, h32.data(), total * sizeof(float), hipMemcpyHostToDevice);
     const double **m64[14] = {&c->met.dx, &c->met.dy, &c->met.dz, &c->met.dxn, &c->met.dyn,
                               &c->met.dzn, &c->met.rhoa, &c->met.rhoan, &c->met.ae,
                               &c->met.aw, &c->met.an, &c->met.as, &c->met.au, &c->met.ad};
@@ -270,7 +270,7 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
         return DCM_OUT_OF_MEMORY;
     }
     c->dscal = (double *)p;
-    hipMemset(c->dscal, 0, 64 * sizeof(double));
+    (void)hipMemset(c->dscal, 0, 64 * sizeof(double));
     if (hipHostMalloc(&p, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         set_error("hipHostMalloc failed");
         return DCM_OUT_OF_MEMORY;
@@ -281,7 +281,7 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
         return DCM_OUT_OF_MEMORY;
     }
     c->solver = (SolverScalars *)p;
-    hipMemset(c->solver, 0, 2 * sizeof(SolverScalars));
+    (void)hipMemset(c->solver, 0, 2 * sizeof(SolverScalars));
     if (hipHostMalloc(&p, 2 * sizeof(SolverScalars), hipHostMallocDefault) != hipSuccess) {
         set_error("hipHostMalloc failed");
         return DCM_OUT_OF_MEMORY;
@@ -297,27 +297,27 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
 
 int dcm_context_destroy(dcm_ctx *c) {
     if (!c) return DCM_BAD_ARG;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (auto &m : c->mirrors) hipFree(m.second.dev);
     for (double *p : c->work64) hipFree(p);
     for (float *p : c->work32) hipFree(p);
-    hipFree(c->sv64);
-    hipFree(c->sv32);
-    hipFree(c->partials);
-    hipFree(c->hm_partials);
-    hipFree(c->d1d);
-    hipFree(c->dmet);
-    hipFree(c->dmet32);
-    hipFree(c->dscal);
-    hipHostFree(c->hscal);
-    hipFree(c->solver);
-    hipHostFree(c->h_solver);
+    (void)hipFree(c->sv64);
+    (void)hipFree(c->sv32);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->hm_partials);
+    (void)hipFree(c->d1d);
+    (void)hipFree(c->dmet);
+    (void)hipFree(c->dmet32);
+    (void)hipFree(c->dscal);
+    (void)hipHostFree(c->hscal);
+    (void)hipFree(c->solver);
+    (void)hipHostFree(c->h_solver);
     for (auto &p : c->pending) {
-        hipEventDestroy(p.second.first);
-        hipEventDestroy(p.second.second);
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
     }
     for (auto e : c->event_pool) hipEventDestroy(e);
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return DCM_OK;
 }

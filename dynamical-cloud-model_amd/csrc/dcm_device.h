@@ -63,6 +63,35 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NS], double *partial
     }
 }
 
+// As block_sum_store, storing at partials[s*stride + idx] (idx: the block's
+// tile number when blocks are renumbered).
+template <int NS>
+__device__ __forceinline__ void block_sum_store_at(double (&v)[NS], double *partials,
+                                                   long long stride, int idx) {
+    __shared__ double red[NS][kBlock / 64];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[s] += __shfl_down(v[s], off, 64);
+    }
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) red[s][wave] = v[s];
+    }
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double acc = red[s][0];
+#pragma unroll
+            for (int q = 1; q < kBlock / 64; ++q) acc += red[s][q];
+            partials[s * stride + idx] = acc;
+        }
+    }
+}
+
 __device__ __forceinline__ double block_min_store(double v, double *partials) {
     __shared__ double red[kBlock / 64];
 #pragma unroll
@@ -89,8 +118,17 @@ __device__ __forceinline__ double block_min_store(double v, double *partials) {
 __device__ __forceinline__ double reduce_slot_1024(const double *p, int n) {
     __shared__ double red[1024];
     const int t = threadIdx.x;
-    double acc = 0.0;
-    for (int b = t; b < n; b += 1024) acc += p[b];
+    // four independent chains keep several loads in flight
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = t;
+    for (; b + 3 * 1024 < n; b += 4 * 1024) {
+        a0 += p[b];
+        a1 += p[b + 1024];
+        a2 += p[b + 2048];
+        a3 += p[b + 3072];
+    }
+    for (; b < n; b += 1024) a0 += p[b];
+    double acc = (a0 + a1) + (a2 + a3);
     red[t] = acc;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {

@@ -79,7 +79,7 @@ struct SolverScalars {
     int breakdown;     // |omega| < tiny
     int done;          // converged (device-side early exit for chunks)
     int nonfinite;
-    int pad;
+    int iters;         // iterations performed (counted on the device)
 };
 
 // Per-kernel timing (profiling mode): accumulated over launches.

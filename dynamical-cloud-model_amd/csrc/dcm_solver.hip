@@ -8,7 +8,7 @@
 // fills the halo of its input: periodic x,y faces, then Neumann copies
 // x(0)=x(1), x(kp+1)=x(kp).  Those fills only ever feed the stencil, so here
 // the stencil reads the interior through wrapped / clamped indices and the
-// solver's work vectors are compact (kp*jp*ip, no halo).
+// solver's work vectors, the iterate included, are compact (kp*jp*ip).
 //
 // Iteration (dynamics.f90:390-430), with every whole-array statement fused
 // into the stencil pass that produces or consumes it:
@@ -23,7 +23,16 @@
 // Each value is computed by the reference's expression in its operand order,
 // so only the dot products' summation order differs from the reference (a
 // fixed two-level tree here, deterministic run to run).  The recurrence
-// scalars live on the device; the host reads them once per iteration.
+// scalars live on the device and every kernel tests the device-side
+// convergence flag, so the host only polls every few iterations.
+//
+// Stencil kernels march: a 256-thread block owns a kt x jt tile of (k,j)
+// columns (kt = 64 lanes along the contiguous k, or 32 for shallow grids) and
+// walks `ic` consecutive i planes, keeping the i-1, i, i+1 values of its own
+// cells in registers; j and k neighbours come through L1/L2.  Offsets are
+// 32-bit and blocks are renumbered so that each XCD (block id mod 8) walks a
+// contiguous range of tiles, which keeps the j-neighbour columns in its own
+// L2.
 //
 // Mixed precision (dcm_bicgstab_mixed): the same kernels with FP32 vectors
 // and FP64 dot products solve A d = r for the FP64 residual r = b - A x; the
@@ -49,12 +58,14 @@ __device__ __forceinline__ long long cidx(const CGeo &g, int k, int j, int i) {
 
 template <typename R> struct Coef;
 template <> struct Coef<double> {
-    const double *ae, *aw, *an, *as, *au, *ad;
+    const double *__restrict__ ae, *__restrict__ aw, *__restrict__ an, *__restrict__ as,
+        *__restrict__ au, *__restrict__ ad;
     __host__ __device__ Coef(const Metrics &m, const Metrics32 &)
         : ae(m.ae), aw(m.aw), an(m.an), as(m.as), au(m.au), ad(m.ad) {}
 };
 template <> struct Coef<float> {
-    const float *ae, *aw, *an, *as, *au, *ad;
+    const float *__restrict__ ae, *__restrict__ aw, *__restrict__ an, *__restrict__ as,
+        *__restrict__ au, *__restrict__ ad;
     __host__ __device__ Coef(const Metrics &, const Metrics32 &m)
         : ae(m.ae), aw(m.aw), an(m.an), as(m.as), au(m.au), ad(m.ad) {}
 };
@@ -68,6 +79,11 @@ struct CoefPack {
 template <typename R>
 __device__ __forceinline__ R coef_ap(const Coef<R> &C, int k, int j, int i) {
     return (((((-C.ae[i]) - C.aw[i]) - C.an[j]) - C.as[j]) - C.au[k]) - C.ad[k];
+}
+// the same sum with its first two terms (the i part) precomputed
+template <typename R>
+__device__ __forceinline__ R coef_ap_i(const Coef<R> &C, R api, int k, int j) {
+    return (((api - C.an[j]) - C.as[j]) - C.au[k]) - C.ad[k];
 }
 
 // neighbour indices of (k,j,i): periodic in i,j; Neumann copy in k
@@ -85,40 +101,109 @@ __device__ __forceinline__ Nbr nbrs(int k, int j, int i, int kp, int jp, int ip)
     return n;
 }
 
-// the stencil sum of mat_ax (dynamics.f90:216-224), coefficient at (k,j,i)
+// the stencil sum of mat_ax (dynamics.f90:216-224) with the coefficients of
+// the centre cell
+template <typename R>
+__device__ __forceinline__ R stencil(R aw, R ae, R as, R an, R ad, R au, R ap, R xw, R xe, R xs,
+                                     R xn, R xd, R xu, R xc) {
+    return ((((((aw * xw + ae * xe) + as * xs) + an * xn) + ad * xd) + au * xu) + ap * xc) + R(0);
+}
 template <typename R>
 __device__ __forceinline__ R stencil(const Coef<R> &C, int k, int j, int i, R ap, R xw, R xe,
                                      R xs, R xn, R xd, R xu, R xc) {
-    return ((((((C.aw[i] * xw + C.ae[i] * xe) + C.as[j] * xs) + C.an[j] * xn) + C.ad[k] * xd) +
-             C.au[k] * xu) +
-            ap * xc) +
-           R(0);
+    return stencil<R>(C.aw[i], C.ae[i], C.as[j], C.an[j], C.ad[k], C.au[k], ap, xw, xe, xs, xn,
+                      xd, xu, xc);
 }
 
-#define SOLVER_PROLOGUE                                              \
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;         \
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;         \
-    const int i = blockIdx.z + 1;                                    \
-    const bool act = k <= g.kp && j <= g.jp;
+// ---------------------------------------------------------------------------
+// marching geometry
+// ---------------------------------------------------------------------------
+struct MG {
+    int ip, jp, kp;
+    int sj, si;            // strides of j and i in a compact vector
+    int kt, jt;            // tile lanes along k and rows along j (kt*jt = 256)
+    int nkb, njb, nic, ic; // tiles along k, j; i chunks and their length
+    int nblocks;
+};
+
+static MG march_geo(const Geo &g, int ic) {
+    MG m;
+    m.ip = g.ip;
+    m.jp = g.jp;
+    m.kp = g.kp;
+    m.sj = g.kp;
+    m.si = g.kp * g.jp;
+    m.kt = g.kp > 32 ? 64 : 32;
+    m.jt = kBlock / m.kt;
+    m.nkb = (g.kp + m.kt - 1) / m.kt;
+    m.njb = (g.jp + m.jt - 1) / m.jt;
+    m.ic = ic;
+    m.nic = (g.ip + ic - 1) / ic;
+    m.nblocks = m.nkb * m.njb * m.nic;
+    return m;
+}
+static unsigned march_grid(const MG &m) { return (unsigned)(((m.nblocks + 7) / 8) * 8); }
+
+// Tile of this block (blocks renumbered so that XCD x = id mod 8 owns the
+// contiguous tile range [x*per, (x+1)*per)).  False for the padding blocks.
+struct Tile {
+    int k, j, i0, i1, tile;
+    bool act;
+};
+__device__ __forceinline__ bool march_tile(const MG &m, Tile &T) {
+    const int b = blockIdx.x;
+    const int per = (m.nblocks + 7) >> 3;
+    T.tile = (b & 7) * per + (b >> 3);
+    if (T.tile >= m.nblocks) return false;
+    const int kb = T.tile % m.nkb;
+    const int r = T.tile / m.nkb;
+    const int jb = r % m.njb;
+    const int icb = r / m.njb;
+    T.k = kb * m.kt + (int)(threadIdx.x % m.kt) + 1;
+    T.j = jb * m.jt + (int)(threadIdx.x / m.kt) + 1;
+    T.i0 = icb * m.ic + 1;
+    T.i1 = T.i0 + m.ic - 1;
+    if (T.i1 > m.ip) T.i1 = m.ip;
+    T.act = T.k <= m.kp && T.j <= m.jp;
+    return true;
+}
+
+// column offsets of a tile cell: base (i = 1) and the j/k neighbour deltas
+struct Cols {
+    int base, ds, dn, dd, du;
+};
+__device__ __forceinline__ Cols cols(const MG &m, int k, int j) {
+    Cols c;
+    c.base = (j - 1) * m.sj + (k - 1);
+    c.ds = (j == 1 ? m.jp - 1 : -1) * m.sj;
+    c.dn = (j == m.jp ? -(m.jp - 1) : 1) * m.sj;
+    c.dd = k == 1 ? 0 : -1;
+    c.du = k == m.kp ? 0 : 1;
+    return c;
+}
 
 // ---------------------------------------------------------------------------
-// initial residual (dynamics.f90:370-381): r = b - A x, cr = r, p = v1 = 0;
-// partial sums of <b,b> and <r,r>.  x and b are padded FP64 arrays.
+// initial residual (dynamics.f90:370-381): r = b - A x, cr = r, p = v1 = 0,
+// x gathered into the compact iterate xc; partial sums of <b,b> and <r,r>.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_resid0(const double *x, const double *b, double *r,
-                                                double *cr, double *p, double *v1, Geo pg,
-                                                CGeo g, CoefPack cp, double *partials,
+__global__ __launch_bounds__(256) void k_resid0(const double *__restrict__ x,
+                                                const double *__restrict__ b, double *r,
+                                                double *cr, double *p, double *v1, double *xc,
+                                                Geo pg, CGeo g, CoefPack cp, double *partials,
                                                 long long stride) {
-    SOLVER_PROLOGUE
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
     const Coef<double> C(cp.m64, cp.m32);
     double s[2] = {0.0, 0.0};
-    if (act) {
+    if (k <= g.kp && j <= g.jp) {
         const Nbr n = nbrs(k, j, i, g.kp, g.jp, g.ip);
         const double ap = coef_ap(C, k, j, i);
+        const double xcv = x[pidx(pg, k, j, i)];
         const double ax =
             stencil<double>(C, k, j, i, ap, x[pidx(pg, k, j, n.iw)], x[pidx(pg, k, j, n.ie)],
                             x[pidx(pg, k, n.js, i)], x[pidx(pg, k, n.jn, i)],
-                            x[pidx(pg, n.kd, j, i)], x[pidx(pg, n.ku, j, i)], x[pidx(pg, k, j, i)]);
+                            x[pidx(pg, n.kd, j, i)], x[pidx(pg, n.ku, j, i)], xcv);
         const double bb = b[pidx(pg, k, j, i)];
         const double rr = bb - ax;
         const long long c = cidx(g, k, j, i);
@@ -126,6 +211,7 @@ __global__ __launch_bounds__(256) void k_resid0(const double *x, const double *b
         cr[c] = rr;
         p[c] = 0.0;
         v1[c] = 0.0;
+        xc[c] = xcv;
         s[0] = bb * bb;
         s[1] = rr * rr;
     }
@@ -133,14 +219,17 @@ __global__ __launch_bounds__(256) void k_resid0(const double *x, const double *b
 }
 
 // FP64 true residual with an FP32 copy for the inner solve:
-// r32 = float(b - A x); partial sums of <r,r>, sum(r) (and <b,b> if asked)
-__global__ __launch_bounds__(256) void k_resid_true(const double *x, const double *b,
-                                                    float *r32, Geo pg, CGeo g, CoefPack cp,
+// r32 = float(b - A x); partial sums of <r,r>, sum(r), <b,b>
+__global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x,
+                                                    const double *__restrict__ b, float *r32,
+                                                    Geo pg, CGeo g, CoefPack cp,
                                                     double *partials, long long stride) {
-    SOLVER_PROLOGUE
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
     const Coef<double> C(cp.m64, cp.m32);
     double s[3] = {0.0, 0.0, 0.0};
-    if (act) {
+    if (k <= g.kp && j <= g.jp) {
         const Nbr n = nbrs(k, j, i, g.kp, g.jp, g.ip);
         const double ap = coef_ap(C, k, j, i);
         const double ax =
@@ -158,125 +247,162 @@ __global__ __launch_bounds__(256) void k_resid_true(const double *x, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// preconditioner sweeps (dynamics.f90:241-270)
+// preconditioner sweeps (dynamics.f90:241-270), marching kernels
 // ---------------------------------------------------------------------------
 // MODE 0: t = r - bet*(omg*v1)     (dynamics.f90:394)
 // MODE 1: t = r - alf*v1           (dynamics.f90:405, the "s" vector)
 template <typename R, int MODE>
-__device__ __forceinline__ R tval(const R *r, const R *v1, long long q, R a, R b) {
+__device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__ v1, int q, R a,
+                                  R b) {
     if (MODE == 0) return r[q] - a * (b * v1[q]);
     return r[q] - a * v1[q];
 }
 
+// sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p, s0 = t/a_p evaluated at
+// the centre and the six neighbours (dynamics.f90:260-266)
 template <typename R, int MODE>
-__global__ __launch_bounds__(256) void k_jac_first(const R *r, const R *v1, R *t_out, R *s_out,
-                                                   CGeo g, CoefPack cp,
-                                                   const SolverScalars *sc) {
+__global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
+                                                   const R *__restrict__ v1,
+                                                   R *__restrict__ t_out, R *__restrict__ s_out,
+                                                   MG m, CoefPack cp,
+                                                   const SolverScalars *__restrict__ sc) {
     if (sc->done || sc->breakdown) return;
-    SOLVER_PROLOGUE
-    if (!act) return;
+    Tile T;
+    if (!march_tile(m, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
-    const Nbr n = nbrs(k, j, i, g.kp, g.jp, g.ip);
-    const long long c = cidx(g, k, j, i);
-    // s0 = t/a_p at the centre and the six (wrapped/clamped) neighbours
-    auto s0 = [&](int kk, int jj, int ii) -> R {
-        const long long q = cidx(g, kk, jj, ii);
-        return tval<R, MODE>(r, v1, q, a, bo) / coef_ap(C, kk, jj, ii);
-    };
-    const R tc = tval<R, MODE>(r, v1, c, a, bo);
-    const R ap = coef_ap(C, k, j, i);
-    const R sc0 = tc / ap;
-    const R As = stencil<R>(C, k, j, i, ap, s0(k, j, n.iw), s0(k, j, n.ie), s0(k, n.js, i),
-                            s0(k, n.jn, i), s0(n.kd, j, i), s0(n.ku, j, i), sc0);
-    t_out[c] = tc;
-    s_out[c] = sc0 + (tc - As) / ap;
+    const int k = T.k, j = T.j;
+    const Cols q = cols(m, k, j);
+    const int js = j == 1 ? m.jp : j - 1, jn = j == m.jp ? 1 : j + 1;
+    const int kd = k == 1 ? 1 : k - 1, ku = k == m.kp ? m.kp : k + 1;
+    const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+    const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
+    int cw = (iw - 1) * m.si + q.base;
+    int cc = (T.i0 - 1) * m.si + q.base;
+    R s0w = tval<R, MODE>(r, v1, cw, a, bo) / coef_ap(C, k, j, iw);
+    R tc = tval<R, MODE>(r, v1, cc, a, bo);
+    R s0c = tc / coef_ap(C, k, j, T.i0);
+    for (int i = T.i0; i <= T.i1; ++i) {
+        const int ie = i == m.ip ? 1 : i + 1;
+        const int ce = (ie - 1) * m.si + q.base;
+        const R te = tval<R, MODE>(r, v1, ce, a, bo);
+        const R s0e = te / coef_ap(C, k, j, ie);
+        const R api = (-C.ae[i]) - C.aw[i];
+        const R ap = coef_ap_i(C, api, k, j);
+        const R s0s = tval<R, MODE>(r, v1, cc + q.ds, a, bo) / coef_ap_i(C, api, k, js);
+        const R s0n = tval<R, MODE>(r, v1, cc + q.dn, a, bo) / coef_ap_i(C, api, k, jn);
+        const R s0d = q.dd ? tval<R, MODE>(r, v1, cc - 1, a, bo) / coef_ap_i(C, api, kd, j) : s0c;
+        const R s0u = q.du ? tval<R, MODE>(r, v1, cc + 1, a, bo) / coef_ap_i(C, api, ku, j) : s0c;
+        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n, s0d,
+                                s0u, s0c);
+        t_out[cc] = tc;
+        s_out[cc] = s0c + (tc - As) / ap;
+        s0w = s0c;
+        s0c = s0e;
+        tc = te;
+        cc = ce;
+    }
 }
 
 // s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead
 template <typename R, bool P_UPDATE>
-__global__ __launch_bounds__(256) void k_jac(const R *s, const R *t, R *s_out, CGeo g,
-                                             CoefPack cp, const SolverScalars *sc) {
+__global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
+                                             R *__restrict__ s_out, MG m, CoefPack cp,
+                                             const SolverScalars *__restrict__ sc) {
     if (sc->done || sc->breakdown) return;
-    SOLVER_PROLOGUE
-    if (!act) return;
+    Tile T;
+    if (!march_tile(m, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
-    const Nbr n = nbrs(k, j, i, g.kp, g.jp, g.ip);
-    const long long c = cidx(g, k, j, i);
-    const R ap = coef_ap(C, k, j, i);
-    const R sc0 = s[c];
-    const R As = stencil<R>(C, k, j, i, ap, s[cidx(g, k, j, n.iw)], s[cidx(g, k, j, n.ie)],
-                            s[cidx(g, k, n.js, i)], s[cidx(g, k, n.jn, i)],
-                            s[cidx(g, n.kd, j, i)], s[cidx(g, n.ku, j, i)], sc0);
-    const R snew = sc0 + (t[c] - As) / ap;
-    if (P_UPDATE) {
-        const R bet = (R)sc->bet;
-        s_out[c] = snew + bet * s_out[c];  // p = s + bet*p (dynamics.f90:399)
-    } else {
-        s_out[c] = snew;
+    const int k = T.k, j = T.j;
+    const Cols q = cols(m, k, j);
+    const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+    const R bet = P_UPDATE ? (R)sc->bet : R(0);
+    const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
+    int cc = (T.i0 - 1) * m.si + q.base;
+    R xw = s[(iw - 1) * m.si + q.base];
+    R xc = s[cc];
+    for (int i = T.i0; i <= T.i1; ++i) {
+        const int ie = i == m.ip ? 1 : i + 1;
+        const int ce = (ie - 1) * m.si + q.base;
+        const R xe = s[ce];
+        const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
+        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, s[cc + q.ds],
+                                s[cc + q.dn], s[cc + q.dd], s[cc + q.du], xc);
+        const R snew = xc + (t[cc] - As) / ap;
+        if (P_UPDATE)
+            s_out[cc] = snew + bet * s_out[cc];  // p = s + bet*p (dynamics.f90:399)
+        else
+            s_out[cc] = snew;
+        xw = xc;
+        xc = xe;
+        cc = ce;
     }
 }
 
-// y = A x; partial sums of <a,y> (and <y,y> when TWO)
+// y = A x; partial sums of <a,y> (and <y,y> first when TWO)
 template <typename R, bool TWO>
-__global__ __launch_bounds__(256) void k_matvec_dot(const R *x, const R *a, R *y, CGeo g,
-                                                    CoefPack cp, const SolverScalars *sc,
-                                                    double *partials, long long stride) {
+__global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
+                                                    const R *__restrict__ a, R *__restrict__ y,
+                                                    MG m, CoefPack cp,
+                                                    const SolverScalars *__restrict__ sc,
+                                                    double *partials, int stride) {
     if (sc->done || sc->breakdown) return;
-    SOLVER_PROLOGUE
-    const Coef<R> C(cp.m64, cp.m32);
-    double s[2] = {0.0, 0.0};
-    if (act) {
-        const Nbr n = nbrs(k, j, i, g.kp, g.jp, g.ip);
-        const long long c = cidx(g, k, j, i);
-        const R ap = coef_ap(C, k, j, i);
-        const R yv = stencil<R>(C, k, j, i, ap, x[cidx(g, k, j, n.iw)], x[cidx(g, k, j, n.ie)],
-                                x[cidx(g, k, n.js, i)], x[cidx(g, k, n.jn, i)],
-                                x[cidx(g, n.kd, j, i)], x[cidx(g, n.ku, j, i)], x[c]);
-        y[c] = yv;
-        if (TWO) {
-            s[0] = (double)yv * (double)yv;   // tt = <t,t>
-            s[1] = (double)yv * (double)a[c]; // ts = <t,s>
-        } else {
-            s[0] = (double)a[c] * (double)yv; // nrm = <cr,v1>
+    Tile T;
+    if (!march_tile(m, T)) return;
+    double acc[2] = {0.0, 0.0};
+    if (T.act) {
+        const Coef<R> C(cp.m64, cp.m32);
+        const int k = T.k, j = T.j;
+        const Cols q = cols(m, k, j);
+        const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+        const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
+        int cc = (T.i0 - 1) * m.si + q.base;
+        R xw = x[(iw - 1) * m.si + q.base];
+        R xc = x[cc];
+        for (int i = T.i0; i <= T.i1; ++i) {
+            const int ie = i == m.ip ? 1 : i + 1;
+            const int ce = (ie - 1) * m.si + q.base;
+            const R xe = x[ce];
+            const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
+            const R yv = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, x[cc + q.ds],
+                                    x[cc + q.dn], x[cc + q.dd], x[cc + q.du], xc);
+            y[cc] = yv;
+            if (TWO) {
+                acc[0] += (double)yv * (double)yv;     // tt = <t,t>
+                acc[1] += (double)yv * (double)a[cc];  // ts = <t,s>
+            } else {
+                acc[0] += (double)a[cc] * (double)yv;  // nrm = <cr,v1>
+            }
+            xw = xc;
+            xc = xe;
+            cc = ce;
         }
     }
-    if (TWO)
-        block_sum_store<2>(s, partials, stride);
-    else {
-        double s1[1] = {s[0]};
-        block_sum_store<1>(s1, partials, stride);
-    }
+    block_sum_store_at<2>(acc, partials, stride, T.tile);
 }
 
-// x = x + alf*p + omg*cs (padded x, FP64); r = s - omg*t; <r,r>, <r,cr>
+// x = x + alf*p + omg*cs; r = s - omg*t; partial sums of <r,r>, <r,cr>
+// (dynamics.f90:419-423), all compact vectors, grid-stride
 template <typename R>
-__global__ __launch_bounds__(256) void k_update(void *xv, const R *p, const R *cs, const R *s,
-                                                const R *t, const R *cr, R *r, Geo pg, CGeo g,
-                                                const SolverScalars *sc, double *partials,
-                                                long long stride, int x_is_compact) {
+__global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__restrict__ p,
+                                                const R *__restrict__ cs,
+                                                const R *__restrict__ s,
+                                                const R *__restrict__ t,
+                                                const R *__restrict__ cr, R *__restrict__ r,
+                                                int n, const SolverScalars *__restrict__ sc,
+                                                double *partials, int stride) {
     if (sc->done || sc->breakdown) return;
-    SOLVER_PROLOGUE
+    const R alf = (R)sc->alf, omg = (R)sc->omg;
     double acc[2] = {0.0, 0.0};
-    if (act) {
-        const long long c = cidx(g, k, j, i);
-        const R alf = (R)sc->alf, omg = (R)sc->omg;
-        if (x_is_compact) {
-            // inner correction d, kept in the vector precision
-            R *xr = (R *)xv;
-            xr[c] = (xr[c] + alf * p[c]) + omg * cs[c];
-        } else {
-            double *x = (double *)xv;
-            const long long xi = pidx(pg, k, j, i);
-            x[xi] = (x[xi] + sc->alf * (double)p[c]) + sc->omg * (double)cs[c];
-        }
-        const R rv = s[c] - omg * t[c];
-        r[c] = rv;
-        acc[0] = (double)rv * (double)rv;
-        acc[1] = (double)rv * (double)cr[c];
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+        x[q] = (x[q] + alf * p[q]) + omg * cs[q];
+        const R rv = s[q] - omg * t[q];
+        r[q] = rv;
+        acc[0] += (double)rv * (double)rv;
+        acc[1] += (double)rv * (double)cr[q];
     }
-    block_sum_store<2>(acc, partials, stride);
+    block_sum_store_at<2>(acc, partials, stride, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,6 +428,7 @@ __global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, do
         sc->rho = rr;  // <r,cr> with cr = r (:391)
         sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
         sc->breakdown = 0;
+        sc->iters = 0;
         sc->nonfinite = !isfinite(sc->err);
         sc->done = (sc->err < tol) || sc->nonfinite;
     }
@@ -321,6 +448,7 @@ __global__ void k_fin_t(const double *partials, int nb, SolverScalars *sc) {
     const double tt = reduce_slot_1024(partials, nb);
     const double ts = reduce_slot_1024(partials + nb, nb);
     if (threadIdx.x == 0) {
+        sc->iters += 1;
         sc->tt = tt;
         sc->ts = ts;
         sc->omg = ts / tt;  // :412
@@ -343,6 +471,51 @@ __global__ void k_fin_r(const double *partials, int nb, SolverScalars *sc, doubl
     }
 }
 
+// inner solve start: residuals measured relative to ||r_inner,0||
+__global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *sc,
+                                 double rtol) {
+    const double rr = reduce_slot_1024(partials, nb);
+    if (threadIdx.x == 0) {
+        sc->sc_err = rr > 0.0 ? sqrt(rr) : 1.0;
+        sc->rr = rr;
+        sc->err = sqrt(rr) / sc->sc_err;
+        sc->alf = 1.0;
+        sc->omg = 1.0;
+        sc->nrm = 1.0;
+        sc->rho = rr;
+        sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
+        sc->breakdown = 0;
+        sc->iters = 0;
+        sc->nonfinite = !isfinite(rr);
+        sc->done = (rr == 0.0) || (sc->err < rtol) || sc->nonfinite;
+    }
+}
+
+// three generic sums into dst[0..2]
+__global__ void k_finalize_sums_3(const double *partials, int nb, double *dst) {
+    const double a = reduce_slot_1024(partials, nb);
+    const double b = reduce_slot_1024(partials + nb, nb);
+    const double d = reduce_slot_1024(partials + 2 * (long long)nb, nb);
+    if (threadIdx.x == 0) {
+        dst[0] = a;
+        dst[1] = b;
+        dst[2] = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary rows and copies
+// ---------------------------------------------------------------------------
+// compact iterate back into the interior of the padded x
+__global__ void k_scatter_x(const double *__restrict__ xc, double *__restrict__ x, Geo pg,
+                            CGeo g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    x[pidx(pg, k, j, i)] = xc[cidx(g, k, j, i)];
+}
+
 // final boundary rows of x (dynamics.f90:437-442), or the Neumann copies left
 // by mat_ax on the early-return path (:383); whole z planes
 __global__ void k_x_rows(double *x, const double *sw, Geo g, double dzn0, int final_rows) {
@@ -358,8 +531,39 @@ __global__ void k_x_rows(double *x, const double *sw, Geo g, double dzn0, int fi
     }
 }
 
+// d (compact FP32) added to the padded FP64 x: x = x + d
+__global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    x[pidx(pg, k, j, i)] += (double)d[cidx(g, k, j, i)];
+}
+
+// inner start: r = r32 - mean, cr = r, p = v1 = d = 0; <r,r>
+__global__ void k_inner_init(const float *r32, float *r, float *cr, float *p, float *v1,
+                             float *d, CGeo g, const double *mean_sum, double *partials,
+                             long long stride) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    double s[2] = {0.0, 0.0};
+    if (k <= g.kp && j <= g.jp) {
+        const long long c = cidx(g, k, j, i);
+        const float mean = (float)(*mean_sum / (double)g.n);
+        const float rv = r32[c] - mean;
+        r[c] = rv;
+        cr[c] = rv;
+        p[c] = 0.f;
+        v1[c] = 0.f;
+        d[c] = 0.f;
+        s[0] = (double)rv * (double)rv;
+    }
+    block_sum_store<2>(s, partials, stride);
+}
+
 // ---------------------------------------------------------------------------
-// whole-array helpers for the standalone precondition API
+// whole-array helpers for the standalone precondition / matvec API
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double ap_whole(const Geo &g, const Coef<double> &C, long long t) {
     const int k = (int)(t % g.nk);
@@ -402,64 +606,6 @@ __global__ void k_dot_padded(const double *x, const double *y, Geo g, double *pa
     }
     block_sum_store<1>(s, partials, 0);
 }
-// d (compact FP32) added to the padded FP64 x: x = x + d
-__global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g) {
-    SOLVER_PROLOGUE
-    if (!act) return;
-    x[pidx(pg, k, j, i)] += (double)d[cidx(g, k, j, i)];
-}
-// inner start: r = r32 - mean, cr = r, p = v1 = d = 0; <r,r>
-__global__ void k_inner_init(const float *r32, float *r, float *cr, float *p, float *v1,
-                             float *d, CGeo g, const double *mean_sum, double *partials,
-                             long long stride) {
-    SOLVER_PROLOGUE
-    double s[2] = {0.0, 0.0};
-    if (act) {
-        const long long c = cidx(g, k, j, i);
-        const float mean = (float)(*mean_sum / (double)g.n);
-        const float rv = r32[c] - mean;
-        r[c] = rv;
-        cr[c] = rv;
-        p[c] = 0.f;
-        v1[c] = 0.f;
-        d[c] = 0.f;
-        s[0] = (double)rv * (double)rv;
-    }
-    block_sum_store<2>(s, partials, stride);
-}
-
-// three generic sums into dst[0..2]
-__global__ void k_finalize_sums_3(const double *partials, int nb, double *dst) {
-    const double a = reduce_slot_1024(partials, nb);
-    const double b = reduce_slot_1024(partials + nb, nb);
-    const double d = reduce_slot_1024(partials + 2 * (long long)nb, nb);
-    if (threadIdx.x == 0) {
-        dst[0] = a;
-        dst[1] = b;
-        dst[2] = d;
-    }
-}
-
-// inner solve start: residuals measured relative to ||r_inner,0||
-__global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *sc,
-                                 double rtol) {
-    const double rr = reduce_slot_1024(partials, nb);
-    if (threadIdx.x == 0) {
-        sc->sc_err = rr > 0.0 ? sqrt(rr) : 1.0;
-        sc->rr = rr;
-        sc->err = sqrt(rr) / sc->sc_err;
-        sc->alf = 1.0;
-        sc->omg = 1.0;
-        sc->nrm = 1.0;
-        sc->rho = rr;
-        sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
-        sc->breakdown = 0;
-        sc->nonfinite = !isfinite(rr);
-        sc->done = (rr == 0.0) || (sc->err < rtol) || sc->nonfinite;
-    }
-}
-
-#undef SOLVER_PROLOGUE
 
 // ---------------------------------------------------------------------------
 // host side
@@ -473,62 +619,81 @@ static CGeo cgeo(const Geo &g) {
     return c;
 }
 
-// One BiCGSTAB run on compact vectors of precision R.  The vectors r, cr, p,
-// v1 must already hold the initial state and sc the initial scalars.
-// x: padded FP64 (FP64 path) or a compact R correction (mixed inner).
+constexpr int kMarchPlanes = 8;     // i planes per marching block
+constexpr int kUpdateBlocks = 2048; // grid of the streaming update
+constexpr int kPollEvery = 4;       // host polls the device flags this often
+
+// BiCGSTAB iterations on compact vectors of precision R.  V: r, cr, p, v1, t,
+// s, cs, two sweep buffers, x.  The initial state and scalars must be set.
 template <typename R>
-static int bicgstab_iterate(dcm_ctx *c, R *const *V, void *x, int x_compact, double tol,
-                            int itmax, int check_every, SolverOut *out) {
+static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, SolverOut *out) {
     const Geo &pg = c->geo;
     const CGeo g = cgeo(pg);
-    const Interior L = interior_launch(pg);
-    const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+    const MG m = march_geo(pg, kMarchPlanes);
+    const unsigned mg = march_grid(m);
+    const int nbm = m.nblocks;
     CoefPack cp{c->met, c->met32};
     R *r = V[0], *cr = V[1], *p = V[2], *v1 = V[3], *t = V[4], *s = V[5], *cs = V[6],
-      *wa = V[7], *wb = V[8];
+      *wa = V[7], *wb = V[8], *x = V[9];
     SolverScalars *sc = c->solver;
     double *part = c->partials;
-    int it = 0;
+    const int n = (int)g.n;
     if (itmax < 1) itmax = 1;
-    for (it = 1; it <= itmax; ++it) {
-        int tok = prof_begin(c, "jacobi");
-        hipLaunchKernelGGL((k_jac_first<R, 0>), L.grid, L.block, 0, c->stream, r, v1, t, wa, g,
-                           cp, sc);
-        hipLaunchKernelGGL((k_jac<R, false>), L.grid, L.block, 0, c->stream, wa, t, wb, g, cp, sc);
-        hipLaunchKernelGGL((k_jac<R, true>), L.grid, L.block, 0, c->stream, wb, t, p, g, cp, sc);
-        prof_end(c, tok);
-        tok = prof_begin(c, "matvec_dot");
-        hipLaunchKernelGGL((k_matvec_dot<R, false>), L.grid, L.block, 0, c->stream, p, cr, v1, g,
-                           cp, sc, part, (long long)nb);
-        prof_end(c, tok);
-        hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nb, sc);
-        tok = prof_begin(c, "jacobi");
-        hipLaunchKernelGGL((k_jac_first<R, 1>), L.grid, L.block, 0, c->stream, r, v1, s, wa, g,
-                           cp, sc);
-        hipLaunchKernelGGL((k_jac<R, false>), L.grid, L.block, 0, c->stream, wa, s, wb, g, cp, sc);
-        hipLaunchKernelGGL((k_jac<R, false>), L.grid, L.block, 0, c->stream, wb, s, cs, g, cp, sc);
-        prof_end(c, tok);
-        tok = prof_begin(c, "matvec_dot");
-        hipLaunchKernelGGL((k_matvec_dot<R, true>), L.grid, L.block, 0, c->stream, cs, s, t, g,
-                           cp, sc, part, (long long)nb);
-        prof_end(c, tok);
-        hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nb, sc);
-        tok = prof_begin(c, "update");
-        hipLaunchKernelGGL(k_update<R>, L.grid, L.block, 0, c->stream, x, p, cs, s, t, cr, r, pg,
-                           g, sc, part, (long long)nb, x_compact);
-        prof_end(c, tok);
-        hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(1024), 0, c->stream, part, nb, sc, tol);
-        DCM_LAUNCH_CHECK();
-        if (it % check_every == 0 || it == itmax) {
-            DCM_HIP(hipMemcpyAsync(c->h_solver, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost,
-                                   c->stream));
-            DCM_HIP(hipStreamSynchronize(c->stream));
-            if (c->h_solver->breakdown || c->h_solver->done) break;
+    int launched = 0;
+    while (launched < itmax) {
+        const int chunk = (itmax - launched) < kPollEvery ? (itmax - launched) : kPollEvery;
+        for (int q = 0; q < chunk; ++q) {
+            int tok = prof_begin(c, "jac_first");
+            hipLaunchKernelGGL((k_jac_first<R, 0>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
+                               t, wa, m, cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "jac");
+            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, t, wb,
+                               m, cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "jac_p");
+            hipLaunchKernelGGL((k_jac<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, wb, t, p, m,
+                               cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "matvec_dot");
+            hipLaunchKernelGGL((k_matvec_dot<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, p,
+                               cr, v1, m, cp, sc, part, nbm);
+            prof_end(c, tok);
+            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbm, sc);
+            tok = prof_begin(c, "jac_first");
+            hipLaunchKernelGGL((k_jac_first<R, 1>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
+                               s, wa, m, cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "jac");
+            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, s, wb,
+                               m, cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "jac");
+            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wb, s, cs,
+                               m, cp, sc);
+            prof_end(c, tok);
+            tok = prof_begin(c, "matvec_dot");
+            hipLaunchKernelGGL((k_matvec_dot<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, cs,
+                               s, t, m, cp, sc, part, nbm);
+            prof_end(c, tok);
+            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nbm, sc);
+            tok = prof_begin(c, "update");
+            hipLaunchKernelGGL(k_update<R>, dim3(kUpdateBlocks), dim3(kBlock), 0, c->stream, x, p,
+                               cs, s, t, cr, r, n, sc, part, kUpdateBlocks);
+            prof_end(c, tok);
+            hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(1024), 0, c->stream, part, kUpdateBlocks,
+                               sc, tol);
         }
+        DCM_LAUNCH_CHECK();
+        launched += chunk;
+        DCM_HIP(hipMemcpyAsync(c->h_solver, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost,
+                               c->stream));
+        DCM_HIP(hipStreamSynchronize(c->stream));
+        if (c->h_solver->breakdown || c->h_solver->done) break;
     }
-    // the device stops at the first converged iteration; count from the host
-    // copy of the scalars is exact only when check_every == 1
-    out->iters = it > itmax ? itmax : it;
+    // the kernels after convergence were no-ops; the device counted the
+    // iterations that ran
+    out->iters = c->h_solver->iters;
     out->err = c->h_solver->err;
     if (c->h_solver->breakdown) return DCM_BREAKDOWN;
     if (c->h_solver->nonfinite) {
@@ -546,14 +711,18 @@ int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, i
     const CGeo g = cgeo(pg);
     const Interior L = interior_launch(pg);
     const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+    const MG m = march_geo(pg, kMarchPlanes);
     CoefPack cp{c->met, c->met32};
-    DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
-    DCM_CHECK(ensure_solver_vectors64(c, 9));
-    double *V[9];
-    for (int q = 0; q < 9; ++q) V[q] = c->sv64 + (size_t)q * g.n;
+    size_t need = 4 * (size_t)nb + 64;
+    if (need < 4 * (size_t)m.nblocks + 64) need = 4 * (size_t)m.nblocks + 64;
+    if (need < 4 * (size_t)kUpdateBlocks) need = 4 * (size_t)kUpdateBlocks;
+    DCM_CHECK(ensure_partials(c, need));
+    DCM_CHECK(ensure_solver_vectors64(c, 10));
+    double *V[10];
+    for (int q = 0; q < 10; ++q) V[q] = c->sv64 + (size_t)q * g.n;
     int tok = prof_begin(c, "resid0");
-    hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, c->stream, x, b, V[0], V[1], V[2], V[3], pg,
-                       g, cp, c->partials, (long long)nb);
+    hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, c->stream, x, b, V[0], V[1], V[2], V[3], V[9],
+                       pg, g, cp, c->partials, (long long)nb);
     prof_end(c, tok);
     hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->solver,
                        tol, 1);
@@ -571,7 +740,9 @@ int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, i
         out->iters = 0;
         out->err = c->h_solver->err;
     } else {
-        st = bicgstab_iterate<double>(c, V, x, 0, tol, itmax, 1, out);
+        st = bicgstab_iterate<double>(c, V, tol, itmax, out);
+        hipLaunchKernelGGL(k_scatter_x, L.grid, L.block, 0, c->stream, V[9], x, pg, g);
+        DCM_LAUNCH_CHECK();
     }
     if (st == DCM_OK) {
         // the halo of x as the reference leaves it: periodic faces from
@@ -592,12 +763,16 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
     const CGeo g = cgeo(pg);
     const Interior L = interior_launch(pg);
     const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+    const MG m = march_geo(pg, kMarchPlanes);
     CoefPack cp{c->met, c->met32};
-    DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
+    size_t need = 4 * (size_t)nb + 64;
+    if (need < 4 * (size_t)m.nblocks + 64) need = 4 * (size_t)m.nblocks + 64;
+    if (need < 4 * (size_t)kUpdateBlocks) need = 4 * (size_t)kUpdateBlocks;
+    DCM_CHECK(ensure_partials(c, need));
     DCM_CHECK(ensure_solver_vectors32(c, 11));
     float *V[11];
     for (int q = 0; q < 11; ++q) V[q] = c->sv32 + (size_t)q * g.n;
-    float *r32 = V[9], *d = V[10];
+    float *r32 = V[10], *d = V[9];
     SolverScalars *sc = c->solver;
     out->iters = 0;
     out->outer = 0;
@@ -633,17 +808,16 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
         hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, c->stream, r32, V[0], V[1], V[2],
                            V[3], d, g, c->dscal + 41, c->partials, (long long)nb);
         DCM_LAUNCH_CHECK();
-        // sc_err of the inner run = ||r_inner,0||: its recursive residual
-        // err_inner = ||r_k|| / ||r_0||, stopped at inner_rtol, or once the
-        // outer target is surely met.
+        // the inner recursive residual is relative to ||r_inner,0||; stop at
+        // inner_rtol, or as soon as the outer target is surely met
         double rtol = inner_rtol;
-        const double need = 0.5 * tol * sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
-        if (need > rtol) rtol = need;
+        const double need_rel = 0.5 * tol * sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
+        if (need_rel > rtol) rtol = need_rel;
         hipLaunchKernelGGL(k_fin_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
                            sc, rtol);
         DCM_LAUNCH_CHECK();
         SolverOut io;
-        int st = bicgstab_iterate<float>(c, V, (void *)d, 1, rtol, itmax - out->iters, 1, &io);
+        int st = bicgstab_iterate<float>(c, V, rtol, itmax - out->iters, &io);
         out->iters += io.iters;
         if (st == DCM_NONFINITE) return st;
         // a breakdown or stagnation of the inner solve still returns its

@@ -88,6 +88,8 @@ _SIGS = {
     "dcm_model_profile": ([_vp, _i], _i),
     "dcm_model_profile_filter": ([_vp, C.c_char_p], _i),
     "dcm_model_set_inner_rtol": ([_vp, _d], _i),
+    "dcm_model_set_check_residual": ([_vp, _i], _i),
+    "dcm_stream_copy": ([_vp, _vp, _vp, C.c_size_t, _dp], _i),
     "dcm_model_kernel_time": ([_vp, C.c_char_p, _dp, _ip], _i),
 }
 
@@ -275,6 +277,13 @@ class Context:
         _call("dcm_true_residual", self.h, x.ptr, b.ptr, C.byref(out))
         return out.value
 
+    def stream_copy(self, dst, src, n=None) -> float:
+        """Plain device copy of n doubles; returns its kernel time in ms."""
+        n = dst.nbytes // 8 if n is None else n
+        ms = _d()
+        _call("dcm_stream_copy", self.h, dst.ptr, src.ptr, n, C.byref(ms))
+        return ms.value
+
     # -- a13, a14 -------------------------------------------------------------
     def advance_momentum(self, dt, u, v, w, zu, zv, zw, su, sv, sw, p):
         _call("dcm_advance_momentum", self.h, dt, u.ptr, v.ptr, w.ptr, zu.ptr, zv.ptr, zw.ptr,
@@ -316,6 +325,9 @@ class Model:
 
     def set_inner_rtol(self, rtol):
         _call("dcm_model_set_inner_rtol", self.h, rtol)
+
+    def set_check_residual(self, on=True):
+        _call("dcm_model_set_check_residual", self.h, int(on))
 
     def set(self, field, a, index=0):
         a = np.ascontiguousarray(a, dtype=np.float64)

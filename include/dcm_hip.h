@@ -102,7 +102,7 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
 int dcm_upwind3d(dcm_ctx *ctx, double dt, const double *u, const double *v,
                  const double *w, double *psi, int neumann);
 /* mpdata_3d (:219-714): positive-definite shift, kord passes, optional
- * non-oscillatory limiter.  *lbc/*ubc are shifted and restored as in the
+ * non-oscillatory limiter.  lbc and ubc are shifted and restored as in the
  * reference.  *early_exit (may be NULL) reports the zero-sum exit. */
 int dcm_mpdata3d(dcm_ctx *ctx, double dt, const double *u, const double *v,
                  const double *w, double *psi, double *lbc, double *ubc,
@@ -221,6 +221,18 @@ int dcm_model_profile(dcm_model *m, int enable);
 int dcm_model_profile_filter(dcm_model *m, const char *name);
 /* Relative residual reduction of each FP32 inner solve (mixed; default 1e-5). */
 int dcm_model_set_inner_rtol(dcm_model *m, double rtol);
+/* When on, every pressure solve of the FP64 path is followed by the FP64 true
+ * residual ||b-Ax||/max(||b||,0.01); the largest goes to max_true_resid
+ * (the mixed path always measures it). */
+int dcm_model_set_check_residual(dcm_model *m, int on);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement utility (no counterpart in the reference): a plain device   */
+/* copy of n doubles, the STREAM-copy yardstick and the byte calibration   */
+/* for hardware counters.  Returns the kernel time in ms when ms != NULL.  */
+/* ---------------------------------------------------------------------- */
+int dcm_stream_copy(dcm_ctx *ctx, double *dst, const double *src, size_t n,
+                    double *ms);
 int dcm_model_kernel_time(dcm_model *m, const char *name, double *avg_ms,
                           int *launches);
 

@@ -1,0 +1,35 @@
+"""The C-ABI library loads on a CPU-only host and exports every entry point
+that include/dcm_hip.h declares; the ctypes binding declares the same set.
+No compute call is made here (there is no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from dcm_amd import hip as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dcm_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dcm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_bound():
+    assert declared() == H.declared_symbols()
+
+
+@pytest.mark.skipif(not os.path.exists(H.LIB_PATH), reason="libdcm_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(H.LIB_PATH)
+    missing = [name for name in declared() if not hasattr(lib, name)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(H.LIB_PATH), reason="libdcm_hip.so not built")
+def test_version_string():
+    assert H.load().dcm_version().decode().startswith("dcm_hip")

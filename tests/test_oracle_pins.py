@@ -1,0 +1,648 @@
+"""Pins of the CPU oracle against checks that do not restate its formulas.
+
+Each oracle routine is compared with an independent formulation: an
+emulation of the reference's MPI message passing for the halo fills, closed
+forms (Fourier eigenvalues, exact Courant-1 shifts, no-op cases), dense
+matrices built from the textbook operator, a direct FFT/DCT Poisson solve, a
+textbook BiCGSTAB, a textbook MPDATA written from Smolarkiewicz (1984) and
+Smolarkiewicz & Grabowski (1990), and conservation / monotonicity /
+projection properties.  A transcription slip in the oracle (dropped term,
+wrong sign, wrong index, transposed operand) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.fft
+
+import oracle as O
+from dcm_amd import workloads as W
+from helpers import field, interior, rel_l2, small_domain
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    O.build()
+
+
+# ---------------------------------------------------------------------------
+# a1 fills: emulation of mpi_module.f90's exchange on a 1x1x1 Cartesian grid
+# ---------------------------------------------------------------------------
+PROC_NULL = -1
+
+
+def _find_rank(we, sn, du):
+    """mpi_find_rank (mpi_module.f90:264-295) for dims (1,1,1), periods
+    (T,T,F): the x-y neighbours wrap to rank 0, any z step is PROC_NULL."""
+    return PROC_NULL if du != 0 else 0
+
+
+def _region(a, kl, ku, jl, ju, il, iu):
+    return a[il:iu + 1, jl:ju + 1, kl:ku + 1]
+
+
+def emulate_exchange(a, kp, jp, ip, lbc, ubc, kind):
+    """Run the reference's exchange routines as message passing on one rank:
+    copies for self-neighbours, constant z planes for PROC_NULL, then (full)
+    the 12 edge and 8 corner messages whose buffers are pre-filled from the
+    send region and overwritten only by a receive from a real rank."""
+    a = a.copy()
+    me = 0
+    # faces (exchange_along_dim / _wo): self copies in x and y
+    if kind in ("full", "along_dim", "along_dim_wo", "full_wo"):
+        _region(a, 1, kp, 1, jp, 0, 0)[...] = _region(a, 1, kp, 1, jp, ip, ip)
+        _region(a, 1, kp, 1, jp, ip + 1, ip + 1)[...] = _region(a, 1, kp, 1, jp, 1, 1)
+        _region(a, 1, kp, 0, 0, 1, ip)[...] = _region(a, 1, kp, jp, jp, 1, ip)
+        _region(a, 1, kp, jp + 1, jp + 1, 1, ip)[...] = _region(a, 1, kp, 1, 1, 1, ip)
+    if kind in ("full", "along_dim", "along_z"):
+        a[:, :, kp + 1] = ubc
+        a[:, :, 0] = lbc
+    if kind in ("full", "full_wo"):
+        # (send region, receive region, neighbour offset (we,sn,du)) from
+        # mpi_module.f90:965-1136, halo 1
+        msgs = [
+            ((1, kp, 1, 1, 1, 1), (1, kp, jp + 1, jp + 1, ip + 1, ip + 1), (-1, -1, 0)),
+            ((1, kp, jp, jp, 1, 1), (1, kp, 0, 0, ip + 1, ip + 1), (-1, 1, 0)),
+            ((1, kp, 1, 1, ip, ip), (1, kp, jp + 1, jp + 1, 0, 0), (1, -1, 0)),
+            ((1, kp, jp, jp, ip, ip), (1, kp, 0, 0, 0, 0), (1, 1, 0)),
+            ((1, 1, 1, 1, 1, ip), (kp + 1, kp + 1, jp + 1, jp + 1, 1, ip), (0, -1, -1)),
+            ((1, 1, jp, jp, 1, ip), (kp + 1, kp + 1, 0, 0, 1, ip), (0, 1, -1)),
+            ((kp, kp, 1, 1, 1, ip), (0, 0, jp + 1, jp + 1, 1, ip), (0, -1, 1)),
+            ((kp, kp, jp, jp, 1, ip), (0, 0, 0, 0, 1, ip), (0, 1, 1)),
+            ((1, 1, 1, jp, 1, 1), (kp + 1, kp + 1, 1, jp, ip + 1, ip + 1), (-1, 0, -1)),
+            ((1, 1, 1, jp, ip, ip), (kp + 1, kp + 1, 1, jp, 0, 0), (1, 0, -1)),
+            ((kp, kp, 1, jp, 1, 1), (0, 0, 1, jp, ip + 1, ip + 1), (-1, 0, 1)),
+            ((kp, kp, 1, jp, ip, ip), (0, 0, 1, jp, 0, 0), (1, 0, 1)),
+            # corners
+            ((1, 1, 1, 1, 1, 1), (kp + 1, kp + 1, jp + 1, jp + 1, ip + 1, ip + 1), (-1, -1, -1)),
+            ((1, 1, jp, jp, 1, 1), (kp + 1, kp + 1, 0, 0, ip + 1, ip + 1), (-1, 1, -1)),
+            ((kp, kp, 1, 1, 1, 1), (0, 0, jp + 1, jp + 1, ip + 1, ip + 1), (-1, -1, 1)),
+            ((kp, kp, jp, jp, 1, 1), (0, 0, 0, 0, ip + 1, ip + 1), (-1, 1, 1)),
+            ((1, 1, 1, 1, ip, ip), (kp + 1, kp + 1, jp + 1, jp + 1, 0, 0), (1, -1, -1)),
+            ((1, 1, jp, jp, ip, ip), (kp + 1, kp + 1, 0, 0, 0, 0), (1, 1, -1)),
+            ((kp, kp, 1, 1, ip, ip), (0, 0, jp + 1, jp + 1, 0, 0), (1, -1, 1)),
+            ((kp, kp, jp, jp, ip, ip), (0, 0, 0, 0, 0, 0), (1, 1, 1)),
+        ]
+        bufs = []
+        for send, recv, off in msgs:
+            buf = _region(a, *send).copy()          # pre-filled from the send region
+            src = _find_rank(*[-o for o in off])    # the rank that would send to us
+            if src != me and src != PROC_NULL:      # never on one rank
+                raise AssertionError("unexpected neighbour")
+            bufs.append((recv, buf))                # PROC_NULL: buffer unchanged
+        for recv, buf in bufs:                      # write buffers after Waitall
+            _region(a, *recv)[...] = buf
+    return a
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (1, 3, 4), (6, 1, 2), (7, 5, 9)])
+@pytest.mark.parametrize("kind", ["full", "along_dim", "along_dim_wo", "full_wo", "along_z"])
+def test_fills_match_mpi_emulation(shape, kind):
+    ip, jp, kp = shape
+    dom = small_domain(ip, jp, kp)
+    g = O.Grid(dom)
+    tags = np.arange(np.prod(dom.shape), dtype=np.float64).reshape(dom.shape) + 1.0
+    want = emulate_exchange(tags, kp, jp, ip, -7.0, -9.0, kind)
+    got = tags.copy()
+    if kind in ("full", "along_dim", "along_z"):
+        getattr(O, "fill_" + kind)(g, got, -7.0, -9.0)
+    else:
+        getattr(O, "fill_" + kind)(g, got)
+    np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------
+# a2 donor cell
+# ---------------------------------------------------------------------------
+def _pow2_domain(ip=8, jp=6, kp=5):
+    d = W.make_domain(ip, jp, kp, 64.0, 64.0, 64.0)
+    d.rhoa = np.ones(kp + 2)
+    d.rhoan = np.ones(kp + 2)
+    return d
+
+
+def test_upwind_courant_one_shift_is_exact():
+    dom = _pow2_domain()
+    g = O.Grid(dom)
+    psi = field(dom, 1, 0.0, 1.0)
+    O.fill_full(g, psi)
+    u = np.full(dom.shape, 64.0)  # u dt / dx = 1
+    v = np.zeros(dom.shape)
+    w = np.zeros(dom.shape)
+    new = psi.copy()
+    O.upwind3d(g, 1.0, u, v, w, new)
+    np.testing.assert_array_equal(interior(new), psi[0:-2, 1:-1, 1:-1])
+
+
+def test_upwind_textbook_1d():
+    dom = small_domain(16, 3, 3)
+    g = O.Grid(dom)
+    psi = field(dom, 2, 0.0, 1.0)
+    O.fill_full(g, psi)
+    c = -0.37
+    u = np.full(dom.shape, c * dom.dxn[1])
+    zero = np.zeros(dom.shape)
+    new = psi.copy()
+    O.upwind3d(g, 1.0, u, zero, zero, new)
+    p = interior(psi)
+    ref = p - c * (np.roll(p, -1, axis=0) - p)  # upwind for c < 0
+    np.testing.assert_allclose(interior(new), ref, rtol=0, atol=1e-15)
+
+
+def test_upwind_conserves_density_weighted_mass():
+    dom = small_domain(10, 8, 7, varying_theta=True)
+    g = O.Grid(dom)
+    psi = field(dom, 3, 0.0, 1.0)
+    O.fill_full(g, psi)
+    u, v, w = (field(dom, 4 + n, -1, 1) * 10.0 for n in range(3))
+    for a in (u, v, w):
+        O.fill_full(g, a)
+    w[:, :, 0] = 0.0
+    w[:, :, dom.kp] = 0.0  # no flux through floor and lid
+    m0 = (interior(psi) * dom.rhoan[None, None, 1:-1]).sum()
+    O.upwind3d(g, 1.0, u, v, w, psi)
+    m1 = (interior(psi) * dom.rhoan[None, None, 1:-1]).sum()
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
+# ---------------------------------------------------------------------------
+# a3/a4 MPDATA: textbook restatement (Smolarkiewicz 1984; Smolarkiewicz &
+# Grabowski 1990), constant density, written with periodic rolls.
+# ---------------------------------------------------------------------------
+EPS = 1e-60
+
+
+def _shift(a, axis, n):
+    return np.roll(a, -n, axis=axis)
+
+
+def textbook_mpdata(psi, U, V, Wz, dx, dy, dz, dt, limiter=True):
+    """One MPDATA step (kord=2) for a field on a periodic box padded with
+    zeros in z.  U,V,Wz are face velocities (face i+1/2 stored at i), psi >= 0.
+    Courant numbers: c = u dt / dx etc.  Returns the new field."""
+    def donor(f, cx, cy, cz):
+        def flux(cf, fl, fr):
+            return np.maximum(cf, 0) * fl + np.minimum(cf, 0) * fr
+        Fx = flux(cx, f, _shift(f, 0, 1))
+        Fy = flux(cy, f, _shift(f, 1, 1))
+        Fz = flux(cz, f, _shift(f, 2, 1))
+        return f - (Fx - _shift(Fx, 0, -1)) - (Fy - _shift(Fy, 1, -1)) - (Fz - _shift(Fz, 2, -1))
+
+    cx, cy, cz = U * dt / dx, V * dt / dy, Wz * dt / dz
+    p1 = donor(psi, cx, cy, cz)
+    f = p1
+
+    def A(ax):  # (f_{+1} - f) / (f_{+1} + f), at faces of axis ax
+        fp = _shift(f, ax, 1)
+        return (fp - f) / (fp + f + EPS)
+
+    def B(ax, other):  # cross derivative term across axis `other` at faces of `ax`
+        fpo = _shift(_shift(f, ax, 1), other, 1)
+        fo = _shift(f, other, 1)
+        fmo = _shift(_shift(f, ax, 1), other, -1)
+        fm = _shift(f, other, -1)
+        return 0.5 * (fpo + fo - fmo - fm) / (fpo + fo + fmo + fm + EPS)
+
+    def avg(c, ax, other):  # Courant number of `other` averaged to faces of `ax`
+        return 0.25 * (c + _shift(c, ax, 1) + _shift(c, other, -1) + _shift(_shift(c, ax, 1), other, -1))
+
+    def div_term(cc, ax):
+        # eq 38: 0.25 c (c_{+1} - c_{-1} + divergence of the other components)
+        d = 0.0
+        for c2, a2 in ((cx, 0), (cy, 1), (cz, 2)):
+            dc = c2 - _shift(c2, a2, -1)
+            d = d + dc + _shift(dc, ax, 1)
+        return 0.25 * cc * d
+
+    cs = (cx, cy, cz)
+    anti = []
+    for ax in range(3):
+        c = cs[ax]
+        t = (np.abs(c) - c * c) * A(ax)
+        for other in range(3):
+            if other != ax:
+                t = t - 2.0 * 0.5 * c * avg(cs[other], ax, other) * B(ax, other)
+        t = t - div_term(c, ax)
+        anti.append(t)
+    if limiter:
+        psi_pairs = [psi, p1]
+        mx = np.maximum.reduce([np.maximum.reduce([q, _shift(q, 0, 1), _shift(q, 0, -1), _shift(q, 1, 1),
+                                                    _shift(q, 1, -1), _shift(q, 2, 1), _shift(q, 2, -1)])
+                                for q in psi_pairs])
+        mn = np.minimum.reduce([np.minimum.reduce([q, _shift(q, 0, 1), _shift(q, 0, -1), _shift(q, 1, 1),
+                                                    _shift(q, 1, -1), _shift(q, 2, 1), _shift(q, 2, -1)])
+                                for q in psi_pairs])
+        ax_, ay_, az_ = anti
+        inflow = (np.maximum(_shift(ax_, 0, -1), 0) * _shift(f, 0, -1) - np.minimum(ax_, 0) * _shift(f, 0, 1)
+                  + np.maximum(_shift(ay_, 1, -1), 0) * _shift(f, 1, -1) - np.minimum(ay_, 0) * _shift(f, 1, 1)
+                  + np.maximum(_shift(az_, 2, -1), 0) * _shift(f, 2, -1) - np.minimum(az_, 0) * _shift(f, 2, 1))
+        outflow = (np.maximum(ax_, 0) - np.minimum(_shift(ax_, 0, -1), 0)
+                   + np.maximum(ay_, 0) - np.minimum(_shift(ay_, 1, -1), 0)
+                   + np.maximum(az_, 0) - np.minimum(_shift(az_, 2, -1), 0)) * f
+        bup = (mx - f) / (inflow + EPS)
+        bdn = (f - mn) / (outflow + EPS)
+        lim = []
+        for ax, c in enumerate(anti):
+            lim.append(np.minimum.reduce([np.ones_like(c), bdn, _shift(bup, ax, 1)]) * np.maximum(c, 0)
+                       + np.minimum.reduce([np.ones_like(c), bup, _shift(bdn, ax, 1)]) * np.minimum(c, 0))
+        anti = lim
+    return donor(p1, *anti)
+
+
+@pytest.mark.parametrize("limiter", [False, True])
+def test_mpdata_matches_textbook_constant_density(limiter):
+    ip, jp, kp = 10, 8, 9
+    d = 32.0
+    dom = W.make_domain(ip, jp, kp, d, d, d)
+    dom.rhoa = np.ones(kp + 2)
+    dom.rhoan = np.ones(kp + 2)
+    g = O.Grid(dom)
+    r = np.random.default_rng(5)
+    psi = np.zeros(dom.shape)
+    psi[1:-1, 1:-1, 3:-3] = r.random((ip, jp, kp - 4))   # zero near floor and lid
+    O.fill_full(g, psi)
+    u, v, w = (np.zeros(dom.shape) for _ in range(3))
+    u[1:-1, 1:-1, 1:-1] = 0.3 * d * (r.random((ip, jp, kp)) - 0.5)
+    v[1:-1, 1:-1, 1:-1] = 0.3 * d * (r.random((ip, jp, kp)) - 0.5)
+    w[1:-1, 1:-1, 2:-3] = 0.3 * d * (r.random((ip, jp, kp - 3)) - 0.5)  # w = 0 near floor, lid
+    for a in (u, v, w):
+        O.fill_full(g, a)
+    got = psi.copy()
+    O.mpdata3d(g, 1.0, u, v, w, got, 0.0, 0.0, 2, limiter, 0)
+    # textbook on the interior box, periodic in x,y, zero-padded in z
+    P = np.zeros((ip, jp, kp + 2))
+    P[:, :, 1:-1] = interior(psi)
+    def faces(a):
+        F = np.zeros((ip, jp, kp + 2))
+        F[:, :, 1:-1] = interior(a)
+        return F
+    want = textbook_mpdata(P, faces(u), faces(v), faces(w), d, d, d, 1.0, limiter)[:, :, 1:-1]
+    np.testing.assert_allclose(interior(got), want, rtol=1e-12, atol=1e-13)
+
+
+def test_mpdata_conservation_positivity_monotonicity():
+    dom = small_domain(12, 10, 9, varying_theta=True)
+    g = O.Grid(dom)
+    psi = np.zeros(dom.shape)
+    psi[3:8, 3:7, 3:6] = 1.0  # a step: textbook MPDATA without limiter overshoots
+    O.fill_full(g, psi)
+    # a uniform (non-divergent) flow: the limited scheme may not leave the
+    # initial range (Smolarkiewicz & Grabowski 1990); divergent flow may
+    # concentrate the scalar legitimately, so it is not used here
+    u, v, w = (np.zeros(dom.shape) for _ in range(3))
+    u[...] = 0.2 * dom.dx[1]
+    v[...] = -0.15 * dom.dy[1]
+    for a in (u, v):
+        O.fill_full(g, a)
+    m0 = (interior(psi) * dom.rhoan[None, None, 1:-1]).sum()
+    lim = psi.copy()
+    O.mpdata3d(g, 1.0, u, v, w, lim, 0.0, 0.0, 2, True, 0)
+    m1 = (interior(lim) * dom.rhoan[None, None, 1:-1]).sum()
+    assert abs(m1 - m0) <= 1e-12 * m0
+    assert interior(lim).min() >= -1e-12 and interior(lim).max() <= 1.0 + 1e-12
+    nolim = psi.copy()
+    O.mpdata3d(g, 1.0, u, v, w, nolim, 0.0, 0.0, 2, False, 0)
+    assert interior(nolim).max() > 1.0 + 1e-6 or interior(nolim).min() < -1e-6
+
+
+def test_mpdata_zero_sum_exit_leaves_field():
+    dom = small_domain(6, 5, 4)
+    g = O.Grid(dom)
+    psi = np.full(dom.shape, 2.5)
+    u, v, w = (field(dom, n) for n in range(3))
+    r, lb, ub = O.mpdata3d(g, 1.0, u, v, w, psi, 2.5, 2.5)
+    assert r == 1 and (lb, ub) == (2.5, 2.5)
+    np.testing.assert_array_equal(psi, 2.5)
+
+
+def test_mpdata_vec_single_field_equals_scalar_with_interior_min():
+    """With one field and a halo that does not hold the minimum, the vector
+    variant (interior minimum) equals the scalar one (whole-array minimum)."""
+    dom = small_domain(9, 7, 6, varying_theta=True)
+    g = O.Grid(dom)
+    psi = field(dom, 21, 0.5, 2.0)
+    O.fill_full(g, psi, 0.7, 0.8)
+    u, v, w = (field(dom, 22 + n, -1, 1) * 5 for n in range(3))
+    for a in (u, v, w):
+        O.fill_full(g, a)
+    a1 = psi.copy()
+    O.mpdata3d(g, 1.0, u, v, w, a1, 0.7, 0.8)
+    a2 = psi[None].copy()
+    O.mpdata_vec3d(g, 1.0, u, v, w, a2, [0.7], [0.8])
+    np.testing.assert_array_equal(a1, a2[0])
+
+
+# ---------------------------------------------------------------------------
+# a5 adv_ref_state
+# ---------------------------------------------------------------------------
+def test_adv_ref_state_closed_forms():
+    dom = small_domain(6, 5, 8)
+    g = O.Grid(dom)
+    w = field(dom, 31)
+    psi = field(dom, 32)
+    same = psi.copy()
+    O.adv_ref_state(g, 1.0, w, same, np.full(dom.kp + 2, 300.0))
+    np.testing.assert_array_equal(same, psi)  # isentropic: exact no-op
+    # unit density, uniform w, linear theta: d(psi) = -dt * w * dtheta/dz
+    dom.rhoa = np.ones(dom.kp + 2)
+    dom.rhoan = np.ones(dom.kp + 2)
+    g = O.Grid(dom)
+    wu = np.full(dom.shape, 0.75)
+    gam = 0.004
+    ref = 300.0 + gam * dom.zn
+    a = np.zeros(dom.shape)
+    O.adv_ref_state(g, 2.0, wu, a, ref)
+    np.testing.assert_allclose(a[:, :, 1:-1], -2.0 * 0.75 * gam, rtol=1e-12)
+    np.testing.assert_array_equal(a[:, :, [0, -1]], 0.0)
+
+
+# ---------------------------------------------------------------------------
+# a6 horizontal means
+# ---------------------------------------------------------------------------
+def test_hmeans_numpy():
+    dom = small_domain(7, 6, 5)
+    g = O.Grid(dom)
+    fs = [field(dom, 40 + n) for n in range(5)]
+    out = O.hmeans(g, fs[0], fs[1], fs[2], fs[3], np.stack([fs[4]]))
+    for a, b in zip(out[:4], fs[:4]):
+        np.testing.assert_allclose(a, b[1:-1, 1:-1, :].mean(axis=(0, 1)), rtol=1e-14, atol=1e-16)
+    np.testing.assert_allclose(out[4][0], fs[4][1:-1, 1:-1, :].mean(axis=(0, 1)), rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# a7 sources: uniform flow, hydrostatic balance, projection
+# ---------------------------------------------------------------------------
+def test_sources_uniform_flow_has_no_tendency():
+    dom = small_domain(8, 6, 7)
+    g = O.Grid(dom)
+    u = np.full(dom.shape, 3.0)
+    v = np.full(dom.shape, -2.0)
+    w = np.zeros(dom.shape)
+    th = np.zeros(dom.shape)
+    outs = [np.zeros(dom.shape) for _ in range(5)]
+    O.sources(g, u, v, w, th, dom.theta, np.zeros(dom.kp + 2), *outs)
+    for a in outs:
+        np.testing.assert_allclose(interior(a), 0.0, atol=1e-15)
+
+
+def test_sources_buoyancy_of_a_layer():
+    """Horizontally uniform theta' at rest: su=sv=0 and sw is the buoyancy
+    g*theta'/theta-bar averaged to w levels (weighted by rhoa)."""
+    dom = small_domain(6, 5, 8)
+    g = O.Grid(dom)
+    zero = np.zeros(dom.shape)
+    th = np.zeros(dom.shape)
+    th[:, :, 3] = 1.5
+    thbar = th[1:-1, 1:-1, :].mean(axis=(0, 1))
+    su, sv, sw, sth, rhs = (np.zeros(dom.shape) for _ in range(5))
+    O.sources(g, zero, zero, zero, th, dom.theta, thbar, su, sv, sw, sth, rhs)
+    np.testing.assert_array_equal(interior(su), 0.0)
+    exp = np.zeros(dom.kp + 2)
+    for k in (2, 3):  # w levels touching the layer
+        exp[k] = dom.rhoa[k] * 9.81 * 1.5 / (2 * dom.theta[k] + thbar[k] + thbar[k + 1])
+    exp[dom.kp] = 0.0
+    np.testing.assert_allclose(sw[3, 2, 1:dom.kp + 1], exp[1:dom.kp + 1], rtol=1e-14)
+
+
+def test_pressure_projection_makes_mass_flux_divergence_free():
+    """sources -> Poisson -> advance_momentum from rest: the new rho-weighted
+    velocity has (discretely) zero divergence in the interior."""
+    dom = small_domain(12, 10, 10, varying_theta=True)
+    g = O.Grid(dom)
+    zero = np.zeros(dom.shape)
+    th = field(dom, 51, -1.0, 1.0)
+    O.fill_full(g, th)
+    thbar = O.hmeans(g, zero, zero, zero, th)[3]
+    su, sv, sw, sth, rhs = (np.zeros(dom.shape) for _ in range(5))
+    O.sources(g, zero, zero, zero, th, dom.theta, thbar, su, sv, sw, sth, rhs)
+    O.fill_along_dim_wo(g, rhs)
+    p = np.zeros(dom.shape)
+    st, it, err = O.bicgstab(g, sw, p, rhs, 1e-11)
+    assert st == 0
+    O.fill_along_dim_wo(g, p)
+    u, v, w = (np.zeros(dom.shape) for _ in range(3))
+    O.advance_momentum(g, 2.0, u, v, w, zero, zero, zero, su, sv, sw, p)
+    for a in (u, v):
+        O.fill_along_dim_wo(g, a)
+    def div(u, v, w):
+        d = np.zeros((dom.ip, dom.jp, dom.kp))
+        for k in range(1, dom.kp + 1):
+            d[:, :, k - 1] = (dom.rhoan[k] * (u[1:-1, 1:-1, k] - u[:-2, 1:-1, k]) / dom.dx[0]
+                              + dom.rhoan[k] * (v[1:-1, 1:-1, k] - v[1:-1, :-2, k]) / dom.dy[0]
+                              + (dom.rhoa[k] * w[1:-1, 1:-1, k] - dom.rhoa[k - 1] * w[1:-1, 1:-1, k - 1]) / dom.dz[0])
+        return d
+    d_new = div(u, v, w)[:, :, 1:-1]
+    # the same increments without the pressure gradient
+    u0, v0, w0 = (np.zeros(dom.shape) for _ in range(3))
+    O.advance_momentum(g, 2.0, u0, v0, w0, zero, zero, zero, su, sv, sw, np.zeros(dom.shape))
+    for a in (u0, v0):
+        O.fill_along_dim_wo(g, a)
+    d_old = div(u0, v0, w0)[:, :, 1:-1]
+    assert np.linalg.norm(d_new) < 1e-6 * np.linalg.norm(d_old)
+
+
+# ---------------------------------------------------------------------------
+# a9-a11 operator, preconditioner, dot product
+# ---------------------------------------------------------------------------
+def _laplacian_dense(dom, with_diag=False):
+    """The 7-point operator from its definition: periodic x,y, Neumann z
+    (ghost = first interior level), coefficients 1/(d(i-1) dn(i)).  With
+    with_diag, also the unfolded centre weight -sum(coefficients), which the
+    Jacobi sweeps divide by (the Neumann ghost is not folded into it)."""
+    ip, jp, kp = dom.ip, dom.jp, dom.kp
+    n = ip * jp * kp
+    A = np.zeros((n, n))
+    diag = np.zeros(n)
+    idx = lambda k, j, i: ((i - 1) * jp + (j - 1)) * kp + (k - 1)
+    for i in range(1, ip + 1):
+        for j in range(1, jp + 1):
+            for k in range(1, kp + 1):
+                r = idx(k, j, i)
+                terms = [
+                    (1.0 / (dom.dx[i - 1] * dom.dxn[i]), (k, j, i % ip + 1)),
+                    (1.0 / (dom.dx[i - 1] * dom.dxn[i - 1]), (k, j, (i - 2) % ip + 1)),
+                    (1.0 / (dom.dy[j - 1] * dom.dyn[j]), (k, j % jp + 1, i)),
+                    (1.0 / (dom.dy[j - 1] * dom.dyn[j - 1]), (k, (j - 2) % jp + 1, i)),
+                    (1.0 / (dom.dz[k - 1] * dom.dzn[k]), (min(k + 1, kp), j, i)),
+                    (1.0 / (dom.dz[k - 1] * dom.dzn[k - 1]), (max(k - 1, 1), j, i)),
+                ]
+                for cval, (kk, jj, ii) in terms:
+                    A[r, idx(kk, jj, ii)] += cval
+                    A[r, r] -= cval
+                    diag[r] -= cval
+    return (A, diag) if with_diag else A
+
+
+def _to_vec(a):
+    return np.ascontiguousarray(interior(a)).reshape(-1)
+
+
+def test_matvec_dense_and_fourier():
+    dom = small_domain(4, 5, 6, stretched=True)
+    g = O.Grid(dom)
+    x = field(dom, 61)
+    ax = np.zeros(dom.shape)
+    O.matvec(g, x, ax)
+    A = _laplacian_dense(dom)
+    np.testing.assert_allclose(_to_vec(ax), A @ _to_vec(x), rtol=1e-12, atol=1e-12)
+    # eigenfunction of the uniform-grid operator
+    dom = small_domain(8, 6, 5)
+    g = O.Grid(dom)
+    m, l, n = 2, 1, 3
+    I, J, K = np.meshgrid(np.arange(dom.ip + 2), np.arange(dom.jp + 2), np.arange(dom.kp + 2), indexing="ij")
+    x = np.cos(2 * math.pi * m * I / dom.ip) * np.cos(2 * math.pi * l * J / dom.jp) * np.cos(math.pi * n * (K - 0.5) / dom.kp)
+    lam = (2 * (math.cos(2 * math.pi * m / dom.ip) - 1) / dom.dx[0] ** 2
+           + 2 * (math.cos(2 * math.pi * l / dom.jp) - 1) / dom.dy[0] ** 2
+           + 2 * (math.cos(math.pi * n / dom.kp) - 1) / dom.dz[0] ** 2)
+    ax = np.zeros(dom.shape)
+    O.matvec(g, x.copy(), ax)
+    np.testing.assert_allclose(interior(ax), lam * interior(x), atol=1e-14)
+
+
+def test_precondition_is_neumann_series_jacobi():
+    dom = small_domain(4, 4, 5, stretched=True)
+    g = O.Grid(dom)
+    r = field(dom, 71)
+    s = np.zeros(dom.shape)
+    O.precondition(g, s, r, 3)
+    A, diag = _laplacian_dense(dom, with_diag=True)
+    Dinv = np.diag(1.0 / diag)
+    N = np.eye(A.shape[0]) - Dinv @ A
+    M = sum(np.linalg.matrix_power(N, q) for q in range(4)) @ Dinv
+    np.testing.assert_allclose(_to_vec(s), M @ _to_vec(r), rtol=1e-11, atol=1e-12)
+
+
+def test_inn_prod_exact_on_integers():
+    dom = small_domain(9, 7, 8)
+    g = O.Grid(dom)
+    r = np.random.default_rng(3)
+    x = r.integers(-1000, 1000, dom.shape).astype(np.float64)
+    y = r.integers(-1000, 1000, dom.shape).astype(np.float64)
+    assert O.inn_prod(g, x, y) == float((interior(x).astype(np.int64) * interior(y).astype(np.int64)).sum())
+
+
+# ---------------------------------------------------------------------------
+# a12 BiCGSTAB
+# ---------------------------------------------------------------------------
+def _compatible_rhs(dom, seed):
+    r = np.random.default_rng(seed)
+    b = np.zeros(dom.shape)
+    b[1:-1, 1:-1, 1:-1] = r.standard_normal((dom.ip, dom.jp, dom.kp))
+    b[1:-1, 1:-1, 1:-1] -= b[1:-1, 1:-1, 1:-1].mean()
+    return b
+
+
+def test_bicgstab_matches_fft_dct_direct_solve():
+    dom = small_domain(16, 12, 10)
+    g = O.Grid(dom)
+    b = _compatible_rhs(dom, 81)
+    x = np.zeros(dom.shape)
+    st, it, err = O.bicgstab(g, np.zeros(dom.shape), x, b.copy(), 1e-12)
+    assert st == 0
+    # direct: FFT in x,y and DCT-II in z diagonalise the operator
+    B = scipy.fft.dct(np.fft.fft2(interior(b), axes=(0, 1)), type=2, axis=2, norm="ortho")
+    m = np.arange(dom.ip)[:, None, None]
+    l = np.arange(dom.jp)[None, :, None]
+    n = np.arange(dom.kp)[None, None, :]
+    lam = (2 * (np.cos(2 * np.pi * m / dom.ip) - 1) / dom.dx[0] ** 2
+           + 2 * (np.cos(2 * np.pi * l / dom.jp) - 1) / dom.dy[0] ** 2
+           + 2 * (np.cos(np.pi * n / dom.kp) - 1) / dom.dz[0] ** 2)
+    lam[0, 0, 0] = 1.0
+    X = B / lam
+    X[0, 0, 0] = 0.0
+    xd = np.real(np.fft.ifft2(scipy.fft.idct(X, type=2, axis=2, norm="ortho"), axes=(0, 1)))
+    xi = interior(x)
+    assert rel_l2(xd, xi - xi.mean()) < 1e-9
+
+
+def textbook_bicgstab(A, M, b, x, tol, sc):
+    """Preconditioned BiCGSTAB, van der Vorst (1992) / Templates Alg. 7.7 with
+    right preconditioning; stop on ||r|| / sc < tol."""
+    r = b - A @ x
+    rt = r.copy()
+    rho_old = alpha = omega = 1.0
+    v = np.zeros_like(b)
+    p = np.zeros_like(b)
+    for it in range(1, 500):
+        rho = rt @ r
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        ph = M @ p
+        v = A @ ph
+        alpha = rho / (rt @ v)
+        s = r - alpha * v
+        sh = M @ s
+        t = A @ sh
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * ph + omega * sh
+        r = s - omega * t
+        rho_old = rho
+        if np.linalg.norm(r) / sc < tol:
+            return x, it
+    raise AssertionError("no convergence")
+
+
+def test_bicgstab_matches_textbook_iterates():
+    dom = small_domain(5, 4, 6, stretched=True)
+    g = O.Grid(dom)
+    A, diag = _laplacian_dense(dom, with_diag=True)
+    # on a stretched grid A is not symmetric: a right-hand side in the range
+    # of A is A times something
+    b = np.zeros(dom.shape)
+    b[1:-1, 1:-1, 1:-1] = (A @ _to_vec(field(dom, 91))).reshape(dom.ip, dom.jp, dom.kp)
+    x0 = 0.1 * field(dom, 92)
+    x = x0.copy()
+    st, it, err = O.bicgstab(g, np.zeros(dom.shape), x, b.copy(), 1e-9)
+    assert st == 0
+    Dinv = np.diag(1.0 / diag)
+    N = np.eye(A.shape[0]) - Dinv @ A
+    M = sum(np.linalg.matrix_power(N, q) for q in range(4)) @ Dinv
+    bv = _to_vec(b)
+    sc = max(np.linalg.norm(bv), 0.01)
+    xt, itt = textbook_bicgstab(A, M, bv, _to_vec(x0), 1e-9, sc)
+    assert it == itt
+    assert rel_l2(xt, _to_vec(x)) < 1e-9
+
+
+def test_bicgstab_true_residual_and_failure_codes():
+    dom = small_domain(8, 6, 6)
+    g = O.Grid(dom)
+    b = _compatible_rhs(dom, 95)
+    x = np.zeros(dom.shape)
+    st, it, err = O.bicgstab(g, np.zeros(dom.shape), x, b.copy(), 1e-10)
+    A = _laplacian_dense(dom)
+    rr = np.linalg.norm(_to_vec(b) - A @ _to_vec(x)) / max(np.linalg.norm(_to_vec(b)), 0.01)
+    assert st == 0 and rr < 1e-9
+    x = np.zeros(dom.shape)
+    st, it, err = O.bicgstab(g, np.zeros(dom.shape), x, b.copy(), 1e-30, itmax=4)
+    assert st == O.lib().or_bicgstab.restype(1).value and it == 4
+
+
+# ---------------------------------------------------------------------------
+# a13-a15 momentum update, scalars, the step
+# ---------------------------------------------------------------------------
+def test_advance_momentum_constant_pressure_and_zero_sources():
+    dom = small_domain(6, 5, 4)
+    g = O.Grid(dom)
+    zu, zv, zw = (field(dom, 100 + n) for n in range(3))
+    zero = np.zeros(dom.shape)
+    u, v, w = (np.zeros(dom.shape) for _ in range(3))
+    O.advance_momentum(g, 2.0, u, v, w, zu, zv, zw, zero, zero, zero, np.full(dom.shape, 7.0))
+    for a, z in ((u, zu), (v, zv), (w, zw)):
+        np.testing.assert_array_equal(a[1:-1, 1:-1, 0:-1], z[1:-1, 1:-1, 0:-1])
+
+
+def test_model_conserves_theta_mass_and_is_deterministic():
+    case = W.warm_bubble(12, 12, 12, steps=3)
+    m1 = O.Model(case)
+    rn = case.dom.rhoan[None, None, 1:-1]
+    mass0 = (interior(m1.th) * rn).sum()
+    m1.step(3)
+    mass1 = (interior(m1.th) * rn).sum()
+    # theta' is shifted by its minimum inside MPDATA, so the conserved sum
+    # is of (theta' - m); with w=0 at floor and lid the total is kept
+    assert abs(mass1 - mass0) < 1e-10 * abs(mass0) + 1e-9
+    m2 = O.Model(case)
+    m2.step(3)
+    np.testing.assert_array_equal(m1.th, m2.th)
+    assert m1.iters == m2.iters

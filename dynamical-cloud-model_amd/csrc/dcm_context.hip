@@ -298,9 +298,9 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
 int dcm_context_destroy(dcm_ctx *c) {
     if (!c) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(c->stream);
-    for (auto &m : c->mirrors) hipFree(m.second.dev);
-    for (double *p : c->work64) hipFree(p);
-    for (float *p : c->work32) hipFree(p);
+    for (auto &m : c->mirrors) (void)hipFree(m.second.dev);
+    for (double *p : c->work64) (void)hipFree(p);
+    for (float *p : c->work32) (void)hipFree(p);
     (void)hipFree(c->sv64);
     (void)hipFree(c->sv32);
     (void)hipFree(c->partials);
@@ -316,7 +316,7 @@ int dcm_context_destroy(dcm_ctx *c) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
     }
-    for (auto e : c->event_pool) hipEventDestroy(e);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return DCM_OK;
@@ -410,7 +410,7 @@ int dcm_mirror_download(dcm_ctx *c, void *host) {
 int dcm_mirror_release_all(dcm_ctx *c) {
     if (!c) return DCM_BAD_ARG;
     DCM_HIP(hipStreamSynchronize(c->stream));
-    for (auto &m : c->mirrors) hipFree(m.second.dev);
+    for (auto &m : c->mirrors) (void)hipFree(m.second.dev);
     c->mirrors.clear();
     return DCM_OK;
 }

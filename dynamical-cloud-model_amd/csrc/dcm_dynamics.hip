@@ -166,6 +166,20 @@ __global__ __launch_bounds__(256) void k_advance_momentum(
     w[c] = zw[c] + ((sw[c] - (p[c + 1] - pc) / M.dzn[k]) * dt) / M.rhoa[k];
 }
 
+// divergence_calc (diagnostics.f90:140-148) on the interior
+__global__ __launch_bounds__(256) void k_divergence(const double *u, const double *v,
+                                                    const double *w, double *div, Geo g,
+                                                    Metrics M) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    div[c] = M.rhoan[k] * (u[c] - u[c - g.si]) / M.dx[i - 1] +
+             M.rhoan[k] * (v[c] - v[c - g.sj]) / M.dy[j - 1] +
+             (M.rhoa[k] * w[c] - M.rhoa[k - 1] * w[c - 1]) / M.dz[k - 1];
+}
+
 // advance_scalar_fields_3d (sgm/subgrid_3d.f90:601-621), whole arrays
 __global__ void k_axpy_dt(double *a, const double *s, double dt, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -385,6 +399,14 @@ int dcm_advance_momentum(dcm_ctx *c, double dt, double *u, double *v, double *w,
     return advance_momentum_dev(c, dt, u, v, w, zu, zv, zw, su, sv, sw, p);
 }
 
+int dcm_divergence(dcm_ctx *c, const double *u, const double *v, const double *w, double *div) {
+    if (!c || !u || !v || !w || !div) return DCM_BAD_ARG;
+    Interior L = interior_launch(c->geo, 0);
+    hipLaunchKernelGGL(k_divergence, L.grid, L.block, 0, c->stream, u, v, w, div, c->geo, c->met);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
 int dcm_advance_scalars(dcm_ctx *c, double dt, double *q, const double *sq, double *th,
                         const double *sth, int nq, int moisture) {
     if (!c || !th || !sth) return DCM_BAD_ARG;
@@ -470,8 +492,8 @@ int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_
 int dcm_model_destroy(dcm_model *m) {
     if (!m) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(m->c->stream);
-    for (void *p : m->allocs) hipFree(p);
-    if (m->shadow) hipFree(m->shadow);
+    for (void *p : m->allocs) (void)hipFree(p);
+    if (m->shadow) (void)hipFree(m->shadow);
     delete m;
     return DCM_OK;
 }

@@ -75,6 +75,7 @@ _SIGS = {
     "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
     "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
     "dcm_advance_momentum": ([_vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "dcm_divergence": ([_vp, _vp, _vp, _vp, _vp], _i),
     "dcm_advance_scalars": ([_vp, _d, _vp, _vp, _vp, _vp, _i, _i], _i),
     "dcm_model_create": ([_vp, _i, _i, _ip, _ip, _d, _i, _i, _i, _d, _dp, _dp, _dp, _dp, _dp,
                           C.POINTER(_vp)], _i),
@@ -283,6 +284,9 @@ class Context:
         ms = _d()
         _call("dcm_stream_copy", self.h, dst.ptr, src.ptr, n, C.byref(ms))
         return ms.value
+
+    def divergence(self, u, v, w, div):
+        _call("dcm_divergence", self.h, u.ptr, v.ptr, w.ptr, div.ptr)
 
     # -- a13, a14 -------------------------------------------------------------
     def advance_momentum(self, dt, u, v, w, zu, zv, zw, su, sv, sw, p):

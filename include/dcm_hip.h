@@ -126,6 +126,11 @@ int dcm_hmeans(dcm_ctx *ctx, const double *u, const double *v,
                double *ubar, double *vbar, double *wbar, double *thbar,
                double *qbar);
 
+/* divergence_calc (diagnostics.f90:120-150): div = rhoan(dU/dx + dV/dy) +
+ * d(rhoa w)/dz on the interior; the halo of div is left as passed. */
+int dcm_divergence(dcm_ctx *ctx, const double *u, const double *v,
+                   const double *w, double *div);
+
 /* ---------------------------------------------------------------------- */
 /* a7. sources (dynamics.f90:639-1000), inviscid branch with damping,     */
 /* forcing and subsidence off.  theta/thbar are HOST arrays (kp+2).       */

@@ -814,6 +814,19 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     return status;
 }
 
+/* divergence_calc, diagnostics.f90:140-148 */
+void or_divergence(const or_grid *g, const double *u, const double *v,
+                   const double *w, double *div) {
+    for (int i = 1; i <= g->ip; ++i)
+        for (int j = 1; j <= g->jp; ++j)
+            for (int k = 1; k <= g->kp; ++k)
+                A3(div, k, j, i) =
+                    g->rhoan[k] * (A3(u, k, j, i) - A3(u, k, j, i - 1)) / g->dx[i - 1] +
+                    g->rhoan[k] * (A3(v, k, j, i) - A3(v, k, j - 1, i)) / g->dy[j - 1] +
+                    (g->rhoa[k] * A3(w, k, j, i) - g->rhoa[k - 1] * A3(w, k - 1, j, i)) /
+                        g->dz[k - 1];
+}
+
 /* advance_momentum, dynamics.f90:1019-1099 (dt is the leapfrog 2*dt) */
 void or_advance_momentum(const or_grid *g, double dt, double *u, double *v,
                          double *w, const double *zu, const double *zv,

@@ -83,6 +83,10 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
                 double tol, int test_solver, int first_call, int itmax,
                 int *iters, double *err);
 
+/* divergence_calc, diagnostics.f90:120-150 (interior; halo of div kept) */
+void or_divergence(const or_grid *g, const double *u, const double *v,
+                   const double *w, double *div);
+
 void or_advance_momentum(const or_grid *g, double dt, double *u, double *v,
                          double *w, const double *zu, const double *zv,
                          const double *zw, const double *su, const double *sv,

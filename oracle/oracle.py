@@ -88,6 +88,7 @@ def lib():
                                   C.c_int, _ip, _dp]
         L.or_bicgstab.restype = C.c_int
         L.or_advance_momentum.argtypes = [g, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.or_divergence.argtypes = [g, _dp, _dp, _dp, _dp]
         L.or_advance_scalars.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
         m = C.POINTER(_Model)
         L.or_model_init_bcs.argtypes = [g, m]
@@ -205,6 +206,10 @@ def bicgstab(g: Grid, sw, x, b, tol=1e-10, test_solver=False, first_call=True,
                            ptr(b), tol, int(test_solver), int(first_call), itmax,
                            C.byref(it), C.byref(err))
     return st, it.value, err.value
+
+
+def divergence(g: Grid, u, v, w, div):
+    lib().or_divergence(g.ref, ptr(u), ptr(v), ptr(w), ptr(div))
 
 
 def advance_momentum(g: Grid, dt, u, v, w, zu, zv, zw, su, sv, sw, p):

@@ -288,6 +288,17 @@ def test_advance_momentum_bitwise():
             np.testing.assert_array_equal(d[k].get(), arrs[k])
 
 
+def test_divergence_bitwise():
+    dom = small_domain(12, 10, 8, stretched=True)
+    g = O.Grid(dom)
+    u, v, w, div = (field(dom, 170 + n) for n in range(4))
+    with _ctx(dom) as c:
+        d = [c.array(a) for a in (u, v, w, div)]
+        c.divergence(*d)
+        O.divergence(g, u, v, w, div)
+        np.testing.assert_array_equal(d[3].get(), div)
+
+
 def test_advance_scalars_bitwise():
     dom = small_domain(8, 6, 5)
     g = O.Grid(dom)

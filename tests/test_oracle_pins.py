@@ -621,6 +621,29 @@ def test_bicgstab_true_residual_and_failure_codes():
 # ---------------------------------------------------------------------------
 # a13-a15 momentum update, scalars, the step
 # ---------------------------------------------------------------------------
+def test_divergence_of_linear_mass_flux():
+    """u growing by c1*dx per column, v by c2*dy, rhoa*w by c3*dz per level:
+    the divergence is rhoan*(c1+c2)+c3 everywhere, on a stretched grid; the
+    halo of div is untouched."""
+    dom = small_domain(7, 6, 5, stretched=True)
+    g = O.Grid(dom)
+    c1, c2, c3 = 0.3, -0.7, 0.11
+    xs = np.concatenate([[0.0], np.cumsum(dom.dx[:-1])])
+    ys = np.concatenate([[0.0], np.cumsum(dom.dy[:-1])])
+    zs = np.concatenate([[0.0], np.cumsum(dom.dz[:-1])])
+    u = np.broadcast_to(c1 * xs[:, None, None], dom.shape).copy()
+    v = np.broadcast_to(c2 * ys[None, :, None], dom.shape).copy()
+    w = np.broadcast_to((c3 * zs / dom.rhoa)[None, None, :], dom.shape).copy()
+    div = np.full(dom.shape, 9.0)
+    O.divergence(g, u, v, w, div)
+    exp = dom.rhoan[1:-1] * (c1 + c2) + c3
+    np.testing.assert_allclose(div[1:-1, 1:-1, 1:-1], np.broadcast_to(exp, (dom.ip, dom.jp, dom.kp)),
+                               rtol=1e-12, atol=1e-12)
+    halo = np.ones(dom.shape, bool)
+    halo[1:-1, 1:-1, 1:-1] = False
+    assert np.all(div[halo] == 9.0)
+
+
 def test_advance_momentum_constant_pressure_and_zero_sources():
     dom = small_domain(6, 5, 4)
     g = O.Grid(dom)

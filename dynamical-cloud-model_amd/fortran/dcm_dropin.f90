@@ -315,12 +315,12 @@ contains
         db = dcm_put(ctx, 15, c_loc(b), nb)
         st = dcm_bicgstab(ctx, xg, yg, zg, dsw, dx3, db, tol, to_cint(test_solver), 0_c_int, &
             iters, err)
-        ! the reference prints and carries on when itmax is reached (:432-435)
+        ! the reference prints and stops when itmax is reached (:431-434)
         if (st == DCM_NOT_CONVERGED) then
-            print *, 'bicgstab: no convergence after ', iters, ' iterations, error ', err
-        else
-            call dcm_check(st, 'bicgstab')
+            print *, 'convergence failure with error,it ', err, iters
+            stop
         endif
+        call dcm_check(st, 'bicgstab')
         call dcm_take(ctx, 14, c_loc(x), nb)
         call dcm_take(ctx, 15, c_loc(b), nb)
     end subroutine bicgstab

@@ -164,11 +164,25 @@ def _interior(a: np.ndarray) -> np.ndarray:
     return a[1:-1, 1:-1, 1:-1]
 
 
+def _box(centres: np.ndarray, c: float, radius: float) -> slice:
+    """Index range of the cells whose centre is within radius of c along one
+    axis (with a relative margin, so every cell the exact test accepts is
+    inside); cells outside the range are outside the sphere."""
+    idx = np.nonzero(np.abs(centres - c) <= radius * (1.0 + 1e-9))[0]
+    if idx.size == 0:
+        return slice(0, 0)
+    return slice(int(idx[0]), int(idx[-1]) + 1)
+
+
 def _bubble(dom: Domain, amp: float, radius: float, x0: float, y0: float,
             z0: float) -> np.ndarray:
-    X, Y, Z = np.meshgrid(dom.xn, dom.yn, dom.zn, indexing="ij")
+    """amp cos^2(pi r / 2R) for r <= R, else 0, on every cell (halos too)."""
+    out = np.zeros(dom.shape)
+    bx, by, bz = _box(dom.xn, x0, radius), _box(dom.yn, y0, radius), _box(dom.zn, z0, radius)
+    X, Y, Z = np.meshgrid(dom.xn[bx], dom.yn[by], dom.zn[bz], indexing="ij")
     r = np.sqrt((X - x0) ** 2 + (Y - y0) ** 2 + (Z - z0) ** 2)
-    return np.where(r <= radius, amp * np.cos(math.pi * r / (2.0 * radius)) ** 2, 0.0)
+    out[bx, by, bz] = np.where(r <= radius, amp * np.cos(math.pi * r / (2.0 * radius)) ** 2, 0.0)
+    return out
 
 
 def _noise(dom: Domain, seed: int, field: int, amp: float) -> np.ndarray:
@@ -180,7 +194,8 @@ def _noise(dom: Domain, seed: int, field: int, amp: float) -> np.ndarray:
 
 def _blobs(dom: Domain, seed: int, field: int, nblob: int, rmin: float,
            rmax: float) -> np.ndarray:
-    X, Y, Z = np.meshgrid(dom.xn, dom.yn, dom.zn, indexing="ij")
+    """nblob seeded spheres of a seeded value (later blobs overwrite earlier
+    ones), zero on the halo."""
     out = np.zeros(dom.shape)
     ctr = np.arange(5 * nblob, dtype=np.uint64) + np.uint64(1 << 40)
     r = uniform(seed, field, ctr).reshape(nblob, 5)
@@ -193,8 +208,11 @@ def _blobs(dom: Domain, seed: int, field: int, nblob: int, rmin: float,
         cz = zhi * r[b, 2]
         rad = rmin + (rmax - rmin) * r[b, 3]
         val = r[b, 4]
+        bx, by, bz = _box(dom.xn, cx, rad), _box(dom.yn, cy, rad), _box(dom.zn, cz, rad)
+        X, Y, Z = np.meshgrid(dom.xn[bx], dom.yn[by], dom.zn[bz], indexing="ij")
         inside = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= rad * rad
-        out = np.where(inside, val, out)
+        sub = out[bx, by, bz]
+        out[bx, by, bz] = np.where(inside, val, sub)
     out[0, :, :] = 0.0
     out[-1, :, :] = 0.0
     out[:, 0, :] = 0.0

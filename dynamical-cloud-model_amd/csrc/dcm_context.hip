@@ -9,6 +9,7 @@
 #include "dcm_internal.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace dcm {
@@ -291,6 +292,10 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
         set_error("device initialisation failed");
         return DCM_HIP_ERROR;
     }
+    // DCM_SUM_ORDER=sequential selects the serial summation order for callers
+    // that cannot reach dcm_set_sum_order (the Fortran drop-in modules)
+    const char *order = std::getenv("DCM_SUM_ORDER");
+    c->seq_sums = order && std::strcmp(order, "sequential") == 0;
     *out = c;
     return DCM_OK;
 }

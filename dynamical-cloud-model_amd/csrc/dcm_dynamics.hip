@@ -123,8 +123,28 @@ __global__ void k_level_means(const double *partials, int nchunks, Geo g, double
     out[k] = s / (double)(g.ip * g.jp);
 }
 
+// per-level means in the reference's loop order (the SUM intrinsic over
+// (k, 1:jpp, 1:ipp), diagnostics.f90:55-60: j inner, i outer), one thread
+// per level
+__global__ void k_level_means_seq(const double *a, Geo g, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.nk) return;
+    double s = 0.0;
+    for (int i = 1; i <= g.ip; ++i)
+        for (int j = 1; j <= g.jp; ++j) s = s + a[pidx(g, k, j, i)];
+    out[k] = s / (double)(g.ip * g.jp);
+}
+
 int level_means(dcm_ctx *c, const double *a, double *out_dev) {
     const Geo &g = c->geo;
+    if (c->seq_sums) {
+        int tok = prof_begin(c, "hmeans");
+        hipLaunchKernelGGL(k_level_means_seq, dim3((g.nk + 63) / 64), dim3(64), 0, c->stream, a,
+                           g, out_dev);
+        DCM_LAUNCH_CHECK();
+        prof_end(c, tok);
+        return DCM_OK;
+    }
     const long long ncol = (long long)g.ip * g.jp;
     const int nchunks = (int)((ncol + kHmCols - 1) / kHmCols);
     const size_t need = (size_t)nchunks * g.nk;
@@ -630,6 +650,7 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
         stats->iters_total += so.iters;
         if (so.iters > stats->iters_max) stats->iters_max = so.iters;
         stats->outer_total += so.outer;
+        stats->fallbacks += so.fallback;
         stats->last_err = so.err;
         if (true_resid > stats->max_true_resid) stats->max_true_resid = true_resid;
     }

@@ -80,6 +80,10 @@ struct SolverScalars {
     int done;          // converged (device-side early exit for chunks)
     int nonfinite;
     int iters;         // iterations performed (counted on the device)
+    int guard;         // inner solve: a non-finite or zero-denominator scalar
+                       // ends the iteration as a breakdown, before the update
+    int best_it;       // inner solve: iteration of the best residual so far
+    double best_err;
 };
 
 // Per-kernel timing (profiling mode): accumulated over launches.
@@ -97,6 +101,7 @@ struct Mirror {
 struct SolverOut {
     int iters = 0;   // BiCGSTAB iterations (inner iterations for mixed)
     int outer = 0;   // refinement sweeps (mixed)
+    int fallback = 0; // mixed: the FP64 solver finished the solve
     double err = 0.0;
 };
 
@@ -131,6 +136,9 @@ struct dcm_ctx {
     double *d1d = nullptr;           // scratch for 1-D host arrays (theta...)
     size_t d1d_cap = 0;
     bool first_solve = true;         // bicgstab_first (dynamics.f90:16)
+    // global sums in the reference's sequential loop order (one thread per
+    // sum) instead of the two-level tree: bit-identical to a serial CPU run
+    bool seq_sums = false;
     std::unordered_map<const void *, dcm::Mirror> mirrors;
     // profiling
     bool profiling = false;

@@ -218,10 +218,10 @@ __global__ __launch_bounds__(256) void k_resid0(const double *__restrict__ x,
     block_sum_store<2>(s, partials, stride);
 }
 
-// FP64 true residual with an FP32 copy for the inner solve:
-// r32 = float(b - A x); partial sums of <r,r>, sum(r), <b,b>
+// FP64 true residual r = b - A x (kept, compact, when r64 is not null) and
+// the partial sums of <r,r>, sum(r), <b,b>
 __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x,
-                                                    const double *__restrict__ b, float *r32,
+                                                    const double *__restrict__ b, double *r64,
                                                     Geo pg, CGeo g, CoefPack cp,
                                                     double *partials, long long stride) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -238,12 +238,65 @@ __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x
                             x[pidx(pg, n.kd, j, i)], x[pidx(pg, n.ku, j, i)], x[pidx(pg, k, j, i)]);
         const double bb = b[pidx(pg, k, j, i)];
         const double rr = bb - ax;
-        if (r32) r32[cidx(g, k, j, i)] = (float)rr;
+        if (r64) r64[cidx(g, k, j, i)] = rr;
         s[0] = rr * rr;
         s[1] = rr;
         s[2] = bb * bb;
     }
     block_sum_store<3>(s, partials, stride);
+}
+
+// ---------------------------------------------------------------------------
+// sums in the reference's loop order (inn_prod, dynamics.f90:470-476: i
+// outer, j, k inner, one running sum).  One thread; used when the context
+// asks for sequential sums, so that a run is bit-identical to a serial CPU
+// run of the same arithmetic.  out[0] = <a0,b0>, out[1] = <a1,b1> (a1 may be
+// null).
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_seq_dots(const R *__restrict__ a0, const R *__restrict__ b0,
+                           const R *__restrict__ a1, const R *__restrict__ b1, long long n,
+                           double *out, const SolverScalars *__restrict__ sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc && (sc->done || sc->breakdown)) return;
+    double s0 = 0.0, s1 = 0.0;
+    for (long long q = 0; q < n; ++q) {
+        s0 = s0 + (double)a0[q] * (double)b0[q];
+        if (a1) s1 = s1 + (double)a1[q] * (double)b1[q];
+    }
+    out[0] = s0;
+    out[1] = s1;
+}
+
+// <b,b> over the interior of a padded b and <r,r> of the compact r
+__global__ void k_seq_resid0(const double *__restrict__ b, const double *__restrict__ r, Geo pg,
+                             CGeo g, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double bb = 0.0, rr = 0.0;
+    long long q = 0;
+    for (int i = 1; i <= g.ip; ++i)
+        for (int j = 1; j <= g.jp; ++j)
+            for (int k = 1; k <= g.kp; ++k, ++q) {
+                const double bv = b[pidx(pg, k, j, i)];
+                bb = bb + bv * bv;
+                rr = rr + r[q] * r[q];
+            }
+    out[0] = bb;
+    out[1] = rr;
+}
+
+// <x,y> over the interior of padded arrays
+__global__ void k_seq_dot_padded(const double *__restrict__ x, const double *__restrict__ y,
+                                 Geo g, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 1; i <= g.ip; ++i)
+        for (int j = 1; j <= g.jp; ++j)
+            for (int k = 1; k <= g.kp; ++k) {
+                const long long c = pidx(g, k, j, i);
+                s = s + x[c] * y[c];
+            }
+    out[0] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -258,8 +311,16 @@ __device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__
     return r[q] - a * v1[q];
 }
 
-// sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p, s0 = t/a_p evaluated at
-// the centre and the six neighbours (dynamics.f90:260-266)
+// sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p with s0 = t/a_p
+// (dynamics.f90:254-266).  s0 is computed once per cell and plane and shared
+// through LDS: each plane of the tile plus a one-cell halo in j and k (the
+// 7-point stencil needs no corners) lives in one of three LDS buffers, so one
+// barrier per plane separates a buffer's writes from its reads.  Every
+// thread takes part (threads past the edge of the grid evaluate the wrapped
+// j / clamped k cell, which is valid memory), only active ones store.  The
+// raw loads of plane i+2 are issued while plane i is computed.
+constexpr int kLdsCells = 396;  // max (jt+2)*(kt+2) over the tile shapes: 6*66
+
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
                                                    const R *__restrict__ v1,
@@ -268,44 +329,114 @@ __global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
                                                    const SolverScalars *__restrict__ sc) {
     if (sc->done || sc->breakdown) return;
     Tile T;
-    if (!march_tile(m, T) || !T.act) return;
+    if (!march_tile(m, T)) return;
+    __shared__ R lds[3 * kLdsCells];
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
+    const int tid = (int)threadIdx.x;
+    const int kl = tid % m.kt, jl = tid / m.kt;
     const int k = T.k, j = T.j;
-    const Cols q = cols(m, k, j);
-    const int js = j == 1 ? m.jp : j - 1, jn = j == m.jp ? 1 : j + 1;
-    const int kd = k == 1 ? 1 : k - 1, ku = k == m.kp ? m.kp : k + 1;
-    const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+    const int k0 = k - kl, j0 = j - jl;
+    const int kc = k > m.kp ? m.kp : k;
+    const int jw = (j - 1) % m.jp + 1;
+    const int col = (jw - 1) * m.sj + (kc - 1);
+    const int W = m.kt + 2;  // LDS row length
+    // halo cell of this thread, if any: rows j0-1, j0+jt (one per k lane),
+    // then columns k0-1, k0+kt (one per j row)
+    bool halo = false;
+    int hk = 1, hj = 1, hslot = 0;
+    if (tid < 2 * m.kt) {
+        const int side = tid / m.kt, kk = tid % m.kt;
+        int jj = side ? j0 + m.jt : j0 - 1;
+        jj = ((jj - 1) % m.jp + m.jp) % m.jp + 1;
+        hk = k0 + kk > m.kp ? m.kp : k0 + kk;
+        hj = jj;
+        hslot = (side ? m.jt + 1 : 0) * W + kk + 1;
+        halo = true;
+    } else if (tid < 2 * m.kt + 2 * m.jt) {
+        const int u = tid - 2 * m.kt, side = u / m.jt, jj0 = u % m.jt;
+        int kk = side ? k0 + m.kt : k0 - 1;
+        kk = kk < 1 ? 1 : (kk > m.kp ? m.kp : kk);
+        hk = kk;
+        hj = (j0 + jj0 - 1) % m.jp + 1;
+        hslot = (jj0 + 1) * W + (side ? m.kt + 1 : 0);
+        halo = true;
+    }
+    const int hcol = (hj - 1) * m.sj + (hk - 1);
+    const int oslot = (jl + 1) * W + kl + 1;
+    const R as = C.as[jw], an = C.an[jw], ad = C.ad[kc], au = C.au[kc];
+
+    auto tv = [&](R rv, R vv) -> R {
+        if (MODE == 0) return rv - a * (bo * vv);
+        return rv - a * vv;
+    };
+    auto nexti = [&](int i) { return i == m.ip ? 1 : i + 1; };
+
+    // plane i0-1 (own cell) and plane i0 (own cell and halo)
     const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
-    int cw = (iw - 1) * m.si + q.base;
-    int cc = (T.i0 - 1) * m.si + q.base;
-    R s0w = tval<R, MODE>(r, v1, cw, a, bo) / coef_ap(C, k, j, iw);
-    R tc = tval<R, MODE>(r, v1, cc, a, bo);
-    R s0c = tc / coef_ap(C, k, j, T.i0);
+    R api = (-C.ae[iw]) - C.aw[iw];
+    int q = (iw - 1) * m.si;
+    R s0w = tv(r[q + col], v1[q + col]) / coef_ap_i(C, api, kc, jw);
+    api = (-C.ae[T.i0]) - C.aw[T.i0];
+    q = (T.i0 - 1) * m.si;
+    R tc = tv(r[q + col], v1[q + col]);
+    R s0c = tc / coef_ap_i(C, api, kc, jw);
+    lds[(T.i0 % 3) * kLdsCells + oslot] = s0c;
+    if (halo)
+        lds[(T.i0 % 3) * kLdsCells + hslot] =
+            tv(r[q + hcol], v1[q + hcol]) / coef_ap_i(C, api, hk, hj);
+    // raw loads of plane i0+1
+    int ie = nexti(T.i0);
+    q = (ie - 1) * m.si;
+    R r_o = r[q + col], v_o = v1[q + col];
+    R r_h = halo ? r[q + hcol] : R(0), v_h = halo ? v1[q + hcol] : R(0);
     for (int i = T.i0; i <= T.i1; ++i) {
-        const int ie = i == m.ip ? 1 : i + 1;
-        const int ce = (ie - 1) * m.si + q.base;
-        const R te = tval<R, MODE>(r, v1, ce, a, bo);
-        const R s0e = te / coef_ap(C, k, j, ie);
-        const R api = (-C.ae[i]) - C.aw[i];
-        const R ap = coef_ap_i(C, api, k, j);
-        const R s0s = tval<R, MODE>(r, v1, cc + q.ds, a, bo) / coef_ap_i(C, api, k, js);
-        const R s0n = tval<R, MODE>(r, v1, cc + q.dn, a, bo) / coef_ap_i(C, api, k, jn);
-        const R s0d = q.dd ? tval<R, MODE>(r, v1, cc - 1, a, bo) / coef_ap_i(C, api, kd, j) : s0c;
-        const R s0u = q.du ? tval<R, MODE>(r, v1, cc + 1, a, bo) / coef_ap_i(C, api, ku, j) : s0c;
-        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n, s0d,
-                                s0u, s0c);
-        t_out[cc] = tc;
-        s_out[cc] = s0c + (tc - As) / ap;
+        // plane i+1 from the loads in flight
+        const R api_e = (-C.ae[ie]) - C.aw[ie];
+        const R te = tv(r_o, v_o);
+        const R s0e = te / coef_ap_i(C, api_e, kc, jw);
+        R hval = R(0);
+        if (halo && i < T.i1) hval = tv(r_h, v_h) / coef_ap_i(C, api_e, hk, hj);
+        // loads of plane i+2
+        const int iee = nexti(ie);
+        if (i < T.i1) {
+            const int qn = (iee - 1) * m.si;
+            r_o = r[qn + col];
+            v_o = v1[qn + col];
+            if (halo) {
+                r_h = r[qn + hcol];
+                v_h = v1[qn + hcol];
+            }
+        }
+        if (i < T.i1) {
+            const int b = ((i + 1) % 3) * kLdsCells;
+            lds[b + oslot] = s0e;
+            if (halo) lds[b + hslot] = hval;
+        }
+        __syncthreads();
+        const int b0 = (i % 3) * kLdsCells;
+        const R s0s = lds[b0 + oslot - W], s0n = lds[b0 + oslot + W];
+        const R s0d = k == 1 ? s0c : lds[b0 + oslot - 1];
+        const R s0u = k >= m.kp ? s0c : lds[b0 + oslot + 1];
+        if (T.act) {
+            const R ap = coef_ap_i(C, api, k, j);
+            const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n,
+                                    s0d, s0u, s0c);
+            const int cc = (i - 1) * m.si + col;
+            t_out[cc] = tc;
+            s_out[cc] = s0c + (tc - As) / ap;
+        }
         s0w = s0c;
         s0c = s0e;
         tc = te;
-        cc = ce;
+        api = api_e;
+        ie = iee;
     }
 }
 
-// s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead
+// s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead.  The
+// loads of plane i+1 are issued while plane i is computed.
 template <typename R, bool P_UPDATE>
 __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
                                              R *__restrict__ s_out, MG m, CoefPack cp,
@@ -320,27 +451,51 @@ __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *_
     const R bet = P_UPDATE ? (R)sc->bet : R(0);
     const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
     int cc = (T.i0 - 1) * m.si + q.base;
+    int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
+    int ce = (ie - 1) * m.si + q.base;
     R xw = s[(iw - 1) * m.si + q.base];
-    R xc = s[cc];
+    R xc = s[cc], xe = s[ce];
+    R xs = s[cc + q.ds], xn = s[cc + q.dn], xd = s[cc + q.dd], xu = s[cc + q.du];
+    R tc = t[cc];
+    R po = P_UPDATE ? s_out[cc] : R(0);
     for (int i = T.i0; i <= T.i1; ++i) {
-        const int ie = i == m.ip ? 1 : i + 1;
-        const int ce = (ie - 1) * m.si + q.base;
-        const R xe = s[ce];
+        R nxe = R(0), nxs = R(0), nxn = R(0), nxd = R(0), nxu = R(0), ntc = R(0), npo = R(0);
+        int nie = ie, nce = ce;
+        if (i < T.i1) {
+            nie = ie == m.ip ? 1 : ie + 1;
+            nce = (nie - 1) * m.si + q.base;
+            nxe = s[nce];
+            nxs = s[ce + q.ds];
+            nxn = s[ce + q.dn];
+            nxd = s[ce + q.dd];
+            nxu = s[ce + q.du];
+            ntc = t[ce];
+            if (P_UPDATE) npo = s_out[ce];
+        }
         const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
-        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, s[cc + q.ds],
-                                s[cc + q.dn], s[cc + q.dd], s[cc + q.du], xc);
-        const R snew = xc + (t[cc] - As) / ap;
+        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu, xc);
+        const R snew = xc + (tc - As) / ap;
         if (P_UPDATE)
-            s_out[cc] = snew + bet * s_out[cc];  // p = s + bet*p (dynamics.f90:399)
+            s_out[cc] = snew + bet * po;  // p = s + bet*p (dynamics.f90:399)
         else
             s_out[cc] = snew;
         xw = xc;
         xc = xe;
+        xe = nxe;
+        xs = nxs;
+        xn = nxn;
+        xd = nxd;
+        xu = nxu;
+        tc = ntc;
+        po = npo;
         cc = ce;
+        ce = nce;
+        ie = nie;
     }
 }
 
-// y = A x; partial sums of <a,y> (and <y,y> first when TWO)
+// y = A x; partial sums of <a,y> (and <y,y> first when TWO).  Loads of plane
+// i+1 in flight while plane i is computed.
 template <typename R, bool TWO>
 __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
                                                     const R *__restrict__ a, R *__restrict__ y,
@@ -358,25 +513,46 @@ __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
         const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
         const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
         int cc = (T.i0 - 1) * m.si + q.base;
+        int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
+        int ce = (ie - 1) * m.si + q.base;
         R xw = x[(iw - 1) * m.si + q.base];
-        R xc = x[cc];
+        R xc = x[cc], xe = x[ce];
+        R xs = x[cc + q.ds], xn = x[cc + q.dn], xd = x[cc + q.dd], xu = x[cc + q.du];
+        R ac = a[cc];
         for (int i = T.i0; i <= T.i1; ++i) {
-            const int ie = i == m.ip ? 1 : i + 1;
-            const int ce = (ie - 1) * m.si + q.base;
-            const R xe = x[ce];
+            R nxe = R(0), nxs = R(0), nxn = R(0), nxd = R(0), nxu = R(0), nac = R(0);
+            int nie = ie, nce = ce;
+            if (i < T.i1) {
+                nie = ie == m.ip ? 1 : ie + 1;
+                nce = (nie - 1) * m.si + q.base;
+                nxe = x[nce];
+                nxs = x[ce + q.ds];
+                nxn = x[ce + q.dn];
+                nxd = x[ce + q.dd];
+                nxu = x[ce + q.du];
+                nac = a[ce];
+            }
             const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
-            const R yv = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, x[cc + q.ds],
-                                    x[cc + q.dn], x[cc + q.dd], x[cc + q.du], xc);
+            const R yv = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu,
+                                    xc);
             y[cc] = yv;
             if (TWO) {
-                acc[0] += (double)yv * (double)yv;     // tt = <t,t>
-                acc[1] += (double)yv * (double)a[cc];  // ts = <t,s>
+                acc[0] += (double)yv * (double)yv;  // tt = <t,t>
+                acc[1] += (double)yv * (double)ac;  // ts = <t,s>
             } else {
-                acc[0] += (double)a[cc] * (double)yv;  // nrm = <cr,v1>
+                acc[0] += (double)ac * (double)yv;  // nrm = <cr,v1>
             }
             xw = xc;
             xc = xe;
+            xe = nxe;
+            xs = nxs;
+            xn = nxn;
+            xd = nxd;
+            xu = nxu;
+            ac = nac;
             cc = ce;
+            ce = nce;
+            ie = nie;
         }
     }
     block_sum_store_at<2>(acc, partials, stride, T.tile);
@@ -410,6 +586,8 @@ __global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__re
 // ---------------------------------------------------------------------------
 // tiny = 1.e-16 is a default-real literal (dynamics.f90:321)
 __device__ constexpr double kTinyOmega = (double)1.e-16f;
+// inner (FP32) solves: iterations without a 10% gain that count as stagnation
+constexpr int kStall = 40;
 
 __global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, double tol,
                            int use_bb) {
@@ -429,6 +607,7 @@ __global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, do
         sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
         sc->breakdown = 0;
         sc->iters = 0;
+        sc->guard = 0;
         sc->nonfinite = !isfinite(sc->err);
         sc->done = (sc->err < tol) || sc->nonfinite;
     }
@@ -440,6 +619,7 @@ __global__ void k_fin_v(const double *partials, int nb, SolverScalars *sc) {
     if (threadIdx.x == 0) {
         sc->nrm = nrm;
         sc->alf = sc->rho / nrm;  // :404
+        if (sc->guard && !(nrm != 0.0 && isfinite(sc->alf))) sc->breakdown = 1;
     }
 }
 
@@ -453,6 +633,7 @@ __global__ void k_fin_t(const double *partials, int nb, SolverScalars *sc) {
         sc->ts = ts;
         sc->omg = ts / tt;  // :412
         if (fabs(sc->omg) < kTinyOmega) sc->breakdown = 1;  // :414
+        if (sc->guard && !(tt != 0.0 && isfinite(sc->omg))) sc->breakdown = 1;
     }
 }
 
@@ -468,6 +649,17 @@ __global__ void k_fin_r(const double *partials, int nb, SolverScalars *sc, doubl
         sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
         sc->nonfinite = !isfinite(sc->err);
         sc->done = (sc->err < tol) || sc->nonfinite;
+        if (sc->guard) {
+            // an inner solve that has not improved its best residual by 10%
+            // for kStall iterations has stagnated at its precision: stop and
+            // let the FP64 refinement restart it from the true residual
+            if (sc->err < 0.9 * sc->best_err) {
+                sc->best_err = sc->err;
+                sc->best_it = sc->iters;
+            } else if (sc->iters - sc->best_it >= kStall) {
+                sc->done = 1;
+            }
+        }
     }
 }
 
@@ -486,6 +678,9 @@ __global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *
         sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
         sc->breakdown = 0;
         sc->iters = 0;
+        sc->guard = 1;
+        sc->best_err = sc->err;
+        sc->best_it = 0;
         sc->nonfinite = !isfinite(rr);
         sc->done = (rr == 0.0) || (sc->err < rtol) || sc->nonfinite;
     }
@@ -540,8 +735,9 @@ __global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g) {
     x[pidx(pg, k, j, i)] += (double)d[cidx(g, k, j, i)];
 }
 
-// inner start: r = r32 - mean, cr = r, p = v1 = d = 0; <r,r>
-__global__ void k_inner_init(const float *r32, float *r, float *cr, float *p, float *v1,
+// inner start: r = float(r64 - mean) (the mean removed in FP64, so that a
+// large mean does not swamp the rest in FP32), cr = r, p = v1 = d = 0; <r,r>
+__global__ void k_inner_init(const double *r64, float *r, float *cr, float *p, float *v1,
                              float *d, CGeo g, const double *mean_sum, double *partials,
                              long long stride) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -550,8 +746,8 @@ __global__ void k_inner_init(const float *r32, float *r, float *cr, float *p, fl
     double s[2] = {0.0, 0.0};
     if (k <= g.kp && j <= g.jp) {
         const long long c = cidx(g, k, j, i);
-        const float mean = (float)(*mean_sum / (double)g.n);
-        const float rv = r32[c] - mean;
+        const double mean = *mean_sum / (double)g.n;
+        const float rv = (float)(r64[c] - mean);
         r[c] = rv;
         cr[c] = rv;
         p[c] = 0.f;
@@ -638,6 +834,8 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
     SolverScalars *sc = c->solver;
     double *part = c->partials;
     const int n = (int)g.n;
+    const bool seq = c->seq_sums;
+    const int nbv = seq ? 1 : nbm, nbu = seq ? 1 : kUpdateBlocks;
     if (itmax < 1) itmax = 1;
     int launched = 0;
     while (launched < itmax) {
@@ -659,7 +857,10 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
             hipLaunchKernelGGL((k_matvec_dot<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, p,
                                cr, v1, m, cp, sc, part, nbm);
             prof_end(c, tok);
-            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbm, sc);
+            if (seq)
+                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, cr, v1,
+                                   (const R *)nullptr, (const R *)nullptr, g.n, part, sc);
+            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
             tok = prof_begin(c, "jac_first");
             hipLaunchKernelGGL((k_jac_first<R, 1>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
                                s, wa, m, cp, sc);
@@ -676,13 +877,18 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
             hipLaunchKernelGGL((k_matvec_dot<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, cs,
                                s, t, m, cp, sc, part, nbm);
             prof_end(c, tok);
-            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nbm, sc);
+            if (seq)
+                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, t, t, t, s,
+                                   g.n, part, sc);
+            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
             tok = prof_begin(c, "update");
             hipLaunchKernelGGL(k_update<R>, dim3(kUpdateBlocks), dim3(kBlock), 0, c->stream, x, p,
                                cs, s, t, cr, r, n, sc, part, kUpdateBlocks);
             prof_end(c, tok);
-            hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(1024), 0, c->stream, part, kUpdateBlocks,
-                               sc, tol);
+            if (seq)
+                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, r, r, r, cr,
+                                   g.n, part, sc);
+            hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(1024), 0, c->stream, part, nbu, sc, tol);
         }
         DCM_LAUNCH_CHECK();
         launched += chunk;
@@ -724,8 +930,11 @@ int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, i
     hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, c->stream, x, b, V[0], V[1], V[2], V[3], V[9],
                        pg, g, cp, c->partials, (long long)nb);
     prof_end(c, tok);
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb, c->solver,
-                       tol, 1);
+    if (c->seq_sums)
+        hipLaunchKernelGGL(k_seq_resid0, dim3(1), dim3(64), 0, c->stream, b, V[0], pg, g,
+                           c->partials);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                       c->seq_sums ? 1 : nb, c->solver, tol, 1);
     DCM_LAUNCH_CHECK();
     DCM_HIP(hipMemcpyAsync(c->h_solver, c->solver, sizeof(SolverScalars), hipMemcpyDeviceToHost,
                            c->stream));
@@ -769,19 +978,24 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
     if (need < 4 * (size_t)m.nblocks + 64) need = 4 * (size_t)m.nblocks + 64;
     if (need < 4 * (size_t)kUpdateBlocks) need = 4 * (size_t)kUpdateBlocks;
     DCM_CHECK(ensure_partials(c, need));
-    DCM_CHECK(ensure_solver_vectors32(c, 11));
-    float *V[11];
-    for (int q = 0; q < 11; ++q) V[q] = c->sv32 + (size_t)q * g.n;
-    float *r32 = V[10], *d = V[9];
+    DCM_CHECK(ensure_solver_vectors32(c, 10));
+    float *V[10];
+    for (int q = 0; q < 10; ++q) V[q] = c->sv32 + (size_t)q * g.n;
+    float *d = V[9];
+    double *r64;  // FP64 true residual, compact
+    DCM_CHECK(work_field64(c, 2, &r64));
     SolverScalars *sc = c->solver;
     out->iters = 0;
     out->outer = 0;
-    double sc_err = 0.0;
+    out->fallback = 0;
+    double sc_err = 0.0, err_prev = 0.0;
+    double *xsave = nullptr;  // x before the last correction
     const int max_outer = 50;
+    const int inner_cap = 2000;  // FP32 iterations per refinement sweep
     for (int outer = 0; outer <= max_outer; ++outer) {
         // true FP64 residual, its FP32 copy and its mean
         int tok = prof_begin(c, "resid_true");
-        hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, c->stream, x, b, r32, pg, g, cp,
+        hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, c->stream, x, b, r64, pg, g, cp,
                            c->partials, (long long)nb);
         prof_end(c, tok);
         hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
@@ -797,15 +1011,30 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
         }
         const double err = sqrt(rr) / sc_err;
         out->err = err;
-        if (!std::isfinite(err)) {
-            set_error("bicgstab_mixed: non-finite residual");
+        if (outer == 0 && !std::isfinite(err)) {
+            set_error("bicgstab_mixed: non-finite initial residual");
             return DCM_NONFINITE;
+        }
+        if (outer > 0 && !(err < err_prev)) {
+            // the last FP32 correction did not reduce the true residual (an
+            // inner breakdown, stagnation at FP32 resolution, or overflow):
+            // restore x and let the FP64 solver of the reference finish
+            DCM_HIP(hipMemcpyAsync(x, xsave, pg.ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                                   c->stream));
+            SolverOut o64;
+            const int left = itmax - out->iters > 1 ? itmax - out->iters : 1;
+            const int st = bicgstab64(c, sw, x, b, tol, left, &o64);
+            out->iters += o64.iters;
+            out->err = o64.err;
+            out->fallback = 1;
+            return st;
         }
         if (err < tol) break;
         if (outer == max_outer || out->iters >= itmax) return DCM_NOT_CONVERGED;
+        err_prev = err;
         out->outer += 1;
         // inner solve A d = r - mean(r), to a relative reduction inner_rtol
-        hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, c->stream, r32, V[0], V[1], V[2],
+        hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, c->stream, r64, V[0], V[1], V[2],
                            V[3], d, g, c->dscal + 41, c->partials, (long long)nb);
         DCM_LAUNCH_CHECK();
         // the inner recursive residual is relative to ||r_inner,0||; stop at
@@ -817,11 +1046,16 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
                            sc, rtol);
         DCM_LAUNCH_CHECK();
         SolverOut io;
-        int st = bicgstab_iterate<float>(c, V, rtol, itmax - out->iters, &io);
+        int cap = itmax - out->iters;
+        if (cap > inner_cap) cap = inner_cap;
+        (void)bicgstab_iterate<float>(c, V, rtol, cap, &io);
         out->iters += io.iters;
-        if (st == DCM_NONFINITE) return st;
-        // a breakdown or stagnation of the inner solve still returns its
-        // best correction; the outer loop judges the true residual
+        // whatever the inner status, its correction is judged by the true
+        // residual at the top of the loop; a guarded breakdown stops before
+        // the update that would use a non-finite scalar
+        if (!xsave) DCM_CHECK(work_field64(c, 1, &xsave));
+        DCM_HIP(hipMemcpyAsync(xsave, x, pg.ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
         hipLaunchKernelGGL(k_add_correction, L.grid, L.block, 0, c->stream, x, d, pg, g);
         DCM_LAUNCH_CHECK();
     }
@@ -873,9 +1107,16 @@ int dcm_inn_prod(dcm_ctx *c, const double *x, const double *y, double *out) {
     const Interior L = interior_launch(c->geo);
     const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
     DCM_CHECK(ensure_partials(c, (size_t)nb + 16));
-    hipLaunchKernelGGL(k_dot_padded, L.grid, L.block, 0, c->stream, x, y, c->geo, c->partials);
-    DCM_LAUNCH_CHECK();
-    DCM_CHECK(finalize_sums(c, c->partials, nb, 1, c->dscal + 40));
+    if (c->seq_sums) {
+        hipLaunchKernelGGL(k_seq_dot_padded, dim3(1), dim3(64), 0, c->stream, x, y, c->geo,
+                           c->dscal + 40);
+        DCM_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(k_dot_padded, L.grid, L.block, 0, c->stream, x, y, c->geo,
+                           c->partials);
+        DCM_LAUNCH_CHECK();
+        DCM_CHECK(finalize_sums(c, c->partials, nb, 1, c->dscal + 40));
+    }
     DCM_HIP(hipMemcpyAsync(c->hscal + 40, c->dscal + 40, sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
     DCM_HIP(hipStreamSynchronize(c->stream));
@@ -891,7 +1132,7 @@ int dcm_true_residual(dcm_ctx *c, const double *x, const double *b, double *out)
     const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
     CoefPack cp{c->met, c->met32};
     DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
-    hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, c->stream, x, b, (float *)nullptr, pg,
+    hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, c->stream, x, b, (double *)nullptr, pg,
                        g, cp, c->partials, (long long)nb);
     hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
                        c->dscal + 40);
@@ -938,6 +1179,12 @@ int dcm_bicgstab(dcm_ctx *c, const double *xg, const double *yg, const double *z
     if (iters) *iters = o.iters;
     if (err) *err = o.err;
     return st;
+}
+
+int dcm_set_sum_order(dcm_ctx *c, int order) {
+    if (!c || (order != DCM_SUM_TREE && order != DCM_SUM_SEQUENTIAL)) return DCM_BAD_ARG;
+    c->seq_sums = order == DCM_SUM_SEQUENTIAL;
+    return DCM_OK;
 }
 
 int dcm_bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,

@@ -20,6 +20,7 @@ STATUS = {0: "DCM_OK", 1: "DCM_NOT_CONVERGED", 2: "DCM_BREAKDOWN", 3: "DCM_NONFI
           4: "DCM_BAD_ARG", 5: "DCM_HIP_ERROR", 6: "DCM_NO_DEVICE", 7: "DCM_OUT_OF_MEMORY"}
 
 FP64, MIXED = 0, 1
+SUM_TREE, SUM_SEQUENTIAL = 0, 1
 (F_U, F_V, F_W, F_ZU, F_ZV, F_ZW, F_TH, F_P, F_PSRC, F_SU, F_SV, F_SW, F_STH, F_Q,
  F_THBAR) = range(15)
 
@@ -38,7 +39,7 @@ class DcmError(RuntimeError):
 
 class StepStats(C.Structure):
     _fields_ = [("steps", _i), ("iters_total", _i), ("iters_max", _i), ("outer_total", _i),
-                ("last_err", _d), ("max_true_resid", _d)]
+                ("fallbacks", _i), ("reserved", _i), ("last_err", _d), ("max_true_resid", _d)]
 
 
 _SIGS = {
@@ -71,6 +72,7 @@ _SIGS = {
     "dcm_matvec": ([_vp, _vp, _vp], _i),
     "dcm_precondition": ([_vp, _vp, _vp, _i], _i),
     "dcm_inn_prod": ([_vp, _vp, _vp, _dp], _i),
+    "dcm_set_sum_order": ([_vp, _i], _i),
     "dcm_bicgstab": ([_vp, _dp, _dp, _dp, _vp, _vp, _vp, _d, _i, _i, _ip, _dp], _i),
     "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
     "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
@@ -251,6 +253,10 @@ class Context:
 
     def precondition(self, s, r, prit=3):
         _call("dcm_precondition", self.h, s.ptr, r.ptr, prit)
+
+    def set_sum_order(self, order):
+        """SUM_TREE (default) or SUM_SEQUENTIAL (the reference's serial order)."""
+        _call("dcm_set_sum_order", self.h, order)
 
     def inn_prod(self, x, y) -> float:
         out = _d()

@@ -26,7 +26,7 @@ contains
         integer(dcm_i4b), intent(in) :: dims(3)
         if (dims(1) /= 1 .or. dims(2) /= 1 .or. dims(3) /= 1) then
             print *, 'dcm: the GPU path runs one rank per GPU (dims must be 1,1,1)'
-            stop
+            error stop 1
         endif
     end subroutine one_rank
 
@@ -268,6 +268,7 @@ end module advection_s_3d
 !> module d_solver (dynamics.f90:5-21)
 module d_solver
     use iso_c_binding
+    use, intrinsic :: ieee_arithmetic, only: ieee_is_finite
     use dcm_hip
     use dcm_dropin_util
     implicit none
@@ -319,6 +320,15 @@ contains
         if (st == DCM_NOT_CONVERGED) then
             print *, 'convergence failure with error,it ', err, iters
             stop
+        endif
+        if (st == DCM_NONFINITE) then
+            ! name the input that carries the non-finite values
+            call dcm_take(ctx, 14, c_loc(x), nb)
+            call dcm_take(ctx, 15, c_loc(b), nb)
+            print *, 'bicgstab: non-finite entries in the interior of x: ', &
+                count(.not. ieee_is_finite(x(1:kp, 1:jp, 1:ip))), ', of b: ', &
+                count(.not. ieee_is_finite(b(1:kp, 1:jp, 1:ip))), ', after ', iters, &
+                ' iterations'
         endif
         call dcm_check(st, 'bicgstab')
         call dcm_take(ctx, 14, c_loc(x), nb)
@@ -398,11 +408,12 @@ contains
         dv = dcm_put(ctx, 2, c_loc(v), nb)
         dw = dcm_put(ctx, 3, c_loc(w), nb)
         dth = dcm_put(ctx, 13, c_loc(th), nb)
-        dsu = dcm_slot(ctx, 8, nb)
-        dsv = dcm_slot(ctx, 9, nb)
-        dsw = dcm_slot(ctx, 10, nb)
+        ! inout: the cells sources does not write keep the caller's values
+        dsu = dcm_put(ctx, 8, c_loc(su), nb)
+        dsv = dcm_put(ctx, 9, c_loc(sv), nb)
+        dsw = dcm_put(ctx, 10, c_loc(sw), nb)
         dsth = dcm_put(ctx, 11, c_loc(sth), nb)
-        drhs = dcm_slot(ctx, 12, nb)
+        drhs = dcm_put(ctx, 12, c_loc(rhs), nb)
         call dcm_check(dcm_sources(ctx, du, dv, dw, dth, theta, thbar, dsu, dsv, dsw, dsth, &
             drhs), 'sources')
         call dcm_take(ctx, 8, c_loc(su), nb)

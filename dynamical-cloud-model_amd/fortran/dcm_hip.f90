@@ -12,13 +12,14 @@ module dcm_hip
     integer(c_int), parameter, public :: DCM_OK = 0, DCM_NOT_CONVERGED = 1, &
         DCM_BREAKDOWN = 2, DCM_NONFINITE = 3
     integer(c_int), parameter, public :: DCM_FP64 = 0, DCM_MIXED = 1
+    integer(c_int), parameter, public :: DCM_SUM_TREE = 0, DCM_SUM_SEQUENTIAL = 1
     integer(c_int), parameter, public :: DCM_F_U = 0, DCM_F_V = 1, DCM_F_W = 2, &
         DCM_F_ZU = 3, DCM_F_ZV = 4, DCM_F_ZW = 5, DCM_F_TH = 6, DCM_F_P = 7, &
         DCM_F_PSRC = 8, DCM_F_SU = 9, DCM_F_SV = 10, DCM_F_SW = 11, DCM_F_STH = 12, &
         DCM_F_Q = 13, DCM_F_THBAR = 14
 
     type, bind(c), public :: dcm_step_stats
-        integer(c_int) :: steps, iters_total, iters_max, outer_total
+        integer(c_int) :: steps, iters_total, iters_max, outer_total, fallbacks, reserved
         real(c_double) :: last_err, max_true_resid
     end type dcm_step_stats
 
@@ -29,7 +30,7 @@ module dcm_hip
         dcm_mirror_upload, dcm_mirror_download, dcm_mirror_release_all, dcm_last_error, &
         dcm_fill_full, dcm_fill_along_dim, dcm_fill_along_dim_wo, dcm_fill_full_wo, &
         dcm_fill_along_z, dcm_upwind3d, dcm_mpdata3d, dcm_mpdata_vec3d, dcm_adv_ref_state, &
-        dcm_hmeans, dcm_sources, dcm_divergence, dcm_bicgstab, dcm_advance_momentum, dcm_advance_scalars, &
+        dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_bicgstab, dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
         dcm_model_get, dcm_model_init, dcm_model_step
 
@@ -158,6 +159,11 @@ module dcm_hip
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx, u, v, w, th, su, sv, sw, sth, rhs
             real(c_double), intent(in) :: theta(*), thbar(*)
+        end function
+        integer(c_int) function dcm_set_sum_order(ctx, order) bind(c, name="dcm_set_sum_order")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: order
         end function
         integer(c_int) function dcm_divergence(ctx, u, v, w, div) bind(c, name="dcm_divergence")
             import :: c_int, c_ptr
@@ -292,7 +298,7 @@ contains
         integer, intent(in) :: l_h, r_h
         if (l_h /= 1 .or. r_h /= 1) then
             print *, 'dcm: the GPU path supports a halo of one cell only'
-            stop
+            error stop 1
         endif
     end subroutine need_halo1
 
@@ -342,7 +348,7 @@ contains
             n = n + 1
         enddo
         print *, what, ' failed with status ', st, ' ', msg(1:n)
-        stop
+        error stop 1
     end subroutine dcm_check
 
 end module dcm_hip

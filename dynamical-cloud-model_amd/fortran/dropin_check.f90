@@ -60,8 +60,22 @@ program dropin_check
     zu = u
     zv = v
     zw = w
+    su = 0.0_wp
+    sv = 0.0_wp
+    sw = 0.0_wp
+    rhs = 0.0_wp
     sth = 0.0_wp
     sq = 0.0_wp
+    strain = 0.0_wp
+    vism = 0.0_wp
+    vist = 0.0_wp
+    viss = 0.0_wp
+    rad_power = 0.0_wp
+    flux_u = 0.0_wp
+    flux_d = 0.0_wp
+    ngs = 0.0_wp
+    lamgs = 0.0_wp
+    mugs = 0.0_wp
     div = 0.0_wp
     rad1 = 0.0_wp
     mvrecv = 0.0_wp
@@ -79,6 +93,11 @@ program dropin_check
         0.0_wp, 1, 1, tri, tri, tri, tri, tri, 1, 1, 1, flux_u, flux_d, rad_power, rad1, &
         rad1, rad1, rad1, rad1, rad1, rad1, rad1, 2000, 1, 1, 0, 0, 0, 0.0_wp, 0.0_wp, &
         0.0_wp, 0.0_wp, 0, 0.0_wp, 1, ngs, lamgs, mugs, 1, mvrecv)
+    call report('sources su', su)
+    call report('sources sv', sv)
+    call report('sources sw', sw)
+    call report('sources rhs', rhs(1:kp, 1:jp, 1:ip))
+    call report('p (input)', p)
     call lateral_faces(rhs)
     call bicgstab(comm3d, id, rank, dims, coords, dt, xg, yg, zg, dx, dy, dz, dxn, dyn, dzn, &
         rhoa, rhoan, ip, jp, kp, l_h, r_h, su, sv, sw, zu, zv, zw, p, rhs, tol, .false.)
@@ -105,6 +124,17 @@ program dropin_check
     close(iu)
 
 contains
+
+    !> print where an array holds non-finite values (nothing when it is clean)
+    subroutine report(what, a)
+        use, intrinsic :: ieee_arithmetic, only: ieee_is_finite
+        character(len=*), intent(in) :: what
+        real(wp), intent(in) :: a(:, :, :)
+        integer :: nbad
+        nbad = count(.not. ieee_is_finite(a))
+        if (nbad > 0) print *, what, ': ', nbad, ' non-finite of ', size(a), &
+            ' first at ', findloc(.not. ieee_is_finite(a), .true.)
+    end subroutine report
 
     !> exchange_along_dim_wo on one rank (mpi_module.f90:1285-1494): the
     !> x and y faces, interior rows only

@@ -147,6 +147,14 @@ int dcm_sources(dcm_ctx *ctx, const double *u, const double *v,
 int dcm_matvec(dcm_ctx *ctx, double *x, double *ax);
 /* precondition (:241-270): prit Jacobi sweeps, s = M r. */
 int dcm_precondition(dcm_ctx *ctx, double *s, const double *r, int prit);
+/* Order of every global sum of the context (dot products, horizontal
+ * means).  DCM_SUM_TREE (default): a fixed two-level tree, deterministic run
+ * to run.  DCM_SUM_SEQUENTIAL: the reference's serial loop order (i outer, j,
+ * k inner; inn_prod :470-476, diagnostics.f90:55-60), one GPU thread per sum;
+ * bit-identical to a serial CPU run, slow, meant for verification.  A new
+ * context starts sequential when the environment has DCM_SUM_ORDER=sequential. */
+enum { DCM_SUM_TREE = 0, DCM_SUM_SEQUENTIAL = 1 };
+int dcm_set_sum_order(dcm_ctx *ctx, int order);
 /* inn_prod (:458-482): interior dot product, result to *out (host). */
 int dcm_inn_prod(dcm_ctx *ctx, const double *x, const double *y, double *out);
 /* bicgstab (:293-444).  xg,yg,zg are host coordinate arrays used only for
@@ -196,6 +204,8 @@ typedef struct {
     int iters_total;      /* BiCGSTAB iterations (inner, for mixed)         */
     int iters_max;        /* largest per-step iteration count               */
     int outer_total;      /* refinement sweeps (mixed only)                 */
+    int fallbacks;        /* mixed solves finished by the FP64 solver       */
+    int reserved;         /* zero                                           */
     double last_err;      /* last step's final residual                     */
     double max_true_resid;/* largest FP64 true residual seen (0 if unchecked)*/
 } dcm_step_stats;

@@ -728,8 +728,28 @@ void or_precondition(const or_grid *g, double *s, const double *r, int prit) {
     free(t);
 }
 
+/* Summation order of or_inn_prod.  0: the reference's serial loop.  1: 64
+ * interleaved partial sums combined pairwise, a different but equally valid
+ * order, used only to measure how far results move when only the order of
+ * the dot products changes. */
+static int g_sum_blocked = 0;
+void or_set_sum_blocked(int on) { g_sum_blocked = on; }
+
 /* inn_prod, dynamics.f90:458-482: sequential, i-j-k loop order */
 double or_inn_prod(const or_grid *g, const double *x, const double *y) {
+    if (g_sum_blocked) {
+        double part[64] = {0};
+        size_t c = 0;
+        for (int i = 1; i <= g->ip; ++i)
+            for (int j = 1; j <= g->jp; ++j)
+                for (int k = 1; k <= g->kp; ++k) {
+                    part[c & 63] = part[c & 63] + A3(x, k, j, i) * A3(y, k, j, i);
+                    ++c;
+                }
+        for (int w = 32; w >= 1; w >>= 1)
+            for (int q = 0; q < w; ++q) part[q] = part[q] + part[q + w];
+        return part[0];
+    }
     double s = 0.0;
     for (int i = 1; i <= g->ip; ++i)
         for (int j = 1; j <= g->jp; ++j)

@@ -87,6 +87,9 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
 void or_divergence(const or_grid *g, const double *u, const double *v,
                    const double *w, double *div);
 
+/* 1: blocked summation in or_inn_prod (sensitivity measurements only) */
+void or_set_sum_blocked(int on);
+
 void or_advance_momentum(const or_grid *g, double dt, double *u, double *v,
                          double *w, const double *zu, const double *zv,
                          const double *zw, const double *su, const double *sv,

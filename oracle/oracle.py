@@ -89,6 +89,7 @@ def lib():
         L.or_bicgstab.restype = C.c_int
         L.or_advance_momentum.argtypes = [g, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_divergence.argtypes = [g, _dp, _dp, _dp, _dp]
+        L.or_set_sum_blocked.argtypes = [C.c_int]
         L.or_advance_scalars.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
         m = C.POINTER(_Model)
         L.or_model_init_bcs.argtypes = [g, m]
@@ -206,6 +207,11 @@ def bicgstab(g: Grid, sw, x, b, tol=1e-10, test_solver=False, first_call=True,
                            ptr(b), tol, int(test_solver), int(first_call), itmax,
                            C.byref(it), C.byref(err))
     return st, it.value, err.value
+
+
+def set_sum_blocked(on: bool):
+    """Blocked (64-way) summation in the dot products instead of the serial loop."""
+    lib().or_set_sum_blocked(int(on))
 
 
 def divergence(g: Grid, u, v, w, div):

@@ -4,7 +4,8 @@ not need a CPU run of the same size:
 * every pressure solve ends at FP64 true residual ||b-Ax||/max(||b||,0.01)
   below the 1e-10 tolerance, on both paths;
 * MPDATA in flux form with periodic sides and a rigid floor and lid
-  conserves the density-weighted integral of every advected scalar: to FP64
+  conserves the density-weighted integral of every advected scalar (theta
+  only over a constant base state, where adv_ref_state adds nothing): to FP64
   rounding on the FP64 path (1e-12), to FP32 flux rounding on the mixed path
   (1e-6);
 * the mixed path stays within 1e-5 (relative L2) of the FP64 path.
@@ -18,7 +19,7 @@ import pytest
 import oracle as O
 from dcm_amd import hip as H
 from dcm_amd import workloads as W
-from helpers import rel_l2
+from helpers import lateral_edge_mask, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +37,9 @@ def _run(case, precision):
         try:
             m.set_check_residual(True)
             st = m.step(STEPS)
+            print(f"  precision {precision}: iterations {st.iters_total}, outer "
+                  f"{st.outer_total}, FP64 fallbacks {st.fallbacks}, true residual "
+                  f"{st.max_true_resid:.3g}", flush=True)
             out = {"th": m.get(H.F_TH), "u": m.get(H.F_U), "w": m.get(H.F_W)}
             for n in range(case.nq):
                 out[f"q{n}"] = m.get(H.F_Q, n)
@@ -50,7 +54,9 @@ def test_full_size_residual_conservation_and_mixed_error(name):
     case = W.CONFIGS[name](steps=STEPS)
     dom = case.dom
     print(f"full size {name}: {dom.ip}x{dom.jp}x{dom.kp}", flush=True)
-    m0 = {"th": _mass(dom, case.th)}
+    # theta is conserved only where the reference-state advection is a no-op
+    # (a constant base state); q always
+    m0 = {"th": _mass(dom, case.th)} if np.ptp(dom.theta) == 0.0 else {}
     for n in range(case.nq):
         m0[f"q{n}"] = _mass(dom, case.q[n])
     results = {}
@@ -68,10 +74,12 @@ def test_full_size_residual_conservation_and_mixed_error(name):
 
 
 def test_warm_bubble_64_cubed_matches_oracle():
-    """The reference's CPU-runnable case at its full size, two steps."""
+    """The reference's CPU-runnable case at its full size, two steps, with
+    sequential sums: iteration counts equal and every field bit-identical."""
     case = W.warm_bubble(64, 64, 64, steps=2)
     om = O.Model(case)
     with H.Context(case.dom) as c:
+        c.set_sum_order(H.SUM_SEQUENTIAL)
         gm = H.Model(c, case)
         try:
             for s in range(2):
@@ -80,6 +88,8 @@ def test_warm_bubble_64_cubed_matches_oracle():
                 assert st.iters_total == om.iters[-1], (s, st.iters_total, om.iters[-1])
             (u, v, w), _ = om.levels()
             for fid, ref in ((H.F_U, u), (H.F_V, v), (H.F_W, w), (H.F_TH, om.th)):
-                assert rel_l2(ref, gm.get(fid)) < 1e-12, fid
+                np.testing.assert_array_equal(gm.get(fid), ref, err_msg=str(fid))
+            mask = ~lateral_edge_mask(case.dom)  # p's edges are stale in the reference
+            np.testing.assert_array_equal(gm.get(H.F_P)[mask], om.p[mask])
         finally:
             gm.close()

@@ -313,27 +313,49 @@ def test_advance_scalars_bitwise():
 
 
 # ---------------------------------------------------------------- a15 model step
-def _compare_models(case, steps, tol=1e-12):
+def _compare_models(case, steps, seq=True):
+    """GPU model against the oracle model, step by step.
+
+    With sequential sums (the reference's serial order for every dot product
+    and mean) every operation matches the oracle's, so the iteration counts
+    are equal and every field is bit-identical.  With the default tree sums
+    only the summation order differs; BiCGSTAB stops on a 1e-10 residual, so
+    the iterate can move by up to about tol and the iteration count by a few
+    (the oracle itself does the same when its own sums are reordered:
+    test_oracle_pins.py::test_sum_order_sensitivity).
+    """
     om = O.Model(case)
     with H.Context(case.dom) as c:
+        if seq:
+            c.set_sum_order(H.SUM_SEQUENTIAL)
         gm = H.Model(c, case)
         try:
             for s in range(steps):
                 st = gm.step(1)
                 om.step(1)
-                assert st.iters_total == om.iters[-1], (s, st.iters_total, om.iters[-1])
+                if seq:
+                    assert st.iters_total == om.iters[-1], (s, st.iters_total, om.iters[-1])
+                else:
+                    assert abs(st.iters_total - om.iters[-1]) <= max(3, om.iters[-1] // 10)
             (u, v, w), (zu, zv, zw) = om.levels()
             pairs = {"u": (H.F_U, u), "v": (H.F_V, v), "w": (H.F_W, w), "zu": (H.F_ZU, zu),
-                     "zv": (H.F_ZV, zv), "zw": (H.F_ZW, zw), "th": (H.F_TH, om.th)}
-            for name, (fid, ref) in pairs.items():
-                got = gm.get(fid)
-                assert rel_l2(ref, got) < tol, (name, rel_l2(ref, got))
-            mask = ~lateral_edge_mask(case.dom)
-            got = gm.get(H.F_P)
-            assert rel_l2(om.p[mask], got[mask]) < tol, ("p", rel_l2(om.p[mask], got[mask]))
+                     "zv": (H.F_ZV, zv), "zw": (H.F_ZW, zw), "th": (H.F_TH, om.th),
+                     "p": (H.F_P, om.p)}
             for n in range(case.nq):
-                got = gm.get(H.F_Q, n)
-                assert rel_l2(om.q[n], got) < tol, ("q", n, rel_l2(om.q[n], got))
+                pairs[f"q{n}"] = (H.F_Q, om.q[n])
+            mask = ~lateral_edge_mask(case.dom)
+            for name, (fid, ref) in pairs.items():
+                got = gm.get(fid, int(name[1:]) if name[0] == "q" else 0)
+                if name == "p":
+                    # the lateral edges of p are stale in the reference
+                    # (exchange_along_dim_wo fills faces only); compare the rest
+                    ref, got = ref[mask], got[mask]
+                if seq:
+                    np.testing.assert_array_equal(got, ref, err_msg=name)
+                else:
+                    if name == "p":
+                        ref, got = ref - ref.mean(), got - got.mean()
+                    assert rel_l2(ref, got) < 1e-8, (name, rel_l2(ref, got))
         finally:
             gm.close()
 
@@ -351,3 +373,10 @@ def test_model_moist_bubble_parity():
 def test_model_kelvin_helmholtz_parity():
     case = W.kelvin_helmholtz(24, 4, 16, steps=5)
     _compare_models(case, 5)
+
+
+@pytest.mark.parametrize("make", [lambda: W.warm_bubble(16, 16, 16, steps=4),
+                                  lambda: W.kelvin_helmholtz(24, 4, 16, steps=4)])
+def test_model_tree_sums_within_solver_tolerance(make):
+    case = make()
+    _compare_models(case, 4, seq=False)

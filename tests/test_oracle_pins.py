@@ -669,3 +669,25 @@ def test_model_conserves_theta_mass_and_is_deterministic():
     m2.step(3)
     np.testing.assert_array_equal(m1.th, m2.th)
     assert m1.iters == m2.iters
+
+
+def test_sum_order_sensitivity():
+    """Why model parity is stated for one summation order: reordering only the
+    dot products of the oracle (serial loop -> 64-way blocked sum) changes the
+    BiCGSTAB trajectory at 64^3.  Both runs meet the 1e-10 residual, yet the
+    iteration counts differ and the fields move far beyond 1e-12."""
+    case = W.warm_bubble(64, 64, 64, steps=2)
+    runs = []
+    for blocked in (False, True):
+        O.set_sum_blocked(blocked)
+        try:
+            m = O.Model(case)
+            m.step(2)
+            runs.append((list(m.iters), m.th.copy(), m.p.copy()))
+        finally:
+            O.set_sum_blocked(False)
+    (ia, tha, pa), (ib, thb, pb) = runs
+    assert ia != ib
+    I = (slice(1, -1),) * 3
+    dp = np.linalg.norm((pa - pa[I].mean())[I] - (pb - pb[I].mean())[I]) / np.linalg.norm(pa[I])
+    assert 1e-12 < dp < 1e-7

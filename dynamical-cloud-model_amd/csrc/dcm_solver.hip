@@ -86,6 +86,18 @@ __device__ __forceinline__ R coef_ap_i(const Coef<R> &C, R api, int k, int j) {
     return (((api - C.an[j]) - C.as[j]) - C.au[k]) - C.ad[k];
 }
 
+// x / a_p.  FP64: the correctly rounded quotient of the reference.  FP32
+// (the inner solve of the mixed path): a multiply by the hardware reciprocal
+// (within about 1 ulp), which keeps the Jacobi preconditioner a fixed linear
+// map at a fraction of the cost of an IEEE division.
+template <typename R> __device__ __forceinline__ R div_ap(R x, R ap);
+template <> __device__ __forceinline__ double div_ap<double>(double x, double ap) {
+    return x / ap;
+}
+template <> __device__ __forceinline__ float div_ap<float>(float x, float ap) {
+    return x * __builtin_amdgcn_rcpf(ap);
+}
+
 // neighbour indices of (k,j,i): periodic in i,j; Neumann copy in k
 struct Nbr {
     int iw, ie, js, jn, kd, ku;
@@ -311,7 +323,78 @@ __device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__
     return r[q] - a * v1[q];
 }
 
-// sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p with s0 = t/a_p
+// First sweep, variant 0: fused in registers, s0 = t/a_p evaluated at the
+// centre and the six neighbours (dynamics.f90:254-266).
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
+                                                   const R *__restrict__ v1,
+                                                   R *__restrict__ t_out, R *__restrict__ s_out,
+                                                   MG m, CoefPack cp,
+                                                   const SolverScalars *__restrict__ sc) {
+    if (sc->done || sc->breakdown) return;
+    Tile T;
+    if (!march_tile(m, T) || !T.act) return;
+    const Coef<R> C(cp.m64, cp.m32);
+    const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
+    const R bo = (R)sc->omg;
+    const int k = T.k, j = T.j;
+    const Cols q = cols(m, k, j);
+    const int js = j == 1 ? m.jp : j - 1, jn = j == m.jp ? 1 : j + 1;
+    const int kd = k == 1 ? 1 : k - 1, ku = k == m.kp ? m.kp : k + 1;
+    const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+    const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
+    int cw = (iw - 1) * m.si + q.base;
+    int cc = (T.i0 - 1) * m.si + q.base;
+    R s0w = div_ap<R>(tval<R, MODE>(r, v1, cw, a, bo), coef_ap(C, k, j, iw));
+    R tc = tval<R, MODE>(r, v1, cc, a, bo);
+    R s0c = div_ap<R>(tc, coef_ap(C, k, j, T.i0));
+    for (int i = T.i0; i <= T.i1; ++i) {
+        const int ie = i == m.ip ? 1 : i + 1;
+        const int ce = (ie - 1) * m.si + q.base;
+        const R te = tval<R, MODE>(r, v1, ce, a, bo);
+        const R s0e = div_ap<R>(te, coef_ap(C, k, j, ie));
+        const R api = (-C.ae[i]) - C.aw[i];
+        const R ap = coef_ap_i(C, api, k, j);
+        const R s0s = div_ap<R>(tval<R, MODE>(r, v1, cc + q.ds, a, bo), coef_ap_i(C, api, k, js));
+        const R s0n = div_ap<R>(tval<R, MODE>(r, v1, cc + q.dn, a, bo), coef_ap_i(C, api, k, jn));
+        const R s0d = q.dd ? div_ap<R>(tval<R, MODE>(r, v1, cc - 1, a, bo), coef_ap_i(C, api, kd, j)) : s0c;
+        const R s0u = q.du ? div_ap<R>(tval<R, MODE>(r, v1, cc + 1, a, bo), coef_ap_i(C, api, ku, j)) : s0c;
+        const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n, s0d,
+                                s0u, s0c);
+        t_out[cc] = tc;
+        s_out[cc] = s0c + div_ap<R>(tc - As, ap);
+        s0w = s0c;
+        s0c = s0e;
+        tc = te;
+        cc = ce;
+    }
+}
+
+// First sweep, variant 2: a streaming pass t = r - bet*(omg*v1) (or
+// r - alf*v1) with s0 = t/a_p, then the plain sweep k_jac on (s0, t).
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_jac_prep(const R *__restrict__ r,
+                                                  const R *__restrict__ v1,
+                                                  R *__restrict__ t_out, R *__restrict__ s0_out,
+                                                  MG m, CoefPack cp,
+                                                  const SolverScalars *__restrict__ sc) {
+    if (sc->done || sc->breakdown) return;
+    Tile T;
+    if (!march_tile(m, T) || !T.act) return;
+    const Coef<R> C(cp.m64, cp.m32);
+    const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
+    const R bo = (R)sc->omg;
+    const int k = T.k, j = T.j;
+    const Cols q = cols(m, k, j);
+    int cc = (T.i0 - 1) * m.si + q.base;
+    for (int i = T.i0; i <= T.i1; ++i, cc += m.si) {
+        const R tc = tval<R, MODE>(r, v1, cc, a, bo);
+        t_out[cc] = tc;
+        s0_out[cc] = div_ap<R>(tc, coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j));
+    }
+}
+
+// First sweep, variant 1: sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p with s0 = t/a_p
 // (dynamics.f90:254-266).  s0 is computed once per cell and plane and shared
 // through LDS: each plane of the tile plus a one-cell halo in j and k (the
 // 7-point stencil needs no corners) lives in one of three LDS buffers, so one
@@ -377,15 +460,15 @@ __global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
     const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
     R api = (-C.ae[iw]) - C.aw[iw];
     int q = (iw - 1) * m.si;
-    R s0w = tv(r[q + col], v1[q + col]) / coef_ap_i(C, api, kc, jw);
+    R s0w = div_ap<R>(tv(r[q + col], v1[q + col]), coef_ap_i(C, api, kc, jw));
     api = (-C.ae[T.i0]) - C.aw[T.i0];
     q = (T.i0 - 1) * m.si;
     R tc = tv(r[q + col], v1[q + col]);
-    R s0c = tc / coef_ap_i(C, api, kc, jw);
+    R s0c = div_ap<R>(tc, coef_ap_i(C, api, kc, jw));
     lds[(T.i0 % 3) * kLdsCells + oslot] = s0c;
     if (halo)
         lds[(T.i0 % 3) * kLdsCells + hslot] =
-            tv(r[q + hcol], v1[q + hcol]) / coef_ap_i(C, api, hk, hj);
+            div_ap<R>(tv(r[q + hcol], v1[q + hcol]), coef_ap_i(C, api, hk, hj));
     // raw loads of plane i0+1
     int ie = nexti(T.i0);
     q = (ie - 1) * m.si;
@@ -395,9 +478,9 @@ __global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
         // plane i+1 from the loads in flight
         const R api_e = (-C.ae[ie]) - C.aw[ie];
         const R te = tv(r_o, v_o);
-        const R s0e = te / coef_ap_i(C, api_e, kc, jw);
+        const R s0e = div_ap<R>(te, coef_ap_i(C, api_e, kc, jw));
         R hval = R(0);
-        if (halo && i < T.i1) hval = tv(r_h, v_h) / coef_ap_i(C, api_e, hk, hj);
+        if (halo && i < T.i1) hval = div_ap<R>(tv(r_h, v_h), coef_ap_i(C, api_e, hk, hj));
         // loads of plane i+2
         const int iee = nexti(ie);
         if (i < T.i1) {
@@ -425,7 +508,7 @@ __global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
                                     s0d, s0u, s0c);
             const int cc = (i - 1) * m.si + col;
             t_out[cc] = tc;
-            s_out[cc] = s0c + (tc - As) / ap;
+            s_out[cc] = s0c + div_ap<R>(tc - As, ap);
         }
         s0w = s0c;
         s0c = s0e;
@@ -474,7 +557,7 @@ __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *_
         }
         const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
         const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu, xc);
-        const R snew = xc + (tc - As) / ap;
+        const R snew = xc + div_ap<R>(tc - As, ap);
         if (P_UPDATE)
             s_out[cc] = snew + bet * po;  // p = s + bet*p (dynamics.f90:399)
         else
@@ -815,9 +898,43 @@ static CGeo cgeo(const Geo &g) {
     return c;
 }
 
-constexpr int kMarchPlanes = 8;     // i planes per marching block
+// tuning: i planes per marching block, and the first-sweep variant per
+// precision (0 fused in registers, 1 fused through LDS, 2 streaming pass +
+// plain sweep).  Measured at 256x256x128 on MI355X: FP64 is fastest split
+// (its IEEE divisions make the fused kernel issue-bound), FP32 fused in
+// registers; 8 and 16 planes are equal, 4 and 32 slower.
+#ifndef DCM_MARCH_PLANES
+#define DCM_MARCH_PLANES 8
+#endif
+#ifndef DCM_JAC_FIRST64
+#define DCM_JAC_FIRST64 2
+#endif
+#ifndef DCM_JAC_FIRST32
+#define DCM_JAC_FIRST32 0
+#endif
+constexpr int kMarchPlanes = DCM_MARCH_PLANES;
 constexpr int kUpdateBlocks = 2048; // grid of the streaming update
 constexpr int kPollEvery = 4;       // host polls the device flags this often
+
+// the first Jacobi sweep with its input t: t and s1 (into `s1`); `tmp` is
+// free scratch (variant 2 keeps s0 there)
+template <typename R, int MODE>
+static void first_sweep(dcm_ctx *c, const R *r, const R *v1, R *t, R *s1, R *tmp, const MG &m,
+                        unsigned mg, const CoefPack &cp, const SolverScalars *sc) {
+    const int variant = sizeof(R) == 8 ? DCM_JAC_FIRST64 : DCM_JAC_FIRST32;
+    if (variant == 0) {
+        hipLaunchKernelGGL((k_jac_first_reg<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r,
+                           v1, t, s1, m, cp, sc);
+    } else if (variant == 1) {
+        hipLaunchKernelGGL((k_jac_first<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
+                           t, s1, m, cp, sc);
+    } else {
+        hipLaunchKernelGGL((k_jac_prep<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1, t,
+                           tmp, m, cp, sc);
+        hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, tmp, t, s1,
+                           m, cp, sc);
+    }
+}
 
 // BiCGSTAB iterations on compact vectors of precision R.  V: r, cr, p, v1, t,
 // s, cs, two sweep buffers, x.  The initial state and scalars must be set.
@@ -842,8 +959,7 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
         const int chunk = (itmax - launched) < kPollEvery ? (itmax - launched) : kPollEvery;
         for (int q = 0; q < chunk; ++q) {
             int tok = prof_begin(c, "jac_first");
-            hipLaunchKernelGGL((k_jac_first<R, 0>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
-                               t, wa, m, cp, sc);
+            first_sweep<R, 0>(c, r, v1, t, wa, wb, m, mg, cp, sc);
             prof_end(c, tok);
             tok = prof_begin(c, "jac");
             hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, t, wb,
@@ -862,8 +978,7 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
                                    (const R *)nullptr, (const R *)nullptr, g.n, part, sc);
             hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
             tok = prof_begin(c, "jac_first");
-            hipLaunchKernelGGL((k_jac_first<R, 1>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
-                               s, wa, m, cp, sc);
+            first_sweep<R, 1>(c, r, v1, s, wa, wb, m, mg, cp, sc);
             prof_end(c, tok);
             tok = prof_begin(c, "jac");
             hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, s, wb,

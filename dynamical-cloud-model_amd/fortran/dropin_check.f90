@@ -48,14 +48,18 @@ program dropin_check
         rhoa(0:kp+1), rhoan(0:kp+1), theta(0:kp+1), thetan(0:kp+1), xg(0:ip+1), yg(0:jp+1), &
         zg(0:kp+1))
     read(iu) dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, theta, thetan, xg, yg, zg
-    allocate(u(0:kp+1, 0:jp+1, 0:ip+1), v, w, th, p, zu, zv, zw, su, sv, sw, sth, rhs, strain, &
-        vism, vist, div, psi, rad_power, mold=u)
-    allocate(q(0:kp+1, 0:jp+1, 0:ip+1, nq), sq, viss, mold=q)
+    ! (a MOLD must already be allocated: give u its shape first)
+    allocate(u(0:kp+1, 0:jp+1, 0:ip+1))
+    allocate(v, w, th, p, zu, zv, zw, su, sv, sw, sth, rhs, strain, vism, vist, div, psi, &
+        rad_power, mold=u)
+    allocate(q(0:kp+1, 0:jp+1, 0:ip+1, nq))
+    allocate(sq, viss, mold=q)
     read(iu) u, v, w, th, q, p
     close(iu)
     allocate(ubar(0:kp+1), vbar(0:kp+1), wbar(0:kp+1), thbar(0:kp+1), zero_k(0:kp+1), &
         qbar(0:kp+1, nq), lbcq(nq), ubcq(nq))
-    allocate(flux_u(0:kp+1, 0:jp+1, 0:ip+1, 1), flux_d, ngs, lamgs, mugs, mold=flux_u)
+    allocate(flux_u(0:kp+1, 0:jp+1, 0:ip+1, 1))
+    allocate(flux_d, ngs, lamgs, mugs, mold=flux_u)
     zero_k = 0.0_wp
     zu = u
     zv = v

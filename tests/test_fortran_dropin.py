@@ -89,33 +89,55 @@ def _read_output(path, dom, nq):
     return out
 
 
-@pytest.mark.gpu
-@pytest.mark.skipif(not os.path.exists(CHECK), reason="dcm_dropin_check not built")
-@pytest.mark.parametrize("monotone", [True, False])
-def test_dropin_sequence_matches_oracle(tmp_path, monotone):
-    # a model-like state: halos filled as after the driver's exchange_full; a
-    # uniform grid, so that the Poisson problem is solvable (random stretching
-    # of dz against dzn makes it inconsistent and BiCGSTAB diverges)
+MOCK = os.path.join(ROOT, "oracle", "libdcm_abi_mock.so")
+
+
+def _case():
+    """A model-like state: halos filled as after the driver's exchange_full,
+    on a uniform grid, so that the Poisson problem is solvable (a random
+    stretching of dz against dzn makes it inconsistent and BiCGSTAB
+    diverges)."""
     dom = small_domain(12, 10, 9, varying_theta=True)
     g = O.Grid(dom)
-    nq, kord, dt, tol = 2, 2, 0.5, 1e-10
     u, v, w = (field(dom, 300 + n, -2.0, 2.0) for n in range(3))
     th = field(dom, 303, 0.0, 1.0)
     for a in (u, v, w, th):
         O.fill_full(g, a, 0.0, 0.0)
     q = np.stack([field(dom, 304, 0.0, 1.0), field(dom, 305, 0.0, 0.1)])
     p = field(dom, 306, -0.1, 0.1)
+    return dom, (u, v, w, th, q, p)
+
+
+def _run_check(tmp_path, monotone, env):
+    dom, fields = _case()
+    nq, kord, dt, tol = 2, 2, 0.5, 1e-10
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_input(fin, dom, nq, kord, monotone, dt, tol, (u, v, w, th, q, p))
+    _write_input(fin, dom, nq, kord, monotone, dt, tol, fields)
     print(f"dcm_dropin_check 12x10x9 monotone={monotone}", flush=True)
-    # sequential sums: the drop-in's solve follows the oracle's trajectory exactly
-    env = dict(os.environ, DCM_SUM_ORDER="sequential")
     r = subprocess.run([CHECK, str(fin), str(fout)], capture_output=True, text=True,
-                       timeout=120, env=env)
+                       timeout=120, env=dict(os.environ, **env))
     # a Fortran stop exits with status 0, so the output file is the signal
     assert r.returncode == 0 and fout.exists(), r.stdout + r.stderr
     got = _read_output(fout, dom, nq)
-    exp, _ = _oracle_sequence(dom, nq, kord, monotone, dt, tol, u, v, w, th, q, p)
+    exp, _ = _oracle_sequence(dom, nq, kord, monotone, dt, tol, *fields)
     for name in ("thbar", "qbar", "div", "su", "sv", "sw", "sth", "rhs", "psi", "p", "u", "v",
                  "w", "th", "q"):
         np.testing.assert_array_equal(got[name], exp[name], err_msg=name)
+
+
+@pytest.mark.skipif(not (os.path.exists(CHECK) and os.path.exists(MOCK)),
+                    reason="dcm_dropin_check or the oracle's ABI mock not built")
+def test_fortran_layer_on_the_cpu(tmp_path):
+    """The Fortran modules over the C-ABI served on the CPU by the oracle:
+    checks the Fortran side alone (argument order and bounds, logicals,
+    per-argument buffers, uploads and downloads) without a GPU."""
+    _run_check(tmp_path, True, {"LD_PRELOAD": MOCK})
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CHECK), reason="dcm_dropin_check not built")
+@pytest.mark.parametrize("monotone", [True, False])
+def test_dropin_sequence_matches_oracle(tmp_path, monotone):
+    """On the GPU, with sequential sums, so that the solve follows the
+    oracle's trajectory exactly."""
+    _run_check(tmp_path, monotone, {"DCM_SUM_ORDER": "sequential"})

@@ -312,6 +312,94 @@ __global__ void k_seq_dot_padded(const double *__restrict__ x, const double *__r
 }
 
 // ---------------------------------------------------------------------------
+// scalar recurrences (one block; reduce the partial sums first)
+// ---------------------------------------------------------------------------
+// tiny = 1.e-16 is a default-real literal (dynamics.f90:321)
+__device__ constexpr double kTinyOmega = (double)1.e-16f;
+// inner (FP32) solves: iterations without a 10% gain that count as stagnation
+constexpr int kStall = 40;
+
+__global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, double tol,
+                           int use_bb) {
+    const double bb = reduce_slot_1024(partials, nb);
+    const double rr = reduce_slot_1024(partials + nb, nb);
+    if (threadIdx.x == 0) {
+        double sc_err = use_bb ? sqrt(bb) : sc->sc_err;
+        if (use_bb && sc_err < 0.01) sc_err = 0.01;  // :371
+        sc->bb = bb;
+        sc->sc_err = sc_err;
+        sc->rr = rr;
+        sc->err = sqrt(rr) / sc_err;
+        sc->alf = 1.0;
+        sc->omg = 1.0;
+        sc->nrm = 1.0;
+        sc->rho = rr;  // <r,cr> with cr = r (:391)
+        sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
+        sc->breakdown = 0;
+        sc->iters = 0;
+        sc->guard = 0;
+        sc->nonfinite = !isfinite(sc->err);
+        sc->done = (sc->err < tol) || sc->nonfinite;
+    }
+}
+
+// scalar recurrences after each reduction (dynamics.f90:404, :412-414,
+// :423-425); called by a one-block kernel or by the last block of the
+// kernel that produced the partial sums
+__device__ __forceinline__ void fin_v_apply(SolverScalars *sc, double nrm) {
+    sc->nrm = nrm;
+    sc->alf = sc->rho / nrm;  // :404
+    if (sc->guard && !(nrm != 0.0 && isfinite(sc->alf))) sc->breakdown = 1;
+}
+__device__ __forceinline__ void fin_t_apply(SolverScalars *sc, double tt, double ts) {
+    sc->iters += 1;
+    sc->tt = tt;
+    sc->ts = ts;
+    sc->omg = ts / tt;  // :412
+    if (fabs(sc->omg) < kTinyOmega) sc->breakdown = 1;  // :414
+    if (sc->guard && !(tt != 0.0 && isfinite(sc->omg))) sc->breakdown = 1;
+}
+__device__ __forceinline__ void fin_r_apply(SolverScalars *sc, double rr, double rcr,
+                                            double tol) {
+    sc->rr = rr;
+    sc->err = sqrt(rr) / sc->sc_err;  // :423
+    sc->nrm = sc->rho;                // :425
+    sc->rho = rcr;                    // next iteration's <r,cr> (:391)
+    sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
+    sc->nonfinite = !isfinite(sc->err);
+    sc->done = (sc->err < tol) || sc->nonfinite;
+    if (sc->guard) {
+        // an inner solve that has not improved its best residual by 10% for
+        // kStall iterations has stagnated at its precision: stop and let the
+        // FP64 refinement restart it from the true residual
+        if (sc->err < 0.9 * sc->best_err) {
+            sc->best_err = sc->err;
+            sc->best_it = sc->iters;
+        } else if (sc->iters - sc->best_it >= kStall) {
+            sc->done = 1;
+        }
+    }
+}
+
+__global__ void k_fin_v(const double *partials, int nb, SolverScalars *sc) {
+    if (sc->done || sc->breakdown) return;
+    const double nrm = reduce_slot_1024(partials, nb);
+    if (threadIdx.x == 0) fin_v_apply(sc, nrm);
+}
+__global__ void k_fin_t(const double *partials, int nb, SolverScalars *sc) {
+    if (sc->done || sc->breakdown) return;
+    const double tt = reduce_slot_1024(partials, nb);
+    const double ts = reduce_slot_1024(partials + nb, nb);
+    if (threadIdx.x == 0) fin_t_apply(sc, tt, ts);
+}
+__global__ void k_fin_r(const double *partials, int nb, SolverScalars *sc, double tol) {
+    if (sc->done || sc->breakdown) return;
+    const double rr = reduce_slot_1024(partials, nb);
+    const double rcr = reduce_slot_1024(partials + nb, nb);
+    if (threadIdx.x == 0) fin_r_apply(sc, rr, rcr, tol);
+}
+
+// ---------------------------------------------------------------------------
 // preconditioner sweeps (dynamics.f90:241-270), marching kernels
 // ---------------------------------------------------------------------------
 // MODE 0: t = r - bet*(omg*v1)     (dynamics.f90:394)
@@ -578,7 +666,10 @@ __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *_
 }
 
 // y = A x; partial sums of <a,y> (and <y,y> first when TWO).  Loads of plane
-// i+1 in flight while plane i is computed.
+// i+1 in flight while plane i is computed.  (A last-arriving-block reduction
+// that applies the scalar recurrence in place was measured 1.6x slower
+// overall: every block's device-scope release fence writes back its XCD's
+// L2.  One-block kernels reduce instead.)
 template <typename R, bool TWO>
 __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
                                                     const R *__restrict__ a, R *__restrict__ y,
@@ -641,6 +732,192 @@ __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
     block_sum_store_at<2>(acc, partials, stride, T.tile);
 }
 
+// ---------------------------------------------------------------------------
+// The whole preconditioner in one pass (temporal blocking of precondition,
+// dynamics.f90:241-270, prit = 3): t = r - bet*(omg*v1) (MODE 0) or
+// r - alf*v1 (MODE 1), s0 = t/a_p, three Jacobi sweeps, then p = s3 + bet*p
+// (MODE 0) or cs = s3 with s = t stored (MODE 1).  HBM sees r and v1 once
+// and the outputs once; the intermediate sweeps live in LDS.
+//
+// A block owns a KT x JT tile of (k,j) columns and marches IC i-planes.  At
+// step n it evaluates level 0 on plane n+3, level 1 on n+2, level 2 on n+1
+// and the output level on n, each over the tile grown by the halo that the
+// next level's 7-point stencil needs (3, 2, 1, 0 cells in j and k).  Level L
+// keeps three planes in an LDS ring, t keeps four, all in one layout of
+// (JT+6) x (KT+6) cells.  Cells outside [1,kp] in k are never read (the
+// Neumann copy makes the k-neighbour of k=1 and k=kp the cell itself); cells
+// outside [1,jp] in j are the periodic images, evaluated at the wrapped
+// index.  Each value is the unfused path's expression on the same operands,
+// so the result is bit-identical to the three separate sweeps.
+// ---------------------------------------------------------------------------
+struct PGeo {
+    int ip, jp, kp, sj, si;
+    int nkb, njb, nic, ic, nblocks;
+};
+
+template <int KT, int JT>
+static PGeo precond_geo(const Geo &g, int ic) {
+    PGeo p;
+    p.ip = g.ip;
+    p.jp = g.jp;
+    p.kp = g.kp;
+    p.sj = g.kp;
+    p.si = g.kp * g.jp;
+    p.nkb = (g.kp + KT - 1) / KT;
+    p.njb = (g.jp + JT - 1) / JT;
+    p.ic = ic;
+    p.nic = (g.ip + ic - 1) / ic;
+    p.nblocks = p.nkb * p.njb * p.nic;
+    return p;
+}
+
+__device__ __forceinline__ int wrap1(int x, int n) { return ((x - 1) % n + n) % n + 1; }
+
+// one Jacobi sweep of the fused preconditioner over the tile grown by HH
+// cells, plane step mstep; all index arithmetic by compile-time constants
+template <typename R, int KT, int JT, int HH, int NS>
+__device__ __forceinline__ void precond_sweep(const R (*Sin)[NS], R (*Sout)[NS],
+                                              const R (*TT)[NS], const Coef<R> &C,
+                                              const int *JW, const R *AS, const R *AN,
+                                              const R *AD, const R *AU, int k0, int kp, int ip,
+                                              int mstep, int tid) {
+    constexpr int H = 3, W = KT + 2 * H, NT = KT * JT;
+    constexpr int RW = KT + 2 * HH, NR = (JT + 2 * HH) * RW;
+    const int P = wrap1(mstep, ip);
+    const R api = (-C.ae[P]) - C.aw[P];
+    const R aw = C.aw[P], ae = C.ae[P];
+    const int sc = ((mstep % 3) + 3) % 3, sw = ((mstep + 2) % 3 + 3) % 3,
+              se = ((mstep + 1) % 3 + 3) % 3;
+    const R *xin = Sin[sc];
+    const R *xwp = Sin[sw];
+    const R *xep = Sin[se];
+    const R *tt = TT[((mstep % 4) + 4) % 4];
+    R *xo = Sout[sc];
+#pragma unroll 1
+    for (int p0 = 0; p0 < NR; p0 += NT) {
+        const int idx = p0 + tid;
+        if (idx < NR) {
+            const int row = idx / RW + (H - HH), col = idx % RW + (H - HH);
+            const int k = k0 + col - H;
+            if (k >= 1 && k <= kp) {
+                const int L = row * W + col;
+                const R xc = xin[L];
+                const R ap = (((api - AN[row]) - AS[row]) - AU[col]) - AD[col];
+                const R xd = k == 1 ? xc : xin[L - 1];
+                const R xu = k == kp ? xc : xin[L + 1];
+                const R As = stencil<R>(aw, ae, AS[row], AN[row], AD[col], AU[col], ap, xwp[L],
+                                        xep[L], xin[L - W], xin[L + W], xd, xu, xc);
+                xo[L] = xc + div_ap<R>(tt[L] - As, ap);
+            }
+        }
+    }
+    (void)JW;
+}
+
+template <typename R, int MODE, int KT, int JT>
+__global__ __launch_bounds__(KT *JT) void k_precond3(const R *__restrict__ r,
+                                                     const R *__restrict__ v1,
+                                                     R *__restrict__ out_t, R *__restrict__ out,
+                                                     PGeo g, CoefPack cp,
+                                                     const SolverScalars *__restrict__ sc) {
+    constexpr int H = 3, W = KT + 2 * H, RWS = JT + 2 * H, NS = RWS * W, NT = KT * JT;
+    __shared__ R S0[3][NS], S1[3][NS], S2[3][NS], TT[4][NS];
+    // per-row (wrapped j) and per-column (k) coefficients of the tile + halo
+    __shared__ int JW[RWS];
+    __shared__ R AS[RWS], AN[RWS], AD[W], AU[W];
+    if (sc->done || sc->breakdown) return;
+    const int b = blockIdx.x;
+    const int per = (g.nblocks + 7) >> 3;
+    const int tile = (b & 7) * per + (b >> 3);
+    if (tile >= g.nblocks) return;
+    const int kb = tile % g.nkb, rr = tile / g.nkb;
+    const int jb = rr % g.njb, icb = rr / g.njb;
+    const int k0 = kb * KT + 1, j0 = jb * JT + 1;
+    const int i0 = icb * g.ic + 1;
+    const int i1 = i0 + g.ic - 1 > g.ip ? g.ip : i0 + g.ic - 1;
+    const Coef<R> C(cp.m64, cp.m32);
+    const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
+    const R bo = (R)sc->omg;
+    const R bet = (R)sc->bet;
+    const int tid = (int)threadIdx.x;
+    if (tid < RWS) {
+        const int j = wrap1(j0 - H + tid, g.jp);
+        JW[tid] = j;
+        AS[tid] = C.as[j];
+        AN[tid] = C.an[j];
+    }
+    if (tid < W) {
+        int k = k0 - H + tid;
+        k = k < 1 ? 1 : (k > g.kp ? g.kp : k);
+        AD[tid] = C.ad[k];
+        AU[tid] = C.au[k];
+    }
+    __syncthreads();
+
+    for (int n = i0 - 6; n <= i1; ++n) {
+        // level 0 on plane n+3: t and s0 = t/a_p over the tile grown by 3
+        {
+            const int P = wrap1(n + 3, g.ip);
+            const R api = (-C.ae[P]) - C.aw[P];
+            R *s0 = S0[((n + 3) % 3 + 3) % 3];
+            R *tt = TT[((n + 3) % 4 + 4) % 4];
+            const int base = (P - 1) * g.si;
+#pragma unroll 1
+            for (int p0 = 0; p0 < NS; p0 += NT) {
+                const int idx = p0 + tid;
+                if (idx < NS) {
+                    const int row = idx / W, col = idx % W;
+                    int k = k0 - H + col;
+                    k = k < 1 ? 1 : (k > g.kp ? g.kp : k);
+                    const int q = base + (JW[row] - 1) * g.sj + (k - 1);
+                    const R tv = MODE == 0 ? r[q] - a * (bo * v1[q]) : r[q] - a * v1[q];
+                    tt[idx] = tv;
+                    s0[idx] = div_ap<R>(tv, (((api - AN[row]) - AS[row]) - AU[col]) - AD[col]);
+                }
+            }
+        }
+        __syncthreads();
+        if (n >= i0 - 4)
+            precond_sweep<R, KT, JT, 2, NS>(S0, S1, TT, C, JW, AS, AN, AD, AU, k0, g.kp, g.ip,
+                                            n + 2, tid);
+        __syncthreads();
+        if (n >= i0 - 2)
+            precond_sweep<R, KT, JT, 1, NS>(S1, S2, TT, C, JW, AS, AN, AD, AU, k0, g.kp, g.ip,
+                                            n + 1, tid);
+        __syncthreads();
+        if (n >= i0) {
+            // output level on plane n, own cells
+            const int P = n;  // i0 <= n <= i1 <= ip
+            const int jr = tid / KT, kr = tid % KT;
+            const int k = k0 + kr, j = j0 + jr;
+            if (k <= g.kp && j <= g.jp) {
+                const R api = (-C.ae[P]) - C.aw[P];
+                const int row = jr + H, col = kr + H;
+                const int L = row * W + col;
+                const R *xin = S2[(n % 3 + 3) % 3];
+                const R xc = xin[L];
+                const R ap = (((api - AN[row]) - AS[row]) - AU[col]) - AD[col];
+                const R xd = k == 1 ? xc : xin[L - 1];
+                const R xu = k == g.kp ? xc : xin[L + 1];
+                const R As = stencil<R>(C.aw[P], C.ae[P], AS[row], AN[row], AD[col], AU[col], ap,
+                                        S2[((n + 2) % 3 + 3) % 3][L],
+                                        S2[((n + 1) % 3 + 3) % 3][L], xin[L - W], xin[L + W], xd,
+                                        xu, xc);
+                const R tv = TT[(n % 4 + 4) % 4][L];
+                const R s3 = xc + div_ap<R>(tv - As, ap);
+                const int q = (P - 1) * g.si + (j - 1) * g.sj + (k - 1);
+                if (MODE == 0) {
+                    out[q] = s3 + bet * out[q];  // p = s + bet*p (dynamics.f90:399)
+                } else {
+                    out[q] = s3;    // cs
+                    out_t[q] = tv;  // s = r - alf*v1 (dynamics.f90:405)
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // x = x + alf*p + omg*cs; r = s - omg*t; partial sums of <r,r>, <r,cr>
 // (dynamics.f90:419-423), all compact vectors, grid-stride
 template <typename R>
@@ -665,87 +942,8 @@ __global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__re
 }
 
 // ---------------------------------------------------------------------------
-// scalar recurrences (one block; reduce the partial sums first)
+// inner-solve start and generic sums
 // ---------------------------------------------------------------------------
-// tiny = 1.e-16 is a default-real literal (dynamics.f90:321)
-__device__ constexpr double kTinyOmega = (double)1.e-16f;
-// inner (FP32) solves: iterations without a 10% gain that count as stagnation
-constexpr int kStall = 40;
-
-__global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, double tol,
-                           int use_bb) {
-    const double bb = reduce_slot_1024(partials, nb);
-    const double rr = reduce_slot_1024(partials + nb, nb);
-    if (threadIdx.x == 0) {
-        double sc_err = use_bb ? sqrt(bb) : sc->sc_err;
-        if (use_bb && sc_err < 0.01) sc_err = 0.01;  // :371
-        sc->bb = bb;
-        sc->sc_err = sc_err;
-        sc->rr = rr;
-        sc->err = sqrt(rr) / sc_err;
-        sc->alf = 1.0;
-        sc->omg = 1.0;
-        sc->nrm = 1.0;
-        sc->rho = rr;  // <r,cr> with cr = r (:391)
-        sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
-        sc->breakdown = 0;
-        sc->iters = 0;
-        sc->guard = 0;
-        sc->nonfinite = !isfinite(sc->err);
-        sc->done = (sc->err < tol) || sc->nonfinite;
-    }
-}
-
-__global__ void k_fin_v(const double *partials, int nb, SolverScalars *sc) {
-    if (sc->done || sc->breakdown) return;
-    const double nrm = reduce_slot_1024(partials, nb);
-    if (threadIdx.x == 0) {
-        sc->nrm = nrm;
-        sc->alf = sc->rho / nrm;  // :404
-        if (sc->guard && !(nrm != 0.0 && isfinite(sc->alf))) sc->breakdown = 1;
-    }
-}
-
-__global__ void k_fin_t(const double *partials, int nb, SolverScalars *sc) {
-    if (sc->done || sc->breakdown) return;
-    const double tt = reduce_slot_1024(partials, nb);
-    const double ts = reduce_slot_1024(partials + nb, nb);
-    if (threadIdx.x == 0) {
-        sc->iters += 1;
-        sc->tt = tt;
-        sc->ts = ts;
-        sc->omg = ts / tt;  // :412
-        if (fabs(sc->omg) < kTinyOmega) sc->breakdown = 1;  // :414
-        if (sc->guard && !(tt != 0.0 && isfinite(sc->omg))) sc->breakdown = 1;
-    }
-}
-
-__global__ void k_fin_r(const double *partials, int nb, SolverScalars *sc, double tol) {
-    if (sc->done || sc->breakdown) return;
-    const double rr = reduce_slot_1024(partials, nb);
-    const double rcr = reduce_slot_1024(partials + nb, nb);
-    if (threadIdx.x == 0) {
-        sc->rr = rr;
-        sc->err = sqrt(rr) / sc->sc_err;  // :423
-        sc->nrm = sc->rho;                // :425
-        sc->rho = rcr;                    // next iteration's <r,cr> (:391)
-        sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
-        sc->nonfinite = !isfinite(sc->err);
-        sc->done = (sc->err < tol) || sc->nonfinite;
-        if (sc->guard) {
-            // an inner solve that has not improved its best residual by 10%
-            // for kStall iterations has stagnated at its precision: stop and
-            // let the FP64 refinement restart it from the true residual
-            if (sc->err < 0.9 * sc->best_err) {
-                sc->best_err = sc->err;
-                sc->best_it = sc->iters;
-            } else if (sc->iters - sc->best_it >= kStall) {
-                sc->done = 1;
-            }
-        }
-    }
-}
-
 // inner solve start: residuals measured relative to ||r_inner,0||
 __global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *sc,
                                  double rtol) {
@@ -936,6 +1134,44 @@ static void first_sweep(dcm_ctx *c, const R *r, const R *v1, R *t, R *s1, R *tmp
     }
 }
 
+// the preconditioner fused into one pass (k_precond3): tile shape per
+// precision and grid depth, i planes per block
+#ifndef DCM_FUSED64
+#define DCM_FUSED64 0
+#endif
+#ifndef DCM_FUSED32
+#define DCM_FUSED32 0
+#endif
+constexpr int kPrecondPlanes = 16;
+
+template <typename R, int MODE, int KT, int JT>
+static void precond3_launch(dcm_ctx *c, const R *r, const R *v1, R *out_t, R *out,
+                            const CoefPack &cp, const SolverScalars *sc) {
+    const PGeo pg = precond_geo<KT, JT>(c->geo, kPrecondPlanes);
+    const unsigned grid = (unsigned)(((pg.nblocks + 7) / 8) * 8);
+    hipLaunchKernelGGL((k_precond3<R, MODE, KT, JT>), dim3(grid), dim3(KT * JT), 0, c->stream, r,
+                       v1, out_t, out, pg, cp, sc);
+}
+
+// MODE 0: p = M t + bet*p with t = r - bet*(omg*v1); MODE 1: s = r - alf*v1
+// and cs = M s
+template <typename R, int MODE>
+static void precond3(dcm_ctx *c, const R *r, const R *v1, R *out_t, R *out, const CoefPack &cp,
+                     const SolverScalars *sc) {
+    const bool deep = c->geo.kp > 32;
+    if constexpr (sizeof(R) == 4) {
+        if (deep)
+            precond3_launch<R, MODE, 64, 16>(c, r, v1, out_t, out, cp, sc);
+        else
+            precond3_launch<R, MODE, 32, 32>(c, r, v1, out_t, out, cp, sc);
+    } else {
+        if (deep)
+            precond3_launch<R, MODE, 64, 4>(c, r, v1, out_t, out, cp, sc);
+        else
+            precond3_launch<R, MODE, 32, 8>(c, r, v1, out_t, out, cp, sc);
+    }
+}
+
 // BiCGSTAB iterations on compact vectors of precision R.  V: r, cr, p, v1, t,
 // s, cs, two sweep buffers, x.  The initial state and scalars must be set.
 template <typename R>
@@ -952,23 +1188,31 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
     double *part = c->partials;
     const int n = (int)g.n;
     const bool seq = c->seq_sums;
+    const bool fused = sizeof(R) == 8 ? DCM_FUSED64 : DCM_FUSED32;
     const int nbv = seq ? 1 : nbm, nbu = seq ? 1 : kUpdateBlocks;
     if (itmax < 1) itmax = 1;
     int launched = 0;
     while (launched < itmax) {
         const int chunk = (itmax - launched) < kPollEvery ? (itmax - launched) : kPollEvery;
         for (int q = 0; q < chunk; ++q) {
-            int tok = prof_begin(c, "jac_first");
-            first_sweep<R, 0>(c, r, v1, t, wa, wb, m, mg, cp, sc);
-            prof_end(c, tok);
-            tok = prof_begin(c, "jac");
-            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, t, wb,
-                               m, cp, sc);
-            prof_end(c, tok);
-            tok = prof_begin(c, "jac_p");
-            hipLaunchKernelGGL((k_jac<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, wb, t, p, m,
-                               cp, sc);
-            prof_end(c, tok);
+            int tok;
+            if (fused) {
+                tok = prof_begin(c, "precond");
+                precond3<R, 0>(c, r, v1, (R *)nullptr, p, cp, sc);
+                prof_end(c, tok);
+            } else {
+                tok = prof_begin(c, "jac_first");
+                first_sweep<R, 0>(c, r, v1, t, wa, wb, m, mg, cp, sc);
+                prof_end(c, tok);
+                tok = prof_begin(c, "jac");
+                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, t,
+                                   wb, m, cp, sc);
+                prof_end(c, tok);
+                tok = prof_begin(c, "jac_p");
+                hipLaunchKernelGGL((k_jac<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, wb, t,
+                                   p, m, cp, sc);
+                prof_end(c, tok);
+            }
             tok = prof_begin(c, "matvec_dot");
             hipLaunchKernelGGL((k_matvec_dot<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, p,
                                cr, v1, m, cp, sc, part, nbm);
@@ -977,17 +1221,23 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
                 hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, cr, v1,
                                    (const R *)nullptr, (const R *)nullptr, g.n, part, sc);
             hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
-            tok = prof_begin(c, "jac_first");
-            first_sweep<R, 1>(c, r, v1, s, wa, wb, m, mg, cp, sc);
-            prof_end(c, tok);
-            tok = prof_begin(c, "jac");
-            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, s, wb,
-                               m, cp, sc);
-            prof_end(c, tok);
-            tok = prof_begin(c, "jac");
-            hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wb, s, cs,
-                               m, cp, sc);
-            prof_end(c, tok);
+            if (fused) {
+                tok = prof_begin(c, "precond");
+                precond3<R, 1>(c, r, v1, s, cs, cp, sc);
+                prof_end(c, tok);
+            } else {
+                tok = prof_begin(c, "jac_first");
+                first_sweep<R, 1>(c, r, v1, s, wa, wb, m, mg, cp, sc);
+                prof_end(c, tok);
+                tok = prof_begin(c, "jac");
+                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, s,
+                                   wb, m, cp, sc);
+                prof_end(c, tok);
+                tok = prof_begin(c, "jac");
+                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wb, s,
+                                   cs, m, cp, sc);
+                prof_end(c, tok);
+            }
             tok = prof_begin(c, "matvec_dot");
             hipLaunchKernelGGL((k_matvec_dot<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, cs,
                                s, t, m, cp, sc, part, nbm);
@@ -1306,7 +1556,7 @@ int dcm_bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, doubl
                        int *iters, int *outer, double *err) {
     if (!c || !sw || !x || !b) return DCM_BAD_ARG;
     SolverOut o;
-    int st = bicgstab_mixed(c, sw, x, b, tol, itmax > 0 ? itmax : 9999, 1e-5, &o);
+    int st = bicgstab_mixed(c, sw, x, b, tol, itmax > 0 ? itmax : 9999, 1e-4, &o);
     if (iters) *iters = o.iters;
     if (outer) *outer = o.outer;
     if (err) *err = o.err;

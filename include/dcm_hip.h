@@ -234,7 +234,7 @@ int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats);
 int dcm_model_profile(dcm_model *m, int enable);
 /* Time only the kernel group `name` (NULL: every group). */
 int dcm_model_profile_filter(dcm_model *m, const char *name);
-/* Relative residual reduction of each FP32 inner solve (mixed; default 1e-5). */
+/* Relative residual reduction of each FP32 inner solve (mixed; default 1e-4). */
 int dcm_model_set_inner_rtol(dcm_model *m, double rtol);
 /* When on, every pressure solve of the FP64 path is followed by the FP64 true
  * residual ||b-Ax||/max(||b||,0.01); the largest goes to max_true_resid

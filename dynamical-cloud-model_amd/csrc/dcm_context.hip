@@ -149,6 +149,24 @@ int prof_begin(dcm_ctx *c, const char *name) {
     return (int)c->pending.size() - 1;
 }
 
+bool prof_events(dcm_ctx *c, const char *name, hipEvent_t *start, hipEvent_t *stop) {
+    if (!c->profiling) return false;
+    if (!c->prof_only.empty() && c->prof_only != name) return false;
+    hipEvent_t a, b;
+    if (c->event_pool.size() >= 2) {
+        a = c->event_pool.back();
+        c->event_pool.pop_back();
+        b = c->event_pool.back();
+        c->event_pool.pop_back();
+    } else {
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return false;
+    }
+    c->pending.push_back({name, {a, b}});
+    *start = a;
+    *stop = b;
+    return true;
+}
+
 void prof_end(dcm_ctx *c, int token) {
     if (token < 0) return;
     (void)hipEventRecord(c->pending[token].second.second, c->stream);

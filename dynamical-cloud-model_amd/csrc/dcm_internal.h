@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -162,6 +163,20 @@ int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **ou
 // profiling bracket: returns an event pair index, or -1 when disabled
 int prof_begin(dcm_ctx *c, const char *name);
 void prof_end(dcm_ctx *c, int token);
+// events for one timed launch of `name` (false when not profiled); the
+// launch records them itself (hipExtLaunchKernelGGL), so the measured time
+// is the kernel's own execution, as a kernel trace reports it
+bool prof_events(dcm_ctx *c, const char *name, hipEvent_t *start, hipEvent_t *stop);
+
+template <typename F, typename... Args>
+inline void launch_timed(dcm_ctx *c, const char *name, F kernel, dim3 grid, dim3 block,
+                         Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof_events(c, name, &a, &b))
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, a, b, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, args...);
+}
 int prof_collect(dcm_ctx *c);
 
 // ---------------------------------------------------------------------------

@@ -1204,19 +1204,14 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
                 tok = prof_begin(c, "jac_first");
                 first_sweep<R, 0>(c, r, v1, t, wa, wb, m, mg, cp, sc);
                 prof_end(c, tok);
-                tok = prof_begin(c, "jac");
-                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, t,
-                                   wb, m, cp, sc);
-                prof_end(c, tok);
-                tok = prof_begin(c, "jac_p");
-                hipLaunchKernelGGL((k_jac<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, wb, t,
-                                   p, m, cp, sc);
-                prof_end(c, tok);
+                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wa,
+                             (const R *)t, wb, m, cp, (const SolverScalars *)sc);
+                launch_timed(c, "jac_p", k_jac<R, true>, dim3(mg), dim3(kBlock), (const R *)wb,
+                             (const R *)t, p, m, cp, (const SolverScalars *)sc);
             }
-            tok = prof_begin(c, "matvec_dot");
-            hipLaunchKernelGGL((k_matvec_dot<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, p,
-                               cr, v1, m, cp, sc, part, nbm);
-            prof_end(c, tok);
+            launch_timed(c, "matvec_dot", k_matvec_dot<R, false>, dim3(mg), dim3(kBlock),
+                         (const R *)p, (const R *)cr, v1, m, cp, (const SolverScalars *)sc, part,
+                         nbm);
             if (seq)
                 hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, cr, v1,
                                    (const R *)nullptr, (const R *)nullptr, g.n, part, sc);
@@ -1229,27 +1224,21 @@ static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, Solv
                 tok = prof_begin(c, "jac_first");
                 first_sweep<R, 1>(c, r, v1, s, wa, wb, m, mg, cp, sc);
                 prof_end(c, tok);
-                tok = prof_begin(c, "jac");
-                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wa, s,
-                                   wb, m, cp, sc);
-                prof_end(c, tok);
-                tok = prof_begin(c, "jac");
-                hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, wb, s,
-                                   cs, m, cp, sc);
-                prof_end(c, tok);
+                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wa,
+                             (const R *)s, wb, m, cp, (const SolverScalars *)sc);
+                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wb,
+                             (const R *)s, cs, m, cp, (const SolverScalars *)sc);
             }
-            tok = prof_begin(c, "matvec_dot");
-            hipLaunchKernelGGL((k_matvec_dot<R, true>), dim3(mg), dim3(kBlock), 0, c->stream, cs,
-                               s, t, m, cp, sc, part, nbm);
-            prof_end(c, tok);
+            launch_timed(c, "matvec_dot", k_matvec_dot<R, true>, dim3(mg), dim3(kBlock),
+                         (const R *)cs, (const R *)s, t, m, cp, (const SolverScalars *)sc, part,
+                         nbm);
             if (seq)
                 hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, t, t, t, s,
                                    g.n, part, sc);
             hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
-            tok = prof_begin(c, "update");
-            hipLaunchKernelGGL(k_update<R>, dim3(kUpdateBlocks), dim3(kBlock), 0, c->stream, x, p,
-                               cs, s, t, cr, r, n, sc, part, kUpdateBlocks);
-            prof_end(c, tok);
+            launch_timed(c, "update", k_update<R>, dim3(kUpdateBlocks), dim3(kBlock), x,
+                         (const R *)p, (const R *)cs, (const R *)s, (const R *)t, (const R *)cr, r,
+                         n, (const SolverScalars *)sc, part, (int)kUpdateBlocks);
             if (seq)
                 hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, r, r, r, cr,
                                    g.n, part, sc);

@@ -83,6 +83,23 @@ int ensure_solver_vectors64(dcm_ctx *c, size_t count) {
     return DCM_OK;
 }
 
+int ensure_host_flags(dcm_ctx *c, size_t n, int **out) {
+    if (c->h_flags_cap < n) {
+        if (c->h_flags) {
+            DCM_HIP(hipStreamSynchronize(c->stream));
+            DCM_HIP(hipHostFree(c->h_flags));
+            c->h_flags = nullptr;
+            c->h_flags_cap = 0;
+        }
+        void *p = nullptr;
+        DCM_HIP(hipHostMalloc(&p, n * sizeof(int)));
+        c->h_flags = (int *)p;
+        c->h_flags_cap = n;
+    }
+    *out = c->h_flags;
+    return DCM_OK;
+}
+
 int ensure_solver_vectors32(dcm_ctx *c, size_t count) {
     if (c->sv32_count >= count) return DCM_OK;
     if (c->sv32) {
@@ -340,7 +357,10 @@ int dcm_context_destroy(dcm_ctx *c) {
         (void)hipEventDestroy(p.second.second);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
+    c->solve_tables.release();
+    c->flags.release();
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (!c->borrowed_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DCM_OK;
 }

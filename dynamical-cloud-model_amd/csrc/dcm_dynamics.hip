@@ -620,9 +620,9 @@ int dcm_model_init(dcm_model *m) {
 }
 
 // one pass of the model_driver loop, driver_code.f90:304-676 (hot path)
-static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
+// model_driver up to the pressure solve (driver_code.f90:433-487)
+static int model_pre_solve(dcm_model *m) {
     dcm_ctx *c = m->c;
-    const Geo &g = c->geo;
     double *cur[3], *prev[3], *fut[3];
     m->levels(cur, prev, fut);
     // sth = 0 (:340) stays zero: sources' inviscid branch only fills it, and
@@ -632,28 +632,26 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
                           m->sv, m->sw, nullptr, m->psrc));          // :455-479
     DCM_CHECK(fill<double>(c, m->psrc, FILL_ALONG_DIM_WO, 0.0, 0.0)); // :485-487
-    SolverOut so;
+    return DCM_OK;
+}
+
+static void add_stats(dcm_step_stats *stats, const SolverOut &so, double true_resid) {
+    if (!stats) return;
+    stats->iters_total += so.iters;
+    if (so.iters > stats->iters_max) stats->iters_max = so.iters;
+    stats->outer_total += so.outer;
+    stats->fallbacks += so.fallback;
+    stats->last_err = so.err;
+    if (true_resid > stats->max_true_resid) stats->max_true_resid = true_resid;
+}
+
+// model_driver after the pressure solve (driver_code.f90:506-674)
+static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
+    dcm_ctx *c = m->c;
+    const Geo &g = c->geo;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
     int st;
-    if (m->precision == DCM_MIXED)
-        st = bicgstab_mixed(c, m->sw, m->p, m->psrc, m->ptol, 9999, m->inner_rtol, &so);
-    else {
-        c->first_solve = false;
-        st = bicgstab64(c, m->sw, m->p, m->psrc, m->ptol, 9999, &so);  // :496-500
-    }
-    if (st != DCM_OK) return st;
-    double true_resid = 0.0;
-    if (m->precision == DCM_MIXED)
-        true_resid = so.err;  // the refinement loop's exit test is the true residual
-    else if (m->check_residual)
-        DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
-    if (stats) {
-        stats->iters_total += so.iters;
-        if (so.iters > stats->iters_max) stats->iters_max = so.iters;
-        stats->outer_total += so.outer;
-        stats->fallbacks += so.fallback;
-        stats->last_err = so.err;
-        if (true_resid > stats->max_true_resid) stats->max_true_resid = true_resid;
-    }
     // :506-507 exchange_along_dim_wo(p): the solver already left the faces
     // periodic, so the fill is an exact no-op here.
     if (m->advection_scheme == 0 || m->advection_scheme == 1) {
@@ -713,6 +711,27 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     return DCM_OK;
 }
 
+static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
+    dcm_ctx *c = m->c;
+    DCM_CHECK(model_pre_solve(m));
+    SolverOut so;
+    int st;
+    if (m->precision == DCM_MIXED)
+        st = bicgstab_mixed(c, m->sw, m->p, m->psrc, m->ptol, 9999, m->inner_rtol, &so);
+    else {
+        c->first_solve = false;
+        st = bicgstab64(c, m->sw, m->p, m->psrc, m->ptol, 9999, &so);  // :496-500
+    }
+    if (st != DCM_OK) return st;
+    double true_resid = 0.0;
+    if (m->precision == DCM_MIXED)
+        true_resid = so.err;  // the refinement loop's exit test is the true residual
+    else if (m->check_residual)
+        DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
+    add_stats(stats, so, true_resid);
+    return model_post_solve(m, stats);
+}
+
 int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {
     if (!m || nsteps < 0) return DCM_BAD_ARG;
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -745,6 +764,163 @@ int dcm_model_set_inner_rtol(dcm_model *m, double rtol) {
 int dcm_model_set_check_residual(dcm_model *m, int on) {
     if (!m) return DCM_BAD_ARG;
     m->check_residual = on != 0;
+    return DCM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ensembles: independent models of uneven size stepped together.  Every
+// member's kernels go to one stream; the pressure solves of all members run
+// in one batched BiCGSTAB (each launch serves every member that has not
+// converged), so the iteration loop costs one set of launches per iteration
+// for the whole ensemble.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct dcm_ensemble {
+    std::vector<dcm_model *> mem;
+    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> saved;  // the members' own streams
+    int precision = DCM_FP64;
+    double ptol = 1e-10;
+    int march_planes = 0;
+};
+
+namespace dcm {
+
+static int ensemble_solve(dcm_ensemble *e, std::vector<SolverOut> &outs) {
+    const int n = (int)e->mem.size();
+    std::vector<SolveTask> tasks(n);
+    for (int q = 0; q < n; ++q) {
+        dcm_model *m = e->mem[q];
+        tasks[q] = SolveTask{m->c, m->sw, m->p, m->psrc};
+        m->c->first_solve = false;
+    }
+    std::vector<int> st(n, DCM_OK);
+    outs.assign(n, SolverOut());
+    dcm_ctx *c0 = e->mem[0]->c;
+    int r;
+    if (e->precision == DCM_MIXED)
+        r = bicgstab_mixed_batch(c0, tasks.data(), n, e->ptol, 9999, e->mem[0]->inner_rtol,
+                                 outs.data(), st.data());
+    else
+        r = bicgstab64_batch(c0, tasks.data(), n, e->ptol, 9999, outs.data(), st.data());
+    return r;
+}
+
+// Time a few batched iterations of the current systems at each candidate
+// plane count and keep the fastest.  Only solver scratch is touched: the
+// iterate and residual vectors are rebuilt from p and psrc by every solve.
+static int ensemble_autotune(dcm_ensemble *e, int *best_out) {
+    static const int cand[] = {4, 8, 16};
+    const int n = (int)e->mem.size();
+    dcm_ctx *c0 = e->mem[0]->c;
+    hipEvent_t a, b;
+    DCM_HIP(hipEventCreate(&a));
+    DCM_HIP(hipEventCreate(&b));
+    float best_ms = 1e30f;
+    int best = 8;
+    std::vector<SolveTask> tasks(n);
+    for (int q = 0; q < n; ++q)
+        tasks[q] = SolveTask{e->mem[q]->c, e->mem[q]->sw, e->mem[q]->p, e->mem[q]->psrc};
+    for (int ci = 0; ci < 3; ++ci) {
+        for (int q = 0; q < n; ++q) e->mem[q]->c->march_planes = cand[ci];
+        std::vector<SolverOut> o(n);
+        std::vector<int> st(n);
+        DCM_HIP(hipEventRecord(a, e->stream));
+        DCM_CHECK(bicgstab_probe(c0, tasks.data(), n, 8, o.data(), st.data()));
+        DCM_HIP(hipEventRecord(b, e->stream));
+        DCM_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        DCM_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cand[ci];
+        }
+    }
+    for (int q = 0; q < n; ++q) e->mem[q]->c->march_planes = best;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->march_planes = best;
+    if (best_out) *best_out = best;
+    return DCM_OK;
+}
+
+}  // namespace dcm
+
+extern "C" {
+
+int dcm_ensemble_create(int n, dcm_model *const *models, dcm_ensemble **out) {
+    if (!out || n < 1 || !models) return DCM_BAD_ARG;
+    for (int q = 0; q < n; ++q) {
+        if (!models[q]) return DCM_BAD_ARG;
+        for (int u = 0; u < q; ++u)
+            if (models[u] == models[q] || models[u]->c == models[q]->c) {
+                set_error("dcm_ensemble_create: members need their own models and contexts");
+                return DCM_BAD_ARG;
+            }
+        if (models[q]->precision != models[0]->precision || models[q]->ptol != models[0]->ptol) {
+            set_error("dcm_ensemble_create: members must share precision and ptol");
+            return DCM_BAD_ARG;
+        }
+    }
+    dcm_ensemble *e = new dcm_ensemble();
+    e->precision = models[0]->precision;
+    e->ptol = models[0]->ptol;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        set_error("dcm_ensemble_create: hipStreamCreate failed");
+        return DCM_HIP_ERROR;
+    }
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *c = models[q]->c;
+        (void)hipStreamSynchronize(c->stream);
+        e->saved.push_back(c->stream);
+        c->stream = e->stream;
+        c->borrowed_stream = true;
+        e->mem.push_back(models[q]);
+    }
+    *out = e;
+    return DCM_OK;
+}
+
+int dcm_ensemble_destroy(dcm_ensemble *e) {
+    if (!e) return DCM_BAD_ARG;
+    (void)hipStreamSynchronize(e->stream);
+    for (size_t q = 0; q < e->mem.size(); ++q) {
+        dcm_ctx *c = e->mem[q]->c;
+        c->stream = e->saved[q];
+        c->borrowed_stream = false;
+    }
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return DCM_OK;
+}
+
+int dcm_ensemble_autotune(dcm_ensemble *e, int *march_planes) {
+    if (!e) return DCM_BAD_ARG;
+    return ensemble_autotune(e, march_planes);
+}
+
+int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
+    if (!e || nsteps < 0) return DCM_BAD_ARG;
+    const int n = (int)e->mem.size();
+    if (stats) std::memset(stats, 0, sizeof(dcm_step_stats) * n);
+    std::vector<SolverOut> outs;
+    for (int s = 0; s < nsteps; ++s) {
+        for (int q = 0; q < n; ++q) DCM_CHECK(model_pre_solve(e->mem[q]));
+        DCM_CHECK(ensemble_solve(e, outs));
+        for (int q = 0; q < n; ++q) {
+            dcm_model *m = e->mem[q];
+            double true_resid = 0.0;
+            if (m->precision == DCM_MIXED)
+                true_resid = outs[q].err;
+            else if (m->check_residual)
+                DCM_CHECK(dcm_true_residual(m->c, m->p, m->psrc, &true_resid));
+            add_stats(stats ? stats + q : nullptr, outs[q], true_resid);
+            DCM_CHECK(model_post_solve(m, stats ? stats + q : nullptr));
+        }
+    }
+    DCM_HIP(hipStreamSynchronize(e->stream));
     return DCM_OK;
 }
 

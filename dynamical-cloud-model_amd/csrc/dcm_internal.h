@@ -85,6 +85,7 @@ struct SolverScalars {
                        // ends the iteration as a breakdown, before the update
     int best_it;       // inner solve: iteration of the best residual so far
     double best_err;
+    double tol;        // convergence test of this solve (err < tol)
 };
 
 // Per-kernel timing (profiling mode): accumulated over launches.
@@ -99,6 +100,31 @@ struct Mirror {
 };
 
 // Result of one pressure solve.
+// A grow-only device buffer, freed with its owner.
+struct DeviceScratch {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes, void **out) {
+        if (cap < bytes) {
+            if (ptr) {
+                (void)hipDeviceSynchronize();
+                (void)hipFree(ptr);
+                ptr = nullptr;
+                cap = 0;
+            }
+            if (hipMalloc(&ptr, bytes) != hipSuccess) return DCM_OUT_OF_MEMORY;
+            cap = bytes;
+        }
+        *out = ptr;
+        return DCM_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
 struct SolverOut {
     int iters = 0;   // BiCGSTAB iterations (inner iterations for mixed)
     int outer = 0;   // refinement sweeps (mixed)
@@ -140,6 +166,15 @@ struct dcm_ctx {
     // global sums in the reference's sequential loop order (one thread per
     // sum) instead of the two-level tree: bit-identical to a serial CPU run
     bool seq_sums = false;
+    // i planes per marching block of the solver (0: the built-in default)
+    int march_planes = 0;
+    // device tables of a batched solve and the members' flags (device and
+    // pinned host)
+    dcm::DeviceScratch solve_tables, flags;
+    int *h_flags = nullptr;
+    size_t h_flags_cap = 0;
+    // the stream is borrowed (an ensemble's), not owned
+    bool borrowed_stream = false;
     std::unordered_map<const void *, dcm::Mirror> mirrors;
     // profiling
     bool profiling = false;
@@ -159,6 +194,7 @@ int work_field32(dcm_ctx *c, int slot, float **out);    // padded fp32 scratch
 int ensure_partials(dcm_ctx *c, size_t n);
 int ensure_solver_vectors64(dcm_ctx *c, size_t count);
 int ensure_solver_vectors32(dcm_ctx *c, size_t count);
+int ensure_host_flags(dcm_ctx *c, size_t n, int **out);  // pinned ints
 int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **out);
 // profiling bracket: returns an event pair index, or -1 when disabled
 int prof_begin(dcm_ctx *c, const char *name);
@@ -233,6 +269,19 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
 int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
                     float *vf, float *wf);
 int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref_dev);
+// one pressure system of a batched solve: its context (geometry,
+// coefficients, scalars, work vectors) and its padded sw, x, b
+struct SolveTask {
+    dcm_ctx *c;
+    const double *sw;
+    double *x, *b;
+};
+int bicgstab64_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, int itmax,
+                     SolverOut *outs, int *status);
+int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, int itmax,
+                         double inner_rtol, SolverOut *outs, int *status);
+int bicgstab_probe(dcm_ctx *c, const SolveTask *tasks, int n, int iters, SolverOut *outs,
+                   int *status);
 int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
                SolverOut *out);
 int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,

@@ -162,8 +162,7 @@ struct Tile {
     int k, j, i0, i1, tile;
     bool act;
 };
-__device__ __forceinline__ bool march_tile(const MG &m, Tile &T) {
-    const int b = blockIdx.x;
+__device__ __forceinline__ bool march_tile(const MG &m, int b, Tile &T) {
     const int per = (m.nblocks + 7) >> 3;
     T.tile = (b & 7) * per + (b >> 3);
     if (T.tile >= m.nblocks) return false;
@@ -178,6 +177,43 @@ __device__ __forceinline__ bool march_tile(const MG &m, Tile &T) {
     if (T.i1 > m.ip) T.i1 = m.ip;
     T.act = T.k <= m.kp && T.j <= m.jp;
     return true;
+}
+
+// ---------------------------------------------------------------------------
+// Solve members.  The iteration kernels serve any number of independent
+// systems (one domain, or the members of an ensemble) in one launch: a block
+// finds its member in a table and works on that member's vectors with that
+// member's geometry, coefficients and scalars.  Members own disjoint,
+// 8-aligned ranges of the marching grid (so the XCD renumbering of a member
+// is unchanged) and of the update grid; the one-block reductions run one
+// block per member.  With one member the block maps are not needed.
+// ---------------------------------------------------------------------------
+struct IterMember {
+    MG m;
+    CoefPack cp;
+    SolverScalars *sc;
+    double *part;     // partial sums: 2 slots of max(nblocks, nbu)
+    void *V[10];      // r, cr, p, v1, t, s, cs, wa, wb, x (compact, precision R)
+    int n, nbu;       // cells; update-kernel blocks
+    int mblock0, ublock0;
+};
+enum { VR = 0, VCR, VP, VV1, VT, VS, VCS, VWA, VWB, VX };
+
+struct IterArgs {
+    const IterMember *mem;
+    const int *mmap;  // marching block -> member (null with one member)
+    const int *umap;  // update block -> member (null with one member)
+};
+
+__device__ __forceinline__ const IterMember &member_m(const IterArgs &A, int &lb) {
+    const IterMember &M = A.mem[A.mmap ? A.mmap[blockIdx.x] : 0];
+    lb = (int)blockIdx.x - M.mblock0;
+    return M;
+}
+__device__ __forceinline__ const IterMember &member_u(const IterArgs &A, int &lb) {
+    const IterMember &M = A.mem[A.umap ? A.umap[blockIdx.x] : 0];
+    lb = (int)blockIdx.x - M.ublock0;
+    return M;
 }
 
 // column offsets of a tile cell: base (i = 1) and the j/k neighbour deltas
@@ -266,18 +302,21 @@ __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x
 // null).
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void k_seq_dots(const R *__restrict__ a0, const R *__restrict__ b0,
-                           const R *__restrict__ a1, const R *__restrict__ b1, long long n,
-                           double *out, const SolverScalars *__restrict__ sc) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (sc && (sc->done || sc->breakdown)) return;
+__global__ void k_seq_dots(IterArgs A, int a0i, int b0i, int a1i, int b1i) {
+    const IterMember &M = A.mem[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    const SolverScalars *sc = M.sc;
+    if (sc->done || sc->breakdown) return;
+    const R *a0 = (const R *)M.V[a0i], *b0 = (const R *)M.V[b0i];
+    const R *a1 = a1i >= 0 ? (const R *)M.V[a1i] : nullptr;
+    const R *b1 = b1i >= 0 ? (const R *)M.V[b1i] : nullptr;
     double s0 = 0.0, s1 = 0.0;
-    for (long long q = 0; q < n; ++q) {
+    for (long long q = 0; q < M.n; ++q) {
         s0 = s0 + (double)a0[q] * (double)b0[q];
         if (a1) s1 = s1 + (double)a1[q] * (double)b1[q];
     }
-    out[0] = s0;
-    out[1] = s1;
+    M.part[0] = s0;
+    M.part[1] = s1;
 }
 
 // <b,b> over the interior of a padded b and <r,r> of the compact r
@@ -338,6 +377,7 @@ __global__ void k_fin_init(const double *partials, int nb, SolverScalars *sc, do
         sc->breakdown = 0;
         sc->iters = 0;
         sc->guard = 0;
+        sc->tol = tol;
         sc->nonfinite = !isfinite(sc->err);
         sc->done = (sc->err < tol) || sc->nonfinite;
     }
@@ -359,15 +399,14 @@ __device__ __forceinline__ void fin_t_apply(SolverScalars *sc, double tt, double
     if (fabs(sc->omg) < kTinyOmega) sc->breakdown = 1;  // :414
     if (sc->guard && !(tt != 0.0 && isfinite(sc->omg))) sc->breakdown = 1;
 }
-__device__ __forceinline__ void fin_r_apply(SolverScalars *sc, double rr, double rcr,
-                                            double tol) {
+__device__ __forceinline__ void fin_r_apply(SolverScalars *sc, double rr, double rcr) {
     sc->rr = rr;
     sc->err = sqrt(rr) / sc->sc_err;  // :423
     sc->nrm = sc->rho;                // :425
     sc->rho = rcr;                    // next iteration's <r,cr> (:391)
     sc->bet = (sc->rho / sc->nrm) * (sc->alf / sc->omg);
     sc->nonfinite = !isfinite(sc->err);
-    sc->done = (sc->err < tol) || sc->nonfinite;
+    sc->done = (sc->err < sc->tol) || sc->nonfinite;
     if (sc->guard) {
         // an inner solve that has not improved its best residual by 10% for
         // kStall iterations has stagnated at its precision: stop and let the
@@ -381,22 +420,59 @@ __device__ __forceinline__ void fin_r_apply(SolverScalars *sc, double rr, double
     }
 }
 
-__global__ void k_fin_v(const double *partials, int nb, SolverScalars *sc) {
+// nb: the member's partial count (its tiles, or 1 after sequential sums)
+__device__ __forceinline__ int fin_count(const IterMember &M, int kind, int seq) {
+    if (seq) return 1;
+    return kind == 0 ? M.m.nblocks : M.nbu;
+}
+__global__ void k_fin_v1(const double *partials, int nb, SolverScalars *sc) {
     if (sc->done || sc->breakdown) return;
     const double nrm = reduce_slot_1024(partials, nb);
     if (threadIdx.x == 0) fin_v_apply(sc, nrm);
 }
-__global__ void k_fin_t(const double *partials, int nb, SolverScalars *sc) {
+__global__ void k_fin_t1(const double *partials, int nb, SolverScalars *sc) {
     if (sc->done || sc->breakdown) return;
     const double tt = reduce_slot_1024(partials, nb);
     const double ts = reduce_slot_1024(partials + nb, nb);
     if (threadIdx.x == 0) fin_t_apply(sc, tt, ts);
 }
-__global__ void k_fin_r(const double *partials, int nb, SolverScalars *sc, double tol) {
+__global__ void k_fin_r1(const double *partials, int nb, SolverScalars *sc) {
     if (sc->done || sc->breakdown) return;
     const double rr = reduce_slot_1024(partials, nb);
     const double rcr = reduce_slot_1024(partials + nb, nb);
-    if (threadIdx.x == 0) fin_r_apply(sc, rr, rcr, tol);
+    if (threadIdx.x == 0) fin_r_apply(sc, rr, rcr);
+}
+__global__ void k_fin_v(IterArgs A, int seq) {
+    const IterMember &M = A.mem[blockIdx.x];
+    SolverScalars *sc = M.sc;
+    if (sc->done || sc->breakdown) return;
+    const double nrm = reduce_slot_1024(M.part, fin_count(M, 0, seq));
+    if (threadIdx.x == 0) fin_v_apply(sc, nrm);
+}
+__global__ void k_fin_t(IterArgs A, int seq) {
+    const IterMember &M = A.mem[blockIdx.x];
+    SolverScalars *sc = M.sc;
+    if (sc->done || sc->breakdown) return;
+    const int nb = fin_count(M, 0, seq);
+    const double tt = reduce_slot_1024(M.part, nb);
+    const double ts = reduce_slot_1024(M.part + nb, nb);
+    if (threadIdx.x == 0) fin_t_apply(sc, tt, ts);
+}
+__global__ void k_fin_r(IterArgs A, int seq) {
+    const IterMember &M = A.mem[blockIdx.x];
+    SolverScalars *sc = M.sc;
+    if (sc->done || sc->breakdown) return;
+    const int nb = fin_count(M, 1, seq);
+    const double rr = reduce_slot_1024(M.part, nb);
+    const double rcr = reduce_slot_1024(M.part + nb, nb);
+    if (threadIdx.x == 0) fin_r_apply(sc, rr, rcr);
+}
+// flags of every member for the host: done | breakdown << 1 | nonfinite << 2
+__global__ void k_gather_flags(IterArgs A, int nmem, int *flags) {
+    for (int q = threadIdx.x; q < nmem; q += blockDim.x) {
+        const SolverScalars *sc = A.mem[q].sc;
+        flags[q] = (sc->done ? 1 : 0) | (sc->breakdown ? 2 : 0) | (sc->nonfinite ? 4 : 0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -414,14 +490,14 @@ __device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__
 // First sweep, variant 0: fused in registers, s0 = t/a_p evaluated at the
 // centre and the six neighbours (dynamics.f90:254-266).
 template <typename R, int MODE>
-__global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
+__device__ __forceinline__ void jac_first_reg_body(const R *__restrict__ r,
                                                    const R *__restrict__ v1,
                                                    R *__restrict__ t_out, R *__restrict__ s_out,
-                                                   MG m, CoefPack cp,
-                                                   const SolverScalars *__restrict__ sc) {
+                                                   const MG &m, const CoefPack &cp,
+                                                   const SolverScalars *__restrict__ sc, int lb) {
     if (sc->done || sc->breakdown) return;
     Tile T;
-    if (!march_tile(m, T) || !T.act) return;
+    if (!march_tile(m, lb, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
@@ -458,17 +534,33 @@ __global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
     }
 }
 
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
+                                                       const R *__restrict__ v1,
+                                                       R *__restrict__ t_out,
+                                                       R *__restrict__ s_out, MG m, CoefPack cp,
+                                                       const SolverScalars *__restrict__ sc) {
+    jac_first_reg_body<R, MODE>(r, v1, t_out, s_out, m, cp, sc, (int)blockIdx.x);
+}
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_jac_first_reg_b(IterArgs A, int ri, int vi, int ti,
+                                                         int oi) {
+    int lb;
+    const IterMember &M = member_m(A, lb);
+    jac_first_reg_body<R, MODE>((const R *)M.V[ri], (const R *)M.V[vi], (R *)M.V[ti],
+                                (R *)M.V[oi], M.m, M.cp, M.sc, lb);
+}
+
 // First sweep, variant 2: a streaming pass t = r - bet*(omg*v1) (or
 // r - alf*v1) with s0 = t/a_p, then the plain sweep k_jac on (s0, t).
 template <typename R, int MODE>
-__global__ __launch_bounds__(256) void k_jac_prep(const R *__restrict__ r,
-                                                  const R *__restrict__ v1,
-                                                  R *__restrict__ t_out, R *__restrict__ s0_out,
-                                                  MG m, CoefPack cp,
-                                                  const SolverScalars *__restrict__ sc) {
+__device__ __forceinline__ void jac_prep_body(const R *__restrict__ r, const R *__restrict__ v1,
+                                              R *__restrict__ t_out, R *__restrict__ s0_out,
+                                              const MG &m, const CoefPack &cp,
+                                              const SolverScalars *__restrict__ sc, int lb) {
     if (sc->done || sc->breakdown) return;
     Tile T;
-    if (!march_tile(m, T) || !T.act) return;
+    if (!march_tile(m, lb, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
@@ -482,139 +574,31 @@ __global__ __launch_bounds__(256) void k_jac_prep(const R *__restrict__ r,
     }
 }
 
-// First sweep, variant 1: sweep 1 with its input t: s1 = s0 + (t - A s0)/a_p with s0 = t/a_p
-// (dynamics.f90:254-266).  s0 is computed once per cell and plane and shared
-// through LDS: each plane of the tile plus a one-cell halo in j and k (the
-// 7-point stencil needs no corners) lives in one of three LDS buffers, so one
-// barrier per plane separates a buffer's writes from its reads.  Every
-// thread takes part (threads past the edge of the grid evaluate the wrapped
-// j / clamped k cell, which is valid memory), only active ones store.  The
-// raw loads of plane i+2 are issued while plane i is computed.
-constexpr int kLdsCells = 396;  // max (jt+2)*(kt+2) over the tile shapes: 6*66
-
 template <typename R, int MODE>
-__global__ __launch_bounds__(256) void k_jac_first(const R *__restrict__ r,
-                                                   const R *__restrict__ v1,
-                                                   R *__restrict__ t_out, R *__restrict__ s_out,
-                                                   MG m, CoefPack cp,
-                                                   const SolverScalars *__restrict__ sc) {
-    if (sc->done || sc->breakdown) return;
-    Tile T;
-    if (!march_tile(m, T)) return;
-    __shared__ R lds[3 * kLdsCells];
-    const Coef<R> C(cp.m64, cp.m32);
-    const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
-    const R bo = (R)sc->omg;
-    const int tid = (int)threadIdx.x;
-    const int kl = tid % m.kt, jl = tid / m.kt;
-    const int k = T.k, j = T.j;
-    const int k0 = k - kl, j0 = j - jl;
-    const int kc = k > m.kp ? m.kp : k;
-    const int jw = (j - 1) % m.jp + 1;
-    const int col = (jw - 1) * m.sj + (kc - 1);
-    const int W = m.kt + 2;  // LDS row length
-    // halo cell of this thread, if any: rows j0-1, j0+jt (one per k lane),
-    // then columns k0-1, k0+kt (one per j row)
-    bool halo = false;
-    int hk = 1, hj = 1, hslot = 0;
-    if (tid < 2 * m.kt) {
-        const int side = tid / m.kt, kk = tid % m.kt;
-        int jj = side ? j0 + m.jt : j0 - 1;
-        jj = ((jj - 1) % m.jp + m.jp) % m.jp + 1;
-        hk = k0 + kk > m.kp ? m.kp : k0 + kk;
-        hj = jj;
-        hslot = (side ? m.jt + 1 : 0) * W + kk + 1;
-        halo = true;
-    } else if (tid < 2 * m.kt + 2 * m.jt) {
-        const int u = tid - 2 * m.kt, side = u / m.jt, jj0 = u % m.jt;
-        int kk = side ? k0 + m.kt : k0 - 1;
-        kk = kk < 1 ? 1 : (kk > m.kp ? m.kp : kk);
-        hk = kk;
-        hj = (j0 + jj0 - 1) % m.jp + 1;
-        hslot = (jj0 + 1) * W + (side ? m.kt + 1 : 0);
-        halo = true;
-    }
-    const int hcol = (hj - 1) * m.sj + (hk - 1);
-    const int oslot = (jl + 1) * W + kl + 1;
-    const R as = C.as[jw], an = C.an[jw], ad = C.ad[kc], au = C.au[kc];
-
-    auto tv = [&](R rv, R vv) -> R {
-        if (MODE == 0) return rv - a * (bo * vv);
-        return rv - a * vv;
-    };
-    auto nexti = [&](int i) { return i == m.ip ? 1 : i + 1; };
-
-    // plane i0-1 (own cell) and plane i0 (own cell and halo)
-    const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
-    R api = (-C.ae[iw]) - C.aw[iw];
-    int q = (iw - 1) * m.si;
-    R s0w = div_ap<R>(tv(r[q + col], v1[q + col]), coef_ap_i(C, api, kc, jw));
-    api = (-C.ae[T.i0]) - C.aw[T.i0];
-    q = (T.i0 - 1) * m.si;
-    R tc = tv(r[q + col], v1[q + col]);
-    R s0c = div_ap<R>(tc, coef_ap_i(C, api, kc, jw));
-    lds[(T.i0 % 3) * kLdsCells + oslot] = s0c;
-    if (halo)
-        lds[(T.i0 % 3) * kLdsCells + hslot] =
-            div_ap<R>(tv(r[q + hcol], v1[q + hcol]), coef_ap_i(C, api, hk, hj));
-    // raw loads of plane i0+1
-    int ie = nexti(T.i0);
-    q = (ie - 1) * m.si;
-    R r_o = r[q + col], v_o = v1[q + col];
-    R r_h = halo ? r[q + hcol] : R(0), v_h = halo ? v1[q + hcol] : R(0);
-    for (int i = T.i0; i <= T.i1; ++i) {
-        // plane i+1 from the loads in flight
-        const R api_e = (-C.ae[ie]) - C.aw[ie];
-        const R te = tv(r_o, v_o);
-        const R s0e = div_ap<R>(te, coef_ap_i(C, api_e, kc, jw));
-        R hval = R(0);
-        if (halo && i < T.i1) hval = div_ap<R>(tv(r_h, v_h), coef_ap_i(C, api_e, hk, hj));
-        // loads of plane i+2
-        const int iee = nexti(ie);
-        if (i < T.i1) {
-            const int qn = (iee - 1) * m.si;
-            r_o = r[qn + col];
-            v_o = v1[qn + col];
-            if (halo) {
-                r_h = r[qn + hcol];
-                v_h = v1[qn + hcol];
-            }
-        }
-        if (i < T.i1) {
-            const int b = ((i + 1) % 3) * kLdsCells;
-            lds[b + oslot] = s0e;
-            if (halo) lds[b + hslot] = hval;
-        }
-        __syncthreads();
-        const int b0 = (i % 3) * kLdsCells;
-        const R s0s = lds[b0 + oslot - W], s0n = lds[b0 + oslot + W];
-        const R s0d = k == 1 ? s0c : lds[b0 + oslot - 1];
-        const R s0u = k >= m.kp ? s0c : lds[b0 + oslot + 1];
-        if (T.act) {
-            const R ap = coef_ap_i(C, api, k, j);
-            const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n,
-                                    s0d, s0u, s0c);
-            const int cc = (i - 1) * m.si + col;
-            t_out[cc] = tc;
-            s_out[cc] = s0c + div_ap<R>(tc - As, ap);
-        }
-        s0w = s0c;
-        s0c = s0e;
-        tc = te;
-        api = api_e;
-        ie = iee;
-    }
+__global__ __launch_bounds__(256) void k_jac_prep(const R *__restrict__ r,
+                                                  const R *__restrict__ v1,
+                                                  R *__restrict__ t_out, R *__restrict__ s0_out,
+                                                  MG m, CoefPack cp,
+                                                  const SolverScalars *__restrict__ sc) {
+    jac_prep_body<R, MODE>(r, v1, t_out, s0_out, m, cp, sc, (int)blockIdx.x);
+}
+template <typename R, int MODE>
+__global__ __launch_bounds__(256) void k_jac_prep_b(IterArgs A, int ri, int vi, int ti, int oi) {
+    int lb;
+    const IterMember &M = member_m(A, lb);
+    jac_prep_body<R, MODE>((const R *)M.V[ri], (const R *)M.V[vi], (R *)M.V[ti], (R *)M.V[oi],
+                           M.m, M.cp, M.sc, lb);
 }
 
 // s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead.  The
 // loads of plane i+1 are issued while plane i is computed.
 template <typename R, bool P_UPDATE>
-__global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
-                                             R *__restrict__ s_out, MG m, CoefPack cp,
-                                             const SolverScalars *__restrict__ sc) {
+__device__ __forceinline__ void jac_body(const R *__restrict__ s, const R *__restrict__ t,
+                                         R *__restrict__ s_out, const MG &m, const CoefPack &cp,
+                                         const SolverScalars *__restrict__ sc, int lb) {
     if (sc->done || sc->breakdown) return;
     Tile T;
-    if (!march_tile(m, T) || !T.act) return;
+    if (!march_tile(m, lb, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const int k = T.k, j = T.j;
     const Cols q = cols(m, k, j);
@@ -665,20 +649,35 @@ __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *_
     }
 }
 
+template <typename R, bool P_UPDATE>
+__global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
+                                             R *__restrict__ s_out, MG m, CoefPack cp,
+                                             const SolverScalars *__restrict__ sc) {
+    jac_body<R, P_UPDATE>(s, t, s_out, m, cp, sc, (int)blockIdx.x);
+}
+template <typename R, bool P_UPDATE>
+__global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int oi) {
+    int lb;
+    const IterMember &M = member_m(A, lb);
+    jac_body<R, P_UPDATE>((const R *)M.V[si_], (const R *)M.V[ti], (R *)M.V[oi], M.m, M.cp, M.sc,
+                          lb);
+}
+
 // y = A x; partial sums of <a,y> (and <y,y> first when TWO).  Loads of plane
 // i+1 in flight while plane i is computed.  (A last-arriving-block reduction
 // that applies the scalar recurrence in place was measured 1.6x slower
 // overall: every block's device-scope release fence writes back its XCD's
 // L2.  One-block kernels reduce instead.)
 template <typename R, bool TWO>
-__global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
-                                                    const R *__restrict__ a, R *__restrict__ y,
-                                                    MG m, CoefPack cp,
-                                                    const SolverScalars *__restrict__ sc,
-                                                    double *partials, int stride) {
+__device__ __forceinline__ void matvec_dot_body(const R *__restrict__ x, const R *__restrict__ a,
+                                                R *__restrict__ y, const MG &m,
+                                                const CoefPack &cp,
+                                                const SolverScalars *__restrict__ sc,
+                                                double *partials, int lb) {
     if (sc->done || sc->breakdown) return;
+    const int stride = m.nblocks;
     Tile T;
-    if (!march_tile(m, T)) return;
+    if (!march_tile(m, lb, T)) return;
     double acc[2] = {0.0, 0.0};
     if (T.act) {
         const Coef<R> C(cp.m64, cp.m32);
@@ -732,213 +731,62 @@ __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
     block_sum_store_at<2>(acc, partials, stride, T.tile);
 }
 
-// ---------------------------------------------------------------------------
-// The whole preconditioner in one pass (temporal blocking of precondition,
-// dynamics.f90:241-270, prit = 3): t = r - bet*(omg*v1) (MODE 0) or
-// r - alf*v1 (MODE 1), s0 = t/a_p, three Jacobi sweeps, then p = s3 + bet*p
-// (MODE 0) or cs = s3 with s = t stored (MODE 1).  HBM sees r and v1 once
-// and the outputs once; the intermediate sweeps live in LDS.
-//
-// A block owns a KT x JT tile of (k,j) columns and marches IC i-planes.  At
-// step n it evaluates level 0 on plane n+3, level 1 on n+2, level 2 on n+1
-// and the output level on n, each over the tile grown by the halo that the
-// next level's 7-point stencil needs (3, 2, 1, 0 cells in j and k).  Level L
-// keeps three planes in an LDS ring, t keeps four, all in one layout of
-// (JT+6) x (KT+6) cells.  Cells outside [1,kp] in k are never read (the
-// Neumann copy makes the k-neighbour of k=1 and k=kp the cell itself); cells
-// outside [1,jp] in j are the periodic images, evaluated at the wrapped
-// index.  Each value is the unfused path's expression on the same operands,
-// so the result is bit-identical to the three separate sweeps.
-// ---------------------------------------------------------------------------
-struct PGeo {
-    int ip, jp, kp, sj, si;
-    int nkb, njb, nic, ic, nblocks;
-};
-
-template <int KT, int JT>
-static PGeo precond_geo(const Geo &g, int ic) {
-    PGeo p;
-    p.ip = g.ip;
-    p.jp = g.jp;
-    p.kp = g.kp;
-    p.sj = g.kp;
-    p.si = g.kp * g.jp;
-    p.nkb = (g.kp + KT - 1) / KT;
-    p.njb = (g.jp + JT - 1) / JT;
-    p.ic = ic;
-    p.nic = (g.ip + ic - 1) / ic;
-    p.nblocks = p.nkb * p.njb * p.nic;
-    return p;
+template <typename R, bool TWO>
+__global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
+                                                    const R *__restrict__ a, R *__restrict__ y,
+                                                    MG m, CoefPack cp,
+                                                    const SolverScalars *__restrict__ sc,
+                                                    double *partials) {
+    matvec_dot_body<R, TWO>(x, a, y, m, cp, sc, partials, (int)blockIdx.x);
 }
-
-__device__ __forceinline__ int wrap1(int x, int n) { return ((x - 1) % n + n) % n + 1; }
-
-// one Jacobi sweep of the fused preconditioner over the tile grown by HH
-// cells, plane step mstep; all index arithmetic by compile-time constants
-template <typename R, int KT, int JT, int HH, int NS>
-__device__ __forceinline__ void precond_sweep(const R (*Sin)[NS], R (*Sout)[NS],
-                                              const R (*TT)[NS], const Coef<R> &C,
-                                              const int *JW, const R *AS, const R *AN,
-                                              const R *AD, const R *AU, int k0, int kp, int ip,
-                                              int mstep, int tid) {
-    constexpr int H = 3, W = KT + 2 * H, NT = KT * JT;
-    constexpr int RW = KT + 2 * HH, NR = (JT + 2 * HH) * RW;
-    const int P = wrap1(mstep, ip);
-    const R api = (-C.ae[P]) - C.aw[P];
-    const R aw = C.aw[P], ae = C.ae[P];
-    const int sc = ((mstep % 3) + 3) % 3, sw = ((mstep + 2) % 3 + 3) % 3,
-              se = ((mstep + 1) % 3 + 3) % 3;
-    const R *xin = Sin[sc];
-    const R *xwp = Sin[sw];
-    const R *xep = Sin[se];
-    const R *tt = TT[((mstep % 4) + 4) % 4];
-    R *xo = Sout[sc];
-#pragma unroll 1
-    for (int p0 = 0; p0 < NR; p0 += NT) {
-        const int idx = p0 + tid;
-        if (idx < NR) {
-            const int row = idx / RW + (H - HH), col = idx % RW + (H - HH);
-            const int k = k0 + col - H;
-            if (k >= 1 && k <= kp) {
-                const int L = row * W + col;
-                const R xc = xin[L];
-                const R ap = (((api - AN[row]) - AS[row]) - AU[col]) - AD[col];
-                const R xd = k == 1 ? xc : xin[L - 1];
-                const R xu = k == kp ? xc : xin[L + 1];
-                const R As = stencil<R>(aw, ae, AS[row], AN[row], AD[col], AU[col], ap, xwp[L],
-                                        xep[L], xin[L - W], xin[L + W], xd, xu, xc);
-                xo[L] = xc + div_ap<R>(tt[L] - As, ap);
-            }
-        }
-    }
-    (void)JW;
-}
-
-template <typename R, int MODE, int KT, int JT>
-__global__ __launch_bounds__(KT *JT) void k_precond3(const R *__restrict__ r,
-                                                     const R *__restrict__ v1,
-                                                     R *__restrict__ out_t, R *__restrict__ out,
-                                                     PGeo g, CoefPack cp,
-                                                     const SolverScalars *__restrict__ sc) {
-    constexpr int H = 3, W = KT + 2 * H, RWS = JT + 2 * H, NS = RWS * W, NT = KT * JT;
-    __shared__ R S0[3][NS], S1[3][NS], S2[3][NS], TT[4][NS];
-    // per-row (wrapped j) and per-column (k) coefficients of the tile + halo
-    __shared__ int JW[RWS];
-    __shared__ R AS[RWS], AN[RWS], AD[W], AU[W];
-    if (sc->done || sc->breakdown) return;
-    const int b = blockIdx.x;
-    const int per = (g.nblocks + 7) >> 3;
-    const int tile = (b & 7) * per + (b >> 3);
-    if (tile >= g.nblocks) return;
-    const int kb = tile % g.nkb, rr = tile / g.nkb;
-    const int jb = rr % g.njb, icb = rr / g.njb;
-    const int k0 = kb * KT + 1, j0 = jb * JT + 1;
-    const int i0 = icb * g.ic + 1;
-    const int i1 = i0 + g.ic - 1 > g.ip ? g.ip : i0 + g.ic - 1;
-    const Coef<R> C(cp.m64, cp.m32);
-    const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
-    const R bo = (R)sc->omg;
-    const R bet = (R)sc->bet;
-    const int tid = (int)threadIdx.x;
-    if (tid < RWS) {
-        const int j = wrap1(j0 - H + tid, g.jp);
-        JW[tid] = j;
-        AS[tid] = C.as[j];
-        AN[tid] = C.an[j];
-    }
-    if (tid < W) {
-        int k = k0 - H + tid;
-        k = k < 1 ? 1 : (k > g.kp ? g.kp : k);
-        AD[tid] = C.ad[k];
-        AU[tid] = C.au[k];
-    }
-    __syncthreads();
-
-    for (int n = i0 - 6; n <= i1; ++n) {
-        // level 0 on plane n+3: t and s0 = t/a_p over the tile grown by 3
-        {
-            const int P = wrap1(n + 3, g.ip);
-            const R api = (-C.ae[P]) - C.aw[P];
-            R *s0 = S0[((n + 3) % 3 + 3) % 3];
-            R *tt = TT[((n + 3) % 4 + 4) % 4];
-            const int base = (P - 1) * g.si;
-#pragma unroll 1
-            for (int p0 = 0; p0 < NS; p0 += NT) {
-                const int idx = p0 + tid;
-                if (idx < NS) {
-                    const int row = idx / W, col = idx % W;
-                    int k = k0 - H + col;
-                    k = k < 1 ? 1 : (k > g.kp ? g.kp : k);
-                    const int q = base + (JW[row] - 1) * g.sj + (k - 1);
-                    const R tv = MODE == 0 ? r[q] - a * (bo * v1[q]) : r[q] - a * v1[q];
-                    tt[idx] = tv;
-                    s0[idx] = div_ap<R>(tv, (((api - AN[row]) - AS[row]) - AU[col]) - AD[col]);
-                }
-            }
-        }
-        __syncthreads();
-        if (n >= i0 - 4)
-            precond_sweep<R, KT, JT, 2, NS>(S0, S1, TT, C, JW, AS, AN, AD, AU, k0, g.kp, g.ip,
-                                            n + 2, tid);
-        __syncthreads();
-        if (n >= i0 - 2)
-            precond_sweep<R, KT, JT, 1, NS>(S1, S2, TT, C, JW, AS, AN, AD, AU, k0, g.kp, g.ip,
-                                            n + 1, tid);
-        __syncthreads();
-        if (n >= i0) {
-            // output level on plane n, own cells
-            const int P = n;  // i0 <= n <= i1 <= ip
-            const int jr = tid / KT, kr = tid % KT;
-            const int k = k0 + kr, j = j0 + jr;
-            if (k <= g.kp && j <= g.jp) {
-                const R api = (-C.ae[P]) - C.aw[P];
-                const int row = jr + H, col = kr + H;
-                const int L = row * W + col;
-                const R *xin = S2[(n % 3 + 3) % 3];
-                const R xc = xin[L];
-                const R ap = (((api - AN[row]) - AS[row]) - AU[col]) - AD[col];
-                const R xd = k == 1 ? xc : xin[L - 1];
-                const R xu = k == g.kp ? xc : xin[L + 1];
-                const R As = stencil<R>(C.aw[P], C.ae[P], AS[row], AN[row], AD[col], AU[col], ap,
-                                        S2[((n + 2) % 3 + 3) % 3][L],
-                                        S2[((n + 1) % 3 + 3) % 3][L], xin[L - W], xin[L + W], xd,
-                                        xu, xc);
-                const R tv = TT[(n % 4 + 4) % 4][L];
-                const R s3 = xc + div_ap<R>(tv - As, ap);
-                const int q = (P - 1) * g.si + (j - 1) * g.sj + (k - 1);
-                if (MODE == 0) {
-                    out[q] = s3 + bet * out[q];  // p = s + bet*p (dynamics.f90:399)
-                } else {
-                    out[q] = s3;    // cs
-                    out_t[q] = tv;  // s = r - alf*v1 (dynamics.f90:405)
-                }
-            }
-        }
-        __syncthreads();
-    }
+template <typename R, bool TWO>
+__global__ __launch_bounds__(256) void k_matvec_dot_b(IterArgs A, int xi, int ai, int yi) {
+    int lb;
+    const IterMember &M = member_m(A, lb);
+    matvec_dot_body<R, TWO>((const R *)M.V[xi], (const R *)M.V[ai], (R *)M.V[yi], M.m, M.cp,
+                            M.sc, M.part, lb);
 }
 
 // x = x + alf*p + omg*cs; r = s - omg*t; partial sums of <r,r>, <r,cr>
 // (dynamics.f90:419-423), all compact vectors, grid-stride
 template <typename R>
-__global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__restrict__ p,
-                                                const R *__restrict__ cs,
-                                                const R *__restrict__ s,
-                                                const R *__restrict__ t,
-                                                const R *__restrict__ cr, R *__restrict__ r,
-                                                int n, const SolverScalars *__restrict__ sc,
-                                                double *partials, int stride) {
+__device__ __forceinline__ void update_body(R *__restrict__ x, const R *__restrict__ p,
+                                            const R *__restrict__ cs, const R *__restrict__ s,
+                                            const R *__restrict__ t, const R *__restrict__ cr,
+                                            R *__restrict__ r, int n, int nbu,
+                                            const SolverScalars *__restrict__ sc,
+                                            double *partials, int lb) {
     if (sc->done || sc->breakdown) return;
+    const int stride = nbu;
     const R alf = (R)sc->alf, omg = (R)sc->omg;
     double acc[2] = {0.0, 0.0};
-    for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    for (int q = lb * kBlock + threadIdx.x; q < n; q += nbu * kBlock) {
         x[q] = (x[q] + alf * p[q]) + omg * cs[q];
         const R rv = s[q] - omg * t[q];
         r[q] = rv;
         acc[0] += (double)rv * (double)rv;
         acc[1] += (double)rv * (double)cr[q];
     }
-    block_sum_store_at<2>(acc, partials, stride, blockIdx.x);
+    block_sum_store_at<2>(acc, partials, stride, lb);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__restrict__ p,
+                                                const R *__restrict__ cs,
+                                                const R *__restrict__ s,
+                                                const R *__restrict__ t,
+                                                const R *__restrict__ cr, R *__restrict__ r,
+                                                int n, int nbu,
+                                                const SolverScalars *__restrict__ sc,
+                                                double *partials) {
+    update_body<R>(x, p, cs, s, t, cr, r, n, nbu, sc, partials, (int)blockIdx.x);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_update_b(IterArgs A) {
+    int lb;
+    const IterMember &M = member_u(A, lb);
+    update_body<R>((R *)M.V[VX], (const R *)M.V[VP], (const R *)M.V[VCS], (const R *)M.V[VS],
+                   (const R *)M.V[VT], (const R *)M.V[VCR], (R *)M.V[VR], M.n, M.nbu, M.sc,
+                   M.part, lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -960,6 +808,7 @@ __global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *
         sc->breakdown = 0;
         sc->iters = 0;
         sc->guard = 1;
+        sc->tol = rtol;
         sc->best_err = sc->err;
         sc->best_it = 0;
         sc->nonfinite = !isfinite(rr);
@@ -1097,10 +946,13 @@ static CGeo cgeo(const Geo &g) {
 }
 
 // tuning: i planes per marching block, and the first-sweep variant per
-// precision (0 fused in registers, 1 fused through LDS, 2 streaming pass +
-// plain sweep).  Measured at 256x256x128 on MI355X: FP64 is fastest split
-// (its IEEE divisions make the fused kernel issue-bound), FP32 fused in
-// registers; 8 and 16 planes are equal, 4 and 32 slower.
+// precision (0 fused in registers, 2 streaming pass + plain sweep).
+// Measured at 256x256x128 on MI355X: FP64 is fastest split (its IEEE
+// divisions make the fused kernel issue-bound), FP32 fused in registers; 8
+// and 16 planes are equal, 4 and 32 slower.  (An LDS-shared first sweep and
+// a temporally blocked preconditioner holding all three sweeps in LDS were
+// both bit-identical and slower: the stencil kernels here are bound by load
+// latency, not by their HBM bytes.)
 #ifndef DCM_MARCH_PLANES
 #define DCM_MARCH_PLANES 8
 #endif
@@ -1111,313 +963,563 @@ static CGeo cgeo(const Geo &g) {
 #define DCM_JAC_FIRST32 0
 #endif
 constexpr int kMarchPlanes = DCM_MARCH_PLANES;
-constexpr int kUpdateBlocks = 2048; // grid of the streaming update
+constexpr int kUpdateBlocks = 2048; // largest grid of the streaming update
 constexpr int kPollEvery = 4;       // host polls the device flags this often
 
-// the first Jacobi sweep with its input t: t and s1 (into `s1`); `tmp` is
-// free scratch (variant 2 keeps s0 there)
-template <typename R, int MODE>
-static void first_sweep(dcm_ctx *c, const R *r, const R *v1, R *t, R *s1, R *tmp, const MG &m,
-                        unsigned mg, const CoefPack &cp, const SolverScalars *sc) {
-    const int variant = sizeof(R) == 8 ? DCM_JAC_FIRST64 : DCM_JAC_FIRST32;
-    if (variant == 0) {
-        hipLaunchKernelGGL((k_jac_first_reg<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r,
-                           v1, t, s1, m, cp, sc);
-    } else if (variant == 1) {
-        hipLaunchKernelGGL((k_jac_first<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1,
-                           t, s1, m, cp, sc);
-    } else {
-        hipLaunchKernelGGL((k_jac_prep<R, MODE>), dim3(mg), dim3(kBlock), 0, c->stream, r, v1, t,
-                           tmp, m, cp, sc);
-        hipLaunchKernelGGL((k_jac<R, false>), dim3(mg), dim3(kBlock), 0, c->stream, tmp, t, s1,
-                           m, cp, sc);
+// update-kernel blocks of a member: a full grid for large systems, about four
+// elements per thread for small ones
+static int update_blocks(long long n) {
+    long long b = (n + 4LL * kBlock - 1) / (4LL * kBlock);
+    if (b < 8) b = 8;
+    return (int)(b > kUpdateBlocks ? kUpdateBlocks : b);
+}
+
+// Host-side description of one system of a batched solve.
+struct SolveSys {
+    dcm_ctx *c;       // the member's context: geometry, coefficients, scalars
+    void *V[10];      // its compact vectors (precision of the solve)
+    int march_planes; // i planes per marching block
+};
+
+// The device tables of a batched solve (member table and block maps),
+// uploaded for each solve; `tables` is a device scratch the caller owns.
+struct SolvePlan {
+    std::vector<IterMember> mem;
+    std::vector<int> mmap, umap;
+    unsigned mgrid = 0, ugrid = 0;
+    IterArgs args{};
+};
+
+static int plan_solve(hipStream_t st, const SolveSys *sys, int nsys, SolvePlan &P,
+                      DeviceScratch &tables) {
+    P.mem.resize(nsys);
+    P.mmap.clear();
+    P.umap.clear();
+    unsigned mb = 0, ub = 0;
+    for (int q = 0; q < nsys; ++q) {
+        dcm_ctx *c = sys[q].c;
+        IterMember &M = P.mem[q];
+        M.m = march_geo(c->geo, sys[q].march_planes);
+        M.n = c->geo.ip * c->geo.jp * c->geo.kp;
+        M.nbu = update_blocks(M.n);
+        const int np = M.m.nblocks > M.nbu ? M.m.nblocks : M.nbu;
+        DCM_CHECK(ensure_partials(c, 2 * (size_t)np + 64));
+        M.cp = CoefPack{c->met, c->met32};
+        M.sc = c->solver;
+        M.part = c->partials;
+        for (int v = 0; v < 10; ++v) M.V[v] = sys[q].V[v];
+        M.mblock0 = (int)mb;
+        M.ublock0 = (int)ub;
+        const unsigned mg = march_grid(M.m);
+        if (nsys > 1) {
+            P.mmap.insert(P.mmap.end(), mg, q);
+            P.umap.insert(P.umap.end(), (size_t)M.nbu, q);
+        }
+        mb += mg;
+        ub += (unsigned)M.nbu;
     }
-}
-
-// the preconditioner fused into one pass (k_precond3): tile shape per
-// precision and grid depth, i planes per block
-#ifndef DCM_FUSED64
-#define DCM_FUSED64 0
-#endif
-#ifndef DCM_FUSED32
-#define DCM_FUSED32 0
-#endif
-constexpr int kPrecondPlanes = 16;
-
-template <typename R, int MODE, int KT, int JT>
-static void precond3_launch(dcm_ctx *c, const R *r, const R *v1, R *out_t, R *out,
-                            const CoefPack &cp, const SolverScalars *sc) {
-    const PGeo pg = precond_geo<KT, JT>(c->geo, kPrecondPlanes);
-    const unsigned grid = (unsigned)(((pg.nblocks + 7) / 8) * 8);
-    hipLaunchKernelGGL((k_precond3<R, MODE, KT, JT>), dim3(grid), dim3(KT * JT), 0, c->stream, r,
-                       v1, out_t, out, pg, cp, sc);
-}
-
-// MODE 0: p = M t + bet*p with t = r - bet*(omg*v1); MODE 1: s = r - alf*v1
-// and cs = M s
-template <typename R, int MODE>
-static void precond3(dcm_ctx *c, const R *r, const R *v1, R *out_t, R *out, const CoefPack &cp,
-                     const SolverScalars *sc) {
-    const bool deep = c->geo.kp > 32;
-    if constexpr (sizeof(R) == 4) {
-        if (deep)
-            precond3_launch<R, MODE, 64, 16>(c, r, v1, out_t, out, cp, sc);
-        else
-            precond3_launch<R, MODE, 32, 32>(c, r, v1, out_t, out, cp, sc);
-    } else {
-        if (deep)
-            precond3_launch<R, MODE, 64, 4>(c, r, v1, out_t, out, cp, sc);
-        else
-            precond3_launch<R, MODE, 32, 8>(c, r, v1, out_t, out, cp, sc);
+    P.mgrid = mb;
+    P.ugrid = ub;
+    const size_t bm = sizeof(IterMember) * nsys;
+    const size_t bmap = sizeof(int) * (P.mmap.size() + P.umap.size());
+    char *d;
+    DCM_CHECK(tables.ensure(bm + bmap + 256, (void **)&d));
+    DCM_HIP(hipMemcpyAsync(d, P.mem.data(), bm, hipMemcpyHostToDevice, st));
+    int *dmap = (int *)(d + ((bm + 255) / 256) * 256);
+    if (nsys > 1) {
+        DCM_HIP(hipMemcpyAsync(dmap, P.mmap.data(), sizeof(int) * P.mmap.size(),
+                               hipMemcpyHostToDevice, st));
+        DCM_HIP(hipMemcpyAsync(dmap + P.mmap.size(), P.umap.data(),
+                               sizeof(int) * P.umap.size(), hipMemcpyHostToDevice, st));
     }
+    DCM_HIP(hipStreamSynchronize(st));  // the host vectors may change after return
+    P.args.mem = (const IterMember *)d;
+    P.args.mmap = nsys > 1 ? dmap : nullptr;
+    P.args.umap = nsys > 1 ? dmap + P.mmap.size() : nullptr;
+    return DCM_OK;
 }
 
-// BiCGSTAB iterations on compact vectors of precision R.  V: r, cr, p, v1, t,
-// s, cs, two sweep buffers, x.  The initial state and scalars must be set.
+// Launches of one BiCGSTAB iteration.  One system: the kernels take its
+// vectors and geometry as arguments.  Several: the batched kernels find
+// their member in the device table.
 template <typename R>
-static int bicgstab_iterate(dcm_ctx *c, R *const *V, double tol, int itmax, SolverOut *out) {
-    const Geo &pg = c->geo;
-    const CGeo g = cgeo(pg);
-    const MG m = march_geo(pg, kMarchPlanes);
-    const unsigned mg = march_grid(m);
-    const int nbm = m.nblocks;
-    CoefPack cp{c->met, c->met32};
-    R *r = V[0], *cr = V[1], *p = V[2], *v1 = V[3], *t = V[4], *s = V[5], *cs = V[6],
-      *wa = V[7], *wb = V[8], *x = V[9];
-    SolverScalars *sc = c->solver;
-    double *part = c->partials;
-    const int n = (int)g.n;
-    const bool seq = c->seq_sums;
-    const bool fused = sizeof(R) == 8 ? DCM_FUSED64 : DCM_FUSED32;
-    const int nbv = seq ? 1 : nbm, nbu = seq ? 1 : kUpdateBlocks;
+struct IterLaunch {
+    dcm_ctx *c;
+    const SolvePlan &P;
+    int nsys, seq;
+    bool one() const { return nsys == 1; }
+    const IterMember &M() const { return P.mem[0]; }
+    R *v(int i) const { return (R *)P.mem[0].V[i]; }
+
+    void jac(const char *name, int si, int ti, int oi) const {
+        if (one())
+            launch_timed(c, name, k_jac<R, false>, dim3(P.mgrid), dim3(kBlock), (const R *)v(si),
+                         (const R *)v(ti), v(oi), M().m, M().cp, (const SolverScalars *)M().sc);
+        else
+            launch_timed(c, name, k_jac_b<R, false>, dim3(P.mgrid), dim3(kBlock), P.args, si, ti,
+                         oi);
+    }
+    void jac_p(int si, int ti, int oi) const {
+        if (one())
+            launch_timed(c, "jac_p", k_jac<R, true>, dim3(P.mgrid), dim3(kBlock),
+                         (const R *)v(si), (const R *)v(ti), v(oi), M().m, M().cp,
+                         (const SolverScalars *)M().sc);
+        else
+            launch_timed(c, "jac_p", k_jac_b<R, true>, dim3(P.mgrid), dim3(kBlock), P.args, si,
+                         ti, oi);
+    }
+    template <int MODE> void first(int ti) const {
+        const int variant = sizeof(R) == 8 ? DCM_JAC_FIRST64 : DCM_JAC_FIRST32;
+        const int tok = prof_begin(c, "jac_first");
+        if (variant == 0) {
+            if (one())
+                hipLaunchKernelGGL((k_jac_first_reg<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
+                                   c->stream, (const R *)v(VR), (const R *)v(VV1), v(ti), v(VWA),
+                                   M().m, M().cp, (const SolverScalars *)M().sc);
+            else
+                hipLaunchKernelGGL((k_jac_first_reg_b<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
+                                   c->stream, P.args, (int)VR, (int)VV1, ti, (int)VWA);
+        } else {
+            if (one())
+                hipLaunchKernelGGL((k_jac_prep<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
+                                   c->stream, (const R *)v(VR), (const R *)v(VV1), v(ti), v(VWB),
+                                   M().m, M().cp, (const SolverScalars *)M().sc);
+            else
+                hipLaunchKernelGGL((k_jac_prep_b<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
+                                   c->stream, P.args, (int)VR, (int)VV1, ti, (int)VWB);
+            jac("jac_first", VWB, ti, VWA);
+        }
+        prof_end(c, tok);
+    }
+    template <bool TWO> void matvec(int xi, int ai, int yi) const {
+        if (one())
+            launch_timed(c, "matvec_dot", k_matvec_dot<R, TWO>, dim3(P.mgrid), dim3(kBlock),
+                         (const R *)v(xi), (const R *)v(ai), v(yi), M().m, M().cp,
+                         (const SolverScalars *)M().sc, M().part);
+        else
+            launch_timed(c, "matvec_dot", k_matvec_dot_b<R, TWO>, dim3(P.mgrid), dim3(kBlock),
+                         P.args, xi, ai, yi);
+    }
+    void update() const {
+        if (one())
+            launch_timed(c, "update", k_update<R>, dim3(P.ugrid), dim3(kBlock), v(VX),
+                         (const R *)v(VP), (const R *)v(VCS), (const R *)v(VS), (const R *)v(VT),
+                         (const R *)v(VCR), v(VR), M().n, M().nbu, (const SolverScalars *)M().sc,
+                         M().part);
+        else
+            launch_timed(c, "update", k_update_b<R>, dim3(P.ugrid), dim3(kBlock), P.args);
+    }
+    // 0: alf (<cr,v1>), 1: omg (<t,t>, <t,s>), 2: the residual (<r,r>, <r,cr>)
+    void fin(int which) const {
+        if (seq) {
+            const int a0 = which == 0 ? VCR : (which == 1 ? VT : VR);
+            const int b0 = which == 0 ? VV1 : (which == 1 ? VT : VR);
+            const int a1 = which == 0 ? -1 : (which == 1 ? VT : VR);
+            const int b1 = which == 0 ? -1 : (which == 1 ? VS : VCR);
+            hipLaunchKernelGGL(k_seq_dots<R>, dim3(nsys), dim3(64), 0, c->stream, P.args, a0, b0,
+                               a1, b1);
+        }
+        if (one()) {
+            const int nb = seq ? 1 : (which == 2 ? M().nbu : M().m.nblocks);
+            if (which == 0)
+                hipLaunchKernelGGL(k_fin_v1, dim3(1), dim3(1024), 0, c->stream, M().part, nb,
+                                   M().sc);
+            else if (which == 1)
+                hipLaunchKernelGGL(k_fin_t1, dim3(1), dim3(1024), 0, c->stream, M().part, nb,
+                                   M().sc);
+            else
+                hipLaunchKernelGGL(k_fin_r1, dim3(1), dim3(1024), 0, c->stream, M().part, nb,
+                                   M().sc);
+        } else {
+            if (which == 0)
+                hipLaunchKernelGGL(k_fin_v, dim3(nsys), dim3(1024), 0, c->stream, P.args, seq);
+            else if (which == 1)
+                hipLaunchKernelGGL(k_fin_t, dim3(nsys), dim3(1024), 0, c->stream, P.args, seq);
+            else
+                hipLaunchKernelGGL(k_fin_r, dim3(nsys), dim3(1024), 0, c->stream, P.args, seq);
+        }
+    }
+    // dynamics.f90:391-425
+    void iteration() const {
+        first<0>(VT);               // t = r - bet*(omg*v1), sweep 1 -> wa
+        jac("jac", VWA, VT, VWB);   // sweep 2 -> wb
+        jac_p(VWB, VT, VP);         // sweep 3, p = s + bet*p
+        matvec<false>(VP, VCR, VV1);  // v1 = A p, <cr,v1>
+        fin(0);
+        first<1>(VS);               // s = r - alf*v1, sweep 1 -> wa
+        jac("jac", VWA, VS, VWB);
+        jac("jac", VWB, VS, VCS);   // cs = M s
+        matvec<true>(VCS, VS, VT);  // t = A cs, <t,t>, <t,s>
+        fin(1);
+        update();                   // x, r; <r,r>, <r,cr>
+        fin(2);
+    }
+};
+
+// BiCGSTAB iterations (dynamics.f90:390-430) of a batch of independent
+// systems in lockstep, every launch serving all of them; a member that has
+// converged or broken down skips its blocks.  Each member's initial state and
+// scalars (its tolerance included) must be set.  `c` supplies the stream, the
+// profiling and the flags buffers; outs[q] receives member q's iterations and
+// residual, status[q] its status.
+template <typename R>
+static int bicgstab_iterate_batch(dcm_ctx *c, const SolveSys *sys, int nsys, int itmax,
+                                  SolverOut *outs, int *status) {
+    SolvePlan P;
+    DCM_CHECK(plan_solve(c->stream, sys, nsys, P, c->solve_tables));
+    const IterArgs &A = P.args;
+    const IterLaunch<R> L{c, P, nsys, c->seq_sums ? 1 : 0};
+    int *dflags, *hflags;
+    DCM_CHECK(c->flags.ensure(sizeof(int) * nsys, (void **)&dflags));
+    DCM_CHECK(ensure_host_flags(c, nsys, &hflags));
     if (itmax < 1) itmax = 1;
     int launched = 0;
     while (launched < itmax) {
         const int chunk = (itmax - launched) < kPollEvery ? (itmax - launched) : kPollEvery;
-        for (int q = 0; q < chunk; ++q) {
-            int tok;
-            if (fused) {
-                tok = prof_begin(c, "precond");
-                precond3<R, 0>(c, r, v1, (R *)nullptr, p, cp, sc);
-                prof_end(c, tok);
-            } else {
-                tok = prof_begin(c, "jac_first");
-                first_sweep<R, 0>(c, r, v1, t, wa, wb, m, mg, cp, sc);
-                prof_end(c, tok);
-                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wa,
-                             (const R *)t, wb, m, cp, (const SolverScalars *)sc);
-                launch_timed(c, "jac_p", k_jac<R, true>, dim3(mg), dim3(kBlock), (const R *)wb,
-                             (const R *)t, p, m, cp, (const SolverScalars *)sc);
-            }
-            launch_timed(c, "matvec_dot", k_matvec_dot<R, false>, dim3(mg), dim3(kBlock),
-                         (const R *)p, (const R *)cr, v1, m, cp, (const SolverScalars *)sc, part,
-                         nbm);
-            if (seq)
-                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, cr, v1,
-                                   (const R *)nullptr, (const R *)nullptr, g.n, part, sc);
-            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
-            if (fused) {
-                tok = prof_begin(c, "precond");
-                precond3<R, 1>(c, r, v1, s, cs, cp, sc);
-                prof_end(c, tok);
-            } else {
-                tok = prof_begin(c, "jac_first");
-                first_sweep<R, 1>(c, r, v1, s, wa, wb, m, mg, cp, sc);
-                prof_end(c, tok);
-                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wa,
-                             (const R *)s, wb, m, cp, (const SolverScalars *)sc);
-                launch_timed(c, "jac", k_jac<R, false>, dim3(mg), dim3(kBlock), (const R *)wb,
-                             (const R *)s, cs, m, cp, (const SolverScalars *)sc);
-            }
-            launch_timed(c, "matvec_dot", k_matvec_dot<R, true>, dim3(mg), dim3(kBlock),
-                         (const R *)cs, (const R *)s, t, m, cp, (const SolverScalars *)sc, part,
-                         nbm);
-            if (seq)
-                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, t, t, t, s,
-                                   g.n, part, sc);
-            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(1024), 0, c->stream, part, nbv, sc);
-            launch_timed(c, "update", k_update<R>, dim3(kUpdateBlocks), dim3(kBlock), x,
-                         (const R *)p, (const R *)cs, (const R *)s, (const R *)t, (const R *)cr, r,
-                         n, (const SolverScalars *)sc, part, (int)kUpdateBlocks);
-            if (seq)
-                hipLaunchKernelGGL(k_seq_dots<R>, dim3(1), dim3(64), 0, c->stream, r, r, r, cr,
-                                   g.n, part, sc);
-            hipLaunchKernelGGL(k_fin_r, dim3(1), dim3(1024), 0, c->stream, part, nbu, sc, tol);
-        }
+        for (int q = 0; q < chunk; ++q) L.iteration();
         DCM_LAUNCH_CHECK();
         launched += chunk;
-        DCM_HIP(hipMemcpyAsync(c->h_solver, sc, sizeof(SolverScalars), hipMemcpyDeviceToHost,
+        hipLaunchKernelGGL(k_gather_flags, dim3(1), dim3(64), 0, c->stream, A, nsys, dflags);
+        DCM_HIP(hipMemcpyAsync(hflags, dflags, sizeof(int) * nsys, hipMemcpyDeviceToHost,
                                c->stream));
         DCM_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_solver->breakdown || c->h_solver->done) break;
+        bool all = true;
+        for (int q = 0; q < nsys; ++q) all = all && (hflags[q] & 3) != 0;
+        if (all) break;
     }
     // the kernels after convergence were no-ops; the device counted the
     // iterations that ran
-    out->iters = c->h_solver->iters;
-    out->err = c->h_solver->err;
-    if (c->h_solver->breakdown) return DCM_BREAKDOWN;
-    if (c->h_solver->nonfinite) {
-        set_error("bicgstab: non-finite residual");
-        return DCM_NONFINITE;
+    for (int q = 0; q < nsys; ++q) {
+        dcm_ctx *mc = sys[q].c;
+        DCM_HIP(hipMemcpyAsync(mc->h_solver, mc->solver, sizeof(SolverScalars),
+                               hipMemcpyDeviceToHost, c->stream));
     }
-    if (!(c->h_solver->err < tol)) return DCM_NOT_CONVERGED;
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    int worst = DCM_OK;
+    for (int q = 0; q < nsys; ++q) {
+        const SolverScalars *h = sys[q].c->h_solver;
+        outs[q].iters = h->iters;
+        outs[q].err = h->err;
+        int st = DCM_OK;
+        if (h->breakdown)
+            st = DCM_BREAKDOWN;
+        else if (h->nonfinite)
+            st = DCM_NONFINITE;
+        else if (!(h->err < h->tol))
+            st = DCM_NOT_CONVERGED;
+        status[q] = st;
+        if (st != DCM_OK && worst == DCM_OK) worst = st;
+    }
+    if (worst == DCM_NONFINITE) set_error("bicgstab: non-finite residual");
+    return worst;
+}
+
+// one system: the batch engine with one member
+
+
+// The reference's FP64 BiCGSTAB (dynamics.f90:293-444) on padded x, b for a
+// batch of independent systems (one per context) that share c's stream:
+// every member's setup, one batched iteration loop, every member's finish.
+int bicgstab64_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, int itmax,
+                     SolverOut *outs, int *status) {
+    std::vector<SolveSys> sys;
+    std::vector<int> idx;
+    std::vector<char> early(n, 0), iterated(n, 0);
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        const Geo &pg = mc->geo;
+        const CGeo g = cgeo(pg);
+        const Interior L = interior_launch(pg);
+        const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+        DCM_CHECK(ensure_partials(mc, 4 * (size_t)nb + 64));
+        DCM_CHECK(ensure_solver_vectors64(mc, 10));
+        double *V = mc->sv64;
+        CoefPack cp{mc->met, mc->met32};
+        int tok = prof_begin(c, "resid0");
+        hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, mc->stream, tasks[q].x, tasks[q].b, V,
+                           V + g.n, V + 2 * g.n, V + 3 * g.n, V + 9 * g.n, pg, g, cp,
+                           mc->partials, (long long)nb);
+        prof_end(c, tok);
+        if (c->seq_sums)
+            hipLaunchKernelGGL(k_seq_resid0, dim3(1), dim3(64), 0, mc->stream, tasks[q].b, V, pg,
+                               g, mc->partials);
+        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, mc->stream, mc->partials,
+                           c->seq_sums ? 1 : nb, mc->solver, tol, 1);
+        DCM_LAUNCH_CHECK();
+        DCM_HIP(hipMemcpyAsync(mc->h_solver, mc->solver, sizeof(SolverScalars),
+                               hipMemcpyDeviceToHost, mc->stream));
+    }
+    for (int q = 0; q < n; ++q) DCM_HIP(hipStreamSynchronize(tasks[q].c->stream));
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        status[q] = DCM_OK;
+        outs[q] = SolverOut();
+        if (mc->h_solver->nonfinite) {
+            set_error("bicgstab: non-finite initial residual");
+            status[q] = DCM_NONFINITE;
+        } else if (mc->h_solver->err < tol) {
+            early[q] = 1;  // :383, return before the loop
+            outs[q].err = mc->h_solver->err;
+        } else {
+            SolveSys sy;
+            sy.c = mc;
+            for (int v = 0; v < 10; ++v) sy.V[v] = mc->sv64 + (size_t)v * mc->geo.ip * mc->geo.jp *
+                                                                 mc->geo.kp;
+            sy.march_planes = mc->march_planes > 0 ? mc->march_planes : kMarchPlanes;
+            sys.push_back(sy);
+            idx.push_back(q);
+            iterated[q] = 1;
+        }
+    }
+    if (!sys.empty()) {
+        std::vector<SolverOut> o(sys.size());
+        std::vector<int> st(sys.size());
+        (void)bicgstab_iterate_batch<double>(c, sys.data(), (int)sys.size(), itmax, o.data(),
+                                             st.data());
+        for (size_t u = 0; u < sys.size(); ++u) {
+            outs[idx[u]] = o[u];
+            status[idx[u]] = st[u];
+        }
+    }
+    int worst = DCM_OK;
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        const Geo &pg = mc->geo;
+        const CGeo g = cgeo(pg);
+        const Interior L = interior_launch(pg);
+        if (iterated[q])
+            hipLaunchKernelGGL(k_scatter_x, L.grid, L.block, 0, mc->stream,
+                               mc->sv64 + 9 * (size_t)g.n, tasks[q].x, pg, g);
+        if (status[q] == DCM_OK) {
+            // the halo of x as the reference leaves it: periodic faces from
+            // mat_ax, then the boundary rows of :437-442 (or mat_ax's Neumann
+            // copies on the early return)
+            DCM_CHECK(fill<double>(mc, tasks[q].x, FILL_ALONG_DIM_WO, 0.0, 0.0));
+            hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, mc->stream, tasks[q].x,
+                               tasks[q].sw, pg, mc->h_dzn[0], early[q] ? 0 : 1);
+        }
+        DCM_LAUNCH_CHECK();
+        if (status[q] != DCM_OK && worst == DCM_OK) worst = status[q];
+    }
+    return worst;
+}
+
+// A fixed number of FP64 iterations of a batch from its current x and b
+// (tolerance 0, so no member stops early), for timing: only solver scratch
+// (residual and iterate vectors, partials, scalars) is written.
+int bicgstab_probe(dcm_ctx *c, const SolveTask *tasks, int n, int iters, SolverOut *outs,
+                   int *status) {
+    std::vector<SolveSys> sys(n);
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        const Geo &pg = mc->geo;
+        const CGeo g = cgeo(pg);
+        const Interior L = interior_launch(pg);
+        const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+        DCM_CHECK(ensure_partials(mc, 4 * (size_t)nb + 64));
+        DCM_CHECK(ensure_solver_vectors64(mc, 10));
+        double *V = mc->sv64;
+        CoefPack cp{mc->met, mc->met32};
+        hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, mc->stream, tasks[q].x, tasks[q].b, V,
+                           V + g.n, V + 2 * g.n, V + 3 * g.n, V + 9 * g.n, pg, g, cp,
+                           mc->partials, (long long)nb);
+        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, mc->stream, mc->partials, nb,
+                           mc->solver, 0.0, 1);
+        DCM_LAUNCH_CHECK();
+        sys[q].c = mc;
+        for (int v = 0; v < 10; ++v) sys[q].V[v] = V + (size_t)v * g.n;
+        sys[q].march_planes = mc->march_planes > 0 ? mc->march_planes : kMarchPlanes;
+    }
+    (void)bicgstab_iterate_batch<double>(c, sys.data(), n, iters, outs, status);
     return DCM_OK;
 }
 
-// FP64 BiCGSTAB of the reference on padded x, b (dynamics.f90:293-444)
 int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
                SolverOut *out) {
-    const Geo &pg = c->geo;
-    const CGeo g = cgeo(pg);
-    const Interior L = interior_launch(pg);
-    const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
-    const MG m = march_geo(pg, kMarchPlanes);
-    CoefPack cp{c->met, c->met32};
-    size_t need = 4 * (size_t)nb + 64;
-    if (need < 4 * (size_t)m.nblocks + 64) need = 4 * (size_t)m.nblocks + 64;
-    if (need < 4 * (size_t)kUpdateBlocks) need = 4 * (size_t)kUpdateBlocks;
-    DCM_CHECK(ensure_partials(c, need));
-    DCM_CHECK(ensure_solver_vectors64(c, 10));
-    double *V[10];
-    for (int q = 0; q < 10; ++q) V[q] = c->sv64 + (size_t)q * g.n;
-    int tok = prof_begin(c, "resid0");
-    hipLaunchKernelGGL(k_resid0, L.grid, L.block, 0, c->stream, x, b, V[0], V[1], V[2], V[3], V[9],
-                       pg, g, cp, c->partials, (long long)nb);
-    prof_end(c, tok);
-    if (c->seq_sums)
-        hipLaunchKernelGGL(k_seq_resid0, dim3(1), dim3(64), 0, c->stream, b, V[0], pg, g,
-                           c->partials);
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, c->stream, c->partials,
-                       c->seq_sums ? 1 : nb, c->solver, tol, 1);
-    DCM_LAUNCH_CHECK();
-    DCM_HIP(hipMemcpyAsync(c->h_solver, c->solver, sizeof(SolverScalars), hipMemcpyDeviceToHost,
-                           c->stream));
-    DCM_HIP(hipStreamSynchronize(c->stream));
+    const SolveTask t{c, sw, x, b};
     int st = DCM_OK;
-    bool early = false;
-    if (c->h_solver->nonfinite) {
-        set_error("bicgstab: non-finite initial residual");
-        st = DCM_NONFINITE;
-    } else if (c->h_solver->err < tol) {
-        early = true;  // :383, return before the loop
-        out->iters = 0;
-        out->err = c->h_solver->err;
-    } else {
-        st = bicgstab_iterate<double>(c, V, tol, itmax, out);
-        hipLaunchKernelGGL(k_scatter_x, L.grid, L.block, 0, c->stream, V[9], x, pg, g);
-        DCM_LAUNCH_CHECK();
-    }
-    if (st == DCM_OK) {
-        // the halo of x as the reference leaves it: periodic faces from
-        // mat_ax, then the boundary rows of :437-442 (or mat_ax's Neumann
-        // copies on the early return)
-        DCM_CHECK(fill<double>(c, x, FILL_ALONG_DIM_WO, 0.0, 0.0));
-        hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, c->stream, x, sw, pg,
-                           c->h_dzn[0], early ? 0 : 1);
-        DCM_LAUNCH_CHECK();
-    }
+    DCM_CHECK(bicgstab64_batch(c, &t, 1, tol, itmax, out, &st));
     return st;
 }
 
-// Mixed precision: FP32 inner BiCGSTAB inside FP64 iterative refinement.
-int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
-                   double inner_rtol, SolverOut *out) {
-    const Geo &pg = c->geo;
-    const CGeo g = cgeo(pg);
-    const Interior L = interior_launch(pg);
-    const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
-    const MG m = march_geo(pg, kMarchPlanes);
-    CoefPack cp{c->met, c->met32};
-    size_t need = 4 * (size_t)nb + 64;
-    if (need < 4 * (size_t)m.nblocks + 64) need = 4 * (size_t)m.nblocks + 64;
-    if (need < 4 * (size_t)kUpdateBlocks) need = 4 * (size_t)kUpdateBlocks;
-    DCM_CHECK(ensure_partials(c, need));
-    DCM_CHECK(ensure_solver_vectors32(c, 10));
-    float *V[10];
-    for (int q = 0; q < 10; ++q) V[q] = c->sv32 + (size_t)q * g.n;
-    float *d = V[9];
-    double *r64;  // FP64 true residual, compact
-    DCM_CHECK(work_field64(c, 2, &r64));
-    SolverScalars *sc = c->solver;
-    out->iters = 0;
-    out->outer = 0;
-    out->fallback = 0;
-    double sc_err = 0.0, err_prev = 0.0;
-    double *xsave = nullptr;  // x before the last correction
+// Mixed precision for a batch: FP32 inner BiCGSTAB (all members in one
+// batched loop) inside each member's FP64 iterative refinement.
+int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, int itmax,
+                         double inner_rtol, SolverOut *outs, int *status) {
+    struct Mem {
+        double sc_err = 0.0, err_prev = 0.0;
+        double *r64 = nullptr, *xsave = nullptr;
+        float *V[10] = {};
+        bool active = true, fallback = false;
+    };
+    std::vector<Mem> M(n);
     const int max_outer = 50;
     const int inner_cap = 2000;  // FP32 iterations per refinement sweep
-    for (int outer = 0; outer <= max_outer; ++outer) {
-        // true FP64 residual, its FP32 copy and its mean
-        int tok = prof_begin(c, "resid_true");
-        hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, c->stream, x, b, r64, pg, g, cp,
-                           c->partials, (long long)nb);
-        prof_end(c, tok);
-        hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
-                           c->dscal + 40);
-        DCM_LAUNCH_CHECK();
-        DCM_HIP(hipMemcpyAsync(c->hscal + 40, c->dscal + 40, 3 * sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream));
-        DCM_HIP(hipStreamSynchronize(c->stream));
-        const double rr = c->hscal[40], bb = c->hscal[42];
-        if (outer == 0) {
-            sc_err = sqrt(bb);
-            if (sc_err < 0.01) sc_err = 0.01;
-        }
-        const double err = sqrt(rr) / sc_err;
-        out->err = err;
-        if (outer == 0 && !std::isfinite(err)) {
-            set_error("bicgstab_mixed: non-finite initial residual");
-            return DCM_NONFINITE;
-        }
-        if (outer > 0 && !(err < err_prev)) {
-            // the last FP32 correction did not reduce the true residual (an
-            // inner breakdown, stagnation at FP32 resolution, or overflow):
-            // restore x and let the FP64 solver of the reference finish
-            DCM_HIP(hipMemcpyAsync(x, xsave, pg.ntot * sizeof(double), hipMemcpyDeviceToDevice,
-                                   c->stream));
-            SolverOut o64;
-            const int left = itmax - out->iters > 1 ? itmax - out->iters : 1;
-            const int st = bicgstab64(c, sw, x, b, tol, left, &o64);
-            out->iters += o64.iters;
-            out->err = o64.err;
-            out->fallback = 1;
-            return st;
-        }
-        if (err < tol) break;
-        if (outer == max_outer || out->iters >= itmax) return DCM_NOT_CONVERGED;
-        err_prev = err;
-        out->outer += 1;
-        // inner solve A d = r - mean(r), to a relative reduction inner_rtol
-        hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, c->stream, r64, V[0], V[1], V[2],
-                           V[3], d, g, c->dscal + 41, c->partials, (long long)nb);
-        DCM_LAUNCH_CHECK();
-        // the inner recursive residual is relative to ||r_inner,0||; stop at
-        // inner_rtol, or as soon as the outer target is surely met
-        double rtol = inner_rtol;
-        const double need_rel = 0.5 * tol * sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
-        if (need_rel > rtol) rtol = need_rel;
-        hipLaunchKernelGGL(k_fin_inner_init, dim3(1), dim3(1024), 0, c->stream, c->partials, nb,
-                           sc, rtol);
-        DCM_LAUNCH_CHECK();
-        SolverOut io;
-        int cap = itmax - out->iters;
-        if (cap > inner_cap) cap = inner_cap;
-        (void)bicgstab_iterate<float>(c, V, rtol, cap, &io);
-        out->iters += io.iters;
-        // whatever the inner status, its correction is judged by the true
-        // residual at the top of the loop; a guarded breakdown stops before
-        // the update that would use a non-finite scalar
-        if (!xsave) DCM_CHECK(work_field64(c, 1, &xsave));
-        DCM_HIP(hipMemcpyAsync(xsave, x, pg.ntot * sizeof(double), hipMemcpyDeviceToDevice,
-                               c->stream));
-        hipLaunchKernelGGL(k_add_correction, L.grid, L.block, 0, c->stream, x, d, pg, g);
-        DCM_LAUNCH_CHECK();
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        const Geo &pg = mc->geo;
+        const Interior L = interior_launch(pg);
+        const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+        DCM_CHECK(ensure_partials(mc, 4 * (size_t)nb + 64));
+        DCM_CHECK(ensure_solver_vectors32(mc, 10));
+        const size_t cells = (size_t)pg.ip * pg.jp * pg.kp;
+        for (int v = 0; v < 10; ++v) M[q].V[v] = mc->sv32 + (size_t)v * cells;
+        DCM_CHECK(work_field64(mc, 2, &M[q].r64));
+        outs[q] = SolverOut();
+        status[q] = DCM_OK;
     }
-    DCM_CHECK(fill<double>(c, x, FILL_ALONG_DIM_WO, 0.0, 0.0));
-    hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, c->stream, x, sw, pg, c->h_dzn[0],
-                       1);
-    DCM_LAUNCH_CHECK();
-    return DCM_OK;
+    for (int outer = 0; outer <= max_outer; ++outer) {
+        // true FP64 residual of every active member, its sums to the host
+        for (int q = 0; q < n; ++q) {
+            if (!M[q].active) continue;
+            dcm_ctx *mc = tasks[q].c;
+            const Geo &pg = mc->geo;
+            const CGeo g = cgeo(pg);
+            const Interior L = interior_launch(pg);
+            const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+            CoefPack cp{mc->met, mc->met32};
+            int tok = prof_begin(c, "resid_true");
+            hipLaunchKernelGGL(k_resid_true, L.grid, L.block, 0, mc->stream, tasks[q].x,
+                               tasks[q].b, M[q].r64, pg, g, cp, mc->partials, (long long)nb);
+            prof_end(c, tok);
+            hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, mc->stream,
+                               mc->partials, nb, mc->dscal + 40);
+            DCM_LAUNCH_CHECK();
+            DCM_HIP(hipMemcpyAsync(mc->hscal + 40, mc->dscal + 40, 3 * sizeof(double),
+                                   hipMemcpyDeviceToHost, mc->stream));
+        }
+        for (int q = 0; q < n; ++q)
+            if (M[q].active) DCM_HIP(hipStreamSynchronize(tasks[q].c->stream));
+        std::vector<SolveSys> sys;
+        std::vector<int> idx;
+        for (int q = 0; q < n; ++q) {
+            if (!M[q].active) continue;
+            dcm_ctx *mc = tasks[q].c;
+            const Geo &pg = mc->geo;
+            const CGeo g = cgeo(pg);
+            const Interior L = interior_launch(pg);
+            const int nb = (int)(L.grid.x * L.grid.y * L.grid.z);
+            const double rr = mc->hscal[40], bb = mc->hscal[42];
+            if (outer == 0) {
+                M[q].sc_err = sqrt(bb);
+                if (M[q].sc_err < 0.01) M[q].sc_err = 0.01;
+            }
+            const double err = sqrt(rr) / M[q].sc_err;
+            outs[q].err = err;
+            if (outer == 0 && !std::isfinite(err)) {
+                set_error("bicgstab_mixed: non-finite initial residual");
+                status[q] = DCM_NONFINITE;
+                M[q].active = false;
+                continue;
+            }
+            if (outer > 0 && !(err < M[q].err_prev)) {
+                // the last FP32 correction did not reduce the true residual
+                // (an inner breakdown, stagnation at FP32 resolution, or
+                // overflow): restore x and let the reference's FP64 solver
+                // finish this member
+                DCM_HIP(hipMemcpyAsync(tasks[q].x, M[q].xsave, pg.ntot * sizeof(double),
+                                       hipMemcpyDeviceToDevice, mc->stream));
+                M[q].active = false;
+                M[q].fallback = true;
+                continue;
+            }
+            if (err < tol) {
+                M[q].active = false;
+                continue;
+            }
+            if (outer == max_outer || outs[q].iters >= itmax) {
+                status[q] = DCM_NOT_CONVERGED;
+                M[q].active = false;
+                continue;
+            }
+            M[q].err_prev = err;
+            outs[q].outer += 1;
+            // inner solve A d = r - mean(r), to a relative reduction
+            // inner_rtol, or as soon as the outer target is surely met
+            hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, mc->stream, M[q].r64, M[q].V[0],
+                               M[q].V[1], M[q].V[2], M[q].V[3], M[q].V[9], g, mc->dscal + 41,
+                               mc->partials, (long long)nb);
+            double rtol = inner_rtol;
+            const double need_rel = 0.5 * tol * M[q].sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
+            if (need_rel > rtol) rtol = need_rel;
+            hipLaunchKernelGGL(k_fin_inner_init, dim3(1), dim3(1024), 0, mc->stream,
+                               mc->partials, nb, mc->solver, rtol);
+            DCM_LAUNCH_CHECK();
+            SolveSys sy;
+            sy.c = mc;
+            for (int v = 0; v < 10; ++v) sy.V[v] = M[q].V[v];
+            sy.march_planes = mc->march_planes > 0 ? mc->march_planes : kMarchPlanes;
+            sys.push_back(sy);
+            idx.push_back(q);
+        }
+        if (sys.empty()) break;
+        int cap = inner_cap;
+        for (int u : idx)
+            if (itmax - outs[u].iters < cap) cap = itmax - outs[u].iters;
+        if (cap < 1) cap = 1;
+        // (each member's inner tolerance is in its scalars)
+        std::vector<SolverOut> o(sys.size());
+        std::vector<int> st(sys.size());
+        (void)bicgstab_iterate_batch<float>(c, sys.data(), (int)sys.size(), cap, o.data(),
+                                            st.data());
+        for (size_t u = 0; u < sys.size(); ++u) {
+            const int q = idx[u];
+            dcm_ctx *mc = tasks[q].c;
+            const Geo &pg = mc->geo;
+            const CGeo g = cgeo(pg);
+            const Interior L = interior_launch(pg);
+            outs[q].iters += o[u].iters;
+            // whatever the inner status, its correction is judged by the true
+            // residual at the top of the loop; a guarded breakdown stops
+            // before the update that would use a non-finite scalar
+            if (!M[q].xsave) DCM_CHECK(work_field64(mc, 1, &M[q].xsave));
+            DCM_HIP(hipMemcpyAsync(M[q].xsave, tasks[q].x, pg.ntot * sizeof(double),
+                                   hipMemcpyDeviceToDevice, mc->stream));
+            hipLaunchKernelGGL(k_add_correction, L.grid, L.block, 0, mc->stream, tasks[q].x,
+                               M[q].V[9], pg, g);
+            DCM_LAUNCH_CHECK();
+        }
+    }
+    // members whose FP32 corrections stalled: the FP64 solver, batched
+    std::vector<SolveTask> fb;
+    std::vector<int> fidx;
+    for (int q = 0; q < n; ++q)
+        if (M[q].fallback) {
+            fb.push_back(tasks[q]);
+            fidx.push_back(q);
+        }
+    if (!fb.empty()) {
+        std::vector<SolverOut> o(fb.size());
+        std::vector<int> st(fb.size());
+        (void)bicgstab64_batch(c, fb.data(), (int)fb.size(), tol, itmax, o.data(), st.data());
+        for (size_t u = 0; u < fb.size(); ++u) {
+            const int q = fidx[u];
+            outs[q].iters += o[u].iters;
+            outs[q].err = o[u].err;
+            outs[q].fallback = 1;
+            status[q] = st[u];
+        }
+    }
+    int worst = DCM_OK;
+    for (int q = 0; q < n; ++q) {
+        if (status[q] == DCM_OK && !M[q].fallback) {
+            dcm_ctx *mc = tasks[q].c;
+            DCM_CHECK(fill<double>(mc, tasks[q].x, FILL_ALONG_DIM_WO, 0.0, 0.0));
+            hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, mc->stream, tasks[q].x,
+                               tasks[q].sw, mc->geo, mc->h_dzn[0], 1);
+            DCM_LAUNCH_CHECK();
+        }
+        if (status[q] != DCM_OK && worst == DCM_OK) worst = status[q];
+    }
+    return worst;
+}
+
+int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
+                   double inner_rtol, SolverOut *out) {
+    const SolveTask t{c, sw, x, b};
+    int st = DCM_OK;
+    DCM_CHECK(bicgstab_mixed_batch(c, &t, 1, tol, itmax, inner_rtol, out, &st));
+    return st;
 }
 
 }  // namespace dcm

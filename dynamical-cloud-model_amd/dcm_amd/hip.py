@@ -94,6 +94,10 @@ _SIGS = {
     "dcm_model_set_check_residual": ([_vp, _i], _i),
     "dcm_stream_copy": ([_vp, _vp, _vp, C.c_size_t, _dp], _i),
     "dcm_model_kernel_time": ([_vp, C.c_char_p, _dp, _ip], _i),
+    "dcm_ensemble_create": ([_i, C.POINTER(_vp), C.POINTER(_vp)], _i),
+    "dcm_ensemble_destroy": ([_vp], _i),
+    "dcm_ensemble_autotune": ([_vp, _ip], _i),
+    "dcm_ensemble_step": ([_vp, _i, C.POINTER(StepStats)], _i),
 }
 
 _lib = None
@@ -371,3 +375,38 @@ class Model:
         if self.h:
             _call("dcm_model_destroy", self.h)
             self.h = None
+
+
+class Ensemble:
+    """dcm_ensemble: independent models stepped together; their pressure
+    solves run as one batched BiCGSTAB.  The models stay owned by the
+    caller and must not be stepped on their own while in the ensemble."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        arr = (_vp * len(self.models))(*[m.h for m in self.models])
+        h = _vp()
+        _call("dcm_ensemble_create", len(self.models), arr, C.byref(h))
+        self.h = h
+
+    def autotune(self) -> int:
+        planes = _i()
+        _call("dcm_ensemble_autotune", self.h, C.byref(planes))
+        return planes.value
+
+    def step(self, n=1):
+        """Advance every member n steps; one StepStats per member."""
+        st = (StepStats * len(self.models))()
+        _call("dcm_ensemble_step", self.h, n, st)
+        return list(st)
+
+    def close(self):
+        if self.h:
+            _call("dcm_ensemble_destroy", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -242,6 +242,25 @@ int dcm_model_set_inner_rtol(dcm_model *m, double rtol);
 int dcm_model_set_check_residual(dcm_model *m, int on);
 
 /* ---------------------------------------------------------------------- */
+/* Ensembles (batching): independent models of uneven size stepped        */
+/* together.  Every member's kernels run on one stream of the ensemble,   */
+/* and the pressure solves of all members are one batched BiCGSTAB: each  */
+/* launch serves every member that has not converged.  No counterpart in  */
+/* the reference, which runs one domain per job.  Members keep their own  */
+/* models and contexts (created as usual) and must share precision and    */
+/* ptol; while in an ensemble a member must not be stepped on its own.    */
+/* ---------------------------------------------------------------------- */
+typedef struct dcm_ensemble dcm_ensemble;
+int dcm_ensemble_create(int n, dcm_model *const *models, dcm_ensemble **out);
+int dcm_ensemble_destroy(dcm_ensemble *e);
+/* Time the batched iteration at each candidate i-plane count of the solver
+ * tiles (4, 8, 16) on the members' current systems and keep the fastest;
+ * the model state is not changed.  *march_planes (may be NULL) gets it. */
+int dcm_ensemble_autotune(dcm_ensemble *e, int *march_planes);
+/* Advance every member nsteps steps; stats (may be NULL) is an array of n. */
+int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats);
+
+/* ---------------------------------------------------------------------- */
 /* Measurement utility (no counterpart in the reference): a plain device   */
 /* copy of n doubles, the STREAM-copy yardstick and the byte calibration   */
 /* for hardware counters.  Returns the kernel time in ms when ms != NULL.  */

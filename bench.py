@@ -95,6 +95,63 @@ def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rto
     return res, snap, (ctx, model)
 
 
+def run_ensemble(H, W, warmup, steps, members=64, seed=404):
+    """The uneven warm-bubble ensemble two ways: every member's pressure
+    solve batched into one mixed-precision BiCGSTAB per step (after the
+    autotuner picks the solver tile), and the members one after another on
+    the plain FP64 path.  Cell-steps/s over the whole ensemble."""
+    cases = W.ensemble_members(members, seed=seed, steps=warmup + steps)
+    cells = sum(c.dom.cells for c in cases)
+    out = {"members": len(cases), "cells": cells,
+           "shapes": sorted({f"{c.dom.ip}x{c.dom.jp}x{c.dom.kp}" for c in cases})}
+    ctxs = [H.Context(c.dom) for c in cases]
+    models = [H.Model(x, c, H.MIXED) for x, c in zip(ctxs, cases)]
+    try:
+        with H.Ensemble(models) as e:
+            t0 = time.perf_counter()
+            planes = e.autotune()
+            tune_s = time.perf_counter() - t0
+            if warmup:
+                e.step(warmup)
+            t0 = time.perf_counter()
+            stats = e.step(steps)
+            wall = time.perf_counter() - t0
+        out["batched_mixed"] = {
+            "value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "autotuned_march_planes": planes, "autotune_s": tune_s,
+            "iters_total": sum(s.iters_total for s in stats),
+            "iters_max_member": max(s.iters_max for s in stats),
+            "fallbacks": sum(s.fallbacks for s in stats),
+            "max_true_resid": max(s.max_true_resid for s in stats)}
+    finally:
+        for m in models:
+            m.close()
+        for x in ctxs:
+            x.close()
+    wall = 0.0
+    iters = 0
+    resid = 0.0
+    for c in cases:
+        with H.Context(c.dom) as x:
+            m = H.Model(x, c, H.FP64)
+            try:
+                m.set_check_residual(True)
+                if warmup:
+                    m.step(warmup)
+                t0 = time.perf_counter()
+                st = m.step(steps)
+                wall += time.perf_counter() - t0
+                iters += st.iters_total
+                resid = max(resid, st.max_true_resid)
+            finally:
+                m.close()
+    out["sequential_fp64"] = {"value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                              "iters_total": iters, "max_true_resid": resid}
+    out["speedup_batched_mixed_vs_sequential_fp64"] = (out["batched_mixed"]["value"] /
+                                                       out["sequential_fp64"]["value"])
+    return out
+
+
 def rel_l2(a, b, center=False):
     a = a[1:-1, 1:-1, 1:-1]
     b = b[1:-1, 1:-1, 1:-1]
@@ -156,6 +213,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", default="64x64:1")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10,
+                    help="timed steps of the 64-member ensemble comparison (0: skip)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
@@ -243,6 +302,14 @@ def main(argv=None):
         a, b = snaps["fp64"], snaps["mixed"]
         line["mixed_error_vs_fp64"] = {
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
+    if args.ensemble_steps > 0:
+        print(f"[bench] ensemble: 64 uneven members, {args.ensemble_steps} steps",
+              file=sys.stderr, flush=True)
+        line["ensemble"] = run_ensemble(H, W, 1, args.ensemble_steps)
+        ens = line["ensemble"]
+        print(f"[bench] ensemble: batched mixed {ens['batched_mixed']['value']:.4g}, "
+              f"sequential FP64 {ens['sequential_fp64']['value']:.4g} cell-steps/s",
+              file=sys.stderr, flush=True)
     if not args.no_stream:
         line["stream_copy_GBps"] = stream_copy_gbps(H, case.dom)
     if not args.no_cpu:

@@ -41,6 +41,7 @@
 // singular (periodic + Neumann), so each inner right-hand side has its mean
 // removed.
 #include <cfloat>
+#include <cstddef>
 #include <cmath>
 
 #include "dcm_device.h"
@@ -56,24 +57,45 @@ __device__ __forceinline__ long long cidx(const CGeo &g, int k, int j, int i) {
     return ((long long)(i - 1) * g.jp + (j - 1)) * g.kp + (k - 1);
 }
 
-template <typename R> struct Coef;
-template <> struct Coef<double> {
-    const double *__restrict__ ae, *__restrict__ aw, *__restrict__ an, *__restrict__ as,
-        *__restrict__ au, *__restrict__ ad;
-    __host__ __device__ Coef(const Metrics &m, const Metrics32 &)
-        : ae(m.ae), aw(m.aw), an(m.an), as(m.as), au(m.au), ad(m.ad) {}
+// Global-memory pointers.  A pointer read from device memory (a member
+// table) is a generic pointer to the compiler, which then issues FLAT
+// accesses; the kernel bodies take their buffers as global pointers, which
+// gives GLOBAL loads and stores whichever way the pointer arrived.
+template <typename T> using gp = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gp<T> glob(T *p) { return (gp<T>)p; }
+
+// the six 1-D coefficient arrays of the operator at precision R
+template <typename R> struct Coef {
+    gp<const R> __restrict__ ae;
+    gp<const R> __restrict__ aw;
+    gp<const R> __restrict__ an;
+    gp<const R> __restrict__ as;
+    gp<const R> __restrict__ au;
+    gp<const R> __restrict__ ad;
+    template <typename M> __device__ explicit Coef(const M &m)
+        : ae(glob(m.ae)), aw(glob(m.aw)), an(glob(m.an)), as(glob(m.as)), au(glob(m.au)),
+          ad(glob(m.ad)) {}
+    __device__ Coef(const Metrics &m64, const Metrics32 &m32);
 };
-template <> struct Coef<float> {
-    const float *__restrict__ ae, *__restrict__ aw, *__restrict__ an, *__restrict__ as,
-        *__restrict__ au, *__restrict__ ad;
-    __host__ __device__ Coef(const Metrics &, const Metrics32 &m)
-        : ae(m.ae), aw(m.aw), an(m.an), as(m.as), au(m.au), ad(m.ad) {}
-};
+template <> __device__ inline Coef<double>::Coef(const Metrics &m, const Metrics32 &)
+    : Coef(m) {}
+template <> __device__ inline Coef<float>::Coef(const Metrics &, const Metrics32 &m)
+    : Coef(m) {}
 
 struct CoefPack {
     Metrics m64;
     Metrics32 m32;
 };
+
+// the iteration has stopped for this system (converged or broken down):
+// breakdown and done are adjacent ints, tested with one 8-byte load
+static_assert(offsetof(SolverScalars, done) == offsetof(SolverScalars, breakdown) + 4 &&
+                  offsetof(SolverScalars, breakdown) % 8 == 0,
+              "SolverScalars: breakdown and done must share an aligned 8-byte word");
+__device__ __forceinline__ bool halted(gp<const SolverScalars> sc) {
+    return *(gp<const unsigned long long>)&sc->breakdown != 0ull;
+}
+__device__ __forceinline__ bool halted(const SolverScalars *sc) { return halted(glob(sc)); }
 
 // a_p of set_mat_a (dynamics.f90:90-92)
 template <typename R>
@@ -145,7 +167,10 @@ static MG march_geo(const Geo &g, int ic) {
     m.kp = g.kp;
     m.sj = g.kp;
     m.si = g.kp * g.jp;
-    m.kt = g.kp > 32 ? 64 : 32;
+    // 64 lanes along k unless 32 leaves fewer idle lanes (kp = 96: three
+    // full tiles of 32 instead of a full and a half-empty tile of 64)
+    const int w64 = (64 - g.kp % 64) % 64, w32 = (32 - g.kp % 32) % 32;
+    m.kt = g.kp > 32 && w64 <= w32 ? 64 : 32;
     m.jt = kBlock / m.kt;
     m.nkb = (g.kp + m.kt - 1) / m.kt;
     m.njb = (g.jp + m.jt - 1) / m.jt;
@@ -156,16 +181,23 @@ static MG march_geo(const Geo &g, int ic) {
 }
 static unsigned march_grid(const MG &m) { return (unsigned)(((m.nblocks + 7) / 8) * 8); }
 
-// Tile of this block (blocks renumbered so that XCD x = id mod 8 owns the
-// contiguous tile range [x*per, (x+1)*per)).  False for the padding blocks.
+// Tile of block b of a one-system launch: blocks are renumbered so that XCD
+// x = b mod 8 owns the contiguous tile range [x*per, (x+1)*per).  -1 for the
+// padding blocks.
+__device__ __forceinline__ int march_tile_id(const MG &m, int b) {
+    const int per = (m.nblocks + 7) >> 3;
+    const int t = (b & 7) * per + (b >> 3);
+    return t < m.nblocks ? t : -1;
+}
+// the cells of tile `tile` (kb fastest, then jb, then the i chunk); false
+// for a padding block (tile < 0)
 struct Tile {
     int k, j, i0, i1, tile;
     bool act;
 };
-__device__ __forceinline__ bool march_tile(const MG &m, int b, Tile &T) {
-    const int per = (m.nblocks + 7) >> 3;
-    T.tile = (b & 7) * per + (b >> 3);
-    if (T.tile >= m.nblocks) return false;
+__device__ __forceinline__ bool march_tile(const MG &m, int tile, Tile &T) {
+    if (tile < 0) return false;
+    T.tile = tile;
     const int kb = T.tile % m.nkb;
     const int r = T.tile / m.nkb;
     const int jb = r % m.njb;
@@ -183,10 +215,14 @@ __device__ __forceinline__ bool march_tile(const MG &m, int b, Tile &T) {
 // Solve members.  The iteration kernels serve any number of independent
 // systems (one domain, or the members of an ensemble) in one launch: a block
 // finds its member in a table and works on that member's vectors with that
-// member's geometry, coefficients and scalars.  Members own disjoint,
-// 8-aligned ranges of the marching grid (so the XCD renumbering of a member
-// is unchanged) and of the update grid; the one-block reductions run one
-// block per member.  With one member the block maps are not needed.
+// member's geometry, coefficients and scalars.  The marching table gives
+// each block a (member, tile) pair: the members' tiles are concatenated in
+// member order and the list is split into 8 contiguous ranges, one per XCD,
+// as a single domain's tiles are, so a small member stays within one XCD's
+// L2 and a large one is split into i-chunk ranges.  Partial sums are stored
+// by tile, so a member's sums are those of the same system solved alone.
+// The update grid is split into per-member block ranges; the one-block
+// reductions run one block per member.  With one member no table is needed.
 // ---------------------------------------------------------------------------
 struct IterMember {
     MG m;
@@ -195,23 +231,29 @@ struct IterMember {
     double *part;     // partial sums: 2 slots of max(nblocks, nbu)
     void *V[10];      // r, cr, p, v1, t, s, cs, wa, wb, x (compact, precision R)
     int n, nbu;       // cells; update-kernel blocks
-    int mblock0, ublock0;
+    int ublock0;      // first update block
 };
 enum { VR = 0, VCR, VP, VV1, VT, VS, VCS, VWA, VWB, VX };
 
+// marching table entry: member << kTileBits | tile; kNoTile for padding
+constexpr int kTileBits = 20;
+constexpr unsigned kNoTile = 0xffffffffu;
+constexpr int kMaxMembers = 1 << (32 - kTileBits);
+
 struct IterArgs {
     const IterMember *mem;
-    const int *mmap;  // marching block -> member (null with one member)
-    const int *umap;  // update block -> member (null with one member)
+    const unsigned *mmap;  // marching block -> member and tile (batched launches)
+    const int *umap;       // update block -> member (batched launches)
 };
 
-__device__ __forceinline__ const IterMember &member_m(const IterArgs &A, int &lb) {
-    const IterMember &M = A.mem[A.mmap ? A.mmap[blockIdx.x] : 0];
-    lb = (int)blockIdx.x - M.mblock0;
-    return M;
+// the member of a marching block and its tile (-1: a padding block)
+__device__ __forceinline__ const IterMember &member_m(const IterArgs &A, int &tile) {
+    const unsigned e = A.mmap[blockIdx.x];
+    tile = e == kNoTile ? -1 : (int)(e & ((1u << kTileBits) - 1));
+    return A.mem[e == kNoTile ? 0 : (e >> kTileBits)];
 }
 __device__ __forceinline__ const IterMember &member_u(const IterArgs &A, int &lb) {
-    const IterMember &M = A.mem[A.umap ? A.umap[blockIdx.x] : 0];
+    const IterMember &M = A.mem[A.umap[blockIdx.x]];
     lb = (int)blockIdx.x - M.ublock0;
     return M;
 }
@@ -306,7 +348,7 @@ __global__ void k_seq_dots(IterArgs A, int a0i, int b0i, int a1i, int b1i) {
     const IterMember &M = A.mem[blockIdx.x];
     if (threadIdx.x != 0) return;
     const SolverScalars *sc = M.sc;
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const R *a0 = (const R *)M.V[a0i], *b0 = (const R *)M.V[b0i];
     const R *a1 = a1i >= 0 ? (const R *)M.V[a1i] : nullptr;
     const R *b1 = b1i >= 0 ? (const R *)M.V[b1i] : nullptr;
@@ -426,18 +468,18 @@ __device__ __forceinline__ int fin_count(const IterMember &M, int kind, int seq)
     return kind == 0 ? M.m.nblocks : M.nbu;
 }
 __global__ void k_fin_v1(const double *partials, int nb, SolverScalars *sc) {
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const double nrm = reduce_slot_1024(partials, nb);
     if (threadIdx.x == 0) fin_v_apply(sc, nrm);
 }
 __global__ void k_fin_t1(const double *partials, int nb, SolverScalars *sc) {
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const double tt = reduce_slot_1024(partials, nb);
     const double ts = reduce_slot_1024(partials + nb, nb);
     if (threadIdx.x == 0) fin_t_apply(sc, tt, ts);
 }
 __global__ void k_fin_r1(const double *partials, int nb, SolverScalars *sc) {
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const double rr = reduce_slot_1024(partials, nb);
     const double rcr = reduce_slot_1024(partials + nb, nb);
     if (threadIdx.x == 0) fin_r_apply(sc, rr, rcr);
@@ -445,14 +487,14 @@ __global__ void k_fin_r1(const double *partials, int nb, SolverScalars *sc) {
 __global__ void k_fin_v(IterArgs A, int seq) {
     const IterMember &M = A.mem[blockIdx.x];
     SolverScalars *sc = M.sc;
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const double nrm = reduce_slot_1024(M.part, fin_count(M, 0, seq));
     if (threadIdx.x == 0) fin_v_apply(sc, nrm);
 }
 __global__ void k_fin_t(IterArgs A, int seq) {
     const IterMember &M = A.mem[blockIdx.x];
     SolverScalars *sc = M.sc;
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const int nb = fin_count(M, 0, seq);
     const double tt = reduce_slot_1024(M.part, nb);
     const double ts = reduce_slot_1024(M.part + nb, nb);
@@ -461,7 +503,7 @@ __global__ void k_fin_t(IterArgs A, int seq) {
 __global__ void k_fin_r(IterArgs A, int seq) {
     const IterMember &M = A.mem[blockIdx.x];
     SolverScalars *sc = M.sc;
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const int nb = fin_count(M, 1, seq);
     const double rr = reduce_slot_1024(M.part, nb);
     const double rcr = reduce_slot_1024(M.part + nb, nb);
@@ -481,7 +523,7 @@ __global__ void k_gather_flags(IterArgs A, int nmem, int *flags) {
 // MODE 0: t = r - bet*(omg*v1)     (dynamics.f90:394)
 // MODE 1: t = r - alf*v1           (dynamics.f90:405, the "s" vector)
 template <typename R, int MODE>
-__device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__ v1, int q, R a,
+__device__ __forceinline__ R tval(gp<const R> __restrict__ r, gp<const R> __restrict__ v1, int q, R a,
                                   R b) {
     if (MODE == 0) return r[q] - a * (b * v1[q]);
     return r[q] - a * v1[q];
@@ -490,14 +532,14 @@ __device__ __forceinline__ R tval(const R *__restrict__ r, const R *__restrict__
 // First sweep, variant 0: fused in registers, s0 = t/a_p evaluated at the
 // centre and the six neighbours (dynamics.f90:254-266).
 template <typename R, int MODE>
-__device__ __forceinline__ void jac_first_reg_body(const R *__restrict__ r,
-                                                   const R *__restrict__ v1,
-                                                   R *__restrict__ t_out, R *__restrict__ s_out,
+__device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
+                                                   gp<const R> __restrict__ v1,
+                                                   gp<R> __restrict__ t_out, gp<R> __restrict__ s_out,
                                                    const MG &m, const CoefPack &cp,
-                                                   const SolverScalars *__restrict__ sc, int lb) {
-    if (sc->done || sc->breakdown) return;
+                                                   gp<const SolverScalars> __restrict__ sc, int tile) {
+    if (halted(sc)) return;
     Tile T;
-    if (!march_tile(m, lb, T) || !T.act) return;
+    if (!march_tile(m, tile, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
@@ -540,27 +582,29 @@ __global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
                                                        R *__restrict__ t_out,
                                                        R *__restrict__ s_out, MG m, CoefPack cp,
                                                        const SolverScalars *__restrict__ sc) {
-    jac_first_reg_body<R, MODE>(r, v1, t_out, s_out, m, cp, sc, (int)blockIdx.x);
+    jac_first_reg_body<R, MODE>(glob(r), glob(v1), glob(t_out), glob(s_out), m, cp, glob(sc),
+                                march_tile_id(m, (int)blockIdx.x));
 }
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_jac_first_reg_b(IterArgs A, int ri, int vi, int ti,
                                                          int oi) {
-    int lb;
-    const IterMember &M = member_m(A, lb);
-    jac_first_reg_body<R, MODE>((const R *)M.V[ri], (const R *)M.V[vi], (R *)M.V[ti],
-                                (R *)M.V[oi], M.m, M.cp, M.sc, lb);
+    int tile;
+    const IterMember &M = member_m(A, tile);
+    jac_first_reg_body<R, MODE>(glob((const R *)M.V[ri]), glob((const R *)M.V[vi]),
+                                glob((R *)M.V[ti]), glob((R *)M.V[oi]), M.m, M.cp,
+                                glob((const SolverScalars *)M.sc), tile);
 }
 
 // First sweep, variant 2: a streaming pass t = r - bet*(omg*v1) (or
 // r - alf*v1) with s0 = t/a_p, then the plain sweep k_jac on (s0, t).
 template <typename R, int MODE>
-__device__ __forceinline__ void jac_prep_body(const R *__restrict__ r, const R *__restrict__ v1,
-                                              R *__restrict__ t_out, R *__restrict__ s0_out,
+__device__ __forceinline__ void jac_prep_body(gp<const R> __restrict__ r, gp<const R> __restrict__ v1,
+                                              gp<R> __restrict__ t_out, gp<R> __restrict__ s0_out,
                                               const MG &m, const CoefPack &cp,
-                                              const SolverScalars *__restrict__ sc, int lb) {
-    if (sc->done || sc->breakdown) return;
+                                              gp<const SolverScalars> __restrict__ sc, int tile) {
+    if (halted(sc)) return;
     Tile T;
-    if (!march_tile(m, lb, T) || !T.act) return;
+    if (!march_tile(m, tile, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
@@ -580,25 +624,27 @@ __global__ __launch_bounds__(256) void k_jac_prep(const R *__restrict__ r,
                                                   R *__restrict__ t_out, R *__restrict__ s0_out,
                                                   MG m, CoefPack cp,
                                                   const SolverScalars *__restrict__ sc) {
-    jac_prep_body<R, MODE>(r, v1, t_out, s0_out, m, cp, sc, (int)blockIdx.x);
+    jac_prep_body<R, MODE>(glob(r), glob(v1), glob(t_out), glob(s0_out), m, cp, glob(sc),
+                           march_tile_id(m, (int)blockIdx.x));
 }
 template <typename R, int MODE>
 __global__ __launch_bounds__(256) void k_jac_prep_b(IterArgs A, int ri, int vi, int ti, int oi) {
-    int lb;
-    const IterMember &M = member_m(A, lb);
-    jac_prep_body<R, MODE>((const R *)M.V[ri], (const R *)M.V[vi], (R *)M.V[ti], (R *)M.V[oi],
-                           M.m, M.cp, M.sc, lb);
+    int tile;
+    const IterMember &M = member_m(A, tile);
+    jac_prep_body<R, MODE>(glob((const R *)M.V[ri]), glob((const R *)M.V[vi]),
+                           glob((R *)M.V[ti]), glob((R *)M.V[oi]), M.m, M.cp,
+                           glob((const SolverScalars *)M.sc), tile);
 }
 
 // s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead.  The
 // loads of plane i+1 are issued while plane i is computed.
 template <typename R, bool P_UPDATE>
-__device__ __forceinline__ void jac_body(const R *__restrict__ s, const R *__restrict__ t,
-                                         R *__restrict__ s_out, const MG &m, const CoefPack &cp,
-                                         const SolverScalars *__restrict__ sc, int lb) {
-    if (sc->done || sc->breakdown) return;
+__device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R> __restrict__ t,
+                                         gp<R> __restrict__ s_out, const MG &m, const CoefPack &cp,
+                                         gp<const SolverScalars> __restrict__ sc, int tile) {
+    if (halted(sc)) return;
     Tile T;
-    if (!march_tile(m, lb, T) || !T.act) return;
+    if (!march_tile(m, tile, T) || !T.act) return;
     const Coef<R> C(cp.m64, cp.m32);
     const int k = T.k, j = T.j;
     const Cols q = cols(m, k, j);
@@ -653,14 +699,16 @@ template <typename R, bool P_UPDATE>
 __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
                                              R *__restrict__ s_out, MG m, CoefPack cp,
                                              const SolverScalars *__restrict__ sc) {
-    jac_body<R, P_UPDATE>(s, t, s_out, m, cp, sc, (int)blockIdx.x);
+    jac_body<R, P_UPDATE>(glob(s), glob(t), glob(s_out), m, cp, glob(sc),
+                          march_tile_id(m, (int)blockIdx.x));
 }
 template <typename R, bool P_UPDATE>
 __global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int oi) {
-    int lb;
-    const IterMember &M = member_m(A, lb);
-    jac_body<R, P_UPDATE>((const R *)M.V[si_], (const R *)M.V[ti], (R *)M.V[oi], M.m, M.cp, M.sc,
-                          lb);
+    int tile;
+    const IterMember &M = member_m(A, tile);
+    jac_body<R, P_UPDATE>(glob((const R *)M.V[si_]), glob((const R *)M.V[ti]),
+                          glob((R *)M.V[oi]), M.m, M.cp,
+                          glob((const SolverScalars *)M.sc), tile);
 }
 
 // y = A x; partial sums of <a,y> (and <y,y> first when TWO).  Loads of plane
@@ -669,15 +717,15 @@ __global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int 
 // overall: every block's device-scope release fence writes back its XCD's
 // L2.  One-block kernels reduce instead.)
 template <typename R, bool TWO>
-__device__ __forceinline__ void matvec_dot_body(const R *__restrict__ x, const R *__restrict__ a,
-                                                R *__restrict__ y, const MG &m,
+__device__ __forceinline__ void matvec_dot_body(gp<const R> __restrict__ x, gp<const R> __restrict__ a,
+                                                gp<R> __restrict__ y, const MG &m,
                                                 const CoefPack &cp,
-                                                const SolverScalars *__restrict__ sc,
-                                                double *partials, int lb) {
-    if (sc->done || sc->breakdown) return;
+                                                gp<const SolverScalars> __restrict__ sc,
+                                                double *partials, int tile) {
+    if (halted(sc)) return;
     const int stride = m.nblocks;
     Tile T;
-    if (!march_tile(m, lb, T)) return;
+    if (!march_tile(m, tile, T)) return;
     double acc[2] = {0.0, 0.0};
     if (T.act) {
         const Coef<R> C(cp.m64, cp.m32);
@@ -737,26 +785,28 @@ __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
                                                     MG m, CoefPack cp,
                                                     const SolverScalars *__restrict__ sc,
                                                     double *partials) {
-    matvec_dot_body<R, TWO>(x, a, y, m, cp, sc, partials, (int)blockIdx.x);
+    matvec_dot_body<R, TWO>(glob(x), glob(a), glob(y), m, cp, glob(sc), partials,
+                            march_tile_id(m, (int)blockIdx.x));
 }
 template <typename R, bool TWO>
 __global__ __launch_bounds__(256) void k_matvec_dot_b(IterArgs A, int xi, int ai, int yi) {
-    int lb;
-    const IterMember &M = member_m(A, lb);
-    matvec_dot_body<R, TWO>((const R *)M.V[xi], (const R *)M.V[ai], (R *)M.V[yi], M.m, M.cp,
-                            M.sc, M.part, lb);
+    int tile;
+    const IterMember &M = member_m(A, tile);
+    matvec_dot_body<R, TWO>(glob((const R *)M.V[xi]), glob((const R *)M.V[ai]),
+                            glob((R *)M.V[yi]), M.m, M.cp,
+                            glob((const SolverScalars *)M.sc), M.part, tile);
 }
 
 // x = x + alf*p + omg*cs; r = s - omg*t; partial sums of <r,r>, <r,cr>
 // (dynamics.f90:419-423), all compact vectors, grid-stride
 template <typename R>
-__device__ __forceinline__ void update_body(R *__restrict__ x, const R *__restrict__ p,
-                                            const R *__restrict__ cs, const R *__restrict__ s,
-                                            const R *__restrict__ t, const R *__restrict__ cr,
-                                            R *__restrict__ r, int n, int nbu,
-                                            const SolverScalars *__restrict__ sc,
+__device__ __forceinline__ void update_body(gp<R> __restrict__ x, gp<const R> __restrict__ p,
+                                            gp<const R> __restrict__ cs, gp<const R> __restrict__ s,
+                                            gp<const R> __restrict__ t, gp<const R> __restrict__ cr,
+                                            gp<R> __restrict__ r, int n, int nbu,
+                                            gp<const SolverScalars> __restrict__ sc,
                                             double *partials, int lb) {
-    if (sc->done || sc->breakdown) return;
+    if (halted(sc)) return;
     const int stride = nbu;
     const R alf = (R)sc->alf, omg = (R)sc->omg;
     double acc[2] = {0.0, 0.0};
@@ -778,14 +828,17 @@ __global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__re
                                                 int n, int nbu,
                                                 const SolverScalars *__restrict__ sc,
                                                 double *partials) {
-    update_body<R>(x, p, cs, s, t, cr, r, n, nbu, sc, partials, (int)blockIdx.x);
+    update_body<R>(glob(x), glob(p), glob(cs), glob(s), glob(t), glob(cr), glob(r), n, nbu,
+                   glob(sc), partials, (int)blockIdx.x);
 }
 template <typename R>
 __global__ __launch_bounds__(256) void k_update_b(IterArgs A) {
     int lb;
     const IterMember &M = member_u(A, lb);
-    update_body<R>((R *)M.V[VX], (const R *)M.V[VP], (const R *)M.V[VCS], (const R *)M.V[VS],
-                   (const R *)M.V[VT], (const R *)M.V[VCR], (R *)M.V[VR], M.n, M.nbu, M.sc,
+    update_body<R>(glob((R *)M.V[VX]), glob((const R *)M.V[VP]),
+                   glob((const R *)M.V[VCS]), glob((const R *)M.V[VS]),
+                   glob((const R *)M.V[VT]), glob((const R *)M.V[VCR]),
+                   glob((R *)M.V[VR]), M.n, M.nbu, glob((const SolverScalars *)M.sc),
                    M.part, lb);
 }
 
@@ -985,17 +1038,23 @@ struct SolveSys {
 // uploaded for each solve; `tables` is a device scratch the caller owns.
 struct SolvePlan {
     std::vector<IterMember> mem;
-    std::vector<int> mmap, umap;
+    std::vector<unsigned> mmap;
+    std::vector<int> umap;
     unsigned mgrid = 0, ugrid = 0;
     IterArgs args{};
 };
 
 static int plan_solve(hipStream_t st, const SolveSys *sys, int nsys, SolvePlan &P,
                       DeviceScratch &tables) {
+    if (nsys > kMaxMembers) {
+        set_error("batched solve: more than 4096 systems");
+        return DCM_BAD_ARG;
+    }
     P.mem.resize(nsys);
     P.mmap.clear();
     P.umap.clear();
-    unsigned mb = 0, ub = 0;
+    unsigned ub = 0;
+    long long tiles = 0;
     for (int q = 0; q < nsys; ++q) {
         dcm_ctx *c = sys[q].c;
         IterMember &M = P.mem[q];
@@ -1008,34 +1067,48 @@ static int plan_solve(hipStream_t st, const SolveSys *sys, int nsys, SolvePlan &
         M.sc = c->solver;
         M.part = c->partials;
         for (int v = 0; v < 10; ++v) M.V[v] = sys[q].V[v];
-        M.mblock0 = (int)mb;
         M.ublock0 = (int)ub;
-        const unsigned mg = march_grid(M.m);
-        if (nsys > 1) {
-            P.mmap.insert(P.mmap.end(), mg, q);
-            P.umap.insert(P.umap.end(), (size_t)M.nbu, q);
+        if (M.m.nblocks >= (1 << kTileBits)) {
+            set_error("batched solve: a system has more than 2^20 tiles");
+            return DCM_BAD_ARG;
         }
-        mb += mg;
+        if (nsys > 1) P.umap.insert(P.umap.end(), (size_t)M.nbu, q);
+        tiles += M.m.nblocks;
         ub += (unsigned)M.nbu;
     }
-    P.mgrid = mb;
+    if (nsys == 1) {
+        P.mgrid = march_grid(P.mem[0].m);
+    } else {
+        // the members' tiles in order, split into 8 contiguous XCD ranges:
+        // block b runs entry (b mod 8) * per + b / 8 of the list
+        const long long per = (tiles + 7) / 8;
+        P.mgrid = (unsigned)(8 * per);
+        P.mmap.assign(P.mgrid, kNoTile);
+        long long e = 0;
+        for (int q = 0; q < nsys; ++q)
+            for (int t = 0; t < P.mem[q].m.nblocks; ++t, ++e) {
+                const long long b = (e % per) * 8 + e / per;
+                P.mmap[b] = ((unsigned)q << kTileBits) | (unsigned)t;
+            }
+    }
     P.ugrid = ub;
     const size_t bm = sizeof(IterMember) * nsys;
-    const size_t bmap = sizeof(int) * (P.mmap.size() + P.umap.size());
+    const size_t bmap = sizeof(unsigned) * P.mmap.size() + sizeof(int) * P.umap.size();
     char *d;
     DCM_CHECK(tables.ensure(bm + bmap + 256, (void **)&d));
     DCM_HIP(hipMemcpyAsync(d, P.mem.data(), bm, hipMemcpyHostToDevice, st));
-    int *dmap = (int *)(d + ((bm + 255) / 256) * 256);
+    unsigned *dmap = (unsigned *)(d + ((bm + 255) / 256) * 256);
+    int *dumap = (int *)(dmap + P.mmap.size());
     if (nsys > 1) {
-        DCM_HIP(hipMemcpyAsync(dmap, P.mmap.data(), sizeof(int) * P.mmap.size(),
+        DCM_HIP(hipMemcpyAsync(dmap, P.mmap.data(), sizeof(unsigned) * P.mmap.size(),
                                hipMemcpyHostToDevice, st));
-        DCM_HIP(hipMemcpyAsync(dmap + P.mmap.size(), P.umap.data(),
-                               sizeof(int) * P.umap.size(), hipMemcpyHostToDevice, st));
+        DCM_HIP(hipMemcpyAsync(dumap, P.umap.data(), sizeof(int) * P.umap.size(),
+                               hipMemcpyHostToDevice, st));
     }
     DCM_HIP(hipStreamSynchronize(st));  // the host vectors may change after return
     P.args.mem = (const IterMember *)d;
     P.args.mmap = nsys > 1 ? dmap : nullptr;
-    P.args.umap = nsys > 1 ? dmap + P.mmap.size() : nullptr;
+    P.args.umap = nsys > 1 ? dumap : nullptr;
     return DCM_OK;
 }
 

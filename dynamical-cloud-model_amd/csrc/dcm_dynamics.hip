@@ -1,6 +1,7 @@
 // dcm_dynamics.hip -- momentum sources, horizontal means, the momentum and
 // scalar updates, and the model_driver time step on device-resident state.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 
 #include "dcm_device.h"
@@ -772,9 +773,22 @@ int dcm_model_set_check_residual(dcm_model *m, int on) {
 // member's kernels go to one stream; the pressure solves of all members run
 // in one batched BiCGSTAB (each launch serves every member that has not
 // converged), so the iteration loop costs one set of launches per iteration
-// for the whole ensemble.
+// for the whole ensemble.  The stages before and after the solve (a few
+// dozen small kernels per member) are captured into HIP graphs, one per
+// phase of the three-level time rotation, and replayed from then on.
 // ---------------------------------------------------------------------------
 }  // extern "C"
+
+namespace dcm {
+// side streams of an ensemble (the box's hardware queues per process)
+constexpr int kEnsembleLanes = 4;
+// a captured phase of the step, valid while every member's key (its
+// time-level phase n mod 3, precision and summation order) is unchanged
+struct PhaseGraph {
+    std::vector<int> key;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace dcm
 
 struct dcm_ensemble {
     std::vector<dcm_model *> mem;
@@ -783,9 +797,95 @@ struct dcm_ensemble {
     int precision = DCM_FP64;
     double ptol = 1e-10;
     int march_planes = 0;
+    bool use_graphs = true;
+    bool warm = false;  // one step ran eagerly: every scratch buffer exists
+    dcm::PhaseGraph pre[3], post[3];
+    // the stages around the solve run the members on side streams (members
+    // balanced by cells), forked from and joined back into `stream`
+    std::vector<hipStream_t> side;
+    std::vector<int> lane;  // member -> side stream
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
 namespace dcm {
+
+static void drop_graphs(dcm_ensemble *e) {
+    for (int r = 0; r < 3; ++r) {
+        for (PhaseGraph *g : {&e->pre[r], &e->post[r]}) {
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            g->exec = nullptr;
+            g->key.clear();
+        }
+    }
+}
+
+// Run one phase of every member: replay its graph when one was captured for
+// the members' current time levels, else capture it (the host code of the
+// phase runs once, at capture) and launch it.  `host_replay` repeats the
+// phase's host-side bookkeeping on a replay.
+template <typename Capture, typename Replay>
+static int run_phase(dcm_ensemble *e, PhaseGraph *slots, const std::vector<int> &key,
+                     Capture &&capture, Replay &&host_replay) {
+    for (int r = 0; r < 3; ++r)
+        if (slots[r].exec && slots[r].key == key) {
+            host_replay();
+            DCM_HIP(hipGraphLaunch(slots[r].exec, e->stream));
+            return DCM_OK;
+        }
+    PhaseGraph *slot = &slots[0];
+    for (int r = 0; r < 3; ++r)
+        if (!slots[r].exec) {
+            slot = &slots[r];
+            break;
+        }
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    slot->exec = nullptr;
+    slot->key.clear();
+    DCM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    const int st = capture();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+    if (st != DCM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (ec != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        return DCM_HIP_ERROR;
+    }
+    const hipError_t ei = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        slot->exec = nullptr;
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        return DCM_HIP_ERROR;
+    }
+    slot->key = key;
+    DCM_HIP(hipGraphLaunch(slot->exec, e->stream));
+    return DCM_OK;
+}
+
+// Run `body` on every member with the members spread over the side
+// streams: the side streams wait for the work already queued on the
+// ensemble stream, and the ensemble stream waits for all of them after.
+template <typename Body> static int fan_out(dcm_ensemble *e, Body &&body) {
+    const int n = (int)e->mem.size();
+    DCM_HIP(hipEventRecord(e->fork, e->stream));
+    for (hipStream_t s : e->side) DCM_HIP(hipStreamWaitEvent(s, e->fork, 0));
+    int st = DCM_OK;
+    for (int q = 0; q < n && st == DCM_OK; ++q) {
+        dcm_model *m = e->mem[q];
+        m->c->stream = e->side[e->lane[q]];
+        st = body(m);
+        m->c->stream = e->stream;
+    }
+    for (size_t k = 0; k < e->side.size(); ++k) {
+        DCM_HIP(hipEventRecord(e->join[k], e->side[k]));
+        DCM_HIP(hipStreamWaitEvent(e->stream, e->join[k], 0));
+    }
+    return st;
+}
 
 static int ensemble_solve(dcm_ensemble *e, std::vector<SolverOut> &outs) {
     const int n = (int)e->mem.size();
@@ -879,6 +979,37 @@ int dcm_ensemble_create(int n, dcm_model *const *models, dcm_ensemble **out) {
         c->borrowed_stream = true;
         e->mem.push_back(models[q]);
     }
+    // side streams for the stages around the solve; members go to the
+    // least-loaded stream, largest first
+    const int nside = n < kEnsembleLanes ? n : kEnsembleLanes;
+    bool ok = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < nside && ok; ++k) {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+        if (s) e->side.push_back(s);
+        if (ev) e->join.push_back(ev);
+    }
+    if (!ok) {
+        (void)dcm_ensemble_destroy(e);
+        set_error("dcm_ensemble_create: stream or event creation failed");
+        return DCM_HIP_ERROR;
+    }
+    std::vector<int> order(n);
+    for (int q = 0; q < n; ++q) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        return models[a]->c->geo.ntot > models[b]->c->geo.ntot;
+    });
+    std::vector<long long> load(nside, 0);
+    e->lane.assign(n, 0);
+    for (int q : order) {
+        int k = 0;
+        for (int u = 1; u < nside; ++u)
+            if (load[u] < load[k]) k = u;
+        e->lane[q] = k;
+        load[k] += models[q]->c->geo.ntot;
+    }
     *out = e;
     return DCM_OK;
 }
@@ -886,11 +1017,16 @@ int dcm_ensemble_create(int n, dcm_model *const *models, dcm_ensemble **out) {
 int dcm_ensemble_destroy(dcm_ensemble *e) {
     if (!e) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(e->stream);
+    for (hipStream_t s : e->side) (void)hipStreamSynchronize(s);
+    drop_graphs(e);
     for (size_t q = 0; q < e->mem.size(); ++q) {
         dcm_ctx *c = e->mem[q]->c;
         c->stream = e->saved[q];
         c->borrowed_stream = false;
     }
+    for (hipStream_t s : e->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return DCM_OK;
@@ -901,13 +1037,41 @@ int dcm_ensemble_autotune(dcm_ensemble *e, int *march_planes) {
     return ensemble_autotune(e, march_planes);
 }
 
+int dcm_ensemble_set_graphs(dcm_ensemble *e, int enable) {
+    if (!e) return DCM_BAD_ARG;
+    DCM_HIP(hipStreamSynchronize(e->stream));
+    e->use_graphs = enable != 0;
+    if (!e->use_graphs) drop_graphs(e);
+    return DCM_OK;
+}
+
 int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
     if (!e || nsteps < 0) return DCM_BAD_ARG;
     const int n = (int)e->mem.size();
     if (stats) std::memset(stats, 0, sizeof(dcm_step_stats) * n);
     std::vector<SolverOut> outs;
+    std::vector<int> key(n);
+    auto pre = [&]() -> int {
+        return fan_out(e, [](dcm_model *m) { return model_pre_solve(m); });
+    };
+    auto post = [&]() -> int {
+        return fan_out(e, [](dcm_model *m) { return model_post_solve(m, nullptr); });
+    };
+    auto advance_levels = [&]() {
+        for (int q = 0; q < n; ++q) e->mem[q]->n += 1;  // model_post_solve's host part
+    };
     for (int s = 0; s < nsteps; ++s) {
-        for (int q = 0; q < n; ++q) DCM_CHECK(model_pre_solve(e->mem[q]));
+        bool graphs = e->use_graphs && e->warm;
+        for (int q = 0; q < n; ++q) {
+            // what the captured kernels and their arguments depend on
+            const dcm_model *m = e->mem[q];
+            key[q] = m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0);
+            graphs = graphs && !m->c->profiling;
+        }
+        if (graphs)
+            DCM_CHECK(run_phase(e, e->pre, key, pre, [] {}));
+        else
+            DCM_CHECK(pre());
         DCM_CHECK(ensemble_solve(e, outs));
         for (int q = 0; q < n; ++q) {
             dcm_model *m = e->mem[q];
@@ -917,8 +1081,13 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
             else if (m->check_residual)
                 DCM_CHECK(dcm_true_residual(m->c, m->p, m->psrc, &true_resid));
             add_stats(stats ? stats + q : nullptr, outs[q], true_resid);
-            DCM_CHECK(model_post_solve(m, stats ? stats + q : nullptr));
+            if (stats) stats[q].steps += 1;
         }
+        if (graphs)
+            DCM_CHECK(run_phase(e, e->post, key, post, advance_levels));
+        else
+            DCM_CHECK(post());
+        e->warm = true;
     }
     DCM_HIP(hipStreamSynchronize(e->stream));
     return DCM_OK;

@@ -98,6 +98,7 @@ _SIGS = {
     "dcm_ensemble_destroy": ([_vp], _i),
     "dcm_ensemble_autotune": ([_vp, _ip], _i),
     "dcm_ensemble_step": ([_vp, _i, C.POINTER(StepStats)], _i),
+    "dcm_ensemble_set_graphs": ([_vp, _i], _i),
 }
 
 _lib = None
@@ -393,6 +394,11 @@ class Ensemble:
         planes = _i()
         _call("dcm_ensemble_autotune", self.h, C.byref(planes))
         return planes.value
+
+    def set_graphs(self, on=True):
+        """Replay the stages around the solve from captured HIP graphs (on by
+        default) or launch them one by one."""
+        _call("dcm_ensemble_set_graphs", self.h, 1 if on else 0)
 
     def step(self, n=1):
         """Advance every member n steps; one StepStats per member."""

@@ -257,8 +257,15 @@ int dcm_ensemble_destroy(dcm_ensemble *e);
  * tiles (4, 8, 16) on the members' current systems and keep the fastest;
  * the model state is not changed.  *march_planes (may be NULL) gets it. */
 int dcm_ensemble_autotune(dcm_ensemble *e, int *march_planes);
-/* Advance every member nsteps steps; stats (may be NULL) is an array of n. */
+/* Advance every member nsteps steps; stats (may be NULL) is an array of n.
+ * From the second step on, the stages before and after the pressure solve
+ * are replayed from HIP graphs captured on first use (one per phase of the
+ * three-level time rotation), unless graphs are switched off or a member
+ * is being profiled. */
 int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats);
+/* Graph capture of the step's stages on (default) or off; off releases the
+ * captured graphs. */
+int dcm_ensemble_set_graphs(dcm_ensemble *e, int enable);
 
 /* ---------------------------------------------------------------------- */
 /* Measurement utility (no counterpart in the reference): a plain device   */

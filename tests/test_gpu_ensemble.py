@@ -1,8 +1,9 @@
 """Ensembles: independent members of uneven size stepped together, their
-pressure solves batched into one BiCGSTAB loop.  A member must evolve
-exactly as it does on its own (the batched kernels run each member's
-arithmetic, partial sums included, unchanged), and with the reference's
-sequential sums exactly as the oracle."""
+pressure solves batched into one BiCGSTAB loop and the stages around the
+solve replayed from captured HIP graphs.  A member must evolve exactly as it
+does on its own (the batched kernels run each member's arithmetic, partial
+sums included, unchanged), and with the reference's sequential sums exactly
+as the oracle."""
 import numpy as np
 import pytest
 
@@ -16,12 +17,12 @@ pytestmark = pytest.mark.gpu
 FIELDS = ((H.F_U, "u"), (H.F_V, "v"), (H.F_W, "w"), (H.F_TH, "th"), (H.F_P, "p"))
 
 
-def _members():
+def _members(steps=7):
     """Uneven members: different shapes, depths, bubbles and tracer counts."""
-    return [W.warm_bubble(16, 12, 10, steps=3, amp=2.0, seed=11, name="a"),
-            W.moist_bubble(12, 16, 8, delta=50.0, dt=0.5, steps=3, ntracer=1, seed=12),
-            W.warm_bubble(20, 8, 40, steps=3, amp=1.0, radius=300.0, seed=13, name="c"),
-            W.warm_bubble(8, 8, 16, steps=3, amp=3.0, seed=14, name="d")]
+    return [W.warm_bubble(16, 12, 10, steps=steps, amp=2.0, seed=11, name="a"),
+            W.moist_bubble(12, 16, 8, delta=50.0, dt=0.5, steps=steps, ntracer=1, seed=12),
+            W.warm_bubble(20, 8, 40, steps=steps, amp=1.0, radius=300.0, seed=13, name="c"),
+            W.warm_bubble(8, 8, 16, steps=steps, amp=3.0, seed=14, name="d")]
 
 
 def _open(cases, precision, seq=False):
@@ -51,8 +52,10 @@ def _snapshot(case, m):
 
 @pytest.mark.parametrize("precision", [H.FP64, H.MIXED])
 def test_ensemble_members_evolve_as_alone(precision):
+    """Seven steps: the first eager, the next three capture the graphs of the
+    three time-level phases, the last three replay them."""
     cases = _members()
-    steps = 3
+    steps = 7
     # alone
     alone = []
     for case in cases:
@@ -77,15 +80,39 @@ def test_ensemble_members_evolve_as_alone(precision):
         _close(ctxs, models)
 
 
-def test_ensemble_with_sequential_sums_matches_oracle():
+def test_graph_replays_match_eager_steps():
+    """Two copies of an ensemble, one replaying captured graphs and one
+    launching every kernel, compared after every step (eager solves and host
+    synchronisations run between the replays)."""
     cases = _members()
+    ctx_g, mod_g = _open(cases, H.MIXED)
+    ctx_e, mod_e = _open(cases, H.MIXED)
+    try:
+        with H.Ensemble(mod_g) as eg, H.Ensemble(mod_e) as ee:
+            ee.set_graphs(False)
+            for s in range(7):
+                sg = eg.step(1)
+                se = ee.step(1)
+                for q, case in enumerate(cases):
+                    assert sg[q].iters_total == se[q].iters_total, (s, q)
+                    a, b = _snapshot(case, mod_g[q]), _snapshot(case, mod_e[q])
+                    for name in a:
+                        np.testing.assert_array_equal(a[name], b[name],
+                                                      err_msg=f"step {s} member {q} {name}")
+    finally:
+        _close(ctx_g, mod_g)
+        _close(ctx_e, mod_e)
+
+
+def test_ensemble_with_sequential_sums_matches_oracle():
+    cases = _members(4)
     ctxs, models = _open(cases, H.FP64, seq=True)
     oracles = [O.Model(case) for case in cases]
     try:
         with H.Ensemble(models) as e:
             planes = e.autotune()
             assert planes in (4, 8, 16)
-            for s in range(2):
+            for s in range(4):
                 stats = e.step(1)
                 for q, om in enumerate(oracles):
                     om.step(1)
@@ -102,7 +129,7 @@ def test_ensemble_with_sequential_sums_matches_oracle():
 
 
 def test_autotune_leaves_the_state_unchanged():
-    cases = _members()[:2]
+    cases = _members(1)[:2]
     ctxs, models = _open(cases, H.FP64)
     try:
         before = [_snapshot(c, m) for c, m in zip(cases, models)]
@@ -117,14 +144,15 @@ def test_autotune_leaves_the_state_unchanged():
 
 
 def test_uneven_64_member_ensemble_converges():
-    """The 64-member uneven ensemble (nx, ny in 32..128, nz in 32..96): one
-    mixed-precision step, every member's FP64 true residual below 1e-10."""
-    cases = W.ensemble_members(64, seed=404, steps=1)
+    """The 64-member uneven ensemble (nx, ny in 32..128, nz in 32..96): three
+    mixed-precision steps (the last two through captured graphs), every
+    member's FP64 true residual below 1e-10."""
+    cases = W.ensemble_members(64, seed=404, steps=3)
     print(f"ensemble of {len(cases)}: {sum(c.dom.cells for c in cases)} cells", flush=True)
     ctxs, models = _open(cases, H.MIXED)
     try:
         with H.Ensemble(models) as e:
-            stats = e.step(1)
+            stats = e.step(3)
         assert all(s.max_true_resid < 1e-10 for s in stats), [s.max_true_resid for s in stats]
     finally:
         _close(ctxs, models)

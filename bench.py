@@ -95,11 +95,17 @@ def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rto
     return res, snap, (ctx, model)
 
 
-def run_ensemble(H, W, warmup, steps, members=64, seed=404):
+def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
     """The uneven warm-bubble ensemble two ways: every member's pressure
     solve batched into one mixed-precision BiCGSTAB per step (after the
-    autotuner picks the solver tile), and the members one after another on
-    the plain FP64 path.  Cell-steps/s over the whole ensemble."""
+    autotuner picks the solver tile; the stages around the solve replayed
+    from HIP graphs), and the members one after another on the plain FP64
+    path.  Cell-steps/s over the whole ensemble.
+
+    With ``profile`` the batched Jacobi sweeps are timed with HIP events
+    (which switches the graphs off) for the roofline: algorithmic bytes are
+    12 B x 3 sweeps per inner iteration x the cells of each member that
+    iterated, summed over members, against the summed launch durations."""
     cases = W.ensemble_members(members, seed=seed, steps=warmup + steps)
     cells = sum(c.dom.cells for c in cases)
     out = {"members": len(cases), "cells": cells,
@@ -113,16 +119,27 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404):
             tune_s = time.perf_counter() - t0
             if warmup:
                 e.step(warmup)
+            if profile:
+                models[0].profile(True, only=DOMINANT)
             t0 = time.perf_counter()
             stats = e.step(steps)
             wall = time.perf_counter() - t0
+            kernel = None
+            if profile:
+                avg_ms, launches = models[0].kernel_time(DOMINANT)
+                models[0].profile(False)
+                swept = sum(c.dom.cells * s.iters_total for c, s in zip(cases, stats))
+                kernel = {"name": "k_jac_b<float,false>", "avg_ms": avg_ms, "launches": launches,
+                          "algorithmic_bytes": JAC_WORDS * 4 * JAC_PER_ITER * swept}
         out["batched_mixed"] = {
             "value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
             "autotuned_march_planes": planes, "autotune_s": tune_s,
+            "graphs": not profile,
             "iters_total": sum(s.iters_total for s in stats),
             "iters_max_member": max(s.iters_max for s in stats),
             "fallbacks": sum(s.fallbacks for s in stats),
-            "max_true_resid": max(s.max_true_resid for s in stats)}
+            "max_true_resid": max(s.max_true_resid for s in stats),
+            "kernel": kernel}
     finally:
         for m in models:
             m.close()
@@ -150,6 +167,37 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404):
     out["speedup_batched_mixed_vs_sequential_fp64"] = (out["batched_mixed"]["value"] /
                                                        out["sequential_fp64"]["value"])
     return out
+
+
+def ensemble_line(H, W, args):
+    """One JSON line for the 64-member ensemble alone: value = the batched
+    mixed path's cell-steps/s (graphs on), the roofline from a second, event
+    -timed run of the same steps."""
+    print(f"[bench] ensemble: 64 uneven members, warmup {args.warmup}, timed {args.steps} steps",
+          file=sys.stderr, flush=True)
+    ens = run_ensemble(H, W, args.warmup, args.steps)
+    prof = run_ensemble(H, W, args.warmup, args.steps, profile=True)["batched_mixed"]["kernel"]
+    b = ens["batched_mixed"]
+    print(f"[bench] ensemble: batched mixed {b['value']:.4g}, sequential FP64 "
+          f"{ens['sequential_fp64']['value']:.4g} cell-steps/s", file=sys.stderr, flush=True)
+    achieved = None
+    if prof and prof["avg_ms"]:
+        achieved = prof["algorithmic_bytes"] / (prof["avg_ms"] * prof["launches"] * 1e-3) / 1e9
+    line = {
+        "metric": "cell-steps/s, 64-member ensemble: batched mixed vs sequential FP64",
+        "value": b["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "ensemble64_uneven_warm_bubbles", "members": ens["members"],
+                   "cells": ens["cells"], "delta_m": 50.0, "dt_s": 1.0, "fields": 1, "kord": 2,
+                   "monotone": True, "ptol": 1e-10, "seed": 404, "parallelism": "single GPU"},
+        "ensemble": ens,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS if achieved else None, "traffic": None,
+                     "kernel": "k_jac_b<float,false>", "timing": prof},
+    }
+    print(json.dumps(line))
+    return 0
 
 
 def rel_l2(a, b, center=False):
@@ -215,6 +263,9 @@ def main(argv=None):
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10,
                     help="timed steps of the 64-member ensemble comparison (0: skip)")
+    ap.add_argument("--workload", choices=("moist_bubble", "ensemble"), default="moist_bubble",
+                    help="ensemble: only the 64-member uneven ensemble, batched mixed vs "
+                         "sequential FP64, --steps timed steps")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
@@ -223,6 +274,9 @@ def main(argv=None):
 
     from dcm_amd import hip as H
     from dcm_amd import workloads as W
+
+    if args.workload == "ensemble":
+        return ensemble_line(H, W, args)
 
     ip, jp, kp = (int(x) for x in args.grid.split("x"))
     case = W.moist_bubble(ip, jp, kp)

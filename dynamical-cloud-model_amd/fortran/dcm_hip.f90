@@ -235,6 +235,37 @@ module dcm_hip
             integer(c_int), value :: nsteps
             type(dcm_step_stats), intent(out) :: stats
         end function
+        ! ensembles of independent models (each with its own context)
+        integer(c_int) function dcm_ensemble_create(n, models, ens) &
+                bind(c, name="dcm_ensemble_create")
+            import :: c_int, c_ptr
+            integer(c_int), value :: n
+            type(c_ptr), intent(in) :: models(*)
+            type(c_ptr), intent(out) :: ens
+        end function
+        integer(c_int) function dcm_ensemble_destroy(ens) bind(c, name="dcm_ensemble_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ens
+        end function
+        integer(c_int) function dcm_ensemble_autotune(ens, march_planes) &
+                bind(c, name="dcm_ensemble_autotune")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ens
+            integer(c_int), intent(out) :: march_planes
+        end function
+        integer(c_int) function dcm_ensemble_step(ens, nsteps, stats) &
+                bind(c, name="dcm_ensemble_step")
+            import :: c_int, c_ptr, dcm_step_stats
+            type(c_ptr), value :: ens
+            integer(c_int), value :: nsteps
+            type(dcm_step_stats), intent(out) :: stats(*)
+        end function
+        integer(c_int) function dcm_ensemble_set_graphs(ens, enable) &
+                bind(c, name="dcm_ensemble_set_graphs")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ens
+            integer(c_int), value :: enable
+        end function
     end interface
 
     ! one device context per grid shape, created on first use

@@ -33,3 +33,13 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.skipif(not os.path.exists(H.LIB_PATH), reason="libdcm_hip.so not built")
 def test_version_string():
     assert H.load().dcm_version().decode().startswith("dcm_hip")
+
+
+def test_fortran_bindings_name_declared_entry_points():
+    """Every bind(c, name=...) of the Fortran binding module names a function
+    that the header declares (a typo would only show at link time of a
+    model build)."""
+    src = open(os.path.join(ROOT, "dynamical-cloud-model_amd", "fortran", "dcm_hip.f90")).read()
+    bound = set(re.findall(r'bind\(c,\s*name="(dcm_[a-z0-9_]+)"\)', src, flags=re.I))
+    assert bound, "no bindings found"
+    assert not bound - set(declared()), sorted(bound - set(declared()))

@@ -529,6 +529,12 @@ __device__ __forceinline__ R tval(gp<const R> __restrict__ r, gp<const R> __rest
     return r[q] - a * v1[q];
 }
 
+// the same from loaded values
+template <typename R, int MODE> __device__ __forceinline__ R tv(R r, R v1, R a, R b) {
+    if (MODE == 0) return r - a * (b * v1);
+    return r - a * v1;
+}
+
 // First sweep, variant 0: fused in registers, s0 = t/a_p evaluated at the
 // centre and the six neighbours (dynamics.f90:254-266).
 template <typename R, int MODE>
@@ -548,23 +554,48 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
     const int js = j == 1 ? m.jp : j - 1, jn = j == m.jp ? 1 : j + 1;
     const int kd = k == 1 ? 1 : k - 1, ku = k == m.kp ? m.kp : k + 1;
     const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+    // On the floor (lid) row the k-1 (k+1) neighbour is the cell itself
+    // (q.dd = 0, kd = k): its s0 is then s0c by the same operations, so the
+    // neighbour is loaded through cc + q.dd without a branch.
     const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
-    int cw = (iw - 1) * m.si + q.base;
     int cc = (T.i0 - 1) * m.si + q.base;
-    R s0w = div_ap<R>(tval<R, MODE>(r, v1, cw, a, bo), coef_ap(C, k, j, iw));
-    R tc = tval<R, MODE>(r, v1, cc, a, bo);
+    int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
+    int ce = (ie - 1) * m.si + q.base;
+    const int cw = (iw - 1) * m.si + q.base;
+    R s0w = div_ap<R>(tv<R, MODE>(r[cw], v1[cw], a, bo), coef_ap(C, k, j, iw));
+    R tc = tv<R, MODE>(r[cc], v1[cc], a, bo);
     R s0c = div_ap<R>(tc, coef_ap(C, k, j, T.i0));
+    // r, v1 at the j and k neighbours of plane i, and at the centre of i+1
+    R rs = r[cc + q.ds], vs = v1[cc + q.ds], rn = r[cc + q.dn], vn = v1[cc + q.dn];
+    R rd = r[cc + q.dd], vd = v1[cc + q.dd], ru = r[cc + q.du], vu = v1[cc + q.du];
+    R re = r[ce], ve = v1[ce];
     for (int i = T.i0; i <= T.i1; ++i) {
-        const int ie = i == m.ip ? 1 : i + 1;
-        const int ce = (ie - 1) * m.si + q.base;
-        const R te = tval<R, MODE>(r, v1, ce, a, bo);
+        // loads for plane i+1 in flight while plane i is computed
+        R nrs = R(0), nvs = R(0), nrn = R(0), nvn = R(0), nrd = R(0), nvd = R(0), nru = R(0),
+          nvu = R(0), nre = R(0), nve = R(0);
+        int nie = ie, nce = ce;
+        if (i < T.i1) {
+            nie = ie == m.ip ? 1 : ie + 1;
+            nce = (nie - 1) * m.si + q.base;
+            nrs = r[ce + q.ds];
+            nvs = v1[ce + q.ds];
+            nrn = r[ce + q.dn];
+            nvn = v1[ce + q.dn];
+            nrd = r[ce + q.dd];
+            nvd = v1[ce + q.dd];
+            nru = r[ce + q.du];
+            nvu = v1[ce + q.du];
+            nre = r[nce];
+            nve = v1[nce];
+        }
+        const R te = tv<R, MODE>(re, ve, a, bo);
         const R s0e = div_ap<R>(te, coef_ap(C, k, j, ie));
         const R api = (-C.ae[i]) - C.aw[i];
         const R ap = coef_ap_i(C, api, k, j);
-        const R s0s = div_ap<R>(tval<R, MODE>(r, v1, cc + q.ds, a, bo), coef_ap_i(C, api, k, js));
-        const R s0n = div_ap<R>(tval<R, MODE>(r, v1, cc + q.dn, a, bo), coef_ap_i(C, api, k, jn));
-        const R s0d = q.dd ? div_ap<R>(tval<R, MODE>(r, v1, cc - 1, a, bo), coef_ap_i(C, api, kd, j)) : s0c;
-        const R s0u = q.du ? div_ap<R>(tval<R, MODE>(r, v1, cc + 1, a, bo), coef_ap_i(C, api, ku, j)) : s0c;
+        const R s0s = div_ap<R>(tv<R, MODE>(rs, vs, a, bo), coef_ap_i(C, api, k, js));
+        const R s0n = div_ap<R>(tv<R, MODE>(rn, vn, a, bo), coef_ap_i(C, api, k, jn));
+        const R s0d = div_ap<R>(tv<R, MODE>(rd, vd, a, bo), coef_ap_i(C, api, kd, j));
+        const R s0u = div_ap<R>(tv<R, MODE>(ru, vu, a, bo), coef_ap_i(C, api, ku, j));
         const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n, s0d,
                                 s0u, s0c);
         t_out[cc] = tc;
@@ -572,7 +603,19 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
         s0w = s0c;
         s0c = s0e;
         tc = te;
+        rs = nrs;
+        vs = nvs;
+        rn = nrn;
+        vn = nvn;
+        rd = nrd;
+        vd = nvd;
+        ru = nru;
+        vu = nvu;
+        re = nre;
+        ve = nve;
         cc = ce;
+        ce = nce;
+        ie = nie;
     }
 }
 

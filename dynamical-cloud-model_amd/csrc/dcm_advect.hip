@@ -23,6 +23,7 @@
 // psi1, U*, beta and the velocity shadows are FP32 (half the bytes), while
 // psi, the shift m and the output stay FP64.
 #include <cfloat>
+#include <vector>
 
 #include "dcm_device.h"
 
@@ -627,6 +628,267 @@ int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *
     return DCM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Vertical-only advection, FP64: first_order_upstream_vert_3d
+// (pamm/sfvt/advection_3d.F90:737-791), mpdata_vert_3d (:820-1126) and
+// mpdata_vec_vert_3d (:1779-2087), the sedimentation transport of PAMM.
+// As in mpdata_core, the input field stays untouched until the last kernel:
+// the first pass reads psi - m, later passes ping-pong between scratch fields
+// whose z planes hold bc - m, and the result is written back as value + m.
+// ---------------------------------------------------------------------------
+// one donor pass along z (:754-789) from src (minus *shift when given) into
+// dst; neumann 2: the floor face of level 1 takes w(1) (:762-766); neumann
+// 1: zero-gradient fluxes at floor and lid (:780-787).  With do_sum, block
+// partial sums of the shifted source (the zero-sum test, :868-870).
+__global__ __launch_bounds__(256) void k_vupwind(const double *src, const double *shift,
+                                                 const double *w, double *dst, Geo g,
+                                                 Metrics mt, double dt, int neumann,
+                                                 const int *flag, int do_sum,
+                                                 double *partials) {
+    if (flag && *flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    const bool act = k <= g.kp && j <= g.jp;
+    const double m = shift ? *shift : 0.0;
+    double s[1] = {0.0};
+    if (act) {
+        const long long c = pidx(g, k, j, i);
+        auto P = [&](long long q) { return shift ? src[q] - m : src[q]; };
+        const double pc = P(c), pu = P(c + 1), pd = P(c - 1);
+        const double wk = w[c];
+        const double wm = (k == 1 && neumann == 2) ? wk : w[c - 1];
+        const double den = (2.0 * mt.dzn[k]) * mt.rhoan[k];
+        double fr = ((((wk + fabs(wk)) * mt.rhoa[k]) * pc + ((wk - fabs(wk)) * mt.rhoa[k]) * pu) *
+                     dt) / den;
+        double fl = ((((wm + fabs(wm)) * mt.rhoa[k - 1]) * pd +
+                      ((wm - fabs(wm)) * mt.rhoa[k - 1]) * pc) * dt) / den;
+        if (neumann == 1) {
+            if (k == 1) fl = fr;
+            if (k == g.kp) fr = fl;
+        }
+        dst[c] = pc - (fr - fl);
+        s[0] = pc;
+    }
+    if (do_sum) block_sum_store<1>(s, partials, 0);
+}
+
+// antidiffusive vertical velocity (:942-958; vector variant :1905-1921)
+__global__ __launch_bounds__(256) void k_vantidiff(const double *wt, const double *po,
+                                                   double *ws, Geo g, Metrics mt, double dt,
+                                                   const int *flag) {
+    if (*flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double *dz = mt.dz, *ra = mt.rhoa, *rn = mt.rhoan;
+    const double gb3 = 0.5 * (rn[k] + rn[k + 1]);
+    const double u_div3 = 0.0, u_j_bar3 = 0.0;
+    const double wc = wt[c], p0 = po[c], p1 = po[c + 1];
+    double v = ((((fabs(wc) * dz[k] - ((dt * (ra[k] / gb3)) * wc) * wc) * (p1 - p0)) /
+                 ((p1 + p0) + Small<double>::v)) / mt.dzn[k]) - u_j_bar3;
+    v = v - ((0.25 * dt) * wc) *
+                (((ra[k + 1] * wt[c + 1] - ra[k - 1] * wt[c - 1]) / (dz[k - 1] * gb3)) + u_div3);
+    ws[c] = v;
+}
+
+// limiter bounds (:1000-1046; vector variant :1964-2010): psi_store is the
+// input field minus its shift, psi_old the current pass
+__global__ __launch_bounds__(256) void k_vbeta(const double *psi, const double *shift,
+                                               const double *po, const double *wt, double *bup,
+                                               double *bdn, Geo g, Metrics mt, double dt,
+                                               const int *flag) {
+    if (*flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double m = *shift;
+    const double vals[6] = {psi[c - 1] - m, psi[c] - m, psi[c + 1] - m,
+                            po[c - 1],      po[c],      po[c + 1]};
+    double mx = vals[0], mn = vals[0];
+#pragma unroll
+    for (int q = 1; q < 6; ++q) {
+        if (vals[q] > mx) mx = vals[q];
+        if (vals[q] < mn) mn = vals[q];
+    }
+    const double *ra = mt.rhoa, *rn = mt.rhoan, *dzn = mt.dzn;
+    const double pc = po[c], wm = wt[c - 1], wc = wt[c];
+    const double d1 = (dt * ((pos0(wm) * ra[k - 1]) * po[c - 1] - (neg0(wc) * ra[k]) * po[c + 1])) /
+                          dzn[k] + Small<double>::v;
+    const double d2 = (dt * ((pos0(wc) * ra[k]) * pc - (neg0(wm) * ra[k - 1]) * pc)) / dzn[k] +
+                      Small<double>::v;
+    bup[c] = (rn[k] * (mx - pc)) / d1;
+    bdn[c] = (rn[k] * (pc - mn)) / d2;
+}
+
+// limited vertical velocity, eq 18 (:1071-1076; vector variant :2028-2033)
+__global__ __launch_bounds__(256) void k_vlimit(const double *wt, const double *bup,
+                                                const double *bdn, double *ws, Geo g,
+                                                const int *flag) {
+    if (*flag) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double wc = wt[c];
+    ws[c] = min3(1.0, bdn[c], bup[c + 1]) * pos0(wc) + min3(1.0, bup[c], bdn[c + 1]) * neg0(wc);
+}
+
+// result of a vertical MPDATA call for one field: the interior from the last
+// pass plus m, the z planes (bc - m) + m (exchange_along_z after every
+// pass), every other cell (psi - m) + m; after the zero-sum exit
+// (psi - m) + m everywhere (:870-875, :1123-1125)
+__global__ void k_vfinish(double *psi, const double *res, const double *shift, const int *flag,
+                          Geo g, double lbc, double ubc, int passes) {
+    const double m = *shift;
+    const bool exit = *flag != 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.ntot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nk);
+        const long long r = t / g.nk;
+        const int j = (int)(r % g.nj), i = (int)(r / g.nj);
+        double v;
+        if (exit || passes == 0) {
+            v = (psi[t] - m) + m;
+        } else if (k == 0 || k == g.kp + 1) {
+            v = ((k == 0 ? lbc : ubc) - m) + m;
+        } else if (j >= 1 && j <= g.jp && i >= 1 && i <= g.ip) {
+            v = res[t] + m;
+        } else {
+            v = (psi[t] - m) + m;
+        }
+        psi[t] = v;
+    }
+}
+
+// psi(k,:,:) += sign * prof(k) over the whole padded array (mpdata_3d_add,
+// :172-174 and :183-185)
+__global__ void k_add_profile(double *psi, const double *prof, double sign, Geo g) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < g.ntot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nk);
+        psi[t] = sign > 0.0 ? psi[t] + prof[k] : psi[t] - prof[k];
+    }
+}
+
+// Shared driver of mpdata_vert_3d and mpdata_vec_vert_3d: nf fields of psi
+// (field 1 drives the velocities and the limiter), lbc/ubc of each field
+// (host values before the shift).  The shifts and the exit flag are left in
+// c->dscal like mpdata_core's.
+int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf,
+                     const double *lbc, const double *ubc, int kord, int monotone, int neumann,
+                     int interior_min) {
+    const Geo &g = c->geo;
+    const Interior L = interior_launch(g);
+    const Metrics mt = c->met;
+    double *dm = c->dscal + 16;
+    int *flag = (int *)(c->dscal + 8);
+    if (nf < 1 || nf > 32) {
+        set_error("vertical mpdata: a category must hold 1..32 fields");
+        return DCM_BAD_ARG;
+    }
+    if (kord < 0) {
+        set_error("vertical mpdata: kord must be >= 0");
+        return DCM_BAD_ARG;
+    }
+    const unsigned nbmin = 1024;
+    const unsigned nbint = L.grid.x * L.grid.y * L.grid.z;
+    DCM_CHECK(ensure_partials(c, (size_t)nbint + nbmin + 16));
+    // shifts: minimum of the whole array (scalar form, :861) or of the
+    // interior (vector form, :1822), with lbc and ubc
+    for (int n = 0; n < nf; ++n) {
+        hipLaunchKernelGGL(k_min, dim3(nbmin), dim3(kBlock), 0, c->stream,
+                           psi + (size_t)n * g.ntot, g, interior_min, c->partials);
+        hipLaunchKernelGGL(k_min_finish, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                           (int)nbmin, lbc[n], ubc[n], dm + n, flag);
+        DCM_LAUNCH_CHECK();
+    }
+    // velocity and limiter scratch, two ping-pong fields per advected field
+    double *W1, *W2, *bup, *bdn;
+    DCM_CHECK(work_field64(c, 0, &W1));
+    DCM_CHECK(work_field64(c, 1, &W2));
+    DCM_CHECK(work_field64(c, 6, &bup));
+    DCM_CHECK(work_field64(c, 7, &bdn));
+    std::vector<double *> cur(nf), nxt(nf);
+    for (int n = 0; n < nf; ++n) {
+        DCM_CHECK(work_field64(c, kSlotPsi + n, &cur[n]));
+        DCM_CHECK(work_field64(c, kSlotPsi + nf + n, &nxt[n]));
+    }
+    const double *wt = w;
+    double *wt_sav = W1;
+    int it2 = 0;
+    for (int it = 1; it <= kord; ++it) {
+        if (it > 1) {
+            it2 += 1;
+            hipLaunchKernelGGL(k_vantidiff, L.grid, L.block, 0, c->stream, wt, cur[0], wt_sav, g,
+                               mt, dt, flag);
+            DCM_LAUNCH_CHECK();
+            if (it2 % 2 == 1) {
+                wt = W1;
+                wt_sav = W2;
+            } else {
+                wt = W2;
+                wt_sav = W1;
+            }
+            DCM_CHECK(fill<double>(c, (double *)wt, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
+        }
+        if (it > 1 && monotone) {
+            it2 += 1;
+            hipLaunchKernelGGL(k_vbeta, L.grid, L.block, 0, c->stream, psi, dm, cur[0], wt, bup,
+                               bdn, g, mt, dt, flag);
+            DCM_LAUNCH_CHECK();
+            DCM_CHECK(fill<double>(c, bup, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
+            DCM_CHECK(fill<double>(c, bdn, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
+            hipLaunchKernelGGL(k_vlimit, L.grid, L.block, 0, c->stream, wt, bup, bdn, wt_sav, g,
+                               flag);
+            DCM_LAUNCH_CHECK();
+            wt = W2;
+            wt_sav = W1;
+            DCM_CHECK(fill<double>(c, (double *)wt, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
+        }
+        for (int n = 0; n < nf; ++n) {
+            const bool first = it == 1;
+            // the first pass reads the input minus its shift and, for field
+            // 1, sums it for the zero-sum test
+            hipLaunchKernelGGL(k_vupwind, L.grid, L.block, 0, c->stream,
+                               first ? (const double *)(psi + (size_t)n * g.ntot) : cur[n],
+                               first ? (const double *)(dm + n) : nullptr, wt, nxt[n], g, mt, dt,
+                               neumann, first ? nullptr : flag, (first && n == 0) ? 1 : 0,
+                               c->partials);
+            DCM_LAUNCH_CHECK();
+            if (first && n == 0) {
+                hipLaunchKernelGGL(k_exit_flag, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                                   (int)nbint, flag);
+                DCM_LAUNCH_CHECK();
+            }
+            DCM_CHECK(fill<double>(c, nxt[n], FILL_ALONG_Z, lbc[n], ubc[n], dm + n, 1, flag));
+            std::swap(cur[n], nxt[n]);
+        }
+    }
+    if (kord < 1) {
+        // no pass: only the zero-sum test
+        hipLaunchKernelGGL(k_vupwind, L.grid, L.block, 0, c->stream, (const double *)psi,
+                           (const double *)dm, w, nxt[0], g, mt, dt, neumann, nullptr, 1,
+                           c->partials);
+        hipLaunchKernelGGL(k_exit_flag, dim3(1), dim3(1024), 0, c->stream, c->partials,
+                           (int)nbint, flag);
+        DCM_LAUNCH_CHECK();
+    }
+    for (int n = 0; n < nf; ++n) {
+        hipLaunchKernelGGL(k_vfinish, dim3(2048), dim3(kBlock), 0, c->stream,
+                           psi + (size_t)n * g.ntot, (const double *)cur[n],
+                           (const double *)(dm + n), (const int *)flag, g, lbc[n], ubc[n],
+                           kord);
+        DCM_LAUNCH_CHECK();
+    }
+    return DCM_OK;
+}
+
 }  // namespace dcm
 
 using namespace dcm;
@@ -681,6 +943,70 @@ int dcm_mpdata_vec3d(dcm_ctx *c, double dt, const double *u, const double *v, co
                      double *psi, int nq, double *lbc, double *ubc, int kord, int monotone,
                      int neumann, int *early_exit) {
     return mpdata_api(c, dt, u, v, w, psi, nq, lbc, ubc, kord, monotone, neumann, early_exit, 1);
+}
+
+int dcm_mpdata3d_add(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+                     double *psi, const double *psi_1d, double *lbc, double *ubc, int kord,
+                     int monotone, int neumann, int *early_exit) {
+    if (!c || !psi_1d) return DCM_BAD_ARG;
+    const Geo &g = c->geo;
+    const double *prof;
+    DCM_CHECK(upload_1d(c, psi_1d, (size_t)g.kp + 2, 1, &prof));
+    hipLaunchKernelGGL(k_add_profile, dim3(2048), dim3(kBlock), 0, c->stream, psi, prof, 1.0, g);
+    DCM_LAUNCH_CHECK();
+    DCM_CHECK(mpdata_api(c, dt, u, v, w, psi, 1, lbc, ubc, kord, monotone, neumann, early_exit,
+                         0));
+    hipLaunchKernelGGL(k_add_profile, dim3(2048), dim3(kBlock), 0, c->stream, psi, prof, -1.0,
+                       g);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+int dcm_upwind_vert3d(dcm_ctx *c, double dt, const double *w, double *psi, int neumann) {
+    if (!c || !w || !psi) return DCM_BAD_ARG;
+    const Geo &g = c->geo;
+    const Interior L = interior_launch(g);
+    double *tmp;
+    DCM_CHECK(work_field64(c, kSlotPsi, &tmp));
+    // fluxes from the old field into scratch, then the in-place update
+    // (advection_3d.F90:754-789)
+    hipLaunchKernelGGL(k_vupwind, L.grid, L.block, 0, c->stream, (const double *)psi,
+                       (const double *)nullptr, w, tmp, g, c->met, dt, neumann,
+                       (const int *)nullptr, 0, (double *)nullptr);
+    DCM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_interior, L.grid, L.block, 0, c->stream, tmp, psi, g);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+static int mpdata_vert_api(dcm_ctx *c, double dt, const double *w, double *psi, int nq,
+                           double *lbc, double *ubc, int kord, int monotone, int neumann,
+                           int *early_exit, int interior_min) {
+    if (!c || !w || !psi || !lbc || !ubc || nq < 1 || nq > 32) return DCM_BAD_ARG;
+    DCM_CHECK(mpdata_vert_core(c, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann,
+                               interior_min));
+    // lbc/ubc leave as (bc - m) + m, like the Fortran INTENT(INOUT) scalars
+    DCM_HIP(hipMemcpyAsync(c->hscal, c->dscal, 48 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < nq; ++n) {
+        const double m = c->hscal[16 + n];
+        lbc[n] = (lbc[n] - m) + m;
+        ubc[n] = (ubc[n] - m) + m;
+    }
+    if (early_exit) *early_exit = *(int *)(c->hscal + 8);
+    return DCM_OK;
+}
+
+int dcm_mpdata_vert3d(dcm_ctx *c, double dt, const double *w, double *psi, double *lbc,
+                      double *ubc, int kord, int monotone, int neumann, int *early_exit) {
+    return mpdata_vert_api(c, dt, w, psi, 1, lbc, ubc, kord, monotone, neumann, early_exit, 0);
+}
+
+int dcm_mpdata_vec_vert3d(dcm_ctx *c, double dt, const double *w, double *psi, int nq,
+                          double *lbc, double *ubc, int kord, int monotone, int neumann,
+                          int *early_exit) {
+    return mpdata_vert_api(c, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann, early_exit, 1);
 }
 
 int dcm_adv_ref_state(dcm_ctx *c, double dt, const double *w, double *psi,

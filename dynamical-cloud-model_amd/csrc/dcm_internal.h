@@ -266,6 +266,10 @@ template <typename R>
 int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *psi, int nf,
                 const double *lbc, const double *ubc, int kord, int monotone, int neumann,
                 int interior_min);
+// vertical-only MPDATA of nf fields (FP64); interior_min: the vector form
+int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf,
+                     const double *lbc, const double *ubc, int kord, int monotone, int neumann,
+                     int interior_min);
 int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
                     float *vf, float *wf);
 int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref_dev);

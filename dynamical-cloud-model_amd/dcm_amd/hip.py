@@ -67,6 +67,10 @@ _SIGS = {
     "dcm_mpdata3d": ([_vp, _d, _vp, _vp, _vp, _vp, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_mpdata_vec3d": ([_vp, _d, _vp, _vp, _vp, _vp, _i, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_adv_ref_state": ([_vp, _d, _vp, _vp, _dp], _i),
+    "dcm_mpdata3d_add": ([_vp, _d, _vp, _vp, _vp, _vp, _dp, _dp, _dp, _i, _i, _i, _ip], _i),
+    "dcm_upwind_vert3d": ([_vp, _d, _vp, _vp, _i], _i),
+    "dcm_mpdata_vert3d": ([_vp, _d, _vp, _vp, _dp, _dp, _i, _i, _i, _ip], _i),
+    "dcm_mpdata_vec_vert3d": ([_vp, _d, _vp, _vp, _i, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_hmeans": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _dp, _dp, _dp, _dp, _dp], _i),
     "dcm_sources": ([_vp, _vp, _vp, _vp, _vp, _dp, _dp, _vp, _vp, _vp, _vp, _vp], _i),
     "dcm_matvec": ([_vp, _vp, _vp], _i),
@@ -236,6 +240,32 @@ class Context:
     def adv_ref_state(self, dt, w, psi, ref):
         ref = np.ascontiguousarray(ref, dtype=np.float64)
         _call("dcm_adv_ref_state", self.h, dt, w.ptr, psi.ptr, _dptr(ref))
+
+    def mpdata3d_add(self, dt, u, v, w, psi, psi_1d, lbc=0.0, ubc=0.0, kord=2, monotone=True,
+                     neumann=0):
+        prof = np.ascontiguousarray(psi_1d, dtype=np.float64)
+        lb, ub, ex = _d(lbc), _d(ubc), _i(0)
+        _call("dcm_mpdata3d_add", self.h, dt, u.ptr, v.ptr, w.ptr, psi.ptr, _dptr(prof),
+              C.byref(lb), C.byref(ub), kord, int(monotone), neumann, C.byref(ex))
+        return ex.value, lb.value, ub.value
+
+    def upwind_vert3d(self, dt, w, psi, neumann=0):
+        _call("dcm_upwind_vert3d", self.h, dt, w.ptr, psi.ptr, neumann)
+
+    def mpdata_vert3d(self, dt, w, psi, lbc=0.0, ubc=0.0, kord=2, monotone=True, neumann=0):
+        lb, ub, ex = _d(lbc), _d(ubc), _i(0)
+        _call("dcm_mpdata_vert3d", self.h, dt, w.ptr, psi.ptr, C.byref(lb), C.byref(ub), kord,
+              int(monotone), neumann, C.byref(ex))
+        return ex.value, lb.value, ub.value
+
+    def mpdata_vec_vert3d(self, dt, w, psi, nq, lbc=None, ubc=None, kord=2, monotone=True,
+                          neumann=0):
+        lb = np.zeros(nq) if lbc is None else np.array(lbc, dtype=np.float64)
+        ub = np.zeros(nq) if ubc is None else np.array(ubc, dtype=np.float64)
+        ex = _i(0)
+        _call("dcm_mpdata_vec_vert3d", self.h, dt, w.ptr, psi.ptr, nq, _dptr(lb), _dptr(ub),
+              kord, int(monotone), neumann, C.byref(ex))
+        return ex.value, lb, ub
 
     # -- a6, a7 ---------------------------------------------------------------
     def hmeans(self, u, v, w, th, q=None, nq=0):

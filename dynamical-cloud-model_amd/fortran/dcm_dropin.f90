@@ -202,64 +202,112 @@ contains
         call dcm_take(ctx, 4, c_loc(psi_3d), nb)
     end subroutine adv_ref_state
 
-    !> mpdata_3d_add (pamm/sfvt/advection_3d.F90:139-217) is not on the GPU path
+    !> mpdata_3d_add (pamm/sfvt/advection_3d.F90:139-190): psi_1d added to
+    !> every column (halo levels included), mpdata_3d, psi_1d removed
     subroutine mpdata_3d_add(dt, dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, ip, jp, kp, l_h, r_h, &
-            u, v, w, psi_in, lbc, ubc, kord, monotone, neumann, comm3d, id, dims, coords)
+            u, v, w, psi_in, psi_1d, lbc, ubc, kord, monotone, neumann, comm3d, id, dims, coords)
         integer(dcm_i4b), intent(in) :: id, comm3d
         integer(dcm_i4b), dimension(3), intent(in) :: dims, coords
         real(dcm_wp), intent(in) :: dt
         integer(dcm_i4b), intent(in) :: ip, jp, kp, l_h, r_h, kord
-        real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -l_h+1:ip+r_h), intent(in) :: u
-        real(dcm_wp), dimension(-r_h+1:kp+r_h, -l_h+1:jp+r_h, -r_h+1:ip+r_h), intent(in) :: v
-        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), intent(in) :: w
+        real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -l_h+1:ip+r_h), &
+            intent(in), target :: u
+        real(dcm_wp), dimension(-r_h+1:kp+r_h, -l_h+1:jp+r_h, -r_h+1:ip+r_h), &
+            intent(in), target :: v
+        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
+            intent(in), target :: w
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
-            intent(inout) :: psi_in
+            intent(inout), target :: psi_in
+        real(dcm_wp), dimension(-r_h+1:kp+r_h), intent(in) :: psi_1d
         real(dcm_wp), intent(inout) :: lbc, ubc
         real(dcm_wp), dimension(-l_h+1:ip+r_h), intent(in) :: dx, dxn
         real(dcm_wp), dimension(-l_h+1:jp+r_h), intent(in) :: dy, dyn
         real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in) :: dz, dzn, rhoa, rhoan
         logical, intent(in) :: monotone
         integer(dcm_i4b), intent(in) :: neumann
-        print *, 'dcm: mpdata_3d_add (advection_scheme=2) is not on the GPU path'
-        stop
+        type(c_ptr) :: ctx, du, dv, dw, dpsi
+        integer(c_size_t) :: nb
+        integer(c_int) :: early
+
+        call need_halo1(l_h, r_h)
+        call one_rank(dims)
+        ctx = dcm_context(ip, jp, kp, dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan)
+        nb = field_bytes(ip, jp, kp)
+        du = dcm_put(ctx, 1, c_loc(u), nb)
+        dv = dcm_put(ctx, 2, c_loc(v), nb)
+        dw = dcm_put(ctx, 3, c_loc(w), nb)
+        dpsi = dcm_put(ctx, 4, c_loc(psi_in), nb)
+        call dcm_check(dcm_mpdata3d_add(ctx, dt, du, dv, dw, dpsi, psi_1d, lbc, ubc, &
+            int(kord, c_int), to_cint(monotone), int(neumann, c_int), early), 'mpdata_3d_add')
+        call dcm_take(ctx, 4, c_loc(psi_in), nb)
     end subroutine mpdata_3d_add
 
-    !> mpdata_vert_3d (pamm/sfvt/advection_3d.F90:820-1146): sedimentation
-    !> advection of the microphysics, not on the GPU path
+    !> mpdata_vert_3d (pamm/sfvt/advection_3d.F90:820-1126): vertical-only
+    !> MPDATA, the sedimentation transport of the microphysics
     subroutine mpdata_vert_3d(dt, dz, dzn, rhoa, rhoan, ip, jp, kp, l_h, r_h, w, psi_in, lbc, &
             ubc, kord, monotone, neumann, comm3d, id, dims, coords)
         integer(dcm_i4b), intent(in) :: id, comm3d
         integer(dcm_i4b), dimension(3), intent(in) :: dims, coords
         real(dcm_wp), intent(in) :: dt
         integer(dcm_i4b), intent(in) :: ip, jp, kp, l_h, r_h, kord
-        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), intent(in) :: w
+        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
+            intent(in), target :: w
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
-            intent(inout) :: psi_in
+            intent(inout), target :: psi_in
         real(dcm_wp), intent(inout) :: lbc, ubc
         real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in) :: dz, dzn, rhoa, rhoan
         logical, intent(in) :: monotone
         integer(dcm_i4b), intent(in) :: neumann
-        print *, 'dcm: mpdata_vert_3d (microphysics sedimentation) is not on the GPU path'
-        stop
+        type(c_ptr) :: ctx, dw, dpsi
+        integer(c_size_t) :: nb
+        integer(c_int) :: early
+
+        call need_halo1(l_h, r_h)
+        call one_rank(dims)
+        ! only the vertical metrics arrive here: a provisional context
+        ctx = dcm_context(ip, jp, kp, ones(ip), ones(jp), dz, ones(ip), ones(jp), dzn, rhoa, &
+            rhoan, provisional=.true.)
+        nb = field_bytes(ip, jp, kp)
+        dw = dcm_put(ctx, 3, c_loc(w), nb)
+        dpsi = dcm_put(ctx, 4, c_loc(psi_in), nb)
+        call dcm_check(dcm_mpdata_vert3d(ctx, dt, dw, dpsi, lbc, ubc, int(kord, c_int), &
+            to_cint(monotone), int(neumann, c_int), early), 'mpdata_vert_3d')
+        call dcm_take(ctx, 4, c_loc(psi_in), nb)
     end subroutine mpdata_vert_3d
 
-    !> mpdata_vec_vert_3d (pamm/sfvt/advection_3d.F90:1779-2130): not on the
-    !> GPU path
+    !> mpdata_vec_vert_3d (pamm/sfvt/advection_3d.F90:1779-2087): nq fields of
+    !> one category falling with field 1's corrected velocities (the call of
+    !> pamm/microphysics.f90:1522-1560)
     subroutine mpdata_vec_vert_3d(dt, dz, dzn, rhoa, rhoan, ip, jp, kp, nq, l_h, r_h, w, &
             psi_in, lbc, ubc, kord, monotone, neumann, comm3d, id, dims, coords)
         integer(dcm_i4b), intent(in) :: id, comm3d
         integer(dcm_i4b), dimension(3), intent(in) :: dims, coords
         real(dcm_wp), intent(in) :: dt
         integer(dcm_i4b), intent(in) :: ip, jp, kp, nq, l_h, r_h, kord
-        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), intent(in) :: w
+        real(dcm_wp), dimension(-l_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
+            intent(in), target :: w
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h, 1:nq), &
-            intent(inout) :: psi_in
+            intent(inout), target :: psi_in
         real(dcm_wp), intent(inout), dimension(nq) :: lbc, ubc
         real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in) :: dz, dzn, rhoa, rhoan
         logical, intent(in) :: monotone
         integer(dcm_i4b), intent(in) :: neumann
-        print *, 'dcm: mpdata_vec_vert_3d (microphysics sedimentation) is not on the GPU path'
-        stop
+        type(c_ptr) :: ctx, dw, dpsi
+        integer(c_size_t) :: nb
+        integer(c_int) :: early
+
+        call need_halo1(l_h, r_h)
+        call one_rank(dims)
+        if (nq < 1) return
+        ctx = dcm_context(ip, jp, kp, ones(ip), ones(jp), dz, ones(ip), ones(jp), dzn, rhoa, &
+            rhoan, provisional=.true.)
+        nb = field_bytes(ip, jp, kp)
+        dw = dcm_put(ctx, 3, c_loc(w), nb)
+        dpsi = dcm_put(ctx, 16, c_loc(psi_in), nb * nq)
+        call dcm_check(dcm_mpdata_vec_vert3d(ctx, dt, dw, dpsi, int(nq, c_int), lbc, ubc, &
+            int(kord, c_int), to_cint(monotone), int(neumann, c_int), early), &
+            'mpdata_vec_vert_3d')
+        call dcm_take(ctx, 16, c_loc(psi_in), nb * nq)
     end subroutine mpdata_vec_vert_3d
 
 end module advection_s_3d

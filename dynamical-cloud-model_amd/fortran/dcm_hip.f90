@@ -30,9 +30,13 @@ module dcm_hip
         dcm_mirror_upload, dcm_mirror_download, dcm_mirror_release_all, dcm_last_error, &
         dcm_fill_full, dcm_fill_along_dim, dcm_fill_along_dim_wo, dcm_fill_full_wo, &
         dcm_fill_along_z, dcm_upwind3d, dcm_mpdata3d, dcm_mpdata_vec3d, dcm_adv_ref_state, &
-        dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_bicgstab, dcm_advance_momentum, dcm_advance_scalars, &
+        dcm_mpdata3d_add, dcm_upwind_vert3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, &
+        dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_bicgstab, &
+        dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
-        dcm_model_get, dcm_model_init, dcm_model_step
+        dcm_model_get, dcm_model_init, dcm_model_step, &
+        dcm_ensemble_create, dcm_ensemble_destroy, dcm_ensemble_autotune, dcm_ensemble_step, &
+        dcm_ensemble_set_graphs
 
     interface
         integer(c_int) function dcm_context_create(ip, jp, kp, dx, dy, dz, dxn, dyn, dzn, &
@@ -135,6 +139,41 @@ module dcm_hip
                 monotone, neumann, early_exit) bind(c, name="dcm_mpdata_vec3d")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx, u, v, w, psi
+            real(c_double), value :: dt
+            integer(c_int), value :: nq, kord, monotone, neumann
+            real(c_double), intent(inout) :: lbc(*), ubc(*)
+            integer(c_int), intent(out) :: early_exit
+        end function
+        integer(c_int) function dcm_mpdata3d_add(ctx, dt, u, v, w, psi, psi_1d, lbc, ubc, kord, &
+                monotone, neumann, early_exit) bind(c, name="dcm_mpdata3d_add")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, u, v, w, psi
+            real(c_double), value :: dt
+            real(c_double), intent(in) :: psi_1d(*)
+            real(c_double), intent(inout) :: lbc, ubc
+            integer(c_int), value :: kord, monotone, neumann
+            integer(c_int), intent(out) :: early_exit
+        end function
+        integer(c_int) function dcm_upwind_vert3d(ctx, dt, w, psi, neumann) &
+                bind(c, name="dcm_upwind_vert3d")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, w, psi
+            real(c_double), value :: dt
+            integer(c_int), value :: neumann
+        end function
+        integer(c_int) function dcm_mpdata_vert3d(ctx, dt, w, psi, lbc, ubc, kord, monotone, &
+                neumann, early_exit) bind(c, name="dcm_mpdata_vert3d")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, w, psi
+            real(c_double), value :: dt
+            real(c_double), intent(inout) :: lbc, ubc
+            integer(c_int), value :: kord, monotone, neumann
+            integer(c_int), intent(out) :: early_exit
+        end function
+        integer(c_int) function dcm_mpdata_vec_vert3d(ctx, dt, w, psi, nq, lbc, ubc, kord, &
+                monotone, neumann, early_exit) bind(c, name="dcm_mpdata_vec_vert3d")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, w, psi
             real(c_double), value :: dt
             integer(c_int), value :: nq, kord, monotone, neumann
             real(c_double), intent(inout) :: lbc(*), ubc(*)

@@ -8,10 +8,14 @@
 !> thetan, xg, yg, zg (each n+2); padded fields u, v, w, th, q(nq), p.
 !> output: thbar, qbar(nq), div, su, sv, sw, sth, rhs, p, u, v, w (advanced),
 !> th (after adv_ref_state and mpdata_3d), q (after mpdata_vec_3d),
-!> psi (first_order_upstream_3d of the input th).
+!> psi (first_order_upstream_3d of the input th), then on the input fields:
+!> thv (mpdata_vert_3d of th with w), qsed (mpdata_vec_vert_3d of q with w,
+!> as PAMM calls it: kord 1, no limiter, neumann 2), tha (mpdata_3d_add of th
+!> with the base-state profile theta).
 program dropin_check
     use iso_c_binding
-    use advection_s_3d, only: mpdata_3d, mpdata_vec_3d, first_order_upstream_3d, adv_ref_state
+    use advection_s_3d, only: mpdata_3d, mpdata_vec_3d, first_order_upstream_3d, adv_ref_state, &
+        mpdata_vert_3d, mpdata_vec_vert_3d, mpdata_3d_add
     use d_solver, only: bicgstab, sources, advance_momentum
     use diagnostics, only: horizontal_means, divergence_calc
     implicit none
@@ -23,9 +27,9 @@ program dropin_check
         thetan, xg, yg, zg, ubar, vbar, wbar, thbar, zero_k, lbcq, ubcq
     real(wp), allocatable :: qbar(:, :)
     real(wp), allocatable, dimension(:, :, :) :: u, v, w, th, p, zu, zv, zw, su, sv, sw, sth, &
-        rhs, strain, vism, vist, div, psi, rad_power
+        rhs, strain, vism, vist, div, psi, rad_power, thv, tha
     real(wp), allocatable, dimension(:, :, :, :) :: q, sq, viss, flux_u, flux_d, ngs, lamgs, &
-        mugs
+        mugs, qsed
     real(wp) :: rad1(1), mvrecv(1), tri(1)
     character(len=1024) :: fin, fout
     integer :: iu
@@ -51,11 +55,14 @@ program dropin_check
     ! (a MOLD must already be allocated: give u its shape first)
     allocate(u(0:kp+1, 0:jp+1, 0:ip+1))
     allocate(v, w, th, p, zu, zv, zw, su, sv, sw, sth, rhs, strain, vism, vist, div, psi, &
-        rad_power, mold=u)
+        rad_power, thv, tha, mold=u)
     allocate(q(0:kp+1, 0:jp+1, 0:ip+1, nq))
-    allocate(sq, viss, mold=q)
+    allocate(sq, viss, qsed, mold=q)
     read(iu) u, v, w, th, q, p
     close(iu)
+    thv = th
+    tha = th
+    qsed = q
     allocate(ubar(0:kp+1), vbar(0:kp+1), wbar(0:kp+1), thbar(0:kp+1), zero_k(0:kp+1), &
         qbar(0:kp+1, nq), lbcq(nq), ubcq(nq))
     allocate(flux_u(0:kp+1, 0:jp+1, 0:ip+1, 1))
@@ -122,9 +129,22 @@ program dropin_check
         u, v, w, q, lbcq, ubcq, kord, monotone, 0, comm3d, id, dims, coords)
     call advance_momentum(comm3d, id, rank, 2.0_wp * dt, dx, dy, dz, dxn, dyn, dzn, rhoa, &
         rhoan, ip, jp, kp, l_h, r_h, u, v, w, zu, zv, zw, su, sv, sw, p, dims, coords)
+    ! vertical-only and profile-added advection of the input fields
+    lbc = 0.0_wp
+    ubc = 0.0_wp
+    call mpdata_vert_3d(dt, dz, dzn, rhoa, rhoan, ip, jp, kp, l_h, r_h, zw, thv, lbc, ubc, &
+        kord, monotone, 0, comm3d, id, dims, coords)
+    lbcq = 0.0_wp
+    ubcq = 0.0_wp
+    call mpdata_vec_vert_3d(dt, dz, dzn, rhoa, rhoan, ip, jp, kp, nq, l_h, r_h, zw, qsed, lbcq, &
+        ubcq, 1, .false., 2, comm3d, id, dims, coords)
+    lbc = 0.0_wp
+    ubc = 0.0_wp
+    call mpdata_3d_add(dt, dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, ip, jp, kp, l_h, r_h, zu, zv, &
+        zw, tha, theta, lbc, ubc, kord, monotone, 0, comm3d, id, dims, coords)
 
     open(newunit=iu, file=trim(fout), access='stream', form='unformatted', status='replace')
-    write(iu) thbar, qbar, div, su, sv, sw, sth, rhs, p, u, v, w, th, q, psi
+    write(iu) thbar, qbar, div, su, sv, sw, sth, rhs, p, u, v, w, th, q, psi, thv, qsed, tha
     close(iu)
 
 contains

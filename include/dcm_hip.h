@@ -116,6 +116,26 @@ int dcm_mpdata_vec3d(dcm_ctx *ctx, double dt, const double *u,
 /* adv_ref_state (:1148-1218): psi_ref is a host array (kp+2). */
 int dcm_adv_ref_state(dcm_ctx *ctx, double dt, const double *w, double *psi,
                       const double *psi_ref);
+/* mpdata_3d_add (:139-190): psi_1d (host array, kp+2) added to every
+ * column, halo levels included, then mpdata_3d, then psi_1d removed. */
+int dcm_mpdata3d_add(dcm_ctx *ctx, double dt, const double *u, const double *v,
+                     const double *w, double *psi, const double *psi_1d, double *lbc,
+                     double *ubc, int kord, int monotone, int neumann, int *early_exit);
+/* Vertical-only advection, used by PAMM sedimentation (pamm/microphysics.f90:
+ * 1522-1560).  first_order_upstream_vert_3d (:737-791): donor cell along z,
+ * psi updated in place; neumann 2 gives the floor face of level 1 the
+ * velocity w(1) (sedimentation out of the domain), neumann 1 zero-gradient
+ * fluxes at floor and lid. */
+int dcm_upwind_vert3d(dcm_ctx *ctx, double dt, const double *w, double *psi, int neumann);
+/* mpdata_vert_3d (:820-1126): as dcm_mpdata3d with the vertical velocity
+ * only; shift minimum over the whole array. */
+int dcm_mpdata_vert3d(dcm_ctx *ctx, double dt, const double *w, double *psi, double *lbc,
+                      double *ubc, int kord, int monotone, int neumann, int *early_exit);
+/* mpdata_vec_vert_3d (:1779-2087): nq contiguous padded fields share field
+ * 1's velocities and limiter; shift minima over the interior. */
+int dcm_mpdata_vec_vert3d(dcm_ctx *ctx, double dt, const double *w, double *psi, int nq,
+                          double *lbc, double *ubc, int kord, int monotone, int neumann,
+                          int *early_exit);
 
 /* ---------------------------------------------------------------------- */
 /* a6. horizontal_means (diagnostics.f90:31-107).  Outputs are HOST arrays */

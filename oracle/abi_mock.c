@@ -149,6 +149,39 @@ int dcm_mpdata_vec3d(void *ctx, double dt, const double *u, const double *v, con
     return 0;
 }
 
+int dcm_mpdata3d_add(void *ctx, double dt, const double *u, const double *v, const double *w,
+                     double *psi, const double *psi_1d, double *lbc, double *ubc, int kord,
+                     int monotone, int neumann, int *early_exit) {
+    mock_ctx *c = ctx;
+    const int e = or_mpdata3d_add(&c->g, dt, u, v, w, psi, psi_1d, lbc, ubc, kord, monotone,
+                                  neumann);
+    if (early_exit) *early_exit = e;
+    return 0;
+}
+
+int dcm_upwind_vert3d(void *ctx, double dt, const double *w, double *psi, int neumann) {
+    mock_ctx *c = ctx;
+    or_upwind_vert3d(&c->g, dt, w, psi, neumann);
+    return 0;
+}
+
+int dcm_mpdata_vert3d(void *ctx, double dt, const double *w, double *psi, double *lbc,
+                      double *ubc, int kord, int monotone, int neumann, int *early_exit) {
+    mock_ctx *c = ctx;
+    const int e = or_mpdata_vert3d(&c->g, dt, w, psi, lbc, ubc, kord, monotone, neumann);
+    if (early_exit) *early_exit = e;
+    return 0;
+}
+
+int dcm_mpdata_vec_vert3d(void *ctx, double dt, const double *w, double *psi, int nq,
+                          double *lbc, double *ubc, int kord, int monotone, int neumann,
+                          int *early_exit) {
+    mock_ctx *c = ctx;
+    const int e = or_mpdata_vec_vert3d(&c->g, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann);
+    if (early_exit) *early_exit = e;
+    return 0;
+}
+
 int dcm_adv_ref_state(void *ctx, double dt, const double *w, double *psi, const double *ref) {
     mock_ctx *c = ctx;
     or_adv_ref_state(&c->g, dt, w, psi, ref);

@@ -528,6 +528,251 @@ int or_mpdata_vec3d(const or_grid *g, double dt, const double *u,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* vertical-only advection (used by PAMM sedimentation)                     */
+/* ------------------------------------------------------------------------ */
+
+/* first_order_upstream_vert_3d, pamm/sfvt/advection_3d.F90:737-791.
+ * neumann 2: the bottom face of level 1 takes w(1) (:762-766); neumann 1:
+ * zero-gradient fluxes at the floor and lid (:780-787). */
+void or_upwind_vert3d(const or_grid *g, double dt, const double *w, double *psi,
+                      int neumann) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
+    const size_t n = (size_t)ip * jp * kp;
+    double *fr = zalloc(n), *fl = zalloc(n);
+#define F3(f, k, j, i) f[((size_t)((i)-1) * jp + ((j)-1)) * kp + ((k)-1)]
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double wk = A3(w, k, j, i);
+                const double wm = (k == 1 && neumann == 2) ? wk : A3(w, k - 1, j, i);
+                F3(fr, k, j, i) = ((((wk + fabs(wk)) * ra[k]) * A3(psi, k, j, i) +
+                                    ((wk - fabs(wk)) * ra[k]) * A3(psi, k + 1, j, i)) * dt) /
+                                  ((2.0 * dzn[k]) * rn[k]);
+                F3(fl, k, j, i) = ((((wm + fabs(wm)) * ra[k - 1]) * A3(psi, k - 1, j, i) +
+                                    ((wm - fabs(wm)) * ra[k - 1]) * A3(psi, k, j, i)) * dt) /
+                                  ((2.0 * dzn[k]) * rn[k]);
+            }
+    if (neumann == 1) {
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j) F3(fl, 1, j, i) = F3(fr, 1, j, i);
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j) F3(fr, kp, j, i) = F3(fl, kp, j, i);
+    }
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k)
+                A3(psi, k, j, i) = A3(psi, k, j, i) - (F3(fr, k, j, i) - F3(fl, k, j, i));
+#undef F3
+    free(fr);
+    free(fl);
+}
+
+/* Vertical antidiffusive velocity, pamm/sfvt/advection_3d.F90:942-958
+ * (vector variant :1905-1921): eq 13 of Smolarkiewicz (1984) with the
+ * cross terms zero and the divergence term of eq 38. */
+static void antidiffusive_vert(const or_grid *g, double dt, const double *wt,
+                               const double *po, double *ws) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *dz = g->dz, *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double gb3 = 0.5 * (rn[k] + rn[k + 1]);
+                const double u_div3 = 0.0, u_j_bar3 = 0.0;
+                const double wc = A3(wt, k, j, i);
+                const double p0 = A3(po, k, j, i), p1 = A3(po, k + 1, j, i);
+                double v = ((((fabs(wc) * dz[k] - ((dt * (ra[k] / gb3)) * wc) * wc) * (p1 - p0)) /
+                             ((p1 + p0) + SMALL)) / dzn[k]) - u_j_bar3;
+                v = v - ((0.25 * dt) * wc) *
+                            (((ra[k + 1] * A3(wt, k + 1, j, i) - ra[k - 1] * A3(wt, k - 1, j, i)) /
+                              (dz[k - 1] * gb3)) + u_div3);
+                A3(ws, k, j, i) = v;
+            }
+}
+
+/* Vertical limiter, pamm/sfvt/advection_3d.F90:1000-1080 (vector variant
+ * :1964-2036); psi = psi_store, po = psi_old. */
+static void limiter_vert(const or_grid *g, double dt, const double *wt,
+                         const double *psi, const double *po, double *ws) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const size_t N = NTOT(g);
+    const double *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
+    double *bup = zalloc(N), *bdn = zalloc(N);
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double vals[6] = {A3(psi, k - 1, j, i), A3(psi, k, j, i),
+                                        A3(psi, k + 1, j, i), A3(po, k - 1, j, i),
+                                        A3(po, k, j, i),      A3(po, k + 1, j, i)};
+                double mx = vals[0], mn = vals[0];
+                for (int q = 1; q < 6; ++q) {
+                    if (vals[q] > mx) mx = vals[q];
+                    if (vals[q] < mn) mn = vals[q];
+                }
+                const double pc = A3(po, k, j, i);
+                const double wm = A3(wt, k - 1, j, i), wc = A3(wt, k, j, i);
+                const double d1 = (dt * ((pos0(wm) * ra[k - 1]) * A3(po, k - 1, j, i) -
+                                         (neg0(wc) * ra[k]) * A3(po, k + 1, j, i))) / dzn[k] +
+                                  SMALL;
+                const double d2 =
+                    (dt * ((pos0(wc) * ra[k]) * pc - (neg0(wm) * ra[k - 1]) * pc)) / dzn[k] + SMALL;
+                A3(bup, k, j, i) = (rn[k] * (mx - pc)) / d1;
+                A3(bdn, k, j, i) = (rn[k] * (pc - mn)) / d2;
+            }
+    or_fill_along_z(g, bup, 0.0, 0.0);
+    or_fill_along_z(g, bdn, 0.0, 0.0);
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double wc = A3(wt, k, j, i);
+                A3(ws, k, j, i) = min3(1.0, A3(bdn, k, j, i), A3(bup, k + 1, j, i)) * pos0(wc) +
+                                  min3(1.0, A3(bup, k, j, i), A3(bdn, k + 1, j, i)) * neg0(wc);
+            }
+    free(bup);
+    free(bdn);
+}
+
+/* Pass loop shared by mpdata_vert_3d (:896-1115) and mpdata_vec_vert_3d
+ * (:1859-2071): fields[0] supplies psi_old and psi_store. */
+static void mpdata_vert_core(const or_grid *g, double dt, const double *w, double *fields,
+                             int nf, const double *lbc, const double *ubc, int kord,
+                             int monotone, int neumann) {
+    const size_t N = NTOT(g);
+    double *ws1 = zalloc(N), *ws2 = zalloc(N); /* w_store2 zeroed :876 */
+    double *psi_store = NULL;
+    const double *wt = w;
+    double *wt_sav = ws1;
+    if (kord > 1) {
+        psi_store = zalloc(N); /* field 1 is the one read (:1851-1855) */
+        memcpy(psi_store, fields, N * sizeof(double));
+    }
+    int it2 = 0;
+    for (int it = 1; it <= kord; ++it) {
+        if (it > 1) {
+            it2 += 1;
+            antidiffusive_vert(g, dt, wt, fields, wt_sav);
+            if (it2 % 2 == 1) {
+                wt = ws1;
+                wt_sav = ws2;
+            } else {
+                wt = ws2;
+                wt_sav = ws1;
+            }
+            or_fill_along_z(g, (double *)wt, 0.0, 0.0);
+        }
+        if (it > 1 && monotone) {
+            it2 += 1;
+            limiter_vert(g, dt, wt, psi_store, fields, wt_sav);
+            wt = ws2;
+            wt_sav = ws1;
+            or_fill_along_z(g, (double *)wt, 0.0, 0.0);
+        }
+        for (int n = 0; n < nf; ++n) {
+            double *f = fields + (size_t)n * N;
+            or_upwind_vert3d(g, dt, wt, f, neumann);
+            if (it <= kord && kord >= 1) or_fill_along_z(g, f, lbc[n], ubc[n]);
+        }
+    }
+    free(ws1);
+    free(ws2);
+    free(psi_store);
+}
+
+/* mpdata_vert_3d, pamm/sfvt/advection_3d.F90:820-1126 */
+int or_mpdata_vert3d(const or_grid *g, double dt, const double *w, double *psi,
+                     double *lbc, double *ubc, int kord, int monotone, int neumann) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const size_t N = NTOT(g);
+    /* minimum over the whole array and lbc/ubc: :861-866 */
+    double m = psi[0];
+    for (size_t q = 1; q < N; ++q)
+        if (psi[q] < m) m = psi[q];
+    if (*lbc < m) m = *lbc;
+    if (*ubc < m) m = *ubc;
+    for (size_t q = 0; q < N; ++q) psi[q] = psi[q] - m;
+    *lbc = *lbc - m;
+    *ubc = *ubc - m;
+    double s = 0.0; /* :868-869 */
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) s = s + A3(psi, k, j, i);
+    if (s < SMALL) { /* :870-875 */
+        for (size_t q = 0; q < N; ++q) psi[q] = psi[q] + m;
+        *lbc = *lbc + m;
+        *ubc = *ubc + m;
+        return 1;
+    }
+    mpdata_vert_core(g, dt, w, psi, 1, lbc, ubc, kord, monotone, neumann);
+    for (size_t q = 0; q < N; ++q) psi[q] = psi[q] + m; /* :1123-1125 */
+    *lbc = *lbc + m;
+    *ubc = *ubc + m;
+    return 0;
+}
+
+/* mpdata_vec_vert_3d, pamm/sfvt/advection_3d.F90:1779-2087 */
+int or_mpdata_vec_vert3d(const or_grid *g, double dt, const double *w, double *psi, int nq,
+                         double *lbc, double *ubc, int kord, int monotone, int neumann) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const size_t N = NTOT(g);
+    double *mg = zalloc((size_t)nq);
+    /* per-field minimum over the interior and lbc/ubc: :1821-1827 */
+    for (int n = 0; n < nq; ++n) {
+        double *f = psi + (size_t)n * N;
+        double m = A3(f, 1, 1, 1);
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j)
+                for (int k = 1; k <= kp; ++k)
+                    if (A3(f, k, j, i) < m) m = A3(f, k, j, i);
+        if (lbc[n] < m) m = lbc[n];
+        if (ubc[n] < m) m = ubc[n];
+        mg[n] = m;
+        for (size_t q = 0; q < N; ++q) f[q] = f[q] - m;
+        lbc[n] = lbc[n] - m;
+        ubc[n] = ubc[n] - m;
+    }
+    double s = 0.0; /* field 1: :1829-1830 */
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) s = s + A3(psi, k, j, i);
+    if (s < SMALL) {
+        for (int n = 0; n < nq; ++n) {
+            double *f = psi + (size_t)n * N;
+            for (size_t q = 0; q < N; ++q) f[q] = f[q] + mg[n];
+            lbc[n] = lbc[n] + mg[n];
+            ubc[n] = ubc[n] + mg[n];
+        }
+        free(mg);
+        return 1;
+    }
+    mpdata_vert_core(g, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann);
+    for (int n = 0; n < nq; ++n) {
+        double *f = psi + (size_t)n * N;
+        for (size_t q = 0; q < N; ++q) f[q] = f[q] + mg[n];
+        lbc[n] = lbc[n] + mg[n];
+        ubc[n] = ubc[n] + mg[n];
+    }
+    free(mg);
+    return 0;
+}
+
+/* mpdata_3d_add, pamm/sfvt/advection_3d.F90:139-190: a 1-D profile added
+ * to every column (halo levels included), mpdata_3d, the profile removed. */
+int or_mpdata3d_add(const or_grid *g, double dt, const double *u, const double *v,
+                    const double *w, double *psi, const double *psi_1d, double *lbc,
+                    double *ubc, int kord, int monotone, int neumann) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    for (int k = 0; k <= kp + 1; ++k)
+        for (int i = 0; i <= ip + 1; ++i)
+            for (int j = 0; j <= jp + 1; ++j) A3(psi, k, j, i) = A3(psi, k, j, i) + psi_1d[k];
+    const int e = or_mpdata3d(g, dt, u, v, w, psi, lbc, ubc, kord, monotone, neumann);
+    for (int k = 0; k <= kp + 1; ++k)
+        for (int i = 0; i <= ip + 1; ++i)
+            for (int j = 0; j <= jp + 1; ++j) A3(psi, k, j, i) = A3(psi, k, j, i) - psi_1d[k];
+    return e;
+}
+
 /* adv_ref_state, pamm/sfvt/advection_3d.F90:1148-1218 (halo columns too) */
 void or_adv_ref_state(const or_grid *g, double dt, const double *w,
                       double *psi, const double *ref) {

@@ -52,6 +52,16 @@ int or_mpdata_vec3d(const or_grid *g, double dt, const double *u,
                     int neumann);
 void or_adv_ref_state(const or_grid *g, double dt, const double *w,
                       double *psi, const double *psi_ref);
+/* vertical-only advection (PAMM sedimentation) and mpdata_3d_add */
+void or_upwind_vert3d(const or_grid *g, double dt, const double *w, double *psi,
+                      int neumann);
+int or_mpdata_vert3d(const or_grid *g, double dt, const double *w, double *psi,
+                     double *lbc, double *ubc, int kord, int monotone, int neumann);
+int or_mpdata_vec_vert3d(const or_grid *g, double dt, const double *w, double *psi, int nq,
+                         double *lbc, double *ubc, int kord, int monotone, int neumann);
+int or_mpdata3d_add(const or_grid *g, double dt, const double *u, const double *v,
+                    const double *w, double *psi, const double *psi_1d, double *lbc,
+                    double *ubc, int kord, int monotone, int neumann);
 
 /* diagnostics.f90 horizontal_means (computed over every level 0..kp+1) */
 void or_hmeans(const or_grid *g, const double *u, const double *v,

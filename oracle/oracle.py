@@ -78,6 +78,15 @@ def lib():
                                       C.c_int, C.c_int, C.c_int]
         L.or_mpdata_vec3d.restype = C.c_int
         L.or_adv_ref_state.argtypes = [g, d, _dp, _dp, _dp]
+        L.or_upwind_vert3d.argtypes = [g, d, _dp, _dp, C.c_int]
+        L.or_mpdata_vert3d.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+        L.or_mpdata_vert3d.restype = C.c_int
+        L.or_mpdata_vec_vert3d.argtypes = [g, d, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                           C.c_int]
+        L.or_mpdata_vec_vert3d.restype = C.c_int
+        L.or_mpdata3d_add.argtypes = [g, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                      C.c_int, C.c_int]
+        L.or_mpdata3d_add.restype = C.c_int
         L.or_hmeans.argtypes = [g, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.or_sources.argtypes = [g, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_matvec.argtypes = [g, _dp, _dp]
@@ -163,6 +172,36 @@ def mpdata_vec3d(g: Grid, dt, u, v, w, psi, lbc=None, ubc=None, kord=2,
     r = lib().or_mpdata_vec3d(g.ref, dt, ptr(u), ptr(v), ptr(w), ptr(psi), nq,
                               ptr(lb), ptr(ub), kord, int(monotone), neumann)
     return r, lb, ub
+
+
+def upwind_vert3d(g: Grid, dt, w, psi, neumann=0):
+    lib().or_upwind_vert3d(g.ref, dt, ptr(w), ptr(psi), neumann)
+
+
+def mpdata_vert3d(g: Grid, dt, w, psi, lbc=0.0, ubc=0.0, kord=2, monotone=True, neumann=0):
+    lb, ub = C.c_double(lbc), C.c_double(ubc)
+    r = lib().or_mpdata_vert3d(g.ref, dt, ptr(w), ptr(psi), C.byref(lb), C.byref(ub), kord,
+                               int(monotone), neumann)
+    return r, lb.value, ub.value
+
+
+def mpdata_vec_vert3d(g: Grid, dt, w, psi, lbc=None, ubc=None, kord=2, monotone=True,
+                      neumann=0):
+    nq = psi.shape[0]
+    lb = np.zeros(nq) if lbc is None else np.array(lbc, dtype=np.float64)
+    ub = np.zeros(nq) if ubc is None else np.array(ubc, dtype=np.float64)
+    r = lib().or_mpdata_vec_vert3d(g.ref, dt, ptr(w), ptr(psi), nq, ptr(lb), ptr(ub), kord,
+                                   int(monotone), neumann)
+    return r, lb, ub
+
+
+def mpdata3d_add(g: Grid, dt, u, v, w, psi, psi_1d, lbc=0.0, ubc=0.0, kord=2, monotone=True,
+                 neumann=0):
+    prof = np.ascontiguousarray(psi_1d, dtype=np.float64)
+    lb, ub = C.c_double(lbc), C.c_double(ubc)
+    r = lib().or_mpdata3d_add(g.ref, dt, ptr(u), ptr(v), ptr(w), ptr(psi), ptr(prof),
+                              C.byref(lb), C.byref(ub), kord, int(monotone), neumann)
+    return r, lb.value, ub.value
 
 
 def adv_ref_state(g: Grid, dt, w, psi, ref):

@@ -50,6 +50,7 @@ def _write_input(path, dom, nq, kord, monotone, dt, tol, fields):
 def _oracle_sequence(dom, nq, kord, monotone, dt, tol, u, v, w, th, q, p):
     """The program's call sequence on the oracle."""
     g = O.Grid(dom)
+    inputs = (u, v, w, th, q, p)
     u, v, w, th, q, p = (a.copy() for a in (u, v, w, th, q, p))
     zu, zv, zw = u.copy(), v.copy(), w.copy()
     means = O.hmeans(g, u, v, w, th, q)
@@ -68,8 +69,14 @@ def _oracle_sequence(dom, nq, kord, monotone, dt, tol, u, v, w, th, q, p):
     O.mpdata3d(g, dt, u, v, w, th, 0.0, 0.0, kord, monotone)
     O.mpdata_vec3d(g, dt, u, v, w, q, np.zeros(nq), np.zeros(nq), kord, monotone)
     O.advance_momentum(g, 2.0 * dt, u, v, w, zu, zv, zw, su, sv, sw, p)
+    # vertical-only and profile-added advection of the input fields
+    th0, q0 = inputs[3], inputs[4]
+    thv, qsed, tha = th0.copy(), q0.copy(), th0.copy()
+    O.mpdata_vert3d(g, dt, zw, thv, 0.0, 0.0, kord, monotone, 0)
+    O.mpdata_vec_vert3d(g, dt, zw, qsed, np.zeros(nq), np.zeros(nq), 1, False, 2)
+    O.mpdata3d_add(g, dt, zu, zv, zw, tha, dom.theta, 0.0, 0.0, kord, monotone, 0)
     return dict(thbar=thbar, qbar=qbar, div=div, su=su, sv=sv, sw=sw, sth=sth, rhs=rhs, p=p,
-                u=u, v=v, w=w, th=th, q=q, psi=psi), it
+                u=u, v=v, w=w, th=th, q=q, psi=psi, thv=thv, qsed=qsed, tha=tha), it
 
 
 def _read_output(path, dom, nq):
@@ -85,6 +92,12 @@ def _read_output(path, dom, nq):
     out["q"] = a[o:o + nq * n3].reshape((nq,) + dom.shape)
     o += nq * n3
     out["psi"] = a[o:o + n3].reshape(dom.shape)
+    o += n3
+    out["thv"] = a[o:o + n3].reshape(dom.shape)
+    o += n3
+    out["qsed"] = a[o:o + nq * n3].reshape((nq,) + dom.shape)
+    o += nq * n3
+    out["tha"] = a[o:o + n3].reshape(dom.shape)
     assert o + n3 == a.size
     return out
 
@@ -121,7 +134,7 @@ def _run_check(tmp_path, monotone, env):
     got = _read_output(fout, dom, nq)
     exp, _ = _oracle_sequence(dom, nq, kord, monotone, dt, tol, *fields)
     for name in ("thbar", "qbar", "div", "su", "sv", "sw", "sth", "rhs", "psi", "p", "u", "v",
-                 "w", "th", "q"):
+                 "w", "th", "q", "thv", "qsed", "tha"):
         np.testing.assert_array_equal(got[name], exp[name], err_msg=name)
 
 

@@ -169,6 +169,59 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
     return out
 
 
+def sedimentation(H, W, calls=20, cpu=True):
+    """PAMM's sedimentation call (pamm/microphysics.f90:1522-1530):
+    mpdata_vec_vert_3d of a three-field category falling at up to 8 m/s,
+    kord 1, no limiter, neumann 2, on the 256x256x128 grid.  Wall time per
+    call through the C-ABI (each call returns lbc/ubc to the host, as the
+    Fortran interface does).  Canonical bytes per cell: read the three
+    fields and w, write the three fields (7 words); the positivity shift and
+    the zero-sum test each read the fields once more in this implementation
+    (minimum per field, sum of field 1), and the pass writes to scratch
+    before the result is copied back: 6 words per field plus w."""
+    dom = W.make_domain(256, 256, 128, 50.0, 50.0, 50.0)
+    nq = 3
+    rng = np.random.default_rng(606)
+    psi = np.zeros((nq,) + dom.shape)
+    psi[:, 1:-1, 1:-1, 1:-1] = 1e-3 * rng.random((nq, 256, 256, 128))
+    vq = np.zeros(dom.shape)
+    vq[1:-1, 1:-1, 0:-1] = -8.0 * rng.random((256, 256, 129))
+    vq[:, :, -2] = 0.0
+    cells = dom.cells
+    with H.Context(dom) as c:
+        dw, dpsi = c.array(vq), c.array(psi)
+        c.mpdata_vec_vert3d(0.5, dw, dpsi, nq, [0.0] * nq, [0.0] * nq, 1, False, 2)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            c.mpdata_vec_vert3d(0.5, dw, dpsi, nq, [0.0] * nq, [0.0] * nq, 1, False, 2)
+        wall = (time.perf_counter() - t0) / calls
+    canonical = 8.0 * (2 * nq + 1) * cells
+    moved = 8.0 * (6 * nq + nq) * cells
+    out = {"call": "mpdata_vec_vert_3d, 3 fields, kord 1, no limiter, neumann 2",
+           "grid": "256x256x128", "ms_per_call": 1e3 * wall,
+           "cell_steps_per_s": cells / wall,
+           "canonical_GBps": canonical / wall / 1e9, "moved_GBps": moved / wall / 1e9}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.build()
+        sd = W.make_domain(64, 64, 128, 50.0, 50.0, 50.0)
+        g = O.Grid(sd)
+        sp = np.zeros((nq,) + sd.shape)
+        sp[:, 1:-1, 1:-1, 1:-1] = 1e-3 * rng.random((nq, 64, 64, 128))
+        sw = np.zeros(sd.shape)
+        sw[1:-1, 1:-1, 0:-1] = -8.0 * rng.random((64, 64, 129))
+        t0 = time.perf_counter()
+        n = 5
+        for _ in range(n):
+            O.mpdata_vec_vert3d(g, 0.5, sw, sp, np.zeros(nq), np.zeros(nq), 1, False, 2)
+        cw = (time.perf_counter() - t0) / n
+        out["cpu_baseline"] = {"value": sd.cells / cw, "unit": "cell-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"oracle, same call on 64x64x128, "
+                                                         f"{n} calls, {n * cw:.2f} s"}
+    return out
+
+
 def ensemble_line(H, W, args):
     """One JSON line for the 64-member ensemble alone: value = the batched
     mixed path's cell-steps/s (graphs on), the roofline from a second, event
@@ -356,6 +409,8 @@ def main(argv=None):
         a, b = snaps["fp64"], snaps["mixed"]
         line["mixed_error_vs_fp64"] = {
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
+    print("[bench] sedimentation (vertical MPDATA) 256x256x128", file=sys.stderr, flush=True)
+    line["sedimentation"] = sedimentation(H, W, cpu=not args.no_cpu)
     if args.ensemble_steps > 0:
         print(f"[bench] ensemble: 64 uneven members, {args.ensemble_steps} steps",
               file=sys.stderr, flush=True)

@@ -123,6 +123,53 @@ def make_domain(ip: int, jp: int, kp: int, dx: float, dy: float, dz: float,
 
 
 @dataclasses.dataclass
+class Physics:
+    """The options of the reference's sources beyond the inviscid branch
+    (dynamics.f90:639-1000): Smagorinsky-Lilly subgrid viscosity with the
+    Monin-Obukhov surface layer, a damping layer, large-scale forcing and
+    subsidence.  1-D profiles have kp+2 entries."""
+
+    viscous: bool = True
+    damping_layer: bool = False
+    forcing: bool = False
+    divergence: bool = False
+    forcing_tau: float = 3600.0
+    z0: float = 1e-4
+    z0th: float = 1e-4
+    lamsq: np.ndarray = None
+    dampfacn: np.ndarray = None
+    dampfac: np.ndarray = None
+    u_force: np.ndarray = None
+    v_force: np.ndarray = None
+    w_subs: np.ndarray = None
+
+
+def physics_profiles(dom: Domain, u=None, v=None, viscous=True, cvis=0.23, z0=1e-4, z0th=1e-4,
+                     damping_layer=False, damping_thickness=None, damping_tau=1000.0,
+                     forcing=False, forcing_tau=3600.0, divergence=False, divergence_val=0.0,
+                     divergence_hgt=1000.0) -> Physics:
+    """Profiles as the reference's allocate_and_set builds them
+    (initialisation.f90:449-452 mixing length, :466-491 damping layer above
+    zn(kp) - damping_thickness, :838-853 forcing profile from column (1,1)
+    and the subsidence profile).  dx, dy are the domain's uniform spacings."""
+    dx, dy = float(dom.dx[1]), float(dom.dy[1])
+    lamsq = 1.0 / (1.0 / (cvis * (dx + dy + dom.dz) / 3.0) ** 2 + 1.0 / (0.4 * (dom.z + z0)) ** 2)
+    n = dom.kp + 2
+    dampfacn, dampfac = np.zeros(n), np.zeros(n)
+    if damping_layer:
+        thick = damping_thickness if damping_thickness is not None else 0.25 * dom.zn[dom.kp]
+        ztop = dom.zn[dom.kp]
+        dampfacn[dom.zn > ztop - thick] = -1.0 / damping_tau
+        dampfac[dom.z > ztop - thick] = -1.0 / damping_tau
+    u_force = np.zeros(n) if u is None else np.array(u[1, 1, :], dtype=np.float64)
+    v_force = np.zeros(n) if v is None else np.array(v[1, 1, :], dtype=np.float64)
+    w_subs = np.where(dom.z <= divergence_hgt, -divergence_val * dom.z,
+                      -divergence_val * divergence_hgt)
+    return Physics(viscous, damping_layer, forcing, divergence, forcing_tau, z0, z0th, lamsq,
+                   dampfacn, dampfac, u_force, v_force, w_subs)
+
+
+@dataclasses.dataclass
 class Case:
     """A runnable configuration: domain, run settings and initial fields."""
 
@@ -144,6 +191,7 @@ class Case:
     w: np.ndarray = None
     th: np.ndarray = None
     q: np.ndarray = None  # (nq, ip+2, jp+2, kp+2)
+    physics: Physics = None  # None: the inviscid branch of sources
 
     @property
     def moisture(self) -> bool:

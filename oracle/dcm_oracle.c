@@ -822,12 +822,10 @@ void or_hmeans(const or_grid *g, const double *u, const double *v,
 /* momentum sources and Poisson right-hand side                            */
 /* ------------------------------------------------------------------------ */
 
-/* sources, dynamics.f90:639-1000 (viscous, damping_layer, forcing and
- * divergence all off) */
-void or_sources(const or_grid *g, const double *u, const double *v,
-                const double *w, const double *th, const double *theta,
-                const double *thbar, double *su, double *sv, double *sw,
-                double *sth, double *rhs) {
+/* advective tendencies and buoyancy of sources, dynamics.f90:743-797 */
+static void sources_advective(const or_grid *g, const double *u, const double *v,
+                              const double *w, const double *th, const double *theta,
+                              const double *thbar, double *su, double *sv, double *sw) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
     const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
@@ -879,11 +877,12 @@ void or_sources(const or_grid *g, const double *u, const double *v,
                            (((2.0 * theta[k]) + thbar[k]) + thbar[k + 1]);
                 A3(sw, k, j, i) = ra[k] * (((X + Y) + Z) + B);
             }
-    /* inviscid branch: exchange_full on su, sv, sw, sth (:909-916) */
-    or_fill_full(g, su, 0.0, 0.0);
-    or_fill_full(g, sv, 0.0, 0.0);
-    or_fill_full(g, sw, 0.0, 0.0);
-    or_fill_full(g, sth, 0.0, 0.0);
+}
+
+/* floor and lid rows and the Poisson right-hand side, dynamics.f90:922-998 */
+static void sources_finish(const or_grid *g, double *su, double *sv, double *sw, double *rhs) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
     /* floor and lid rows: :922-973 */
     for (int i = 0; i <= ip + 1; ++i)
         for (int j = 0; j <= jp + 1; ++j) {
@@ -910,6 +909,306 @@ void or_sources(const or_grid *g, const double *u, const double *v,
             A3(rhs, 0, j, i) = ((A3(su, 0, j, i) - A3(su, 0, j, i - 1)) / dx[i - 1] +
                                 (A3(sv, 0, j, i) - A3(sv, 0, j - 1, i)) / dy[j - 1]) +
                                (A3(sw, 0, j, i) + A3(sw, 1, j, i)) / dz[1];
+}
+
+/* sources, dynamics.f90:639-1000 (viscous, damping_layer, forcing and
+ * divergence all off) */
+void or_sources(const or_grid *g, const double *u, const double *v,
+                const double *w, const double *th, const double *theta,
+                const double *thbar, double *su, double *sv, double *sw,
+                double *sth, double *rhs) {
+    sources_advective(g, u, v, w, th, theta, thbar, su, sv, sw);
+    /* inviscid branch: exchange_full on su, sv, sw, sth (:909-916) */
+    or_fill_full(g, su, 0.0, 0.0);
+    or_fill_full(g, sv, 0.0, 0.0);
+    or_fill_full(g, sw, 0.0, 0.0);
+    or_fill_full(g, sth, 0.0, 0.0);
+    sources_finish(g, su, sv, sw, rhs);
+}
+
+/* ------------------------------------------------------------------------ */
+/* subgrid model (sgm/subgrid_3d.f90) and the other terms of sources        */
+/* ------------------------------------------------------------------------ */
+
+/* parameters of subgrid_3d.f90:13-16 */
+static const double SG_PRN = 0.95, SG_SUBB = 40.0, SG_SUBC = 16.0, SG_SUBG = 1.2,
+                    SG_SUBH = 0.0, SG_RIC = 0.25, SG_SMALL = 1e-10, SG_KVON = 0.4;
+#define SG_SUBA (1.0 / SG_PRN)
+#define SG_SUBF SG_SUBA
+static inline double sq2(double x) { return x * x; }
+/* x**4._wp: evaluated as two squarings (the reference's libm pow is within
+ * an ulp of it) */
+static inline double pow4(double x) {
+    const double t = x * x;
+    return t * t;
+}
+
+/* 2 S_ij S_ij on the w points, subgrid_3d.f90:91-131 (the sixth term of
+ * the 2 s12 s12 average repeats the second, as written there) */
+static double strain_rate(const or_grid *g, const double *u, const double *v, const double *w,
+                          int k, int j, int i) {
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
+    const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
+    double s = sq2((A3(u, k + 1, j, i) - A3(u, k + 1, j, i - 1)) / dx[i - 1]) +
+               sq2((A3(u, k, j, i) - A3(u, k, j, i - 1)) / dx[i - 1]);
+    s = (s + sq2((A3(v, k + 1, j, i) - A3(v, k + 1, j - 1, i)) / dy[j - 1])) +
+        sq2((A3(v, k, j, i) - A3(v, k, j - 1, i)) / dy[j - 1]);
+    s = (s + sq2((A3(w, k, j, i) - A3(w, k - 1, j, i)) / dz[k - 1])) +
+        sq2((A3(w, k + 1, j, i) - A3(w, k, j, i)) / dz[k]);
+    s = s + 0.5 * (sq2((A3(u, k + 1, j, i) - A3(u, k, j, i)) / dzn[k] +
+                       (A3(w, k, j, i + 1) - A3(w, k, j, i)) / dxn[i]) +
+                   sq2((A3(u, k + 1, j, i - 1) - A3(u, k, j, i - 1)) / dz[k] +
+                       (A3(w, k, j, i) - A3(w, k, j, i - 1)) / dxn[i - 1]));
+    s = s + 0.5 * (sq2((A3(w, k, j, i) - A3(w, k, j - 1, i)) / dyn[j - 1] +
+                       (A3(v, k + 1, j - 1, i) - A3(v, k, j - 1, i)) / dzn[k]) +
+                   sq2((A3(w, k, j + 1, i) - A3(w, k, j, i)) / dyn[j] +
+                       (A3(v, k + 1, j, i) - A3(v, k, j, i)) / dzn[k]));
+    const double p1 = (A3(u, k, j, i - 1) - A3(u, k, j - 1, i - 1)) / dyn[j - 1] +
+                      (A3(v, k, j - 1, i) - A3(v, k, j - 1, i - 1)) / dxn[i - 1];
+    const double p2 = (A3(u, k, j + 1, i - 1) - A3(u, k, j, i - 1)) / dyn[j] +
+                      (A3(v, k, j, i) - A3(v, k, j, i - 1)) / dxn[i - 1];
+    const double p3 = (A3(u, k, j, i) - A3(u, k, j - 1, i)) / dyn[j - 1] +
+                      (A3(v, k, j - 1, i + 1) - A3(v, k, j - 1, i)) / dxn[i];
+    const double p4 = (A3(u, k, j + 1, i) - A3(u, k, j, i)) / dyn[j] +
+                      (A3(v, k, j, i + 1) - A3(v, k, j, i)) / dxn[i];
+    const double p5 = (A3(u, k + 1, j, i - 1) - A3(u, k + 1, j - 1, i - 1)) / dyn[j - 1] +
+                      (A3(v, k + 1, j - 1, i) - A3(v, k + 1, j - 1, i - 1)) / dxn[i - 1];
+    const double p6 = (A3(u, k, j + 1, i - 1) - A3(u, k, j, i - 1)) / dyn[j] +
+                      (A3(v, k, j, i) - A3(v, k, j, i - 1)) / dxn[i - 1];
+    const double p7 = (A3(u, k + 1, j, i) - A3(u, k + 1, j - 1, i)) / dyn[j - 1] +
+                      (A3(v, k + 1, j - 1, i + 1) - A3(v, k + 1, j - 1, i)) / dxn[i];
+    const double p8 = (A3(u, k + 1, j + 1, i) - A3(u, k + 1, j, i)) / dyn[j] +
+                      (A3(v, k + 1, j, i + 1) - A3(v, k + 1, j, i)) / dxn[i];
+    s = s + 0.125 * (((((((sq2(p1) + sq2(p2)) + sq2(p3)) + sq2(p4)) + sq2(p5)) + sq2(p6)) +
+                      sq2(p7)) + sq2(p8));
+    return s;
+}
+
+/* richardson, subgrid_3d.f90:465-491: the gradient Richardson number and
+ * the stability functions fm, fh.  1.e-15 is a default-real literal. */
+static void richardson_fns(double strain, double th_k, double th_kp1, double thn_k,
+                           double thn_kp1, double dzn_k, double *fm, double *fh) {
+    const double rip = ((GRAV * (th_kp1 / thn_kp1 - th_k / thn_k)) / dzn_k) /
+                       (strain + (double)1.e-15f);
+    if (rip < 0.0) {
+        *fm = sqrt(1.0 - SG_SUBC * rip); /* (.)**0.5_wp */
+        *fh = SG_SUBA * sqrt(1.0 - SG_SUBB * rip);
+    } else if (rip >= 0.0 && rip < SG_RIC) {
+        const double x = pow4(1.0 - rip / SG_RIC);
+        *fm = x * (1.0 - SG_SUBH * rip);
+        *fh = (SG_SUBF * x) * (1.0 - SG_SUBG * rip);
+    } else {
+        *fm = 0.0;
+        *fh = 0.0;
+    }
+}
+
+/* monin_obukhov, subgrid_3d.f90:373-433: surface-layer viscosities at k=0
+ * (Jacobson's non-iterative scheme) */
+static void monin_obukhov(const or_grid *g, const or_physics *ph, const double *th,
+                          const double *u, const double *v, double *vism, double *vist) {
+    const int ip = g->ip, jp = g->jp;
+    const double kv2 = SG_KVON * SG_KVON, L = ph->log_z1_z0, z1 = ph->z[1];
+    const double z0 = ph->z0, z0th = ph->z0th, th0 = ph->theta[0], thn1 = ph->thetan[1];
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j) {
+            const double vmag = 0.5 * sqrt(sq2(A3(u, 1, j, i) + A3(u, 1, j, i - 1)) +
+                                           sq2(A3(v, 1, j, i) + A3(v, 1, j - 1, i))) +
+                                SG_SMALL;
+            const double dth = (A3(th, 1, j, i) + thn1) - th0;
+            const double rib = ((GRAV * dth) * sq2(z1 - z0)) / ((th0 * sq2(vmag)) * (z1 - z0th));
+            double gm, gh;
+            if (rib <= 0.0) {
+                const double q = sqrt(fabs((rib * z1) / z0));
+                gm = 1.0 - (9.4 * rib) / (1.0 + ((70.0 * kv2) * q) / sq2(L));
+                gh = 1.0 - (9.4 * rib) / (1.0 + ((50.0 * kv2) * q) / sq2(L));
+            } else {
+                gm = 1.0 / sq2(1.0 + 4.7 * rib);
+                gh = 1.0 / sq2(1.0 + 4.7 * rib);
+            }
+            const double ustar = ((SG_KVON * vmag) / L) * sqrt(gm);
+            const double thstar = (((kv2 * vmag) * dth) / ((ustar * SG_PRN) * sq2(L))) * gh;
+            A3(vism, 0, j, i) = (sq2(ustar) * (z1 - z0)) / vmag;
+            A3(vist, 0, j, i) = (ustar * thstar) / (dth / (z1 - z0th) + SG_SMALL);
+        }
+}
+
+/* viscous flux divergence of a scalar (sth :288-312, sq :322-349); vt is
+ * vist, already multiplied by rhoa */
+static void diffuse_scalar(const or_grid *g, const double *vt, const double *T, double *sT) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz, *dxn = g->dxn, *dyn = g->dyn;
+    const double *rn = g->rhoan;
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double ax = ((A3(vt, k, j, i) + A3(vt, k, j, i - 1)) + A3(vt, k - 1, j, i)) +
+                                  A3(vt, k - 1, j, i - 1);
+                const double bx = ((A3(vt, k, j, i + 1) + A3(vt, k, j, i)) + A3(vt, k - 1, j, i + 1)) +
+                                  A3(vt, k - 1, j, i);
+                const double ay = ((A3(vt, k, j, i) + A3(vt, k, j - 1, i)) + A3(vt, k - 1, j, i)) +
+                                  A3(vt, k - 1, j - 1, i);
+                const double by = ((A3(vt, k, j + 1, i) + A3(vt, k, j, i)) + A3(vt, k - 1, j + 1, i)) +
+                                  A3(vt, k - 1, j, i);
+                const double tc = A3(T, k, j, i);
+                const double X = (0.25 * ((ax * (A3(T, k, j, i - 1) - tc)) / dxn[i - 1] -
+                                          (bx * (tc - A3(T, k, j, i + 1))) / dxn[i])) /
+                                 dx[i - 1];
+                const double Y = (0.25 * ((ay * (A3(T, k, j - 1, i) - tc)) / dyn[j - 1] -
+                                          (by * (tc - A3(T, k, j + 1, i))) / dyn[j])) /
+                                 dy[j - 1];
+                const double Z = ((A3(vt, k - 1, j, i) * (A3(T, k - 1, j, i) - tc)) / dz[k - 1] -
+                                  (A3(vt, k, j, i) * (tc - A3(T, k + 1, j, i))) / dz[k]) /
+                                 dz[k - 1];
+                A3(sT, k, j, i) = A3(sT, k, j, i) + ((X + Y) + Z) / rn[k];
+            }
+}
+
+/* calculate_subgrid_3d, sgm/subgrid_3d.f90:38-357 */
+void or_subgrid(const or_grid *g, const or_physics *ph, const double *u, const double *v,
+                const double *w, const double *zu, const double *zv, const double *zw,
+                const double *th, const double *q, double *vism, double *vist, double *viss,
+                double *strain, double *su, double *sv, double *sw, double *sth, double *sq) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const size_t N = NTOT(g);
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
+    const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
+    const double *ra = g->rhoa, *rn = g->rhoan, *thn = ph->thetan;
+    /* strain rate :91-131, Richardson functions :137, viscosities :143-159 */
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double s = strain_rate(g, u, v, w, k, j, i);
+                double fm, fh;
+                richardson_fns(s, A3(th, k, j, i), A3(th, k + 1, j, i), thn[k], thn[k + 1],
+                               dzn[k], &fm, &fh);
+                A3(strain, k, j, i) = s;
+                A3(vism, k, j, i) = (ph->lamsq[k] * sqrt(s)) * fm;
+                A3(vist, k, j, i) = ((ph->lamsq[k] * sqrt(s)) * fh) * ra[k];
+            }
+    /* no viscosity in the two levels below the lid :163-166 */
+    for (int k = kp - 1; k <= kp; ++k)
+        for (int i = 0; i <= ip + 1; ++i)
+            for (int j = 0; j <= jp + 1; ++j) {
+                A3(vism, k, j, i) = 0.0;
+                A3(vist, k, j, i) = 0.0;
+            }
+    monin_obukhov(g, ph, th, u, v, vism, vist); /* :167-172 */
+    or_fill_full_wo(g, vism);                    /* :176-179 */
+    or_fill_full_wo(g, vist);
+    if (ph->moisture) /* :182-187 */
+        for (int n = 0; n < ph->nq; ++n) memcpy(viss + (size_t)n * N, vist, N * sizeof(double));
+    /* stress tensor :189-223 */
+    double *t11 = zalloc(N), *t22 = zalloc(N), *t33 = zalloc(N);
+    double *t12 = zalloc(N), *t13 = zalloc(N), *t23 = zalloc(N);
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const double r1 = rn[k] * 1.0;
+                const double vv = A3(vism, k, j, i) + A3(vism, k - 1, j, i);
+                A3(t11, k, j, i) = ((r1 * vv) * (A3(zu, k, j, i) - A3(zu, k, j, i - 1))) / dx[i - 1];
+                A3(t22, k, j, i) = ((r1 * vv) * (A3(zv, k, j, i) - A3(zv, k, j - 1, i))) / dy[j - 1];
+                A3(t33, k, j, i) = ((r1 * vv) * (A3(zw, k, j, i) - A3(zw, k - 1, j, i))) / dz[k - 1];
+                const double s8 =
+                    ((((((A3(vism, k, j, i) + A3(vism, k, j + 1, i)) + A3(vism, k - 1, j, i)) +
+                        A3(vism, k - 1, j + 1, i)) + A3(vism, k, j, i + 1)) +
+                      A3(vism, k, j + 1, i + 1)) + A3(vism, k - 1, j, i + 1)) +
+                    A3(vism, k - 1, j + 1, i + 1);
+                A3(t12, k, j, i) = ((rn[k] * 0.125) * s8) *
+                                   ((A3(zu, k, j + 1, i) - A3(zu, k, j, i)) / dyn[j] +
+                                    (A3(zv, k, j, i + 1) - A3(zv, k, j, i)) / dxn[i]);
+            }
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                A3(t13, k, j, i) = ((ra[k] * 0.5) * (A3(vism, k, j, i) + A3(vism, k, j, i + 1))) *
+                                   ((A3(zu, k + 1, j, i) - A3(zu, k, j, i)) / dzn[k] +
+                                    (A3(zw, k, j, i + 1) - A3(zw, k, j, i)) / dxn[i]);
+                A3(t23, k, j, i) = ((ra[k] * 0.5) * (A3(vism, k, j, i) + A3(vism, k, j + 1, i))) *
+                                   ((A3(zw, k, j + 1, i) - A3(zw, k, j, i)) / dyn[j] +
+                                    (A3(zv, k + 1, j, i) - A3(zv, k, j, i)) / dzn[k]);
+            }
+    double *taus[6] = {t11, t22, t33, t12, t13, t23}; /* :228-239 */
+    for (int n = 0; n < 6; ++n) or_fill_along_dim(g, taus[n], 0.0, 0.0);
+    /* momentum sources :241-278: sw on 1..kp-1, su and sv on 2..kp-1 */
+    const int ktmp = kp - 1, ktmp1 = 2;
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j) {
+            for (int k = 1; k <= ktmp; ++k)
+                A3(sw, k, j, i) = A3(sw, k, j, i) +
+                                  (((A3(t33, k + 1, j, i) - A3(t33, k, j, i)) / dzn[k] +
+                                    (A3(t13, k, j, i) - A3(t13, k, j, i - 1)) / dx[i - 1]) +
+                                   (A3(t23, k, j, i) - A3(t23, k, j - 1, i)) / dy[j - 1]);
+            for (int k = ktmp1; k <= ktmp; ++k) {
+                A3(su, k, j, i) = A3(su, k, j, i) +
+                                  (((A3(t11, k, j, i + 1) - A3(t11, k, j, i)) / dxn[i] +
+                                    (A3(t12, k, j, i) - A3(t12, k, j - 1, i)) / dy[j - 1]) +
+                                   (A3(t13, k, j, i) - A3(t13, k - 1, j, i)) / dz[k - 1]);
+                A3(sv, k, j, i) = A3(sv, k, j, i) +
+                                  (((A3(t22, k, j + 1, i) - A3(t22, k, j, i)) / dyn[j] +
+                                    (A3(t12, k, j, i) - A3(t12, k, j, i - 1)) / dx[i - 1]) +
+                                   (A3(t23, k, j, i) - A3(t23, k - 1, j, i)) / dz[k - 1]);
+            }
+        }
+    or_fill_full(g, su, 0.0, 0.0); /* :279-284 */
+    or_fill_full(g, sv, 0.0, 0.0);
+    or_fill_full(g, sw, 0.0, 0.0);
+    diffuse_scalar(g, vist, th, sth); /* :288-315 */
+    or_fill_full(g, sth, 0.0, 0.0);
+    if (ph->moisture) /* :320-354 */
+        for (int n = 0; n < ph->nq; ++n) {
+            diffuse_scalar(g, vist, q + (size_t)n * N, sq + (size_t)n * N);
+            or_fill_full(g, sq + (size_t)n * N, 0.0, 0.0);
+        }
+    for (int n = 0; n < 6; ++n) free(taus[n]);
+}
+
+/* sources, dynamics.f90:639-1000, every option but radiation */
+void or_sources_phys(const or_grid *g, const or_physics *ph, const double *u, const double *v,
+                     const double *w, const double *zu, const double *zv, const double *zw,
+                     const double *th, const double *q, const double *theta,
+                     const double *thbar, double *su, double *sv, double *sw, double *sth,
+                     double *sq, double *strain, double *vism, double *vist, double *viss,
+                     double *rhs) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *ra = g->rhoa, *rn = g->rhoan;
+    sources_advective(g, u, v, w, th, theta, thbar, su, sv, sw);
+    if (ph->damping_layer) /* :802-839 */
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j)
+                for (int k = 1; k <= kp; ++k) {
+                    A3(su, k, j, i) = A3(su, k, j, i) +
+                                      (rn[k] * ph->dampfacn[k]) * (A3(zu, k, j, i) - ph->ubar[k]);
+                    A3(sv, k, j, i) = A3(sv, k, j, i) +
+                                      (rn[k] * ph->dampfacn[k]) * (A3(zv, k, j, i) - ph->vbar[k]);
+                    A3(sw, k, j, i) = A3(sw, k, j, i) +
+                                      (ra[k] * ph->dampfac[k]) * (A3(zw, k, j, i) - 0.0);
+                    A3(sth, k, j, i) = A3(sth, k, j, i) + ph->dampfacn[k] * (A3(th, k, j, i) - 0.0);
+                }
+    if (ph->forcing) /* :841-860 */
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j)
+                for (int k = 1; k <= kp; ++k) {
+                    A3(su, k, j, i) = A3(su, k, j, i) -
+                                      (rn[k] * (A3(zu, k, j, i) - ph->u_force[k])) / ph->forcing_tau;
+                    A3(sv, k, j, i) = A3(sv, k, j, i) -
+                                      (rn[k] * (A3(zv, k, j, i) - ph->v_force[k])) / ph->forcing_tau;
+                }
+    if (ph->divergence) /* :863-873 */
+        for (int i = 1; i <= ip; ++i)
+            for (int j = 1; j <= jp; ++j)
+                for (int k = 1; k <= kp; ++k)
+                    A3(sw, k, j, i) = A3(sw, k, j, i) - rn[k] * (ph->wbar[k] - ph->w_subs[k]);
+    if (ph->viscous) { /* :878-906 */
+        or_subgrid(g, ph, u, v, w, zu, zv, zw, th, q, vism, vist, viss, strain, su, sv, sw, sth,
+                   sq);
+    } else { /* :907-917 */
+        or_fill_full(g, su, 0.0, 0.0);
+        or_fill_full(g, sv, 0.0, 0.0);
+        or_fill_full(g, sw, 0.0, 0.0);
+        or_fill_full(g, sth, 0.0, 0.0);
+    }
+    sources_finish(g, su, sv, sw, rhs);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1189,8 +1488,18 @@ int or_model_step(const or_grid *g, or_model *m) {
     or_hmeans(g, u, v, w, m->th, m->moisture ? m->q : NULL,
               m->moisture ? m->nq : 0, m->ubar, m->vbar, m->wbar, m->thbar,
               m->qbar); /* :433-435 */
-    or_sources(g, u, v, w, m->th, m->theta, m->thbar, m->su, m->sv, m->sw,
-               m->sth, m->psrc); /* :455-479 */
+    if (m->phys) { /* :455-479 with the physics options */
+        or_physics ph = *m->phys;
+        ph.ubar = m->ubar;
+        ph.vbar = m->vbar;
+        ph.wbar = m->wbar;
+        or_sources_phys(g, &ph, u, v, w, zu, zv, zw, m->th, m->q, m->theta, m->thbar, m->su,
+                        m->sv, m->sw, m->sth, m->sq, m->strain, m->vism, m->vist, m->viss,
+                        m->psrc);
+    } else {
+        or_sources(g, u, v, w, m->th, m->theta, m->thbar, m->su, m->sv, m->sw,
+                   m->sth, m->psrc); /* :455-479 */
+    }
     or_fill_along_dim_wo(g, m->psrc); /* :485-487 */
     int iters = 0;
     double err = 0.0;

@@ -74,6 +74,36 @@ void or_sources(const or_grid *g, const double *u, const double *v,
                 const double *w, const double *th, const double *theta,
                 const double *thbar, double *su, double *sv, double *sw,
                 double *sth, double *rhs);
+/* Physics options of sources (dynamics.f90:639-1000) beyond the inviscid
+ * branch.  1-D arrays have kp+2 entries (index 0..kp+1). */
+typedef struct {
+    int viscous, moisture, damping_layer, forcing, divergence;
+    int nq;                               /* advected q fields (moisture) */
+    double forcing_tau, z0, z0th;
+    /* log(z(1)/z0): one scalar per grid, supplied by the caller so that
+     * every implementation uses the same rounded value */
+    double log_z1_z0;
+    const double *ubar, *vbar, *wbar;     /* horizontal means */
+    const double *dampfacn, *dampfac;     /* damping rates, scalar / w levels */
+    const double *u_force, *v_force, *w_subs;
+    const double *lamsq;                  /* squared mixing length, w levels */
+    const double *z;                      /* heights of the w levels */
+    const double *theta, *thetan;         /* base state, w / scalar levels */
+} or_physics;
+/* calculate_subgrid_3d, sgm/subgrid_3d.f90:38-357 (with richardson
+ * :448-493 and monin_obukhov :373-433): adds the viscous terms to su, sv, sw,
+ * sth (and sq when moisture); fills vism, vist, strain (and viss) */
+void or_subgrid(const or_grid *g, const or_physics *ph, const double *u, const double *v,
+                const double *w, const double *zu, const double *zv, const double *zw,
+                const double *th, const double *q, double *vism, double *vist, double *viss,
+                double *strain, double *su, double *sv, double *sw, double *sth, double *sq);
+/* sources with every option of the reference except radiation */
+void or_sources_phys(const or_grid *g, const or_physics *ph, const double *u, const double *v,
+                     const double *w, const double *zu, const double *zv, const double *zw,
+                     const double *th, const double *q, const double *theta,
+                     const double *thbar, double *su, double *sv, double *sw, double *sth,
+                     double *sq, double *strain, double *vism, double *vist, double *viss,
+                     double *rhs);
 
 /* dynamics.f90 pressure solver pieces */
 void or_matvec(const or_grid *g, double *x, double *ax);  /* mat_ax */
@@ -130,6 +160,10 @@ typedef struct {
     double t1, t2;      /* saved lbc/ubc of the theta advection */
     int last_iters;     /* bicgstab iterations of the last step */
     double last_err;    /* its final recursive residual */
+    /* physics options of sources (NULL: the inviscid branch) and the
+     * subgrid work fields (strain, vism, vist; viss has nq fields) */
+    const or_physics *phys;
+    double *strain, *vism, *vist, *viss;
 } or_model;
 
 /* initial z-boundary rows of the previous time level (driver_code.f90:287-295) */

@@ -11,6 +11,7 @@ place exactly like the Fortran routine it restates.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -28,6 +29,17 @@ class _Grid(C.Structure):
                 ("dx", _dp), ("dy", _dp), ("dz", _dp),
                 ("dxn", _dp), ("dyn", _dp), ("dzn", _dp),
                 ("rhoa", _dp), ("rhoan", _dp)]
+
+
+class _Physics(C.Structure):
+    _fields_ = [("viscous", C.c_int), ("moisture", C.c_int), ("damping_layer", C.c_int),
+                ("forcing", C.c_int), ("divergence", C.c_int), ("nq", C.c_int),
+                ("forcing_tau", C.c_double), ("z0", C.c_double), ("z0th", C.c_double),
+                ("log_z1_z0", C.c_double),
+                ("ubar", _dp), ("vbar", _dp), ("wbar", _dp),
+                ("dampfacn", _dp), ("dampfac", _dp),
+                ("u_force", _dp), ("v_force", _dp), ("w_subs", _dp),
+                ("lamsq", _dp), ("z", _dp), ("theta", _dp), ("thetan", _dp)]
 
 
 class _Model(C.Structure):
@@ -48,7 +60,28 @@ class _Model(C.Structure):
                 ("qbar", _dp),
                 ("n", C.c_int), ("solver_first", C.c_int),
                 ("t1", C.c_double), ("t2", C.c_double),
-                ("last_iters", C.c_int), ("last_err", C.c_double)]
+                ("last_iters", C.c_int), ("last_err", C.c_double),
+                ("phys", C.POINTER(_Physics)),
+                ("strain", _dp), ("vism", _dp), ("vist", _dp), ("viss", _dp)]
+
+
+class Physics:
+    """An or_physics struct built from a workloads.Physics and its domain;
+    the horizontal means are arrays of this object (set them before use)."""
+
+    def __init__(self, dom, ph, nq=0):
+        n = dom.kp + 2
+        self.ubar, self.vbar, self.wbar = np.zeros(n), np.zeros(n), np.zeros(n)
+        arr = lambda a: np.ascontiguousarray(np.zeros(n) if a is None else a, dtype=np.float64)
+        self._keep = [arr(a) for a in (ph.dampfacn, ph.dampfac, ph.u_force, ph.v_force,
+                                        ph.w_subs, ph.lamsq, dom.z, dom.theta, dom.thetan)]
+        dfn, df, uf, vf, ws, lam, z, th, thn = self._keep
+        self.s = _Physics(int(ph.viscous), int(nq > 0), int(ph.damping_layer), int(ph.forcing),
+                          int(ph.divergence), nq, ph.forcing_tau, ph.z0, ph.z0th,
+                          math.log(dom.z[1] / ph.z0), ptr(self.ubar), ptr(self.vbar),
+                          ptr(self.wbar), ptr(dfn), ptr(df), ptr(uf), ptr(vf), ptr(ws),
+                          ptr(lam), ptr(z), ptr(th), ptr(thn))
+        self.ref = C.byref(self.s)
 
 
 def build() -> None:
@@ -78,6 +111,9 @@ def lib():
                                       C.c_int, C.c_int, C.c_int]
         L.or_mpdata_vec3d.restype = C.c_int
         L.or_adv_ref_state.argtypes = [g, d, _dp, _dp, _dp]
+        ph = C.POINTER(_Physics)
+        L.or_subgrid.argtypes = [g, ph] + [_dp] * 17
+        L.or_sources_phys.argtypes = [g, ph] + [_dp] * 20
         L.or_upwind_vert3d.argtypes = [g, d, _dp, _dp, C.c_int]
         L.or_mpdata_vert3d.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
         L.or_mpdata_vert3d.restype = C.c_int
@@ -204,6 +240,20 @@ def mpdata3d_add(g: Grid, dt, u, v, w, psi, psi_1d, lbc=0.0, ubc=0.0, kord=2, mo
     return r, lb.value, ub.value
 
 
+def subgrid(g: Grid, ph: Physics, u, v, w, zu, zv, zw, th, q, vism, vist, viss, strain, su, sv,
+            sw, sth, sq):
+    lib().or_subgrid(g.ref, ph.ref, *[ptr(a) for a in (u, v, w, zu, zv, zw, th, q, vism, vist,
+                                                        viss, strain, su, sv, sw, sth, sq)])
+
+
+def sources_phys(g: Grid, ph: Physics, u, v, w, zu, zv, zw, th, q, theta, thbar, su, sv, sw,
+                 sth, sq, strain, vism, vist, viss, rhs):
+    lib().or_sources_phys(g.ref, ph.ref, *[ptr(np.ascontiguousarray(a)) if a is not None else None
+                                           for a in (u, v, w, zu, zv, zw, th, q, theta, thbar)],
+                          *[ptr(a) for a in (su, sv, sw, sth, sq, strain, vism, vist, viss,
+                                             rhs)])
+
+
 def adv_ref_state(g: Grid, dt, w, psi, ref):
     ref = np.ascontiguousarray(ref, dtype=np.float64)
     lib().or_adv_ref_state(g.ref, dt, ptr(w), ptr(psi), ptr(ref))
@@ -311,6 +361,14 @@ class Model:
                         ptr(self.psrc), ptr(self.sq),
                         ptr(self.ubar), ptr(self.vbar), ptr(self.wbar),
                         ptr(self.thbar), ptr(self.qbar), 0, 1, 0.0, 0.0, 0, 0.0)
+        self.phys = None
+        if case.physics is not None:
+            self.phys = Physics(d, case.physics, case.nq)
+            self.strain, self.vism, self.vist = z(), z(), z()
+            self.viss = np.zeros((nq,) + sh)
+            self.m.phys = C.pointer(self.phys.s)
+            self.m.strain, self.m.vism = ptr(self.strain), ptr(self.vism)
+            self.m.vist, self.m.viss = ptr(self.vist), ptr(self.viss)
         lib().or_model_init_bcs(self.g.ref, C.byref(self.m))
         self.iters = []
 

@@ -127,8 +127,10 @@ int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **ou
         return DCM_BAD_ARG;
     }
     if (c->d1d_cap < need) {
-        const size_t cap = stride * 8;
-        if (slot >= 8) {
+        // every slot is allocated at the first upload, so earlier slots of
+        // the same call stay valid
+        const size_t cap = stride * 16;
+        if (slot >= 16) {
             set_error("1-D scratch slot out of range");
             return DCM_BAD_ARG;
         }

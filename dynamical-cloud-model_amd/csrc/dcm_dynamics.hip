@@ -3,6 +3,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <string>
 
 #include "dcm_device.h"
 
@@ -283,9 +284,12 @@ __global__ void k_init_rows_top(double *zu, double *zv, double *zw, Geo g) {
 // ---------------------------------------------------------------------------
 // internal entry points used by the model and the C-ABI
 // ---------------------------------------------------------------------------
+// sources (dynamics.f90:639-1000).  ph == nullptr: the inviscid branch;
+// otherwise the damping layer, forcing and subsidence terms and, when
+// ph->viscous, the subgrid model on sf.
 int sources_dev(dcm_ctx *c, const double *u, const double *v, const double *w, const double *th,
                 const double *theta_d, const double *thbar_d, double *su, double *sv, double *sw,
-                double *sth, double *rhs) {
+                double *sth, double *rhs, const PhysicsDev *ph, const SubgridFields *sf) {
     const Geo &g = c->geo;
     Interior L = interior_launch(g);
     int tok = prof_begin(c, "sources");
@@ -293,11 +297,17 @@ int sources_dev(dcm_ctx *c, const double *u, const double *v, const double *w, c
                        su, sv, sw, g, c->met);
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
-    // inviscid branch: exchange_full(su,sv,sw,sth) (dynamics.f90:909-916)
-    DCM_CHECK(fill<double>(c, su, FILL_FULL, 0.0, 0.0));
-    DCM_CHECK(fill<double>(c, sv, FILL_FULL, 0.0, 0.0));
-    DCM_CHECK(fill<double>(c, sw, FILL_FULL, 0.0, 0.0));
-    if (sth) DCM_CHECK(fill<double>(c, sth, FILL_FULL, 0.0, 0.0));
+    if (ph) DCM_CHECK(relax_dev(c, *ph, sf->zu, sf->zv, sf->zw, th, su, sv, sw, sth));
+    if (ph && ph->viscous) {
+        // the subgrid model fills su, sv, sw, sth itself (:878-906)
+        DCM_CHECK(subgrid_dev(c, *ph, u, v, w, th, *sf, su, sv, sw, sth));
+    } else {
+        // inviscid branch: exchange_full(su,sv,sw,sth) (dynamics.f90:907-917)
+        DCM_CHECK(fill<double>(c, su, FILL_FULL, 0.0, 0.0));
+        DCM_CHECK(fill<double>(c, sv, FILL_FULL, 0.0, 0.0));
+        DCM_CHECK(fill<double>(c, sw, FILL_FULL, 0.0, 0.0));
+        if (sth) DCM_CHECK(fill<double>(c, sth, FILL_FULL, 0.0, 0.0));
+    }
     hipLaunchKernelGGL(k_sources_rows, dim3(256), dim3(kBlock), 0, c->stream, su, sv, sw, g);
     DCM_LAUNCH_CHECK();
     Interior R = interior_launch(g, 2);
@@ -410,7 +420,117 @@ int dcm_sources(dcm_ctx *c, const double *u, const double *v, const double *w, c
     const double *dth, *dthbar;
     DCM_CHECK(upload_1d(c, theta, (size_t)c->geo.kp + 2, 1, &dth));
     DCM_CHECK(upload_1d(c, thbar, (size_t)c->geo.kp + 2, 2, &dthbar));
-    return sources_dev(c, u, v, w, th, dth, dthbar, su, sv, sw, sth, rhs);
+    return sources_dev(c, u, v, w, th, dth, dthbar, su, sv, sw, sth, rhs, nullptr, nullptr);
+}
+
+// the device form of a dcm_physics: host profiles uploaded to 1-D scratch
+// slots 3..12 (ubar..wbar may be NULL when damping and subsidence are off)
+static int physics_upload(dcm_ctx *c, const dcm_physics *h, const double *ubar,
+                          const double *vbar, const double *wbar, int nq, PhysicsDev *ph) {
+    const size_t n = (size_t)c->geo.kp + 2;
+    *ph = PhysicsDev();
+    ph->viscous = h->viscous != 0;
+    ph->damping_layer = h->damping_layer != 0;
+    ph->forcing = h->forcing != 0;
+    ph->divergence = h->divergence != 0;
+    ph->moisture = nq > 0;
+    ph->nq = nq;
+    ph->forcing_tau = h->forcing_tau;
+    ph->z0 = h->z0;
+    ph->z0th = h->z0th;
+    auto up = [&](const double *a, int slot, const double **out, const char *what) -> int {
+        if (!a) {
+            set_error(std::string("dcm_physics: ") + what + " is required by the options set");
+            return DCM_BAD_ARG;
+        }
+        return upload_1d(c, a, n, slot, out);
+    };
+    if (ph->damping_layer) {
+        DCM_CHECK(up(ubar, 3, &ph->ubar, "ubar"));
+        DCM_CHECK(up(vbar, 4, &ph->vbar, "vbar"));
+        DCM_CHECK(up(h->dampfacn, 6, &ph->dampfacn, "dampfacn"));
+        DCM_CHECK(up(h->dampfac, 7, &ph->dampfac, "dampfac"));
+    }
+    if (ph->forcing) {
+        if (!(h->forcing_tau != 0.0)) {
+            set_error("dcm_physics: forcing_tau must be non-zero");
+            return DCM_BAD_ARG;
+        }
+        DCM_CHECK(up(h->u_force, 8, &ph->u_force, "u_force"));
+        DCM_CHECK(up(h->v_force, 9, &ph->v_force, "v_force"));
+    }
+    if (ph->divergence) {
+        DCM_CHECK(up(wbar, 5, &ph->wbar, "wbar"));
+        DCM_CHECK(up(h->w_subs, 10, &ph->w_subs, "w_subs"));
+    }
+    if (ph->viscous) {
+        if (!h->z || !h->theta || !h->thetan || !(h->z0 > 0.0)) {
+            set_error("dcm_physics: the subgrid model needs z, theta, thetan and z0 > 0");
+            return DCM_BAD_ARG;
+        }
+        DCM_CHECK(up(h->lamsq, 11, &ph->lamsq, "lamsq"));
+        DCM_CHECK(up(h->thetan, 12, &ph->thetan, "thetan"));
+        ph->z1 = h->z[1];
+        ph->theta0 = h->theta[0];
+        ph->thetan1 = h->thetan[1];
+        ph->log_z1_z0 = std::log(h->z[1] / h->z0);
+    }
+    return DCM_OK;
+}
+
+int dcm_subgrid(dcm_ctx *c, const dcm_physics *ph, const double *u, const double *v,
+                const double *w, const double *zu, const double *zv, const double *zw,
+                const double *th, const double *q, int nq, double *vism, double *vist,
+                double *viss, double *strain, double *su, double *sv, double *sw, double *sth,
+                double *sq) {
+    if (!c || !ph || !u || !v || !w || !zu || !zv || !zw || !th || !vism || !vist || !strain ||
+        !su || !sv || !sw || !sth || nq < 0 || (nq > 0 && (!q || !sq || !viss)))
+        return DCM_BAD_ARG;
+    dcm_physics h = *ph;
+    h.viscous = 1;
+    h.damping_layer = h.forcing = h.divergence = 0;
+    PhysicsDev d;
+    DCM_CHECK(physics_upload(c, &h, nullptr, nullptr, nullptr, nq, &d));
+    SubgridFields sf;
+    sf.zu = zu;
+    sf.zv = zv;
+    sf.zw = zw;
+    sf.q = q;
+    sf.sq = sq;
+    sf.strain = strain;
+    sf.vism = vism;
+    sf.vist = vist;
+    sf.viss = viss;
+    return subgrid_dev(c, d, u, v, w, th, sf, su, sv, sw, sth);
+}
+
+int dcm_sources_phys(dcm_ctx *c, const dcm_physics *ph, const double *ubar, const double *vbar,
+                     const double *wbar, const double *u, const double *v, const double *w,
+                     const double *zu, const double *zv, const double *zw, const double *th,
+                     const double *q, int nq, const double *theta, const double *thbar,
+                     double *su, double *sv, double *sw, double *sth, double *sq,
+                     double *strain, double *vism, double *vist, double *viss, double *rhs) {
+    if (!c || !ph || !u || !v || !w || !zu || !zv || !zw || !th || !theta || !thbar || !su ||
+        !sv || !sw || !sth || !rhs || nq < 0)
+        return DCM_BAD_ARG;
+    if (ph->viscous && (!strain || !vism || !vist || (nq > 0 && (!q || !sq || !viss))))
+        return DCM_BAD_ARG;
+    PhysicsDev d;
+    DCM_CHECK(physics_upload(c, ph, ubar, vbar, wbar, nq, &d));
+    const double *dth, *dthbar;
+    DCM_CHECK(upload_1d(c, theta, (size_t)c->geo.kp + 2, 1, &dth));
+    DCM_CHECK(upload_1d(c, thbar, (size_t)c->geo.kp + 2, 2, &dthbar));
+    SubgridFields sf;
+    sf.zu = zu;
+    sf.zv = zv;
+    sf.zw = zw;
+    sf.q = q;
+    sf.sq = sq;
+    sf.strain = strain;
+    sf.vism = vism;
+    sf.vist = vist;
+    sf.viss = viss;
+    return sources_dev(c, u, v, w, th, dth, dthbar, su, sv, sw, sth, rhs, &d, &sf);
 }
 
 int dcm_advance_momentum(dcm_ctx *c, double dt, double *u, double *v, double *w,
@@ -631,7 +751,7 @@ static int model_pre_solve(dcm_model *m) {
     // :601-603 (q += sq*dt, th += sth*dt) is an exact no-op and is skipped.
     DCM_CHECK(level_means(c, m->th, m->thbar()));                  // :433-435
     DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
-                          m->sv, m->sw, nullptr, m->psrc));          // :455-479
+                          m->sv, m->sw, nullptr, m->psrc, nullptr, nullptr));          // :455-479
     DCM_CHECK(fill<double>(c, m->psrc, FILL_ALONG_DIM_WO, 0.0, 0.0)); // :485-487
     return DCM_OK;
 }

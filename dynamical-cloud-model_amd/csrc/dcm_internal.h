@@ -273,6 +273,33 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
 int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
                     float *vf, float *wf);
 int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref_dev);
+
+// Physics options of sources beyond the inviscid branch, device side: 1-D
+// profiles are device arrays of kp+2.  log_z1_z0 = log(z(1)/z0) is computed
+// on the host once per grid.
+struct PhysicsDev {
+    int viscous = 0, moisture = 0, damping_layer = 0, forcing = 0, divergence = 0, nq = 0;
+    double forcing_tau = 1.0, z0 = 1e-4, z0th = 1e-4, log_z1_z0 = 0.0;
+    double z1 = 0.0, theta0 = 0.0, thetan1 = 0.0;
+    const double *ubar = nullptr, *vbar = nullptr, *wbar = nullptr;
+    const double *dampfacn = nullptr, *dampfac = nullptr;
+    const double *u_force = nullptr, *v_force = nullptr, *w_subs = nullptr;
+    const double *lamsq = nullptr, *thetan = nullptr;
+};
+// the 3-D fields the subgrid model reads and writes besides u, v, w, th and
+// the momentum / theta tendencies (q, sq and viss have nq fields)
+struct SubgridFields {
+    const double *zu = nullptr, *zv = nullptr, *zw = nullptr, *q = nullptr;
+    double *sq = nullptr, *strain = nullptr, *vism = nullptr, *vist = nullptr, *viss = nullptr;
+};
+// work-field slots of the six stress components
+constexpr int kSlotStress = 80;
+int subgrid_dev(dcm_ctx *c, const PhysicsDev &ph, const double *u, const double *v,
+                const double *w, const double *th, const SubgridFields &sf, double *su,
+                double *sv, double *sw, double *sth);
+int relax_dev(dcm_ctx *c, const PhysicsDev &ph, const double *zu, const double *zv,
+              const double *zw, const double *th, double *su, double *sv, double *sw,
+              double *sth);
 // one pressure system of a batched solve: its context (geometry,
 // coefficients, scalars, work vectors) and its padded sw, x, b
 struct SolveTask {

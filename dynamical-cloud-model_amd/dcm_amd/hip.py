@@ -42,6 +42,26 @@ class StepStats(C.Structure):
                 ("fallbacks", _i), ("reserved", _i), ("last_err", _d), ("max_true_resid", _d)]
 
 
+class PhysicsStruct(C.Structure):
+    """dcm_physics: options of sources beyond the inviscid branch."""
+    _fields_ = [("viscous", _i), ("damping_layer", _i), ("forcing", _i), ("divergence", _i),
+                ("forcing_tau", _d), ("z0", _d), ("z0th", _d),
+                ("lamsq", _dp), ("dampfacn", _dp), ("dampfac", _dp), ("u_force", _dp),
+                ("v_force", _dp), ("w_subs", _dp), ("z", _dp), ("theta", _dp), ("thetan", _dp)]
+
+
+def physics_struct(dom, ph):
+    """A dcm_physics for a workloads.Physics on its domain; returns the
+    struct and the arrays it points into (keep both alive)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+            for a in (ph.lamsq, ph.dampfacn, ph.dampfac, ph.u_force, ph.v_force, ph.w_subs,
+                      dom.z, dom.theta, dom.thetan)]
+    st = PhysicsStruct(int(ph.viscous), int(ph.damping_layer), int(ph.forcing),
+                       int(ph.divergence), ph.forcing_tau, ph.z0, ph.z0th,
+                       *[_dptr(a) if a is not None else None for a in keep])
+    return st, keep
+
+
 _SIGS = {
     "dcm_last_error": ([], C.c_char_p),
     "dcm_version": ([], C.c_char_p),
@@ -67,6 +87,9 @@ _SIGS = {
     "dcm_mpdata3d": ([_vp, _d, _vp, _vp, _vp, _vp, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_mpdata_vec3d": ([_vp, _d, _vp, _vp, _vp, _vp, _i, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_adv_ref_state": ([_vp, _d, _vp, _vp, _dp], _i),
+    "dcm_subgrid": ([_vp, C.POINTER(PhysicsStruct)] + [_vp] * 8 + [_i] + [_vp] * 9, _i),
+    "dcm_sources_phys": ([_vp, C.POINTER(PhysicsStruct), _dp, _dp, _dp] + [_vp] * 8 + [_i] +
+                         [_dp, _dp] + [_vp] * 10, _i),
     "dcm_mpdata3d_add": ([_vp, _d, _vp, _vp, _vp, _vp, _dp, _dp, _dp, _i, _i, _i, _ip], _i),
     "dcm_upwind_vert3d": ([_vp, _d, _vp, _vp, _i], _i),
     "dcm_mpdata_vert3d": ([_vp, _d, _vp, _vp, _dp, _dp, _i, _i, _i, _ip], _i),
@@ -281,6 +304,27 @@ class Context:
         thbar = np.ascontiguousarray(thbar, dtype=np.float64)
         _call("dcm_sources", self.h, u.ptr, v.ptr, w.ptr, th.ptr, _dptr(theta), _dptr(thbar),
               su.ptr, sv.ptr, sw.ptr, sth.ptr if sth else None, rhs.ptr)
+
+    def subgrid(self, ph, u, v, w, zu, zv, zw, th, q, nq, vism, vist, viss, strain, su, sv, sw,
+                sth, sq):
+        st, keep = physics_struct(self.dom, ph)
+        _call("dcm_subgrid", self.h, C.byref(st), u.ptr, v.ptr, w.ptr, zu.ptr, zv.ptr, zw.ptr,
+              th.ptr, q.ptr if q is not None else None, nq, vism.ptr, vist.ptr,
+              viss.ptr if viss is not None else None, strain.ptr, su.ptr, sv.ptr, sw.ptr,
+              sth.ptr, sq.ptr if sq is not None else None)
+
+    def sources_phys(self, ph, ubar, vbar, wbar, u, v, w, zu, zv, zw, th, q, nq, theta, thbar,
+                     su, sv, sw, sth, sq, strain, vism, vist, viss, rhs):
+        st, keep = physics_struct(self.dom, ph)
+        host = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (ubar, vbar, wbar, theta, thbar)]
+        ub, vb, wb, tht, thb = host
+        opt = lambda a: a.ptr if a is not None else None
+        _call("dcm_sources_phys", self.h, C.byref(st), *[_dptr(a) if a is not None else None
+                                                         for a in (ub, vb, wb)],
+              u.ptr, v.ptr, w.ptr, zu.ptr, zv.ptr, zw.ptr, th.ptr, opt(q), nq, _dptr(tht),
+              _dptr(thb), su.ptr, sv.ptr, sw.ptr, sth.ptr, opt(sq), opt(strain), opt(vism),
+              opt(vist), opt(viss), rhs.ptr)
 
     # -- a8-a12 solver --------------------------------------------------------
     def matvec(self, x, ax):

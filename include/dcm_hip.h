@@ -160,6 +160,44 @@ int dcm_sources(dcm_ctx *ctx, const double *u, const double *v,
                 const double *thbar, double *su, double *sv, double *sw,
                 double *sth, double *rhs);
 
+/* Options of sources beyond the inviscid branch (dynamics.f90:639-1000):
+ * the subgrid model (viscous; sgm/subgrid_3d.f90:38-493), the damping layer
+ * (:802-839), forcing (:841-860) and subsidence (divergence, :863-873).
+ * Profiles are HOST arrays of kp+2 and may be NULL when no option set reads
+ * them: lamsq (squared mixing length on w levels), z (heights of the w
+ * levels), theta/thetan (base state) for viscous; dampfacn/dampfac for the
+ * damping layer; u_force/v_force for forcing; w_subs for subsidence.
+ * Radiation is not on this path. */
+typedef struct dcm_physics {
+    int viscous, damping_layer, forcing, divergence;
+    double forcing_tau, z0, z0th;
+    const double *lamsq, *dampfacn, *dampfac, *u_force, *v_force, *w_subs;
+    const double *z, *theta, *thetan;
+} dcm_physics;
+/* calculate_subgrid_3d (sgm/subgrid_3d.f90:38-357): Smagorinsky-Lilly
+ * viscosities with Richardson-number stability functions and the
+ * Monin-Obukhov surface layer; adds the viscous terms to su, sv, sw, sth
+ * (and to the nq fields of sq when nq > 0, copying vist into viss) and
+ * fills them as the reference does.  Device fields; zu, zv, zw are the
+ * previous time level. */
+int dcm_subgrid(dcm_ctx *ctx, const dcm_physics *ph, const double *u, const double *v,
+                const double *w, const double *zu, const double *zv, const double *zw,
+                const double *th, const double *q, int nq, double *vism, double *vist,
+                double *viss, double *strain, double *su, double *sv, double *sw, double *sth,
+                double *sq);
+/* sources with every option but radiation: the inviscid terms, then the
+ * options of ph, then the floor/lid rows and the Poisson right-hand side.
+ * ubar, vbar, wbar are the horizontal means (HOST, kp+2; needed by the
+ * damping layer and subsidence), theta and thbar HOST arrays as in
+ * dcm_sources; the subgrid fields are used when ph->viscous. */
+int dcm_sources_phys(dcm_ctx *ctx, const dcm_physics *ph, const double *ubar,
+                     const double *vbar, const double *wbar, const double *u,
+                     const double *v, const double *w, const double *zu, const double *zv,
+                     const double *zw, const double *th, const double *q, int nq,
+                     const double *theta, const double *thbar, double *su, double *sv,
+                     double *sw, double *sth, double *sq, double *strain, double *vism,
+                     double *vist, double *viss, double *rhs);
+
 /* ---------------------------------------------------------------------- */
 /* a8-a12. Pressure solver (dynamics.f90:24-482).                         */
 /* ---------------------------------------------------------------------- */

@@ -203,6 +203,12 @@ __global__ __launch_bounds__(256) void k_divergence(const double *u, const doubl
 }
 
 // advance_scalar_fields_3d (sgm/subgrid_3d.f90:601-621), whole arrays
+__global__ void k_zero_field(double *a, long long n) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x)
+        a[t] = 0.0;
+}
+
 __global__ void k_axpy_dt(double *a, const double *s, double dt, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x)
@@ -355,6 +361,13 @@ struct dcm_model {
     double inner_rtol = 1e-4;  // fewest inner iterations at 256x256x128 (1e-3..1e-5 measured)
     bool check_residual = false;
     std::vector<void *> allocs;
+    // physics options of sources (dcm_model_set_physics): device profiles
+    // in phys1d (ubar vbar wbar dampfacn dampfac u_force v_force w_subs
+    // lamsq thetan, kp+2 each) and the subgrid fields
+    bool phys_on = false;
+    dcm::PhysicsDev phys;
+    double *phys1d = nullptr;
+    double *strain = nullptr, *vism = nullptr, *vist = nullptr, *viss = nullptr, *sq = nullptr;
 
     double *theta() { return oned; }
     double *thetan() { return oned + (c->geo.kp + 2); }
@@ -639,6 +652,97 @@ int dcm_model_destroy(dcm_model *m) {
     return DCM_OK;
 }
 
+int dcm_model_set_physics(dcm_model *m, const dcm_physics *ph) {
+    if (!m) return DCM_BAD_ARG;
+    DCM_HIP(hipStreamSynchronize(m->c->stream));
+    if (!ph) {
+        m->phys_on = false;
+        return DCM_OK;
+    }
+    dcm_ctx *c = m->c;
+    const Geo &g = c->geo;
+    const size_t nk = (size_t)g.kp + 2;
+    if (ph->viscous && (!ph->z || !ph->theta || !ph->thetan || !ph->lamsq || !(ph->z0 > 0.0))) {
+        set_error("dcm_model_set_physics: the subgrid model needs lamsq, z, theta, thetan, z0 > 0");
+        return DCM_BAD_ARG;
+    }
+    if (ph->damping_layer && (!ph->dampfacn || !ph->dampfac)) {
+        set_error("dcm_model_set_physics: the damping layer needs dampfacn and dampfac");
+        return DCM_BAD_ARG;
+    }
+    if (ph->forcing && (!ph->u_force || !ph->v_force || !(ph->forcing_tau != 0.0))) {
+        set_error("dcm_model_set_physics: forcing needs u_force, v_force and forcing_tau != 0");
+        return DCM_BAD_ARG;
+    }
+    if (ph->divergence && !ph->w_subs) {
+        set_error("dcm_model_set_physics: subsidence needs w_subs");
+        return DCM_BAD_ARG;
+    }
+    void *p;
+    if (!m->phys1d) {
+        DCM_CHECK(model_alloc(m, 10 * nk * sizeof(double), &p));
+        m->phys1d = (double *)p;
+    }
+    if (!m->sth) {
+        DCM_CHECK(model_alloc(m, g.ntot * sizeof(double), &p));
+        m->sth = (double *)p;
+    }
+    if (ph->viscous) {
+        double **f[3] = {&m->strain, &m->vism, &m->vist};
+        for (double **a : f)
+            if (!*a) {
+                DCM_CHECK(model_alloc(m, g.ntot * sizeof(double), &p));
+                *a = (double *)p;
+            }
+        if (m->nq > 0 && !m->sq) {
+            DCM_CHECK(model_alloc(m, (size_t)m->nq * g.ntot * sizeof(double), &p));
+            m->sq = (double *)p;
+            DCM_CHECK(model_alloc(m, (size_t)m->nq * g.ntot * sizeof(double), &p));
+            m->viss = (double *)p;
+        }
+    }
+    // profiles: 0 ubar 1 vbar 2 wbar (computed each step) 3 dampfacn
+    // 4 dampfac 5 u_force 6 v_force 7 w_subs 8 lamsq 9 thetan
+    std::vector<double> h(10 * nk, 0.0);
+    const double *src[10] = {nullptr, nullptr, nullptr, ph->dampfacn, ph->dampfac, ph->u_force,
+                             ph->v_force, ph->w_subs, ph->lamsq, ph->thetan};
+    for (int a = 3; a < 10; ++a)
+        if (src[a]) std::memcpy(h.data() + a * nk, src[a], nk * sizeof(double));
+    DCM_HIP(hipMemcpyAsync(m->phys1d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    PhysicsDev d;
+    d.viscous = ph->viscous != 0;
+    d.damping_layer = ph->damping_layer != 0;
+    d.forcing = ph->forcing != 0;
+    d.divergence = ph->divergence != 0;
+    d.moisture = m->nq > 0 && m->sq != nullptr;
+    d.nq = m->nq;
+    d.forcing_tau = ph->forcing_tau;
+    d.z0 = ph->z0;
+    d.z0th = ph->z0th;
+    const double *P = m->phys1d;
+    d.ubar = P;
+    d.vbar = P + nk;
+    d.wbar = P + 2 * nk;
+    d.dampfacn = P + 3 * nk;
+    d.dampfac = P + 4 * nk;
+    d.u_force = P + 5 * nk;
+    d.v_force = P + 6 * nk;
+    d.w_subs = P + 7 * nk;
+    d.lamsq = P + 8 * nk;
+    d.thetan = P + 9 * nk;
+    if (d.viscous) {
+        d.z1 = ph->z[1];
+        d.theta0 = ph->theta[0];
+        d.thetan1 = ph->thetan[1];
+        d.log_z1_z0 = std::log(ph->z[1] / ph->z0);
+    }
+    m->phys = d;
+    m->phys_on = true;
+    return DCM_OK;
+}
+
 int dcm_model_set_precision(dcm_model *m, int precision) {
     if (!m || (precision != DCM_FP64 && precision != DCM_MIXED)) return DCM_BAD_ARG;
     m->precision = precision;
@@ -746,12 +850,42 @@ static int model_pre_solve(dcm_model *m) {
     dcm_ctx *c = m->c;
     double *cur[3], *prev[3], *fut[3];
     m->levels(cur, prev, fut);
-    // sth = 0 (:340) stays zero: sources' inviscid branch only fills it, and
-    // the model never writes it; sq is likewise zero, so the scalar update of
-    // :601-603 (q += sq*dt, th += sth*dt) is an exact no-op and is skipped.
+    // Without physics options sth = 0 (:340) stays zero (sources' inviscid
+    // branch only fills it) and so does sq: the scalar update of :601-603
+    // (q += sq*dt, th += sth*dt) is an exact no-op and is skipped.
+    if (m->phys_on) {
+        // :340-343: sth = 0 and sq = 0 before the sources add to them; the
+        // means the damping layer and subsidence read (:433-435)
+        const long long n = c->geo.ntot;
+        hipLaunchKernelGGL(k_zero_field, dim3(2048), dim3(kBlock), 0, c->stream, m->sth, n);
+        if (m->sq)
+            hipLaunchKernelGGL(k_zero_field, dim3(2048), dim3(kBlock), 0, c->stream, m->sq,
+                               n * m->nq);
+        DCM_LAUNCH_CHECK();
+        if (m->phys.damping_layer) {
+            DCM_CHECK(level_means(c, cur[0], (double *)m->phys.ubar));
+            DCM_CHECK(level_means(c, cur[1], (double *)m->phys.vbar));
+        }
+        if (m->phys.divergence) DCM_CHECK(level_means(c, cur[2], (double *)m->phys.wbar));
+    }
     DCM_CHECK(level_means(c, m->th, m->thbar()));                  // :433-435
-    DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
-                          m->sv, m->sw, nullptr, m->psrc, nullptr, nullptr));          // :455-479
+    if (m->phys_on) {
+        SubgridFields sf;
+        sf.zu = prev[0];
+        sf.zv = prev[1];
+        sf.zw = prev[2];
+        sf.q = m->q;
+        sf.sq = m->sq;
+        sf.strain = m->strain;
+        sf.vism = m->vism;
+        sf.vist = m->vist;
+        sf.viss = m->viss;
+        DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
+                              m->sv, m->sw, m->sth, m->psrc, &m->phys, &sf));  // :455-479
+    } else {
+        DCM_CHECK(sources_dev(c, cur[0], cur[1], cur[2], m->th, m->theta(), m->thbar(), m->su,
+                              m->sv, m->sw, nullptr, m->psrc, nullptr, nullptr));  // :455-479
+    }
     DCM_CHECK(fill<double>(c, m->psrc, FILL_ALONG_DIM_WO, 0.0, 0.0)); // :485-487
     return DCM_OK;
 }
@@ -815,6 +949,15 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
     }
     DCM_CHECK(advance_momentum_dev(c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
                                    prev[2], m->su, m->sv, m->sw, m->p));  // :592-594
+    if (m->phys_on) {  // advance_scalar_fields_3d, :601-603
+        const long long n = g.ntot;
+        hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->th, m->sth,
+                           m->dt, n);
+        if (m->sq && m->nq > 0)
+            hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->q, m->sq,
+                               m->dt, n * m->nq);
+        DCM_LAUNCH_CHECK();
+    }
     {
         int tok = prof_begin(c, "filter");
         hipLaunchKernelGGL(k_filter<false>, dim3(4096), dim3(kBlock), 0, c->stream, cur[0],

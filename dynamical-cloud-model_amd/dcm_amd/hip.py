@@ -109,6 +109,7 @@ _SIGS = {
     "dcm_model_create": ([_vp, _i, _i, _ip, _ip, _d, _i, _i, _i, _d, _dp, _dp, _dp, _dp, _dp,
                           C.POINTER(_vp)], _i),
     "dcm_model_destroy": ([_vp], _i),
+    "dcm_model_set_physics": ([_vp, C.POINTER(PhysicsStruct)], _i),
     "dcm_model_set_precision": ([_vp, _i], _i),
     "dcm_model_set": ([_vp, _i, _i, _dp], _i),
     "dcm_model_get": ([_vp, _i, _i, _dp], _i),
@@ -401,6 +402,8 @@ class Model:
               *[_dptr(a) for a in self._keep], C.byref(h))
         self.h = h
         self.set_precision(precision)
+        if getattr(case, "physics", None) is not None:
+            self.set_physics(case.physics)
         self.set(F_U, case.u)
         self.set(F_V, case.v)
         self.set(F_W, case.w)
@@ -408,6 +411,14 @@ class Model:
         for n in range(case.nq):
             self.set(F_Q, case.q[n], n)
         _call("dcm_model_init", h)
+
+    def set_physics(self, ph):
+        """Physics options of the sources (a workloads.Physics; None: inviscid)."""
+        if ph is None:
+            _call("dcm_model_set_physics", self.h, None)
+            return
+        st, keep = physics_struct(self.case.dom, ph)
+        _call("dcm_model_set_physics", self.h, C.byref(st))
 
     def set_precision(self, precision):
         _call("dcm_model_set_precision", self.h, precision)

@@ -276,6 +276,12 @@ int dcm_model_create(dcm_ctx *ctx, int nq, int ncat, const int *c_s,
                      const double *thetan, const double *xg, const double *yg,
                      const double *zg, dcm_model **out);
 int dcm_model_destroy(dcm_model *m);
+/* Physics options of the model's sources (see dcm_physics); NULL restores
+ * the inviscid branch.  The profiles are copied; the model then zeroes sth
+ * and sq each step (driver_code.f90:340-343), computes the horizontal means
+ * the damping layer and subsidence read, and applies the scalar update
+ * q += sq*dt, th += sth*dt (:601-603). */
+int dcm_model_set_physics(dcm_model *m, const dcm_physics *ph);
 int dcm_model_set_precision(dcm_model *m, int precision);
 /* host <-> device copies of one model field (index: q field, else 0) */
 int dcm_model_set(dcm_model *m, int field, int index, const double *host);

@@ -222,6 +222,58 @@ def sedimentation(H, W, calls=20, cpu=True):
     return out
 
 
+def physics(H, W, steps, warmup=2, base=None):
+    """The sheared moist bubble with every option of sources but radiation
+    (subgrid viscosity and surface layer, damping layer, forcing,
+    subsidence) on the 256x256x128 grid: model steps in both precisions, and
+    the subgrid stage alone (calculate_subgrid_3d, sgm/subgrid_3d.f90:38-357)
+    timed through dcm_subgrid.  Canonical bytes of the subgrid stage per
+    cell: read u, v, w, zu, zv, zw, th and the nq fields of q; read and
+    write su, sv, sw, sth and the nq fields of sq; write strain, vism, vist
+    and the nq fields of viss: 18 + 4 nq words."""
+    case = W.sheared_moist_bubble(256, 256, 128)
+    cells = case.dom.cells
+    out = {"workload": case.name, "options": "viscous, damping_layer, forcing, divergence",
+           "fields": 1 + case.nq}
+    for pth, prec in (("mixed", H.MIXED), ("fp64", H.FP64)):
+        res, _, handles = run_path(H, case, prec, warmup, steps)
+        handles[1].close()
+        handles[0].close()
+        out[pth] = {k: res[k] for k in ("value", "ms_per_step", "iters_per_step",
+                                        "max_true_resid")}
+        if base and pth in base:
+            out[pth]["vs_inviscid_step"] = res["ms_per_step"] / base[pth]["ms_per_step"]
+        print(f"[bench] physics {pth}: {res['value']:.4g} cell-steps/s, "
+              f"{res['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    nq = case.nq
+    dom = case.dom
+    with H.Context(dom) as c:
+        flds = {n: c.array(a) for n, a in (("u", case.u), ("v", case.v), ("w", case.w),
+                                           ("th", case.th))}
+        for n in ("zu", "zv", "zw"):
+            flds[n] = c.array(flds[n[1]].get())
+        q = c.array(case.q)
+        one = [c.zeros() for _ in range(7)]
+        many = [c.zeros((nq,) + dom.shape) for _ in range(2)]
+        vism, vist, strain, su, sv, sw, sth = one
+        viss, sq = many
+        args = (case.physics, flds["u"], flds["v"], flds["w"], flds["zu"], flds["zv"],
+                flds["zw"], flds["th"], q, nq, vism, vist, viss, strain, su, sv, sw, sth, sq)
+        c.subgrid(*args)
+        c.sync()
+        calls = 10
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            c.subgrid(*args)
+        c.sync()
+        wall = (time.perf_counter() - t0) / calls
+    canonical = 8.0 * (18 + 4 * nq) * cells
+    out["subgrid"] = {"call": f"dcm_subgrid, nq {nq}", "ms_per_call": 1e3 * wall,
+                      "canonical_GBps": canonical / wall / 1e9,
+                      "frac_of_8TBps": canonical / wall / 8e12}
+    return out
+
+
 def ensemble_line(H, W, args):
     """One JSON line for the 64-member ensemble alone: value = the batched
     mixed path's cell-steps/s (graphs on), the roofline from a second, event
@@ -316,6 +368,9 @@ def main(argv=None):
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10,
                     help="timed steps of the 64-member ensemble comparison (0: skip)")
+    ap.add_argument("--physics-steps", type=int, default=10,
+                    help="timed steps of the sheared moist bubble with the options of "
+                         "sources on (0: skip)")
     ap.add_argument("--workload", choices=("moist_bubble", "ensemble"), default="moist_bubble",
                     help="ensemble: only the 64-member uneven ensemble, batched mixed vs "
                          "sequential FP64, --steps timed steps")
@@ -411,6 +466,10 @@ def main(argv=None):
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
     print("[bench] sedimentation (vertical MPDATA) 256x256x128", file=sys.stderr, flush=True)
     line["sedimentation"] = sedimentation(H, W, cpu=not args.no_cpu)
+    if args.physics_steps > 0:
+        print(f"[bench] physics options: sheared moist bubble 256x256x128, "
+              f"{args.physics_steps} steps", file=sys.stderr, flush=True)
+        line["physics"] = physics(H, W, args.physics_steps, base=out)
     if args.ensemble_steps > 0:
         print(f"[bench] ensemble: 64 uneven members, {args.ensemble_steps} steps",
               file=sys.stderr, flush=True)

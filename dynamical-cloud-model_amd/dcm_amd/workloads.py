@@ -301,6 +301,30 @@ def moist_bubble(ip=256, jp=256, kp=128, delta=50.0, dt=0.5, steps=200,
                 u=zero.copy(), v=zero.copy(), w=zero.copy(), th=th, q=q)
 
 
+def sheared_moist_bubble(ip=256, jp=256, kp=128, delta=50.0, dt=0.5, steps=200, ntracer=8,
+                         seed=71, damping=True, forcing=True, subsidence=True, name=None,
+                         ptol=1e-10) -> Case:
+    """The moist bubble in a linearly sheared wind (u = 2 + 0.004 z, v = -u/2)
+    with the subgrid model on, a 300 s damping layer over the top quarter,
+    600 s relaxation towards the initial column (1,1) and subsidence of
+    2e-4 1/s below mid-depth, as the options of the reference's sources
+    (dynamics.f90:639-1000); the surface roughness lengths are 0.05 m
+    (momentum) and 0.01 m (heat)."""
+    case = moist_bubble(ip, jp, kp, delta=delta, dt=dt, steps=steps, ntracer=ntracer,
+                        seed=seed, ptol=ptol,
+                        name=name or f"sheared_moist_bubble_{ip}x{jp}x{kp}")
+    dom = case.dom
+    shear = 2.0 + 0.004 * dom.zn
+    case.u = case.u + shear[None, None, :]
+    case.v = case.v - 0.5 * shear[None, None, :]
+    case.physics = physics_profiles(dom, u=case.u, v=case.v, z0=0.05, z0th=0.01,
+                                    damping_layer=damping, damping_tau=300.0,
+                                    forcing=forcing, forcing_tau=600.0,
+                                    divergence=subsidence, divergence_val=2e-4,
+                                    divergence_hgt=0.5 * dom.z[kp])
+    return case
+
+
 def kelvin_helmholtz(ip=512, jp=4, kp=256, delta=10.0, dt=0.2, steps=1000,
                      seed=303, name=None, ptol=1e-10) -> Case:
     """Shear layer u=10 tanh((z-1280)/50), stable theta plus a 0.5 K jump."""

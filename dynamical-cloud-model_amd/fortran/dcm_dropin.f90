@@ -383,9 +383,10 @@ contains
         call dcm_take(ctx, 15, c_loc(b), nb)
     end subroutine bicgstab
 
-    !> sources (dynamics.f90:639-1000), the inviscid branch: momentum
-    !> advection, buoyancy, halo fills, boundary rows and the Poisson
-    !> right-hand side.  sq, strain, vism, vist and viss are left as passed.
+    !> sources (dynamics.f90:639-1000): momentum advection, buoyancy, the
+    !> damping layer, forcing, subsidence and the subgrid model
+    !> (calculate_subgrid_3d) as the flags ask, halo fills, boundary rows and
+    !> the Poisson right-hand side.  Radiation is not on the GPU path.
     subroutine sources(comm3d, id, rank, dims, coords, dt, xg, yg, zg, zng, dx, dy, dz, dxn, &
             dyn, dzn, ip, jp, kp, l_h, r_h, nq, ubar, vbar, wbar, thbar, qbar, dampfacn, &
             dampfac, u_force, v_force, forcing_tau, w_subs, zu, zv, zw, u, v, w, su, sv, sw, &
@@ -413,14 +414,15 @@ contains
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
             intent(inout), target :: rhs, sth, strain, vism, vist
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h, 1:nq), &
-            intent(in) :: q
+            intent(in), target :: q
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h, 1:nq), &
-            intent(inout) :: sq, viss
+            intent(inout), target :: sq, viss
         real(dcm_wp), dimension(-l_h+1:ip+r_h), intent(in) :: xg, dx, dxn
         real(dcm_wp), dimension(-l_h+1:jp+r_h), intent(in) :: yg, dy, dyn
-        real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in) :: zg, zng, dz, dzn, theta, &
-            thetan, tref, trefn, rhoa, rhoan, lamsq, lamsqn, ubar, vbar, wbar, thbar, &
-            dampfacn, dampfac, u_force, v_force, w_subs
+        real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in) :: zng, dz, dzn, tref, trefn, &
+            rhoa, rhoan, lamsqn, ubar, vbar, wbar, thbar
+        real(dcm_wp), dimension(-l_h+1:kp+r_h), intent(in), target :: zg, theta, thetan, &
+            lamsq, dampfacn, dampfac, u_force, v_force, w_subs
         real(dcm_wp), dimension(-l_h+1:kp+r_h, nq), intent(in) :: qbar
         real(dcm_wp), dimension(-r_h+1:kp+r_h, -r_h+1:jp+r_h, -r_h+1:ip+r_h), &
             intent(in), target :: th
@@ -440,14 +442,16 @@ contains
             start_hour, start_min, start_sec, nrad
         real(dcm_wp), dimension(1-r_h:kp+r_h, 1-r_h:jp+r_h, 1-r_h:ip+r_h, 1:nrad), &
             intent(in) :: ngs, lamgs, mugs
-        type(c_ptr) :: ctx, du, dv, dw, dth, dsu, dsv, dsw, dsth, drhs
+        type(c_ptr) :: ctx, du, dv, dw, dth, dsu, dsv, dsw, dsth, drhs, dzu, dzv, dzw, dq, dsq, &
+            dstrain, dvism, dvist, dviss
+        type(dcm_physics) :: ph
         integer(c_size_t) :: nb
+        integer(c_int) :: nqm
 
         call need_halo1(l_h, r_h)
         call one_rank(dims)
-        if (viscous .or. damping_layer .or. forcing .or. divergence .or. radiation) then
-            print *, 'dcm: sources supports the inviscid branch only ', &
-                '(viscous, damping_layer, forcing, divergence and radiation off)'
+        if (radiation) then
+            print *, 'dcm: radiation is not on the GPU path'
             stop
         endif
         ctx = dcm_context(ip, jp, kp, dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan)
@@ -462,8 +466,55 @@ contains
         dsw = dcm_put(ctx, 10, c_loc(sw), nb)
         dsth = dcm_put(ctx, 11, c_loc(sth), nb)
         drhs = dcm_put(ctx, 12, c_loc(rhs), nb)
-        call dcm_check(dcm_sources(ctx, du, dv, dw, dth, theta, thbar, dsu, dsv, dsw, dsth, &
-            drhs), 'sources')
+        if (.not. (viscous .or. damping_layer .or. forcing .or. divergence)) then
+            call dcm_check(dcm_sources(ctx, du, dv, dw, dth, theta, thbar, dsu, dsv, dsw, dsth, &
+                drhs), 'sources')
+        else
+            ph%viscous = to_cint(viscous)
+            ph%damping_layer = to_cint(damping_layer)
+            ph%forcing = to_cint(forcing)
+            ph%divergence = to_cint(divergence)
+            ph%forcing_tau = forcing_tau
+            ph%z0 = z0
+            ph%z0th = z0th
+            ph%lamsq = c_loc(lamsq)
+            ph%dampfacn = c_loc(dampfacn)
+            ph%dampfac = c_loc(dampfac)
+            ph%u_force = c_loc(u_force)
+            ph%v_force = c_loc(v_force)
+            ph%w_subs = c_loc(w_subs)
+            ph%z = c_loc(zg)
+            ph%theta = c_loc(theta)
+            ph%thetan = c_loc(thetan)
+            nqm = 0
+            if (moisture) nqm = int(nq, c_int)
+            dzu = dcm_put(ctx, 4, c_loc(zu), nb)
+            dzv = dcm_put(ctx, 5, c_loc(zv), nb)
+            dzw = dcm_put(ctx, 6, c_loc(zw), nb)
+            dstrain = dcm_put(ctx, 19, c_loc(strain), nb)
+            dvism = dcm_put(ctx, 20, c_loc(vism), nb)
+            dvist = dcm_put(ctx, 21, c_loc(vist), nb)
+            dq = c_null_ptr
+            dsq = c_null_ptr
+            dviss = c_null_ptr
+            if (nqm > 0) then
+                dq = dcm_put(ctx, 17, c_loc(q), nb * nqm)
+                dsq = dcm_put(ctx, 18, c_loc(sq), nb * nqm)
+                dviss = dcm_put(ctx, 22, c_loc(viss), nb * nqm)
+            endif
+            call dcm_check(dcm_sources_phys(ctx, ph, ubar, vbar, wbar, du, dv, dw, dzu, dzv, &
+                dzw, dth, dq, nqm, theta, thbar, dsu, dsv, dsw, dsth, dsq, dstrain, dvism, &
+                dvist, dviss, drhs), 'sources')
+            if (viscous) then
+                call dcm_take(ctx, 19, c_loc(strain), nb)
+                call dcm_take(ctx, 20, c_loc(vism), nb)
+                call dcm_take(ctx, 21, c_loc(vist), nb)
+                if (nqm > 0) then
+                    call dcm_take(ctx, 18, c_loc(sq), nb * nqm)
+                    call dcm_take(ctx, 22, c_loc(viss), nb * nqm)
+                endif
+            endif
+        endif
         call dcm_take(ctx, 8, c_loc(su), nb)
         call dcm_take(ctx, 9, c_loc(sv), nb)
         call dcm_take(ctx, 10, c_loc(sw), nb)

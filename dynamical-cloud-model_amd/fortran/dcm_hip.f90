@@ -18,6 +18,17 @@ module dcm_hip
         DCM_F_PSRC = 8, DCM_F_SU = 9, DCM_F_SV = 10, DCM_F_SW = 11, DCM_F_STH = 12, &
         DCM_F_Q = 13, DCM_F_THBAR = 14
 
+    !> options of sources beyond the inviscid branch (C dcm_physics); the
+    !> pointers are c_loc of host arrays of kp+2, or c_null_ptr
+    type, bind(c), public :: dcm_physics
+        integer(c_int) :: viscous = 0, damping_layer = 0, forcing = 0, divergence = 0
+        real(c_double) :: forcing_tau = 1.0_c_double, z0 = 1.0e-4_c_double, &
+            z0th = 1.0e-4_c_double
+        type(c_ptr) :: lamsq = c_null_ptr, dampfacn = c_null_ptr, dampfac = c_null_ptr, &
+            u_force = c_null_ptr, v_force = c_null_ptr, w_subs = c_null_ptr, &
+            z = c_null_ptr, theta = c_null_ptr, thetan = c_null_ptr
+    end type dcm_physics
+
     type, bind(c), public :: dcm_step_stats
         integer(c_int) :: steps, iters_total, iters_max, outer_total, fallbacks, reserved
         real(c_double) :: last_err, max_true_resid
@@ -31,6 +42,7 @@ module dcm_hip
         dcm_fill_full, dcm_fill_along_dim, dcm_fill_along_dim_wo, dcm_fill_full_wo, &
         dcm_fill_along_z, dcm_upwind3d, dcm_mpdata3d, dcm_mpdata_vec3d, dcm_adv_ref_state, &
         dcm_mpdata3d_add, dcm_upwind_vert3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, &
+        dcm_subgrid, dcm_sources_phys, dcm_model_set_physics, &
         dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_bicgstab, &
         dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
@@ -143,6 +155,25 @@ module dcm_hip
             integer(c_int), value :: nq, kord, monotone, neumann
             real(c_double), intent(inout) :: lbc(*), ubc(*)
             integer(c_int), intent(out) :: early_exit
+        end function
+        integer(c_int) function dcm_subgrid(ctx, ph, u, v, w, zu, zv, zw, th, q, nq, vism, vist, &
+                viss, strain, su, sv, sw, sth, sq) bind(c, name="dcm_subgrid")
+            import :: c_int, c_ptr, dcm_physics
+            type(c_ptr), value :: ctx, u, v, w, zu, zv, zw, th, q
+            type(dcm_physics), intent(in) :: ph
+            integer(c_int), value :: nq
+            type(c_ptr), value :: vism, vist, viss, strain, su, sv, sw, sth, sq
+        end function
+        integer(c_int) function dcm_sources_phys(ctx, ph, ubar, vbar, wbar, u, v, w, zu, zv, zw, &
+                th, q, nq, theta, thbar, su, sv, sw, sth, sq, strain, vism, vist, viss, rhs) &
+                bind(c, name="dcm_sources_phys")
+            import :: c_int, c_ptr, c_double, dcm_physics
+            type(c_ptr), value :: ctx
+            type(dcm_physics), intent(in) :: ph
+            real(c_double), intent(in) :: ubar(*), vbar(*), wbar(*), theta(*), thbar(*)
+            type(c_ptr), value :: u, v, w, zu, zv, zw, th, q
+            integer(c_int), value :: nq
+            type(c_ptr), value :: su, sv, sw, sth, sq, strain, vism, vist, viss, rhs
         end function
         integer(c_int) function dcm_mpdata3d_add(ctx, dt, u, v, w, psi, psi_1d, lbc, ubc, kord, &
                 monotone, neumann, early_exit) bind(c, name="dcm_mpdata3d_add")
@@ -267,6 +298,11 @@ module dcm_hip
         integer(c_int) function dcm_model_init(model) bind(c, name="dcm_model_init")
             import :: c_int, c_ptr
             type(c_ptr), value :: model
+        end function
+        integer(c_int) function dcm_model_set_physics(model, ph) bind(c, name="dcm_model_set_physics")
+            import :: c_int, c_ptr, dcm_physics
+            type(c_ptr), value :: model
+            type(dcm_physics), intent(in) :: ph
         end function
         integer(c_int) function dcm_model_step(model, nsteps, stats) bind(c, name="dcm_model_step")
             import :: c_int, c_ptr, dcm_step_stats

@@ -5,13 +5,18 @@
 !> usage: dcm_dropin_check <input file> <output file>
 !> input (stream, native byte order): int32 ip, jp, kp, nq, kord, monotone;
 !> float64 dt, tol; 1-D dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, theta,
-!> thetan, xg, yg, zg (each n+2); padded fields u, v, w, th, q(nq), p.
+!> thetan, xg, yg, zg (each n+2); padded fields u, v, w, th, q(nq), p;
+!> float64 z0, z0th, forcing_tau; lamsq, dampfacn, dampfac, u_force, v_force,
+!> w_subs (each kp+2).
 !> output: thbar, qbar(nq), div, su, sv, sw, sth, rhs, p, u, v, w (advanced),
 !> th (after adv_ref_state and mpdata_3d), q (after mpdata_vec_3d),
 !> psi (first_order_upstream_3d of the input th), then on the input fields:
 !> thv (mpdata_vert_3d of th with w), qsed (mpdata_vec_vert_3d of q with w,
 !> as PAMM calls it: kord 1, no limiter, neumann 2), tha (mpdata_3d_add of th
-!> with the base-state profile theta).
+!> with the base-state profile theta), then sources with every option but
+!> radiation on (the advanced u, v, w as the current level, the input ones
+!> as the previous, the input th and q, the means of the input fields):
+!> su, sv, sw, sth, rhs, sq(nq), strain, vism, vist, viss(nq).
 program dropin_check
     use iso_c_binding
     use advection_s_3d, only: mpdata_3d, mpdata_vec_3d, first_order_upstream_3d, adv_ref_state, &
@@ -22,14 +27,16 @@ program dropin_check
     integer, parameter :: wp = c_double, i4b = c_int32_t
     integer(i4b) :: ip, jp, kp, nq, kord, imono, l_h, r_h
     integer(i4b) :: dims(3), coords(3), comm3d, id, rank
-    real(wp) :: dt, tol, lbc, ubc
+    real(wp) :: dt, tol, lbc, ubc, z0, z0th, forcing_tau
     real(wp), allocatable, dimension(:) :: dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, theta, &
-        thetan, xg, yg, zg, ubar, vbar, wbar, thbar, zero_k, lbcq, ubcq
+        thetan, xg, yg, zg, ubar, vbar, wbar, thbar, zero_k, lbcq, ubcq, lamsq, dampfacn, &
+        dampfac, u_force, v_force, w_subs
     real(wp), allocatable :: qbar(:, :)
     real(wp), allocatable, dimension(:, :, :) :: u, v, w, th, p, zu, zv, zw, su, sv, sw, sth, &
-        rhs, strain, vism, vist, div, psi, rad_power, thv, tha
+        rhs, strain, vism, vist, div, psi, rad_power, thv, tha, u0, v0, w0, th0, su2, sv2, &
+        sw2, sth2, rhs2, strain2, vism2, vist2
     real(wp), allocatable, dimension(:, :, :, :) :: q, sq, viss, flux_u, flux_d, ngs, lamgs, &
-        mugs, qsed
+        mugs, qsed, q0, sq2, viss2
     real(wp) :: rad1(1), mvrecv(1), tri(1)
     character(len=1024) :: fin, fout
     integer :: iu
@@ -55,11 +62,21 @@ program dropin_check
     ! (a MOLD must already be allocated: give u its shape first)
     allocate(u(0:kp+1, 0:jp+1, 0:ip+1))
     allocate(v, w, th, p, zu, zv, zw, su, sv, sw, sth, rhs, strain, vism, vist, div, psi, &
-        rad_power, thv, tha, mold=u)
+        rad_power, thv, tha, u0, v0, w0, th0, su2, sv2, sw2, sth2, rhs2, strain2, vism2, &
+        vist2, mold=u)
     allocate(q(0:kp+1, 0:jp+1, 0:ip+1, nq))
-    allocate(sq, viss, qsed, mold=q)
+    allocate(sq, viss, qsed, q0, sq2, viss2, mold=q)
     read(iu) u, v, w, th, q, p
+    allocate(lamsq(0:kp+1), dampfacn(0:kp+1), dampfac(0:kp+1), u_force(0:kp+1), &
+        v_force(0:kp+1), w_subs(0:kp+1))
+    read(iu) z0, z0th, forcing_tau
+    read(iu) lamsq, dampfacn, dampfac, u_force, v_force, w_subs
     close(iu)
+    u0 = u
+    v0 = v
+    w0 = w
+    th0 = th
+    q0 = q
     thv = th
     tha = th
     qsed = q
@@ -143,8 +160,29 @@ program dropin_check
     call mpdata_3d_add(dt, dx, dy, dz, dxn, dyn, dzn, rhoa, rhoan, ip, jp, kp, l_h, r_h, zu, zv, &
         zw, tha, theta, lbc, ubc, kord, monotone, 0, comm3d, id, dims, coords)
 
+    ! every option of sources but radiation (the means are of the input fields)
+    su2 = 0.0_wp
+    sv2 = 0.0_wp
+    sw2 = 0.0_wp
+    sth2 = 0.0_wp
+    rhs2 = 0.0_wp
+    sq2 = 0.0_wp
+    strain2 = 0.0_wp
+    vism2 = 0.0_wp
+    vist2 = 0.0_wp
+    viss2 = 0.0_wp
+    call sources(comm3d, id, rank, dims, coords, dt, xg, yg, zg, zg, dx, dy, dz, dxn, dyn, &
+        dzn, ip, jp, kp, l_h, r_h, nq, ubar, vbar, wbar, thbar, qbar, dampfacn, dampfac, &
+        u_force, v_force, forcing_tau, w_subs, u0, v0, w0, u, v, w, su2, sv2, sw2, q0, sq2, &
+        viss2, rhs2, th0, sth2, strain2, vism2, vist2, theta, thetan, theta, thetan, rhoa, &
+        rhoan, lamsq, lamsq, z0, z0th, .true., .true., .true., .true., .true., .false., 0, &
+        0.0_wp, 1, 1, tri, tri, tri, tri, tri, 1, 1, 1, flux_u, flux_d, rad_power, rad1, &
+        rad1, rad1, rad1, rad1, rad1, rad1, rad1, 2000, 1, 1, 0, 0, 0, 0.0_wp, 0.0_wp, &
+        0.0_wp, 0.0_wp, 0, 0.0_wp, 1, ngs, lamgs, mugs, 1, mvrecv)
+
     open(newunit=iu, file=trim(fout), access='stream', form='unformatted', status='replace')
     write(iu) thbar, qbar, div, su, sv, sw, sth, rhs, p, u, v, w, th, q, psi, thv, qsed, tha
+    write(iu) su2, sv2, sw2, sth2, rhs2, sq2, strain2, vism2, vist2, viss2
     close(iu)
 
 contains

@@ -195,3 +195,65 @@ int dcm_advance_momentum(void *ctx, double dt, double *u, double *v, double *w,
     or_advance_momentum(&c->g, dt, u, v, w, zu, zv, zw, su, sv, sw, p);
     return 0;
 }
+
+/* dcm_physics of include/dcm_hip.h, field for field */
+typedef struct {
+    int viscous, damping_layer, forcing, divergence;
+    double forcing_tau, z0, z0th;
+    const double *lamsq, *dampfacn, *dampfac, *u_force, *v_force, *w_subs;
+    const double *z, *theta, *thetan;
+} mock_physics;
+
+static or_physics to_or_physics(const mock_physics *ph, const double *ubar, const double *vbar,
+                                const double *wbar, int nq) {
+    or_physics o;
+    memset(&o, 0, sizeof(o));
+    o.viscous = ph->viscous;
+    o.moisture = nq > 0;
+    o.damping_layer = ph->damping_layer;
+    o.forcing = ph->forcing;
+    o.divergence = ph->divergence;
+    o.nq = nq;
+    o.forcing_tau = ph->forcing_tau;
+    o.z0 = ph->z0;
+    o.z0th = ph->z0th;
+    o.log_z1_z0 = log(ph->z[1] / ph->z0);
+    o.ubar = ubar;
+    o.vbar = vbar;
+    o.wbar = wbar;
+    o.dampfacn = ph->dampfacn;
+    o.dampfac = ph->dampfac;
+    o.u_force = ph->u_force;
+    o.v_force = ph->v_force;
+    o.w_subs = ph->w_subs;
+    o.lamsq = ph->lamsq;
+    o.z = ph->z;
+    o.theta = ph->theta;
+    o.thetan = ph->thetan;
+    return o;
+}
+
+int dcm_subgrid(void *ctx, const mock_physics *ph, const double *u, const double *v,
+                const double *w, const double *zu, const double *zv, const double *zw,
+                const double *th, const double *q, int nq, double *vism, double *vist,
+                double *viss, double *strain, double *su, double *sv, double *sw, double *sth,
+                double *sq) {
+    mock_ctx *c = ctx;
+    const or_physics o = to_or_physics(ph, NULL, NULL, NULL, nq);
+    or_subgrid(&c->g, &o, u, v, w, zu, zv, zw, th, q, vism, vist, viss, strain, su, sv, sw,
+               sth, sq);
+    return 0;
+}
+
+int dcm_sources_phys(void *ctx, const mock_physics *ph, const double *ubar, const double *vbar,
+                     const double *wbar, const double *u, const double *v, const double *w,
+                     const double *zu, const double *zv, const double *zw, const double *th,
+                     const double *q, int nq, const double *theta, const double *thbar,
+                     double *su, double *sv, double *sw, double *sth, double *sq,
+                     double *strain, double *vism, double *vist, double *viss, double *rhs) {
+    mock_ctx *c = ctx;
+    const or_physics o = to_or_physics(ph, ubar, vbar, wbar, nq);
+    or_sources_phys(&c->g, &o, u, v, w, zu, zv, zw, th, q, theta, thbar, su, sv, sw, sth, sq,
+                    strain, vism, vist, viss, rhs);
+    return 0;
+}

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from dcm_amd import workloads as W
 from helpers import field, small_domain
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,7 +37,7 @@ def test_fortran_library_exports_the_reference_modules():
     assert not missing, missing
 
 
-def _write_input(path, dom, nq, kord, monotone, dt, tol, fields):
+def _write_input(path, dom, nq, kord, monotone, dt, tol, fields, ph):
     with open(path, "wb") as f:
         np.array([dom.ip, dom.jp, dom.kp, nq, kord, int(monotone)], np.int32).tofile(f)
         np.array([dt, tol], np.float64).tofile(f)
@@ -45,9 +46,12 @@ def _write_input(path, dom, nq, kord, monotone, dt, tol, fields):
             np.ascontiguousarray(a, np.float64).tofile(f)
         for a in fields:
             np.ascontiguousarray(a, np.float64).tofile(f)
+        np.array([ph.z0, ph.z0th, ph.forcing_tau], np.float64).tofile(f)
+        for a in (ph.lamsq, ph.dampfacn, ph.dampfac, ph.u_force, ph.v_force, ph.w_subs):
+            np.ascontiguousarray(a, np.float64).tofile(f)
 
 
-def _oracle_sequence(dom, nq, kord, monotone, dt, tol, u, v, w, th, q, p):
+def _oracle_sequence(dom, nq, kord, monotone, dt, tol, phys, u, v, w, th, q, p):
     """The program's call sequence on the oracle."""
     g = O.Grid(dom)
     inputs = (u, v, w, th, q, p)
@@ -75,8 +79,19 @@ def _oracle_sequence(dom, nq, kord, monotone, dt, tol, u, v, w, th, q, p):
     O.mpdata_vert3d(g, dt, zw, thv, 0.0, 0.0, kord, monotone, 0)
     O.mpdata_vec_vert3d(g, dt, zw, qsed, np.zeros(nq), np.zeros(nq), 1, False, 2)
     O.mpdata3d_add(g, dt, zu, zv, zw, tha, dom.theta, 0.0, 0.0, kord, monotone, 0)
+    # every option of sources but radiation: the advanced fields as the
+    # current level, the inputs as the previous one
+    ph = O.Physics(dom, phys, nq=nq)
+    ph.ubar[:], ph.vbar[:], ph.wbar[:] = means[0], means[1], means[2]
+    f3 = lambda: np.zeros(dom.shape)
+    su2, sv2, sw2, sth2, rhs2, strain, vism, vist = (f3() for _ in range(8))
+    sq2, viss = np.zeros((nq,) + dom.shape), np.zeros((nq,) + dom.shape)
+    O.sources_phys(g, ph, u, v, w, inputs[0], inputs[1], inputs[2], th0, q0, dom.theta, thbar,
+                   su2, sv2, sw2, sth2, sq2, strain, vism, vist, viss, rhs2)
     return dict(thbar=thbar, qbar=qbar, div=div, su=su, sv=sv, sw=sw, sth=sth, rhs=rhs, p=p,
-                u=u, v=v, w=w, th=th, q=q, psi=psi, thv=thv, qsed=qsed, tha=tha), it
+                u=u, v=v, w=w, th=th, q=q, psi=psi, thv=thv, qsed=qsed, tha=tha, su2=su2,
+                sv2=sv2, sw2=sw2, sth2=sth2, rhs2=rhs2, sq2=sq2, strain=strain, vism=vism,
+                vist=vist, viss=viss), it
 
 
 def _read_output(path, dom, nq):
@@ -98,7 +113,17 @@ def _read_output(path, dom, nq):
     out["qsed"] = a[o:o + nq * n3].reshape((nq,) + dom.shape)
     o += nq * n3
     out["tha"] = a[o:o + n3].reshape(dom.shape)
-    assert o + n3 == a.size
+    o += n3
+    for name in ("su2", "sv2", "sw2", "sth2", "rhs2"):
+        out[name] = a[o:o + n3].reshape(dom.shape)
+        o += n3
+    out["sq2"] = a[o:o + nq * n3].reshape((nq,) + dom.shape)
+    o += nq * n3
+    for name in ("strain", "vism", "vist"):
+        out[name] = a[o:o + n3].reshape(dom.shape)
+        o += n3
+    out["viss"] = a[o:o + nq * n3].reshape((nq,) + dom.shape)
+    assert o + nq * n3 == a.size
     return out
 
 
@@ -118,24 +143,33 @@ def _case():
         O.fill_full(g, a, 0.0, 0.0)
     q = np.stack([field(dom, 304, 0.0, 1.0), field(dom, 305, 0.0, 0.1)])
     p = field(dom, 306, -0.1, 0.1)
-    return dom, (u, v, w, th, q, p)
+    # every option of sources on: the damping layer over the top third,
+    # forcing towards column (1,1), subsidence below mid-depth
+    phys = W.physics_profiles(dom, u=u + 1.0, v=v - 0.5, viscous=True, damping_layer=True,
+                              damping_thickness=dom.zn[dom.kp] / 3.0, damping_tau=300.0,
+                              forcing=True, forcing_tau=600.0, divergence=True,
+                              divergence_val=2e-4, divergence_hgt=float(dom.z[dom.kp // 2]))
+    return dom, (u, v, w, th, q, p), phys
 
 
 def _run_check(tmp_path, monotone, env):
-    dom, fields = _case()
+    dom, fields, phys = _case()
     nq, kord, dt, tol = 2, 2, 0.5, 1e-10
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_input(fin, dom, nq, kord, monotone, dt, tol, fields)
+    _write_input(fin, dom, nq, kord, monotone, dt, tol, fields, phys)
     print(f"dcm_dropin_check 12x10x9 monotone={monotone}", flush=True)
     r = subprocess.run([CHECK, str(fin), str(fout)], capture_output=True, text=True,
                        timeout=120, env=dict(os.environ, **env))
     # a Fortran stop exits with status 0, so the output file is the signal
     assert r.returncode == 0 and fout.exists(), r.stdout + r.stderr
     got = _read_output(fout, dom, nq)
-    exp, _ = _oracle_sequence(dom, nq, kord, monotone, dt, tol, *fields)
+    exp, _ = _oracle_sequence(dom, nq, kord, monotone, dt, tol, phys, *fields)
     for name in ("thbar", "qbar", "div", "su", "sv", "sw", "sth", "rhs", "psi", "p", "u", "v",
-                 "w", "th", "q", "thv", "qsed", "tha"):
+                 "w", "th", "q", "thv", "qsed", "tha", "su2", "sv2", "sw2", "sth2", "rhs2",
+                 "sq2", "strain", "vism", "vist", "viss"):
         np.testing.assert_array_equal(got[name], exp[name], err_msg=name)
+    # the options changed what sources returns
+    assert not np.array_equal(got["su2"], got["su"]) and np.abs(got["vism"]).max() > 0
 
 
 @pytest.mark.skipif(not (os.path.exists(CHECK) and os.path.exists(MOCK)),

@@ -22,20 +22,10 @@ def _oracle_built():
 
 def sheared_bubble(ip, jp, kp, steps=3, ntracer=2, seed=71, damping=True, forcing=True,
                    subsidence=True):
-    """A moist bubble in a linearly sheared wind with every option of
-    sources on (the reference's namelist.in sets viscous and damping)."""
-    case = W.moist_bubble(ip, jp, kp, delta=50.0, dt=0.5, steps=steps, ntracer=ntracer,
-                          seed=seed)
-    dom = case.dom
-    shear = 2.0 + 0.004 * dom.zn
-    case.u = case.u + shear[None, None, :]
-    case.v = case.v - 0.5 * shear[None, None, :]
-    case.physics = W.physics_profiles(dom, u=case.u, v=case.v, z0=0.05, z0th=0.01,
-                                      damping_layer=damping, damping_tau=300.0,
-                                      forcing=forcing, forcing_tau=600.0,
-                                      divergence=subsidence, divergence_val=2e-4,
-                                      divergence_hgt=0.5 * dom.z[kp])
-    return case
+    """A moist bubble in a sheared wind with every option of sources on (the
+    reference's namelist.in sets viscous and damping)."""
+    return W.sheared_moist_bubble(ip, jp, kp, steps=steps, ntracer=ntracer, seed=seed,
+                                  damping=damping, forcing=forcing, subsidence=subsidence)
 
 
 @pytest.mark.parametrize("shape", [(16, 12, 20), (12, 16, 9)])

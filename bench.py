@@ -274,6 +274,76 @@ def physics(H, W, steps, warmup=2, base=None):
     return out
 
 
+def output(H, W, steps=20, every=5):
+    """The output file (driver_code.f90:725-1145) from the resident mixed
+    model on the 256x256x128 moist bubble: two identical models step the
+    same `steps` steps (so the solver does the same work), one writing a
+    record every `every` steps; plus the pack kernel's time (a snapshot of
+    15 fields into one record in HBM: read 8 and write 4 bytes per cell and
+    field) and the writer thread's file rate.  The file goes to a temporary
+    directory that is removed afterwards."""
+    import shutil
+    import tempfile
+    case = W.moist_bubble(256, 256, 128)
+    cells, nf = case.dom.cells, 6 + case.nq
+    tmp = tempfile.mkdtemp(prefix="dcm_out_")
+
+    def timed(with_output):
+        with H.Context(case.dom) as ctx:
+            m = H.Model(ctx, case, H.MIXED)
+            res = {}
+            try:
+                m.step(2)
+                if not with_output:
+                    t0 = time.perf_counter()
+                    st = m.step(steps)
+                    res["loop"] = time.perf_counter() - t0
+                    res["iters"] = st.iters_total
+                    return res
+                with H.Output(m, os.path.join(tmp, "out.nc")) as out:
+                    out.write(0.0)  # first record: buffers and file warm
+                    out.flush()
+                    ctx.sync()
+                    t0 = time.perf_counter()
+                    out.write(0.0)
+                    ctx.sync()  # the pack kernel has run (the copy and write go on)
+                    res["pack"] = time.perf_counter() - t0
+                    out.flush()
+                    n0, stall0, wr0 = out.stats()
+                    iters = 0
+                    t0 = time.perf_counter()
+                    for s in range(steps):
+                        iters += m.step(1).iters_total
+                        if (s + 1) % every == 0:
+                            out.write(float(s + 1))
+                    res["loop"] = time.perf_counter() - t0
+                    out.flush()
+                    res["total"] = time.perf_counter() - t0
+                    n1, stall1, wr1 = out.stats()
+                    res.update(iters=iters, recs=n1 - n0, stall=stall1 - stall0, wr=wr1 - wr0)
+                return res
+            finally:
+                m.close()
+
+    try:
+        plain = timed(False)
+        wout = timed(True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    rec_bytes = 4.0 * nf * cells
+    recs = wout["recs"]
+    return {"workload": "moist_bubble_256x256x128, mixed", "record_MB": rec_bytes / 1e6,
+            "records": recs, "every_steps": every, "steps": steps,
+            "solver_iterations_equal": plain["iters"] == wout["iters"],
+            "ms_per_step_without_output": 1e3 * plain["loop"] / steps,
+            "ms_per_step_with_output": 1e3 * wout["loop"] / steps,
+            "ms_per_step_including_final_flush": 1e3 * wout["total"] / steps,
+            "host_stall_s": wout["stall"],
+            "pack_ms": 1e3 * wout["pack"], "pack_GBps": 12.0 * nf * cells / wout["pack"] / 1e9,
+            "writer_s_per_record": wout["wr"] / max(recs, 1),
+            "writer_GBps": rec_bytes * recs / max(wout["wr"], 1e-9) / 1e9}
+
+
 def ensemble_line(H, W, args):
     """One JSON line for the 64-member ensemble alone: value = the batched
     mixed path's cell-steps/s (graphs on), the roofline from a second, event
@@ -371,6 +441,8 @@ def main(argv=None):
     ap.add_argument("--physics-steps", type=int, default=10,
                     help="timed steps of the sheared moist bubble with the options of "
                          "sources on (0: skip)")
+    ap.add_argument("--output-steps", type=int, default=20,
+                    help="steps of the output-file leg (a record every 5; 0: skip)")
     ap.add_argument("--workload", choices=("moist_bubble", "ensemble"), default="moist_bubble",
                     help="ensemble: only the 64-member uneven ensemble, batched mixed vs "
                          "sequential FP64, --steps timed steps")
@@ -470,6 +542,10 @@ def main(argv=None):
         print(f"[bench] physics options: sheared moist bubble 256x256x128, "
               f"{args.physics_steps} steps", file=sys.stderr, flush=True)
         line["physics"] = physics(H, W, args.physics_steps, base=out)
+    if args.output_steps > 0:
+        print(f"[bench] output file: {args.output_steps} steps, a record every 5",
+              file=sys.stderr, flush=True)
+        line["output"] = output(H, W, args.output_steps)
     if args.ensemble_steps > 0:
         print(f"[bench] ensemble: 64 uneven members, {args.ensemble_steps} steps",
               file=sys.stderr, flush=True)

@@ -368,6 +368,12 @@ struct dcm_model {
     dcm::PhysicsDev phys;
     double *phys1d = nullptr;
     double *strain = nullptr, *vism = nullptr, *vist = nullptr, *viss = nullptr, *sq = nullptr;
+    // divergence_calc's diagnostic (driver_code.f90:443-444), kept once an
+    // output file is attached
+    double *div = nullptr;
+    // bumped whenever the work of a step changes (physics options, div), so
+    // that an ensemble does not replay graphs captured before the change
+    int gen = 0;
 
     double *theta() { return oned; }
     double *thetan() { return oned + (c->geo.kp + 2); }
@@ -397,6 +403,36 @@ static int model_alloc(dcm_model *m, size_t bytes, void **out) {
     *out = p;
     return DCM_OK;
 }
+
+namespace dcm {
+
+int model_view(dcm_model *m, ModelView *v) {
+    if (!m || !v) return DCM_BAD_ARG;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
+    v->c = m->c;
+    v->u = cur[0];
+    v->v = cur[1];
+    v->w = cur[2];
+    v->th = m->th;
+    v->p = m->p;
+    v->q = m->q;
+    v->nq = m->nq;
+    v->div = m->div;
+    return DCM_OK;
+}
+
+int model_enable_div(dcm_model *m) {
+    if (!m) return DCM_BAD_ARG;
+    if (m->div) return DCM_OK;
+    void *p = nullptr;
+    DCM_CHECK(model_alloc(m, (size_t)m->c->geo.ntot * sizeof(double), &p));  // div = 0 (:461)
+    m->div = (double *)p;
+    ++m->gen;
+    return DCM_OK;
+}
+
+}  // namespace dcm
 
 extern "C" {
 
@@ -655,6 +691,7 @@ int dcm_model_destroy(dcm_model *m) {
 int dcm_model_set_physics(dcm_model *m, const dcm_physics *ph) {
     if (!m) return DCM_BAD_ARG;
     DCM_HIP(hipStreamSynchronize(m->c->stream));
+    ++m->gen;
     if (!ph) {
         m->phys_on = false;
         return DCM_OK;
@@ -869,6 +906,12 @@ static int model_pre_solve(dcm_model *m) {
         if (m->phys.divergence) DCM_CHECK(level_means(c, cur[2], (double *)m->phys.wbar));
     }
     DCM_CHECK(level_means(c, m->th, m->thbar()));                  // :433-435
+    if (m->div) {                                                   // :443-444
+        const Interior L = interior_launch(c->geo);
+        hipLaunchKernelGGL(k_divergence, L.grid, L.block, 0, c->stream, cur[0], cur[1], cur[2],
+                           m->div, c->geo, c->met);
+        DCM_LAUNCH_CHECK();
+    }
     if (m->phys_on) {
         SubgridFields sf;
         sf.zu = prev[0];
@@ -1328,7 +1371,8 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
         for (int q = 0; q < n; ++q) {
             // what the captured kernels and their arguments depend on
             const dcm_model *m = e->mem[q];
-            key[q] = m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0);
+            key[q] = m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
+                     12 * m->gen;
             graphs = graphs && !m->c->profiling;
         }
         if (graphs)

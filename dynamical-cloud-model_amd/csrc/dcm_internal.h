@@ -216,6 +216,18 @@ inline void launch_timed(dcm_ctx *c, const char *name, F kernel, dim3 grid, dim3
 int prof_collect(dcm_ctx *c);
 
 // ---------------------------------------------------------------------------
+// resident model fields, for the output writer (dcm_dynamics.hip)
+// ---------------------------------------------------------------------------
+struct ModelView {
+    dcm_ctx *c;
+    const double *u, *v, *w, *th, *p, *q, *div;  // current momentum level
+    int nq;
+};
+int model_view(dcm_model *m, ModelView *v);
+// allocate the model's divergence diagnostic (zero) and compute it each step
+int model_enable_div(dcm_model *m);
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;

@@ -127,6 +127,16 @@ _SIGS = {
     "dcm_ensemble_autotune": ([_vp, _ip], _i),
     "dcm_ensemble_step": ([_vp, _i, C.POINTER(StepStats)], _i),
     "dcm_ensemble_set_graphs": ([_vp, _i], _i),
+    "dcm_nc_create": ([C.c_char_p, _i, _i, _i, _i, _i, _i, C.c_char_p, _dp, _dp, _dp, _dp, _dp,
+                       _dp, C.POINTER(_vp)], _i),
+    "dcm_nc_put_record": ([_vp, _i, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp], _i),
+    "dcm_nc_close": ([_vp], _i),
+    "dcm_output_create": ([_vp, C.c_char_p, _i, _i, C.c_char_p, _dp, _dp, _dp, _dp, _dp, _dp,
+                           C.POINTER(_vp)], _i),
+    "dcm_output_write": ([_vp, _d, _dp], _i),
+    "dcm_output_flush": ([_vp], _i),
+    "dcm_output_stats": ([_vp, C.POINTER(C.c_longlong), _dp, _dp], _i),
+    "dcm_output_destroy": ([_vp], _i),
 }
 
 _lib = None
@@ -494,6 +504,100 @@ class Ensemble:
     def close(self):
         if self.h:
             _call("dcm_ensemble_destroy", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def q_names_bytes(names, nq):
+    """nq names as the reference's character(len=20) array (blank padded)."""
+    names = list(names or [])
+    names += [""] * (nq - len(names))
+    return b"".join(n.encode()[:20].ljust(20) for n in names[:nq])
+
+
+def _profiles(dom, trefn):
+    n = dom.kp + 2
+    t = np.zeros(n) if trefn is None else trefn
+    return [np.ascontiguousarray(a, dtype=np.float64) for a in (dom.x, dom.y, dom.z, dom.rhoa,
+                                                                dom.thetan, t)]
+
+
+class NcFile:
+    """The reference's output file (driver_code.f90:725-1145) written from
+    host arrays in the padded layout: dcm_nc_create / dcm_nc_put_record.
+    trefn is the reference temperature profile (kp+2; zeros if not given)."""
+
+    def __init__(self, path, dom, moisture=False, nq=0, nprec=0, q_names=None, trefn=None):
+        self.dom = dom
+        self.moisture = bool(moisture)
+        self._keep = _profiles(dom, trefn)
+        names = q_names_bytes(q_names, nq) if moisture else None
+        h = _vp()
+        _call("dcm_nc_create", os.fsencode(path), dom.ip, dom.jp, dom.kp, int(self.moisture), nq,
+              nprec, names, *[_dptr(a) for a in self._keep], C.byref(h))
+        self.h = h
+
+    def put_record(self, n, time, u, v, w, th, p, div, q=None, precip=None):
+        """Record n (1-based); q is (nq, *shape), precip (nprec, ip+2, jp+2, kp)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (u, v, w, th, p, div, q, precip)]
+        _call("dcm_nc_put_record", self.h, n, time,
+              *[_dptr(a) if a is not None else None for a in arrs])
+
+    def close(self):
+        if self.h:
+            _call("dcm_nc_close", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Output:
+    """dcm_output: the output file written asynchronously from a resident
+    model (pack on the GPU, copy on a side stream, file write on a host
+    thread).  Close it before its model.  With moisture, nprec defaults to
+    one precipitation field (written as NetCDF's fill value unless write()
+    is given precip)."""
+
+    def __init__(self, model, path, moisture=None, nprec=None, q_names=None, trefn=None):
+        dom = model.case.dom
+        nq = model.case.nq
+        self.moisture = (nq > 0) if moisture is None else bool(moisture)
+        if nprec is None:
+            nprec = 1 if self.moisture else 0
+        self._keep = _profiles(dom, trefn)
+        names = q_names_bytes(q_names, nq) if self.moisture else None
+        h = _vp()
+        _call("dcm_output_create", model.h, os.fsencode(path), int(self.moisture), nprec, names,
+              *[_dptr(a) for a in self._keep], C.byref(h))
+        self.h = h
+
+    def write(self, time, precip=None):
+        pr = None if precip is None else np.ascontiguousarray(precip, dtype=np.float64)
+        self._precip = pr
+        _call("dcm_output_write", self.h, time, _dptr(pr) if pr is not None else None)
+
+    def flush(self):
+        _call("dcm_output_flush", self.h)
+
+    def stats(self):
+        """(records written, seconds waiting for a buffer, seconds in file writes)"""
+        n, stall, wr = C.c_longlong(), _d(), _d()
+        _call("dcm_output_stats", self.h, C.byref(n), C.byref(stall), C.byref(wr))
+        return n.value, stall.value, wr.value
+
+    def close(self):
+        if self.h:
+            _call("dcm_output_destroy", self.h)
             self.h = None
 
     def __enter__(self):

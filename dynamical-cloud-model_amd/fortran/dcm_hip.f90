@@ -48,7 +48,9 @@ module dcm_hip
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
         dcm_model_get, dcm_model_init, dcm_model_step, &
         dcm_ensemble_create, dcm_ensemble_destroy, dcm_ensemble_autotune, dcm_ensemble_step, &
-        dcm_ensemble_set_graphs
+        dcm_ensemble_set_graphs, &
+        dcm_nc_create, dcm_nc_put_record, dcm_nc_close, dcm_output_create, dcm_output_write, &
+        dcm_output_flush, dcm_output_stats, dcm_output_destroy
 
     interface
         integer(c_int) function dcm_context_create(ip, jp, kp, dx, dy, dz, dxn, dyn, dzn, &
@@ -340,6 +342,61 @@ module dcm_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ens
             integer(c_int), value :: enable
+        end function
+        ! the output file (driver_code.f90:725-1145); path is null-terminated
+        ! (trim(outputfile)//c_null_char), q_names the character(len=20)
+        ! array, precip c_loc of the (1:kp,0:jp+1,0:ip+1,nprec) array or
+        ! c_null_ptr
+        integer(c_int) function dcm_nc_create(path, ip, jp, kp, moisture, nq, nprec, q_names, &
+                x, y, z, rhoa, thetan, trefn, f) bind(c, name="dcm_nc_create")
+            import :: c_int, c_double, c_ptr, c_char
+            character(kind=c_char), intent(in) :: path(*), q_names(*)
+            integer(c_int), value :: ip, jp, kp, moisture, nq, nprec
+            real(c_double), intent(in) :: x(*), y(*), z(*), rhoa(*), thetan(*), trefn(*)
+            type(c_ptr), intent(out) :: f
+        end function
+        integer(c_int) function dcm_nc_put_record(f, n, time, u, v, w, th, p, div, q, precip) &
+                bind(c, name="dcm_nc_put_record")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: f
+            integer(c_int), value :: n
+            real(c_double), value :: time
+            real(c_double), intent(in) :: u(*), v(*), w(*), th(*), p(*), div(*), q(*)
+            type(c_ptr), value :: precip
+        end function
+        integer(c_int) function dcm_nc_close(f) bind(c, name="dcm_nc_close")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: f
+        end function
+        integer(c_int) function dcm_output_create(model, path, moisture, nprec, q_names, x, y, &
+                z, rhoa, thetan, trefn, o) bind(c, name="dcm_output_create")
+            import :: c_int, c_double, c_ptr, c_char
+            type(c_ptr), value :: model
+            character(kind=c_char), intent(in) :: path(*), q_names(*)
+            integer(c_int), value :: moisture, nprec
+            real(c_double), intent(in) :: x(*), y(*), z(*), rhoa(*), thetan(*), trefn(*)
+            type(c_ptr), intent(out) :: o
+        end function
+        integer(c_int) function dcm_output_write(o, time, precip) bind(c, name="dcm_output_write")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: o
+            real(c_double), value :: time
+            type(c_ptr), value :: precip
+        end function
+        integer(c_int) function dcm_output_flush(o) bind(c, name="dcm_output_flush")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: o
+        end function
+        integer(c_int) function dcm_output_stats(o, records, stall_s, write_s) &
+                bind(c, name="dcm_output_stats")
+            import :: c_int, c_double, c_ptr, c_long_long
+            type(c_ptr), value :: o
+            integer(c_long_long), intent(out) :: records
+            real(c_double), intent(out) :: stall_s, write_s
+        end function
+        integer(c_int) function dcm_output_destroy(o) bind(c, name="dcm_output_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: o
         end function
     end interface
 

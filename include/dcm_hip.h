@@ -38,7 +38,9 @@ typedef enum {
     DCM_BAD_ARG = 4,
     DCM_HIP_ERROR = 5,
     DCM_NO_DEVICE = 6,
-    DCM_OUT_OF_MEMORY = 7  /* the reference's "Not enough memory" stops      */
+    DCM_OUT_OF_MEMORY = 7, /* the reference's "Not enough memory" stops      */
+    DCM_IO_ERROR = 8       /* the output file could not be written (the
+                            * reference's check() stops on nf90 errors)     */
 } dcm_status;
 
 typedef enum { DCM_FP64 = 0, DCM_MIXED = 1 } dcm_precision;
@@ -330,6 +332,54 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats);
 /* Graph capture of the step's stages on (default) or off; off releases the
  * captured graphs. */
 int dcm_ensemble_set_graphs(dcm_ensemble *e, int enable);
+
+/* ---------------------------------------------------------------------- */
+/* f3. Output file (driver_code.f90:725-1145, `output`).                   */
+/* ---------------------------------------------------------------------- */
+/* The reference's NetCDF file, written in the classic format by this
+ * library: dims times (unlimited), ip, jp, kp [, nq, nprec, l_q_names];
+ * float variables time, x, y, z, rhoa, thetan, trefn, th, u, v, w, p, div
+ * [, q_names, q, precip] with the reference's names, dimension order and
+ * units.  x, y, z, rhoa, thetan, trefn are the reference's (0:n+1) host
+ * arrays (entries 1..n are written); q_names is nq blank-padded 20-character
+ * names (the Fortran character(len=20) array), or NULL for blanks. */
+typedef struct dcm_ncfile dcm_ncfile;
+/* Host writer, for callers whose fields are on the host: creates the file
+ * (NF90_CLOBBER, :788-964) and writes the fixed variables (:1018-1046). */
+int dcm_nc_create(const char *path, int ip, int jp, int kp, int moisture, int nq, int nprec,
+                  const char *q_names, const double *x, const double *y, const double *z,
+                  const double *rhoa, const double *thetan, const double *trefn,
+                  dcm_ncfile **out);
+/* Record n (1-based, the reference's `n`) from host arrays in the padded
+ * reference layout; q holds nq fields one after another, precip is
+ * (1:kp, 0:jp+1, 0:ip+1, nprec) or NULL (NetCDF's fill value). */
+int dcm_nc_put_record(dcm_ncfile *f, int n, double time, const double *u, const double *v,
+                      const double *w, const double *th, const double *p, const double *div,
+                      const double *q, const double *precip);
+int dcm_nc_close(dcm_ncfile *f);
+
+/* Asynchronous writer attached to a resident model.  dcm_output_write
+ * snapshots the model's fields on the model's stream (a pack kernel turns
+ * them into the record's big-endian floats in HBM, so the next step may
+ * start at once), copies the record to pinned host memory on a copy stream
+ * and returns; a host thread writes it to the file.  Two records may be in
+ * flight; a third call waits for the oldest.  div is the divergence of the
+ * momentum at the start of the previous step (:443-444), so the model
+ * computes it every step once an output is attached (zero before the first
+ * step).  q is written when moisture is set; precip is a host array as in
+ * dcm_nc_put_record, or NULL.  Destroy the output before its model. */
+typedef struct dcm_output dcm_output;
+int dcm_output_create(dcm_model *m, const char *path, int moisture, int nprec,
+                      const char *q_names, const double *x, const double *y, const double *z,
+                      const double *rhoa, const double *thetan, const double *trefn,
+                      dcm_output **out);
+int dcm_output_write(dcm_output *o, double time, const double *precip);
+/* wait until every record written so far is in the file */
+int dcm_output_flush(dcm_output *o);
+/* records written, the host time spent waiting for a free record buffer
+ * and the writer thread's time in file writes (seconds) */
+int dcm_output_stats(dcm_output *o, long long *records, double *stall_s, double *write_s);
+int dcm_output_destroy(dcm_output *o);
 
 /* ---------------------------------------------------------------------- */
 /* Measurement utility (no counterpart in the reference): a plain device   */

@@ -62,6 +62,42 @@ def physics_struct(dom, ph):
     return st, keep
 
 
+class InitConfig(C.Structure):
+    """dcm_init_config of include/dcm_hip.h, field for field."""
+
+    _fields_ = [("ip", _i), ("jp", _i), ("kp", _i),
+                ("dx", _d), ("dy", _d), ("dz", _d), ("dt", _d), ("runtime", _d),
+                ("moisture", _i), ("theta_flag", _i), ("nq", _i),
+                ("iqv", _i), ("iqc", _i), ("inc", _i),
+                ("drop_num", _d), ("psurf", _d), ("tsurf", _d),
+                ("adiabatic_prof", _i),
+                ("adiabatic_frac", _d), ("t_cbase", _d), ("t_ctop", _d), ("rh_above", _d),
+                ("th_jump", _d), ("th_grad", _d),
+                ("n_levels", _i),
+                ("z_read", _dp), ("theta_read", _dp), ("u_read", _dp), ("v_read", _dp),
+                ("q_read", _dp),
+                ("damping_layer", _i), ("damping_thickness", _d), ("damping_tau", _d),
+                ("forcing", _i), ("forcing_tau", _d),
+                ("divergence", _i), ("divergence_val", _d), ("divergence_hgt", _d),
+                ("cvis", _d), ("z0", _d), ("z0th", _d),
+                ("bubble", _i), ("param_theta", _i), ("param_wind", _i),
+                ("param_vmax", _d), ("param_z", _d), ("param_sigz", _d), ("param_delz", _d),
+                ("seed", C.c_ulonglong)]
+
+
+PROFILE_ARRAYS = ("x", "y", "z", "xn", "yn", "zn", "dx", "dy", "dz", "dxn", "dyn", "dzn",
+                   "theta", "thetan", "rhoa", "rhoan", "pref", "prefn", "tref", "trefn",
+                   "lamsq", "lamsqn", "dampfac", "dampfacn", "u_force", "v_force", "w_subs",
+                   "u", "v", "q")
+
+
+class InitProfiles(C.Structure):
+    """dcm_init_profiles of include/dcm_hip.h."""
+
+    _fields_ = [(n, _dp) for n in PROFILE_ARRAYS] + [("thbase", _d), ("thtop", _d),
+                                                      ("ntim", _i)]
+
+
 _SIGS = {
     "dcm_last_error": ([], C.c_char_p),
     "dcm_version": ([], C.c_char_p),
@@ -137,6 +173,16 @@ _SIGS = {
     "dcm_output_flush": ([_vp], _i),
     "dcm_output_stats": ([_vp, C.POINTER(C.c_longlong), _dp, _dp], _i),
     "dcm_output_destroy": ([_vp], _i),
+    "dcm_nml_read": ([C.c_char_p, C.POINTER(_vp)], _i),
+    "dcm_nml_free": ([_vp], _i),
+    "dcm_nml_has": ([_vp, C.c_char_p, C.c_char_p], _i),
+    "dcm_nml_get_real": ([_vp, C.c_char_p, C.c_char_p, _dp], _i),
+    "dcm_nml_get_int": ([_vp, C.c_char_p, C.c_char_p, C.POINTER(_i)], _i),
+    "dcm_nml_get_logical": ([_vp, C.c_char_p, C.c_char_p, C.POINTER(_i)], _i),
+    "dcm_nml_get_string": ([_vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t], _i),
+    "dcm_nml_get_array": ([_vp, C.c_char_p, C.c_char_p, _i, C.POINTER(_i), _dp], _i),
+    "dcm_init_compute_profiles": ([C.POINTER(InitConfig), C.POINTER(InitProfiles)], _i),
+    "dcm_model_init_fields": ([_vp, C.POINTER(InitConfig), C.POINTER(InitProfiles)], _i),
 }
 
 _lib = None
@@ -414,6 +460,12 @@ class Model:
         self.set_precision(precision)
         if getattr(case, "physics", None) is not None:
             self.set_physics(case.physics)
+        init = getattr(case, "init", None)
+        if init is not None:
+            # the initial state generated on the GPU (dcm_amd.init)
+            rc, pr = init
+            _call("dcm_model_init_fields", h, C.byref(rc.cfg), C.byref(pr["_struct"]))
+            return
         self.set(F_U, case.u)
         self.set(F_V, case.v)
         self.set(F_W, case.w)

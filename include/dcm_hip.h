@@ -334,6 +334,83 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats);
 int dcm_ensemble_set_graphs(dcm_ensemble *e, int enable);
 
 /* ---------------------------------------------------------------------- */
+/* f2. Namelist and initialisation (main.f90:74-140, initialisation.f90).  */
+/* ---------------------------------------------------------------------- */
+/* Fortran namelist files (`read(8,nml=run_vars)`, main.f90:77-78 and
+ * :137-138): groups, derived-type components (nm1%dt), subscripted
+ * sections (q_read(1,1:7)), repeat counts, logicals, character values.
+ * Names are case-insensitive; the last assignment wins.  Getters return
+ * DCM_BAD_ARG when the object is absent or not of the asked type. */
+typedef struct dcm_namelist dcm_namelist;
+int dcm_nml_read(const char *path, dcm_namelist **out);
+int dcm_nml_free(dcm_namelist *nl);
+/* 1 if the group (name NULL) or the object exists, else 0 */
+int dcm_nml_has(const dcm_namelist *nl, const char *group, const char *name);
+int dcm_nml_get_real(const dcm_namelist *nl, const char *group, const char *name, double *v);
+int dcm_nml_get_int(const dcm_namelist *nl, const char *group, const char *name, int *v);
+int dcm_nml_get_logical(const dcm_namelist *nl, const char *group, const char *name, int *v);
+int dcm_nml_get_string(const dcm_namelist *nl, const char *group, const char *name, char *buf,
+                       size_t n);
+/* an array of `rank` dimensions `shape` (Fortran order, column-major out):
+ * every assignment to the object is applied in order; elements no
+ * assignment touches keep what out held */
+int dcm_nml_get_array(const dcm_namelist *nl, const char *group, const char *name, int rank,
+                      const int *shape, double *out);
+
+/* The run configuration allocate_and_set reads (the nm1 components of
+ * variables.f90:59-93 it uses, and the sounding). */
+typedef struct dcm_init_config {
+    int ip, jp, kp;
+    double dx, dy, dz, dt, runtime;
+    int moisture, theta_flag, nq;
+    int iqv, iqc, inc;          /* 1-based q fields of vapour, cloud mass, cloud number; 0: none */
+    double drop_num;
+    double psurf, tsurf;
+    int adiabatic_prof;
+    double adiabatic_frac, t_cbase, t_ctop, rh_above, th_jump, th_grad;
+    int n_levels;
+    const double *z_read, *theta_read, *u_read, *v_read;  /* n_levels each */
+    const double *q_read;       /* q_read(nq, n_levels), Fortran order */
+    int damping_layer;
+    double damping_thickness, damping_tau;
+    int forcing;
+    double forcing_tau;
+    int divergence;
+    double divergence_val, divergence_hgt;
+    double cvis, z0, z0th;
+    int bubble, param_theta, param_wind;
+    double param_vmax, param_z, param_sigz, param_delz;
+    unsigned long long seed;    /* of the param_theta noise */
+} dcm_init_config;
+/* Profiles of allocate_and_set, in caller-allocated arrays: x, xn, dx, dxn
+ * of ip+2; y, yn, dy, dyn of jp+2; the others kp+2 (Fortran index 0..n+1);
+ * q nq*(kp+2), field n's column at q[(n-1)*(kp+2)]. */
+typedef struct dcm_init_profiles {
+    double *x, *y, *z, *xn, *yn, *zn, *dx, *dy, *dz, *dxn, *dyn, *dzn;
+    double *theta, *thetan, *rhoa, *rhoan, *pref, *prefn, *tref, *trefn;
+    double *lamsq, *lamsqn, *dampfac, *dampfacn, *u_force, *v_force, *w_subs;
+    double *u, *v;              /* the initial winds, the same in every column */
+    double *q;                  /* the initial moisture fields, the same in every column */
+    double thbase, thtop;       /* thetan(0), thetan(kp+1) (find_base_top) */
+    int ntim;                   /* ceiling(runtime/dt) */
+} dcm_init_profiles;
+/* The 1-D part of allocate_and_set (initialisation.f90:117-888) on one
+ * rank: grid, mixing length, damping layer, hydrostatic reference state
+ * (the sounding's or setup_adiabatic_profile's, :899-1311), winds, forcing,
+ * subsidence and moisture columns.  Host code; VODE and zeroin are
+ * replaced by the exact Exner integral where theta(z) is known and by
+ * RK4 plus Brent's method on the moist adiabat. */
+int dcm_init_compute_profiles(const dcm_init_config *cfg, dcm_init_profiles *prof);
+/* The 3-D initial state generated on the GPU from the profiles: u, v from
+ * the wind columns, w = 0, th the param_theta noise (a counter-based hash
+ * of the cell and cfg->seed replaces Fortran's random_number) or the warm
+ * bubble, q from the moisture columns; then the halos as dcm_model_init
+ * fills them.  The model must have been created on a context of the
+ * profiles' grid, with cfg->nq q fields when moisture is set. */
+int dcm_model_init_fields(dcm_model *m, const dcm_init_config *cfg,
+                          const dcm_init_profiles *prof);
+
+/* ---------------------------------------------------------------------- */
 /* f3. Output file (driver_code.f90:725-1145, `output`).                   */
 /* ---------------------------------------------------------------------- */
 /* The reference's NetCDF file, written in the classic format by this

@@ -119,3 +119,36 @@ def test_namelist_run_matches_oracle():
     host = dataclasses.replace(case, **fields)
     host.init = None
     _compare_models(host, 4, seq=True)
+
+
+def test_run_from_namelist_writes_the_output_file(tmp_path):
+    """The model_driver loop from namelist.in on a reduced grid: five steps
+    of 2.5 s with a record every 5 s (times 0, 5, 10, at the start of steps
+    1, 3, 5), the first record the initial state."""
+    from scipy.io import netcdf_file
+
+    from dcm_amd import run as R
+    over = {"jp": 24, "kp": 40, "output_interval": 5.0}
+    path = tmp_path / "run.nc"
+    res = R.run_namelist(NML, steps=5, output=str(path), overrides=over)
+    assert res["steps"] == 5 and res["records"] == 3
+    assert res["max_true_resid"] < 1e-8
+    case = I.case_from_namelist(NML, steps=1, overrides=over)
+    with H.Context(case.dom) as c:
+        m = H.Model(c, case, H.MIXED)
+        try:
+            th0, q0 = m.get(H.F_TH), [m.get(H.F_Q, n) for n in range(case.nq)]
+        finally:
+            m.close()
+    with netcdf_file(path, "r", mmap=False) as nc:
+        np.testing.assert_array_equal(nc.variables["time"][:], np.array([0, 5, 10], np.float32))
+        assert nc.variables["q"].shape == (3, 1, 24, 40, case.nq)
+        np.testing.assert_array_equal(nc.variables["th"][0],
+                                      th0[1:-1, 1:-1, 1:-1].astype(np.float32))
+        for n in range(case.nq):
+            np.testing.assert_array_equal(nc.variables["q"][0][..., n],
+                                          q0[n][1:-1, 1:-1, 1:-1].astype(np.float32))
+        for name in ("u", "v", "w", "th", "p", "div"):
+            assert np.isfinite(nc.variables[name][:]).all(), name
+        np.testing.assert_array_equal(nc.variables["trefn"][:],
+                                      case.trefn[1:-1].astype(np.float32))

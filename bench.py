@@ -344,6 +344,44 @@ def output(H, W, steps=20, every=5):
             "writer_GBps": rec_bytes * recs / max(wout["wr"], 1e-9) / 1e9}
 
 
+def initialisation(H, W):
+    """A run set up from the reference's namelist.in on the 256x256x128 grid
+    (dcm_amd.init): parsing and the 1-D profiles on the host, then the 3-D
+    initial state generated in HBM (dcm_model_init_fields), against the same
+    model created from host arrays of the same state (uploads)."""
+    import dataclasses
+    from dcm_amd import init as I
+    path = os.path.join(ROOT, "tests", "golden", "namelists", "namelist.in")
+    over = {"ip": 256, "jp": 256, "kp": 128}
+    t0 = time.perf_counter()
+    case = I.case_from_namelist(path, steps=1, overrides=over)
+    t_prof = time.perf_counter() - t0
+    with H.Context(case.dom) as ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        m = H.Model(ctx, case, H.MIXED)
+        ctx.sync()
+        t_dev = time.perf_counter() - t0
+        fields = {"u": m.get(H.F_U), "v": m.get(H.F_V), "w": m.get(H.F_W),
+                  "th": m.get(H.F_TH),
+                  "q": np.stack([m.get(H.F_Q, n) for n in range(case.nq)])}
+        m.close()
+    host = dataclasses.replace(case, **fields)
+    host.init = None
+    with H.Context(case.dom) as ctx:
+        ctx.sync()
+        t0 = time.perf_counter()
+        m = H.Model(ctx, host, H.MIXED)
+        ctx.sync()
+        t_host = time.perf_counter() - t0
+        m.close()
+    nf = 4 + case.nq
+    return {"namelist": "namelist.in (adiabatic profile, param_theta noise), grid 256x256x128",
+            "profiles_ms": 1e3 * t_prof, "model_create_device_init_ms": 1e3 * t_dev,
+            "model_create_from_host_arrays_ms": 1e3 * t_host,
+            "fields": nf, "field_MB": 8.0 * nf * case.dom.cells / 1e6}
+
+
 def ensemble_line(H, W, args):
     """One JSON line for the 64-member ensemble alone: value = the batched
     mixed path's cell-steps/s (graphs on), the roofline from a second, event
@@ -542,6 +580,8 @@ def main(argv=None):
         print(f"[bench] physics options: sheared moist bubble 256x256x128, "
               f"{args.physics_steps} steps", file=sys.stderr, flush=True)
         line["physics"] = physics(H, W, args.physics_steps, base=out)
+    print("[bench] initialisation from namelist.in", file=sys.stderr, flush=True)
+    line["initialisation"] = initialisation(H, W)
     if args.output_steps > 0:
         print(f"[bench] output file: {args.output_steps} steps, a record every 5",
               file=sys.stderr, flush=True)

@@ -166,6 +166,11 @@ struct dcm_ctx {
     // global sums in the reference's sequential loop order (one thread per
     // sum) instead of the two-level tree: bit-identical to a serial CPU run
     bool seq_sums = false;
+    // Chebyshev relaxation of the mixed inner solve's Jacobi preconditioner
+    // (set_chebyshev): the factors of its four sweeps, in order
+    bool cheb_on = false;
+    double cheb_a = 0.0;
+    double cheb_w[4] = {1.0, 1.0, 1.0, 1.0};
     // i planes per marching block of the solver (0: the built-in default)
     int march_planes = 0;
     // device tables of a batched solve and the members' flags (device and
@@ -214,6 +219,12 @@ inline void launch_timed(dcm_ctx *c, const char *name, F kernel, dim3 grid, dim3
         hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, args...);
 }
 int prof_collect(dcm_ctx *c);
+
+// ---------------------------------------------------------------------------
+// Chebyshev relaxation factors of the mixed inner solve's preconditioner on
+// the eigenvalue interval [a, 2] of D^-1 A (a = 0: the reference's Jacobi)
+// ---------------------------------------------------------------------------
+void set_chebyshev(dcm_ctx *c, double a);
 
 // ---------------------------------------------------------------------------
 // resident model fields, for the output writer (dcm_dynamics.hip)

@@ -537,12 +537,15 @@ template <typename R, int MODE> __device__ __forceinline__ R tv(R r, R v1, R a, 
 
 // First sweep, variant 0: fused in registers, s0 = t/a_p evaluated at the
 // centre and the six neighbours (dynamics.f90:254-266).
-template <typename R, int MODE>
+// W: Chebyshev relaxation, s0 = w0 t/a_p and s1 = s0 + w1 (t - A s0)/a_p
+// (otherwise the reference's w0 = w1 = 1, and the factors are not applied)
+template <typename R, int MODE, bool W = false>
 __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
                                                    gp<const R> __restrict__ v1,
                                                    gp<R> __restrict__ t_out, gp<R> __restrict__ s_out,
                                                    const MG &m, const CoefPack &cp,
-                                                   gp<const SolverScalars> __restrict__ sc, int tile) {
+                                                   gp<const SolverScalars> __restrict__ sc, int tile,
+                                                   R w0 = R(1), R w1 = R(1)) {
     if (halted(sc)) return;
     Tile T;
     if (!march_tile(m, tile, T) || !T.act) return;
@@ -562,9 +565,10 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
     int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
     int ce = (ie - 1) * m.si + q.base;
     const int cw = (iw - 1) * m.si + q.base;
-    R s0w = div_ap<R>(tv<R, MODE>(r[cw], v1[cw], a, bo), coef_ap(C, k, j, iw));
+    auto rw = [&](R x) { return W ? w0 * x : x; };
+    R s0w = rw(div_ap<R>(tv<R, MODE>(r[cw], v1[cw], a, bo), coef_ap(C, k, j, iw)));
     R tc = tv<R, MODE>(r[cc], v1[cc], a, bo);
-    R s0c = div_ap<R>(tc, coef_ap(C, k, j, T.i0));
+    R s0c = rw(div_ap<R>(tc, coef_ap(C, k, j, T.i0)));
     // r, v1 at the j and k neighbours of plane i, and at the centre of i+1
     R rs = r[cc + q.ds], vs = v1[cc + q.ds], rn = r[cc + q.dn], vn = v1[cc + q.dn];
     R rd = r[cc + q.dd], vd = v1[cc + q.dd], ru = r[cc + q.du], vu = v1[cc + q.du];
@@ -589,17 +593,18 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
             nve = v1[nce];
         }
         const R te = tv<R, MODE>(re, ve, a, bo);
-        const R s0e = div_ap<R>(te, coef_ap(C, k, j, ie));
+        const R s0e = rw(div_ap<R>(te, coef_ap(C, k, j, ie)));
         const R api = (-C.ae[i]) - C.aw[i];
         const R ap = coef_ap_i(C, api, k, j);
-        const R s0s = div_ap<R>(tv<R, MODE>(rs, vs, a, bo), coef_ap_i(C, api, k, js));
-        const R s0n = div_ap<R>(tv<R, MODE>(rn, vn, a, bo), coef_ap_i(C, api, k, jn));
-        const R s0d = div_ap<R>(tv<R, MODE>(rd, vd, a, bo), coef_ap_i(C, api, kd, j));
-        const R s0u = div_ap<R>(tv<R, MODE>(ru, vu, a, bo), coef_ap_i(C, api, ku, j));
+        const R s0s = rw(div_ap<R>(tv<R, MODE>(rs, vs, a, bo), coef_ap_i(C, api, k, js)));
+        const R s0n = rw(div_ap<R>(tv<R, MODE>(rn, vn, a, bo), coef_ap_i(C, api, k, jn)));
+        const R s0d = rw(div_ap<R>(tv<R, MODE>(rd, vd, a, bo), coef_ap_i(C, api, kd, j)));
+        const R s0u = rw(div_ap<R>(tv<R, MODE>(ru, vu, a, bo), coef_ap_i(C, api, ku, j)));
         const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, s0w, s0e, s0s, s0n, s0d,
                                 s0u, s0c);
         t_out[cc] = tc;
-        s_out[cc] = s0c + div_ap<R>(tc - As, ap);
+        const R d1 = div_ap<R>(tc - As, ap);
+        s_out[cc] = s0c + (W ? w1 * d1 : d1);
         s0w = s0c;
         s0c = s0e;
         tc = te;
@@ -619,23 +624,24 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
     }
 }
 
-template <typename R, int MODE>
+template <typename R, int MODE, bool W = false>
 __global__ __launch_bounds__(256) void k_jac_first_reg(const R *__restrict__ r,
                                                        const R *__restrict__ v1,
                                                        R *__restrict__ t_out,
                                                        R *__restrict__ s_out, MG m, CoefPack cp,
-                                                       const SolverScalars *__restrict__ sc) {
-    jac_first_reg_body<R, MODE>(glob(r), glob(v1), glob(t_out), glob(s_out), m, cp, glob(sc),
-                                march_tile_id(m, (int)blockIdx.x));
+                                                       const SolverScalars *__restrict__ sc,
+                                                       R w0 = R(1), R w1 = R(1)) {
+    jac_first_reg_body<R, MODE, W>(glob(r), glob(v1), glob(t_out), glob(s_out), m, cp, glob(sc),
+                                   march_tile_id(m, (int)blockIdx.x), w0, w1);
 }
-template <typename R, int MODE>
+template <typename R, int MODE, bool W = false>
 __global__ __launch_bounds__(256) void k_jac_first_reg_b(IterArgs A, int ri, int vi, int ti,
-                                                         int oi) {
+                                                         int oi, R w0 = R(1), R w1 = R(1)) {
     int tile;
     const IterMember &M = member_m(A, tile);
-    jac_first_reg_body<R, MODE>(glob((const R *)M.V[ri]), glob((const R *)M.V[vi]),
-                                glob((R *)M.V[ti]), glob((R *)M.V[oi]), M.m, M.cp,
-                                glob((const SolverScalars *)M.sc), tile);
+    jac_first_reg_body<R, MODE, W>(glob((const R *)M.V[ri]), glob((const R *)M.V[vi]),
+                                   glob((R *)M.V[ti]), glob((R *)M.V[oi]), M.m, M.cp,
+                                   glob((const SolverScalars *)M.sc), tile, w0, w1);
 }
 
 // First sweep, variant 2: a streaming pass t = r - bet*(omg*v1) (or
@@ -681,10 +687,11 @@ __global__ __launch_bounds__(256) void k_jac_prep_b(IterArgs A, int ri, int vi, 
 
 // s_out = s + (t - A s)/a_p; with P_UPDATE: p = s_out + bet*p instead.  The
 // loads of plane i+1 are issued while plane i is computed.
-template <typename R, bool P_UPDATE>
+template <typename R, bool P_UPDATE, bool W = false>
 __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R> __restrict__ t,
                                          gp<R> __restrict__ s_out, const MG &m, const CoefPack &cp,
-                                         gp<const SolverScalars> __restrict__ sc, int tile) {
+                                         gp<const SolverScalars> __restrict__ sc, int tile,
+                                         R w = R(1)) {
     if (halted(sc)) return;
     Tile T;
     if (!march_tile(m, tile, T) || !T.act) return;
@@ -718,7 +725,8 @@ __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R>
         }
         const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
         const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu, xc);
-        const R snew = xc + div_ap<R>(tc - As, ap);
+        const R d = div_ap<R>(tc - As, ap);
+        const R snew = xc + (W ? w * d : d);
         if (P_UPDATE)
             s_out[cc] = snew + bet * po;  // p = s + bet*p (dynamics.f90:399)
         else
@@ -738,20 +746,20 @@ __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R>
     }
 }
 
-template <typename R, bool P_UPDATE>
+template <typename R, bool P_UPDATE, bool W = false>
 __global__ __launch_bounds__(256) void k_jac(const R *__restrict__ s, const R *__restrict__ t,
                                              R *__restrict__ s_out, MG m, CoefPack cp,
-                                             const SolverScalars *__restrict__ sc) {
-    jac_body<R, P_UPDATE>(glob(s), glob(t), glob(s_out), m, cp, glob(sc),
-                          march_tile_id(m, (int)blockIdx.x));
+                                             const SolverScalars *__restrict__ sc, R w = R(1)) {
+    jac_body<R, P_UPDATE, W>(glob(s), glob(t), glob(s_out), m, cp, glob(sc),
+                             march_tile_id(m, (int)blockIdx.x), w);
 }
-template <typename R, bool P_UPDATE>
-__global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int oi) {
+template <typename R, bool P_UPDATE, bool W = false>
+__global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int oi, R w = R(1)) {
     int tile;
     const IterMember &M = member_m(A, tile);
-    jac_body<R, P_UPDATE>(glob((const R *)M.V[si_]), glob((const R *)M.V[ti]),
-                          glob((R *)M.V[oi]), M.m, M.cp,
-                          glob((const SolverScalars *)M.sc), tile);
+    jac_body<R, P_UPDATE, W>(glob((const R *)M.V[si_]), glob((const R *)M.V[ti]),
+                             glob((R *)M.V[oi]), M.m, M.cp,
+                             glob((const SolverScalars *)M.sc), tile, w);
 }
 
 // y = A x; partial sums of <a,y> (and <y,y> first when TWO).  Loads of plane
@@ -1167,34 +1175,47 @@ struct IterLaunch {
     const IterMember &M() const { return P.mem[0]; }
     R *v(int i) const { return (R *)P.mem[0].V[i]; }
 
-    void jac(const char *name, int si, int ti, int oi) const {
+    // Chebyshev relaxation of the sweeps: only the FP32 inner solve, and only
+    // with the fused first sweep (the factors of sweeps 0..3 are c->cheb_w)
+    bool cheb() const { return sizeof(R) == 4 && c->cheb_on && DCM_JAC_FIRST32 == 0; }
+    template <bool PU, bool W> void jac_launch(const char *name, int si, int ti, int oi, R w) const {
         if (one())
-            launch_timed(c, name, k_jac<R, false>, dim3(P.mgrid), dim3(kBlock), (const R *)v(si),
-                         (const R *)v(ti), v(oi), M().m, M().cp, (const SolverScalars *)M().sc);
+            launch_timed(c, name, k_jac<R, PU, W>, dim3(P.mgrid), dim3(kBlock), (const R *)v(si),
+                         (const R *)v(ti), v(oi), M().m, M().cp, (const SolverScalars *)M().sc, w);
         else
-            launch_timed(c, name, k_jac_b<R, false>, dim3(P.mgrid), dim3(kBlock), P.args, si, ti,
-                         oi);
+            launch_timed(c, name, k_jac_b<R, PU, W>, dim3(P.mgrid), dim3(kBlock), P.args, si, ti,
+                         oi, w);
+    }
+    // a plain sweep; wi: which relaxation factor under Chebyshev
+    void jac(const char *name, int si, int ti, int oi, int wi) const {
+        if (cheb())
+            jac_launch<false, true>(name, si, ti, oi, (R)c->cheb_w[wi]);
+        else
+            jac_launch<false, false>(name, si, ti, oi, R(1));
     }
     void jac_p(int si, int ti, int oi) const {
-        if (one())
-            launch_timed(c, "jac_p", k_jac<R, true>, dim3(P.mgrid), dim3(kBlock),
-                         (const R *)v(si), (const R *)v(ti), v(oi), M().m, M().cp,
-                         (const SolverScalars *)M().sc);
+        if (cheb())
+            jac_launch<true, true>("jac_p", si, ti, oi, (R)c->cheb_w[3]);
         else
-            launch_timed(c, "jac_p", k_jac_b<R, true>, dim3(P.mgrid), dim3(kBlock), P.args, si,
-                         ti, oi);
+            jac_launch<true, false>("jac_p", si, ti, oi, R(1));
+    }
+    template <int MODE, bool W> void first_reg(int ti, R w0, R w1) const {
+        if (one())
+            hipLaunchKernelGGL((k_jac_first_reg<R, MODE, W>), dim3(P.mgrid), dim3(kBlock), 0,
+                               c->stream, (const R *)v(VR), (const R *)v(VV1), v(ti), v(VWA),
+                               M().m, M().cp, (const SolverScalars *)M().sc, w0, w1);
+        else
+            hipLaunchKernelGGL((k_jac_first_reg_b<R, MODE, W>), dim3(P.mgrid), dim3(kBlock), 0,
+                               c->stream, P.args, (int)VR, (int)VV1, ti, (int)VWA, w0, w1);
     }
     template <int MODE> void first(int ti) const {
         const int variant = sizeof(R) == 8 ? DCM_JAC_FIRST64 : DCM_JAC_FIRST32;
         const int tok = prof_begin(c, "jac_first");
         if (variant == 0) {
-            if (one())
-                hipLaunchKernelGGL((k_jac_first_reg<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
-                                   c->stream, (const R *)v(VR), (const R *)v(VV1), v(ti), v(VWA),
-                                   M().m, M().cp, (const SolverScalars *)M().sc);
+            if (cheb())
+                first_reg<MODE, true>(ti, (R)c->cheb_w[0], (R)c->cheb_w[1]);
             else
-                hipLaunchKernelGGL((k_jac_first_reg_b<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
-                                   c->stream, P.args, (int)VR, (int)VV1, ti, (int)VWA);
+                first_reg<MODE, false>(ti, R(1), R(1));
         } else {
             if (one())
                 hipLaunchKernelGGL((k_jac_prep<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
@@ -1203,7 +1224,7 @@ struct IterLaunch {
             else
                 hipLaunchKernelGGL((k_jac_prep_b<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0,
                                    c->stream, P.args, (int)VR, (int)VV1, ti, (int)VWB);
-            jac("jac_first", VWB, ti, VWA);
+            jac("jac_first", VWB, ti, VWA, 1);
         }
         prof_end(c, tok);
     }
@@ -1258,13 +1279,13 @@ struct IterLaunch {
     // dynamics.f90:391-425
     void iteration() const {
         first<0>(VT);               // t = r - bet*(omg*v1), sweep 1 -> wa
-        jac("jac", VWA, VT, VWB);   // sweep 2 -> wb
+        jac("jac", VWA, VT, VWB, 2);  // sweep 2 -> wb
         jac_p(VWB, VT, VP);         // sweep 3, p = s + bet*p
         matvec<false>(VP, VCR, VV1);  // v1 = A p, <cr,v1>
         fin(0);
         first<1>(VS);               // s = r - alf*v1, sweep 1 -> wa
-        jac("jac", VWA, VS, VWB);
-        jac("jac", VWB, VS, VCS);   // cs = M s
+        jac("jac", VWA, VS, VWB, 2);
+        jac("jac", VWB, VS, VCS, 3);  // cs = M s
         matvec<true>(VCS, VS, VT);  // t = A cs, <t,t>, <t,s>
         fin(1);
         update();                   // x, r; <r,r>, <r,cr>
@@ -1756,6 +1777,12 @@ int dcm_bicgstab(dcm_ctx *c, const double *xg, const double *yg, const double *z
 int dcm_set_sum_order(dcm_ctx *c, int order) {
     if (!c || (order != DCM_SUM_TREE && order != DCM_SUM_SEQUENTIAL)) return DCM_BAD_ARG;
     c->seq_sums = order == DCM_SUM_SEQUENTIAL;
+    return DCM_OK;
+}
+
+int dcm_set_precond_chebyshev(dcm_ctx *c, double a) {
+    if (!c || !(a >= 0.0) || a >= 2.0) return DCM_BAD_ARG;
+    dcm::set_chebyshev(c, a);
     return DCM_OK;
 }
 

@@ -215,6 +215,13 @@ int dcm_precondition(dcm_ctx *ctx, double *s, const double *r, int prit);
  * context starts sequential when the environment has DCM_SUM_ORDER=sequential. */
 enum { DCM_SUM_TREE = 0, DCM_SUM_SEQUENTIAL = 1 };
 int dcm_set_sum_order(dcm_ctx *ctx, int order);
+/* The mixed path's FP32 inner solve (no counterpart in the reference, whose
+ * solve is FP64 throughout) may relax the four sweeps of the Jacobi
+ * preconditioner with Chebyshev factors for the eigenvalue interval
+ * [a, 2] of D^-1 A (default 0.005); a = 0 keeps the reference's plain
+ * sweeps.  The preconditioner stays a fixed linear map at the same cost;
+ * the FP64 path and the FP64 refinement are not affected. */
+int dcm_set_precond_chebyshev(dcm_ctx *ctx, double a);
 /* inn_prod (:458-482): interior dot product, result to *out (host). */
 int dcm_inn_prod(dcm_ctx *ctx, const double *x, const double *y, double *out);
 /* bicgstab (:293-444).  xg,yg,zg are host coordinate arrays used only for

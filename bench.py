@@ -38,7 +38,16 @@ METRIC = "cell-steps/s mixed vs FP64 at 1e-10 pressure resid; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 DOMINANT = "jac"         # Jacobi sweep kernel k_jac<R,false>: s' = s + (t - A s)/a_p
 JAC_WORDS = 3            # reads s and t, writes s'
-JAC_PER_ITER = 3         # k_jac<R,false> launches per BiCGSTAB iteration (sweeps 2, 2', 3')
+PROFILE_STEPS = 2        # steps whose dominant-kernel launches are timed
+# the mixed inner solve's preconditioner: Chebyshev factors on [CHEB_A, 2]
+# with CHEB_DIVISIONS divisions by a_p (the library's default, set explicitly)
+CHEB_A, CHEB_DIVISIONS = 0.002, 12
+
+
+def jac_per_iter(divisions):
+    """k_jac<R,false> launches per BiCGSTAB iteration: the middle sweeps of
+    both preconditioner applications and the last sweep of the second."""
+    return 2 * (divisions - 3) + 1
 
 
 def canonical_bytes_fp64(F, iters):
@@ -46,7 +55,7 @@ def canonical_bytes_fp64(F, iters):
     return 8.0 * (39 + 30 * F + 37 * iters)
 
 
-def canonical_bytes_mixed(F, inner_iters, outer):
+def canonical_bytes_mixed(F, inner_iters, outer, divisions=4):
     """Same stages with each word at its storage precision (bytes/cell-step).
 
     FP64: state (u,v,w,theta,q,p,b,x), sources, RHS, the true residual and
@@ -55,7 +64,9 @@ def canonical_bytes_mixed(F, inner_iters, outer):
     """
     b = 8 + 56 + 32                      # thbar, sources, Poisson RHS
     b += outer * (20 + 24 + 20)          # true residual, inner start, x += d
-    b += 148 * inner_iters               # 37 FP32 words per inner iteration
+    # 37 FP32 words per inner iteration with the reference's four divisions,
+    # each further division one more sweep (3 words) in both applications
+    b += 4 * (37 + 6 * (divisions - 4)) * inner_iters
     b += 24 + 36                         # adv_ref_state, FP32 velocity shadows
     b += F * (8 + 24 + 28 + 32 + 32)     # MPDATA per field (min, 4 passes)
     b += F * 24                          # scalar update (canonical; an exact no-op)
@@ -63,21 +74,29 @@ def canonical_bytes_mixed(F, inner_iters, outer):
     return float(b)
 
 
-def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rtol=None):
+def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rtol=None,
+             plain_sweeps=False):
     ctx = H.Context(case.dom)
+    if plain_sweeps:
+        ctx.set_precond_chebyshev(0.0)  # the reference's four plain Jacobi sweeps
+    else:
+        ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
     model = H.Model(ctx, case, precision)
     model.set_check_residual(True)
     if inner_rtol:
         model.set_inner_rtol(inner_rtol)
     if warmup:
         model.step(warmup)
-    model.profile(True, only=profile_only)
     t0 = time.perf_counter()
     st = model.step(steps)
     t1 = time.perf_counter()
+    wall = t1 - t0
+    # the dominant kernel's launches timed with HIP events in passes of their
+    # own after the timed steps (events on every launch would slow them)
+    model.profile(True, only=profile_only)
+    sp = model.step(PROFILE_STEPS)
     avg_ms, launches = model.kernel_time(profile_only)
     model.profile(False)
-    wall = t1 - t0
     cells = case.dom.cells
     snap = {"th": model.get(H.F_TH), "u": model.get(H.F_U), "w": model.get(H.F_W),
             "p": model.get(H.F_P), "q0": model.get(H.F_Q, 0), "q1": model.get(H.F_Q, 1)}
@@ -90,7 +109,8 @@ def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rto
         "iters_max": st.iters_max,
         "outer_per_step": st.outer_total / steps,
         "max_true_resid": st.max_true_resid,
-        "kernel": {"name": profile_only, "avg_ms": avg_ms, "launches": launches},
+        "kernel": {"name": profile_only, "avg_ms": avg_ms, "launches": launches,
+                   "profiled_steps": PROFILE_STEPS, "profiled_iters": sp.iters_total},
     }
     return res, snap, (ctx, model)
 
@@ -104,13 +124,16 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
 
     With ``profile`` the batched Jacobi sweeps are timed with HIP events
     (which switches the graphs off) for the roofline: algorithmic bytes are
-    12 B x 3 sweeps per inner iteration x the cells of each member that
-    iterated, summed over members, against the summed launch durations."""
+    12 B x the plain sweeps per inner iteration (jac_per_iter) x the cells of
+    each member that iterated, summed over members, against the summed
+    launch durations."""
     cases = W.ensemble_members(members, seed=seed, steps=warmup + steps)
     cells = sum(c.dom.cells for c in cases)
     out = {"members": len(cases), "cells": cells,
            "shapes": sorted({f"{c.dom.ip}x{c.dom.jp}x{c.dom.kp}" for c in cases})}
     ctxs = [H.Context(c.dom) for c in cases]
+    for x in ctxs:
+        x.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
     models = [H.Model(x, c, H.MIXED) for x, c in zip(ctxs, cases)]
     try:
         with H.Ensemble(models) as e:
@@ -129,8 +152,10 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
                 avg_ms, launches = models[0].kernel_time(DOMINANT)
                 models[0].profile(False)
                 swept = sum(c.dom.cells * s.iters_total for c, s in zip(cases, stats))
-                kernel = {"name": "k_jac_b<float,false>", "avg_ms": avg_ms, "launches": launches,
-                          "algorithmic_bytes": JAC_WORDS * 4 * JAC_PER_ITER * swept}
+                kernel = {"name": "k_jac_b<float,false,true>", "avg_ms": avg_ms,
+                          "launches": launches,
+                          "algorithmic_bytes": JAC_WORDS * 4 * jac_per_iter(CHEB_DIVISIONS) *
+                          swept}
         out["batched_mixed"] = {
             "value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
             "autotuned_march_planes": planes, "autotune_s": tune_s,
@@ -472,6 +497,8 @@ def main(argv=None):
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--inner-rtol", type=float, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain-sweeps", action="store_true",
+                    help="skip the mixed path with the reference's plain Jacobi sweeps")
     ap.add_argument("--cpu-sample", default="64x64:1")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10,
@@ -517,6 +544,19 @@ def main(argv=None):
         print(f"[bench] {pth}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} "
               f"ms/step, {res['iters_per_step']:.1f} it/step", file=sys.stderr, flush=True)
 
+    if "mixed" in out and not args.no_plain_sweeps:
+        # the mixed path with the reference's preconditioner (four plain
+        # Jacobi sweeps), to separate the FP32 gain from the Chebyshev one
+        print(f"[bench] mixed with plain Jacobi sweeps: warmup {args.warmup}, timed "
+              f"{args.steps} steps", file=sys.stderr, flush=True)
+        res, _, handles = run_path(H, case, H.MIXED, args.warmup, args.steps,
+                                   inner_rtol=args.inner_rtol, plain_sweeps=True)
+        handles[1].close()
+        handles[0].close()
+        out["mixed_plain_jacobi"] = res
+        print(f"[bench] mixed, plain sweeps: {res['value']:.4g} cell-steps/s, "
+              f"{res['ms_per_step']:.2f} ms/step, {res['iters_per_step']:.1f} it/step",
+              file=sys.stderr, flush=True)
     main_path = "mixed" if "mixed" in out else paths[0]
     r = out[main_path]
     cells = case.dom.cells
@@ -529,21 +569,24 @@ def main(argv=None):
         # the launches per iteration.  avg_launch_ms (every launch) is the
         # figure that rocprofv3's kernel statistics report.
         total_ms = k["avg_ms"] * k["launches"]
-        active = min(k["launches"], JAC_PER_ITER * r["iters_total"])
+        per_iter = jac_per_iter(CHEB_DIVISIONS if main_path == "mixed" else 4)
+        active = min(k["launches"], per_iter * k["profiled_iters"])
         active_ms = total_ms / active
         achieved = JAC_WORDS * word * cells / (active_ms * 1e-3) / 1e9
     traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                 "traffic": traffic,
-                "kernel": f"k_jac<{'float' if word == 4 else 'double'},false>",
+                "kernel": ("k_jac<float,false,true> (a Chebyshev-relaxed sweep)" if word == 4
+                           else "k_jac<double,false,false>"),
                 "algorithmic_bytes_per_launch": JAC_WORDS * word * cells,
                 "avg_active_launch_ms": active_ms,
                 "avg_launch_ms": k["avg_ms"], "launches": k["launches"]}
     step_roof = {}
     for pth, res in out.items():
-        if pth == "mixed":
-            bpc = canonical_bytes_mixed(F, res["iters_per_step"], res["outer_per_step"])
+        if pth.startswith("mixed"):
+            bpc = canonical_bytes_mixed(F, res["iters_per_step"], res["outer_per_step"],
+                                        4 if pth == "mixed_plain_jacobi" else CHEB_DIVISIONS)
         else:
             bpc = canonical_bytes_fp64(F, res["iters_per_step"])
         step_roof[pth] = {"bytes_per_cell_step": bpc,

@@ -170,7 +170,9 @@ struct dcm_ctx {
     // (set_chebyshev): the factors of its four sweeps, in order
     bool cheb_on = false;
     double cheb_a = 0.0;
-    double cheb_w[4] = {1.0, 1.0, 1.0, 1.0};
+    int cheb_deg = 4;  // a_p divisions per application (the reference's 4)
+    double cheb_w[16] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,
+                         1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
     // i planes per marching block of the solver (0: the built-in default)
     int march_planes = 0;
     // device tables of a batched solve and the members' flags (device and
@@ -224,7 +226,7 @@ int prof_collect(dcm_ctx *c);
 // Chebyshev relaxation factors of the mixed inner solve's preconditioner on
 // the eigenvalue interval [a, 2] of D^-1 A (a = 0: the reference's Jacobi)
 // ---------------------------------------------------------------------------
-void set_chebyshev(dcm_ctx *c, double a);
+void set_chebyshev(dcm_ctx *c, double a, int divisions = 4);
 
 // ---------------------------------------------------------------------------
 // resident model fields, for the output writer (dcm_dynamics.hip)

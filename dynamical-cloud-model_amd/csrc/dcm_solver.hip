@@ -1195,7 +1195,7 @@ struct IterLaunch {
     }
     void jac_p(int si, int ti, int oi) const {
         if (cheb())
-            jac_launch<true, true>("jac_p", si, ti, oi, (R)c->cheb_w[3]);
+            jac_launch<true, true>("jac_p", si, ti, oi, (R)c->cheb_w[c->cheb_deg - 1]);
         else
             jac_launch<true, false>("jac_p", si, ti, oi, R(1));
     }
@@ -1278,14 +1278,28 @@ struct IterLaunch {
     }
     // dynamics.f90:391-425
     void iteration() const {
+        // the sweeps between the fused first one and the last: d - 3 of
+        // them, d the preconditioner's number of a_p divisions (4, as the
+        // reference's s = r/a_p and three sweeps, unless Chebyshev sets it)
+        const int d = cheb() ? c->cheb_deg : 4;
         first<0>(VT);               // t = r - bet*(omg*v1), sweep 1 -> wa
-        jac("jac", VWA, VT, VWB, 2);  // sweep 2 -> wb
-        jac_p(VWB, VT, VP);         // sweep 3, p = s + bet*p
+        int cur = VWA;
+        for (int k = 2; k < d - 1; ++k) {
+            const int nxt = cur == VWA ? VWB : VWA;
+            jac("jac", cur, VT, nxt, k);  // sweep k
+            cur = nxt;
+        }
+        jac_p(cur, VT, VP);         // the last sweep, p = s + bet*p
         matvec<false>(VP, VCR, VV1);  // v1 = A p, <cr,v1>
         fin(0);
         first<1>(VS);               // s = r - alf*v1, sweep 1 -> wa
-        jac("jac", VWA, VS, VWB, 2);
-        jac("jac", VWB, VS, VCS, 3);  // cs = M s
+        cur = VWA;
+        for (int k = 2; k < d - 1; ++k) {
+            const int nxt = cur == VWA ? VWB : VWA;
+            jac("jac", cur, VS, nxt, k);
+            cur = nxt;
+        }
+        jac("jac", cur, VS, VCS, d - 1);  // cs = M s
         matvec<true>(VCS, VS, VT);  // t = A cs, <t,t>, <t,s>
         fin(1);
         update();                   // x, r; <r,r>, <r,cr>
@@ -1780,9 +1794,9 @@ int dcm_set_sum_order(dcm_ctx *c, int order) {
     return DCM_OK;
 }
 
-int dcm_set_precond_chebyshev(dcm_ctx *c, double a) {
-    if (!c || !(a >= 0.0) || a >= 2.0) return DCM_BAD_ARG;
-    dcm::set_chebyshev(c, a);
+int dcm_set_precond_chebyshev(dcm_ctx *c, double a, int divisions) {
+    if (!c || !(a >= 0.0) || a >= 2.0 || divisions < 3 || divisions > 16) return DCM_BAD_ARG;
+    dcm::set_chebyshev(c, a, divisions);
     return DCM_OK;
 }
 

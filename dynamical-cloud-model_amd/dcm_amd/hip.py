@@ -136,7 +136,7 @@ _SIGS = {
     "dcm_precondition": ([_vp, _vp, _vp, _i], _i),
     "dcm_inn_prod": ([_vp, _vp, _vp, _dp], _i),
     "dcm_set_sum_order": ([_vp, _i], _i),
-    "dcm_set_precond_chebyshev": ([_vp, _d], _i),
+    "dcm_set_precond_chebyshev": ([_vp, _d, _i], _i),
     "dcm_bicgstab": ([_vp, _dp, _dp, _dp, _vp, _vp, _vp, _d, _i, _i, _ip, _dp], _i),
     "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
     "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
@@ -395,10 +395,10 @@ class Context:
         """SUM_TREE (default) or SUM_SEQUENTIAL (the reference's serial order)."""
         _call("dcm_set_sum_order", self.h, order)
 
-    def set_precond_chebyshev(self, a):
-        """Chebyshev factors on [a, 2] for the mixed inner solve's sweeps
-        (0: the reference's Jacobi sweeps)."""
-        _call("dcm_set_precond_chebyshev", self.h, float(a))
+    def set_precond_chebyshev(self, a, divisions=4):
+        """Chebyshev factors on [a, 2] for the mixed inner solve's sweeps, with
+        `divisions` applications of 1/a_p (0: the reference's Jacobi sweeps)."""
+        _call("dcm_set_precond_chebyshev", self.h, float(a), int(divisions))
 
     def inn_prod(self, x, y) -> float:
         out = _d()

@@ -218,10 +218,12 @@ int dcm_set_sum_order(dcm_ctx *ctx, int order);
 /* The mixed path's FP32 inner solve (no counterpart in the reference, whose
  * solve is FP64 throughout) may relax the four sweeps of the Jacobi
  * preconditioner with Chebyshev factors for the eigenvalue interval
- * [a, 2] of D^-1 A (default 0.005); a = 0 keeps the reference's plain
- * sweeps.  The preconditioner stays a fixed linear map at the same cost;
+ * [a, 2] of D^-1 A, with `divisions` (3..16) applications of 1/a_p per
+ * preconditioner call (the reference's s = r/a_p and three sweeps are 4);
+ * the default is a = 0.002 with 12 divisions.  a = 0 restores the
+ * reference's four plain sweeps.  The preconditioner stays a fixed linear map at the same cost;
  * the FP64 path and the FP64 refinement are not affected. */
-int dcm_set_precond_chebyshev(dcm_ctx *ctx, double a);
+int dcm_set_precond_chebyshev(dcm_ctx *ctx, double a, int divisions);
 /* inn_prod (:458-482): interior dot product, result to *out (host). */
 int dcm_inn_prod(dcm_ctx *ctx, const double *x, const double *y, double *out);
 /* bicgstab (:293-444).  xg,yg,zg are host coordinate arrays used only for

@@ -43,7 +43,8 @@ module dcm_hip
         dcm_fill_along_z, dcm_upwind3d, dcm_mpdata3d, dcm_mpdata_vec3d, dcm_adv_ref_state, &
         dcm_mpdata3d_add, dcm_upwind_vert3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, &
         dcm_subgrid, dcm_sources_phys, dcm_model_set_physics, &
-        dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_bicgstab, &
+        dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_set_precond_chebyshev, &
+        dcm_bicgstab, &
         dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
         dcm_model_get, dcm_model_init, dcm_model_step, &
@@ -236,6 +237,15 @@ module dcm_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: order
+        end function
+        ! the mixed inner solve's preconditioner: Chebyshev factors on [a, 2]
+        ! with `divisions` applications of 1/a_p (a = 0: the reference's)
+        integer(c_int) function dcm_set_precond_chebyshev(ctx, a, divisions) &
+                bind(c, name="dcm_set_precond_chebyshev")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), value :: a
+            integer(c_int), value :: divisions
         end function
         integer(c_int) function dcm_divergence(ctx, u, v, w, div) bind(c, name="dcm_divergence")
             import :: c_int, c_ptr

@@ -247,7 +247,7 @@ def sedimentation(H, W, calls=20, cpu=True):
     return out
 
 
-def physics(H, W, steps, warmup=2, base=None):
+def physics(H, W, steps, warmup=2, base=None, cpu=True):
     """The sheared moist bubble with every option of sources but radiation
     (subgrid viscosity and surface layer, damping layer, forcing,
     subsidence) on the 256x256x128 grid: model steps in both precisions, and
@@ -294,9 +294,34 @@ def physics(H, W, steps, warmup=2, base=None):
         wall = (time.perf_counter() - t0) / calls
     canonical = 8.0 * (18 + 4 * nq) * cells
     out["subgrid"] = {"call": f"dcm_subgrid, nq {nq}", "ms_per_call": 1e3 * wall,
+                      "cells_per_s": cells / wall,
                       "canonical_GBps": canonical / wall / 1e9,
                       "frac_of_8TBps": canonical / wall / 8e12}
+    if cpu:
+        out["subgrid"]["cpu_baseline"] = subgrid_cpu(W, nq)
     return out
+
+
+def subgrid_cpu(W, nq):
+    """The oracle's calculate_subgrid_3d (single thread) on a 64x64x128 sample
+    of the same case, one call."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    case = W.sheared_moist_bubble(64, 64, 128, ntracer=nq - 1)
+    dom = case.dom
+    g = O.Grid(dom)
+    ph = O.Physics(dom, case.physics, nq=nq)
+    z = lambda *sh: np.zeros(sh if sh else dom.shape)
+    vism, vist, strain, su, sv, sw, sth = (z() for _ in range(7))
+    viss, sq = z(nq, *dom.shape), z(nq, *dom.shape)
+    t0 = time.perf_counter()
+    O.subgrid(g, ph, case.u, case.v, case.w, case.u, case.v, case.w, case.th, case.q, vism, vist,
+              viss, strain, su, sv, sw, sth, sq)
+    wall = time.perf_counter() - t0
+    return {"value": dom.cells / wall, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle subgrid model, sheared moist bubble 64x64x128, nq {nq}, one "
+                      f"call, {wall:.2f} s"}
 
 
 def output(H, W, steps=20, every=5):
@@ -357,6 +382,18 @@ def output(H, W, steps=20, every=5):
         shutil.rmtree(tmp, ignore_errors=True)
     rec_bytes = 4.0 * nf * cells
     recs = wout["recs"]
+    # the host alternative: the library's host writer packing one record
+    # from FP64 host arrays of the same state and writing it (dcm_nc_*)
+    tmp2 = tempfile.mkdtemp(prefix="dcm_out_")
+    try:
+        f = [case.u, case.v, case.w, case.th, np.zeros(case.dom.shape), np.zeros(case.dom.shape)]
+        with H.NcFile(os.path.join(tmp2, "host.nc"), case.dom, moisture=True, nq=case.nq,
+                      nprec=1) as nc:
+            t0 = time.perf_counter()
+            nc.put_record(1, 0.0, *f, q=case.q)
+            t_host = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp2, ignore_errors=True)
     return {"workload": "moist_bubble_256x256x128, mixed", "record_MB": rec_bytes / 1e6,
             "records": recs, "every_steps": every, "steps": steps,
             "solver_iterations_equal": plain["iters"] == wout["iters"],
@@ -366,7 +403,11 @@ def output(H, W, steps=20, every=5):
             "host_stall_s": wout["stall"],
             "pack_ms": 1e3 * wout["pack"], "pack_GBps": 12.0 * nf * cells / wout["pack"] / 1e9,
             "writer_s_per_record": wout["wr"] / max(recs, 1),
-            "writer_GBps": rec_bytes * recs / max(wout["wr"], 1e-9) / 1e9}
+            "writer_GBps": rec_bytes * recs / max(wout["wr"], 1e-9) / 1e9,
+            "cpu_baseline": {"value": 1e3 * t_host, "unit": "ms per record", "cores": 1,
+                             "kind": "port",
+                             "sample": "one record packed from FP64 host arrays and written "
+                                       "by the library's host writer (dcm_nc_put_record)"}}
 
 
 def initialisation(H, W):
@@ -401,10 +442,28 @@ def initialisation(H, W):
         t_host = time.perf_counter() - t0
         m.close()
     nf = 4 + case.nq
+    # the host alternative: the same 3-D state built with numpy from the
+    # profiles (columns broadcast, the noise layers), before any upload
+    rc, pr = case.init
+    t0 = time.perf_counter()
+    shape = case.dom.shape
+    hu = np.broadcast_to(pr["u"], shape).copy()
+    hv = np.broadcast_to(pr["v"], shape).copy()
+    hw = np.zeros(shape)
+    hq = np.stack([np.broadcast_to(pr["q"][n], shape).copy() for n in range(case.nq)])
+    zk = np.broadcast_to(pr["z"], shape)
+    r = np.random.default_rng(2).random(shape)
+    layer = (zk > rc.cfg.param_z) & (zk <= rc.cfg.param_z + rc.cfg.param_delz)
+    hth = np.where(layer, -0.001 + (r - 0.5) / 20.0, 0.0)
+    t_cpu = time.perf_counter() - t0
+    del hu, hv, hw, hq, hth, r
     return {"namelist": "namelist.in (adiabatic profile, param_theta noise), grid 256x256x128",
             "profiles_ms": 1e3 * t_prof, "model_create_device_init_ms": 1e3 * t_dev,
             "model_create_from_host_arrays_ms": 1e3 * t_host,
-            "fields": nf, "field_MB": 8.0 * nf * case.dom.cells / 1e6}
+            "fields": nf, "field_MB": 8.0 * nf * case.dom.cells / 1e6,
+            "cpu_baseline": {"value": 1e3 * t_cpu, "unit": "ms", "cores": 1, "kind": "port",
+                             "sample": "the same 3-D initial state built with numpy on the "
+                                       "host from the profiles (no upload), full grid"}}
 
 
 def ensemble_line(H, W, args):
@@ -622,7 +681,7 @@ def main(argv=None):
     if args.physics_steps > 0:
         print(f"[bench] physics options: sheared moist bubble 256x256x128, "
               f"{args.physics_steps} steps", file=sys.stderr, flush=True)
-        line["physics"] = physics(H, W, args.physics_steps, base=out)
+        line["physics"] = physics(H, W, args.physics_steps, base=out, cpu=not args.no_cpu)
     print("[bench] initialisation from namelist.in", file=sys.stderr, flush=True)
     line["initialisation"] = initialisation(H, W)
     if args.output_steps > 0:

@@ -188,32 +188,60 @@ __global__ __launch_bounds__(256) void k_sg_momentum(
     }
 }
 
-// viscous flux divergence of a scalar (sth :288-312, sq :322-349); vt is
-// vist, already multiplied by rhoa
+// viscous flux divergence of the scalars (sth :288-312, sq :322-349): th
+// and then the nq fields of q, in one pass that reads the neighbourhood of
+// vt (vist, already multiplied by rhoa) once per cell; each field's
+// arithmetic is the reference's.  With viss, viss(:,:,:,n) = vist is written
+// on the interior in the same pass (:185; the halo shell by k_viss_shell).
 __global__ __launch_bounds__(256) void k_sg_diffuse(const double *__restrict__ vt,
-                                                    const double *__restrict__ T,
-                                                    double *__restrict__ sT, Geo g, Metrics mt) {
+                                                    const double *__restrict__ th,
+                                                    double *__restrict__ sth,
+                                                    const double *__restrict__ q,
+                                                    double *__restrict__ sq, int nq,
+                                                    double *__restrict__ viss, Geo g,
+                                                    Metrics mt) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 1;
     if (k > g.kp || j > g.jp) return;
     const long long c = pidx(g, k, j, i);
     const long long si = g.si, sj = g.sj;
-    const double ax = ((vt[c] + vt[c - si]) + vt[c - 1]) + vt[c - 1 - si];
-    const double bx = ((vt[c + si] + vt[c]) + vt[c - 1 + si]) + vt[c - 1];
-    const double ay = ((vt[c] + vt[c - sj]) + vt[c - 1]) + vt[c - 1 - sj];
-    const double by = ((vt[c + sj] + vt[c]) + vt[c - 1 + sj]) + vt[c - 1];
-    const double tc = T[c];
-    const double X = (0.25 * ((ax * (T[c - si] - tc)) / mt.dxn[i - 1] -
-                              (bx * (tc - T[c + si])) / mt.dxn[i])) /
-                     mt.dx[i - 1];
-    const double Y = (0.25 * ((ay * (T[c - sj] - tc)) / mt.dyn[j - 1] -
-                              (by * (tc - T[c + sj])) / mt.dyn[j])) /
-                     mt.dy[j - 1];
-    const double Z = ((vt[c - 1] * (T[c - 1] - tc)) / mt.dz[k - 1] -
-                      (vt[c] * (tc - T[c + 1])) / mt.dz[k]) /
-                     mt.dz[k - 1];
-    sT[c] = sT[c] + ((X + Y) + Z) / mt.rhoan[k];
+    const double v0 = vt[c], vd = vt[c - 1];
+    const double ax = ((v0 + vt[c - si]) + vd) + vt[c - 1 - si];
+    const double bx = ((vt[c + si] + v0) + vt[c - 1 + si]) + vd;
+    const double ay = ((v0 + vt[c - sj]) + vd) + vt[c - 1 - sj];
+    const double by = ((vt[c + sj] + v0) + vt[c - 1 + sj]) + vd;
+    const double dxm = mt.dx[i - 1], dxnm = mt.dxn[i - 1], dxn = mt.dxn[i];
+    const double dym = mt.dy[j - 1], dynm = mt.dyn[j - 1], dyn = mt.dyn[j];
+    const double dzm = mt.dz[k - 1], dzk = mt.dz[k], rn = mt.rhoan[k];
+    for (int f = 0; f <= nq; ++f) {
+        const double *T = f == 0 ? th : q + (long long)(f - 1) * g.ntot;
+        double *sT = f == 0 ? sth : sq + (long long)(f - 1) * g.ntot;
+        const double tc = T[c];
+        const double X = (0.25 * ((ax * (T[c - si] - tc)) / dxnm - (bx * (tc - T[c + si])) / dxn)) /
+                         dxm;
+        const double Y = (0.25 * ((ay * (T[c - sj] - tc)) / dynm - (by * (tc - T[c + sj])) / dyn)) /
+                         dym;
+        const double Z = ((vd * (T[c - 1] - tc)) / dzm - (v0 * (tc - T[c + 1])) / dzk) / dzm;
+        sT[c] = sT[c] + ((X + Y) + Z) / rn;
+        if (viss && f > 0) viss[(long long)(f - 1) * g.ntot + c] = v0;
+    }
+}
+
+// viss(:,:,:,n) = vist on the cells k_sg_diffuse does not write: the halo
+// shell (k or j or i outside the interior)
+__global__ void k_viss_shell(const double *__restrict__ vist, double *__restrict__ viss, int nq,
+                             Geo g) {
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < g.ntot;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c / g.si);
+        const long long r = c - (long long)i * g.si;
+        const int j = (int)(r / g.sj);
+        const int k = (int)(r - (long long)j * g.sj);
+        if (i >= 1 && i <= g.ip && j >= 1 && j <= g.jp && k >= 1 && k <= g.kp) continue;
+        const double v = vist[c];
+        for (int n = 0; n < nq; ++n) viss[(long long)n * g.ntot + c] = v;
+    }
 }
 
 // damping layer (dynamics.f90:802-839), forcing (:841-860) and subsidence
@@ -245,13 +273,6 @@ __global__ __launch_bounds__(256) void k_relax(
     sw[c] = d;
 }
 
-__global__ void k_copy_field(const double *__restrict__ src, double *__restrict__ dst,
-                             long long n) {
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x)
-        dst[t] = src[t];
-}
-
 // calculate_subgrid_3d (subgrid_3d.f90:38-357)
 int subgrid_dev(dcm_ctx *c, const PhysicsDev &ph, const double *u, const double *v,
                 const double *w, const double *th, const SubgridFields &sf, double *su,
@@ -270,12 +291,11 @@ int subgrid_dev(dcm_ctx *c, const PhysicsDev &ph, const double *u, const double 
     DCM_CHECK(fill<double>(c, sf.vism, FILL_FULL_WO, 0.0, 0.0));  // :176-179
     DCM_CHECK(fill<double>(c, sf.vist, FILL_FULL_WO, 0.0, 0.0));
     const bool moist = ph.moisture && sf.q && sf.sq && ph.nq > 0;
-    if (moist && sf.viss)
-        for (int n = 0; n < ph.nq; ++n) {
-            hipLaunchKernelGGL(k_copy_field, dim3(2048), dim3(kBlock), 0, c->stream, sf.vist,
-                               sf.viss + (size_t)n * g.ntot, (long long)g.ntot);
-            DCM_LAUNCH_CHECK();
-        }
+    if (moist && sf.viss) {  // :185, the shell here, the interior with the diffusion
+        hipLaunchKernelGGL(k_viss_shell, dim3(2048), dim3(kBlock), 0, c->stream, sf.vist,
+                           sf.viss, ph.nq, g);
+        DCM_LAUNCH_CHECK();
+    }
     double *t[6];
     for (int n = 0; n < 6; ++n) DCM_CHECK(work_field64(c, kSlotStress + n, &t[n]));
     hipLaunchKernelGGL(k_sg_stress, L.grid, L.block, 0, c->stream, sf.zu, sf.zv, sf.zw, sf.vism,
@@ -288,17 +308,14 @@ int subgrid_dev(dcm_ctx *c, const PhysicsDev &ph, const double *u, const double 
     DCM_CHECK(fill<double>(c, su, FILL_FULL, 0.0, 0.0));  // :279-284
     DCM_CHECK(fill<double>(c, sv, FILL_FULL, 0.0, 0.0));
     DCM_CHECK(fill<double>(c, sw, FILL_FULL, 0.0, 0.0));
-    hipLaunchKernelGGL(k_sg_diffuse, L.grid, L.block, 0, c->stream, sf.vist, th, sth, g, mt);
+    hipLaunchKernelGGL(k_sg_diffuse, L.grid, L.block, 0, c->stream, sf.vist, th, sth,
+                       moist ? sf.q : nullptr, moist ? sf.sq : nullptr, moist ? ph.nq : 0,
+                       moist ? sf.viss : nullptr, g, mt);
     DCM_LAUNCH_CHECK();
     DCM_CHECK(fill<double>(c, sth, FILL_FULL, 0.0, 0.0));  // :314-315
     if (moist)
-        for (int n = 0; n < ph.nq; ++n) {
-            double *s = sf.sq + (size_t)n * g.ntot;
-            hipLaunchKernelGGL(k_sg_diffuse, L.grid, L.block, 0, c->stream, sf.vist,
-                               sf.q + (size_t)n * g.ntot, s, g, mt);
-            DCM_LAUNCH_CHECK();
-            DCM_CHECK(fill<double>(c, s, FILL_FULL, 0.0, 0.0));  // :351-352
-        }
+        for (int n = 0; n < ph.nq; ++n)
+            DCM_CHECK(fill<double>(c, sf.sq + (size_t)n * g.ntot, FILL_FULL, 0.0, 0.0));  // :351-352
     prof_end(c, tok);
     return DCM_OK;
 }

@@ -1,5 +1,9 @@
 // dcm_advect.hip -- MPDATA advection (pamm/sfvt/advection_3d.F90, the copy
-// linked into the reference main.exe) on one GPU.
+// linked into the reference main.exe) on one GPU.  The context's
+// mpdata_variant 1 selects the arithmetic of sfvt/advection_3d.F90 instead:
+// its limiter bounds each direction separately with unweighted
+// denominators (:555-682, vertical :1028-1091), and its vector routines use
+// unweighted antidiffusive velocities (:1384-1507, :1874-1909).
 //
 // Data flow of one mpdata_3d / mpdata_vec_3d call (kord=2, limiter on):
 //   K1 min        m = min(psi)  (whole array for mpdata_3d, interior and per
@@ -197,9 +201,11 @@ __global__ __launch_bounds__(256) void k_upwind(const R *src, const R *u, const 
 }
 
 // ---------------------------------------------------------------------------
-// K3: antidiffusive velocities (advection_3d.F90:318-506)
+// K3: antidiffusive velocities (advection_3d.F90:318-506).  G: the
+// density-weighted form; !G: the unweighted one of sfvt/advection_3d.F90's
+// mpdata_vec_3d (:1384-1507).
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, bool G>
 __global__ __launch_bounds__(256) void k_antidiff(const R *ut, const R *vt, const R *wt,
                                                   const R *po, R *us, R *vs, R *ws,
                                                   const int *flag, Geo g, MetPack mp, R dt) {
@@ -221,6 +227,45 @@ __global__ __launch_bounds__(256) void k_antidiff(const R *ut, const R *vt, cons
     const R g1 = rn[k], g2 = rn[k];
     const R g3 = h * (rn[k] + rn[k + 1]);
     const R dxim = M.dx[i - 1], dyjm = M.dy[j - 1], dzkm = M.dz[k - 1];
+    if constexpr (!G) {
+        const R ud1 = (((W(0, 0, 0) + W(0, 0, 1)) - W(-1, 0, 0)) - W(-1, 0, 1)) / dzkm +
+                      (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1)) / dyjm;
+        const R ud2 = (((W(0, 0, 0) + W(0, 1, 0)) - W(-1, 0, 0)) - W(-1, 1, 0)) / dzkm +
+                      (((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1)) / dxim;
+        const R ud3 = (((U(0, 0, 0) + U(1, 0, 0)) - U(0, 0, -1)) - U(1, 0, -1)) / dxim +
+                      (((V(0, 0, 0) + V(1, 0, 0)) - V(0, -1, 0)) - V(1, -1, 0)) / dyjm;
+        const R hz = h * (M.dzn[k - 1] + M.dzn[k]), hy = h * (M.dyn[j - 1] + M.dyn[j]),
+                hx = h * (M.dxn[i - 1] + M.dxn[i]);
+        const R a1 = (((q * (((W(0, 0, 1) + W(0, 0, 0)) + W(-1, 0, 1)) + W(-1, 0, 0))) *
+                       (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))) /
+                      ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small)) / hz;
+        const R b1 = (((q * (((V(0, 0, 1) + V(0, 0, 0)) + V(0, -1, 1)) + V(0, -1, 0))) *
+                       (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))) /
+                      ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small)) / hy;
+        const R a2 = (((q * (((W(0, 1, 0) + W(0, 0, 0)) + W(-1, 1, 0)) + W(-1, 0, 0))) *
+                       (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))) /
+                      ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small)) / hz;
+        const R b2 = (((q * (((U(0, 1, 0) + U(0, 0, 0)) + U(0, 1, -1)) + U(0, 0, -1))) *
+                       (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))) /
+                      ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small)) / hx;
+        const R a3 = (((q * (((V(1, 0, 0) + V(0, 0, 0)) + V(1, -1, 0)) + V(0, -1, 0))) *
+                       (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))) /
+                      ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small)) / hy;
+        const R b3 = (((q * (((U(1, 0, 0) + U(0, 0, 0)) + U(1, 0, -1)) + U(0, 0, -1))) *
+                       (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))) /
+                      ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small)) / hx;
+        const R uc = U(0, 0, 0), vc = V(0, 0, 0), wc = W(0, 0, 0);
+        R usv = ((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))) /
+                 ((P(0, 0, 1) + P(0, 0, 0)) + small)) / M.dxn[i] - ((h * dt) * uc) * (a1 + b1);
+        R vsv = ((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))) /
+                 ((P(0, 1, 0) + P(0, 0, 0)) + small)) / M.dyn[j] - ((h * dt) * vc) * (a2 + b2);
+        R wsv = ((((fabs(wc) * M.dz[k]) - ((dt * wc) * wc)) * (P(1, 0, 0) - P(0, 0, 0))) /
+                 ((P(1, 0, 0) + P(0, 0, 0)) + small)) / M.dzn[k] - ((h * dt) * wc) * (a3 + b3);
+        us[c] = usv - ((q * dt) * uc) * (((U(0, 0, 1) - U(0, 0, -1)) / dxim) + ud1);
+        vs[c] = vsv - ((q * dt) * vc) * (((V(0, 1, 0) - V(0, -1, 0)) / dyjm) + ud2);
+        ws[c] = wsv - ((q * dt) * wc) * (((W(1, 0, 0) - W(-1, 0, 0)) / dzkm) + ud3);
+        return;
+    }
     R u_div1 = (ra[k] * (W(0, 0, 0) + W(0, 0, 1)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 0, 1))) /
                    (dzkm * g1) +
                (rn[k] * (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1))) / (dyjm * g1);
@@ -293,12 +338,59 @@ __global__ __launch_bounds__(256) void k_antidiff(const R *ut, const R *vt, cons
 
 // ---------------------------------------------------------------------------
 // K4: beta_up / beta_down (advection_3d.F90:549-616).  psi_store is psi-m,
-// recomputed from the untouched psi.
+// recomputed from the untouched psi.  The limiter factors of the x, y and z
+// faces are up[0..2] / dn[0..2]: one shared pair for the linked copy, three
+// pairs for the per-direction limiter of sfvt/advection_3d.F90.
 // ---------------------------------------------------------------------------
+template <typename R> struct Betas {
+    R *up[3], *dn[3];
+};
+
+// sfvt/advection_3d.F90:555-636: bounds from the three cells along each
+// axis; one pair of unweighted denominators (grid spacings dx(i-1), dy(j-1),
+// dz(k-1), small inside the dt product) shared by the three directions
 template <typename R>
+__device__ __forceinline__ void beta_directional(const double *psi, double m, const R *po,
+                                                 const R *ut, const R *vt, const R *wt,
+                                                 Betas<R> b, long long c, int k, int j, int i,
+                                                 const Geo &g, const M1<R> &M, R dt) {
+    const long long st[3] = {g.si, g.sj, 1};
+    const R pc = po[c];
+    R mx[3], mn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const long long s = st[d];
+        const R vals[6] = {(R)(psi[c - s] - m), (R)(psi[c] - m), (R)(psi[c + s] - m),
+                           po[c - s],           pc,              po[c + s]};
+        mx[d] = vals[0];
+        mn[d] = vals[0];
+#pragma unroll
+        for (int q = 1; q < 6; ++q) {
+            if (vals[q] > mx[d]) mx[d] = vals[q];
+            if (vals[q] < mn[d]) mn[d] = vals[q];
+        }
+    }
+    const long long SI = g.si, SJ = g.sj;
+    const R small = Small<R>::v;
+    const R d1 = dt * ((((pos0(ut[c - SI]) * po[c - SI] - neg0(ut[c]) * po[c + SI]) / M.dx[i - 1] +
+                         (pos0(vt[c - SJ]) * po[c - SJ] - neg0(vt[c]) * po[c + SJ]) / M.dy[j - 1]) +
+                        (pos0(wt[c - 1]) * po[c - 1] - neg0(wt[c]) * po[c + 1]) / M.dz[k - 1]) +
+                       small);
+    const R d2 = dt * ((((pos0(ut[c]) * pc - neg0(ut[c - SI]) * pc) / M.dx[i - 1] +
+                         (pos0(vt[c]) * pc - neg0(vt[c - SJ]) * pc) / M.dy[j - 1]) +
+                        (pos0(wt[c]) * pc - neg0(wt[c - 1]) * pc) / M.dz[k - 1]) +
+                       small);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b.up[d][c] = (mx[d] - pc) / d1;
+        b.dn[d][c] = (pc - mn[d]) / d2;
+    }
+}
+
+template <typename R, bool PD>
 __global__ __launch_bounds__(256) void k_beta(const double *psi, const double *mptr,
                                               const R *po, const R *ut, const R *vt,
-                                              const R *wt, R *bup, R *bdn, const int *flag,
+                                              const R *wt, Betas<R> b, const int *flag,
                                               Geo g, MetPack mp, R dt) {
     if (flag && *flag) return;
     const M1<R> M(mp);
@@ -309,6 +401,11 @@ __global__ __launch_bounds__(256) void k_beta(const double *psi, const double *m
     const long long SI = g.si, SJ = g.sj;
     const long long c = pidx(g, k, j, i);
     const double m = *mptr;
+    if constexpr (PD) {
+        beta_directional<R>(psi, m, po, ut, vt, wt, b, c, k, j, i, g, M, dt);
+        return;
+    }
+    R *bup = b.up[0], *bdn = b.dn[0];
     const R small = Small<R>::v;
     const R vals[14] = {(R)(psi[c] - m),      (R)(psi[c - SI] - m), (R)(psi[c + SI] - m),
                         (R)(psi[c - SJ] - m), (R)(psi[c + SJ] - m), (R)(psi[c - 1] - m),
@@ -351,7 +448,7 @@ __device__ __forceinline__ R limited(const R *us, const R *bup, const R *bdn, lo
 // needs the limited velocities as its input)
 template <typename R>
 __global__ __launch_bounds__(256) void k_limit(const R *us, const R *vs, const R *ws,
-                                               const R *bup, const R *bdn, R *ul, R *vl, R *wl,
+                                               Betas<R> b, R *ul, R *vl, R *wl,
                                                const int *flag, Geo g) {
     if (flag && *flag) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -359,9 +456,9 @@ __global__ __launch_bounds__(256) void k_limit(const R *us, const R *vs, const R
     const int i = blockIdx.z + 1;
     if (k > g.kp || j > g.jp) return;
     const long long c = pidx(g, k, j, i);
-    ul[c] = limited<R>(us, bup, bdn, c, g.si);
-    vl[c] = limited<R>(vs, bup, bdn, c, g.sj);
-    wl[c] = limited<R>(ws, bup, bdn, c, 1);
+    ul[c] = limited<R>(us, b.up[0], b.dn[0], c, g.si);
+    vl[c] = limited<R>(vs, b.up[1], b.dn[1], c, g.sj);
+    wl[c] = limited<R>(ws, b.up[2], b.dn[2], c, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,7 +469,7 @@ __global__ __launch_bounds__(256) void k_limit(const R *us, const R *vs, const R
 // ---------------------------------------------------------------------------
 template <typename R, bool TO_PSI>
 __global__ __launch_bounds__(256) void k_upwind_limited(
-    const R *src, const R *us, const R *vs, const R *ws, const R *bup, const R *bdn, R *out,
+    const R *src, const R *us, const R *vs, const R *ws, Betas<R> b, R *out,
     double *psi_out, const double *mptr, const int *flag, Geo g, MetPack mp, R dt,
     int neumann) {
     if (flag && *flag) return;
@@ -384,12 +481,12 @@ __global__ __launch_bounds__(256) void k_upwind_limited(
     const long long c = pidx(g, k, j, i);
     const long long cw = pidx(g, k, j, i == 1 ? g.ip : i - 1);
     const long long cs = pidx(g, k, j == 1 ? g.jp : j - 1, i);
-    const R uc = limited<R>(us, bup, bdn, c, g.si);
-    const R ul = limited<R>(us, bup, bdn, cw, g.si);
-    const R vc = limited<R>(vs, bup, bdn, c, g.sj);
-    const R vl = limited<R>(vs, bup, bdn, cs, g.sj);
-    const R wc = limited<R>(ws, bup, bdn, c, 1);
-    const R wl = k == 1 ? R(0) : limited<R>(ws, bup, bdn, c - 1, 1);
+    const R uc = limited<R>(us, b.up[0], b.dn[0], c, g.si);
+    const R ul = limited<R>(us, b.up[0], b.dn[0], cw, g.si);
+    const R vc = limited<R>(vs, b.up[1], b.dn[1], c, g.sj);
+    const R vl = limited<R>(vs, b.up[1], b.dn[1], cs, g.sj);
+    const R wc = limited<R>(ws, b.up[2], b.dn[2], c, 1);
+    const R wl = k == 1 ? R(0) : limited<R>(ws, b.up[2], b.dn[2], c - 1, 1);
     R res = donor<R>(dt, uc, ul, vc, vl, wc, wl, src[c], src[c + g.si], src[c - g.si],
                      src[c + g.sj], src[c - g.sj], src[c + 1], src[c - 1], M.dxn[i], M.dyn[j],
                      M.dzn[k], M.ra[k], M.ra[k - 1], M.rn[k], k, g.kp, neumann);
@@ -439,7 +536,9 @@ int get_buf<float>(dcm_ctx *c, int slot, float **out) { return work_field32(c, s
 // Work-slot map (separate pools for FP64 and FP32):
 //   0-2 antidiffusive U*,V*,W*   3-5 limited (kord>2)   6-7 beta up/down
 //   8.. psi1 per field, then psi2 per field (kord>2)
+//   80-83 the y and z beta pairs of the per-direction limiter
 static constexpr int kSlotPsi = 8;
+static constexpr int kSlotBetaYZ = 80;
 
 // psi(interior) = psi1 + m (kord = 1)
 template <typename R>
@@ -471,6 +570,10 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
         set_error("mpdata: a category must hold 1..32 fields");
         return DCM_BAD_ARG;
     }
+    // sfvt/advection_3d.F90: per-direction limiter in both routines,
+    // unweighted antidiffusive velocities in the vector one (interior_min)
+    const bool pd = c->mpdata_variant == 1;
+    const bool weighted = !(pd && interior_min);
     const unsigned nbmin = 1024;
     const unsigned nbint = L.grid.x * L.grid.y * L.grid.z;
     DCM_CHECK(ensure_partials(c, (size_t)nbint + nbmin + 16));
@@ -514,13 +617,23 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
         if (kord > 2)
             for (int n = 0; n < nf; ++n) DCM_CHECK(get_buf<R>(c, kSlotPsi + nf + n, &s2[n]));
         // velocity triples: X = (u,v,w); A, B scratch
-        R *A[3], *B[3], *bup = nullptr, *bdn = nullptr;
+        R *A[3], *B[3];
+        Betas<R> bt{};
         for (int d = 0; d < 3; ++d) DCM_CHECK(get_buf<R>(c, d, &A[d]));
         if (kord > 2)
             for (int d = 0; d < 3; ++d) DCM_CHECK(get_buf<R>(c, 3 + d, &B[d]));
         if (monotone) {
-            DCM_CHECK(get_buf<R>(c, 6, &bup));
-            DCM_CHECK(get_buf<R>(c, 7, &bdn));
+            DCM_CHECK(get_buf<R>(c, 6, &bt.up[0]));
+            DCM_CHECK(get_buf<R>(c, 7, &bt.dn[0]));
+            for (int d = 1; d < 3; ++d) {
+                if (pd) {
+                    DCM_CHECK(get_buf<R>(c, kSlotBetaYZ + 2 * (d - 1), &bt.up[d]));
+                    DCM_CHECK(get_buf<R>(c, kSlotBetaYZ + 2 * (d - 1) + 1, &bt.dn[d]));
+                } else {
+                    bt.up[d] = bt.up[0];
+                    bt.dn[d] = bt.dn[0];
+                }
+            }
         }
         const R *in[3] = {u, v, w};
         R **anti = A, **other = B;
@@ -529,27 +642,39 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
             const bool last = it == kord;
             // antidiffusive velocities from the current advecting velocity
             int tok = prof_begin(c, "antidiff");
-            hipLaunchKernelGGL(k_antidiff<R>, L.grid, L.block, 0, c->stream, in[0], in[1], in[2],
-                               cur[0], anti[0], anti[1], anti[2], flag, g, mp, dt);
+            if (weighted)
+                hipLaunchKernelGGL((k_antidiff<R, true>), L.grid, L.block, 0, c->stream, in[0],
+                                   in[1], in[2], cur[0], anti[0], anti[1], anti[2], flag, g, mp,
+                                   dt);
+            else
+                hipLaunchKernelGGL((k_antidiff<R, false>), L.grid, L.block, 0, c->stream, in[0],
+                                   in[1], in[2], cur[0], anti[0], anti[1], anti[2], flag, g, mp,
+                                   dt);
             DCM_LAUNCH_CHECK();
             prof_end(c, tok);
             for (int d = 0; d < 3; ++d) DCM_CHECK(fill<R>(c, anti[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
             if (monotone) {
                 tok = prof_begin(c, "beta");
-                hipLaunchKernelGGL(k_beta<R>, L.grid, L.block, 0, c->stream, psi, dm, cur[0],
-                                   anti[0], anti[1], anti[2], bup, bdn, flag, g, mp, dt);
+                if (pd)
+                    hipLaunchKernelGGL((k_beta<R, true>), L.grid, L.block, 0, c->stream, psi, dm,
+                                       cur[0], anti[0], anti[1], anti[2], bt, flag, g, mp, dt);
+                else
+                    hipLaunchKernelGGL((k_beta<R, false>), L.grid, L.block, 0, c->stream, psi,
+                                       dm, cur[0], anti[0], anti[1], anti[2], bt, flag, g, mp, dt);
                 DCM_LAUNCH_CHECK();
                 prof_end(c, tok);
-                DCM_CHECK(fill<R>(c, bup, FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
-                DCM_CHECK(fill<R>(c, bdn, FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
+                for (int d = 0; d < (pd ? 3 : 1); ++d) {
+                    DCM_CHECK(fill<R>(c, bt.up[d], FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
+                    DCM_CHECK(fill<R>(c, bt.dn[d], FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
+                }
             }
             if (last) {
                 for (int n = 0; n < nf; ++n) {
                     tok = prof_begin(c, monotone ? "upwind_limited" : "upwind_last");
                     if (monotone)
                         hipLaunchKernelGGL((k_upwind_limited<R, true>), L.grid, L.block, 0,
-                                           c->stream, cur[n], anti[0], anti[1], anti[2], bup,
-                                           bdn, (R *)nullptr, psi + (size_t)n * g.ntot, dm + n,
+                                           c->stream, cur[n], anti[0], anti[1], anti[2], bt,
+                                           (R *)nullptr, psi + (size_t)n * g.ntot, dm + n,
                                            flag, g, mp, dt, neumann);
                     else
                         hipLaunchKernelGGL((k_upwind<R, true>), L.grid, L.block, 0, c->stream,
@@ -566,7 +691,7 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
             if (monotone) {
                 // eq 18 into the other triple, which the next pass reads
                 hipLaunchKernelGGL(k_limit<R>, L.grid, L.block, 0, c->stream, anti[0], anti[1],
-                                   anti[2], bup, bdn, other[0], other[1], other[2], flag, g);
+                                   anti[2], bt, other[0], other[1], other[2], flag, g);
                 DCM_LAUNCH_CHECK();
                 for (int d = 0; d < 3; ++d)
                     DCM_CHECK(fill<R>(c, other[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
@@ -673,10 +798,12 @@ __global__ __launch_bounds__(256) void k_vupwind(const double *src, const double
     if (do_sum) block_sum_store<1>(s, partials, 0);
 }
 
-// antidiffusive vertical velocity (:942-958; vector variant :1905-1921)
+// antidiffusive vertical velocity (:942-958; vector variant :1905-1921);
+// plain: the unweighted form of sfvt/advection_3d.F90's mpdata_vec_vert_3d
+// (:1897-1909)
 __global__ __launch_bounds__(256) void k_vantidiff(const double *wt, const double *po,
                                                    double *ws, Geo g, Metrics mt, double dt,
-                                                   const int *flag) {
+                                                   const int *flag, int plain) {
     if (*flag) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -687,6 +814,12 @@ __global__ __launch_bounds__(256) void k_vantidiff(const double *wt, const doubl
     const double gb3 = 0.5 * (rn[k] + rn[k + 1]);
     const double u_div3 = 0.0, u_j_bar3 = 0.0;
     const double wc = wt[c], p0 = po[c], p1 = po[c + 1];
+    if (plain) {
+        const double v = ((((fabs(wc) * dz[k] - (dt * wc) * wc) * (p1 - p0)) /
+                           ((p1 + p0) + Small<double>::v)) / mt.dzn[k]) - u_j_bar3;
+        ws[c] = v - ((0.25 * dt) * wc) * (((wt[c + 1] - wt[c - 1]) / dz[k - 1]) + u_div3);
+        return;
+    }
     double v = ((((fabs(wc) * dz[k] - ((dt * (ra[k] / gb3)) * wc) * wc) * (p1 - p0)) /
                  ((p1 + p0) + Small<double>::v)) / mt.dzn[k]) - u_j_bar3;
     v = v - ((0.25 * dt) * wc) *
@@ -695,11 +828,12 @@ __global__ __launch_bounds__(256) void k_vantidiff(const double *wt, const doubl
 }
 
 // limiter bounds (:1000-1046; vector variant :1964-2010): psi_store is the
-// input field minus its shift, psi_old the current pass
+// input field minus its shift, psi_old the current pass.  pd: the unweighted
+// denominators of sfvt/advection_3d.F90 (:1051-1062, :1975-1985)
 __global__ __launch_bounds__(256) void k_vbeta(const double *psi, const double *shift,
                                                const double *po, const double *wt, double *bup,
                                                double *bdn, Geo g, Metrics mt, double dt,
-                                               const int *flag) {
+                                               const int *flag, int pd) {
     if (*flag) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -717,6 +851,15 @@ __global__ __launch_bounds__(256) void k_vbeta(const double *psi, const double *
     }
     const double *ra = mt.rhoa, *rn = mt.rhoan, *dzn = mt.dzn;
     const double pc = po[c], wm = wt[c - 1], wc = wt[c];
+    if (pd) {
+        const double *dz = mt.dz;
+        const double d1 =
+            dt * ((pos0(wm) * po[c - 1] - neg0(wc) * po[c + 1]) / dz[k - 1] + Small<double>::v);
+        const double d2 = dt * ((pos0(wc) * pc - neg0(wm) * pc) / dz[k - 1] + Small<double>::v);
+        bup[c] = (mx - pc) / d1;
+        bdn[c] = (pc - mn) / d2;
+        return;
+    }
     const double d1 = (dt * ((pos0(wm) * ra[k - 1]) * po[c - 1] - (neg0(wc) * ra[k]) * po[c + 1])) /
                           dzn[k] + Small<double>::v;
     const double d2 = (dt * ((pos0(wc) * ra[k]) * pc - (neg0(wm) * ra[k - 1]) * pc)) / dzn[k] +
@@ -788,6 +931,7 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
     const Metrics mt = c->met;
     double *dm = c->dscal + 16;
     int *flag = (int *)(c->dscal + 8);
+    const bool pd = c->mpdata_variant == 1;  // sfvt/advection_3d.F90's arithmetic
     if (nf < 1 || nf > 32) {
         set_error("vertical mpdata: a category must hold 1..32 fields");
         return DCM_BAD_ARG;
@@ -826,7 +970,7 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
         if (it > 1) {
             it2 += 1;
             hipLaunchKernelGGL(k_vantidiff, L.grid, L.block, 0, c->stream, wt, cur[0], wt_sav, g,
-                               mt, dt, flag);
+                               mt, dt, flag, (pd && interior_min) ? 1 : 0);
             DCM_LAUNCH_CHECK();
             if (it2 % 2 == 1) {
                 wt = W1;
@@ -840,7 +984,7 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
         if (it > 1 && monotone) {
             it2 += 1;
             hipLaunchKernelGGL(k_vbeta, L.grid, L.block, 0, c->stream, psi, dm, cur[0], wt, bup,
-                               bdn, g, mt, dt, flag);
+                               bdn, g, mt, dt, flag, pd ? 1 : 0);
             DCM_LAUNCH_CHECK();
             DCM_CHECK(fill<double>(c, bup, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
             DCM_CHECK(fill<double>(c, bdn, FILL_ALONG_Z, 0.0, 0.0, nullptr, 0, flag));
@@ -894,6 +1038,15 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
 using namespace dcm;
 
 extern "C" {
+
+int dcm_set_mpdata_variant(dcm_ctx *c, int variant) {
+    if (!c || (variant != DCM_MPDATA_LINKED && variant != DCM_MPDATA_SFVT)) {
+        set_error("dcm_set_mpdata_variant: the variant is DCM_MPDATA_LINKED or DCM_MPDATA_SFVT");
+        return DCM_BAD_ARG;
+    }
+    c->mpdata_variant = variant;
+    return DCM_OK;
+}
 
 int dcm_upwind3d(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
                  double *psi, int neumann) {

@@ -1372,7 +1372,7 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
             // what the captured kernels and their arguments depend on
             const dcm_model *m = e->mem[q];
             key[q] = m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
-                     12 * m->gen;
+                     12 * m->c->mpdata_variant + 24 * m->gen;
             graphs = graphs && !m->c->profiling;
         }
         if (graphs)

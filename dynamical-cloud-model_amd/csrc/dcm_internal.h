@@ -169,6 +169,9 @@ struct dcm_ctx {
     // Chebyshev relaxation of the mixed inner solve's Jacobi preconditioner
     // (set_chebyshev): the factors of its four sweeps, in order
     bool cheb_on = false;
+    // MPDATA arithmetic: 0 pamm/sfvt/advection_3d.F90 (the copy linked into
+    // the model), 1 sfvt/advection_3d.F90 (dcm_set_mpdata_variant)
+    int mpdata_variant = 0;
     double cheb_a = 0.0;
     int cheb_deg = 4;  // a_p divisions per application (the reference's 4)
     double cheb_w[16] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,

@@ -20,6 +20,8 @@ STATUS = {0: "DCM_OK", 1: "DCM_NOT_CONVERGED", 2: "DCM_BREAKDOWN", 3: "DCM_NONFI
           4: "DCM_BAD_ARG", 5: "DCM_HIP_ERROR", 6: "DCM_NO_DEVICE", 7: "DCM_OUT_OF_MEMORY"}
 
 FP64, MIXED = 0, 1
+# the copy of advection_3d.F90 MPDATA follows (dcm_set_mpdata_variant)
+MPDATA_LINKED, MPDATA_SFVT = 0, 1
 SUM_TREE, SUM_SEQUENTIAL = 0, 1
 (F_U, F_V, F_W, F_ZU, F_ZV, F_ZW, F_TH, F_P, F_PSRC, F_SU, F_SV, F_SW, F_STH, F_Q,
  F_THBAR) = range(15)
@@ -137,6 +139,7 @@ _SIGS = {
     "dcm_inn_prod": ([_vp, _vp, _vp, _dp], _i),
     "dcm_set_sum_order": ([_vp, _i], _i),
     "dcm_set_precond_chebyshev": ([_vp, _d, _i], _i),
+    "dcm_set_mpdata_variant": ([_vp, _i], _i),
     "dcm_bicgstab": ([_vp, _dp, _dp, _dp, _vp, _vp, _vp, _d, _i, _i, _ip, _dp], _i),
     "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
     "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
@@ -394,6 +397,11 @@ class Context:
     def set_sum_order(self, order):
         """SUM_TREE (default) or SUM_SEQUENTIAL (the reference's serial order)."""
         _call("dcm_set_sum_order", self.h, order)
+
+    def set_mpdata_variant(self, variant):
+        """MPDATA_LINKED (pamm/sfvt/advection_3d.F90, the default) or
+        MPDATA_SFVT (sfvt/advection_3d.F90's limiter and vector routines)."""
+        _call("dcm_set_mpdata_variant", self.h, int(variant))
 
     def set_precond_chebyshev(self, a, divisions=4):
         """Chebyshev factors on [a, 2] for the mixed inner solve's sweeps, with

@@ -44,6 +44,7 @@ module dcm_hip
         dcm_mpdata3d_add, dcm_upwind_vert3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, &
         dcm_subgrid, dcm_sources_phys, dcm_model_set_physics, &
         dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_set_precond_chebyshev, &
+        dcm_set_mpdata_variant, &
         dcm_bicgstab, &
         dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
@@ -237,6 +238,14 @@ module dcm_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: order
+        end function
+        ! the copy of advection_3d.F90 MPDATA follows: 0 pamm/sfvt (linked
+        ! into main.exe, the default), 1 sfvt
+        integer(c_int) function dcm_set_mpdata_variant(ctx, variant) &
+                bind(c, name="dcm_set_mpdata_variant")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: variant
         end function
         ! the mixed inner solve's preconditioner: Chebyshev factors on [a, 2]
         ! with `divisions` applications of 1/a_p (a = 0: the reference's)

@@ -100,6 +100,19 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
 /* ---------------------------------------------------------------------- */
 /* a2-a5. Advection (pamm/sfvt/advection_3d.F90, module advection_s_3d).  */
 /* ---------------------------------------------------------------------- */
+/* Which copy of advection_3d.F90 the MPDATA routines (dcm_mpdata3d,
+ * dcm_mpdata_vec3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, dcm_mpdata3d_add
+ * and the model's advection) follow on this context.  DCM_MPDATA_LINKED (the
+ * default): pamm/sfvt/advection_3d.F90, the copy linked into the reference's
+ * main.exe.  DCM_MPDATA_SFVT: sfvt/advection_3d.F90 (same interfaces,
+ * replacing that module when it is the one linked), whose limiter bounds
+ * each direction separately with unweighted denominators (:555-682,
+ * :1028-1091) and whose vector routines use unweighted antidiffusive
+ * velocities (:1384-1507, :1874-1909).  A setting, no counterpart call in
+ * the reference, where the choice is made at link time (pamm/Makefile:1-3). */
+#define DCM_MPDATA_LINKED 0
+#define DCM_MPDATA_SFVT 1
+int dcm_set_mpdata_variant(dcm_ctx *ctx, int variant);
 /* first_order_upstream_3d (:39-116): donor cell, psi updated in place. */
 int dcm_upwind3d(dcm_ctx *ctx, double dt, const double *u, const double *v,
                  const double *w, double *psi, int neumann);

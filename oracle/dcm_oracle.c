@@ -221,6 +221,14 @@ void or_upwind3d(const or_grid *g, double dt, const double *u, const double *v,
     free(fxr); free(fxl); free(fyr); free(fyl); free(fzr); free(fzl);
 }
 
+/* Which copy of advection_3d.F90 the MPDATA routines follow.  0: the one
+ * linked into the model, pamm/sfvt/advection_3d.F90.  1: sfvt/advection_3d.F90,
+ * whose limiter bounds each direction separately with unweighted
+ * denominators (:555-682) and whose vector routines use unweighted
+ * antidiffusive velocities (:1384-1507, :1874-1909). */
+static int g_mpdata_variant = 0;
+void or_set_mpdata_variant(int v) { g_mpdata_variant = v; }
+
 /* Antidiffusive velocities, pamm/sfvt/advection_3d.F90:318-506 (identical
  * arithmetic at :1355-1545 for the vector variant). */
 static void antidiffusive(const or_grid *g, double dt, const double *ut,
@@ -325,6 +333,162 @@ static void antidiffusive(const or_grid *g, double dt, const double *ut,
             }
 }
 
+/* Unweighted antidiffusive velocities of sfvt/advection_3d.F90's
+ * mpdata_vec_3d (:1384-1507): eqs 13, 14 and 38 of Smolarkiewicz (1984)
+ * without the density factors. */
+static void antidiffusive_plain(const or_grid *g, double dt, const double *ut,
+                                const double *vt, const double *wt,
+                                const double *po, double *us, double *vs,
+                                double *ws) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
+    const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+#define U(kk, jj, ii) A3(ut, kk, jj, ii)
+#define V(kk, jj, ii) A3(vt, kk, jj, ii)
+#define W(kk, jj, ii) A3(wt, kk, jj, ii)
+#define P(kk, jj, ii) A3(po, kk, jj, ii)
+                /* :1386-1401 */
+                double u_div1 = (((W(k, j, i) + W(k, j, i + 1)) - W(k - 1, j, i)) - W(k - 1, j, i + 1)) / dz[k - 1] +
+                                (((V(k, j, i) + V(k, j, i + 1)) - V(k, j - 1, i)) - V(k, j - 1, i + 1)) / dy[j - 1];
+                double u_div2 = (((W(k, j, i) + W(k, j + 1, i)) - W(k - 1, j, i)) - W(k - 1, j + 1, i)) / dz[k - 1] +
+                                (((U(k, j, i) + U(k, j + 1, i)) - U(k, j, i - 1)) - U(k, j + 1, i - 1)) / dx[i - 1];
+                double u_div3 = (((U(k, j, i) + U(k + 1, j, i)) - U(k, j, i - 1)) - U(k + 1, j, i - 1)) / dx[i - 1] +
+                                (((V(k, j, i) + V(k + 1, j, i)) - V(k, j - 1, i)) - V(k + 1, j - 1, i)) / dy[j - 1];
+                /* :1412-1473 */
+                double tz1 = (((0.25 * (((W(k, j, i + 1) + W(k, j, i)) + W(k - 1, j, i + 1)) + W(k - 1, j, i))) *
+                               (((P(k + 1, j, i + 1) + P(k + 1, j, i)) - P(k - 1, j, i + 1)) - P(k - 1, j, i))) /
+                              ((((P(k + 1, j, i + 1) + P(k + 1, j, i)) + P(k - 1, j, i + 1)) + P(k - 1, j, i)) + SMALL)) /
+                             (0.5 * (dzn[k - 1] + dzn[k]));
+                double ty1 = (((0.25 * (((V(k, j, i + 1) + V(k, j, i)) + V(k, j - 1, i + 1)) + V(k, j - 1, i))) *
+                               (((P(k, j + 1, i + 1) + P(k, j + 1, i)) - P(k, j - 1, i + 1)) - P(k, j - 1, i))) /
+                              ((((P(k, j + 1, i + 1) + P(k, j + 1, i)) + P(k, j - 1, i + 1)) + P(k, j - 1, i)) + SMALL)) /
+                             (0.5 * (dyn[j - 1] + dyn[j]));
+                double ujb1 = ((0.5 * dt) * U(k, j, i)) * (tz1 + ty1);
+                double tz2 = (((0.25 * (((W(k, j + 1, i) + W(k, j, i)) + W(k - 1, j + 1, i)) + W(k - 1, j, i))) *
+                               (((P(k + 1, j + 1, i) + P(k + 1, j, i)) - P(k - 1, j + 1, i)) - P(k - 1, j, i))) /
+                              ((((P(k + 1, j + 1, i) + P(k + 1, j, i)) + P(k - 1, j + 1, i)) + P(k - 1, j, i)) + SMALL)) /
+                             (0.5 * (dzn[k - 1] + dzn[k]));
+                double tx2 = (((0.25 * (((U(k, j + 1, i) + U(k, j, i)) + U(k, j + 1, i - 1)) + U(k, j, i - 1))) *
+                               (((P(k, j + 1, i + 1) + P(k, j, i + 1)) - P(k, j + 1, i - 1)) - P(k, j, i - 1))) /
+                              ((((P(k, j + 1, i + 1) + P(k, j, i + 1)) + P(k, j + 1, i - 1)) + P(k, j, i - 1)) + SMALL)) /
+                             (0.5 * (dxn[i - 1] + dxn[i]));
+                double ujb2 = ((0.5 * dt) * V(k, j, i)) * (tz2 + tx2);
+                double ty3 = (((0.25 * (((V(k + 1, j, i) + V(k, j, i)) + V(k + 1, j - 1, i)) + V(k, j - 1, i))) *
+                               (((P(k + 1, j + 1, i) + P(k, j + 1, i)) - P(k + 1, j - 1, i)) - P(k, j - 1, i))) /
+                              ((((P(k + 1, j + 1, i) + P(k, j + 1, i)) + P(k + 1, j - 1, i)) + P(k, j - 1, i)) + SMALL)) /
+                             (0.5 * (dyn[j - 1] + dyn[j]));
+                double tx3 = (((0.25 * (((U(k + 1, j, i) + U(k, j, i)) + U(k + 1, j, i - 1)) + U(k, j, i - 1))) *
+                               (((P(k + 1, j, i + 1) + P(k, j, i + 1)) - P(k + 1, j, i - 1)) - P(k, j, i - 1))) /
+                              ((((P(k + 1, j, i + 1) + P(k, j, i + 1)) + P(k + 1, j, i - 1)) + P(k, j, i - 1)) + SMALL)) /
+                             (0.5 * (dxn[i - 1] + dxn[i]));
+                double ujb3 = ((0.5 * dt) * W(k, j, i)) * (ty3 + tx3);
+                /* :1482-1493 */
+                const double uc = U(k, j, i), vc = V(k, j, i), wc = W(k, j, i);
+                double usv = ((((fabs(uc) * dx[i]) - ((dt * uc) * uc)) * (P(k, j, i + 1) - P(k, j, i))) /
+                              ((P(k, j, i + 1) + P(k, j, i)) + SMALL)) / dxn[i] - ujb1;
+                double vsv = ((((fabs(vc) * dy[j]) - ((dt * vc) * vc)) * (P(k, j + 1, i) - P(k, j, i))) /
+                              ((P(k, j + 1, i) + P(k, j, i)) + SMALL)) / dyn[j] - ujb2;
+                double wsv = ((((fabs(wc) * dz[k]) - ((dt * wc) * wc)) * (P(k + 1, j, i) - P(k, j, i))) /
+                              ((P(k + 1, j, i) + P(k, j, i)) + SMALL)) / dzn[k] - ujb3;
+                /* :1502-1507 */
+                usv = usv - ((0.25 * dt) * uc) * (((U(k, j, i + 1) - U(k, j, i - 1)) / dx[i - 1]) + u_div1);
+                vsv = vsv - ((0.25 * dt) * vc) * (((V(k, j + 1, i) - V(k, j - 1, i)) / dy[j - 1]) + u_div2);
+                wsv = wsv - ((0.25 * dt) * wc) * (((W(k + 1, j, i) - W(k - 1, j, i)) / dz[k - 1]) + u_div3);
+                A3(us, k, j, i) = usv;
+                A3(vs, k, j, i) = vsv;
+                A3(ws, k, j, i) = wsv;
+#undef U
+#undef V
+#undef W
+#undef P
+            }
+}
+
+/* eq 18 of Smolarkiewicz & Grabowski (1990) for one face */
+static inline double limited_face(double x, double bdn_c, double bup_c, double bup_n,
+                                  double bdn_n) {
+    return min3(1.0, bdn_c, bup_n) * pos0(x) + min3(1.0, bup_c, bdn_n) * neg0(x);
+}
+
+/* Per-direction limiter of sfvt/advection_3d.F90:555-682 (vector variant
+ * :1561-1688): bounds from the three cells along each axis, one pair of
+ * unweighted denominators (grid spacings dx(i-1), dy(j-1), dz(k-1), small
+ * inside the dt product) shared by the three directions. */
+static void limiter_directional(const or_grid *g, double dt, const double *ut,
+                                const double *vt, const double *wt, const double *psi,
+                                const double *po, double *us, double *vs, double *ws) {
+    const int ip = g->ip, jp = g->jp, kp = g->kp;
+    const size_t N = NTOT(g);
+    const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
+    double *bu[3], *bd[3];
+    for (int d = 0; d < 3; ++d) {
+        bu[d] = zalloc(N);
+        bd[d] = zalloc(N);
+    }
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                /* eqs 20a,b per direction: :555-591 */
+                const int off[3][3] = {{0, 0, 1}, {0, 1, 0}, {1, 0, 0}}; /* (k,j,i) steps */
+                double mx[3], mn[3];
+                for (int d = 0; d < 3; ++d) {
+                    const int sk = off[d][0], sj = off[d][1], si = off[d][2];
+                    const double vals[6] = {
+                        A3(psi, k - sk, j - sj, i - si), A3(psi, k, j, i),
+                        A3(psi, k + sk, j + sj, i + si), A3(po, k - sk, j - sj, i - si),
+                        A3(po, k, j, i),                 A3(po, k + sk, j + sj, i + si)};
+                    mx[d] = vals[0];
+                    mn[d] = vals[0];
+                    for (int q = 1; q < 6; ++q) {
+                        if (vals[q] > mx[d]) mx[d] = vals[q];
+                        if (vals[q] < mn[d]) mn[d] = vals[q];
+                    }
+                }
+                /* eqs 19a,b: :608-636 */
+                const double pc = A3(po, k, j, i);
+                const double d1 =
+                    dt * ((((pos0(A3(ut, k, j, i - 1)) * A3(po, k, j, i - 1) -
+                             neg0(A3(ut, k, j, i)) * A3(po, k, j, i + 1)) / dx[i - 1] +
+                            (pos0(A3(vt, k, j - 1, i)) * A3(po, k, j - 1, i) -
+                             neg0(A3(vt, k, j, i)) * A3(po, k, j + 1, i)) / dy[j - 1]) +
+                           (pos0(A3(wt, k - 1, j, i)) * A3(po, k - 1, j, i) -
+                            neg0(A3(wt, k, j, i)) * A3(po, k + 1, j, i)) / dz[k - 1]) +
+                          SMALL);
+                const double d2 =
+                    dt * ((((pos0(A3(ut, k, j, i)) * pc - neg0(A3(ut, k, j, i - 1)) * pc) / dx[i - 1] +
+                            (pos0(A3(vt, k, j, i)) * pc - neg0(A3(vt, k, j - 1, i)) * pc) / dy[j - 1]) +
+                           (pos0(A3(wt, k, j, i)) * pc - neg0(A3(wt, k - 1, j, i)) * pc) / dz[k - 1]) +
+                          SMALL);
+                for (int d = 0; d < 3; ++d) {
+                    A3(bu[d], k, j, i) = (mx[d] - pc) / d1;
+                    A3(bd[d], k, j, i) = (pc - mn[d]) / d2;
+                }
+            }
+    /* :649-659 */
+    for (int d = 0; d < 3; ++d) {
+        or_fill_along_dim(g, bu[d], 0.0, 0.0);
+        or_fill_along_dim(g, bd[d], 0.0, 0.0);
+    }
+    /* eq 18: :666-682 */
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                A3(us, k, j, i) = limited_face(A3(ut, k, j, i), A3(bd[0], k, j, i), A3(bu[0], k, j, i),
+                                               A3(bu[0], k, j, i + 1), A3(bd[0], k, j, i + 1));
+                A3(vs, k, j, i) = limited_face(A3(vt, k, j, i), A3(bd[1], k, j, i), A3(bu[1], k, j, i),
+                                               A3(bu[1], k, j + 1, i), A3(bd[1], k, j + 1, i));
+                A3(ws, k, j, i) = limited_face(A3(wt, k, j, i), A3(bd[2], k, j, i), A3(bu[2], k, j, i),
+                                               A3(bu[2], k + 1, j, i), A3(bd[2], k + 1, j, i));
+            }
+    for (int d = 0; d < 3; ++d) {
+        free(bu[d]);
+        free(bd[d]);
+    }
+}
+
 /* Non-oscillatory limiter, pamm/sfvt/advection_3d.F90:542-675 (vector variant
  * :1576-1708).  psi = psi_store (start-of-step field), po = psi_old. */
 static void limiter(const or_grid *g, double dt, const double *ut,
@@ -399,7 +563,8 @@ static void limiter(const or_grid *g, double dt, const double *ut,
 static void mpdata_core(const or_grid *g, double dt, const double *u,
                         const double *v, const double *w, double *fields,
                         int nf, const double *lbc, const double *ubc, int kord,
-                        int monotone, int neumann) {
+                        int monotone, int neumann, int vec) {
+    const int plain = vec && g_mpdata_variant == 1;
     const size_t N = NTOT(g);
     double *us1 = zalloc(N), *vs1 = zalloc(N), *ws1 = zalloc(N);
     double *us2 = zalloc(N), *vs2 = zalloc(N), *ws2 = zalloc(N); /* zeroed :296-298 */
@@ -415,7 +580,10 @@ static void mpdata_core(const or_grid *g, double dt, const double *u,
     for (int it = 1; it <= kord; ++it) {
         if (it > 1) {
             it2 += 1;
-            antidiffusive(g, dt, ut, vt, wt, psi_old, ut_sav, vt_sav, wt_sav);
+            if (plain)
+                antidiffusive_plain(g, dt, ut, vt, wt, psi_old, ut_sav, vt_sav, wt_sav);
+            else
+                antidiffusive(g, dt, ut, vt, wt, psi_old, ut_sav, vt_sav, wt_sav);
             if (it2 % 2 == 1) {
                 ut = us1; vt = vs1; wt = ws1;
                 ut_sav = us2; vt_sav = vs2; wt_sav = ws2;
@@ -429,7 +597,10 @@ static void mpdata_core(const or_grid *g, double dt, const double *u,
         }
         if (it > 1 && monotone) {
             it2 += 1;
-            limiter(g, dt, ut, vt, wt, psi_store, psi_old, ut_sav, vt_sav, wt_sav);
+            if (g_mpdata_variant == 1)
+                limiter_directional(g, dt, ut, vt, wt, psi_store, psi_old, ut_sav, vt_sav, wt_sav);
+            else
+                limiter(g, dt, ut, vt, wt, psi_store, psi_old, ut_sav, vt_sav, wt_sav);
             ut = us2; vt = vs2; wt = ws2;
             ut_sav = us1; vt_sav = vs1; wt_sav = ws1;
             or_fill_full(g, (double *)ut, 0.0, 0.0);
@@ -472,7 +643,7 @@ int or_mpdata3d(const or_grid *g, double dt, const double *u, const double *v,
         *ubc = *ubc + m;
         return 1;
     }
-    mpdata_core(g, dt, u, v, w, psi, 1, lbc, ubc, kord, monotone, neumann);
+    mpdata_core(g, dt, u, v, w, psi, 1, lbc, ubc, kord, monotone, neumann, 0);
     for (size_t q = 0; q < N; ++q) psi[q] = psi[q] + m; /* :711-713 */
     *lbc = *lbc + m;
     *ubc = *ubc + m;
@@ -517,7 +688,7 @@ int or_mpdata_vec3d(const or_grid *g, double dt, const double *u,
         free(mg);
         return 1;
     }
-    mpdata_core(g, dt, u, v, w, psi, nq, lbc, ubc, kord, monotone, neumann);
+    mpdata_core(g, dt, u, v, w, psi, nq, lbc, ubc, kord, monotone, neumann, 1);
     for (int n = 0; n < nq; ++n) {
         double *f = psi + (size_t)n * N;
         for (size_t q = 0; q < N; ++q) f[q] = f[q] + mg[n];
@@ -573,7 +744,7 @@ void or_upwind_vert3d(const or_grid *g, double dt, const double *w, double *psi,
  * (vector variant :1905-1921): eq 13 of Smolarkiewicz (1984) with the
  * cross terms zero and the divergence term of eq 38. */
 static void antidiffusive_vert(const or_grid *g, double dt, const double *wt,
-                               const double *po, double *ws) {
+                               const double *po, double *ws, int plain) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     const double *dz = g->dz, *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
     for (int i = 1; i <= ip; ++i)
@@ -583,11 +754,19 @@ static void antidiffusive_vert(const or_grid *g, double dt, const double *wt,
                 const double u_div3 = 0.0, u_j_bar3 = 0.0;
                 const double wc = A3(wt, k, j, i);
                 const double p0 = A3(po, k, j, i), p1 = A3(po, k + 1, j, i);
-                double v = ((((fabs(wc) * dz[k] - ((dt * (ra[k] / gb3)) * wc) * wc) * (p1 - p0)) /
-                             ((p1 + p0) + SMALL)) / dzn[k]) - u_j_bar3;
-                v = v - ((0.25 * dt) * wc) *
-                            (((ra[k + 1] * A3(wt, k + 1, j, i) - ra[k - 1] * A3(wt, k - 1, j, i)) /
-                              (dz[k - 1] * gb3)) + u_div3);
+                double v;
+                if (plain) { /* sfvt/advection_3d.F90:1897-1909 */
+                    v = ((((fabs(wc) * dz[k] - (dt * wc) * wc) * (p1 - p0)) / ((p1 + p0) + SMALL)) /
+                         dzn[k]) - u_j_bar3;
+                    v = v - ((0.25 * dt) * wc) *
+                                (((A3(wt, k + 1, j, i) - A3(wt, k - 1, j, i)) / dz[k - 1]) + u_div3);
+                } else {
+                    v = ((((fabs(wc) * dz[k] - ((dt * (ra[k] / gb3)) * wc) * wc) * (p1 - p0)) /
+                          ((p1 + p0) + SMALL)) / dzn[k]) - u_j_bar3;
+                    v = v - ((0.25 * dt) * wc) *
+                                (((ra[k + 1] * A3(wt, k + 1, j, i) - ra[k - 1] * A3(wt, k - 1, j, i)) /
+                                  (dz[k - 1] * gb3)) + u_div3);
+                }
                 A3(ws, k, j, i) = v;
             }
 }
@@ -598,7 +777,7 @@ static void limiter_vert(const or_grid *g, double dt, const double *wt,
                          const double *psi, const double *po, double *ws) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     const size_t N = NTOT(g);
-    const double *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
+    const double *dz = g->dz, *dzn = g->dzn, *ra = g->rhoa, *rn = g->rhoan;
     double *bup = zalloc(N), *bdn = zalloc(N);
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
@@ -613,6 +792,14 @@ static void limiter_vert(const or_grid *g, double dt, const double *wt,
                 }
                 const double pc = A3(po, k, j, i);
                 const double wm = A3(wt, k - 1, j, i), wc = A3(wt, k, j, i);
+                if (g_mpdata_variant == 1) { /* sfvt/advection_3d.F90:1051-1062 */
+                    const double d1 = dt * ((pos0(wm) * A3(po, k - 1, j, i) -
+                                             neg0(wc) * A3(po, k + 1, j, i)) / dz[k - 1] + SMALL);
+                    const double d2 = dt * ((pos0(wc) * pc - neg0(wm) * pc) / dz[k - 1] + SMALL);
+                    A3(bup, k, j, i) = (mx - pc) / d1;
+                    A3(bdn, k, j, i) = (pc - mn) / d2;
+                    continue;
+                }
                 const double d1 = (dt * ((pos0(wm) * ra[k - 1]) * A3(po, k - 1, j, i) -
                                          (neg0(wc) * ra[k]) * A3(po, k + 1, j, i))) / dzn[k] +
                                   SMALL;
@@ -638,7 +825,8 @@ static void limiter_vert(const or_grid *g, double dt, const double *wt,
  * (:1859-2071): fields[0] supplies psi_old and psi_store. */
 static void mpdata_vert_core(const or_grid *g, double dt, const double *w, double *fields,
                              int nf, const double *lbc, const double *ubc, int kord,
-                             int monotone, int neumann) {
+                             int monotone, int neumann, int vec) {
+    const int plain = vec && g_mpdata_variant == 1;
     const size_t N = NTOT(g);
     double *ws1 = zalloc(N), *ws2 = zalloc(N); /* w_store2 zeroed :876 */
     double *psi_store = NULL;
@@ -652,7 +840,7 @@ static void mpdata_vert_core(const or_grid *g, double dt, const double *w, doubl
     for (int it = 1; it <= kord; ++it) {
         if (it > 1) {
             it2 += 1;
-            antidiffusive_vert(g, dt, wt, fields, wt_sav);
+            antidiffusive_vert(g, dt, wt, fields, wt_sav, plain);
             if (it2 % 2 == 1) {
                 wt = ws1;
                 wt_sav = ws2;
@@ -704,7 +892,7 @@ int or_mpdata_vert3d(const or_grid *g, double dt, const double *w, double *psi,
         *ubc = *ubc + m;
         return 1;
     }
-    mpdata_vert_core(g, dt, w, psi, 1, lbc, ubc, kord, monotone, neumann);
+    mpdata_vert_core(g, dt, w, psi, 1, lbc, ubc, kord, monotone, neumann, 0);
     for (size_t q = 0; q < N; ++q) psi[q] = psi[q] + m; /* :1123-1125 */
     *lbc = *lbc + m;
     *ubc = *ubc + m;
@@ -746,7 +934,7 @@ int or_mpdata_vec_vert3d(const or_grid *g, double dt, const double *w, double *p
         free(mg);
         return 1;
     }
-    mpdata_vert_core(g, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann);
+    mpdata_vert_core(g, dt, w, psi, nq, lbc, ubc, kord, monotone, neumann, 1);
     for (int n = 0; n < nq; ++n) {
         double *f = psi + (size_t)n * N;
         for (size_t q = 0; q < N; ++q) f[q] = f[q] + mg[n];

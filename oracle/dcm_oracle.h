@@ -129,6 +129,8 @@ void or_divergence(const or_grid *g, const double *u, const double *v,
 
 /* 1: blocked summation in or_inn_prod (sensitivity measurements only) */
 void or_set_sum_blocked(int on);
+/* 0: pamm/sfvt/advection_3d.F90 (linked into the model), 1: sfvt/advection_3d.F90 */
+void or_set_mpdata_variant(int v);
 
 void or_advance_momentum(const or_grid *g, double dt, double *u, double *v,
                          double *w, const double *zu, const double *zv,

@@ -135,6 +135,7 @@ def lib():
         L.or_advance_momentum.argtypes = [g, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_divergence.argtypes = [g, _dp, _dp, _dp, _dp]
         L.or_set_sum_blocked.argtypes = [C.c_int]
+        L.or_set_mpdata_variant.argtypes = [C.c_int]
         L.or_advance_scalars.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
         m = C.POINTER(_Model)
         L.or_model_init_bcs.argtypes = [g, m]
@@ -296,6 +297,13 @@ def bicgstab(g: Grid, sw, x, b, tol=1e-10, test_solver=False, first_call=True,
                            ptr(b), tol, int(test_solver), int(first_call), itmax,
                            C.byref(it), C.byref(err))
     return st, it.value, err.value
+
+
+def set_mpdata_variant(v: int):
+    """0: pamm/sfvt/advection_3d.F90 (the copy linked into the model);
+    1: sfvt/advection_3d.F90 (per-direction limiter, unweighted vector
+    antidiffusive velocities)."""
+    lib().or_set_mpdata_variant(int(v))
 
 
 def set_sum_blocked(on: bool):

@@ -176,10 +176,13 @@ def _shift(a, axis, n):
     return np.roll(a, -n, axis=axis)
 
 
-def textbook_mpdata(psi, U, V, Wz, dx, dy, dz, dt, limiter=True):
+def textbook_mpdata(psi, U, V, Wz, dx, dy, dz, dt, limiter=True, directional=False):
     """One MPDATA step (kord=2) for a field on a periodic box padded with
     zeros in z.  U,V,Wz are face velocities (face i+1/2 stored at i), psi >= 0.
-    Courant numbers: c = u dt / dx etc.  Returns the new field."""
+    Courant numbers: c = u dt / dx etc.  Returns the new field.
+    directional: the limiter's bounds of each axis come from the three cells
+    along that axis (one pair of beta per direction), with the same in- and
+    outflow denominators."""
     def donor(f, cx, cy, cz):
         def flux(cf, fl, fr):
             return np.maximum(cf, 0) * fl + np.minimum(cf, 0) * fr
@@ -239,14 +242,137 @@ def textbook_mpdata(psi, U, V, Wz, dx, dy, dz, dt, limiter=True):
         outflow = (np.maximum(ax_, 0) - np.minimum(_shift(ax_, 0, -1), 0)
                    + np.maximum(ay_, 0) - np.minimum(_shift(ay_, 1, -1), 0)
                    + np.maximum(az_, 0) - np.minimum(_shift(az_, 2, -1), 0)) * f
-        bup = (mx - f) / (inflow + EPS)
-        bdn = (f - mn) / (outflow + EPS)
+        bup = [(mx - f) / (inflow + EPS)] * 3
+        bdn = [(f - mn) / (outflow + EPS)] * 3
+        if directional:
+            for ax in range(3):
+                mxd = np.maximum.reduce([np.maximum.reduce([q, _shift(q, ax, 1), _shift(q, ax, -1)])
+                                         for q in psi_pairs])
+                mnd = np.minimum.reduce([np.minimum.reduce([q, _shift(q, ax, 1), _shift(q, ax, -1)])
+                                         for q in psi_pairs])
+                bup[ax] = (mxd - f) / (inflow + EPS)
+                bdn[ax] = (f - mnd) / (outflow + EPS)
         lim = []
         for ax, c in enumerate(anti):
-            lim.append(np.minimum.reduce([np.ones_like(c), bdn, _shift(bup, ax, 1)]) * np.maximum(c, 0)
-                       + np.minimum.reduce([np.ones_like(c), bup, _shift(bdn, ax, 1)]) * np.minimum(c, 0))
+            bu, bd = bup[ax], bdn[ax]
+            lim.append(np.minimum.reduce([np.ones_like(c), bd, _shift(bu, ax, 1)]) * np.maximum(c, 0)
+                       + np.minimum.reduce([np.ones_like(c), bu, _shift(bd, ax, 1)]) * np.minimum(c, 0))
         anti = lim
     return donor(p1, *anti)
+
+
+def _textbook_case(seed=5):
+    """A constant-density uniform box, a field >= 0 that vanishes near floor
+    and lid, random face velocities with w = 0 near floor and lid; returns
+    the oracle grid, its inputs and the textbook's padded arrays."""
+    ip, jp, kp = 10, 8, 9
+    d = 32.0
+    dom = W.make_domain(ip, jp, kp, d, d, d)
+    dom.rhoa = np.ones(kp + 2)
+    dom.rhoan = np.ones(kp + 2)
+    g = O.Grid(dom)
+    r = np.random.default_rng(seed)
+    psi = np.zeros(dom.shape)
+    psi[1:-1, 1:-1, 3:-3] = r.random((ip, jp, kp - 4))
+    O.fill_full(g, psi)
+    u, v, w = (np.zeros(dom.shape) for _ in range(3))
+    u[1:-1, 1:-1, 1:-1] = 0.3 * d * (r.random((ip, jp, kp)) - 0.5)
+    v[1:-1, 1:-1, 1:-1] = 0.3 * d * (r.random((ip, jp, kp)) - 0.5)
+    w[1:-1, 1:-1, 2:-3] = 0.3 * d * (r.random((ip, jp, kp - 3)) - 0.5)
+    for a in (u, v, w):
+        O.fill_full(g, a)
+
+    def pad(a):
+        F = np.zeros((ip, jp, kp + 2))
+        F[:, :, 1:-1] = interior(a)
+        return F
+    return g, d, (u, v, w, psi), (pad(u), pad(v), pad(w), pad(psi))
+
+
+@pytest.fixture
+def sfvt_variant():
+    O.set_mpdata_variant(1)
+    yield
+    O.set_mpdata_variant(0)
+
+
+@pytest.mark.parametrize("vec", [False, True])
+@pytest.mark.parametrize("limiter", [False, True])
+def test_sfvt_mpdata_matches_textbook(sfvt_variant, limiter, vec):
+    """sfvt/advection_3d.F90 on a constant-density uniform box: the
+    per-direction limiter (:555-682) and, in the vector routine, the
+    unweighted antidiffusive velocities (:1384-1507) against the textbook
+    with per-direction bounds; the box is built so that the textbook tells
+    the two limiters apart."""
+    g, d, (u, v, w, psi), (U, V, Wz, P) = _textbook_case()
+    got = psi.copy()[None]
+    if vec:
+        O.mpdata_vec3d(g, 1.0, u, v, w, got, [0.0], [0.0], 2, limiter, 0)
+    else:
+        O.mpdata3d(g, 1.0, u, v, w, got[0], 0.0, 0.0, 2, limiter, 0)
+    want = textbook_mpdata(P, U, V, Wz, d, d, d, 1.0, limiter, directional=True)[:, :, 1:-1]
+    np.testing.assert_allclose(interior(got[0]), want, rtol=1e-12, atol=1e-13)
+    if limiter:
+        other = textbook_mpdata(P, U, V, Wz, d, d, d, 1.0, True)[:, :, 1:-1]
+        assert np.abs(other - want).max() > 1e-6
+
+
+def test_sfvt_switch_scope(sfvt_variant):
+    """Without the limiter sfvt's mpdata_3d is the linked copy's bit for bit
+    (the two differ only in the limiter); its vector routine drops the
+    density weights, so it differs where density varies."""
+    dom = small_domain(9, 7, 6, stretched=True, varying_theta=True)
+    g = O.Grid(dom)
+    psi = field(dom, 21, 0.5, 2.0)
+    O.fill_full(g, psi, 0.7, 0.8)
+    u, v, w = (field(dom, 22 + n, -1, 1) * 5 for n in range(3))
+    for a in (u, v, w):
+        O.fill_full(g, a)
+    out = {}
+    for variant in (0, 1):
+        O.set_mpdata_variant(variant)
+        a = psi.copy()
+        O.mpdata3d(g, 1.0, u, v, w, a, 0.7, 0.8, 2, False, 0)
+        b = psi[None].copy()
+        O.mpdata_vec3d(g, 1.0, u, v, w, b, [0.7], [0.8], 2, False, 0)
+        c = psi.copy()
+        O.mpdata3d(g, 1.0, u, v, w, c, 0.7, 0.8, 2, True, 0)
+        out[variant] = (a, b[0], c)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert rel_l2(out[0][1], out[1][1]) > 1e-8
+    assert rel_l2(out[0][2], out[1][2]) > 1e-8
+
+
+@pytest.mark.parametrize("routine", ["vert", "vec_vert"])
+def test_sfvt_vertical_is_the_linked_copy_at_constant_density(sfvt_variant, routine):
+    """With unit density on a uniform grid the limiter denominators of
+    sfvt/advection_3d.F90 (:1051-1062) and the unweighted vertical
+    antidiffusive velocity (:1897-1909) reduce to the linked copy's, which
+    the vertical textbook tests pin; with varying density they do not."""
+    for unit in (True, False):
+        dom = small_domain(7, 6, 12, varying_theta=True)
+        if unit:
+            dom.rhoa = np.ones(dom.kp + 2)
+            dom.rhoan = np.ones(dom.kp + 2)
+        g = O.Grid(dom)
+        q = np.stack([field(dom, 40 + n, 0.0, 1.0) for n in range(2)])
+        for n in range(2):
+            O.fill_along_z(g, q[n], 0.0, 0.0)
+        w = field(dom, 43, -1, 1) * 0.4 * dom.dz[1]
+        O.fill_along_z(g, w, 0.0, 0.0)
+        res = []
+        for variant in (0, 1):
+            O.set_mpdata_variant(variant)
+            a = q.copy()
+            if routine == "vert":
+                O.mpdata_vert3d(g, 1.0, w, a[0], 0.0, 0.0, 2, True, 0)
+            else:
+                O.mpdata_vec_vert3d(g, 1.0, w, a, [0.0, 0.0], [0.0, 0.0], 2, True, 0)
+            res.append(a)
+        if unit:
+            np.testing.assert_allclose(res[1], res[0], rtol=1e-13, atol=1e-15)
+        else:
+            assert rel_l2(res[0], res[1]) > 1e-10
 
 
 @pytest.mark.parametrize("limiter", [False, True])

@@ -247,6 +247,90 @@ def sedimentation(H, W, calls=20, cpu=True):
     return out
 
 
+def mpdata_variants(H, W, calls=10, model_steps=6, cpu=True):
+    """The two copies of advection_3d.F90 side by side on the 256x256x128
+    moist-bubble grid: mpdata_3d of theta (kord 2, limiter) and
+    mpdata_vec_3d of a three-field category, FP64 through the C-ABI, with
+    the linked copy's arithmetic and with sfvt's (per-direction limiter:
+    six limiter fields instead of two).  Canonical bytes per cell: read the
+    fields and u, v, w, write the fields.  Also the mixed model step under
+    sfvt's arithmetic."""
+    case = W.moist_bubble(256, 256, 128, steps=model_steps)
+    dom = case.dom
+    rng = np.random.default_rng(707)
+    th = np.zeros(dom.shape)
+    th[1:-1, 1:-1, 1:-1] = rng.random((256, 256, 128))
+    q = np.zeros((3,) + dom.shape)
+    q[:, 1:-1, 1:-1, 1:-1] = 1e-3 * rng.random((3, 256, 256, 128))
+    vel = [np.zeros(dom.shape) for _ in range(3)]
+    for a, d in zip(vel, (dom.dx[1], dom.dy[1], dom.dz[1])):
+        a[1:-1, 1:-1, 1:-1] = 0.6 * d * (rng.random((256, 256, 128)) - 0.5)
+    cells = dom.cells
+    out = {"grid": "256x256x128", "kord": 2, "monotone": True}
+    for name, variant in (("linked", H.MPDATA_LINKED), ("sfvt", H.MPDATA_SFVT)):
+        with H.Context(dom) as c:
+            c.set_mpdata_variant(variant)
+            du, dv, dw = (c.array(a) for a in vel)
+            dth, dq = c.array(th), c.array(q)
+            res = {}
+            for call, fn, nf in (("mpdata_3d", lambda: c.mpdata3d(1.0, du, dv, dw, dth), 1),
+                                 ("mpdata_vec_3d, 3 fields",
+                                  lambda: c.mpdata_vec3d(1.0, du, dv, dw, dq, 3), 3)):
+                fn()
+                c.sync()
+                t0 = time.perf_counter()
+                for _ in range(calls):
+                    fn()
+                c.sync()
+                wall = (time.perf_counter() - t0) / calls
+                res[call] = {"ms_per_call": 1e3 * wall, "cell_steps_per_s": cells / wall,
+                             "canonical_GBps": 8.0 * (2 * nf + 3) * cells / wall / 1e9}
+            out[name] = res
+    for call in out["linked"]:
+        out["sfvt"][call]["time_vs_linked"] = (out["sfvt"][call]["ms_per_call"] /
+                                               out["linked"][call]["ms_per_call"])
+    with H.Context(dom) as c:
+        c.set_mpdata_variant(H.MPDATA_SFVT)
+        c.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+        m = H.Model(c, case, H.MIXED)
+        try:
+            m.step(2)
+            c.sync()
+            t0 = time.perf_counter()
+            st = m.step(model_steps)
+            c.sync()
+            wall = (time.perf_counter() - t0) / model_steps
+        finally:
+            m.close()
+    out["sfvt_mixed_model"] = {"ms_per_step": 1e3 * wall, "value": cells / wall,
+                               "unit": "cell-steps/s", "max_true_resid": st.max_true_resid}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.build()
+        O.set_mpdata_variant(1)
+        try:
+            sd = W.make_domain(64, 64, 64, 50.0, 50.0, 50.0)
+            g = O.Grid(sd)
+            sp = np.zeros(sd.shape)
+            sp[1:-1, 1:-1, 1:-1] = rng.random((64, 64, 64))
+            sv = [np.zeros(sd.shape) for _ in range(3)]
+            for a in sv:
+                a[1:-1, 1:-1, 1:-1] = 0.6 * 50.0 * (rng.random((64, 64, 64)) - 0.5)
+            n = 5
+            t0 = time.perf_counter()
+            for _ in range(n):
+                O.mpdata3d(g, 1.0, sv[0], sv[1], sv[2], sp)
+            cw = (time.perf_counter() - t0) / n
+        finally:
+            O.set_mpdata_variant(0)
+        out["cpu_baseline"] = {"value": sd.cells / cw, "unit": "cell-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle, sfvt mpdata_3d on 64x64x64, {n} calls, "
+                                         f"{n * cw:.2f} s"}
+    return out
+
+
 def physics(H, W, steps, warmup=2, base=None, cpu=True):
     """The sheared moist bubble with every option of sources but radiation
     (subgrid viscosity and surface layer, damping layer, forcing,
@@ -678,6 +762,9 @@ def main(argv=None):
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
     print("[bench] sedimentation (vertical MPDATA) 256x256x128", file=sys.stderr, flush=True)
     line["sedimentation"] = sedimentation(H, W, cpu=not args.no_cpu)
+    print("[bench] MPDATA of sfvt/advection_3d.F90 beside the linked copy, 256x256x128",
+          file=sys.stderr, flush=True)
+    line["mpdata_variants"] = mpdata_variants(H, W, cpu=not args.no_cpu)
     if args.physics_steps > 0:
         print(f"[bench] physics options: sheared moist bubble 256x256x128, "
               f"{args.physics_steps} steps", file=sys.stderr, flush=True)

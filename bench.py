@@ -644,6 +644,8 @@ def main(argv=None):
                     help="skip the mixed path with the reference's plain Jacobi sweeps")
     ap.add_argument("--cpu-sample", default="64x64:1")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--mpdata-variant-calls", type=int, default=10,
+                    help="calls per MPDATA variant and routine (0: skip that leg)")
     ap.add_argument("--ensemble-steps", type=int, default=10,
                     help="timed steps of the 64-member ensemble comparison (0: skip)")
     ap.add_argument("--physics-steps", type=int, default=10,
@@ -762,9 +764,11 @@ def main(argv=None):
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
     print("[bench] sedimentation (vertical MPDATA) 256x256x128", file=sys.stderr, flush=True)
     line["sedimentation"] = sedimentation(H, W, cpu=not args.no_cpu)
-    print("[bench] MPDATA of sfvt/advection_3d.F90 beside the linked copy, 256x256x128",
-          file=sys.stderr, flush=True)
-    line["mpdata_variants"] = mpdata_variants(H, W, cpu=not args.no_cpu)
+    if args.mpdata_variant_calls > 0:
+        print("[bench] MPDATA of sfvt/advection_3d.F90 beside the linked copy, 256x256x128",
+              file=sys.stderr, flush=True)
+        line["mpdata_variants"] = mpdata_variants(H, W, calls=args.mpdata_variant_calls,
+                                                  cpu=not args.no_cpu)
     if args.physics_steps > 0:
         print(f"[bench] physics options: sheared moist bubble 256x256x128, "
               f"{args.physics_steps} steps", file=sys.stderr, flush=True)

@@ -38,6 +38,12 @@ METRIC = "cell-steps/s mixed vs FP64 at 1e-10 pressure resid; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 DOMINANT = "jac"         # Jacobi sweep kernel k_jac<R,false>: s' = s + (t - A s)/a_p
 JAC_WORDS = 3            # reads s and t, writes s'
+# the headline mixed path preconditions its FP32 inner solve with a multigrid
+# V-cycle whose finest-level sweeps are the same k_jac<float,false,true>
+# (weight 0.8), five of them per inner iteration ("jac"; the sixth updates p)
+MG_DOMINANT = DOMINANT
+MG_SWEEPS_PER_ITER = 5
+MG_WORDS_PER_ITER = 51.7 # FP32 words per cell of one inner iteration (canonical)
 PROFILE_STEPS = 2        # steps whose dominant-kernel launches are timed
 # the mixed inner solve's preconditioner: Chebyshev factors on [CHEB_A, 2]
 # with CHEB_DIVISIONS divisions by a_p (the library's default, set explicitly)
@@ -55,7 +61,7 @@ def canonical_bytes_fp64(F, iters):
     return 8.0 * (39 + 30 * F + 37 * iters)
 
 
-def canonical_bytes_mixed(F, inner_iters, outer, divisions=4):
+def canonical_bytes_mixed(F, inner_iters, outer, divisions=4, multigrid=False):
     """Same stages with each word at its storage precision (bytes/cell-step).
 
     FP64: state (u,v,w,theta,q,p,b,x), sources, RHS, the true residual and
@@ -66,7 +72,14 @@ def canonical_bytes_mixed(F, inner_iters, outer, divisions=4):
     b += outer * (20 + 24 + 20)          # true residual, inner start, x += d
     # 37 FP32 words per inner iteration with the reference's four divisions,
     # each further division one more sweep (3 words) in both applications
-    b += 4 * (37 + 6 * (divisions - 4)) * inner_iters
+    if multigrid:
+        # two preparations (4 words each, the first sweep included), two
+        # V-cycles (three sweeps of 3 words, the restriction's 2 reads and the
+        # prolongation's read and write on the finest level: 13 words, 8/7
+        # of that over all levels), two matvec-dots (6), the update (8)
+        b += 4 * MG_WORDS_PER_ITER * inner_iters
+    else:
+        b += 4 * (37 + 6 * (divisions - 4)) * inner_iters
     b += 24 + 36                         # adv_ref_state, FP32 velocity shadows
     b += F * (8 + 24 + 28 + 32 + 32)     # MPDATA per field (min, 4 passes)
     b += F * 24                          # scalar update (canonical; an exact no-op)
@@ -74,13 +87,19 @@ def canonical_bytes_mixed(F, inner_iters, outer, divisions=4):
     return float(b)
 
 
-def run_path(H, case, precision, warmup, steps, profile_only=DOMINANT, inner_rtol=None,
-             plain_sweeps=False):
+def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebyshev"):
+    """precond (mixed path): "multigrid" (one V-cycle per application),
+    "chebyshev" (Chebyshev-relaxed Jacobi sweeps) or "plain" (the
+    reference's four Jacobi sweeps)."""
     ctx = H.Context(case.dom)
-    if plain_sweeps:
+    profile_only = DOMINANT
+    if precond == "plain":
         ctx.set_precond_chebyshev(0.0)  # the reference's four plain Jacobi sweeps
     else:
         ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+    if precond == "multigrid" and precision == H.MIXED:
+        ctx.set_precond_multigrid(True)
+        profile_only = MG_DOMINANT
     model = H.Model(ctx, case, precision)
     model.set_check_residual(True)
     if inner_rtol:
@@ -292,6 +311,7 @@ def mpdata_variants(H, W, calls=10, model_steps=6, cpu=True):
     with H.Context(dom) as c:
         c.set_mpdata_variant(H.MPDATA_SFVT)
         c.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+        c.set_precond_multigrid(True)
         m = H.Model(c, case, H.MIXED)
         try:
             m.step(2)
@@ -345,7 +365,7 @@ def physics(H, W, steps, warmup=2, base=None, cpu=True):
     out = {"workload": case.name, "options": "viscous, damping_layer, forcing, divergence",
            "fields": 1 + case.nq}
     for pth, prec in (("mixed", H.MIXED), ("fp64", H.FP64)):
-        res, _, handles = run_path(H, case, prec, warmup, steps)
+        res, _, handles = run_path(H, case, prec, warmup, steps, precond="multigrid")
         handles[1].close()
         handles[0].close()
         out[pth] = {k: res[k] for k in ("value", "ms_per_step", "iters_per_step",
@@ -621,10 +641,10 @@ def stream_copy_gbps(H, dom):
     return 2.0 * n * 8 / (best * 1e-3) / 1e9
 
 
-def load_traffic(workload):
+def load_traffic(workload, kernel=DOMINANT):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{DOMINANT}_{workload}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -640,6 +660,8 @@ def main(argv=None):
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--inner-rtol", type=float, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chebyshev-path", action="store_true",
+                    help="skip the mixed path with Chebyshev-relaxed sweeps")
     ap.add_argument("--no-plain-sweeps", action="store_true",
                     help="skip the mixed path with the reference's plain Jacobi sweeps")
     ap.add_argument("--cpu-sample", default="64x64:1")
@@ -681,7 +703,8 @@ def main(argv=None):
         print(f"[bench] {pth}: warmup {args.warmup}, timed {args.steps} steps", file=sys.stderr,
               flush=True)
         res, snap, handles = run_path(H, case, prec, args.warmup, args.steps,
-                                      inner_rtol=args.inner_rtol if pth == "mixed" else None)
+                                      inner_rtol=args.inner_rtol if pth == "mixed" else None,
+                                      precond="multigrid" if pth == "mixed" else "chebyshev")
         out[pth] = res
         snaps[pth] = snap
         handles[1].close()
@@ -689,13 +712,26 @@ def main(argv=None):
         print(f"[bench] {pth}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} "
               f"ms/step, {res['iters_per_step']:.1f} it/step", file=sys.stderr, flush=True)
 
+    if "mixed" in out and not args.no_chebyshev_path:
+        # the mixed path with Chebyshev-relaxed Jacobi sweeps instead of the
+        # V-cycle, to separate the multigrid gain
+        print(f"[bench] mixed with Chebyshev sweeps: warmup {args.warmup}, timed "
+              f"{args.steps} steps", file=sys.stderr, flush=True)
+        res, _, handles = run_path(H, case, H.MIXED, args.warmup, args.steps,
+                                   inner_rtol=args.inner_rtol, precond="chebyshev")
+        handles[1].close()
+        handles[0].close()
+        out["mixed_chebyshev"] = res
+        print(f"[bench] mixed, Chebyshev sweeps: {res['value']:.4g} cell-steps/s, "
+              f"{res['ms_per_step']:.2f} ms/step, {res['iters_per_step']:.1f} it/step",
+              file=sys.stderr, flush=True)
     if "mixed" in out and not args.no_plain_sweeps:
         # the mixed path with the reference's preconditioner (four plain
         # Jacobi sweeps), to separate the FP32 gain from the Chebyshev one
         print(f"[bench] mixed with plain Jacobi sweeps: warmup {args.warmup}, timed "
               f"{args.steps} steps", file=sys.stderr, flush=True)
         res, _, handles = run_path(H, case, H.MIXED, args.warmup, args.steps,
-                                   inner_rtol=args.inner_rtol, plain_sweeps=True)
+                                   inner_rtol=args.inner_rtol, precond="plain")
         handles[1].close()
         handles[0].close()
         out["mixed_plain_jacobi"] = res
@@ -714,16 +750,17 @@ def main(argv=None):
         # the launches per iteration.  avg_launch_ms (every launch) is the
         # figure that rocprofv3's kernel statistics report.
         total_ms = k["avg_ms"] * k["launches"]
-        per_iter = jac_per_iter(CHEB_DIVISIONS if main_path == "mixed" else 4)
+        per_iter = MG_SWEEPS_PER_ITER if main_path == "mixed" else jac_per_iter(4)
         active = min(k["launches"], per_iter * k["profiled_iters"])
         active_ms = total_ms / active
         achieved = JAC_WORDS * word * cells / (active_ms * 1e-3) / 1e9
-    traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"))
+    traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"),
+                           MG_DOMINANT if main_path == "mixed" else DOMINANT)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                 "traffic": traffic,
-                "kernel": ("k_jac<float,false,true> (a Chebyshev-relaxed sweep)" if word == 4
-                           else "k_jac<double,false,false>"),
+                "kernel": ("k_jac<float,false,true> (a damped-Jacobi sweep of the V-cycle's "
+                           "finest level)" if word == 4 else "k_jac<double,false,false>"),
                 "algorithmic_bytes_per_launch": JAC_WORDS * word * cells,
                 "avg_active_launch_ms": active_ms,
                 "avg_launch_ms": k["avg_ms"], "launches": k["launches"]}
@@ -731,7 +768,8 @@ def main(argv=None):
     for pth, res in out.items():
         if pth.startswith("mixed"):
             bpc = canonical_bytes_mixed(F, res["iters_per_step"], res["outer_per_step"],
-                                        4 if pth == "mixed_plain_jacobi" else CHEB_DIVISIONS)
+                                        4 if pth == "mixed_plain_jacobi" else CHEB_DIVISIONS,
+                                        multigrid=pth == "mixed")
         else:
             bpc = canonical_bytes_fp64(F, res["iters_per_step"])
         step_roof[pth] = {"bytes_per_cell_step": bpc,

@@ -398,6 +398,7 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
 int dcm_context_destroy(dcm_ctx *c) {
     if (!c) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(c->stream);
+    mg_release(c);
     for (auto &m : c->mirrors) (void)hipFree(m.second.dev);
     for (double *p : c->work64) (void)hipFree(p);
     for (float *p : c->work32) (void)hipFree(p);

@@ -172,6 +172,10 @@ struct dcm_ctx {
     // MPDATA arithmetic: 0 pamm/sfvt/advection_3d.F90 (the copy linked into
     // the model), 1 sfvt/advection_3d.F90 (dcm_set_mpdata_variant)
     int mpdata_variant = 0;
+    // multigrid V-cycle preconditioner of the mixed inner solve
+    // (dcm_set_precond_multigrid; dcm_mg.hip): the hierarchy and the switch
+    void *mg = nullptr;
+    bool mg_on = false;
     double cheb_a = 0.0;
     int cheb_deg = 4;  // a_p divisions per application (the reference's 4)
     double cheb_w[16] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,
@@ -230,6 +234,12 @@ int prof_collect(dcm_ctx *c);
 // the eigenvalue interval [a, 2] of D^-1 A (a = 0: the reference's Jacobi)
 // ---------------------------------------------------------------------------
 void set_chebyshev(dcm_ctx *c, double a, int divisions = 4);
+// multigrid preconditioner (dcm_mg.hip)
+bool mg_possible(const Geo &g);
+int mg_prepare(dcm_ctx *c);
+int mg_correct(dcm_ctx *c, const float *b, float *x, const SolverScalars *sc);
+constexpr float kMgOmega = 0.8f;  // damped-Jacobi weight of the V-cycle's smoother
+void mg_release(dcm_ctx *c);
 
 // ---------------------------------------------------------------------------
 // resident model fields, for the output writer (dcm_dynamics.hip)

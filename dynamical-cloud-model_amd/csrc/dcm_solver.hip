@@ -1276,8 +1276,48 @@ struct IterLaunch {
                 hipLaunchKernelGGL(k_fin_r, dim3(nsys), dim3(1024), 0, c->stream, P.args, seq);
         }
     }
+    // the multigrid preconditioner: FP32 inner solve of a single system
+    bool mg() const { return sizeof(R) == 4 && one() && c->mg_on && c->mg; }
+    // the V-cycle applied to the right-hand side in ti, into oi: the first
+    // sweep from zero is the prep kernel's ti/a_p in VWB; one more sweep, the
+    // coarse-grid correction, two sweeps (the last into oi; P: p = x + bet*p)
+    template <bool P> void vcycle(int ti, int oi) const {
+        const R w = (R)kMgOmega;
+        jac_launch<false, true>("jac", VWB, ti, VWA, w);
+        const int tok = prof_begin(c, "mg_correct");
+        (void)mg_correct(c, (const float *)v(ti), (float *)v(VWA),
+                         (const SolverScalars *)M().sc);
+        prof_end(c, tok);
+        jac_launch<false, true>("jac", VWA, ti, VWB, w);
+        jac_launch<P, true>(P ? "jac_p" : "jac", VWB, ti, oi, w);
+    }
+    // dynamics.f90:391-425 with one V-cycle in place of each application of
+    // the Jacobi sweeps: p = V(t) + bet*p, cs = V(s)
+    void iteration_mg() const {
+        if constexpr (sizeof(R) == 4) {
+            const SolverScalars *sc = (const SolverScalars *)M().sc;
+            hipLaunchKernelGGL((k_jac_prep<R, 0>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
+                               (const R *)v(VR), (const R *)v(VV1), v(VT), v(VWB), M().m,
+                               M().cp, sc);  // t = r - bet*(omg*v1), t/a_p
+            vcycle<true>(VT, VP);            // p = V(t) + bet*p
+            matvec<false>(VP, VCR, VV1);     // v1 = A p, <cr,v1>
+            fin(0);
+            hipLaunchKernelGGL((k_jac_prep<R, 1>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
+                               (const R *)v(VR), (const R *)v(VV1), v(VS), v(VWB), M().m,
+                               M().cp, sc);  // s = r - alf*v1, s/a_p
+            vcycle<false>(VS, VCS);          // cs = V(s)
+            matvec<true>(VCS, VS, VT);       // t = A cs, <t,t>, <t,s>
+            fin(1);
+            update();                        // x, r; <r,r>, <r,cr>
+            fin(2);
+        }
+    }
     // dynamics.f90:391-425
     void iteration() const {
+        if (mg()) {
+            iteration_mg();
+            return;
+        }
         // the sweeps between the fused first one and the last: d - 3 of
         // them, d the preconditioner's number of a_p divisions (4, as the
         // reference's s = r/a_p and three sweeps, unless Chebyshev sets it)
@@ -1326,7 +1366,11 @@ static int bicgstab_iterate_batch(dcm_ctx *c, const SolveSys *sys, int nsys, int
     if (itmax < 1) itmax = 1;
     int launched = 0;
     while (launched < itmax) {
-        const int chunk = (itmax - launched) < kPollEvery ? (itmax - launched) : kPollEvery;
+        // a V-cycle-preconditioned solve converges in two or three
+        // iterations: poll after the second and then after every one, so
+        // that no iteration is launched after convergence
+        const int every = L.mg() ? (launched == 0 ? 2 : 1) : kPollEvery;
+        const int chunk = (itmax - launched) < every ? (itmax - launched) : every;
         for (int q = 0; q < chunk; ++q) L.iteration();
         DCM_LAUNCH_CHECK();
         launched += chunk;

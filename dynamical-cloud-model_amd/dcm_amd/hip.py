@@ -140,6 +140,7 @@ _SIGS = {
     "dcm_set_sum_order": ([_vp, _i], _i),
     "dcm_set_precond_chebyshev": ([_vp, _d, _i], _i),
     "dcm_set_mpdata_variant": ([_vp, _i], _i),
+    "dcm_set_precond_multigrid": ([_vp, _i], _i),
     "dcm_bicgstab": ([_vp, _dp, _dp, _dp, _vp, _vp, _vp, _d, _i, _i, _ip, _dp], _i),
     "dcm_bicgstab_mixed": ([_vp, _vp, _vp, _vp, _d, _i, _ip, _ip, _dp], _i),
     "dcm_true_residual": ([_vp, _vp, _vp, _dp], _i),
@@ -397,6 +398,11 @@ class Context:
     def set_sum_order(self, order):
         """SUM_TREE (default) or SUM_SEQUENTIAL (the reference's serial order)."""
         _call("dcm_set_sum_order", self.h, order)
+
+    def set_precond_multigrid(self, on=True):
+        """One multigrid V-cycle in place of the Jacobi sweeps in the mixed
+        path's FP32 inner solve (single models and dcm_bicgstab_mixed)."""
+        _call("dcm_set_precond_multigrid", self.h, int(bool(on)))
 
     def set_mpdata_variant(self, variant):
         """MPDATA_LINKED (pamm/sfvt/advection_3d.F90, the default) or

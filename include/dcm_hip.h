@@ -110,6 +110,17 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
  * :1028-1091) and whose vector routines use unweighted antidiffusive
  * velocities (:1384-1507, :1874-1909).  A setting, no counterpart call in
  * the reference, where the choice is made at link time (pamm/Makefile:1-3). */
+/* The preconditioner of DCM_MIXED's FP32 inner solve of a single model or
+ * dcm_bicgstab_mixed call: on != 0 applies one geometric multigrid V-cycle
+ * (damped-Jacobi smoothing, trilinear transfers, every dimension halved per
+ * level) wherever the reference's iteration applies its Jacobi sweeps
+ * (precondition, dynamics.f90:241-270, inside :390-430), so that the inner
+ * solve needs a few iterations at any size; the outer FP64 refinement still
+ * ends every solve at the requested true residual.  DCM_BAD_ARG (and no
+ * change) when ip, jp and kp do not halve evenly down to 1024 cells.  Off
+ * by default; ensembles keep the Jacobi/Chebyshev sweeps.  No counterpart in
+ * the reference. */
+int dcm_set_precond_multigrid(dcm_ctx *ctx, int on);
 #define DCM_MPDATA_LINKED 0
 #define DCM_MPDATA_SFVT 1
 int dcm_set_mpdata_variant(dcm_ctx *ctx, int variant);

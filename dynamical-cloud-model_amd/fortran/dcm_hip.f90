@@ -44,7 +44,7 @@ module dcm_hip
         dcm_mpdata3d_add, dcm_upwind_vert3d, dcm_mpdata_vert3d, dcm_mpdata_vec_vert3d, &
         dcm_subgrid, dcm_sources_phys, dcm_model_set_physics, &
         dcm_hmeans, dcm_sources, dcm_divergence, dcm_set_sum_order, dcm_set_precond_chebyshev, &
-        dcm_set_mpdata_variant, &
+        dcm_set_mpdata_variant, dcm_set_precond_multigrid, &
         dcm_bicgstab, &
         dcm_advance_momentum, dcm_advance_scalars, &
         dcm_model_create, dcm_model_destroy, dcm_model_set_precision, dcm_model_set, &
@@ -238,6 +238,14 @@ module dcm_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: order
+        end function
+        ! the mixed inner solve's preconditioner: one multigrid V-cycle per
+        ! application (on /= 0) instead of the Jacobi sweeps
+        integer(c_int) function dcm_set_precond_multigrid(ctx, on) &
+                bind(c, name="dcm_set_precond_multigrid")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
         end function
         ! the copy of advection_3d.F90 MPDATA follows: 0 pamm/sfvt (linked
         ! into main.exe, the default), 1 sfvt

@@ -57,6 +57,18 @@ template <> struct M1<float> {
           dzn(p.m32.dzn), ra(p.m32.rhoa), rn(p.m32.rhoan) {}
 };
 
+// a / b in the arithmetic of R.  FP64: the correctly rounded quotient of
+// the reference.  FP32 (the mixed path's MPDATA): a multiply by the
+// hardware reciprocal, within about 1 ulp and a fifth of the instructions;
+// every denominator here is at least FLT_MIN (metrics, or sums of the
+// non-negative shifted field plus small) and bounds its numerator where it
+// can be that small, so no quotient overflows where the IEEE one would not.
+template <typename R> __device__ __forceinline__ R qdiv(R a, R b);
+template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float qdiv<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+
 template <typename R> __device__ __forceinline__ R pos0(R x) { return x > R(0) ? x : R(0); }
 template <typename R> __device__ __forceinline__ R neg0(R x) { return x < R(0) ? x : R(0); }
 template <typename R> __device__ __forceinline__ R min3(R a, R b, R c) {
@@ -72,14 +84,12 @@ __device__ __forceinline__ R donor(R dt, R uc, R ul, R vc, R vl, R wc, R wl, R p
                                    R pn, R ps, R pu, R pd, R dxn, R dyn, R dzn, R rak, R rakm,
                                    R rnk, int k, int kp, int neumann) {
     const R two = R(2);
-    R fxr = (((uc + fabs(uc)) * pc + (uc - fabs(uc)) * pe) * dt) / (two * dxn);
-    R fxl = (((ul + fabs(ul)) * pw + (ul - fabs(ul)) * pc) * dt) / (two * dxn);
-    R fyr = (((vc + fabs(vc)) * pc + (vc - fabs(vc)) * pn) * dt) / (two * dyn);
-    R fyl = (((vl + fabs(vl)) * ps + (vl - fabs(vl)) * pc) * dt) / (two * dyn);
-    R fzr = ((((wc + fabs(wc)) * rak) * pc + ((wc - fabs(wc)) * rak) * pu) * dt) /
-            ((two * dzn) * rnk);
-    R fzl = ((((wl + fabs(wl)) * rakm) * pd + ((wl - fabs(wl)) * rakm) * pc) * dt) /
-            ((two * dzn) * rnk);
+    R fxr = qdiv<R>((((uc + fabs(uc)) * pc + (uc - fabs(uc)) * pe) * dt), (two * dxn));
+    R fxl = qdiv<R>((((ul + fabs(ul)) * pw + (ul - fabs(ul)) * pc) * dt), (two * dxn));
+    R fyr = qdiv<R>((((vc + fabs(vc)) * pc + (vc - fabs(vc)) * pn) * dt), (two * dyn));
+    R fyl = qdiv<R>((((vl + fabs(vl)) * ps + (vl - fabs(vl)) * pc) * dt), (two * dyn));
+    R fzr = qdiv<R>(((((wc + fabs(wc)) * rak) * pc + ((wc - fabs(wc)) * rak) * pu) * dt), ((two * dzn) * rnk));
+    R fzl = qdiv<R>(((((wl + fabs(wl)) * rakm) * pd + ((wl - fabs(wl)) * rakm) * pc) * dt), ((two * dzn) * rnk));
     if (neumann == 1) {  // :95-101
         if (k == 1) fzl = fzr;
         if (k == kp) fzr = fzl;
@@ -228,105 +238,74 @@ __global__ __launch_bounds__(256) void k_antidiff(const R *ut, const R *vt, cons
     const R g3 = h * (rn[k] + rn[k + 1]);
     const R dxim = M.dx[i - 1], dyjm = M.dy[j - 1], dzkm = M.dz[k - 1];
     if constexpr (!G) {
-        const R ud1 = (((W(0, 0, 0) + W(0, 0, 1)) - W(-1, 0, 0)) - W(-1, 0, 1)) / dzkm +
-                      (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1)) / dyjm;
-        const R ud2 = (((W(0, 0, 0) + W(0, 1, 0)) - W(-1, 0, 0)) - W(-1, 1, 0)) / dzkm +
-                      (((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1)) / dxim;
-        const R ud3 = (((U(0, 0, 0) + U(1, 0, 0)) - U(0, 0, -1)) - U(1, 0, -1)) / dxim +
-                      (((V(0, 0, 0) + V(1, 0, 0)) - V(0, -1, 0)) - V(1, -1, 0)) / dyjm;
+        const R ud1 = qdiv<R>((((W(0, 0, 0) + W(0, 0, 1)) - W(-1, 0, 0)) - W(-1, 0, 1)), dzkm) +
+                      qdiv<R>((((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1)), dyjm);
+        const R ud2 = qdiv<R>((((W(0, 0, 0) + W(0, 1, 0)) - W(-1, 0, 0)) - W(-1, 1, 0)), dzkm) +
+                      qdiv<R>((((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1)), dxim);
+        const R ud3 = qdiv<R>((((U(0, 0, 0) + U(1, 0, 0)) - U(0, 0, -1)) - U(1, 0, -1)), dxim) +
+                      qdiv<R>((((V(0, 0, 0) + V(1, 0, 0)) - V(0, -1, 0)) - V(1, -1, 0)), dyjm);
         const R hz = h * (M.dzn[k - 1] + M.dzn[k]), hy = h * (M.dyn[j - 1] + M.dyn[j]),
                 hx = h * (M.dxn[i - 1] + M.dxn[i]);
-        const R a1 = (((q * (((W(0, 0, 1) + W(0, 0, 0)) + W(-1, 0, 1)) + W(-1, 0, 0))) *
-                       (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))) /
-                      ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small)) / hz;
-        const R b1 = (((q * (((V(0, 0, 1) + V(0, 0, 0)) + V(0, -1, 1)) + V(0, -1, 0))) *
-                       (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))) /
-                      ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small)) / hy;
-        const R a2 = (((q * (((W(0, 1, 0) + W(0, 0, 0)) + W(-1, 1, 0)) + W(-1, 0, 0))) *
-                       (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))) /
-                      ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small)) / hz;
-        const R b2 = (((q * (((U(0, 1, 0) + U(0, 0, 0)) + U(0, 1, -1)) + U(0, 0, -1))) *
-                       (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))) /
-                      ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small)) / hx;
-        const R a3 = (((q * (((V(1, 0, 0) + V(0, 0, 0)) + V(1, -1, 0)) + V(0, -1, 0))) *
-                       (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))) /
-                      ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small)) / hy;
-        const R b3 = (((q * (((U(1, 0, 0) + U(0, 0, 0)) + U(1, 0, -1)) + U(0, 0, -1))) *
-                       (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))) /
-                      ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small)) / hx;
+        const R a1 = qdiv<R>((qdiv<R>(((q * (((W(0, 0, 1) + W(0, 0, 0)) + W(-1, 0, 1)) + W(-1, 0, 0))) *
+                       (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))), ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small))), hz);
+        const R b1 = qdiv<R>((qdiv<R>(((q * (((V(0, 0, 1) + V(0, 0, 0)) + V(0, -1, 1)) + V(0, -1, 0))) *
+                       (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))), ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small))), hy);
+        const R a2 = qdiv<R>((qdiv<R>(((q * (((W(0, 1, 0) + W(0, 0, 0)) + W(-1, 1, 0)) + W(-1, 0, 0))) *
+                       (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))), ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small))), hz);
+        const R b2 = qdiv<R>((qdiv<R>(((q * (((U(0, 1, 0) + U(0, 0, 0)) + U(0, 1, -1)) + U(0, 0, -1))) *
+                       (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))), ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small))), hx);
+        const R a3 = qdiv<R>((qdiv<R>(((q * (((V(1, 0, 0) + V(0, 0, 0)) + V(1, -1, 0)) + V(0, -1, 0))) *
+                       (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))), ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small))), hy);
+        const R b3 = qdiv<R>((qdiv<R>(((q * (((U(1, 0, 0) + U(0, 0, 0)) + U(1, 0, -1)) + U(0, 0, -1))) *
+                       (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))), ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small))), hx);
         const R uc = U(0, 0, 0), vc = V(0, 0, 0), wc = W(0, 0, 0);
-        R usv = ((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))) /
-                 ((P(0, 0, 1) + P(0, 0, 0)) + small)) / M.dxn[i] - ((h * dt) * uc) * (a1 + b1);
-        R vsv = ((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))) /
-                 ((P(0, 1, 0) + P(0, 0, 0)) + small)) / M.dyn[j] - ((h * dt) * vc) * (a2 + b2);
-        R wsv = ((((fabs(wc) * M.dz[k]) - ((dt * wc) * wc)) * (P(1, 0, 0) - P(0, 0, 0))) /
-                 ((P(1, 0, 0) + P(0, 0, 0)) + small)) / M.dzn[k] - ((h * dt) * wc) * (a3 + b3);
-        us[c] = usv - ((q * dt) * uc) * (((U(0, 0, 1) - U(0, 0, -1)) / dxim) + ud1);
-        vs[c] = vsv - ((q * dt) * vc) * (((V(0, 1, 0) - V(0, -1, 0)) / dyjm) + ud2);
-        ws[c] = wsv - ((q * dt) * wc) * (((W(1, 0, 0) - W(-1, 0, 0)) / dzkm) + ud3);
+        R usv = qdiv<R>((qdiv<R>((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))), ((P(0, 0, 1) + P(0, 0, 0)) + small))), M.dxn[i]) - ((h * dt) * uc) * (a1 + b1);
+        R vsv = qdiv<R>((qdiv<R>((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))), ((P(0, 1, 0) + P(0, 0, 0)) + small))), M.dyn[j]) - ((h * dt) * vc) * (a2 + b2);
+        R wsv = qdiv<R>((qdiv<R>((((fabs(wc) * M.dz[k]) - ((dt * wc) * wc)) * (P(1, 0, 0) - P(0, 0, 0))), ((P(1, 0, 0) + P(0, 0, 0)) + small))), M.dzn[k]) - ((h * dt) * wc) * (a3 + b3);
+        us[c] = usv - ((q * dt) * uc) * ((qdiv<R>((U(0, 0, 1) - U(0, 0, -1)), dxim)) + ud1);
+        vs[c] = vsv - ((q * dt) * vc) * ((qdiv<R>((V(0, 1, 0) - V(0, -1, 0)), dyjm)) + ud2);
+        ws[c] = wsv - ((q * dt) * wc) * ((qdiv<R>((W(1, 0, 0) - W(-1, 0, 0)), dzkm)) + ud3);
         return;
     }
-    R u_div1 = (ra[k] * (W(0, 0, 0) + W(0, 0, 1)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 0, 1))) /
-                   (dzkm * g1) +
-               (rn[k] * (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1))) / (dyjm * g1);
-    R u_div2 = (ra[k] * (W(0, 0, 0) + W(0, 1, 0)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 1, 0))) /
-                   (dzkm * g2) +
-               (rn[k] * (((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1))) / (dxim * g2);
-    R u_div3 = (rn[k] * (U(0, 0, 0) - U(0, 0, -1)) + rn[k + 1] * (U(1, 0, 0) - U(1, 0, -1))) /
-                   (dxim * g3) +
-               (rn[k] * (V(0, 0, 0) - V(0, -1, 0)) + rn[k + 1] * (V(1, 0, 0) - V(1, -1, 0))) /
-                   (dyjm * g3);
-    R tz1 = ((((q * (((ra[k] * W(0, 0, 1) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 0, 1)) +
-                     ra[k - 1] * W(-1, 0, 0))) / g1) *
-              (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))) /
-             ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small)) /
-            (h * (M.dzn[k - 1] + M.dzn[k]));
-    R ty1 = ((((q * (((rn[k] * V(0, 0, 1) + rn[k] * V(0, 0, 0)) + rn[k] * V(0, -1, 1)) +
-                     rn[k] * V(0, -1, 0))) / g1) *
-              (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))) /
-             ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small)) /
-            (h * (M.dyn[j - 1] + M.dyn[j]));
+    R u_div1 = qdiv<R>((ra[k] * (W(0, 0, 0) + W(0, 0, 1)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 0, 1))), (dzkm * g1)) +
+               qdiv<R>((rn[k] * (((V(0, 0, 0) + V(0, 0, 1)) - V(0, -1, 0)) - V(0, -1, 1))), (dyjm * g1));
+    R u_div2 = qdiv<R>((ra[k] * (W(0, 0, 0) + W(0, 1, 0)) - ra[k - 1] * (W(-1, 0, 0) + W(-1, 1, 0))), (dzkm * g2)) +
+               qdiv<R>((rn[k] * (((U(0, 0, 0) + U(0, 1, 0)) - U(0, 0, -1)) - U(0, 1, -1))), (dxim * g2));
+    R u_div3 = qdiv<R>((rn[k] * (U(0, 0, 0) - U(0, 0, -1)) + rn[k + 1] * (U(1, 0, 0) - U(1, 0, -1))), (dxim * g3)) +
+               qdiv<R>((rn[k] * (V(0, 0, 0) - V(0, -1, 0)) + rn[k + 1] * (V(1, 0, 0) - V(1, -1, 0))), (dyjm * g3));
+    R tz1 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((ra[k] * W(0, 0, 1) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 0, 1)) +
+                     ra[k - 1] * W(-1, 0, 0))), g1)) *
+              (((P(1, 0, 1) + P(1, 0, 0)) - P(-1, 0, 1)) - P(-1, 0, 0))), ((((P(1, 0, 1) + P(1, 0, 0)) + P(-1, 0, 1)) + P(-1, 0, 0)) + small))), (h * (M.dzn[k - 1] + M.dzn[k])));
+    R ty1 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((rn[k] * V(0, 0, 1) + rn[k] * V(0, 0, 0)) + rn[k] * V(0, -1, 1)) +
+                     rn[k] * V(0, -1, 0))), g1)) *
+              (((P(0, 1, 1) + P(0, 1, 0)) - P(0, -1, 1)) - P(0, -1, 0))), ((((P(0, 1, 1) + P(0, 1, 0)) + P(0, -1, 1)) + P(0, -1, 0)) + small))), (h * (M.dyn[j - 1] + M.dyn[j])));
     R ujb1 = ((h * dt) * U(0, 0, 0)) * (tz1 + ty1);
-    R tz2 = ((((q * (((ra[k] * W(0, 1, 0) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 1, 0)) +
-                     ra[k - 1] * W(-1, 0, 0))) / g2) *
-              (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))) /
-             ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small)) /
-            (h * (M.dzn[k - 1] + M.dzn[k]));
-    R tx2 = ((((q * (((rn[k] * U(0, 1, 0) + rn[k] * U(0, 0, 0)) + rn[k] * U(0, 1, -1)) +
-                     rn[k] * U(0, 0, -1))) / g2) *
-              (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))) /
-             ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small)) /
-            (h * (M.dxn[i - 1] + M.dxn[i]));
+    R tz2 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((ra[k] * W(0, 1, 0) + ra[k] * W(0, 0, 0)) + ra[k - 1] * W(-1, 1, 0)) +
+                     ra[k - 1] * W(-1, 0, 0))), g2)) *
+              (((P(1, 1, 0) + P(1, 0, 0)) - P(-1, 1, 0)) - P(-1, 0, 0))), ((((P(1, 1, 0) + P(1, 0, 0)) + P(-1, 1, 0)) + P(-1, 0, 0)) + small))), (h * (M.dzn[k - 1] + M.dzn[k])));
+    R tx2 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((rn[k] * U(0, 1, 0) + rn[k] * U(0, 0, 0)) + rn[k] * U(0, 1, -1)) +
+                     rn[k] * U(0, 0, -1))), g2)) *
+              (((P(0, 1, 1) + P(0, 0, 1)) - P(0, 1, -1)) - P(0, 0, -1))), ((((P(0, 1, 1) + P(0, 0, 1)) + P(0, 1, -1)) + P(0, 0, -1)) + small))), (h * (M.dxn[i - 1] + M.dxn[i])));
     R ujb2 = ((h * dt) * V(0, 0, 0)) * (tz2 + tx2);
-    R ty3 = ((((q * (((rn[k + 1] * V(1, 0, 0) + rn[k] * V(0, 0, 0)) + rn[k + 1] * V(1, -1, 0)) +
-                     rn[k] * V(0, -1, 0))) / g3) *
-              (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))) /
-             ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small)) /
-            (h * (M.dyn[j - 1] + M.dyn[j]));
-    R tx3 = ((((q * (((rn[k + 1] * U(1, 0, 0) + rn[k] * U(0, 0, 0)) + rn[k + 1] * U(1, 0, -1)) +
-                     rn[k] * U(0, 0, -1))) / g3) *
-              (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))) /
-             ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small)) /
-            (h * (M.dxn[i - 1] + M.dxn[i]));
+    R ty3 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((rn[k + 1] * V(1, 0, 0) + rn[k] * V(0, 0, 0)) + rn[k + 1] * V(1, -1, 0)) +
+                     rn[k] * V(0, -1, 0))), g3)) *
+              (((P(1, 1, 0) + P(0, 1, 0)) - P(1, -1, 0)) - P(0, -1, 0))), ((((P(1, 1, 0) + P(0, 1, 0)) + P(1, -1, 0)) + P(0, -1, 0)) + small))), (h * (M.dyn[j - 1] + M.dyn[j])));
+    R tx3 = qdiv<R>((qdiv<R>(((qdiv<R>((q * (((rn[k + 1] * U(1, 0, 0) + rn[k] * U(0, 0, 0)) + rn[k + 1] * U(1, 0, -1)) +
+                     rn[k] * U(0, 0, -1))), g3)) *
+              (((P(1, 0, 1) + P(0, 0, 1)) - P(1, 0, -1)) - P(0, 0, -1))), ((((P(1, 0, 1) + P(0, 0, 1)) + P(1, 0, -1)) + P(0, 0, -1)) + small))), (h * (M.dxn[i - 1] + M.dxn[i])));
     R ujb3 = ((h * dt) * W(0, 0, 0)) * (ty3 + tx3);
     const R uc = U(0, 0, 0), vc = V(0, 0, 0), wc = W(0, 0, 0);
-    R usv = ((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))) /
-             ((P(0, 0, 1) + P(0, 0, 0)) + small)) /
-                M.dxn[i] -
+    R usv = qdiv<R>((qdiv<R>((((fabs(uc) * M.dx[i]) - ((dt * uc) * uc)) * (P(0, 0, 1) - P(0, 0, 0))), ((P(0, 0, 1) + P(0, 0, 0)) + small))), M.dxn[i]) -
             ujb1;
-    R vsv = ((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))) /
-             ((P(0, 1, 0) + P(0, 0, 0)) + small)) /
-                M.dyn[j] -
+    R vsv = qdiv<R>((qdiv<R>((((fabs(vc) * M.dy[j]) - ((dt * vc) * vc)) * (P(0, 1, 0) - P(0, 0, 0))), ((P(0, 1, 0) + P(0, 0, 0)) + small))), M.dyn[j]) -
             ujb2;
-    R wsv = ((((fabs(wc) * M.dz[k]) - (((dt * (ra[k] / g3)) * wc) * wc)) *
-              (P(1, 0, 0) - P(0, 0, 0))) /
-             ((P(1, 0, 0) + P(0, 0, 0)) + small)) /
-                M.dzn[k] -
+    R wsv = qdiv<R>((qdiv<R>((((fabs(wc) * M.dz[k]) - (((dt * (qdiv<R>(ra[k], g3))) * wc) * wc)) *
+              (P(1, 0, 0) - P(0, 0, 0))), ((P(1, 0, 0) + P(0, 0, 0)) + small))), M.dzn[k]) -
             ujb3;
-    usv = usv - ((q * dt) * uc) * (((U(0, 0, 1) - U(0, 0, -1)) / dxim) + u_div1);
-    vsv = vsv - ((q * dt) * vc) * (((V(0, 1, 0) - V(0, -1, 0)) / dyjm) + u_div2);
+    usv = usv - ((q * dt) * uc) * ((qdiv<R>((U(0, 0, 1) - U(0, 0, -1)), dxim)) + u_div1);
+    vsv = vsv - ((q * dt) * vc) * ((qdiv<R>((V(0, 1, 0) - V(0, -1, 0)), dyjm)) + u_div2);
     wsv = wsv - ((q * dt) * wc) *
-                    (((ra[k + 1] * W(1, 0, 0) - ra[k - 1] * W(-1, 0, 0)) / (dzkm * g3)) + u_div3);
+                    ((qdiv<R>((ra[k + 1] * W(1, 0, 0) - ra[k - 1] * W(-1, 0, 0)), (dzkm * g3))) + u_div3);
     us[c] = usv;
     vs[c] = vsv;
     ws[c] = wsv;
@@ -372,18 +351,18 @@ __device__ __forceinline__ void beta_directional(const double *psi, double m, co
     }
     const long long SI = g.si, SJ = g.sj;
     const R small = Small<R>::v;
-    const R d1 = dt * ((((pos0(ut[c - SI]) * po[c - SI] - neg0(ut[c]) * po[c + SI]) / M.dx[i - 1] +
-                         (pos0(vt[c - SJ]) * po[c - SJ] - neg0(vt[c]) * po[c + SJ]) / M.dy[j - 1]) +
-                        (pos0(wt[c - 1]) * po[c - 1] - neg0(wt[c]) * po[c + 1]) / M.dz[k - 1]) +
+    const R d1 = dt * (((qdiv<R>((pos0(ut[c - SI]) * po[c - SI] - neg0(ut[c]) * po[c + SI]), M.dx[i - 1]) +
+                         qdiv<R>((pos0(vt[c - SJ]) * po[c - SJ] - neg0(vt[c]) * po[c + SJ]), M.dy[j - 1])) +
+                        qdiv<R>((pos0(wt[c - 1]) * po[c - 1] - neg0(wt[c]) * po[c + 1]), M.dz[k - 1])) +
                        small);
-    const R d2 = dt * ((((pos0(ut[c]) * pc - neg0(ut[c - SI]) * pc) / M.dx[i - 1] +
-                         (pos0(vt[c]) * pc - neg0(vt[c - SJ]) * pc) / M.dy[j - 1]) +
-                        (pos0(wt[c]) * pc - neg0(wt[c - 1]) * pc) / M.dz[k - 1]) +
+    const R d2 = dt * (((qdiv<R>((pos0(ut[c]) * pc - neg0(ut[c - SI]) * pc), M.dx[i - 1]) +
+                         qdiv<R>((pos0(vt[c]) * pc - neg0(vt[c - SJ]) * pc), M.dy[j - 1])) +
+                        qdiv<R>((pos0(wt[c]) * pc - neg0(wt[c - 1]) * pc), M.dz[k - 1])) +
                        small);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        b.up[d][c] = (mx[d] - pc) / d1;
-        b.dn[d][c] = (pc - mn[d]) / d2;
+        b.up[d][c] = qdiv<R>((mx[d] - pc), d1);
+        b.dn[d][c] = qdiv<R>((pc - mn[d]), d2);
     }
 }
 
@@ -420,19 +399,16 @@ __global__ __launch_bounds__(256) void k_beta(const double *psi, const double *m
     }
     const R *ra = M.ra, *rn = M.rn;
     const R pc = po[c];
-    R d1 = (dt * ((((pos0(ut[c - SI]) * rn[k]) * po[c - SI] - (neg0(ut[c]) * rn[k]) * po[c + SI]) /
-                       M.dxn[i] +
-                   ((pos0(vt[c - SJ]) * rn[k]) * po[c - SJ] - (neg0(vt[c]) * rn[k]) * po[c + SJ]) /
-                       M.dyn[j]) +
-                  ((pos0(wt[c - 1]) * ra[k - 1]) * po[c - 1] - (neg0(wt[c]) * ra[k]) * po[c + 1]) /
-                      M.dzn[k])) +
+    R d1 = (dt * ((qdiv<R>(((pos0(ut[c - SI]) * rn[k]) * po[c - SI] - (neg0(ut[c]) * rn[k]) * po[c + SI]), M.dxn[i]) +
+                   qdiv<R>(((pos0(vt[c - SJ]) * rn[k]) * po[c - SJ] - (neg0(vt[c]) * rn[k]) * po[c + SJ]), M.dyn[j])) +
+                  qdiv<R>(((pos0(wt[c - 1]) * ra[k - 1]) * po[c - 1] - (neg0(wt[c]) * ra[k]) * po[c + 1]), M.dzn[k]))) +
            small;
-    R d2 = dt * (((((pos0(ut[c]) * rn[k]) * pc - (neg0(ut[c - SI]) * rn[k]) * pc) / M.dxn[i] +
-                   ((pos0(vt[c]) * rn[k]) * pc - (neg0(vt[c - SJ]) * rn[k]) * pc) / M.dyn[j]) +
-                  ((pos0(wt[c]) * ra[k]) * pc - (neg0(wt[c - 1]) * ra[k - 1]) * pc) / M.dzn[k]) +
+    R d2 = dt * (((qdiv<R>(((pos0(ut[c]) * rn[k]) * pc - (neg0(ut[c - SI]) * rn[k]) * pc), M.dxn[i]) +
+                   qdiv<R>(((pos0(vt[c]) * rn[k]) * pc - (neg0(vt[c - SJ]) * rn[k]) * pc), M.dyn[j])) +
+                  qdiv<R>(((pos0(wt[c]) * ra[k]) * pc - (neg0(wt[c - 1]) * ra[k - 1]) * pc), M.dzn[k])) +
                  small);
-    bup[c] = (rn[k] * (mx - pc)) / d1;
-    bdn[c] = (rn[k] * (pc - mn)) / d2;
+    bup[c] = qdiv<R>((rn[k] * (mx - pc)), d1);
+    bdn[c] = qdiv<R>((rn[k] * (pc - mn)), d2);
 }
 
 // limited velocity of eq 18 at the face owned by cell c (stride s to the

@@ -628,7 +628,8 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
                                    dt);
             DCM_LAUNCH_CHECK();
             prof_end(c, tok);
-            for (int d = 0; d < 3; ++d) DCM_CHECK(fill<R>(c, anti[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
+            DCM_CHECK(fill_set<R>(c, FillSet<R>{{anti[0], anti[1], anti[2]}}, 3, FILL_FULL, 0.0,
+                                  0.0, flag));
             if (monotone) {
                 tok = prof_begin(c, "beta");
                 if (pd)
@@ -639,10 +640,8 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
                                        dm, cur[0], anti[0], anti[1], anti[2], bt, flag, g, mp, dt);
                 DCM_LAUNCH_CHECK();
                 prof_end(c, tok);
-                for (int d = 0; d < (pd ? 3 : 1); ++d) {
-                    DCM_CHECK(fill<R>(c, bt.up[d], FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
-                    DCM_CHECK(fill<R>(c, bt.dn[d], FILL_ALONG_DIM, 0.0, 0.0, nullptr, 0, flag));
-                }
+                const FillSet<R> bs{{bt.up[0], bt.dn[0], bt.up[1], bt.dn[1], bt.up[2], bt.dn[2]}};
+                DCM_CHECK(fill_set<R>(c, bs, pd ? 6 : 2, FILL_ALONG_DIM, 0.0, 0.0, flag));
             }
             if (last) {
                 for (int n = 0; n < nf; ++n) {
@@ -669,8 +668,8 @@ int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *ps
                 hipLaunchKernelGGL(k_limit<R>, L.grid, L.block, 0, c->stream, anti[0], anti[1],
                                    anti[2], bt, other[0], other[1], other[2], flag, g);
                 DCM_LAUNCH_CHECK();
-                for (int d = 0; d < 3; ++d)
-                    DCM_CHECK(fill<R>(c, other[d], FILL_FULL, 0.0, 0.0, nullptr, 0, flag));
+                DCM_CHECK(fill_set<R>(c, FillSet<R>{{other[0], other[1], other[2]}}, 3, FILL_FULL,
+                                      0.0, 0.0, flag));
                 adv = other;
             }
             for (int n = 0; n < nf; ++n) {

@@ -18,9 +18,10 @@
 namespace dcm {
 
 template <typename T>
-__global__ void k_fill(T *a, Geo g, int kind, double lbc, double ubc, const double *shift,
-                       int mode, const int *skip_if) {
+__global__ void k_fill(FillSet<T> set, Geo g, int kind, double lbc, double ubc,
+                       const double *shift, int mode, const int *skip_if) {
     if (skip_if && *skip_if) return;
+    T *a = set.a[blockIdx.y];  // one array per grid row
     const long long P = (long long)g.nj * g.ni;
     const long long R = 2LL * g.ni + 2LL * g.jp;
     const long long nz = 2 * P;
@@ -113,12 +114,37 @@ int fill(dcm_ctx *c, T *a, FillKind kind, double lbc, double ubc, const double *
     if (nb > 4096) nb = 4096;
     int mode = bc_shift ? (bc_mode == 2 ? 2 : 1) : 0;
     int tok = prof_begin(c, "fill");
-    hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kBlock), 0, c->stream, a, g, (int)kind, lbc,
+    FillSet<T> set{};
+    set.a[0] = a;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nb), dim3(kBlock), 0, c->stream, set, g, (int)kind, lbc,
                        ubc, bc_shift, mode, skip_if);
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
     return DCM_OK;
 }
+
+template <typename T>
+int fill_set(dcm_ctx *c, const FillSet<T> &set, int count, FillKind kind, double lbc,
+             double ubc, const int *skip_if) {
+    const Geo &g = c->geo;
+    if (count < 1 || count > kFillSetMax) {
+        set_error("fill_set: 1..6 arrays");
+        return DCM_BAD_ARG;
+    }
+    const long long n = 2LL * g.nj * g.ni + (long long)g.kp * (2LL * g.ni + 2LL * g.jp);
+    unsigned nb = blocks_for(n);
+    if (nb > 4096) nb = 4096;
+    int tok = prof_begin(c, "fill");
+    hipLaunchKernelGGL(k_fill<T>, dim3(nb, (unsigned)count), dim3(kBlock), 0, c->stream, set, g,
+                       (int)kind, lbc, ubc, (const double *)nullptr, 0, skip_if);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+template int fill_set<double>(dcm_ctx *, const FillSet<double> &, int, FillKind, double, double,
+                              const int *);
+template int fill_set<float>(dcm_ctx *, const FillSet<float> &, int, FillKind, double, double,
+                             const int *);
 template int fill<double>(dcm_ctx *, double *, FillKind, double, double, const double *, int,
                           const int *);
 template int fill<float>(dcm_ctx *, float *, FillKind, double, double, const double *, int,

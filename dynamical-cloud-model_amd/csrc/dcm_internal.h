@@ -285,6 +285,14 @@ enum FillKind { FILL_FULL = 0, FILL_ALONG_DIM, FILL_ALONG_DIM_WO, FILL_FULL_WO, 
 template <typename T>
 int fill(dcm_ctx *c, T *a, FillKind kind, double lbc, double ubc,
          const double *bc_shift = nullptr, int bc_mode = 0, const int *skip_if = nullptr);
+// up to kFillSetMax arrays filled alike (lbc/ubc by value) in one launch
+constexpr int kFillSetMax = 6;
+template <typename T> struct FillSet {
+    T *a[kFillSetMax];
+};
+template <typename T>
+int fill_set(dcm_ctx *c, const FillSet<T> &set, int count, FillKind kind, double lbc,
+             double ubc, const int *skip_if = nullptr);
 // Neumann z rows for the matvec: x(0,:,:)=x(1,:,:), x(kp+1,:,:)=x(kp,:,:)
 int fill_neumann_z(dcm_ctx *c, double *x);
 

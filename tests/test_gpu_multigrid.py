@@ -100,3 +100,46 @@ def test_multigrid_switches_off_and_on():
         assert on[0] < off[0]
         a, g = x.get()[1:-1, 1:-1, 1:-1], x2.get()[1:-1, 1:-1, 1:-1]
         assert rel_l2(a - a.mean(), g - g.mean()) < 1e-8
+
+
+def _q_mass(dom, a):
+    w = (dom.rhoan[1:-1] * dom.dz[1:-1])[None, None, :]
+    return float(np.sum(a[1:-1, 1:-1, 1:-1] * w))
+
+
+@pytest.mark.parametrize("name", ["warm_bubble_64", "moist_bubble_256x256x128",
+                                  "large_domain_512x512x256"])
+def test_multigrid_at_full_size(name):
+    """The workloads' full sizes (the 512x4x256 Kelvin-Helmholtz grid does
+    not halve in y and keeps the sweeps): every solve at FP64 true residual
+    below 1e-10, every q conserved to FP32 flux rounding, within 1e-5 of the
+    FP64 path, and a few inner iterations per solve."""
+    case = W.CONFIGS[name](steps=2)
+    dom = case.dom
+    print(f"multigrid at full size {name}: {dom.ip}x{dom.jp}x{dom.kp}", flush=True)
+    out = {}
+    for prec in (H.FP64, H.MIXED):
+        with H.Context(dom) as c:
+            if prec == H.MIXED:
+                c.set_precond_multigrid(True)
+            m = H.Model(c, case, prec)
+            try:
+                m.set_check_residual(True)
+                st = m.step(2)
+                print(f"  precision {prec}: iterations {st.iters_total}, outer {st.outer_total}, "
+                      f"true residual {st.max_true_resid:.3g}", flush=True)
+                assert st.max_true_resid < 1e-10 and st.fallbacks == 0
+                if prec == H.MIXED:
+                    assert st.iters_total <= 6 * max(st.outer_total, 1), st.iters_total
+                out[prec] = {"th": m.get(H.F_TH), "u": m.get(H.F_U), "w": m.get(H.F_W)}
+                for n in range(case.nq):
+                    q = m.get(H.F_Q, n)
+                    out[prec][f"q{n}"] = q
+                    ref = _q_mass(dom, case.q[n])
+                    scale = max(abs(ref), _q_mass(dom, np.abs(case.q[n])))
+                    assert abs(_q_mass(dom, q) - ref) <= 1e-6 * scale, (prec, n)
+            finally:
+                m.close()
+    for key in out[H.FP64]:
+        a, b = out[H.FP64][key][1:-1, 1:-1, 1:-1], out[H.MIXED][key][1:-1, 1:-1, 1:-1]
+        assert rel_l2(a, b) < 1e-5, key

@@ -15,11 +15,16 @@ W warm-up steps, then K timed steps from the same state:
 
 * FP64  -- the reference arithmetic (parity path);
 * mixed -- FP32 MPDATA intermediates and FP32 inner BiCGSTAB inside FP64
-  iterative refinement, every solve ending at FP64 true residual < 1e-10.
+  iterative refinement, every solve ending at FP64 true residual < 1e-10;
+  the inner solve is preconditioned by one multigrid V-cycle per
+  application (dcm_set_precond_multigrid).  The same mixed path with the
+  Chebyshev-relaxed and with the reference's plain Jacobi sweeps is
+  reported beside it (paths.mixed_chebyshev, paths.mixed_plain_jacobi).
 
 ``value`` is the mixed path's cell-steps/s (interior cells x timed steps /
 wall seconds).  The ``roofline`` object describes the dominant kernel of that
-path, timed with HIP events on the model's stream over the timed region.
+path (the finest level's damped-Jacobi sweep), timed with HIP events on the
+model's stream in two further steps after the timed ones.
 """
 from __future__ import annotations
 

@@ -1366,10 +1366,10 @@ static int bicgstab_iterate_batch(dcm_ctx *c, const SolveSys *sys, int nsys, int
     if (itmax < 1) itmax = 1;
     int launched = 0;
     while (launched < itmax) {
-        // a V-cycle-preconditioned solve converges in two or three
-        // iterations: poll after the second and then after every one, so
-        // that no iteration is launched after convergence
-        const int every = L.mg() ? (launched == 0 ? 2 : 1) : kPollEvery;
+        // a V-cycle-preconditioned solve converges in one to three
+        // iterations: poll after every one, so that no iteration is
+        // launched after convergence
+        const int every = L.mg() ? 1 : kPollEvery;
         const int chunk = (itmax - launched) < every ? (itmax - launched) : every;
         for (int q = 0; q < chunk; ++q) L.iteration();
         DCM_LAUNCH_CHECK();

@@ -1205,6 +1205,20 @@ static int ensemble_solve(dcm_ensemble *e, std::vector<SolverOut> &outs) {
     outs.assign(n, SolverOut());
     dcm_ctx *c0 = e->mem[0]->c;
     int r;
+    bool all_mg = e->precision == DCM_MIXED;
+    for (int q = 0; q < n && all_mg; ++q) all_mg = e->mem[q]->c->mg_on && e->mem[q]->c->mg;
+    if (all_mg) {
+        // every member with the multigrid preconditioner: its own solve, as
+        // when it is stepped alone (the batched iteration has no V-cycle);
+        // the members' contexts all use the ensemble's stream here
+        int worst = DCM_OK;
+        for (int q = 0; q < n; ++q) {
+            const int rq = bicgstab_mixed_batch(e->mem[q]->c, &tasks[q], 1, e->ptol, 9999,
+                                                e->mem[q]->inner_rtol, &outs[q], &st[q]);
+            if (rq != DCM_OK && worst == DCM_OK) worst = rq;
+        }
+        return worst;
+    }
     if (e->precision == DCM_MIXED)
         r = bicgstab_mixed_batch(c0, tasks.data(), n, e->ptol, 9999, e->mem[0]->inner_rtol,
                                  outs.data(), st.data());

@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdcm_oracle.so")
+# DCM_ORACLE_LIB: another build of dcm_oracle.c (the mutation tests load
+# copies with a planted mistake this way)
+LIB_PATH = os.environ.get("DCM_ORACLE_LIB") or os.path.join(HERE, "libdcm_oracle.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

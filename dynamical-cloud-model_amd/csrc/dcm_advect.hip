@@ -1026,6 +1026,16 @@ int dcm_set_mpdata_variant(dcm_ctx *c, int variant) {
 int dcm_upwind3d(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
                  double *psi, int neumann) {
     if (!c || !u || !v || !w || !psi) return DCM_BAD_ARG;
+    return upwind_dev(c, dt, u, v, w, psi, neumann);
+}
+
+}  // extern "C"
+
+namespace dcm {
+
+// first_order_upstream_3d in place on psi (FP64; the model's advection_scheme 0)
+int upwind_dev(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+               double *psi, int neumann) {
     const Geo &g = c->geo;
     const Interior L = interior_launch(g);
     MetPack mp{c->met, c->met32};
@@ -1041,6 +1051,18 @@ int dcm_upwind3d(dcm_ctx *c, double dt, const double *u, const double *v, const 
     DCM_LAUNCH_CHECK();
     return DCM_OK;
 }
+
+// psi += sign * prof(k) over the whole padded array (mpdata_3d_add's profile)
+int add_profile_dev(dcm_ctx *c, double *psi, const double *prof_dev, double sign) {
+    hipLaunchKernelGGL(k_add_profile, dim3(2048), dim3(kBlock), 0, c->stream, psi, prof_dev,
+                       sign, c->geo);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+}  // namespace dcm
+
+extern "C" {
 
 static int mpdata_api(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
                       double *psi, int nq, double *lbc, double *ubc, int kord, int monotone,

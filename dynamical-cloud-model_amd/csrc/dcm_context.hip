@@ -70,9 +70,25 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+// Scratch is allocated on first use.  An allocation (and the zeroing
+// memset that follows it) must never land inside a stream capture, where it
+// would either fail the capture or record a memset node into the graph: the
+// caller has to run the work once without capture first.  Returns an error
+// when c->stream is capturing.
+int no_alloc_in_capture(dcm_ctx *c, const char *what) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        set_error(std::string(what) + ": scratch would be allocated inside a stream capture "
+                                      "(run the step once without capture first)");
+        return DCM_HIP_ERROR;
+    }
+    return DCM_OK;
+}
+
 int work_field64(dcm_ctx *c, int slot, double **out) {
     if ((int)c->work64.size() <= slot) c->work64.resize(slot + 1, nullptr);
     if (!c->work64[slot]) {
+        DCM_CHECK(no_alloc_in_capture(c, "work_field64"));
         void *p = nullptr;
         if (hipMalloc(&p, c->geo.ntot * sizeof(double)) != hipSuccess) {
             set_error("hipMalloc of a work field failed");
@@ -88,6 +104,7 @@ int work_field64(dcm_ctx *c, int slot, double **out) {
 int work_field32(dcm_ctx *c, int slot, float **out) {
     if ((int)c->work32.size() <= slot) c->work32.resize(slot + 1, nullptr);
     if (!c->work32[slot]) {
+        DCM_CHECK(no_alloc_in_capture(c, "work_field32"));
         void *p = nullptr;
         if (hipMalloc(&p, c->geo.ntot * sizeof(float)) != hipSuccess) {
             set_error("hipMalloc of a work field failed");
@@ -102,6 +119,7 @@ int work_field32(dcm_ctx *c, int slot, float **out) {
 
 int ensure_partials(dcm_ctx *c, size_t n) {
     if (c->partials_cap >= n) return DCM_OK;
+    DCM_CHECK(no_alloc_in_capture(c, "ensure_partials"));
     if (c->partials) {
         DCM_HIP(hipStreamSynchronize(c->stream));
         DCM_HIP(hipFree(c->partials));
@@ -119,6 +137,7 @@ int ensure_partials(dcm_ctx *c, size_t n) {
 
 int ensure_solver_vectors64(dcm_ctx *c, size_t count) {
     if (c->sv64_count >= count) return DCM_OK;
+    DCM_CHECK(no_alloc_in_capture(c, "ensure_solver_vectors64"));
     if (c->sv64) {
         DCM_HIP(hipStreamSynchronize(c->stream));
         DCM_HIP(hipFree(c->sv64));
@@ -154,6 +173,7 @@ int ensure_host_flags(dcm_ctx *c, size_t n, int **out) {
 
 int ensure_solver_vectors32(dcm_ctx *c, size_t count) {
     if (c->sv32_count >= count) return DCM_OK;
+    DCM_CHECK(no_alloc_in_capture(c, "ensure_solver_vectors32"));
     if (c->sv32) {
         DCM_HIP(hipStreamSynchronize(c->stream));
         DCM_HIP(hipFree(c->sv32));
@@ -178,6 +198,7 @@ int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **ou
         set_error("1-D array longer than the grid extent");
         return DCM_BAD_ARG;
     }
+    DCM_CHECK(no_alloc_in_capture(c, "upload_1d"));
     if (c->d1d_cap < need) {
         // every slot is allocated at the first upload, so earlier slots of
         // the same call stay valid

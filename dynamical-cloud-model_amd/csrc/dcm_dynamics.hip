@@ -151,6 +151,7 @@ int level_means(dcm_ctx *c, const double *a, double *out_dev) {
     const int nchunks = (int)((ncol + kHmCols - 1) / kHmCols);
     const size_t need = (size_t)nchunks * g.nk;
     if (c->hm_cap < need) {
+        DCM_CHECK(no_alloc_in_capture(c, "level_means"));
         if (c->hm_partials) {
             DCM_HIP(hipStreamSynchronize(c->stream));
             DCM_HIP(hipFree(c->hm_partials));
@@ -339,6 +340,13 @@ int advance_momentum_dev(dcm_ctx *c, double dt, double *u, double *v, double *w,
 
 }  // namespace dcm
 
+// sets *bad when the shift m does not round-trip the bcs: (bc - m) + m != bc
+__global__ void k_bc_roundtrip(const double *m, double lbc, double ubc, int *bad) {
+    if (threadIdx.x != 0) return;
+    const double mm = *m;
+    if ((lbc - mm) + mm != lbc || (ubc - mm) + mm != ubc) *bad = 1;
+}
+
 // ---------------------------------------------------------------------------
 // model state
 // ---------------------------------------------------------------------------
@@ -358,6 +366,11 @@ struct dcm_model {
     std::vector<double> xg, yg, zg;
     int n = 0;               // completed steps (pointer rotation)
     double t1 = 0.0, t2 = 0.0;
+    // advection_scheme 2: the bcs of theta + thetan, thetan(0) and thetan(kp+1)
+    // (find_base_top, mpi_module.f90:306-329), and a device flag set when a
+    // step's (bc - m) + m differs from bc
+    double thbase = 0.0, thtop = 0.0;
+    int *bc_inexact = nullptr;
     double inner_rtol = 1e-4;  // fewest inner iterations at 256x256x128 (1e-3..1e-5 measured)
     bool check_residual = false;
     std::vector<void *> allocs;
@@ -393,6 +406,7 @@ struct dcm_model {
 using namespace dcm;
 
 static int model_alloc(dcm_model *m, size_t bytes, void **out) {
+    DCM_CHECK(no_alloc_in_capture(m->c, "dcm_model"));
     void *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) {
         set_error("dcm_model: out of device memory");
@@ -617,7 +631,7 @@ int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_
                      const double *theta, const double *thetan, const double *xg,
                      const double *yg, const double *zg, dcm_model **out) {
     if (!c || !out || nq < 0 || ncat < 0 || (ncat > 0 && (!c_s || !c_e)) || !theta || !thetan ||
-        (advection_scheme != 0 && advection_scheme != 1) || kord < 1) {
+        advection_scheme < 0 || advection_scheme > 2 || kord < 1) {
         set_error("dcm_model_create: bad arguments");
         return DCM_BAD_ARG;
     }
@@ -658,6 +672,12 @@ int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_
     if (st == DCM_OK) {
         st = model_alloc(m, 3 * (g.kp + 2) * sizeof(double), &p);
         m->oned = (double *)p;
+    }
+    if (st == DCM_OK && advection_scheme == 2) {
+        st = model_alloc(m, 256, &p);
+        m->bc_inexact = (int *)p;
+        m->thbase = thetan[0];
+        m->thtop = thetan[g.kp + 1];
     }
     if (st != DCM_OK) {
         dcm_model_destroy(m);
@@ -964,17 +984,32 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
         wf = vf + g.ntot;
         DCM_CHECK(velocity_shadow(c, cur[0], cur[1], cur[2], uf, vf, wf));
     }
-    const int kord = m->advection_scheme == 0 ? 1 : m->kord;
-    const int mono = m->advection_scheme == 0 ? 0 : m->monotone;
-    {
-        double lb = m->t1, ub = m->t2;
+    if (m->advection_scheme == 0) {
+        // :539-546 first_order_upstream_3d, then exchange_full(th, t1, t2);
+        // FP64 on both paths (one pass, no intermediates to narrow)
+        DCM_CHECK(upwind_dev(c, m->dt, cur[0], cur[1], cur[2], m->th, 0));
+        DCM_CHECK(fill<double>(c, m->th, FILL_FULL, m->t1, m->t2));
+    } else {
+        // :548-551 mpdata_3d(th, t1, t2); :552-556 mpdata_3d_add: theta +
+        // thetan advected with thbase, thtop as its bcs
+        const bool add = m->advection_scheme == 2;
+        double lb = add ? m->thbase : m->t1, ub = add ? m->thtop : m->t2;
+        if (add) DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), 1.0));
         if (mixed)
-            st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, m->th, 1, &lb, &ub, kord, mono,
-                                    0, 0);
+            st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, m->th, 1, &lb, &ub, m->kord,
+                                    m->monotone, 0, 0);
         else
-            st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], m->th, 1, &lb, &ub, kord,
-                                     mono, 0, 0);  // :548-551
+            st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], m->th, 1, &lb, &ub,
+                                     m->kord, m->monotone, 0, 0);
         if (st != DCM_OK) return st;
+        if (add) {
+            // t1 = t2 = 0 come back from (0 - m) + m exactly; thbase and
+            // thtop only when the round trip is exact, which is checked
+            hipLaunchKernelGGL(k_bc_roundtrip, dim3(1), dim3(64), 0, c->stream,
+                               (const double *)(c->dscal + 16), lb, ub, m->bc_inexact);
+            DCM_LAUNCH_CHECK();
+            DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), -1.0));
+        }
     }
     if (m->nq > 0) {  // :571-580 with q1 = q2 = 0
         for (int cat = 0; cat < m->ncat; ++cat) {
@@ -1018,6 +1053,24 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
     return DCM_OK;
 }
 
+// advection_scheme 2: the reference leaves thbase and thtop as (bc - m) + m
+// after every mpdata_3d call (INTENT(INOUT) scalars, advection_3d.F90:
+// 278-280, 711-713); the model keeps them fixed, the same values whenever
+// the round trip is exact.  For potential temperatures it always is: m <= bc
+// <= 2m, so bc - m is exact (Sterbenz) and adding m back restores bc.  A
+// step where it was not is reported here as an error, not left to drift.
+static int model_check_bcs(dcm_model *m) {
+    if (!m->bc_inexact) return DCM_OK;
+    int bad = 0;
+    DCM_HIP(hipMemcpy(&bad, m->bc_inexact, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) {
+        set_error("dcm_model_step: advection_scheme 2 changed thbase/thtop by rounding "
+                  "((bc - m) + m != bc); this model keeps them fixed");
+        return DCM_BAD_ARG;
+    }
+    return DCM_OK;
+}
+
 static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
     DCM_CHECK(model_pre_solve(m));
@@ -1045,6 +1098,7 @@ int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {
     for (int s = 0; s < nsteps; ++s) DCM_CHECK(model_step_once(m, stats));
     DCM_HIP(hipStreamSynchronize(m->c->stream));
     DCM_CHECK(prof_collect(m->c));
+    DCM_CHECK(model_check_bcs(m));
     return DCM_OK;
 }
 
@@ -1104,7 +1158,10 @@ struct dcm_ensemble {
     double ptol = 1e-10;
     int march_planes = 0;
     bool use_graphs = true;
-    bool warm = false;  // one step ran eagerly: every scratch buffer exists
+    // the member keys each phase has already run with, without capture: a
+    // phase is captured only for a key seen before, so every scratch buffer
+    // its kernels use exists by then (no allocation or memset is recorded)
+    std::vector<std::vector<int>> eager[2];
     dcm::PhaseGraph pre[3], post[3];
     // the stages around the solve run the members on side streams (members
     // balanced by cells), forked from and joined back into `stream`
@@ -1205,8 +1262,16 @@ static int ensemble_solve(dcm_ensemble *e, std::vector<SolverOut> &outs) {
     outs.assign(n, SolverOut());
     dcm_ctx *c0 = e->mem[0]->c;
     int r;
-    bool all_mg = e->precision == DCM_MIXED;
-    for (int q = 0; q < n && all_mg; ++q) all_mg = e->mem[q]->c->mg_on && e->mem[q]->c->mg;
+    int n_mg = 0;
+    for (int q = 0; q < n; ++q) n_mg += (e->mem[q]->c->mg_on && e->mem[q]->c->mg) ? 1 : 0;
+    if (e->precision == DCM_MIXED && n_mg > 0 && n_mg < n) {
+        // the batched solve has no V-cycle: a member with the multigrid
+        // setting would silently evolve differently from its solo run
+        set_error("dcm_ensemble_step: either every member or none uses the multigrid "
+                  "preconditioner");
+        return DCM_BAD_ARG;
+    }
+    const bool all_mg = e->precision == DCM_MIXED && n_mg == n;
     if (all_mg) {
         // every member with the multigrid preconditioner: its own solve, as
         // when it is stepped alone (the batched iteration has no V-cycle);
@@ -1380,8 +1445,19 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
     auto advance_levels = [&]() {
         for (int q = 0; q < n; ++q) e->mem[q]->n += 1;  // model_post_solve's host part
     };
+    auto seen = [&](int phase) {
+        for (const std::vector<int> &k : e->eager[phase])
+            if (k == key) return true;
+        return false;
+    };
+    auto remember = [&](int phase) {
+        std::vector<std::vector<int>> &v = e->eager[phase];
+        if (seen(phase)) return;
+        if (v.size() >= 48) v.erase(v.begin());
+        v.push_back(key);
+    };
     for (int s = 0; s < nsteps; ++s) {
-        bool graphs = e->use_graphs && e->warm;
+        bool graphs = e->use_graphs;
         for (int q = 0; q < n; ++q) {
             // what the captured kernels and their arguments depend on
             const dcm_model *m = e->mem[q];
@@ -1389,10 +1465,12 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
                      12 * m->c->mpdata_variant + 24 * m->gen;
             graphs = graphs && !m->c->profiling;
         }
-        if (graphs)
+        if (graphs && seen(0)) {
             DCM_CHECK(run_phase(e, e->pre, key, pre, [] {}));
-        else
+        } else {
             DCM_CHECK(pre());
+            remember(0);
+        }
         DCM_CHECK(ensemble_solve(e, outs));
         for (int q = 0; q < n; ++q) {
             dcm_model *m = e->mem[q];
@@ -1404,13 +1482,15 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
             add_stats(stats ? stats + q : nullptr, outs[q], true_resid);
             if (stats) stats[q].steps += 1;
         }
-        if (graphs)
+        if (graphs && seen(1)) {
             DCM_CHECK(run_phase(e, e->post, key, post, advance_levels));
-        else
+        } else {
             DCM_CHECK(post());
-        e->warm = true;
+            remember(1);
+        }
     }
     DCM_HIP(hipStreamSynchronize(e->stream));
+    for (dcm_model *m : e->mem) DCM_CHECK(model_check_bcs(m));
     return DCM_OK;
 }
 

@@ -203,6 +203,8 @@ namespace dcm {
 // ---------------------------------------------------------------------------
 // helpers implemented in dcm_context.hip
 // ---------------------------------------------------------------------------
+// DCM_HIP_ERROR when c->stream is being captured (scratch must exist first)
+int no_alloc_in_capture(dcm_ctx *c, const char *what);
 int work_field64(dcm_ctx *c, int slot, double **out);   // padded fp64 scratch
 int work_field32(dcm_ctx *c, int slot, float **out);    // padded fp32 scratch
 int ensure_partials(dcm_ctx *c, size_t n);
@@ -319,6 +321,9 @@ int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf
 int velocity_shadow(dcm_ctx *c, const double *u, const double *v, const double *w, float *uf,
                     float *vf, float *wf);
 int adv_ref_state_dev(dcm_ctx *c, double dt, const double *w, double *psi, const double *ref_dev);
+int upwind_dev(dcm_ctx *c, double dt, const double *u, const double *v, const double *w,
+               double *psi, int neumann);
+int add_profile_dev(dcm_ctx *c, double *psi, const double *prof_dev, double sign);
 
 // Physics options of sources beyond the inviscid branch, device side: 1-D
 // profiles are device arrays of kp+2.  log_z1_z0 = log(z(1)/z0) is computed

@@ -154,9 +154,9 @@ def case_from_namelist(path, steps=None, overrides=None, **kw) -> W.Case:
         nm[key] = val
         if any(key == f[0] for f in H.InitConfig._fields_):
             setattr(rc.cfg, key, int(val) if isinstance(val, bool) else val)
-    if nm["advection_scheme"] not in (0, 1):
-        raise NotImplementedError("advection_scheme 2 (reference state added) is not on the "
-                                  "resident model's path")
+    if nm["advection_scheme"] not in (0, 1, 2):
+        raise NotImplementedError(f"advection_scheme {nm['advection_scheme']} is not coded "
+                                  "(the reference stops with 'not coded' too)")
     if nm["radiation"]:
         raise NotImplementedError("radiation is not on the GPU path")
     pr = compute_profiles(rc)

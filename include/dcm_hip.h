@@ -118,8 +118,11 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
  * solve needs a few iterations at any size; the outer FP64 refinement still
  * ends every solve at the requested true residual.  DCM_BAD_ARG (and no
  * change) when ip, jp and kp do not halve evenly down to 1024 cells.  Off
- * by default; ensembles keep the Jacobi/Chebyshev sweeps.  No counterpart in
- * the reference. */
+ * by default.  In a DCM_MIXED ensemble either every member has the setting
+ * (each member is then solved with its own V-cycle, one after another, and
+ * evolves bit for bit as when stepped alone) or none has it (one batched
+ * solve with the Jacobi/Chebyshev sweeps); dcm_ensemble_step returns
+ * DCM_BAD_ARG for a mix.  No counterpart in the reference. */
 int dcm_set_precond_multigrid(dcm_ctx *ctx, int on);
 #define DCM_MPDATA_LINKED 0
 #define DCM_MPDATA_SFVT 1
@@ -304,7 +307,13 @@ typedef struct {
 } dcm_step_stats;
 
 /* theta/thetan/xg/yg/zg: host 1-D arrays; c_s/c_e: 1-based category bounds
- * into q (driver_code.f90:571-580).  advection_scheme: 0 upwind, 1 mpdata. */
+ * into q (driver_code.f90:571-580).  advection_scheme (driver_code.f90:
+ * 511-561): 0 first_order_upstream_3d of theta then exchange_full(th, t1,
+ * t2) (FP64 on both precisions); 1 mpdata_3d; 2 mpdata_3d_add of theta +
+ * thetan with thetan(0), thetan(kp+1) as its bcs and no adv_ref_state.  In
+ * scheme 2 dcm_model_step returns DCM_BAD_ARG if a step's shift would have
+ * changed those bcs by rounding ((bc - m) + m != bc; never for potential
+ * temperatures, where m <= bc <= 2m). */
 int dcm_model_create(dcm_ctx *ctx, int nq, int ncat, const int *c_s,
                      const int *c_e, double dt, int kord, int monotone,
                      int advection_scheme, double ptol, const double *theta,

@@ -1705,9 +1705,12 @@ int or_model_step(const or_grid *g, or_model *m) {
     if (m->advection_scheme == 0) { /* :539-546 */
         or_upwind3d(g, m->dt, u, v, w, m->th, 0);
         or_fill_full(g, m->th, m->t1, m->t2);
-    } else {
+    } else if (m->advection_scheme == 1) {
         or_mpdata3d(g, m->dt, u, v, w, m->th, &m->t1, &m->t2, m->kord,
                     m->monotone, 0); /* :548-551 */
+    } else { /* :552-556: theta + thetan advected, thbase/thtop its bcs */
+        or_mpdata3d_add(g, m->dt, u, v, w, m->th, m->thetan, &m->thbase, &m->thtop,
+                        m->kord, m->monotone, 0);
     }
     if (m->moisture) { /* :571-580, q1 = q2 = 0 */
         for (int c = 0; c < m->ncat; ++c) {

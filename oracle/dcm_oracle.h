@@ -144,7 +144,7 @@ void or_advance_scalars(const or_grid *g, double dt, double *q,
 typedef struct {
     /* configuration */
     double dt, ptol;
-    int kord, monotone, advection_scheme;   /* 0 upwind, 1 mpdata */
+    int kord, monotone, advection_scheme;   /* 0 upwind, 1 mpdata, 2 mpdata_3d_add */
     int moisture, nq, ncat;
     const int *c_s, *c_e;                   /* 1-based category bounds */
     /* 1-D base state (kp+2) */
@@ -166,6 +166,10 @@ typedef struct {
      * subgrid work fields (strain, vism, vist; viss has nq fields) */
     const or_physics *phys;
     double *strain, *vism, *vist, *viss;
+    /* lbc/ubc of theta + thetan in advection_scheme 2: thetan(0) and
+     * thetan(kp+1) (find_base_top, mpi_module.f90:306-329), updated in
+     * place by each mpdata_3d call like the reference's INOUT scalars */
+    double thbase, thtop;
 } or_model;
 
 /* initial z-boundary rows of the previous time level (driver_code.f90:287-295) */

@@ -64,7 +64,8 @@ class _Model(C.Structure):
                 ("t1", C.c_double), ("t2", C.c_double),
                 ("last_iters", C.c_int), ("last_err", C.c_double),
                 ("phys", C.POINTER(_Physics)),
-                ("strain", _dp), ("vism", _dp), ("vist", _dp), ("viss", _dp)]
+                ("strain", _dp), ("vism", _dp), ("vist", _dp), ("viss", _dp),
+                ("thbase", C.c_double), ("thtop", C.c_double)]
 
 
 class Physics:
@@ -371,6 +372,8 @@ class Model:
                         ptr(self.psrc), ptr(self.sq),
                         ptr(self.ubar), ptr(self.vbar), ptr(self.wbar),
                         ptr(self.thbar), ptr(self.qbar), 0, 1, 0.0, 0.0, 0, 0.0)
+        # find_base_top (initialisation.f90:877): the bcs of scheme 2
+        self.m.thbase, self.m.thtop = float(d.thetan[0]), float(d.thetan[d.kp + 1])
         self.phys = None
         if case.physics is not None:
             self.phys = Physics(d, case.physics, case.nq)

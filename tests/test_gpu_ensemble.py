@@ -52,10 +52,11 @@ def _snapshot(case, m):
 
 @pytest.mark.parametrize("precision", [H.FP64, H.MIXED])
 def test_ensemble_members_evolve_as_alone(precision):
-    """Seven steps: the first eager, the next three capture the graphs of the
-    three time-level phases, the last three replay them."""
-    cases = _members()
-    steps = 7
+    """Nine steps: the first three run each time-level phase eagerly (which
+    allocates every scratch buffer), the next three capture the graphs of
+    the three phases, the last three replay them."""
+    cases = _members(9)
+    steps = 9
     # alone
     alone = []
     for case in cases:
@@ -154,5 +155,51 @@ def test_uneven_64_member_ensemble_converges():
         with H.Ensemble(models) as e:
             stats = e.step(3)
         assert all(s.max_true_resid < 1e-10 for s in stats), [s.max_true_resid for s in stats]
+    finally:
+        _close(ctxs, models)
+
+
+def test_ensemble_physics_switched_on_after_capture():
+    """Two copies of a two-member ensemble, one replaying graphs and one
+    eager.  After six steps (graphs of every phase captured) member 0 gets
+    the subgrid model, whose scratch fields do not exist yet: the replaying
+    copy runs the new phases eagerly once before capturing them (no
+    allocation or memset inside a capture), and both copies stay bit for
+    bit equal."""
+    cases = [W.warm_bubble(16, 12, 10, steps=12, amp=2.0, seed=11, name="a"),
+             W.moist_bubble(12, 8, 16, delta=50.0, dt=0.5, steps=12, ntracer=1, seed=12)]
+    ph = W.physics_profiles(cases[0].dom)
+    sets = [_open(cases, H.MIXED) for _ in range(2)]
+    try:
+        (cg, mg), (ce, me) = sets
+        with H.Ensemble(mg) as eg, H.Ensemble(me) as ee:
+            ee.set_graphs(False)
+            for s in range(12):
+                if s == 6:
+                    mg[0].set_physics(ph)
+                    me[0].set_physics(ph)
+                sg, se = eg.step(1), ee.step(1)
+                for q, case in enumerate(cases):
+                    assert sg[q].iters_total == se[q].iters_total, (s, q)
+                    a, b = _snapshot(case, mg[q]), _snapshot(case, me[q])
+                    for name in a:
+                        np.testing.assert_array_equal(a[name], b[name],
+                                                      err_msg=f"step {s} member {q} {name}")
+    finally:
+        for ctxs, models in sets:
+            _close(ctxs, models)
+
+
+def test_ensemble_refuses_a_multigrid_mix():
+    """Mixed-precision members either all use the multigrid preconditioner
+    or none: the batched solve has no V-cycle."""
+    cases = [W.warm_bubble(16, 16, 16, steps=2, seed=11, name="a"),
+             W.warm_bubble(16, 16, 16, steps=2, seed=12, name="b")]
+    ctxs, models = _open(cases, H.MIXED)
+    try:
+        ctxs[0].set_precond_multigrid(True)
+        with H.Ensemble(models) as e:
+            with pytest.raises(H.DcmError):
+                e.step(1)
     finally:
         _close(ctxs, models)

@@ -258,6 +258,33 @@ def test_bicgstab_early_return_and_not_converged():
         assert st == 1 and it == 3
 
 
+def test_bicgstab_test_solver_matches_oracle():
+    """bicgstab(..., test_solver=.true.) (dynamics.f90:354-364): the first
+    call replaces b with the Gaussian exp(-(x^2 + y^2 + (z-4000)^2)/5e6) and
+    solves for it.  That right-hand side has a non-zero mean, which the
+    singular operator cannot reach, so both stop at itmax; under sequential
+    sums b and every iterate agree bit for bit."""
+    dom = small_domain(10, 8, 12, varying_theta=True)
+    g = O.Grid(dom)
+    sw, _ = _poisson_problem(dom, 151)
+    b0 = np.zeros(dom.shape)
+    x0 = 0.01 * field(dom, 152)
+    xr, br = x0.copy(), b0.copy()
+    st, it, err = O.bicgstab(g, sw, xr, br, 1e-10, test_solver=True, itmax=25)
+    assert st == 1 and it == 25
+    assert interior(br).min() > 0.0
+    with _ctx(dom) as c:
+        c.set_sum_order(H.SUM_SEQUENTIAL)
+        dx_, db = c.array(x0), c.array(b0)
+        gst, git, gerr = c.bicgstab(c.array(sw), dx_, db, 1e-10, test_solver=True, itmax=25,
+                                    check=False)
+        assert (gst, git) == (st, it)
+        np.testing.assert_array_equal(interior(db.get()), interior(br))
+        mask = ~lateral_edge_mask(dom)
+        np.testing.assert_array_equal(dx_.get()[mask], xr[mask])
+        assert gerr == err
+
+
 def test_bicgstab_mixed_residual():
     dom = small_domain(16, 16, 16)
     sw, b = _poisson_problem(dom, 141)
@@ -373,6 +400,36 @@ def test_model_moist_bubble_parity():
 def test_model_kelvin_helmholtz_parity():
     case = W.kelvin_helmholtz(24, 4, 16, steps=5)
     _compare_models(case, 5)
+
+
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_model_advection_scheme_parity(scheme):
+    """advection_scheme 0 (donor cell, then exchange_full(th, t1, t2)) and 2
+    (mpdata_3d_add of theta + thetan with thbase, thtop; no adv_ref_state),
+    driver_code.f90:511-561, on the sheared case whose thetan varies: the
+    model is bit-identical to the oracle's under sequential sums."""
+    case = W.kelvin_helmholtz(24, 4, 16, steps=4)
+    case.advection_scheme = scheme
+    _compare_models(case, 4)
+
+
+@pytest.mark.parametrize("scheme", [0, 2])
+def test_model_advection_scheme_mixed(scheme):
+    """The same schemes on the mixed path: every solve at true residual <
+    1e-10 and the fields within 1e-5 of the FP64 path."""
+    case = W.kelvin_helmholtz(24, 4, 16, steps=4)
+    case.advection_scheme = scheme
+    out = {}
+    for prec in (H.FP64, H.MIXED):
+        with H.Context(case.dom) as c:
+            gm = H.Model(c, case, prec)
+            gm.set_check_residual(True)
+            st = gm.step(4)
+            assert st.max_true_resid < 1e-10
+            out[prec] = (gm.get(H.F_TH), gm.get(H.F_U), gm.get(H.F_W))
+            gm.close()
+    for a, b in zip(out[H.FP64], out[H.MIXED]):
+        assert rel_l2(a, b) < 1e-5
 
 
 @pytest.mark.parametrize("make", [lambda: W.warm_bubble(16, 16, 16, steps=4),

@@ -9,6 +9,7 @@ Smolarkiewicz & Grabowski (1990), and conservation / monotonicity /
 projection properties.  A transcription slip in the oracle (dropped term,
 wrong sign, wrong index, transposed operand) fails at least one of them.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -795,6 +796,25 @@ def test_model_conserves_theta_mass_and_is_deterministic():
     m2.step(3)
     np.testing.assert_array_equal(m1.th, m2.th)
     assert m1.iters == m2.iters
+
+
+def test_model_advection_scheme_2_is_scheme_1_at_constant_thetan():
+    """advection_scheme 2 advects theta + thetan with thetan(0), thetan(kp+1)
+    as its bcs (driver_code.f90:552-556) and skips adv_ref_state; with a
+    constant thetan the shift m differs from scheme 1's by exactly that
+    constant, so both schemes agree to rounding.  With a varying thetan
+    (the sheared case) they differ."""
+    case = W.warm_bubble(10, 8, 9, steps=3)
+    a, b = O.Model(case), O.Model(dataclasses.replace(case, advection_scheme=2))
+    a.step(3)
+    b.step(3)
+    assert rel_l2(a.th, b.th) < 1e-12 * np.abs(case.dom.thetan).max()
+    kh = W.kelvin_helmholtz(16, 4, 12, steps=3)
+    m1, m2 = O.Model(kh), O.Model(dataclasses.replace(kh, advection_scheme=2))
+    m1.step(3)
+    m2.step(3)
+    assert rel_l2(m1.th, m2.th) > 1e-9
+    assert (m2.m.thbase, m2.m.thtop) == (kh.dom.thetan[0], kh.dom.thetan[-1])
 
 
 def test_sum_order_sensitivity():

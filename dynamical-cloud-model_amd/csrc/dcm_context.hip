@@ -376,6 +376,46 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
         *m64[a] = c->dmet + offs[a];
         *m32[a] = c->dmet32 + offs[a];
     }
+    // FP32 tables of the mixed path's MPDATA: each entry one FP64 quotient
+    // rounded to FP32
+    {
+        const int ni = ip + 2, nj = jp + 2, nk = kp + 2;
+        std::vector<float> t(4 * (size_t)ni + 4 * (size_t)nj + 7 * (size_t)nk, 0.f);
+        float *x = t.data(), *y = x + 4 * ni, *z = y + 4 * nj;
+        for (int i = 0; i < ni; ++i) {
+            x[4 * i] = (float)dx[i];
+            x[4 * i + 1] = (float)(1.0 / dxn[i]);
+            x[4 * i + 2] = i > 0 ? (float)(1.0 / dx[i - 1]) : 0.f;
+            x[4 * i + 3] = i > 0 ? (float)(2.0 / (dxn[i - 1] + dxn[i])) : 0.f;
+        }
+        for (int j = 0; j < nj; ++j) {
+            y[j] = (float)dy[j];
+            y[nj + j] = (float)(1.0 / dy[j]);
+            y[2 * nj + j] = (float)(1.0 / dyn[j]);
+            y[3 * nj + j] = j > 0 ? (float)(2.0 / (dyn[j - 1] + dyn[j])) : 0.f;
+        }
+        for (int k = 0; k < nk; ++k) {
+            z[k] = (float)dz[k];
+            z[nk + k] = (float)(1.0 / dz[k]);
+            z[2 * nk + k] = (float)(1.0 / dzn[k]);
+            z[3 * nk + k] = k > 0 ? (float)(2.0 / (dzn[k - 1] + dzn[k])) : 0.f;
+            z[4 * nk + k] = (float)rhoa[k];
+            z[5 * nk + k] = (float)rhoan[k];
+            z[6 * nk + k] = (float)(1.0 / rhoan[k]);
+        }
+        void *pt = nullptr;
+        if (hipMalloc(&pt, t.size() * sizeof(float)) != hipSuccess) {
+            set_error("hipMalloc of the FP32 tables failed");
+            return DCM_OUT_OF_MEMORY;
+        }
+        c->dtab32 = (float *)pt;
+        (void)hipMemcpy(c->dtab32, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+        const float *dy32 = c->dtab32 + 4 * ni, *dz32 = dy32 + 4 * nj;
+        c->tab32 = Tab32{(const float4 *)c->dtab32,
+                         dy32,          dy32 + nj,     dy32 + 2 * nj, dy32 + 3 * nj,
+                         dz32,          dz32 + nk,     dz32 + 2 * nk, dz32 + 3 * nk,
+                         dz32 + 4 * nk, dz32 + 5 * nk, dz32 + 6 * nk};
+    }
     if (hipMalloc(&p, 64 * sizeof(double)) != hipSuccess) {
         set_error("hipMalloc failed");
         return DCM_OUT_OF_MEMORY;
@@ -430,6 +470,8 @@ int dcm_context_destroy(dcm_ctx *c) {
     (void)hipFree(c->d1d);
     (void)hipFree(c->dmet);
     (void)hipFree(c->dmet32);
+    (void)hipFree(c->dtab32);
+    (void)hipFree(c->mp32);
     (void)hipFree(c->dscal);
     (void)hipHostFree(c->hscal);
     (void)hipFree(c->solver);

@@ -984,23 +984,45 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
         wf = vf + g.ntot;
         DCM_CHECK(velocity_shadow(c, cur[0], cur[1], cur[2], uf, vf, wf));
     }
+    // the bcs of every q category: q1 = q2 = 0 (:571-580)
+    std::vector<double> zeros(m->nq > 0 ? m->nq : 1, 0.0);
     if (m->advection_scheme == 0) {
         // :539-546 first_order_upstream_3d, then exchange_full(th, t1, t2);
         // FP64 on both paths (one pass, no intermediates to narrow)
         DCM_CHECK(upwind_dev(c, m->dt, cur[0], cur[1], cur[2], m->th, 0));
         DCM_CHECK(fill<double>(c, m->th, FILL_FULL, m->t1, m->t2));
-    } else {
+    }
+    if (mixed) {
+        // theta (mpdata_3d, :548-556) and every q category (mpdata_vec_3d,
+        // :571-580) in one batched FP32 MPDATA: the fields are independent
+        const bool add = m->advection_scheme == 2;
+        double lb = add ? m->thbase : m->t1, ub = add ? m->thtop : m->t2;
+        std::vector<Mp32Cat> cats;
+        if (m->advection_scheme != 0) cats.push_back(Mp32Cat{m->th, 1, &lb, &ub, 1});
+        for (int cat = 0; cat < m->ncat; ++cat) {
+            const int nf = m->c_e[cat] - m->c_s[cat] + 1;
+            cats.push_back(Mp32Cat{m->q + (size_t)(m->c_s[cat] - 1) * g.ntot, nf,
+                                   zeros.data() + (m->c_s[cat] - 1),
+                                   zeros.data() + (m->c_s[cat] - 1), 0});
+        }
+        if (add) DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), 1.0));
+        if (!cats.empty())
+            DCM_CHECK(mpdata32_batch(c, (float)m->dt, uf, vf, wf, cats.data(), (int)cats.size(),
+                                     m->kord, m->monotone, 0));
+        if (add) {
+            hipLaunchKernelGGL(k_bc_roundtrip, dim3(1), dim3(64), 0, c->stream,
+                               (const double *)(c->dscal + 16), lb, ub, m->bc_inexact);
+            DCM_LAUNCH_CHECK();
+            DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), -1.0));
+        }
+    } else if (m->advection_scheme != 0) {
         // :548-551 mpdata_3d(th, t1, t2); :552-556 mpdata_3d_add: theta +
         // thetan advected with thbase, thtop as its bcs
         const bool add = m->advection_scheme == 2;
         double lb = add ? m->thbase : m->t1, ub = add ? m->thtop : m->t2;
         if (add) DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), 1.0));
-        if (mixed)
-            st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, m->th, 1, &lb, &ub, m->kord,
-                                    m->monotone, 0, 0);
-        else
-            st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], m->th, 1, &lb, &ub,
-                                     m->kord, m->monotone, 0, 0);
+        st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], m->th, 1, &lb, &ub, m->kord,
+                                 m->monotone, 0, 0);
         if (st != DCM_OK) return st;
         if (add) {
             // t1 = t2 = 0 come back from (0 - m) + m exactly; thbase and
@@ -1011,17 +1033,13 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
             DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), -1.0));
         }
     }
-    if (m->nq > 0) {  // :571-580 with q1 = q2 = 0
+    if (!mixed && m->nq > 0) {  // :571-580 with q1 = q2 = 0
         for (int cat = 0; cat < m->ncat; ++cat) {
             const int nf = m->c_e[cat] - m->c_s[cat] + 1;
-            std::vector<double> z(nf, 0.0);
             double *f = m->q + (size_t)(m->c_s[cat] - 1) * g.ntot;
-            if (mixed)
-                st = mpdata_core<float>(c, (float)m->dt, uf, vf, wf, f, nf, z.data(), z.data(), m->kord,
-                                        m->monotone, 0, 1);
-            else
-                st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], f, nf, z.data(),
-                                         z.data(), m->kord, m->monotone, 0, 1);
+            const double *z = zeros.data() + (m->c_s[cat] - 1);
+            st = mpdata_core<double>(c, m->dt, cur[0], cur[1], cur[2], f, nf, z, z, m->kord,
+                                     m->monotone, 0, 1);
             if (st != DCM_OK) return st;
         }
     }

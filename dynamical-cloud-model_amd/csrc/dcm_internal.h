@@ -71,6 +71,19 @@ struct Metrics32 {
     const float *ae, *aw, *an, *as, *au, *ad;
 };
 
+// FP32 coefficient tables of the mixed path's MPDATA (dcm_mpdata32.hip),
+// each quotient of FP64 metrics rounded once: spacings, their reciprocals
+// (idy = 1/dy, idyn = 1/dyn), the reciprocal half-sums of the cell widths
+// (ihy(j) = 2/(dyn(j-1) + dyn(j)), 0 at j = 0), and along z the densities
+// and 1/rhoan.  Along x, read once per i plane by a whole block, the four
+// values of plane i are packed: xpk(i) = (dx(i), 1/dxn(i), 1/dx(i-1),
+// 2/(dxn(i-1) + dxn(i))), zeros where i-1 < 0.  Fortran index, n+2 entries.
+struct Tab32 {
+    const float4 *xpk;
+    const float *dy, *idy, *idyn, *ihy;
+    const float *dz, *idz, *idzn, *ihz, *ra, *rn, *irn;
+};
+
 // Scalars of the BiCGSTAB recurrences, resident on the device.
 struct SolverScalars {
     double rho, nrm, alf, omg, bet;
@@ -143,6 +156,11 @@ struct dcm_ctx {
     float *dmet32 = nullptr;
     dcm::Metrics met;
     dcm::Metrics32 met32;
+    float *dtab32 = nullptr;         // the FP32 MPDATA tables, one allocation
+    dcm::Tab32 tab32;
+    // scratch of the batched FP32 MPDATA: mp32_count padded FP32 fields
+    float *mp32 = nullptr;
+    size_t mp32_count = 0;
     std::vector<double> h_dx, h_dy, h_dz, h_dxn, h_dyn, h_dzn, h_rhoa, h_rhoan;
     // reduction scratch
     double *partials = nullptr;      // per-block partial sums (several slots)
@@ -314,6 +332,22 @@ template <typename R>
 int mpdata_core(dcm_ctx *c, R dt, const R *u, const R *v, const R *w, double *psi, int nf,
                 const double *lbc, const double *ubc, int kord, int monotone, int neumann,
                 int interior_min);
+// One call of mpdata_3d (whole_min 1: the minimum over the whole array) or
+// mpdata_vec_3d (whole_min 0: per-field interior minima) on nf consecutive
+// FP64 fields at psi, with host z-plane values lbc[nf], ubc[nf].
+struct Mp32Cat {
+    double *psi;
+    int nf;
+    const double *lbc, *ubc;
+    int whole_min;
+};
+// The mixed path's MPDATA of several such calls at once, FP32 intermediates
+// from the FP32 velocity shadows (dcm_mpdata32.hip).  Categories the batched
+// kernels do not serve (kord != 2, the sfvt/advection_3d.F90 arithmetic)
+// go through mpdata_core<float>.  The shift of field f of the batch is left
+// at dscal[16 + f].
+int mpdata32_batch(dcm_ctx *c, float dt, const float *u, const float *v, const float *w,
+                   const Mp32Cat *cats, int ncat, int kord, int monotone, int neumann);
 // vertical-only MPDATA of nf fields (FP64); interior_min: the vector form
 int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf,
                      const double *lbc, const double *ubc, int kord, int monotone, int neumann,

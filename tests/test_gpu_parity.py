@@ -263,7 +263,10 @@ def test_bicgstab_test_solver_matches_oracle():
     call replaces b with the Gaussian exp(-(x^2 + y^2 + (z-4000)^2)/5e6) and
     solves for it.  That right-hand side has a non-zero mean, which the
     singular operator cannot reach, so both stop at itmax; under sequential
-    sums b and every iterate agree bit for bit."""
+    sums b and every iterate agree bit for bit.  Only the interior of x is
+    compared: after a solve that does not converge the reference stops the
+    program (:431-434), so the halo it would leave is never observed (the
+    oracle's whole-array updates move it, the GPU's compact vectors do not)."""
     dom = small_domain(10, 8, 12, varying_theta=True)
     g = O.Grid(dom)
     sw, _ = _poisson_problem(dom, 151)
@@ -280,8 +283,7 @@ def test_bicgstab_test_solver_matches_oracle():
                                     check=False)
         assert (gst, git) == (st, it)
         np.testing.assert_array_equal(interior(db.get()), interior(br))
-        mask = ~lateral_edge_mask(dom)
-        np.testing.assert_array_equal(dx_.get()[mask], xr[mask])
+        np.testing.assert_array_equal(interior(dx_.get()), interior(xr))
         assert gerr == err
 
 

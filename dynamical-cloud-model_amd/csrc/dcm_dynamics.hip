@@ -977,13 +977,6 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
         DCM_CHECK(fill<double>(c, m->th, FILL_FULL, 0.0, 0.0));            // :526-527
     }
     const bool mixed = m->precision == DCM_MIXED;
-    float *uf = nullptr, *vf = nullptr, *wf = nullptr;
-    if (mixed) {
-        uf = m->shadow;
-        vf = uf + g.ntot;
-        wf = vf + g.ntot;
-        DCM_CHECK(velocity_shadow(c, cur[0], cur[1], cur[2], uf, vf, wf));
-    }
     // the bcs of every q category: q1 = q2 = 0 (:571-580)
     std::vector<double> zeros(m->nq > 0 ? m->nq : 1, 0.0);
     if (m->advection_scheme == 0) {
@@ -1007,8 +1000,8 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
         }
         if (add) DCM_CHECK(add_profile_dev(c, m->th, m->thetan(), 1.0));
         if (!cats.empty())
-            DCM_CHECK(mpdata32_batch(c, (float)m->dt, uf, vf, wf, cats.data(), (int)cats.size(),
-                                     m->kord, m->monotone, 0));
+            DCM_CHECK(mpdata32_batch(c, (float)m->dt, cur[0], cur[1], cur[2], m->shadow,
+                                     cats.data(), (int)cats.size(), m->kord, m->monotone, 0));
         if (add) {
             hipLaunchKernelGGL(k_bc_roundtrip, dim3(1), dim3(64), 0, c->stream,
                                (const double *)(c->dscal + 16), lb, ub, m->bc_inexact);

@@ -341,13 +341,15 @@ struct Mp32Cat {
     const double *lbc, *ubc;
     int whole_min;
 };
-// The mixed path's MPDATA of several such calls at once, FP32 intermediates
-// from the FP32 velocity shadows (dcm_mpdata32.hip).  Categories the batched
-// kernels do not serve (kord != 2, the sfvt/advection_3d.F90 arithmetic)
-// go through mpdata_core<float>.  The shift of field f of the batch is left
-// at dscal[16 + f].
-int mpdata32_batch(dcm_ctx *c, float dt, const float *u, const float *v, const float *w,
-                   const Mp32Cat *cats, int ncat, int kord, int monotone, int neumann);
+// The mixed path's MPDATA of several such calls at once with the FP64
+// velocities u, v, w (padded, halos filled), FP32 intermediates
+// (dcm_mpdata32.hip).  Categories the batched kernels do not serve (kord !=
+// 2, the sfvt/advection_3d.F90 arithmetic) go through mpdata_core<float>
+// with FP32 shadows of u, v, w written to shadow (3 padded fields).  The
+// shift of field f of the batch is left at dscal[16 + f].
+int mpdata32_batch(dcm_ctx *c, float dt, const double *u, const double *v, const double *w,
+                   float *shadow, const Mp32Cat *cats, int ncat, int kord, int monotone,
+                   int neumann);
 // vertical-only MPDATA of nf fields (FP64); interior_min: the vector form
 int mpdata_vert_core(dcm_ctx *c, double dt, const double *w, double *psi, int nf,
                      const double *lbc, const double *ubc, int kord, int monotone, int neumann,

@@ -14,26 +14,38 @@
 //     (Tab32), computed once per thread or per i plane; the nine quotients
 //     per cell that depend on the field are hardware reciprocals, as in the
 //     FP32 instantiation of dcm_advect.hip;
-//   * a thread owns one (k,j) column of a 64x4 or 32x8 tile and marches a
-//     chunk of i planes, keeping the column sums of planes i-1, i, i+1 in
-//     registers (the flux through the lower x face of a cell is the upper x
-//     face's flux of the cell before it);
+//   * the FP32 arrays (the velocity shadows and every intermediate) use an
+//     aligned copy of the padded layout: each (j,i) column has a stride that
+//     is a multiple of 32 elements and level k = 1 starts a 128-byte line,
+//     so a wavefront's 64 consecutive levels are two whole lines (the
+//     reference's kp+2 stride would straddle three);
+//   * a thread owns one (k,j) column of a tile and marches a chunk of i
+//     planes, keeping the column sums of planes i-1, i, i+1 in registers
+//     (the flux through the lower x face of a cell is the upper x face's
+//     flux of the cell before it);
 //   * each pass writes, besides a cell, the halo cells that the reference's
-//     exchange would copy from it (put_full / put_faces): no fill kernel runs;
+//     exchange would copy from it: no fill kernel runs;
+//   * the limiter factors never reach HBM: the last kernel computes them one
+//     i plane ahead into a ring of four planes in LDS (the tile's rows plus
+//     one halo row on each side) and applies the limited donor pass to the
+//     plane behind, so beta and the last donor pass cost one read of their
+//     inputs and one write of psi;
 //   * each pass serves every field (or every category's first field) in one
-//     launch: four passes, a minimum and two small reductions per step
-//     instead of about a dozen launches per field.
+//     launch: per step a minimum, three passes and two small reductions.
 // Offsets are 32-bit: a padded field is below 2^31 elements (checked).
 //
+//   K0 k32_shadow                 FP32 aligned copies of u, v, w
 //   K1 k32_min / k32_min_finish   m(f) = min(psi_f) (+ lbc, ubc)
 //   K2 k32_donor1                 psi1_f = donor(psi_f - m), halo images;
-//                                 block sums of psi - m for the exit test
+//                                 psi_store = psi - m of each category's
+//                                 first field; block sums of psi - m
 //      k32_exit_flags             flag(L) = (sum < small), per category
 //   K3 k32_antidiff               U*, V*, W* of each category (first field)
-//   K4 k32_beta                   beta_up, beta_down of each category
-//   K5 k32_limited                psi_f = donor(psi1_f, limited U*) + m
+//   K4 k32_betalim                beta of the category (LDS), then
+//                                 psi_f = donor(psi1_f, limited U*) + m
 //      k32_exit_unshift           psi = (psi - m) + m where the exit was taken
 #include <cfloat>
+#include <algorithm>
 #include <vector>
 
 #include "dcm_device.h"
@@ -44,17 +56,23 @@
 
 namespace dcm {
 
-constexpr int kMp32Max = 16;   // fields per batched launch
-constexpr int kMp32Planes = 16; // i planes per marching tile
+constexpr int kMp32Max = 16;     // fields per batched launch
+constexpr int kMp32Planes = 16;  // i planes per marching tile
 constexpr int kMinBlocks = 1024;
+constexpr int kK4Threads = 1024; // largest block of the beta + limited kernel
+constexpr int kK4MaxKt = 256;    // its column of levels: kp <= 256
 
-// marching over the padded layout of one field
+// geometry of one field: the FP64 reference layout (sj, si) and the aligned
+// FP32 layout (sj4, si4, level k at column offset k + kK0), and the
+// 256-thread marching tiles of K2 and K3
 struct PM {
     int ip, jp, kp;
-    int sj, si;                        // padded strides
+    int sj, si;                        // FP64 strides
+    int sj4, si4;                      // FP32 strides
     int kt, jt, nkb, njb, ic, nic;     // tile lanes (k), rows (j); tiles; i chunks
     int ntiles;                        // tiles per field
 };
+constexpr int kK0 = 31;  // column offset of level 0 in the aligned layout
 
 static PM pm_geo(const Geo &g, int ic) {
     PM m;
@@ -63,6 +81,8 @@ static PM pm_geo(const Geo &g, int ic) {
     m.kp = g.kp;
     m.sj = (int)g.sj;
     m.si = (int)g.si;
+    m.sj4 = ((g.kp + 2 + 31) / 32) * 32;
+    m.si4 = m.sj4 * g.nj;
     const int w64 = (64 - g.kp % 64) % 64, w32 = (32 - g.kp % 32) % 32;
     m.kt = g.kp > 32 && w64 <= w32 ? 64 : 32;
     m.jt = kBlock / m.kt;
@@ -73,20 +93,30 @@ static PM pm_geo(const Geo &g, int ic) {
     m.ntiles = m.nkb * m.njb * m.nic;
     return m;
 }
+// elements of one aligned FP32 array (a multiple of 64)
+static long long ntot4_of(const PM &m) {
+    const long long n = (long long)m.si4 * (m.ip + 2) + 32;
+    return ((n + 63) / 64) * 64;
+}
 
-// Block -> (field of the launch, tile).  The (field, tile) list is split
-// into 8 contiguous ranges, one per XCD (block b runs on XCD b mod 8), so
-// tiles that share faces share an L2.
+// Block -> item of a launch.  The item list is split into 8 contiguous
+// ranges, one per XCD (block b runs on XCD b mod 8), so tiles that share
+// faces share an L2.
+__device__ __forceinline__ bool xcd_item(int total, int &t) {
+    const int per = (total + 7) >> 3;
+    const int b = (int)blockIdx.x;
+    t = (b & 7) * per + (b >> 3);
+    return t < total;
+}
+static unsigned xcd_grid(long long total) { return (unsigned)(((total + 7) / 8) * 8); }
+
 struct Col {
     int f, tile, k, j, i0, i1;
     bool act;
 };
 __device__ __forceinline__ bool pm_col(const PM &m, int nf, Col &C) {
-    const int total = nf * m.ntiles;
-    const int per = (total + 7) >> 3;
-    const int b = (int)blockIdx.x;
-    const int t = (b & 7) * per + (b >> 3);
-    if (t >= total) return false;
+    int t;
+    if (!xcd_item(nf * m.ntiles, t)) return false;
     C.f = t / m.ntiles;
     C.tile = t - C.f * m.ntiles;
     const int kb = C.tile % m.nkb;
@@ -101,14 +131,13 @@ __device__ __forceinline__ bool pm_col(const PM &m, int nf, Col &C) {
     C.act = C.k <= m.kp && C.j <= m.jp;
     return true;
 }
-static unsigned pm_grid(const PM &m, int nf) {
-    const long long total = (long long)nf * m.ntiles;
-    return (unsigned)(((total + 7) / 8) * 8);
-}
 
 // The batch: fields f = 0..nf-1; category leaders (first fields) L =
-// 0..nlead-1.  Scratch: psi1 of field f at s32 + f*ntot; U*, V*, W*,
-// beta_up, beta_down of leader L at s32 + (nf + 5L + 0..4)*ntot.
+// 0..nlead-1.  FP32 scratch (aligned arrays of ntot4 elements): the velocity
+// shadows u, v, w; psi1 of field f at 3+f; U*, V*, W* of leader L at
+// 3+nf+3L; psi_store of leader L at 3+nf+3*nlead+L; and, for the separate
+// limiter kernels (kp > 256), beta_up and beta_down of leader L at
+// 3+nf+4*nlead+2L.
 struct Mp32Args {
     double *psi[kMp32Max];
     double lbc[kMp32Max], ubc[kMp32Max];
@@ -116,7 +145,7 @@ struct Mp32Args {
     signed char lfield[kMp32Max];  // field of leader slot L
     signed char whole[kMp32Max];   // minimum over the whole array (mpdata_3d)
     float *s32;
-    long long ntot;
+    long long ntot4;
     double *m;                     // shifts, m[f]
     int *flag;                     // zero-sum exit, flag[L]
     double *partials;              // min partials (K1), exit sums (K2)
@@ -124,6 +153,19 @@ struct Mp32Args {
     float dt;
     int neumann;
 };
+__device__ __forceinline__ float *arr(const Mp32Args &A, int slot) {
+    return A.s32 + (size_t)slot * A.ntot4;
+}
+__device__ __forceinline__ float *psi1_of(const Mp32Args &A, int f) { return arr(A, 3 + f); }
+__device__ __forceinline__ float *ustar_of(const Mp32Args &A, int L, int d) {
+    return arr(A, 3 + A.nf + 3 * L + d);
+}
+__device__ __forceinline__ float *store_of(const Mp32Args &A, int L) {
+    return arr(A, 3 + A.nf + 3 * A.nlead + L);
+}
+__device__ __forceinline__ float *beta_of(const Mp32Args &A, int L, int d) {
+    return arr(A, 3 + A.nf + 4 * A.nlead + 2 * L + d);
+}
 
 template <typename T> __device__ __forceinline__ T pos0(T x) { return x > T(0) ? x : T(0); }
 template <typename T> __device__ __forceinline__ T neg0(T x) { return x < T(0) ? x : T(0); }
@@ -131,15 +173,17 @@ __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x);
 __device__ __forceinline__ float min3f(float a, float b, float c) {
     return fminf(fminf(a, b), c);
 }
+__device__ __forceinline__ int wrapi(int x, int n) { return x < 1 ? x + n : (x > n ? x - n : x); }
 
 // A uniform base pointer read at a 32-bit byte offset: the load takes the
 // base from scalar registers and the offset from one vector register, so a
 // stencil of loads costs no address arithmetic (every neighbour has a base
-// of its own, shifted by a constant or a stride).
+// of its own, shifted by a constant or a stride, and the k neighbours are
+// immediates).
 template <typename T> struct Ld {
     const char *b;
     __device__ __forceinline__ explicit Ld(const T *p) : b((const char *)p) {}
-    // element dk along k (an immediate) from the byte offset off
+    // element dk along k from the byte offset off
     __device__ __forceinline__ T at(unsigned off, int dk = 0) const {
         return *(const T *)(b + off + dk * (int)sizeof(T));
     }
@@ -152,52 +196,111 @@ template <typename T> struct St {
     }
 };
 
-// The halo cells that exchange_full copies from interior cell (k,j,i)
-// (offset c): the periodic images on the lateral faces and edges, and on
-// the z planes the z-wrapped edges (the receive from MPI_PROC_NULL keeps the
-// local copy, mpi_module.f90:1244-1263): a z-plane cell outside the lateral
-// interior holds level kp at k = 0 and level 1 at kp+1.  The lateral-interior
-// z-plane cells are stored by the caller (k = 1 and k = kp lanes).
+// The halo cells that exchange_full copies from interior cell (k,j,i) at
+// element c of an array with strides sj, si: the periodic images on the
+// lateral faces and edges (j = jp+1 copies j = 1, j = 0 copies j = jp,
+// likewise along i), and on the z planes the z-wrapped edges (the receive
+// from MPI_PROC_NULL keeps the local copy, mpi_module.f90:1244-1263): a
+// z-plane cell outside the lateral interior holds level kp at k = 0 and
+// level 1 at kp+1.  The lateral-interior z-plane cells are stored by the
+// caller (k = 1 and k = kp lanes).
 template <typename T>
-__device__ __forceinline__ void put_images(T *a, int c, T v, int k, int j, int i, const PM &m) {
+__device__ __forceinline__ void put_images(T *a, int c, T v, int k, int j, int i, const PM &m,
+                                           int sj, int si) {
     int dj[3], di[3];
     int nj = 1, ni = 1;
     dj[0] = 0;
     di[0] = 0;
-    if (j == 1) dj[nj++] = m.jp * m.sj;    // j = jp+1 copies j = 1
-    if (j == m.jp) dj[nj++] = -m.jp * m.sj; // j = 0 copies j = jp
-    if (i == 1) di[ni++] = m.ip * m.si;
-    if (i == m.ip) di[ni++] = -m.ip * m.si;
-    for (int a1 = 0; a1 < nj; ++a1)
-        for (int b1 = 0; b1 < ni; ++b1) {
-            if (a1 == 0 && b1 == 0) continue;
+    if (j == 1) dj[nj++] = m.jp * sj;
+    if (j == m.jp) dj[nj++] = -m.jp * sj;
+    if (i == 1) di[ni++] = m.ip * si;
+    if (i == m.ip) di[ni++] = -m.ip * si;
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int b1 = 0; b1 < 3; ++b1) {
+            if (a1 >= nj || b1 >= ni || (a1 == 0 && b1 == 0)) continue;
             const int o = c + dj[a1] + di[b1];
             a[o] = v;
             if (k == m.kp) a[o - m.kp] = v;  // level 0 of a lateral halo column
             if (k == 1) a[o + m.kp] = v;     // level kp+1
         }
 }
+// psi_store = psi - m at the same halo cells (FP32, aligned c4) from the FP64
+// field's own halo cells (FP64 layout c8), as the reference's psi_store copy
+// of the whole array holds them
+__device__ __forceinline__ void put_store_images(float *q, const double *psi, double mm, int c4,
+                                                 int c8, int k, int j, int i, const PM &m) {
+    int dj4[3], di4[3], dj8[3], di8[3];
+    int nj = 1, ni = 1;
+    dj4[0] = di4[0] = dj8[0] = di8[0] = 0;
+    if (j == 1) {
+        dj4[nj] = m.jp * m.sj4;
+        dj8[nj++] = m.jp * m.sj;
+    }
+    if (j == m.jp) {
+        dj4[nj] = -m.jp * m.sj4;
+        dj8[nj++] = -m.jp * m.sj;
+    }
+    if (i == 1) {
+        di4[ni] = m.ip * m.si4;
+        di8[ni++] = m.ip * m.si;
+    }
+    if (i == m.ip) {
+        di4[ni] = -m.ip * m.si4;
+        di8[ni++] = -m.ip * m.si;
+    }
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int b1 = 0; b1 < 3; ++b1) {
+            if (a1 >= nj || b1 >= ni || (a1 == 0 && b1 == 0)) continue;
+            const int o4 = c4 + dj4[a1] + di4[b1], o8 = c8 + dj8[a1] + di8[b1];
+            q[o4] = (float)(psi[o8] - mm);
+            if (k == m.kp) q[o4 - m.kp] = (float)(psi[o8 - m.kp] - mm);
+            if (k == 1) q[o4 + m.kp] = (float)(psi[o8 + m.kp] - mm);
+        }
+}
 // exchange_along_dim with lbc = ubc = 0 (the limiter factors, :624-627):
 // periodic faces (the edges are left alone: nothing reads them) and zero z
-// planes over the whole plane, lateral halo included.
+// planes over the whole plane, lateral halo included.  Aligned FP32 layout.
 __device__ __forceinline__ void put_faces(float *a, int c, float v, int k, int j, int i,
-                                       const PM &m) {
+                                          const PM &m) {
     int dj[3], di[3];
     int nj = 1, ni = 1;
     dj[0] = 0;
     di[0] = 0;
-    if (j == 1) dj[nj++] = m.jp * m.sj;
-    if (j == m.jp) dj[nj++] = -m.jp * m.sj;
-    if (i == 1) di[ni++] = m.ip * m.si;
-    if (i == m.ip) di[ni++] = -m.ip * m.si;
-    for (int a1 = 0; a1 < nj; ++a1)
-        for (int b1 = 0; b1 < ni; ++b1) {
-            if (a1 == 0 && b1 == 0) continue;
+    if (j == 1) dj[nj++] = m.jp * m.sj4;
+    if (j == m.jp) dj[nj++] = -m.jp * m.sj4;
+    if (i == 1) di[ni++] = m.ip * m.si4;
+    if (i == m.ip) di[ni++] = -m.ip * m.si4;
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int b1 = 0; b1 < 3; ++b1) {
+            if (a1 >= nj || b1 >= ni || (a1 == 0 && b1 == 0)) continue;
             const int o = c + dj[a1] + di[b1];
-            if (a1 == 0 || b1 == 0) a[o] = v;  // a face
+            if (a1 == 0 || b1 == 0) a[o] = v;  // a face, not an edge
             if (k == 1) a[o - 1] = 0.f;
             if (k == m.kp) a[o + 1] = 0.f;
         }
+}
+
+// ---------------------------------------------------------------------------
+// K0: the aligned FP32 copies of u, v, w (every padded cell)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k32_shadow(const double *__restrict__ u,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ w, float *us,
+                                                  float *vs, float *ws, PM m, int nk) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= nk || j > m.jp + 1) return;
+    const int c8 = i * m.si + j * m.sj + k, c4 = i * m.si4 + j * m.sj4 + k + kK0;
+    us[c4] = (float)u[c8];
+    vs[c4] = (float)v[c8];
+    ws[c4] = (float)w[c8];
 }
 
 // ---------------------------------------------------------------------------
@@ -244,27 +347,27 @@ __global__ void k32_min_finish(Mp32Args A, int nb) {
 // ---------------------------------------------------------------------------
 // K2: donor pass on psi - m (first_order_upstream_3d, :74-115), FP32 flux
 // arithmetic from the FP64 difference psi - m; psi1 with the halo of
-// exchange_full(psi1, lbc - m, ubc - m).  Leaders also store block sums of
-// psi - m over the interior (the exit test, :282-289 / :1314-1315).
+// exchange_full(psi1, lbc - m, ubc - m).  Leaders also store psi_store =
+// psi - m (whole array) for the limiter and block sums of psi - m over the
+// interior (the exit test, :282-289 / :1314-1315).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T,
-                                                  const float *__restrict__ u,
-                                                  const float *__restrict__ v,
-                                                  const float *__restrict__ w) {
+__global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T) {
     Col C;
     if (!pm_col(m, A.nf, C)) return;
     const int f = C.f;
     const int L = A.lslot[f];
     const bool lead = A.lfield[L] == f;
     const double *psi = A.psi[f];
-    float *out = A.s32 + (size_t)f * A.ntot;
+    float *out = psi1_of(A, f);
+    float *sto = store_of(A, L);
+    const float *u = arr(A, 0), *v = arr(A, 1), *w = arr(A, 2);
     const double mm = A.m[f];
     double s[1] = {0.0};
     if (C.act) {
-        const int k = C.k, j = C.j, SI = m.si, SJ = m.sj;
+        const int k = C.k, j = C.j, SI = m.si, SJ = m.sj, SI4 = m.si4, SJ4 = m.sj4;
         const Ld<double> Pe(psi + SI), Pn(psi + SJ), Ps(psi - SJ), Pc(psi);
-        const Ld<float> Uc(u), Vc(v), Vs(v - SJ), Wc(w);
-        const St<float> O(out);
+        const Ld<float> Uc(u), Vc(v), Vs(v - SJ4), Wc(w);
+        const St<float> O(out), Q(sto);
         const float dt = A.dt;
         const float cy = dt * T.idyn[j];
         const float czr = ((dt * T.ra[k]) * T.idzn[k]) * T.irn[k];
@@ -273,11 +376,12 @@ __global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T,
         const int nm = A.neumann;
         const bool jedge = j == 1 || j == m.jp;
         int c = C.i0 * SI + j * SJ + k;
-        unsigned o4 = (unsigned)c * 4u, o8 = (unsigned)c * 8u;
+        int c4 = C.i0 * SI4 + j * SJ4 + k + kK0;
+        unsigned o4 = (unsigned)c4 * 4u, o8 = (unsigned)c * 8u;
         float pw = (float)(psi[c - SI] - mm);
         double dc = psi[c] - mm;
         float pc = (float)dc;
-        const float uw = u[c - SI];
+        const float uw = u[c4 - SI4];
         float fxl = pos0(uw) * pw + neg0(uw) * pc;
         for (int i = C.i0; i <= C.i1; ++i) {
             const double de = Pe.at(o8) - mm;
@@ -303,13 +407,21 @@ __global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T,
             O.put(o4, res);
             if (k == 1) O.put(o4, zb, -1);
             if (k == m.kp) O.put(o4, zt, 1);
-            if (jedge || i == 1 || i == m.ip) put_images<float>(out, c, res, k, j, i, m);
-            if (lead) s[0] += dc;
+            const bool edge = jedge || i == 1 || i == m.ip;
+            if (edge) put_images<float>(out, c4, res, k, j, i, m, SJ4, SI4);
+            if (lead) {
+                s[0] += dc;
+                Q.put(o4, pc);
+                if (k == 1) Q.put(o4, pd, -1);
+                if (k == m.kp) Q.put(o4, pu, 1);
+                if (edge) put_store_images(sto, psi, mm, c4, c, k, j, i, m);
+            }
             fxl = fxr;
             pc = pe;
             dc = de;
             c += SI;
-            o4 += (unsigned)SI * 4u;
+            c4 += SI4;
+            o4 += (unsigned)SI4 * 4u;
             o8 += (unsigned)SI * 8u;
         }
     }
@@ -359,21 +471,17 @@ __device__ __forceinline__ void plane_at(Plane &q, const float *p1, const float 
     q.Wm = w[c - 1];
 }
 
-__global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T,
-                                                    const float *__restrict__ u,
-                                                    const float *__restrict__ v,
-                                                    const float *__restrict__ w) {
+__global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T) {
     Col C;
     if (!pm_col(m, A.nlead, C)) return;
     const int L = C.f;
     if (A.flag[L]) return;
     if (!C.act) return;
     const int fld = A.lfield[L];
-    const float *p1 = A.s32 + (size_t)fld * A.ntot;
-    float *us = A.s32 + (size_t)(A.nf + 5 * L) * A.ntot;
-    float *vs = us + A.ntot;
-    float *ws = vs + A.ntot;
-    const int k = C.k, j = C.j, SI = m.si, SJ = m.sj;
+    const float *p1 = psi1_of(A, fld);
+    const float *u = arr(A, 0), *v = arr(A, 1), *w = arr(A, 2);
+    float *us = ustar_of(A, L, 0), *vs = ustar_of(A, L, 1), *ws = ustar_of(A, L, 2);
+    const int k = C.k, j = C.j, SI = m.si4, SJ = m.sj4;
     // plane i+1 (the march's next plane) and plane i's own values
     const Ld<float> nP(p1 + SI), nPn(p1 + SI + SJ), nPs(p1 + SI - SJ), nU(u + SI),
         nUj(u + SI + SJ), nV(v + SI), nVm(v + SI - SJ), nW(w + SI);
@@ -392,7 +500,7 @@ __global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T,
     const float qg1 = q * ig1, qg3 = q * ig3;
     const float wfac = dt * (rak * ig3);
     const bool jedge = j == 1 || j == m.jp;
-    int c = C.i0 * SI + j * SJ + k;
+    int c = C.i0 * SI + j * SJ + k + kK0;
     unsigned o4 = (unsigned)c * 4u;
     Plane pw, pc, pn;
     plane_at(pw, p1, u, v, w, c - SI, SJ, rnk, rnk1);
@@ -471,9 +579,9 @@ __global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T,
             SW.put(o4, 0.f, 1);
         }
         if (jedge || i == 1 || i == m.ip) {
-            put_images<float>(us, c, usv, k, j, i, m);
-            put_images<float>(vs, c, vsv, k, j, i, m);
-            put_images<float>(ws, c, wsv, k, j, i, m);
+            put_images<float>(us, c, usv, k, j, i, m, SJ, SI);
+            put_images<float>(vs, c, vsv, k, j, i, m, SJ, SI);
+            put_images<float>(ws, c, wsv, k, j, i, m, SJ, SI);
         }
         pw = pc;
         pc = pn;
@@ -483,10 +591,286 @@ __global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T,
 }
 
 // ---------------------------------------------------------------------------
-// K4: limiter factors (eqs 19-20 of Smolarkiewicz & Grabowski 1990,
-// :549-616) of each category's first field: bounds over psi_store = psi - m
-// (recomputed from the untouched FP64 field) and psi1 at the cell and its six
+// Limiter factors (eqs 19-20 of Smolarkiewicz & Grabowski 1990, :549-616) of
+// one cell: bounds over psi_store = psi - m and psi1 at the cell and its six
 // neighbours, density-weighted in/outflow denominators.
+// ---------------------------------------------------------------------------
+struct BetaIn {
+    float qc, qw, qe, qn, qs, qu, qd;  // psi_store
+    float oc, ow, oe, on, os, ou, od;  // psi1
+    float uc, uw, vc, vl, wc, wl;      // U*, V*, W* at the six faces
+};
+__device__ __forceinline__ void beta_cell(const BetaIn &x, float rnk, float cx, float cy,
+                                          float czp, float czm, float dt, float &bu,
+                                          float &bd) {
+    const float small = FLT_MIN;
+    float mx = fmaxf(fmaxf(x.qc, x.qw), fmaxf(x.qe, x.qn));
+    mx = fmaxf(fmaxf(mx, x.qs), fmaxf(x.qu, x.qd));
+    mx = fmaxf(fmaxf(mx, x.oc), fmaxf(x.ow, x.oe));
+    mx = fmaxf(fmaxf(mx, x.on), fmaxf(x.os, fmaxf(x.ou, x.od)));
+    float mn = fminf(fminf(x.qc, x.qw), fminf(x.qe, x.qn));
+    mn = fminf(fminf(mn, x.qs), fminf(x.qu, x.qd));
+    mn = fminf(fminf(mn, x.oc), fminf(x.ow, x.oe));
+    mn = fminf(fminf(mn, x.on), fminf(x.os, fminf(x.ou, x.od)));
+    const float d1 = dt * ((cx * (pos0(x.uw) * x.ow - neg0(x.uc) * x.oe) +
+                            cy * (pos0(x.vl) * x.os - neg0(x.vc) * x.on)) +
+                           (czm * pos0(x.wl) * x.od - czp * neg0(x.wc) * x.ou)) +
+                     small;
+    const float d2 = dt * (x.oc * ((cx * (pos0(x.uc) - neg0(x.uw)) + cy * (pos0(x.vc) - neg0(x.vl))) +
+                                   (czp * pos0(x.wc) - czm * neg0(x.wl))) +
+                           small);
+    bu = (rnk * (mx - x.oc)) * rcp(d1);
+    bd = (rnk * (x.oc - mn)) * rcp(d2);
+}
+
+// The inputs of beta from the aligned FP32 arrays at byte offset o (all
+// bases uniform; k neighbours are immediates).  With `next`, the west
+// values are the previous plane's centre values, passed in.
+struct BetaSrc {
+    Ld<float> q, qe, qw, qn, qs, o, oe, ow, on, os, u, uw, v, vs, w;
+    __device__ BetaSrc(const float *sq, const float *p1, const float *us, const float *vs_,
+                       const float *ws, int SI4, int SJ4)
+        : q(sq), qe(sq + SI4), qw(sq - SI4), qn(sq + SJ4), qs(sq - SJ4), o(p1), oe(p1 + SI4),
+          ow(p1 - SI4), on(p1 + SJ4), os(p1 - SJ4), u(us), uw(us - SI4), v(vs_), vs(vs_ - SJ4),
+          w(ws) {}
+};
+__device__ __forceinline__ void beta_load(BetaIn &x, const BetaSrc &S, unsigned o) {
+    x.qc = S.q.at(o);
+    x.qe = S.qe.at(o);
+    x.qn = S.qn.at(o);
+    x.qs = S.qs.at(o);
+    x.qu = S.q.at(o, 1);
+    x.qd = S.q.at(o, -1);
+    x.oc = S.o.at(o);
+    x.oe = S.oe.at(o);
+    x.on = S.on.at(o);
+    x.os = S.os.at(o);
+    x.ou = S.o.at(o, 1);
+    x.od = S.o.at(o, -1);
+    x.uc = S.u.at(o);
+    x.vc = S.v.at(o);
+    x.vl = S.vs.at(o);
+    x.wc = S.w.at(o);
+    x.wl = S.w.at(o, -1);
+}
+__device__ __forceinline__ void beta_load_full(BetaIn &x, const BetaSrc &S, unsigned o) {
+    beta_load(x, S, o);
+    x.qw = S.qw.at(o);
+    x.ow = S.ow.at(o);
+    x.uw = S.uw.at(o);
+}
+
+// eq 18: the limited velocity at a face with antidiffusive velocity x, the
+// factors of the cell below the face (_c) and above it (_o)
+__device__ __forceinline__ float limited(float x, float bd_c, float bu_c, float bu_o,
+                                         float bd_o) {
+    return min3f(1.f, bd_c, bu_o) * pos0(x) + min3f(1.f, bu_c, bd_o) * neg0(x);
+}
+
+// ---------------------------------------------------------------------------
+// K4: beta of the category's first field and the last donor pass of field f
+// (:634-697), + m; psi with the halo of exchange_full(psi, lbc, ubc) after
+// the unshift, i.e. z planes (bc - m) + m.
+//
+// A block owns whole columns of kp <= 256 levels (thread t: level
+// t mod KT + 1, row t / KT of R rows) and marches a chunk of i planes.  In the
+// step for plane i it computes beta of plane i+1 -- its own rows, and the row
+// below and above the tile (the first and last rows' threads compute those
+// too; a lateral halo row takes the periodic copy's values, as
+// exchange_along_dim leaves them) -- into an LDS ring of four planes, waits
+// for the block, then applies the limited donor pass to plane i, reading the
+// factors of planes i-1, i, i+1 from the ring.  The inputs of beta for plane
+// i+1 stay in registers for the donor pass of the next step.  Four planes in
+// the ring make one barrier per step enough: the plane written in a step was
+// last read two steps earlier.  The kernel reads psi_store, never psi, which
+// it writes.
+// ---------------------------------------------------------------------------
+struct K4 {
+    int KT, R, njb, ntiles;  // levels per column slot, rows, tiles
+};
+static K4 k4_geo(const PM &m, int rows) {
+    K4 q;
+    q.KT = ((m.kp + 31) / 32) * 32;
+    q.R = std::max(1, std::min(rows, kK4Threads / q.KT));
+    q.njb = (m.jp + q.R - 1) / q.R;
+    q.ntiles = q.njb * m.nic;
+    return q;
+}
+
+__global__ __launch_bounds__(kK4Threads) void k32_betalim(Mp32Args A, PM m, K4 G, Tab32 T) {
+    // LDS ring: [plane & 3][bu, bd][row slot 0..R+1][level 0..KT-1]
+    extern __shared__ float ring[];
+    int t;
+    if (!xcd_item(A.nf * G.ntiles, t)) return;
+    const int f = t / G.ntiles;
+    const int tile = t - f * G.ntiles;
+    const int L = A.lslot[f];
+    if (A.flag[L]) return;
+    const int lf = A.lfield[L];
+    const int jb = tile % G.njb, icb = tile / G.njb;
+    const int r = (int)threadIdx.x / G.KT;
+    const int k = (int)threadIdx.x % G.KT + 1;
+    const int j0 = jb * G.R + 1;
+    const int j = j0 + r;
+    const int jlast = min(j0 + G.R - 1, m.jp);
+    const int i0 = icb * m.ic + 1;
+    const int i1 = min(i0 + m.ic - 1, m.ip);
+    const bool act = r < G.R && k <= m.kp && j <= jlast;
+    const int KT = G.KT, RS = G.R + 2, PL = 2 * RS * KT;
+    auto slot = [&](int p, int a, int rs, int kk) -> float & {
+        return ring[(p & 3) * PL + a * RS * KT + rs * KT + kk];
+    };
+    const float *sto = store_of(A, L);
+    const float *lp1 = psi1_of(A, lf);
+    const float *us = ustar_of(A, L, 0), *vs = ustar_of(A, L, 1), *ws = ustar_of(A, L, 2);
+    const float *p1 = psi1_of(A, f);
+    double *psi = A.psi[f];
+    const double mm = A.m[f];
+    const float dt = A.dt;
+    const int SI = m.si, SJ = m.sj, SI4 = m.si4, SJ4 = m.sj4;
+    // per-thread constants (inactive threads read valid table entries)
+    const int kk = act ? k : 1;
+    const int jj = act ? j : 1;
+    const float rnk = T.rn[kk];
+    const float czp = T.ra[kk] * T.idzn[kk], czm = T.ra[kk - 1] * T.idzn[kk];
+    const float cyd = dt * T.idyn[jj];
+    const float czr = ((dt * T.ra[kk]) * T.idzn[kk]) * T.irn[kk];
+    const float czl = ((dt * T.ra[kk - 1]) * T.idzn[kk]) * T.irn[kk];
+    const double zb = (A.lbc[f] - mm) + mm, zt = (A.ubc[f] - mm) + mm;
+    const int nm = A.neumann;
+    // the halo rows this thread also computes: the row below the tile (first
+    // row) and above it (last row), as periodic copies
+    const bool below = act && j == j0;
+    const bool above = act && j == jlast;
+    const int jbelow = wrapi(j0 - 1, m.jp), jabove = wrapi(jlast + 1, m.jp);
+    const int rs_above = jlast - j0 + 2;
+    const BetaSrc S(sto, lp1, us, vs, ws, SI4, SJ4);
+    // beta at plane p, row jr, every input loaded
+    auto beta_at = [&](int p, int jr, BetaIn &x, float &bu, float &bd) {
+        const int pp = wrapi(p, m.ip);
+        const unsigned o = (unsigned)(pp * SI4 + jr * SJ4 + k + kK0) * 4u;
+        beta_load_full(x, S, o);
+        beta_cell(x, rnk, rnk * T.xpk[pp].y, rnk * T.idyn[jr], czp, czm, dt, bu, bd);
+    };
+    auto halo_rows = [&](int p) {
+        BetaIn x;
+        float bu, bd;
+        if (below) {
+            beta_at(p, jbelow, x, bu, bd);
+            slot(p, 0, 0, k - 1) = bu;
+            slot(p, 1, 0, k - 1) = bd;
+        }
+        if (above) {
+            beta_at(p, jabove, x, bu, bd);
+            slot(p, 0, rs_above, k - 1) = bu;
+            slot(p, 1, rs_above, k - 1) = bd;
+        }
+    };
+    BetaIn cur, nxt;
+    // prologue: beta of planes i0-1 (the centre of plane i0-1 is all the
+    // donor pass reads of it) and i0
+    if (act) {
+        float bu, bd;
+        beta_at(i0 - 1, j, cur, bu, bd);
+        slot(i0 - 1, 0, r + 1, k - 1) = bu;
+        slot(i0 - 1, 1, r + 1, k - 1) = bd;
+        beta_at(i0, j, cur, bu, bd);
+        slot(i0, 0, r + 1, k - 1) = bu;
+        slot(i0, 1, r + 1, k - 1) = bd;
+        halo_rows(i0);
+    }
+    const bool jedge = j == 1 || j == m.jp;
+    int c8 = i0 * SI + j * SJ + k;
+    int c4 = i0 * SI4 + j * SJ4 + k + kK0;
+    unsigned o4n = (unsigned)(c4 + SI4) * 4u;  // plane i+1 of the own column
+    float fxl = 0.f;
+    for (int i = i0; i <= i1; ++i) {
+        // beta of plane i+1
+        if (act) {
+            float bu, bd;
+            if (i + 1 <= m.ip) {
+                // the plane before i+1 is cur's: its centre values are the
+                // west values of plane i+1
+                beta_load(nxt, S, o4n);
+                nxt.qw = cur.qc;
+                nxt.ow = cur.oc;
+                nxt.uw = cur.uc;
+                beta_cell(nxt, rnk, rnk * T.xpk[i + 1].y, rnk * T.idyn[j], czp, czm, dt, bu, bd);
+            } else {
+                beta_at(i + 1, j, nxt, bu, bd);
+            }
+            slot(i + 1, 0, r + 1, k - 1) = bu;
+            slot(i + 1, 1, r + 1, k - 1) = bd;
+            if (i < i1) halo_rows(i + 1);
+        }
+        __syncthreads();
+        if (act) {
+            // the donor pass of field f at plane i with the limited velocities
+            const float buc = slot(i, 0, r + 1, k - 1), bdc = slot(i, 1, r + 1, k - 1);
+            const float bue = slot(i + 1, 0, r + 1, k - 1), bde = slot(i + 1, 1, r + 1, k - 1);
+            const float bun = slot(i, 0, r + 2, k - 1), bdn = slot(i, 1, r + 2, k - 1);
+            const float bus = slot(i, 0, r, k - 1), bds = slot(i, 1, r, k - 1);
+            // the factors vanish on the z planes (lbc = ubc = 0)
+            const float buu = k < m.kp ? slot(i, 0, r + 1, k) : 0.f;
+            const float bdu = k < m.kp ? slot(i, 1, r + 1, k) : 0.f;
+            const float uc = limited(cur.uc, bdc, buc, bue, bde);
+            const float vc = limited(cur.vc, bdc, buc, bun, bdn);
+            const float vl = limited(cur.vl, bds, bus, buc, bdc);
+            const float wc = limited(cur.wc, bdc, buc, buu, bdu);
+            float wl = 0.f;  // the face below k = 1 is the lbc = 0 plane
+            if (k > 1)
+                wl = limited(cur.wl, slot(i, 1, r + 1, k - 2), slot(i, 0, r + 1, k - 2), buc,
+                             bdc);
+            // psi1 of field f (for the category's first field: the values
+            // beta read)
+            float pc = cur.oc, pw = cur.ow, pe = cur.oe, pn = cur.on, ps = cur.os, pu = cur.ou,
+                  pd = cur.od;
+            if (f != lf) {
+                pc = p1[c4];
+                pw = p1[c4 - SI4];
+                pe = p1[c4 + SI4];
+                pn = p1[c4 + SJ4];
+                ps = p1[c4 - SJ4];
+                pu = p1[c4 + 1];
+                pd = p1[c4 - 1];
+            }
+            if (i == i0) {
+                const float ul = limited(cur.uw, slot(i - 1, 1, r + 1, k - 1),
+                                         slot(i - 1, 0, r + 1, k - 1), buc, bdc);
+                fxl = pos0(ul) * pw + neg0(ul) * pc;
+            }
+            const float fxr = pos0(uc) * pc + neg0(uc) * pe;
+            const float fyr = pos0(vc) * pc + neg0(vc) * pn;
+            const float fyl = pos0(vl) * ps + neg0(vl) * pc;
+            float fzr = (pos0(wc) * pc + neg0(wc) * pu) * czr;
+            float fzl = (pos0(wl) * pd + neg0(wl) * pc) * czl;
+            if (nm == 1) {
+                if (k == 1) fzl = fzr;
+                if (k == m.kp) fzr = fzl;
+            } else if (nm == 2) {
+                if (k == 1) fzl = -fzr;
+                if (k == m.kp) fzr = -fzl;
+            }
+            const float cx = dt * T.xpk[i].y;
+            const float res = ((pc - (fxr - fxl) * cx) - (fyr - fyl) * cyd) - (fzr - fzl);
+            const double rv = (double)res + mm;
+            psi[c8] = rv;
+            if (k == 1) psi[c8 - 1] = zb;
+            if (k == m.kp) psi[c8 + 1] = zt;
+            if (jedge || i == 1 || i == m.ip) put_images<double>(psi, c8, rv, k, j, i, m, SJ, SI);
+            fxl = fxr;
+            cur = nxt;
+        }
+        c8 += SI;
+        c4 += SI4;
+        o4n += (unsigned)SI4 * 4u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Separate limiter kernels for columns of more than 256 levels: beta of each
+// category's first field into two aligned arrays, then the last donor pass.
+// Without the limiter the last donor pass runs with U*, V*, W* themselves.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k32_beta(Mp32Args A, PM m, Tab32 T) {
     Col C;
@@ -495,89 +879,38 @@ __global__ __launch_bounds__(256) void k32_beta(Mp32Args A, PM m, Tab32 T) {
     if (A.flag[L]) return;
     if (!C.act) return;
     const int fld = A.lfield[L];
-    const double *psi = A.psi[fld];
-    const double mm = A.m[fld];
-    const float *p1 = A.s32 + (size_t)fld * A.ntot;
-    float *const base = A.s32 + (size_t)(A.nf + 5 * L) * A.ntot;
-    const float *us = base;
-    const float *vs = base + A.ntot;
-    const float *ws = base + 2 * A.ntot;
-    float *bu = base + 3 * A.ntot;
-    float *bd = base + 4 * A.ntot;
-    const int k = C.k, j = C.j, SI = m.si, SJ = m.sj;
-    const Ld<double> Qe(psi + SI), Qn(psi + SJ), Qs(psi - SJ), Qc(psi);
-    const Ld<float> Oe(p1 + SI), On(p1 + SJ), Os(p1 - SJ), Oc(p1);
-    const Ld<float> Uc(us), Vc(vs), Vl(vs - SJ), Wc(ws);
-    const St<float> BU(bu), BD(bd);
-    const float dt = A.dt, small = FLT_MIN;
+    const float *sto = store_of(A, L);
+    const float *p1 = psi1_of(A, fld);
+    const float *us = ustar_of(A, L, 0), *vs = ustar_of(A, L, 1), *ws = ustar_of(A, L, 2);
+    float *bu = beta_of(A, L, 0), *bd = beta_of(A, L, 1);
+    const int k = C.k, j = C.j;
+    const BetaSrc S(sto, p1, us, vs, ws, m.si4, m.sj4);
+    const float dt = A.dt;
     const float rnk = T.rn[k];
     const float cy = rnk * T.idyn[j];
     const float czp = T.ra[k] * T.idzn[k], czm = T.ra[k - 1] * T.idzn[k];
     const bool jedge = j == 1 || j == m.jp;
-    int c = C.i0 * SI + j * SJ + k;
-    unsigned o4 = (unsigned)c * 4u, o8 = (unsigned)c * 8u;
-    float qw = (float)(psi[c - SI] - mm), qc = (float)(psi[c] - mm);
-    float ow = p1[c - SI], oc = p1[c];
-    float uw = us[c - SI];
     for (int i = C.i0; i <= C.i1; ++i) {
-        const float qe = (float)(Qe.at(o8) - mm);
-        const float qn = (float)(Qn.at(o8) - mm), qs = (float)(Qs.at(o8) - mm);
-        const float qu = (float)(Qc.at(o8, 1) - mm), qd = (float)(Qc.at(o8, -1) - mm);
-        const float oe = Oe.at(o4), on = On.at(o4), os = Os.at(o4), ou = Oc.at(o4, 1),
-                    od = Oc.at(o4, -1);
-        const float uc = Uc.at(o4), vc = Vc.at(o4), vl = Vl.at(o4), wc = Wc.at(o4),
-                    wl = Wc.at(o4, -1);
-        float mx = fmaxf(fmaxf(qc, qw), fmaxf(qe, qn));
-        mx = fmaxf(fmaxf(mx, qs), fmaxf(qu, qd));
-        mx = fmaxf(fmaxf(mx, oc), fmaxf(ow, oe));
-        mx = fmaxf(fmaxf(mx, on), fmaxf(os, fmaxf(ou, od)));
-        float mn = fminf(fminf(qc, qw), fminf(qe, qn));
-        mn = fminf(fminf(mn, qs), fminf(qu, qd));
-        mn = fminf(fminf(mn, oc), fminf(ow, oe));
-        mn = fminf(fminf(mn, on), fminf(os, fminf(ou, od)));
-        const float cx = rnk * T.xpk[i].y;
-        const float d1 = dt * ((cx * (pos0(uw) * ow - neg0(uc) * oe) +
-                                cy * (pos0(vl) * os - neg0(vc) * on)) +
-                               (czm * pos0(wl) * od - czp * neg0(wc) * ou)) +
-                         small;
-        const float d2 = dt * (oc * ((cx * (pos0(uc) - neg0(uw)) + cy * (pos0(vc) - neg0(vl))) +
-                                     (czp * pos0(wc) - czm * neg0(wl))) +
-                               small);
-        const float vu = (rnk * (mx - oc)) * rcp(d1), vd = (rnk * (oc - mn)) * rcp(d2);
-        BU.put(o4, vu);
-        BD.put(o4, vd);
+        const int c4 = i * m.si4 + j * m.sj4 + k + kK0;
+        BetaIn x;
+        beta_load_full(x, S, (unsigned)c4 * 4u);
+        float vu, vd;
+        beta_cell(x, rnk, rnk * T.xpk[i].y, cy, czp, czm, dt, vu, vd);
+        bu[c4] = vu;
+        bd[c4] = vd;
         if (k == 1) {
-            BU.put(o4, 0.f, -1);
-            BD.put(o4, 0.f, -1);
+            bu[c4 - 1] = 0.f;
+            bd[c4 - 1] = 0.f;
         }
         if (k == m.kp) {
-            BU.put(o4, 0.f, 1);
-            BD.put(o4, 0.f, 1);
+            bu[c4 + 1] = 0.f;
+            bd[c4 + 1] = 0.f;
         }
         if (jedge || i == 1 || i == m.ip) {
-            put_faces(bu, c, vu, k, j, i, m);
-            put_faces(bd, c, vd, k, j, i, m);
+            put_faces(bu, c4, vu, k, j, i, m);
+            put_faces(bd, c4, vd, k, j, i, m);
         }
-        qw = qc;
-        qc = qe;
-        ow = oc;
-        oc = oe;
-        uw = uc;
-        c += SI;
-        o4 += (unsigned)SI * 4u;
-        o8 += (unsigned)SI * 8u;
     }
-}
-
-// ---------------------------------------------------------------------------
-// K5: donor pass of each field's psi1 with the limited velocities of eq 18
-// (:634-697) evaluated at the six faces (or with U*, V*, W* themselves when
-// the limiter is off), + m; psi with the halo of exchange_full(psi, lbc, ubc)
-// after the unshift, i.e. z planes (bc - m) + m.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float limited(float x, float bd_c, float bu_c, float bu_o,
-                                         float bd_o) {
-    return min3f(1.f, bd_c, bu_o) * pos0(x) + min3f(1.f, bu_c, bd_o) * neg0(x);
 }
 
 template <bool LIM>
@@ -590,17 +923,14 @@ __global__ __launch_bounds__(256) void k32_limited(Mp32Args A, PM m, Tab32 T) {
     if (!C.act) return;
     double *psi = A.psi[f];
     const double mm = A.m[f];
-    const float *p1 = A.s32 + (size_t)f * A.ntot;
-    const float *us = A.s32 + (size_t)(A.nf + 5 * L) * A.ntot;
-    const float *vs = us + A.ntot;
-    const float *ws = vs + A.ntot;
-    const float *bu = ws + A.ntot;
-    const float *bd = bu + A.ntot;
-    const int k = C.k, j = C.j, SI = m.si, SJ = m.sj;
-    const Ld<float> Pe(p1 + SI), Pn(p1 + SJ), Ps(p1 - SJ), Pc(p1);
-    const Ld<float> Uc(us), Vc(vs), Vl(vs - SJ), Wc(ws);
-    const Ld<float> Bue(bu + SI), Bde(bd + SI), Bun(bu + SJ), Bdn(bd + SJ), Bus(bu - SJ),
-        Bds(bd - SJ), Buc(bu), Bdc(bd);
+    const float *p1 = psi1_of(A, f);
+    const float *us = ustar_of(A, L, 0), *vs = ustar_of(A, L, 1), *ws = ustar_of(A, L, 2);
+    const float *bu = LIM ? beta_of(A, L, 0) : us, *bd = LIM ? beta_of(A, L, 1) : us;
+    const int k = C.k, j = C.j, SI = m.si, SJ = m.sj, SI4 = m.si4, SJ4 = m.sj4;
+    const Ld<float> Pe(p1 + SI4), Pn(p1 + SJ4), Ps(p1 - SJ4), Pc(p1);
+    const Ld<float> Uc(us), Vc(vs), Vl(vs - SJ4), Wc(ws);
+    const Ld<float> Bue(bu + SI4), Bde(bd + SI4), Bun(bu + SJ4), Bdn(bd + SJ4), Bus(bu - SJ4),
+        Bds(bd - SJ4), Buc(bu), Bdc(bd);
     const St<double> S(psi);
     const float dt = A.dt;
     const float cy = dt * T.idyn[j];
@@ -610,14 +940,15 @@ __global__ __launch_bounds__(256) void k32_limited(Mp32Args A, PM m, Tab32 T) {
     const int nm = A.neumann;
     const bool jedge = j == 1 || j == m.jp;
     int c = C.i0 * SI + j * SJ + k;
-    unsigned o4 = (unsigned)c * 4u, o8 = (unsigned)c * 8u;
-    float pw = p1[c - SI], pc = p1[c];
+    int c4 = C.i0 * SI4 + j * SJ4 + k + kK0;
+    unsigned o4 = (unsigned)c4 * 4u, o8 = (unsigned)c * 8u;
+    float pw = p1[c4 - SI4], pc = p1[c4];
     float buc = 0.f, bdc = 0.f;
-    float ul = us[c - SI];
+    float ul = us[c4 - SI4];
     if (LIM) {
-        buc = bu[c];
-        bdc = bd[c];
-        ul = limited(ul, bd[c - SI], bu[c - SI], buc, bdc);
+        buc = bu[c4];
+        bdc = bd[c4];
+        ul = limited(ul, bd[c4 - SI4], bu[c4 - SI4], buc, bdc);
     }
     float fxl = pos0(ul) * pw + neg0(ul) * pc;
     for (int i = C.i0; i <= C.i1; ++i) {
@@ -653,18 +984,18 @@ __global__ __launch_bounds__(256) void k32_limited(Mp32Args A, PM m, Tab32 T) {
         }
         const float cx = dt * T.xpk[i].y;
         const float res = ((pc - (fxr - fxl) * cx) - (fyr - fyl) * cy) - (fzr - fzl);
-        const double r = (double)res + mm;
-        S.put(o8, r);
+        const double rv = (double)res + mm;
+        S.put(o8, rv);
         if (k == 1) S.put(o8, zb, -1);
         if (k == m.kp) S.put(o8, zt, 1);
-        if (jedge || i == 1 || i == m.ip) put_images<double>(psi, c, r, k, j, i, m);
+        if (jedge || i == 1 || i == m.ip) put_images<double>(psi, c, rv, k, j, i, m, SJ, SI);
         fxl = fxr;
         pw = pc;
         pc = pe;
         buc = bue;
         bdc = bde;
         c += SI;
-        o4 += (unsigned)SI * 4u;
+        o4 += (unsigned)SI4 * 4u;
         o8 += (unsigned)SI * 8u;
     }
 }
@@ -683,8 +1014,8 @@ __global__ void k32_exit_unshift(Mp32Args A, long long n) {
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
-static int ensure_mp32(dcm_ctx *c, size_t count) {
-    if (c->mp32_count >= count) return DCM_OK;
+static int ensure_mp32(dcm_ctx *c, size_t count, size_t ntot4) {
+    if (c->mp32_count >= count * ntot4) return DCM_OK;
     DCM_CHECK(no_alloc_in_capture(c, "mpdata32 scratch"));
     if (c->mp32) {
         DCM_HIP(hipStreamSynchronize(c->stream));
@@ -693,22 +1024,26 @@ static int ensure_mp32(dcm_ctx *c, size_t count) {
         c->mp32_count = 0;
     }
     void *p = nullptr;
-    const size_t bytes = count * (size_t)c->geo.ntot * sizeof(float);
+    const size_t bytes = count * ntot4 * sizeof(float);
     if (hipMalloc(&p, bytes) != hipSuccess) {
         set_error("hipMalloc of the FP32 MPDATA scratch failed");
         return DCM_OUT_OF_MEMORY;
     }
     // zero once: halo cells no pass writes (the lateral edges of the limiter
-    // factors) then hold a defined value
+    // factors, the padding between aligned columns) then hold a defined value
     DCM_HIP(hipMemsetAsync(p, 0, bytes, c->stream));
     c->mp32 = (float *)p;
-    c->mp32_count = count;
+    c->mp32_count = count * ntot4;
     return DCM_OK;
 }
 
+// rows of a K4 tile (the last kernel's block holds whole columns)
+constexpr int kK4Rows = 8;
+
 // one batched launch sequence over at most kMp32Max fields
-static int mpdata32_run(dcm_ctx *c, float dt, const float *u, const float *v, const float *w,
-                        const Mp32Cat *cats, int ncat, int monotone, int neumann) {
+static int mpdata32_run(dcm_ctx *c, float dt, const double *u, const double *v,
+                        const double *w, const Mp32Cat *cats, int ncat, int monotone,
+                        int neumann) {
     const Geo &g = c->geo;
     Mp32Args A{};
     int nf = 0, nlead = 0;
@@ -725,11 +1060,14 @@ static int mpdata32_run(dcm_ctx *c, float dt, const float *u, const float *v, co
         ++nlead;
     }
     const PM m = pm_geo(g, kMp32Planes);
-    DCM_CHECK(ensure_mp32(c, (size_t)nf + 5 * (size_t)nlead));
+    const K4 G = k4_geo(m, kK4Rows);
+    const bool fused = monotone && m.kp <= kK4MaxKt;
+    const size_t ntot4 = (size_t)ntot4_of(m);
+    DCM_CHECK(ensure_mp32(c, 3 + (size_t)nf + 6 * (size_t)nlead, ntot4));
     const size_t npart = std::max((size_t)kMinBlocks * nf, (size_t)m.ntiles * nlead);
     DCM_CHECK(ensure_partials(c, npart + 16));
     A.s32 = c->mp32;
-    A.ntot = g.ntot;
+    A.ntot4 = (long long)ntot4;
     A.m = c->dscal + 16;
     A.flag = (int *)(c->dscal + 48);
     A.partials = c->partials;
@@ -738,25 +1076,39 @@ static int mpdata32_run(dcm_ctx *c, float dt, const float *u, const float *v, co
     A.dt = dt;
     A.neumann = neumann;
     const Tab32 &T = c->tab32;
+    float *s = c->mp32;
+    hipLaunchKernelGGL(k32_shadow, dim3((g.nk + 63) / 64, (g.nj + 3) / 4, g.ni), dim3(256), 0,
+                       c->stream, u, v, w, s, s + ntot4, s + 2 * ntot4, m, g.nk);
     int tok = prof_begin(c, "mpdata32_min");
     hipLaunchKernelGGL(k32_min, dim3(kMinBlocks, nf), dim3(kBlock), 0, c->stream, A, g);
     hipLaunchKernelGGL(k32_min_finish, dim3(nf), dim3(1024), 0, c->stream, A, kMinBlocks);
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
-    launch_timed(c, "k32_donor1", k32_donor1, dim3(pm_grid(m, nf)), dim3(kBlock), A, m, T, u, v,
-                 w);
+    launch_timed(c, "k32_donor1", k32_donor1, dim3(xcd_grid((long long)nf * m.ntiles)),
+                 dim3(kBlock), A, m, T);
     hipLaunchKernelGGL(k32_exit_flags, dim3(nlead), dim3(1024), 0, c->stream, A, m.ntiles);
     DCM_LAUNCH_CHECK();
-    launch_timed(c, "k32_antidiff", k32_antidiff, dim3(pm_grid(m, nlead)), dim3(kBlock), A, m, T,
-                 u, v, w);
+    launch_timed(c, "k32_antidiff", k32_antidiff, dim3(xcd_grid((long long)nlead * m.ntiles)),
+                 dim3(kBlock), A, m, T);
     DCM_LAUNCH_CHECK();
-    if (monotone) {
-        launch_timed(c, "k32_beta", k32_beta, dim3(pm_grid(m, nlead)), dim3(kBlock), A, m, T);
-        launch_timed(c, "k32_limited", k32_limited<true>, dim3(pm_grid(m, nf)), dim3(kBlock), A,
-                     m, T);
+    if (fused) {
+        const unsigned threads = (unsigned)(((G.KT * G.R + 63) / 64) * 64);
+        const size_t lds = 4 * 2 * (size_t)(G.R + 2) * G.KT * sizeof(float);
+        const dim3 grid(xcd_grid((long long)nf * G.ntiles));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof_events(c, "k32_betalim", &e0, &e1))
+            hipExtLaunchKernelGGL(k32_betalim, grid, dim3(threads), lds, c->stream, e0, e1, 0, A,
+                                  m, G, T);
+        else
+            hipLaunchKernelGGL(k32_betalim, grid, dim3(threads), lds, c->stream, A, m, G, T);
+    } else if (monotone) {
+        launch_timed(c, "k32_beta", k32_beta, dim3(xcd_grid((long long)nlead * m.ntiles)),
+                     dim3(kBlock), A, m, T);
+        launch_timed(c, "k32_limited", k32_limited<true>,
+                     dim3(xcd_grid((long long)nf * m.ntiles)), dim3(kBlock), A, m, T);
     } else {
-        launch_timed(c, "k32_limited", k32_limited<false>, dim3(pm_grid(m, nf)), dim3(kBlock), A,
-                     m, T);
+        launch_timed(c, "k32_limited", k32_limited<false>,
+                     dim3(xcd_grid((long long)nf * m.ntiles)), dim3(kBlock), A, m, T);
     }
     DCM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k32_exit_unshift, dim3(256, nf), dim3(kBlock), 0, c->stream, A, g.ntot);
@@ -764,15 +1116,26 @@ static int mpdata32_run(dcm_ctx *c, float dt, const float *u, const float *v, co
     return DCM_OK;
 }
 
-int mpdata32_batch(dcm_ctx *c, float dt, const float *u, const float *v, const float *w,
-                   const Mp32Cat *cats, int ncat, int kord, int monotone, int neumann) {
+int mpdata32_batch(dcm_ctx *c, float dt, const double *u, const double *v, const double *w,
+                   float *shadow, const Mp32Cat *cats, int ncat, int kord, int monotone,
+                   int neumann) {
     const Geo &g = c->geo;
-    const bool fast = kord == 2 && c->mpdata_variant == 0 && g.ntot < (1LL << 31);
+    const PM m = pm_geo(g, kMp32Planes);
+    // byte offsets of the aligned FP32 arrays and of the FP64 fields fit 32 bits
+    const bool fits = (long long)ntot4_of(m) * 4 < (1LL << 32) && g.ntot * 8 < (1LL << 32);
+    const bool fast = kord == 2 && c->mpdata_variant == 0 && fits;
+    float *uf = shadow, *vf = shadow + g.ntot, *wf = shadow + 2 * g.ntot;
+    bool shadowed = false;
+    auto generic = [&](const Mp32Cat &q) -> int {
+        if (!shadowed) {
+            DCM_CHECK(velocity_shadow(c, u, v, w, uf, vf, wf));
+            shadowed = true;
+        }
+        return mpdata_core<float>(c, dt, uf, vf, wf, q.psi, q.nf, q.lbc, q.ubc, kord, monotone,
+                                  neumann, q.whole_min ? 0 : 1);
+    };
     if (!fast) {
-        for (int q = 0; q < ncat; ++q)
-            DCM_CHECK(mpdata_core<float>(c, dt, u, v, w, cats[q].psi, cats[q].nf, cats[q].lbc,
-                                         cats[q].ubc, kord, monotone, neumann,
-                                         cats[q].whole_min ? 0 : 1));
+        for (int q = 0; q < ncat; ++q) DCM_CHECK(generic(cats[q]));
         return DCM_OK;
     }
     // batches of whole categories, at most kMp32Max fields each; the shift
@@ -780,9 +1143,7 @@ int mpdata32_batch(dcm_ctx *c, float dt, const float *u, const float *v, const f
     int q = 0;
     while (q < ncat) {
         if (cats[q].nf > kMp32Max) {
-            DCM_CHECK(mpdata_core<float>(c, dt, u, v, w, cats[q].psi, cats[q].nf, cats[q].lbc,
-                                         cats[q].ubc, kord, monotone, neumann,
-                                         cats[q].whole_min ? 0 : 1));
+            DCM_CHECK(generic(cats[q]));
             ++q;
             continue;
         }

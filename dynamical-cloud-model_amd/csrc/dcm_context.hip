@@ -481,6 +481,7 @@ int dcm_context_destroy(dcm_ctx *c) {
         (void)hipEventDestroy(p.second.second);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->iter_graph) (void)hipGraphExecDestroy(c->iter_graph);
     c->solve_tables.release();
     c->flags.release();
     if (c->h_flags) (void)hipHostFree(c->h_flags);

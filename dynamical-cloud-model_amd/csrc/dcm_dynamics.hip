@@ -347,6 +347,17 @@ __global__ void k_bc_roundtrip(const double *m, double lbc, double ubc, int *bad
     if ((lbc - mm) + mm != lbc || (ubc - mm) + mm != ubc) *bad = 1;
 }
 
+namespace dcm {
+// a captured phase of the step (the kernels before or after the pressure
+// solve), valid while the key it was captured for -- the time-level phase
+// n mod 3, precision, summation order, MPDATA variant and the model's work
+// generation of every model it covers -- is unchanged
+struct PhaseGraph {
+    std::vector<int> key;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace dcm
+
 // ---------------------------------------------------------------------------
 // model state
 // ---------------------------------------------------------------------------
@@ -387,6 +398,12 @@ struct dcm_model {
     // bumped whenever the work of a step changes (physics options, div), so
     // that an ensemble does not replay graphs captured before the change
     int gen = 0;
+    // dcm_model_step: the phases before and after the solve replayed from
+    // HIP graphs, each captured once its key has run eagerly (so every
+    // scratch buffer exists: nothing is allocated or cleared in a capture)
+    bool use_graphs = true;
+    std::vector<std::vector<int>> eager[2];
+    dcm::PhaseGraph pre[3], post[3];
 
     double *theta() { return oned; }
     double *thetan() { return oned + (c->geo.kp + 2); }
@@ -702,6 +719,9 @@ int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_
 int dcm_model_destroy(dcm_model *m) {
     if (!m) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(m->c->stream);
+    for (int r = 0; r < 3; ++r)
+        for (dcm::PhaseGraph *g : {&m->pre[r], &m->post[r]})
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
     for (void *p : m->allocs) (void)hipFree(p);
     if (m->shadow) (void)hipFree(m->shadow);
     delete m;
@@ -1082,9 +1102,85 @@ static int model_check_bcs(dcm_model *m) {
     return DCM_OK;
 }
 
+extern "C++" {
+namespace dcm {
+// Run one phase of the step on `stream`: replay its graph when one was
+// captured for `key`, else capture it (the host code of the phase runs
+// once, at capture) and launch it.  `host_replay` repeats the phase's
+// host-side bookkeeping on a replay.
+template <typename Capture, typename Replay>
+static int run_phase(hipStream_t stream, PhaseGraph *slots, const std::vector<int> &key,
+                     Capture &&capture, Replay &&host_replay) {
+    for (int r = 0; r < 3; ++r)
+        if (slots[r].exec && slots[r].key == key) {
+            host_replay();
+            DCM_HIP(hipGraphLaunch(slots[r].exec, stream));
+            return DCM_OK;
+        }
+    PhaseGraph *slot = &slots[0];
+    for (int r = 0; r < 3; ++r)
+        if (!slots[r].exec) {
+            slot = &slots[r];
+            break;
+        }
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    slot->exec = nullptr;
+    slot->key.clear();
+    DCM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    const int st = capture();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream, &graph);
+    if (st != DCM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (ec != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        return DCM_HIP_ERROR;
+    }
+    const hipError_t ei = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        slot->exec = nullptr;
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        return DCM_HIP_ERROR;
+    }
+    slot->key = key;
+    DCM_HIP(hipGraphLaunch(slot->exec, stream));
+    return DCM_OK;
+}
+
+}  // namespace dcm
+
+// what the kernels of a captured phase and their arguments depend on
+static std::vector<int> phase_key(const dcm_model *m) {
+    return {m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
+            12 * m->c->mpdata_variant + 24 * m->gen};
+}
+
+// Run phase 0 (before the solve) or 1 (after it) of a model step: eagerly
+// the first time its key is seen, from a graph afterwards (not while kernels
+// are being timed one by one).
+template <typename Body, typename Replay>
+static int model_phase(dcm_model *m, int phase, Body &&body, Replay &&host_replay) {
+    const std::vector<int> key = phase_key(m);
+    std::vector<std::vector<int>> &seen = m->eager[phase];
+    const bool known = std::find(seen.begin(), seen.end(), key) != seen.end();
+    if (m->use_graphs && !m->c->profiling && known)
+        return run_phase(m->c->stream, phase ? m->post : m->pre, key, body, host_replay);
+    DCM_CHECK(body());
+    if (!known) {
+        if (seen.size() >= 48) seen.erase(seen.begin());
+        seen.push_back(key);
+    }
+    return DCM_OK;
+}
+
+}  // extern "C++"
+
 static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
-    DCM_CHECK(model_pre_solve(m));
+    DCM_CHECK(model_phase(m, 0, [&] { return model_pre_solve(m); }, [] {}));
     SolverOut so;
     int st;
     if (m->precision == DCM_MIXED)
@@ -1100,7 +1196,13 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     else if (m->check_residual)
         DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
     add_stats(stats, so, true_resid);
-    return model_post_solve(m, stats);
+    // model_post_solve's host part on a replay: the level rotation and the
+    // step count
+    return model_phase(m, 1, [&] { return model_post_solve(m, stats); },
+                       [&] {
+                           m->n += 1;
+                           if (stats) stats->steps += 1;
+                       });
 }
 
 int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {
@@ -1133,6 +1235,13 @@ int dcm_model_set_inner_rtol(dcm_model *m, double rtol) {
     return DCM_OK;
 }
 
+int dcm_model_set_graphs(dcm_model *m, int on) {
+    if (!m) return DCM_BAD_ARG;
+    m->use_graphs = on != 0;
+    m->c->graphs = on != 0;
+    return DCM_OK;
+}
+
 int dcm_model_set_check_residual(dcm_model *m, int on) {
     if (!m) return DCM_BAD_ARG;
     m->check_residual = on != 0;
@@ -1153,12 +1262,6 @@ int dcm_model_set_check_residual(dcm_model *m, int on) {
 namespace dcm {
 // side streams of an ensemble (the box's hardware queues per process)
 constexpr int kEnsembleLanes = 4;
-// a captured phase of the step, valid while every member's key (its
-// time-level phase n mod 3, precision and summation order) is unchanged
-struct PhaseGraph {
-    std::vector<int> key;
-    hipGraphExec_t exec = nullptr;
-};
 }  // namespace dcm
 
 struct dcm_ensemble {
@@ -1192,52 +1295,6 @@ static void drop_graphs(dcm_ensemble *e) {
             g->key.clear();
         }
     }
-}
-
-// Run one phase of every member: replay its graph when one was captured for
-// the members' current time levels, else capture it (the host code of the
-// phase runs once, at capture) and launch it.  `host_replay` repeats the
-// phase's host-side bookkeeping on a replay.
-template <typename Capture, typename Replay>
-static int run_phase(dcm_ensemble *e, PhaseGraph *slots, const std::vector<int> &key,
-                     Capture &&capture, Replay &&host_replay) {
-    for (int r = 0; r < 3; ++r)
-        if (slots[r].exec && slots[r].key == key) {
-            host_replay();
-            DCM_HIP(hipGraphLaunch(slots[r].exec, e->stream));
-            return DCM_OK;
-        }
-    PhaseGraph *slot = &slots[0];
-    for (int r = 0; r < 3; ++r)
-        if (!slots[r].exec) {
-            slot = &slots[r];
-            break;
-        }
-    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
-    slot->exec = nullptr;
-    slot->key.clear();
-    DCM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
-    const int st = capture();
-    hipGraph_t graph = nullptr;
-    const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
-    if (st != DCM_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return st;
-    }
-    if (ec != hipSuccess) {
-        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-        return DCM_HIP_ERROR;
-    }
-    const hipError_t ei = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) {
-        slot->exec = nullptr;
-        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-        return DCM_HIP_ERROR;
-    }
-    slot->key = key;
-    DCM_HIP(hipGraphLaunch(slot->exec, e->stream));
-    return DCM_OK;
 }
 
 // Run `body` on every member with the members spread over the side
@@ -1477,7 +1534,7 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
             graphs = graphs && !m->c->profiling;
         }
         if (graphs && seen(0)) {
-            DCM_CHECK(run_phase(e, e->pre, key, pre, [] {}));
+            DCM_CHECK(run_phase(e->stream, e->pre, key, pre, [] {}));
         } else {
             DCM_CHECK(pre());
             remember(0);
@@ -1494,7 +1551,7 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
             if (stats) stats[q].steps += 1;
         }
         if (graphs && seen(1)) {
-            DCM_CHECK(run_phase(e, e->post, key, post, advance_levels));
+            DCM_CHECK(run_phase(e->stream, e->post, key, post, advance_levels));
         } else {
             DCM_CHECK(post());
             remember(1);

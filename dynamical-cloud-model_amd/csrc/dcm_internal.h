@@ -214,6 +214,13 @@ struct dcm_ctx {
     std::unordered_map<std::string, dcm::KernelTimer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
+    // the multigrid inner iteration captured as a HIP graph, valid while the
+    // launch arguments it was captured with (the key's bytes) are unchanged
+    std::vector<char> iter_key;
+    hipGraphExec_t iter_graph = nullptr;
+    // graphs on (dcm_set_graphs): the single-system mixed solve replays its
+    // inner iteration from a graph
+    bool graphs = true;
 };
 
 namespace dcm {

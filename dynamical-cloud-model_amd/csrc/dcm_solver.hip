@@ -44,6 +44,8 @@
 #include <cstddef>
 #include <cmath>
 
+#include <cstring>
+
 #include "dcm_device.h"
 
 namespace dcm {
@@ -1347,6 +1349,47 @@ struct IterLaunch {
     }
 };
 
+// The graph of one multigrid inner iteration of a single system (the
+// iteration's kernels and the gather of its flags), captured on first use
+// and whenever the launch arguments differ from the capture's: the key holds
+// the system's device table entry, the grids, the flags buffer and the
+// hierarchy.
+template <typename R>
+static int iteration_graph(dcm_ctx *c, const SolvePlan &P, const IterLaunch<R> &L,
+                           int *dflags) {
+    std::vector<char> key(sizeof(IterMember) + 4 * sizeof(void *) + 2 * sizeof(unsigned));
+    char *k = key.data();
+    std::memcpy(k, &P.mem[0], sizeof(IterMember));
+    k += sizeof(IterMember);
+    const void *ptrs[4] = {P.args.mem, dflags, c->mg, c->partials};
+    std::memcpy(k, ptrs, sizeof(ptrs));
+    k += sizeof(ptrs);
+    std::memcpy(k, &P.mgrid, sizeof(unsigned));
+    std::memcpy(k + sizeof(unsigned), &P.ugrid, sizeof(unsigned));
+    if (c->iter_graph && key == c->iter_key) return DCM_OK;
+    if (c->iter_graph) (void)hipGraphExecDestroy(c->iter_graph);
+    c->iter_graph = nullptr;
+    c->iter_key.clear();
+    DCM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    L.iteration();
+    hipLaunchKernelGGL(k_gather_flags, dim3(1), dim3(64), 0, c->stream, P.args, 1, dflags);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (ec != hipSuccess) {
+        set_error(std::string("capturing the inner iteration: ") + hipGetErrorString(ec));
+        return DCM_HIP_ERROR;
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->iter_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->iter_graph = nullptr;
+        set_error(std::string("instantiating the inner iteration: ") + hipGetErrorString(ei));
+        return DCM_HIP_ERROR;
+    }
+    c->iter_key = std::move(key);
+    return DCM_OK;
+}
+
 // BiCGSTAB iterations (dynamics.f90:390-430) of a batch of independent
 // systems in lockstep, every launch serving all of them; a member that has
 // converged or broken down skips its blocks.  Each member's initial state and
@@ -1365,16 +1408,27 @@ static int bicgstab_iterate_batch(dcm_ctx *c, const SolveSys *sys, int nsys, int
     DCM_CHECK(ensure_host_flags(c, nsys, &hflags));
     if (itmax < 1) itmax = 1;
     int launched = 0;
+    // A multigrid iteration is about sixty launches, most of them small
+    // coarse-level kernels that finish faster than the host can queue them:
+    // it is captured once as a graph and replayed (not while kernels are
+    // being timed one by one).
+    const bool graph = L.mg() && c->graphs && !c->profiling;
+    if (graph) DCM_CHECK(iteration_graph(c, P, L, dflags));
     while (launched < itmax) {
         // a V-cycle-preconditioned solve converges in one to three
         // iterations: poll after every one, so that no iteration is
         // launched after convergence
         const int every = L.mg() ? 1 : kPollEvery;
         const int chunk = (itmax - launched) < every ? (itmax - launched) : every;
-        for (int q = 0; q < chunk; ++q) L.iteration();
-        DCM_LAUNCH_CHECK();
+        if (graph) {
+            DCM_HIP(hipGraphLaunch(c->iter_graph, c->stream));
+        } else {
+            for (int q = 0; q < chunk; ++q) L.iteration();
+            DCM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_gather_flags, dim3(1), dim3(64), 0, c->stream, A, nsys,
+                               dflags);
+        }
         launched += chunk;
-        hipLaunchKernelGGL(k_gather_flags, dim3(1), dim3(64), 0, c->stream, A, nsys, dflags);
         DCM_HIP(hipMemcpyAsync(hflags, dflags, sizeof(int) * nsys, hipMemcpyDeviceToHost,
                                c->stream));
         DCM_HIP(hipStreamSynchronize(c->stream));

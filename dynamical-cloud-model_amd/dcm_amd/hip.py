@@ -161,6 +161,7 @@ _SIGS = {
     "dcm_model_profile_filter": ([_vp, C.c_char_p], _i),
     "dcm_model_set_inner_rtol": ([_vp, _d], _i),
     "dcm_model_set_check_residual": ([_vp, _i], _i),
+    "dcm_model_set_graphs": ([_vp, _i], _i),
     "dcm_stream_copy": ([_vp, _vp, _vp, C.c_size_t, _dp], _i),
     "dcm_model_kernel_time": ([_vp, C.c_char_p, _dp, _ip], _i),
     "dcm_ensemble_create": ([_i, C.POINTER(_vp), C.POINTER(_vp)], _i),
@@ -510,6 +511,11 @@ class Model:
 
     def set_check_residual(self, on=True):
         _call("dcm_model_set_check_residual", self.h, int(on))
+
+    def set_graphs(self, on=True):
+        """Replay the step's kernels from HIP graphs (default) or launch them
+        one by one (same bits)."""
+        _call("dcm_model_set_graphs", self.h, int(on))
 
     def set(self, field, a, index=0):
         a = np.ascontiguousarray(a, dtype=np.float64)

@@ -348,6 +348,13 @@ int dcm_model_set_inner_rtol(dcm_model *m, double rtol);
  * residual ||b-Ax||/max(||b||,0.01); the largest goes to max_true_resid
  * (the mixed path always measures it). */
 int dcm_model_set_check_residual(dcm_model *m, int on);
+/* HIP graphs in dcm_model_step (default on): the kernels before and after
+ * the pressure solve, and each multigrid inner iteration of the mixed solve,
+ * are captured once and replayed (per-kernel timing, dcm_model_profile,
+ * launches them one by one).  Off: every kernel is launched from the host.
+ * The results are the same bits either way.  No counterpart in the
+ * reference, which has no device launches. */
+int dcm_model_set_graphs(dcm_model *m, int on);
 
 /* ---------------------------------------------------------------------- */
 /* Ensembles (batching): independent models of uneven size stepped        */

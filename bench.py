@@ -22,9 +22,23 @@ W warm-up steps, then K timed steps from the same state:
   reported beside it (paths.mixed_chebyshev, paths.mixed_plain_jacobi).
 
 ``value`` is the mixed path's cell-steps/s (interior cells x timed steps /
-wall seconds).  The ``roofline`` object describes the dominant kernel of that
-path (the finest level's damped-Jacobi sweep), timed with HIP events on the
-model's stream in two further steps after the timed ones.
+wall seconds).  The ``roofline`` object describes the kernel with the
+largest time per step of that path, chosen among its candidates (the three
+FP32 MPDATA passes, each serving all ten fields in one launch, and the
+multigrid's finest-level sweep), each timed with HIP events on the model's
+stream in two further steps after the timed ones.  Its algorithmic bytes
+are the canonical stage bytes of the stages the kernel performs (the last
+MPDATA kernel performs two stages, the limiter and the last donor pass, so
+fusion can put its canonical figure above what it moves; ``traffic`` is what
+it moves, from the committed rocprofv3 counters).
+
+Further legs: the FP64 path with the same V-cycle (precision gain at an
+equal preconditioner), the Kelvin-Helmholtz shear layer (512x4x256, mixed
+against FP64 error), the large domain (512x512x256, HBM footprint), the
+64-member ensemble, and the options of sources, vertical MPDATA, the
+sfvt/advection_3d.F90 arithmetic, initialisation and output.  The CPU
+baseline is the oracle (a C restatement of the reference) on the host's
+cores.
 """
 from __future__ import annotations
 
@@ -43,6 +57,12 @@ METRIC = "cell-steps/s mixed vs FP64 at 1e-10 pressure resid; % HBM roofline"
 HBM_PEAK_GBPS = 8000.0   # MI355X HBM3E
 DOMINANT = "jac"         # Jacobi sweep kernel k_jac<R,false>: s' = s + (t - A s)/a_p
 JAC_WORDS = 3            # reads s and t, writes s'
+# candidates for the dominant kernel of the mixed path: (timer name, kernel,
+# canonical FP32-path bytes per cell and field of the stages it performs)
+MPDATA32 = (("k32_donor1", "k32_donor1 (first donor pass, every field)", 24),
+            ("k32_antidiff", "k32_antidiff (antidiffusive velocities, every field)", 28),
+            ("k32_betalim", "k32_betalim (limiter factors + limited donor pass, every field)",
+             64))
 # the headline mixed path preconditions its FP32 inner solve with a multigrid
 # V-cycle whose finest-level sweeps are the same k_jac<float,false,true>
 # (weight 0.8), five of them per inner iteration ("jac"; the sixth updates p)
@@ -61,12 +81,15 @@ def jac_per_iter(divisions):
     return 2 * (divisions - 3) + 1
 
 
-def canonical_bytes_fp64(F, iters):
-    """Canonical stage model, FP64 words per cell-step: 39 + 30F + 37*it."""
-    return 8.0 * (39 + 30 * F + 37 * iters)
+def canonical_bytes_fp64(F, iters, scalar_update=False):
+    """Canonical stage model, FP64 words per cell-step: 39 + 30F + 37*it
+    (the scalar update's 3 words per field only when the model runs it:
+    without the options of sources it is an exact no-op and skipped)."""
+    return 8.0 * (39 + (30 if scalar_update else 27) * F + 37 * iters)
 
 
-def canonical_bytes_mixed(F, inner_iters, outer, divisions=4, multigrid=False):
+def canonical_bytes_mixed(F, inner_iters, outer, divisions=4, multigrid=False,
+                          scalar_update=False):
     """Same stages with each word at its storage precision (bytes/cell-step).
 
     FP64: state (u,v,w,theta,q,p,b,x), sources, RHS, the true residual and
@@ -87,22 +110,25 @@ def canonical_bytes_mixed(F, inner_iters, outer, divisions=4, multigrid=False):
         b += 4 * (37 + 6 * (divisions - 4)) * inner_iters
     b += 24 + 36                         # adv_ref_state, FP32 velocity shadows
     b += F * (8 + 24 + 28 + 32 + 32)     # MPDATA per field (min, 4 passes)
-    b += F * 24                          # scalar update (canonical; an exact no-op)
+    if scalar_update:
+        b += F * 24                      # scalar update (skipped without sources options)
     b += 80 + 72                         # advance_momentum, time filter
     return float(b)
 
 
-def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebyshev"):
-    """precond (mixed path): "multigrid" (one V-cycle per application),
-    "chebyshev" (Chebyshev-relaxed Jacobi sweeps) or "plain" (the
-    reference's four Jacobi sweeps)."""
+def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebyshev",
+             candidates=()):
+    """precond: "multigrid" (one V-cycle per application), "chebyshev"
+    (Chebyshev-relaxed Jacobi sweeps) or "plain" (the reference's four
+    Jacobi sweeps).  The profiled steps time the finest-level sweep and
+    every kernel named in ``candidates``."""
     ctx = H.Context(case.dom)
     profile_only = DOMINANT
     if precond == "plain":
         ctx.set_precond_chebyshev(0.0)  # the reference's four plain Jacobi sweeps
     else:
         ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
-    if precond == "multigrid" and precision == H.MIXED:
+    if precond == "multigrid":
         ctx.set_precond_multigrid(True)
         profile_only = MG_DOMINANT
     model = H.Model(ctx, case, precision)
@@ -117,9 +143,15 @@ def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebys
     wall = t1 - t0
     # the dominant kernel's launches timed with HIP events in passes of their
     # own after the timed steps (events on every launch would slow them)
-    model.profile(True, only=profile_only)
+    names = [profile_only] + [n for n in candidates if n != profile_only]
+    model.profile(True, only=",".join(names))
     sp = model.step(PROFILE_STEPS)
-    avg_ms, launches = model.kernel_time(profile_only)
+    timed = {}
+    for nm in names:
+        a, n = model.kernel_time(nm)
+        timed[nm] = {"avg_ms": a, "launches": n,
+                     "ms_per_step": a * n / PROFILE_STEPS if a else None}
+    avg_ms, launches = timed[profile_only]["avg_ms"], timed[profile_only]["launches"]
     model.profile(False)
     cells = case.dom.cells
     snap = {"th": model.get(H.F_TH), "u": model.get(H.F_U), "w": model.get(H.F_W),
@@ -135,6 +167,7 @@ def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebys
         "max_true_resid": st.max_true_resid,
         "kernel": {"name": profile_only, "avg_ms": avg_ms, "launches": launches,
                    "profiled_steps": PROFILE_STEPS, "profiled_iters": sp.iters_total},
+        "kernels": timed,
     }
     return res, snap, (ctx, model)
 
@@ -644,21 +677,121 @@ def rel_l2(a, b, center=False):
     return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
 
 
-def cpu_baseline(W, ip, jp, steps):
-    """The CPU oracle (a single-threaded C restatement of the reference) on a
-    bounded sample: the same workload on an ip x jp x 128 sub-domain."""
+def kelvin_helmholtz(H, W, steps, warmup=2):
+    """The shear layer (512x4x256 cells at 10 m, dt=0.2 s) on both paths from
+    the same state: after ``steps`` timed steps, the mixed path's relative L2
+    error against FP64 for p (mean removed), u, w and theta, with both paths'
+    iterations, FP64 fallbacks and worst true residual.  The grid is too
+    thin in y for the V-cycle (mg_possible), so both paths use the
+    Chebyshev-relaxed sweeps."""
+    case = W.kelvin_helmholtz(steps=warmup + steps)
+    out, snaps = {}, {}
+    for name, prec in (("fp64", H.FP64), ("mixed", H.MIXED)):
+        with H.Context(case.dom) as ctx:
+            ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+            model = H.Model(ctx, case, prec)
+            try:
+                model.set_check_residual(True)
+                if warmup:
+                    model.step(warmup)
+                t0 = time.perf_counter()
+                st = model.step(steps)
+                wall = time.perf_counter() - t0
+                snaps[name] = {"p": model.get(H.F_P), "u": model.get(H.F_U),
+                               "w": model.get(H.F_W), "th": model.get(H.F_TH)}
+            finally:
+                model.close()
+        out[name] = {"value": case.dom.cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "iters_per_step": st.iters_total / steps, "iters_max": st.iters_max,
+                     "outer_per_step": st.outer_total / steps, "fallbacks": st.fallbacks,
+                     "max_true_resid": st.max_true_resid}
+    out["grid"] = f"{case.dom.ip}x{case.dom.jp}x{case.dom.kp}"
+    out["steps"] = warmup + steps
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed_error_vs_fp64"] = {k: rel_l2(snaps["fp64"][k], snaps["mixed"][k],
+                                            center=(k == "p")) for k in snaps["fp64"]}
+    return out
+
+
+def large_domain(H, W, steps, warmup=1):
+    """The 512x512x256 domain (3x3 bubbles, theta + qv + 4 tracers, 25 m,
+    dt=0.25 s) on both paths, each with the V-cycle: cell-steps/s, the
+    device memory the model holds (free memory before creation minus after
+    its first step), and the canonical stage bytes per cell-step times the
+    rate."""
+    case = W.large_domain(steps=warmup + steps)
+    F = 1 + case.nq
+    out = {"grid": f"{case.dom.ip}x{case.dom.jp}x{case.dom.kp}", "fields": F}
+    for name, prec in (("fp64", H.FP64), ("mixed", H.MIXED)):
+        free0, total = H.device_memory()
+        with H.Context(case.dom) as ctx:
+            ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+            ctx.set_precond_multigrid(True)
+            model = H.Model(ctx, case, prec)
+            try:
+                model.set_check_residual(True)
+                model.step(warmup)
+                free1, _ = H.device_memory()
+                t0 = time.perf_counter()
+                st = model.step(steps)
+                wall = time.perf_counter() - t0
+            finally:
+                model.close()
+        value = case.dom.cells * steps / wall
+        it = st.iters_total / steps
+        bpc = (canonical_bytes_mixed(F, it, st.outer_total / steps, multigrid=True)
+               if prec == H.MIXED else canonical_bytes_fp64(F, it))
+        out[name] = {"value": value, "ms_per_step": 1e3 * wall / steps, "iters_per_step": it,
+                     "outer_per_step": st.outer_total / steps, "fallbacks": st.fallbacks,
+                     "max_true_resid": st.max_true_resid,
+                     "hbm_footprint_GiB": (free0 - free1) / 2**30,
+                     "hbm_total_GiB": total / 2**30,
+                     "canonical_bytes_per_cell_step": bpc,
+                     "canonical_GBps": bpc * value / 1e9}
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(W, ip, jp, steps, case=None, threads=None):
+    """The CPU oracle (a C restatement of the reference) on the host's
+    cores: the model step of the same workload on an ip x jp x 128 domain
+    (the full grid by default), its element-wise loops over x-slabs on
+    ``threads`` OpenMP threads (at most 16, the host share of one GPU) and
+    its dot products as parallel reductions (an order that changes the last
+    bits of the iteration, not its cost)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
-    case = W.moist_bubble(ip, jp, 128, steps=steps)
-    m = O.Model(case)
-    t0 = time.perf_counter()
-    m.step(steps)
-    wall = time.perf_counter() - t0
-    return {"value": case.dom.cells * steps / wall, "unit": "cell-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle model step, moist bubble {ip}x{jp}x128 (same dx, dt, fields), "
-                      f"{steps} step(s), {wall:.1f} s; BiCGSTAB iterations {m.iters}"}
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+    O.set_sum_blocked(2)
+    if case is None or (case.dom.ip, case.dom.jp) != (ip, jp):
+        case = W.moist_bubble(ip, jp, 128, steps=steps)
+    try:
+        m = O.Model(case)
+        t0 = time.perf_counter()
+        m.step(steps)
+        wall = time.perf_counter() - t0
+    finally:
+        O.set_threads(1)
+        O.set_sum_blocked(0)
+    return {"value": case.dom.cells * steps / wall, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"oracle model step (C restatement of the reference, OpenMP over "
+                      f"x-slabs), moist bubble {ip}x{jp}x128, {steps} step(s) from the "
+                      f"initial state, {wall:.1f} s; BiCGSTAB iterations {m.iters}"}
 
 
 def stream_copy_gbps(H, dom):
@@ -672,6 +805,17 @@ def stream_copy_gbps(H, dom):
     b.free()
     ctx.close()
     return 2.0 * n * 8 / (best * 1e-3) / 1e9
+
+
+def load_busy(workload):
+    """Kernel time per step of a path from the committed rocprofv3 kernel
+    trace summary (sum of every kernel's duration over the traced steps)."""
+    path = os.path.join(ROOT, "profiles", f"busy_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {"kernel_ms_per_step": d["kernel_ms_per_step"], "source": f"profiles/busy_{workload}.json"}
 
 
 def load_traffic(workload, kernel=DOMINANT):
@@ -697,7 +841,14 @@ def main(argv=None):
                     help="skip the mixed path with Chebyshev-relaxed sweeps")
     ap.add_argument("--no-plain-sweeps", action="store_true",
                     help="skip the mixed path with the reference's plain Jacobi sweeps")
-    ap.add_argument("--cpu-sample", default="64x64:1")
+    ap.add_argument("--cpu-sample", default="256x256:2",
+                    help="IPxJP:STEPS of the CPU baseline (the oracle on the host's cores)")
+    ap.add_argument("--no-fp64-multigrid", action="store_true",
+                    help="skip the FP64 path with the V-cycle")
+    ap.add_argument("--kh-steps", type=int, default=100,
+                    help="timed steps per path of the Kelvin-Helmholtz leg (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=3,
+                    help="timed steps per path of the 512x512x256 leg (0: skip)")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--mpdata-variant-calls", type=int, default=10,
                     help="calls per MPDATA variant and routine (0: skip that leg)")
@@ -737,7 +888,9 @@ def main(argv=None):
               flush=True)
         res, snap, handles = run_path(H, case, prec, args.warmup, args.steps,
                                       inner_rtol=args.inner_rtol if pth == "mixed" else None,
-                                      precond="multigrid" if pth == "mixed" else "chebyshev")
+                                      precond="multigrid" if pth == "mixed" else "chebyshev",
+                                      candidates=[n for n, _, _ in MPDATA32]
+                                      if pth == "mixed" else ())
         out[pth] = res
         snaps[pth] = snap
         handles[1].close()
@@ -745,6 +898,19 @@ def main(argv=None):
         print(f"[bench] {pth}: {res['value']:.4g} cell-steps/s, {res['ms_per_step']:.2f} "
               f"ms/step, {res['iters_per_step']:.1f} it/step", file=sys.stderr, flush=True)
 
+    if "mixed" in out and "fp64" in out and not args.no_fp64_multigrid:
+        # the FP64 path with the same V-cycle: the gain of the precision alone
+        print(f"[bench] FP64 with the V-cycle: warmup {args.warmup}, timed {args.steps} steps",
+              file=sys.stderr, flush=True)
+        res, snap, handles = run_path(H, case, H.FP64, args.warmup, args.steps,
+                                      precond="multigrid")
+        handles[1].close()
+        handles[0].close()
+        out["fp64_multigrid"] = res
+        snaps["fp64_multigrid"] = snap
+        print(f"[bench] FP64, V-cycle: {res['value']:.4g} cell-steps/s, "
+              f"{res['ms_per_step']:.2f} ms/step, {res['iters_per_step']:.1f} it/step",
+              file=sys.stderr, flush=True)
     if "mixed" in out and not args.no_chebyshev_path:
         # the mixed path with Chebyshev-relaxed Jacobi sweeps instead of the
         # V-cycle, to separate the multigrid gain
@@ -776,33 +942,65 @@ def main(argv=None):
     cells = case.dom.cells
     word = 4 if main_path == "mixed" else 8
     k = r["kernel"]
-    achieved = active_ms = None
+    cands = {}
     if k["avg_ms"]:
         # Launches after convergence (up to a polling interval) return at
-        # once; average over the launches that swept: the iterations times
-        # the launches per iteration.  avg_launch_ms (every launch) is the
-        # figure that rocprofv3's kernel statistics report.
+        # once; the sweep's launch time is averaged over the launches that
+        # swept: the iterations times the launches per iteration.
+        # avg_launch_ms (every launch) is the figure that rocprofv3's kernel
+        # statistics report.
         total_ms = k["avg_ms"] * k["launches"]
         per_iter = MG_SWEEPS_PER_ITER if main_path == "mixed" else jac_per_iter(4)
         active = min(k["launches"], per_iter * k["profiled_iters"])
-        active_ms = total_ms / active
-        achieved = JAC_WORDS * word * cells / (active_ms * 1e-3) / 1e9
-    traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"),
-                           MG_DOMINANT if main_path == "mixed" else DOMINANT)
+        cands[DOMINANT] = {
+            "kernel": ("k_jac<float,false,true> (a damped-Jacobi sweep of the V-cycle's "
+                       "finest level)" if word == 4 else "k_jac<double,false,false>"),
+            "ms_per_step": total_ms / PROFILE_STEPS,
+            "algorithmic_bytes_per_launch": JAC_WORDS * word * cells,
+            "avg_active_launch_ms": total_ms / active,
+            "avg_launch_ms": k["avg_ms"], "launches": k["launches"]}
+    if main_path == "mixed":
+        for name, label, bpcf in MPDATA32:
+            t = r["kernels"].get(name) or {}
+            if not t.get("avg_ms"):
+                continue
+            # one launch serves every field of the step (F <= 16)
+            per_launch = bpcf * cells * F
+            cands[name] = {"kernel": label, "ms_per_step": t["ms_per_step"],
+                           "algorithmic_bytes_per_launch": per_launch,
+                           "avg_active_launch_ms": t["avg_ms"],
+                           "avg_launch_ms": t["avg_ms"], "launches": t["launches"]}
+    for c in cands.values():
+        c["achieved_GBps"] = c["algorithmic_bytes_per_launch"] / (c["avg_active_launch_ms"] *
+                                                                   1e-3) / 1e9
+        c["frac"] = c["achieved_GBps"] / HBM_PEAK_GBPS
+    dom_name = max(cands, key=lambda n: cands[n]["ms_per_step"]) if cands else DOMINANT
+    d = cands.get(dom_name, {})
+    achieved = d.get("achieved_GBps")
+    traffic = load_traffic(workload + ("_mixed" if main_path == "mixed" else "_fp64"), dom_name)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                 "traffic": traffic,
-                "kernel": ("k_jac<float,false,true> (a damped-Jacobi sweep of the V-cycle's "
-                           "finest level)" if word == 4 else "k_jac<double,false,false>"),
-                "algorithmic_bytes_per_launch": JAC_WORDS * word * cells,
-                "avg_active_launch_ms": active_ms,
-                "avg_launch_ms": k["avg_ms"], "launches": k["launches"]}
+                "kernel": d.get("kernel", dom_name),
+                "algorithmic_bytes_per_launch": d.get("algorithmic_bytes_per_launch"),
+                "avg_active_launch_ms": d.get("avg_active_launch_ms"),
+                "avg_launch_ms": d.get("avg_launch_ms"), "launches": d.get("launches"),
+                "ms_per_step": d.get("ms_per_step"),
+                "candidates": {n: {kk: c[kk] for kk in ("ms_per_step", "achieved_GBps", "frac")}
+                               for n, c in cands.items()}}
+    busy = load_busy(workload + "_mixed") if main_path == "mixed" else None
+    if busy:
+        busy = dict(busy)
+        busy["gpu_busy_fraction"] = busy["kernel_ms_per_step"] / r["ms_per_step"]
     step_roof = {}
     for pth, res in out.items():
         if pth.startswith("mixed"):
             bpc = canonical_bytes_mixed(F, res["iters_per_step"], res["outer_per_step"],
                                         4 if pth == "mixed_plain_jacobi" else CHEB_DIVISIONS,
                                         multigrid=pth == "mixed")
+        elif pth == "fp64_multigrid":
+            # the V-cycle's canonical words in FP64 (twice the FP32 bytes)
+            bpc = 8.0 * (39 + 27 * F) + 2 * 4 * MG_WORDS_PER_ITER * res["iters_per_step"]
         else:
             bpc = canonical_bytes_fp64(F, res["iters_per_step"])
         step_roof[pth] = {"bytes_per_cell_step": bpc,
@@ -828,8 +1026,13 @@ def main(argv=None):
         "roofline": roofline,
         "step_roofline": step_roof,
     }
+    if busy:
+        line["gpu_busy"] = busy
     if "fp64" in out and "mixed" in out:
         line["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if "fp64_multigrid" in out:
+        line["speedup_mixed_vs_fp64_same_precond"] = (out["mixed"]["value"] /
+                                                      out["fp64_multigrid"]["value"])
         a, b = snaps["fp64"], snaps["mixed"]
         line["mixed_error_vs_fp64"] = {
             name: rel_l2(a[name], b[name], center=(name == "p")) for name in a}
@@ -858,12 +1061,31 @@ def main(argv=None):
         print(f"[bench] ensemble: batched mixed {ens['batched_mixed']['value']:.4g}, "
               f"sequential FP64 {ens['sequential_fp64']['value']:.4g} cell-steps/s",
               file=sys.stderr, flush=True)
+    if args.kh_steps > 0:
+        print(f"[bench] Kelvin-Helmholtz 512x4x256: {args.kh_steps} steps per path",
+              file=sys.stderr, flush=True)
+        line["kelvin_helmholtz"] = kelvin_helmholtz(H, W, args.kh_steps)
+        kh = line["kelvin_helmholtz"]
+        print(f"[bench] Kelvin-Helmholtz: mixed {kh['mixed']['value']:.4g}, FP64 "
+              f"{kh['fp64']['value']:.4g} cell-steps/s, errors {kh['mixed_error_vs_fp64']}",
+              file=sys.stderr, flush=True)
+    if args.large_steps > 0:
+        print(f"[bench] large domain 512x512x256: {args.large_steps} steps per path",
+              file=sys.stderr, flush=True)
+        line["large_domain"] = large_domain(H, W, args.large_steps)
+        ld = line["large_domain"]
+        print(f"[bench] large domain: mixed {ld['mixed']['value']:.4g}, FP64 "
+              f"{ld['fp64']['value']:.4g} cell-steps/s", file=sys.stderr, flush=True)
     if not args.no_stream:
         line["stream_copy_GBps"] = stream_copy_gbps(H, case.dom)
     if not args.no_cpu:
         cip, steps = args.cpu_sample.split(":")
         cx, cy = (int(x) for x in cip.split("x"))
-        line["cpu_baseline"] = cpu_baseline(W, cx, cy, int(steps))
+        print(f"[bench] CPU baseline: oracle, moist bubble {cx}x{cy}x128, {steps} step(s)",
+              file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(W, cx, cy, int(steps), case=case)
+        print(f"[bench] CPU baseline: {line['cpu_baseline']['value']:.4g} cell-steps/s on "
+              f"{line['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
     print(json.dumps(line))
     return 0
 

@@ -224,9 +224,23 @@ int upload_1d(dcm_ctx *c, const double *h, size_t n, int slot, const double **ou
     return DCM_OK;
 }
 
+// the profile filter is a comma-separated list of kernel names (empty: all)
+static bool prof_wanted(const dcm_ctx *c, const char *name) {
+    if (!c->profiling) return false;
+    if (c->prof_only.empty()) return true;
+    const size_t n = strlen(name);
+    size_t pos = 0;
+    while (pos <= c->prof_only.size()) {
+        size_t end = c->prof_only.find(',', pos);
+        if (end == std::string::npos) end = c->prof_only.size();
+        if (end - pos == n && c->prof_only.compare(pos, n, name) == 0) return true;
+        pos = end + 1;
+    }
+    return false;
+}
+
 int prof_begin(dcm_ctx *c, const char *name) {
-    if (!c->profiling) return -1;
-    if (!c->prof_only.empty() && c->prof_only != name) return -1;
+    if (!prof_wanted(c, name)) return -1;
     hipEvent_t a, b;
     if (c->event_pool.size() >= 2) {
         a = c->event_pool.back();
@@ -242,8 +256,7 @@ int prof_begin(dcm_ctx *c, const char *name) {
 }
 
 bool prof_events(dcm_ctx *c, const char *name, hipEvent_t *start, hipEvent_t *stop) {
-    if (!c->profiling) return false;
-    if (!c->prof_only.empty() && c->prof_only != name) return false;
+    if (!prof_wanted(c, name)) return false;
     hipEvent_t a, b;
     if (c->event_pool.size() >= 2) {
         a = c->event_pool.back();
@@ -289,6 +302,12 @@ extern "C" {
 const char *dcm_last_error(void) { return g_last_error.c_str(); }
 
 const char *dcm_version(void) { return "dcm_hip 1.0 gfx950"; }
+
+int dcm_device_memory(size_t *free_bytes, size_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return DCM_BAD_ARG;
+    DCM_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return DCM_OK;
+}
 
 int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *dy,
                        const double *dz, const double *dxn, const double *dyn,

@@ -264,7 +264,7 @@ void set_chebyshev(dcm_ctx *c, double a, int divisions = 4);
 // multigrid preconditioner (dcm_mg.hip)
 bool mg_possible(const Geo &g);
 int mg_prepare(dcm_ctx *c);
-int mg_correct(dcm_ctx *c, const float *b, float *x, const SolverScalars *sc);
+template <typename R> int mg_correct(dcm_ctx *c, const R *b, R *x, const SolverScalars *sc);
 constexpr float kMgOmega = 0.8f;  // damped-Jacobi weight of the V-cycle's smoother
 void mg_release(dcm_ctx *c);
 

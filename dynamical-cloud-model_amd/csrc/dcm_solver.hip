@@ -1278,8 +1278,10 @@ struct IterLaunch {
                 hipLaunchKernelGGL(k_fin_r, dim3(nsys), dim3(1024), 0, c->stream, P.args, seq);
         }
     }
-    // the multigrid preconditioner: FP32 inner solve of a single system
-    bool mg() const { return sizeof(R) == 4 && one() && c->mg_on && c->mg; }
+    // the multigrid preconditioner of a single system: the mixed path's FP32
+    // inner solve, or the FP64 BiCGSTAB with the V-cycle in place of the
+    // reference's sweeps
+    bool mg() const { return one() && c->mg_on && c->mg; }
     // the V-cycle applied to the right-hand side in ti, into oi: the first
     // sweep from zero is the prep kernel's ti/a_p in VWB; one more sweep, the
     // coarse-grid correction, two sweeps (the last into oi; P: p = x + bet*p)
@@ -1287,8 +1289,7 @@ struct IterLaunch {
         const R w = (R)kMgOmega;
         jac_launch<false, true>("jac", VWB, ti, VWA, w);
         const int tok = prof_begin(c, "mg_correct");
-        (void)mg_correct(c, (const float *)v(ti), (float *)v(VWA),
-                         (const SolverScalars *)M().sc);
+        (void)mg_correct<R>(c, (const R *)v(ti), v(VWA), (const SolverScalars *)M().sc);
         prof_end(c, tok);
         jac_launch<false, true>("jac", VWA, ti, VWB, w);
         jac_launch<P, true>(P ? "jac_p" : "jac", VWB, ti, oi, w);
@@ -1296,7 +1297,7 @@ struct IterLaunch {
     // dynamics.f90:391-425 with one V-cycle in place of each application of
     // the Jacobi sweeps: p = V(t) + bet*p, cs = V(s)
     void iteration_mg() const {
-        if constexpr (sizeof(R) == 4) {
+        {
             const SolverScalars *sc = (const SolverScalars *)M().sc;
             hipLaunchKernelGGL((k_jac_prep<R, 0>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
                                (const R *)v(VR), (const R *)v(VV1), v(VT), v(VWB), M().m,

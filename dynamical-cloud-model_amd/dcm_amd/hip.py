@@ -103,6 +103,7 @@ class InitProfiles(C.Structure):
 _SIGS = {
     "dcm_last_error": ([], C.c_char_p),
     "dcm_version": ([], C.c_char_p),
+    "dcm_device_memory": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], _i),
     "dcm_context_create": ([_i, _i, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                             C.POINTER(_vp)], _i),
     "dcm_context_destroy": ([_vp], _i),
@@ -218,6 +219,13 @@ def _call(name, *args):
     if st != 0:
         raise DcmError(name, st, load().dcm_last_error().decode())
     return st
+
+
+def device_memory():
+    """(free, total) bytes of device memory of the current device."""
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    _call("dcm_device_memory", C.byref(free), C.byref(total))
+    return free.value, total.value
 
 
 def _dptr(a: np.ndarray):

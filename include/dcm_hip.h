@@ -52,6 +52,10 @@ typedef struct dcm_model dcm_model;
 const char *dcm_last_error(void);
 /* Version string of the library build. */
 const char *dcm_version(void);
+/* Free and total device memory of the current device in bytes (the HBM
+ * footprint of a workload is the drop of *free_bytes across it).  No
+ * counterpart in the reference. */
+int dcm_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 /* ---------------------------------------------------------------------- */
 /* Context: grid metrics and module-level state.                          */
@@ -110,14 +114,17 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
  * :1028-1091) and whose vector routines use unweighted antidiffusive
  * velocities (:1384-1507, :1874-1909).  A setting, no counterpart call in
  * the reference, where the choice is made at link time (pamm/Makefile:1-3). */
-/* The preconditioner of DCM_MIXED's FP32 inner solve of a single model or
- * dcm_bicgstab_mixed call: on != 0 applies one geometric multigrid V-cycle
- * (damped-Jacobi smoothing, trilinear transfers, every dimension halved per
- * level) wherever the reference's iteration applies its Jacobi sweeps
- * (precondition, dynamics.f90:241-270, inside :390-430), so that the inner
- * solve needs a few iterations at any size; the outer FP64 refinement still
- * ends every solve at the requested true residual.  DCM_BAD_ARG (and no
- * change) when ip, jp and kp do not halve evenly down to 1024 cells.  Off
+/* The preconditioner of the pressure solves of a single model, of
+ * dcm_bicgstab_mixed and of dcm_bicgstab: on != 0 applies one geometric
+ * multigrid V-cycle (damped-Jacobi smoothing, trilinear transfers, every
+ * dimension halved per level) wherever the reference's iteration applies its
+ * Jacobi sweeps (precondition, dynamics.f90:241-270, inside :390-430), so
+ * that a solve needs a few iterations at any size.  The mixed path's FP32
+ * inner solve runs an FP32 V-cycle (its outer FP64 refinement still ends
+ * every solve at the requested true residual); the FP64 BiCGSTAB runs the
+ * same cycle in FP64, which is no longer the reference's arithmetic but
+ * measures the precision gain at an equal preconditioner.  DCM_BAD_ARG (and
+ * no change) when ip, jp and kp do not halve evenly down to 1024 cells.  Off
  * by default.  In a DCM_MIXED ensemble either every member has the setting
  * (each member is then solved with its own V-cycle, one after another, and
  * evolves bit for bit as when stepped alone) or none has it (one batched
@@ -340,7 +347,8 @@ int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats);
 /* Average kernel time (ms) of the named kernel over the last timed region,
  * and the number of launches it had; returns DCM_BAD_ARG if unknown. */
 int dcm_model_profile(dcm_model *m, int enable);
-/* Time only the kernel group `name` (NULL: every group). */
+/* Time only the kernel groups named in `name`, a comma-separated list
+ * (NULL or "": every group). */
 int dcm_model_profile_filter(dcm_model *m, const char *name);
 /* Relative residual reduction of each FP32 inner solve (mixed; default 1e-4). */
 int dcm_model_set_inner_rtol(dcm_model *m, double rtol);

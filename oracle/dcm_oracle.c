@@ -23,6 +23,28 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Host threads of the element-wise loops (or_set_threads; 1, the default:
+ * serial).  Every such loop writes disjoint elements from inputs it does not
+ * write, so its results do not depend on the thread count; the global sums
+ * keep the reference's serial order unless or_set_sum_blocked(2) asks for a
+ * parallel reduction (the CPU baseline of the benchmark only). */
+static int g_threads = 1;
+#define OMP_FOR _Pragma("omp parallel for schedule(static) if (g_threads > 1)")
+int or_set_threads(int n) {
+#ifdef _OPENMP
+    g_threads = n > 1 ? n : 1;
+    omp_set_num_threads(g_threads);
+    return g_threads;
+#else
+    (void)n;
+    g_threads = 1;
+    return 1;
+#endif
+}
 
 #define NK(g) ((size_t)(g)->kp + 2)
 #define NJ(g) ((size_t)(g)->jp + 2)
@@ -174,6 +196,7 @@ void or_upwind3d(const or_grid *g, double dt, const double *u, const double *v,
     double *fxr = zalloc(n), *fxl = zalloc(n), *fyr = zalloc(n);
     double *fyl = zalloc(n), *fzr = zalloc(n), *fzl = zalloc(n);
 #define F3(f, k, j, i) f[((size_t)((i)-1) * jp + ((j)-1)) * kp + ((k)-1)]
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -203,14 +226,17 @@ void or_upwind3d(const or_grid *g, double dt, const double *u, const double *v,
             }
     /* neumann flag: advection_3d.F90:95-111 */
     if (neumann == 1 || neumann == 2) {
+        OMP_FOR
         for (int i = 1; i <= ip; ++i)
             for (int j = 1; j <= jp; ++j)
                 F3(fzl, 1, j, i) = neumann == 1 ? F3(fzr, 1, j, i) : -F3(fzr, 1, j, i);
+        OMP_FOR
         for (int i = 1; i <= ip; ++i)
             for (int j = 1; j <= jp; ++j)
                 F3(fzr, kp, j, i) = neumann == 1 ? F3(fzl, kp, j, i) : -F3(fzl, kp, j, i);
     }
     /* advection_3d.F90:115 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k)
@@ -239,6 +265,7 @@ static void antidiffusive(const or_grid *g, double dt, const double *ut,
     const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
     const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
     const double *ra = g->rhoa, *rn = g->rhoan;
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -343,6 +370,7 @@ static void antidiffusive_plain(const or_grid *g, double dt, const double *ut,
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     const double *dx = g->dx, *dy = g->dy, *dz = g->dz;
     const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -428,6 +456,7 @@ static void limiter_directional(const or_grid *g, double dt, const double *ut,
         bu[d] = zalloc(N);
         bd[d] = zalloc(N);
     }
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -473,6 +502,7 @@ static void limiter_directional(const or_grid *g, double dt, const double *ut,
         or_fill_along_dim(g, bd[d], 0.0, 0.0);
     }
     /* eq 18: :666-682 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -500,6 +530,7 @@ static void limiter(const or_grid *g, double dt, const double *ut,
     const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
     const double *ra = g->rhoa, *rn = g->rhoan;
     /* eqs 20a,b of Smolarkiewicz & Grabowski (1990): :549-575 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -518,6 +549,7 @@ static void limiter(const or_grid *g, double dt, const double *ut,
                 A3(pmin, k, j, i) = mn;
             }
     /* eqs 19a,b: :585-616 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -543,6 +575,7 @@ static void limiter(const or_grid *g, double dt, const double *ut,
     or_fill_along_dim(g, bup, 0.0, 0.0);
     or_fill_along_dim(g, bdn, 0.0, 0.0);
     /* eq 18: :634-656 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -1019,6 +1052,7 @@ static void sources_advective(const or_grid *g, const double *u, const double *v
     const double *dxn = g->dxn, *dyn = g->dyn, *dzn = g->dzn;
     const double *ra = g->rhoa, *rn = g->rhoan;
     /* su: :743-757 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -1034,6 +1068,7 @@ static void sources_advective(const or_grid *g, const double *u, const double *v
                 A3(su, k, j, i) = rn[k] * ((X + Y) + Z);
             }
     /* sv: :761-775 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -1049,6 +1084,7 @@ static void sources_advective(const or_grid *g, const double *u, const double *v
                 A3(sv, k, j, i) = rn[k] * ((X + Y) + Z);
             }
     /* sw with buoyancy: :779-797 */
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
@@ -1403,37 +1439,75 @@ void or_sources_phys(const or_grid *g, const or_physics *ph, const double *u, co
 /* pressure solver                                                          */
 /* ------------------------------------------------------------------------ */
 
-/* set_mat_a coefficient, dynamics.f90:84-96; a_p is 1 in the halo (:57) */
-static inline double coef_ap(const or_grid *g, int k, int j, int i) {
-    return (((((-(1.0 / (g->dx[i - 1] * g->dxn[i]))) - 1.0 / (g->dx[i - 1] * g->dxn[i - 1])) -
-              1.0 / (g->dy[j - 1] * g->dyn[j])) -
-             1.0 / (g->dy[j - 1] * g->dyn[j - 1])) -
-            1.0 / (g->dz[k - 1] * g->dzn[k])) -
-           1.0 / (g->dz[k - 1] * g->dzn[k - 1]);
-}
 static inline int is_interior(const or_grid *g, int k, int j, int i) {
     return k >= 1 && k <= g->kp && j >= 1 && j <= g->jp && i >= 1 && i <= g->ip;
 }
 
-/* mat_ax, dynamics.f90:187-229.  Mutates the halo of x. */
-void or_matvec(const or_grid *g, double *x, double *ax) {
+/* The stencil's coefficients as set_mat_a stores them (dynamics.f90:84-96):
+ * the reference keeps a_e..a_p as arrays computed once per solve, so its
+ * stencil costs loads, not divisions.  The same values here come from
+ * per-axis tables (a_e, a_w depend on i only, a_n, a_s on j, a_u, a_d on k)
+ * and a_p is summed from them in the order of :91-93, which gives the same
+ * bits as the inline expression of coef_ap. */
+typedef struct {
+    double *ae, *aw, *an, *as, *au, *ad; /* index 1..n */
+} coef_tables;
+
+static coef_tables coef_tables_make(const or_grid *g) {
+    coef_tables t;
+    t.ae = zalloc((size_t)g->ip + 2);
+    t.aw = zalloc((size_t)g->ip + 2);
+    t.an = zalloc((size_t)g->jp + 2);
+    t.as = zalloc((size_t)g->jp + 2);
+    t.au = zalloc((size_t)g->kp + 2);
+    t.ad = zalloc((size_t)g->kp + 2);
+    for (int i = 1; i <= g->ip; ++i) {
+        t.ae[i] = 1.0 / (g->dx[i - 1] * g->dxn[i]);
+        t.aw[i] = 1.0 / (g->dx[i - 1] * g->dxn[i - 1]);
+    }
+    for (int j = 1; j <= g->jp; ++j) {
+        t.an[j] = 1.0 / (g->dy[j - 1] * g->dyn[j]);
+        t.as[j] = 1.0 / (g->dy[j - 1] * g->dyn[j - 1]);
+    }
+    for (int k = 1; k <= g->kp; ++k) {
+        t.au[k] = 1.0 / (g->dz[k - 1] * g->dzn[k]);
+        t.ad[k] = 1.0 / (g->dz[k - 1] * g->dzn[k - 1]);
+    }
+    return t;
+}
+
+static void coef_tables_free(coef_tables *t) {
+    free(t->ae);
+    free(t->aw);
+    free(t->an);
+    free(t->as);
+    free(t->au);
+    free(t->ad);
+}
+
+static inline double tab_ap(const coef_tables *t, int k, int j, int i) {
+    return ((((-t->ae[i] - t->aw[i]) - t->an[j]) - t->as[j]) - t->au[k]) - t->ad[k];
+}
+
+static void matvec_tab(const or_grid *g, const coef_tables *t, double *x, double *ax) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     or_fill_along_dim(g, x, 0.0, 0.0);
+    OMP_FOR
     for (int i = 0; i <= ip + 1; ++i)
         for (int j = 0; j <= jp + 1; ++j) A3(x, 0, j, i) = A3(x, 1, j, i);
     for (int i = 0; i <= ip + 1; ++i)
         for (int j = 0; j <= jp + 1; ++j) A3(x, kp + 1, j, i) = A3(x, kp, j, i);
-    memset(ax, 0, NTOT(g) * sizeof(double));
+    const size_t slab = (size_t)(jp + 2) * (kp + 2);
+    OMP_FOR
+    for (int i = 0; i <= ip + 1; ++i) memset(ax + slab * i, 0, slab * sizeof(double));
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 1; k <= kp; ++k) {
-                const double ae = 1.0 / (g->dx[i - 1] * g->dxn[i]);
-                const double aw = 1.0 / (g->dx[i - 1] * g->dxn[i - 1]);
-                const double an = 1.0 / (g->dy[j - 1] * g->dyn[j]);
-                const double as = 1.0 / (g->dy[j - 1] * g->dyn[j - 1]);
-                const double au = 1.0 / (g->dz[k - 1] * g->dzn[k]);
-                const double ad = 1.0 / (g->dz[k - 1] * g->dzn[k - 1]);
-                const double ap = coef_ap(g, k, j, i);
+                const double ae = t->ae[i], aw = t->aw[i];
+                const double an = t->an[j], as = t->as[j];
+                const double au = t->au[k], ad = t->ad[k];
+                const double ap = tab_ap(t, k, j, i);
                 A3(ax, k, j, i) = ((((((aw * A3(x, k, j, i - 1) + ae * A3(x, k, j, i + 1)) +
                                        as * A3(x, k, j - 1, i)) + an * A3(x, k, j + 1, i)) +
                                      ad * A3(x, k - 1, j, i)) + au * A3(x, k + 1, j, i)) +
@@ -1441,34 +1515,60 @@ void or_matvec(const or_grid *g, double *x, double *ax) {
             }
 }
 
-/* precondition, dynamics.f90:241-270 (whole-array statements) */
-void or_precondition(const or_grid *g, double *s, const double *r, int prit) {
+/* mat_ax, dynamics.f90:187-229.  Mutates the halo of x. */
+void or_matvec(const or_grid *g, double *x, double *ax) {
+    coef_tables t = coef_tables_make(g);
+    matvec_tab(g, &t, x, ax);
+    coef_tables_free(&t);
+}
+
+/* precondition, dynamics.f90:241-270 (whole-array statements; a_p is 1 in
+ * the halo, :57) */
+static void precondition_tab(const or_grid *g, const coef_tables *ct, double *s, const double *r,
+                             int prit, double *t) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
-    double *t = zalloc(NTOT(g));
+    OMP_FOR
     for (int i = 0; i <= ip + 1; ++i)
         for (int j = 0; j <= jp + 1; ++j)
             for (int k = 0; k <= kp + 1; ++k)
-                A3(s, k, j, i) = A3(r, k, j, i) / (is_interior(g, k, j, i) ? coef_ap(g, k, j, i) : 1.0);
+                A3(s, k, j, i) = A3(r, k, j, i) / (is_interior(g, k, j, i) ? tab_ap(ct, k, j, i) : 1.0);
     for (int it = 1; it <= prit; ++it) {
-        or_matvec(g, s, t);
+        matvec_tab(g, ct, s, t);
+        OMP_FOR
         for (int i = 0; i <= ip + 1; ++i)
             for (int j = 0; j <= jp + 1; ++j)
                 for (int k = 0; k <= kp + 1; ++k)
                     A3(s, k, j, i) = A3(s, k, j, i) + (A3(r, k, j, i) - A3(t, k, j, i)) /
-                                                          (is_interior(g, k, j, i) ? coef_ap(g, k, j, i) : 1.0);
+                                                          (is_interior(g, k, j, i) ? tab_ap(ct, k, j, i) : 1.0);
     }
-    free(t);
+}
+
+void or_precondition(const or_grid *g, double *s, const double *r, int prit) {
+    coef_tables t = coef_tables_make(g);
+    double *w = zalloc(NTOT(g));
+    precondition_tab(g, &t, s, r, prit, w);
+    free(w);
+    coef_tables_free(&t);
 }
 
 /* Summation order of or_inn_prod.  0: the reference's serial loop.  1: 64
  * interleaved partial sums combined pairwise, a different but equally valid
  * order, used only to measure how far results move when only the order of
- * the dot products changes. */
+ * the dot products changes.  2: a parallel reduction over the host threads
+ * (an order that depends on the thread count; for timing only). */
 static int g_sum_blocked = 0;
 void or_set_sum_blocked(int on) { g_sum_blocked = on; }
 
 /* inn_prod, dynamics.f90:458-482: sequential, i-j-k loop order */
 double or_inn_prod(const or_grid *g, const double *x, const double *y) {
+    if (g_sum_blocked == 2) {  /* a parallel reduction (timing only) */
+        double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s) if (g_threads > 1)
+        for (int i = 1; i <= g->ip; ++i)
+            for (int j = 1; j <= g->jp; ++j)
+                for (int k = 1; k <= g->kp; ++k) s += A3(x, k, j, i) * A3(y, k, j, i);
+        return s;
+    }
     if (g_sum_blocked) {
         double part[64] = {0};
         size_t c = 0;
@@ -1497,6 +1597,7 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     const int ip = g->ip, jp = g->jp, kp = g->kp;
     const size_t N = NTOT(g);
     if (first_call && test_solver) { /* :354-364 */
+        OMP_FOR
         for (int i = 1; i <= ip; ++i)
             for (int j = 1; j <= jp; ++j)
                 for (int k = 1; k <= kp; ++k) {
@@ -1506,10 +1607,13 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     }
     double *ax = zalloc(N), *r = zalloc(N), *cr = zalloc(N), *cs = zalloc(N);
     double *p = zalloc(N), *v1 = zalloc(N), *t = zalloc(N), *s = zalloc(N);
+    double *work = zalloc(N); /* precondition's local t */
+    coef_tables ct = coef_tables_make(g);
     int status = OR_OK, it_done = 0;
     double sc_err = sqrt(or_inn_prod(g, b, b));
     if (sc_err < 0.01) sc_err = 0.01; /* :371 */
-    or_matvec(g, x, ax);
+    matvec_tab(g, &ct, x, ax);
+    OMP_FOR
     for (size_t q = 0; q < N; ++q) {
         r[q] = b[q] - ax[q];
         cr[q] = r[q];
@@ -1521,6 +1625,8 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
         *iters = 0;
         *errp = err;
         free(ax); free(r); free(cr); free(cs); free(p); free(v1); free(t); free(s);
+        free(work);
+        coef_tables_free(&ct);
         return OR_OK;
     }
     double alf = 1.0, omg = 1.0, nrm = 1.0, rho, bet, tt, ts;
@@ -1528,15 +1634,18 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     for (it = 1; it <= itmax; ++it) {
         rho = or_inn_prod(g, r, cr);
         bet = (rho / nrm) * (alf / omg);
+        OMP_FOR
         for (size_t q = 0; q < N; ++q) t[q] = r[q] - bet * (omg * v1[q]);
-        or_precondition(g, s, t, 3);
+        precondition_tab(g, &ct, s, t, 3, work);
+        OMP_FOR
         for (size_t q = 0; q < N; ++q) p[q] = s[q] + bet * p[q];
-        or_matvec(g, p, v1);
+        matvec_tab(g, &ct, p, v1);
         nrm = or_inn_prod(g, cr, v1);
         alf = rho / nrm;
+        OMP_FOR
         for (size_t q = 0; q < N; ++q) s[q] = r[q] - alf * v1[q];
-        or_precondition(g, cs, s, 3);
-        or_matvec(g, cs, t);
+        precondition_tab(g, &ct, cs, s, 3, work);
+        matvec_tab(g, &ct, cs, t);
         tt = or_inn_prod(g, t, t);
         ts = or_inn_prod(g, t, s);
         omg = ts / tt;
@@ -1545,7 +1654,9 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
             it_done = it;
             break;
         }
+        OMP_FOR
         for (size_t q = 0; q < N; ++q) x[q] = (x[q] + alf * p[q]) + omg * cs[q];
+        OMP_FOR
         for (size_t q = 0; q < N; ++q) r[q] = s[q] - omg * t[q];
         err = sqrt(or_inn_prod(g, r, r)) / sc_err;
         nrm = rho;
@@ -1554,6 +1665,7 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     }
     if (status == OR_OK && err > tol) status = OR_NOT_CONVERGED; /* :431-434 */
     if (status == OR_OK) { /* :437-442 */
+        OMP_FOR
         for (int i = 0; i <= ip + 1; ++i)
             for (int j = 0; j <= jp + 1; ++j) A3(x, kp + 1, j, i) = A3(x, kp, j, i);
         for (int i = 0; i <= ip + 1; ++i)
@@ -1563,6 +1675,8 @@ int or_bicgstab(const or_grid *g, const double *xg, const double *yg,
     *iters = it_done;
     *errp = err;
     free(ax); free(r); free(cr); free(cs); free(p); free(v1); free(t); free(s);
+    free(work);
+    coef_tables_free(&ct);
     return status;
 }
 
@@ -1585,18 +1699,21 @@ void or_advance_momentum(const or_grid *g, double dt, double *u, double *v,
                          const double *zw, const double *su, const double *sv,
                          const double *sw, const double *p) {
     const int ip = g->ip, jp = g->jp, kp = g->kp;
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 0; k <= kp; ++k)
                 A3(u, k, j, i) = A3(zu, k, j, i) +
                                  ((A3(su, k, j, i) - (A3(p, k, j, i + 1) - A3(p, k, j, i)) / g->dxn[i]) * dt) /
                                      g->rhoan[k];
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 0; k <= kp; ++k)
                 A3(v, k, j, i) = A3(zv, k, j, i) +
                                  ((A3(sv, k, j, i) - (A3(p, k, j + 1, i) - A3(p, k, j, i)) / g->dyn[j]) * dt) /
                                      g->rhoan[k];
+    OMP_FOR
     for (int i = 1; i <= ip; ++i)
         for (int j = 1; j <= jp; ++j)
             for (int k = 0; k <= kp; ++k)
@@ -1729,6 +1846,7 @@ int or_model_step(const or_grid *g, or_model *m) {
     double *lev[3][2] = {{u, zu}, {v, zv}, {w, zw}};
     for (int c = 0; c < 3; ++c) {
         double *a = lev[c][0], *z = lev[c][1];
+        OMP_FOR
         for (size_t x = 0; x < N; ++x) {
             const double tmp = 0.99 * a[x] + 0.01 * z[x];
             z[x] = 0.01 * a[x] + 0.99 * z[x];
@@ -1740,12 +1858,14 @@ int or_model_step(const or_grid *g, or_model *m) {
     or_fill_full(g, u, 0.0, 0.0); /* :656-661 */
     or_fill_full(g, v, 0.0, 0.0);
     or_fill_full(g, w, 0.0, 0.0);
+    OMP_FOR
     for (int i = 0; i <= ip + 1; ++i) /* :663-674 */
         for (int j = 0; j <= jp + 1; ++j) {
             A3(w, 0, j, i) = 0.0;
             A3(u, 0, j, i) = -A3(u, 1, j, i);
             A3(v, 0, j, i) = -A3(v, 1, j, i);
         }
+    OMP_FOR
     for (int i = 0; i <= ip + 1; ++i)
         for (int j = 0; j <= jp + 1; ++j) {
             A3(w, kp + 1, j, i) = 0.0;

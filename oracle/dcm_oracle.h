@@ -129,6 +129,9 @@ void or_divergence(const or_grid *g, const double *u, const double *v,
 
 /* 1: blocked summation in or_inn_prod (sensitivity measurements only) */
 void or_set_sum_blocked(int on);
+/* host threads of the element-wise loops (1: serial, the default); returns
+ * the count in effect (1 without OpenMP) */
+int or_set_threads(int n);
 /* 0: pamm/sfvt/advection_3d.F90 (linked into the model), 1: sfvt/advection_3d.F90 */
 void or_set_mpdata_variant(int v);
 

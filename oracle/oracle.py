@@ -138,6 +138,8 @@ def lib():
         L.or_advance_momentum.argtypes = [g, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_divergence.argtypes = [g, _dp, _dp, _dp, _dp]
         L.or_set_sum_blocked.argtypes = [C.c_int]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
         L.or_set_mpdata_variant.argtypes = [C.c_int]
         L.or_advance_scalars.argtypes = [g, d, _dp, _dp, _dp, _dp, C.c_int, C.c_int]
         m = C.POINTER(_Model)
@@ -309,9 +311,16 @@ def set_mpdata_variant(v: int):
     lib().or_set_mpdata_variant(int(v))
 
 
-def set_sum_blocked(on: bool):
-    """Blocked (64-way) summation in the dot products instead of the serial loop."""
+def set_sum_blocked(on):
+    """Blocked (64-way) summation in the dot products instead of the serial
+    loop; 2: a parallel reduction over the host threads (timing only)."""
     lib().or_set_sum_blocked(int(on))
+
+
+def set_threads(n: int) -> int:
+    """Host threads of the element-wise loops (1: serial, the default); the
+    results of those loops do not depend on it.  Returns the count in effect."""
+    return lib().or_set_threads(int(n))
 
 
 def divergence(g: Grid, u, v, w, div):

@@ -93,3 +93,37 @@ def test_warm_bubble_64_cubed_matches_oracle():
             np.testing.assert_array_equal(gm.get(H.F_P)[mask], om.p[mask])
         finally:
             gm.close()
+
+
+def test_kelvin_helmholtz_60_steps_mixed_tracks_fp64():
+    """The shear layer at its full size (512x4x256) for 60 steps on both
+    paths: every solve at FP64 true residual < 1e-10, no FP64 fallback of
+    the mixed path, and the mixed fields within 1e-5 (relative L2; p with
+    its mean removed, the null space of the Poisson operator) of FP64."""
+    steps = 60
+    case = W.kelvin_helmholtz(steps=steps)
+    out = {}
+    for prec in (H.FP64, H.MIXED):
+        with H.Context(case.dom) as c:
+            m = H.Model(c, case, prec)
+            try:
+                m.set_check_residual(True)
+                st = m.step(steps)
+                print(f"  precision {prec}: iterations {st.iters_total}, outer "
+                      f"{st.outer_total}, FP64 fallbacks {st.fallbacks}, true residual "
+                      f"{st.max_true_resid:.3g}", flush=True)
+                assert st.max_true_resid < 1e-10, (prec, st.max_true_resid)
+                if prec == H.MIXED:
+                    assert st.fallbacks == 0
+                out[prec] = {"p": m.get(H.F_P), "u": m.get(H.F_U), "w": m.get(H.F_W),
+                             "th": m.get(H.F_TH)}
+            finally:
+                m.close()
+    for key in out[H.FP64]:
+        a = out[H.FP64][key][1:-1, 1:-1, 1:-1]
+        b = out[H.MIXED][key][1:-1, 1:-1, 1:-1]
+        if key == "p":
+            a, b = a - a.mean(), b - b.mean()
+        err = rel_l2(a, b)
+        print(f"  {key}: mixed vs FP64 relative L2 {err:.3g}", flush=True)
+        assert err < 1e-5, (key, err)

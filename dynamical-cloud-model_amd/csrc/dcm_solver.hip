@@ -43,6 +43,7 @@
 #include <cfloat>
 #include <cstddef>
 #include <cmath>
+#include <mutex>
 
 #include <cstring>
 
@@ -1772,6 +1773,49 @@ int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double to
     return st;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of dcm_inn_prod_noalloc: the partial sums of one dot
+// product and its result, in a file-scope array that each device holds from
+// module load to process end.  2 MiB of partials cover any grid whose
+// interior launch has at most kNoAllocPartials blocks (ceil(kp/64) x
+// ceil(jp/4) x ip for kp >= 64; 512x512x256 needs 262144).  Host code never
+// passes `scratch` itself to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocPartials = size_t(1) << 18;
+constexpr size_t kNoAllocDoubles = kNoAllocPartials + 32;  // + the result slot, padded
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+static double *g_scratch_dev[kNoAllocMaxDevices];  // device address of `scratch` per device
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on `device` (looked up on the first call
+// for that device, with that device current)
+static int noalloc_scratch(int device, double **out) {
+    if (device < 0 || device >= kNoAllocMaxDevices) {
+        set_error("dcm_inn_prod_noalloc: device ordinal " + std::to_string(device) +
+                  " beyond the " + std::to_string(kNoAllocMaxDevices) + " supported");
+        return DCM_BAD_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        int cur = -1;
+        DCM_HIP(hipGetDevice(&cur));
+        if (cur != device) DCM_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return DCM_HIP_ERROR;
+        }
+        g_scratch_dev[device] = (double *)p;
+    }
+    *out = g_scratch_dev[device];
+    return DCM_OK;
+}
+
 }  // namespace dcm
 
 using namespace dcm;
@@ -1825,6 +1869,34 @@ int dcm_inn_prod(dcm_ctx *c, const double *x, const double *y, double *out) {
     }
     DCM_HIP(hipMemcpyAsync(c->hscal + 40, c->dscal + 40, sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
+    DCM_HIP(hipStreamSynchronize(c->stream));
+    *out = c->hscal[40];
+    return DCM_OK;
+}
+
+int dcm_inn_prod_noalloc(dcm_ctx *c, const double *x, const double *y, double *out) {
+    if (!c || !x || !y || !out) return DCM_BAD_ARG;
+    const Interior L = interior_launch(c->geo);
+    const size_t nb = (size_t)L.grid.x * L.grid.y * L.grid.z;
+    if (nb > kNoAllocPartials) {
+        set_error("dcm_inn_prod_noalloc: the grid's interior launch has " + std::to_string(nb) +
+                  " blocks; the fixed scratch holds the partial sums of at most " +
+                  std::to_string(kNoAllocPartials));
+        return DCM_BAD_ARG;
+    }
+    double *s = nullptr;
+    DCM_CHECK(noalloc_scratch(c->device, &s));
+    double *res = s + kNoAllocPartials;  // every slot written before it is read
+    if (c->seq_sums) {
+        hipLaunchKernelGGL(k_seq_dot_padded, dim3(1), dim3(64), 0, c->stream, x, y, c->geo, res);
+        DCM_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(k_dot_padded, L.grid, L.block, 0, c->stream, x, y, c->geo, s);
+        DCM_LAUNCH_CHECK();
+        DCM_CHECK(finalize_sums(c, s, (int)nb, 1, res));
+    }
+    // into the context's pinned host scalars (allocated with the context)
+    DCM_HIP(hipMemcpyAsync(c->hscal + 40, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     DCM_HIP(hipStreamSynchronize(c->stream));
     *out = c->hscal[40];
     return DCM_OK;

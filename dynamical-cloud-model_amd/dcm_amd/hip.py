@@ -138,6 +138,7 @@ _SIGS = {
     "dcm_matvec": ([_vp, _vp, _vp], _i),
     "dcm_precondition": ([_vp, _vp, _vp, _i], _i),
     "dcm_inn_prod": ([_vp, _vp, _vp, _dp], _i),
+    "dcm_inn_prod_noalloc": ([_vp, _vp, _vp, _dp], _i),
     "dcm_set_sum_order": ([_vp, _i], _i),
     "dcm_set_precond_chebyshev": ([_vp, _d, _i], _i),
     "dcm_set_mpdata_variant": ([_vp, _i], _i),
@@ -426,6 +427,13 @@ class Context:
     def inn_prod(self, x, y) -> float:
         out = _d()
         _call("dcm_inn_prod", self.h, x.ptr, y.ptr, C.byref(out))
+        return out.value
+
+    def inn_prod_noalloc(self, x, y) -> float:
+        """inn_prod with its partial sums in the library's fixed device
+        scratch (no allocation; calls overlapping on one device race)."""
+        out = _d()
+        _call("dcm_inn_prod_noalloc", self.h, x.ptr, y.ptr, C.byref(out))
         return out.value
 
     def bicgstab(self, sw, x, b, tol=1e-10, test_solver=False, itmax=0, check=True):

@@ -260,6 +260,18 @@ int dcm_set_sum_order(dcm_ctx *ctx, int order);
 int dcm_set_precond_chebyshev(dcm_ctx *ctx, double a, int divisions);
 /* inn_prod (:458-482): interior dot product, result to *out (host). */
 int dcm_inn_prod(dcm_ctx *ctx, const double *x, const double *y, double *out);
+/* dcm_inn_prod that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels, same result bits; the partial sums live in a fixed 2 MiB device
+ * array (each device the library runs on holds it until the process ends)
+ * instead of the context's grow-only buffer.  Grids whose interior launch
+ * has more than 2^18 blocks (ceil(kp/64)*ceil(jp/4)*ip for kp >= 64;
+ * 512x512x256 is the largest such cube-like grid) get DCM_BAD_ARG.  Every
+ * call on one device shares that array: calls that overlap on one device
+ * (different streams or threads) race and must be serialised by the
+ * caller. */
+int dcm_inn_prod_noalloc(dcm_ctx *ctx, const double *x, const double *y, double *out);
 /* bicgstab (:293-444).  xg,yg,zg are host coordinate arrays used only for
  * the test_solver right-hand side on the context's first call.  itmax <= 0
  * means the reference's 9999.  *iters / *err (may be NULL) receive the

@@ -194,6 +194,9 @@ struct dcm_ctx {
     // (dcm_set_precond_multigrid; dcm_mg.hip): the hierarchy and the switch
     void *mg = nullptr;
     bool mg_on = false;
+    // levels of at most 16384 cells in one single-block kernel (default), or
+    // each pass of them in its own launch (the same arithmetic)
+    bool mg_tail = true;
     double cheb_a = 0.0;
     int cheb_deg = 4;  // a_p divisions per application (the reference's 4)
     double cheb_w[16] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0,

@@ -25,10 +25,27 @@
 //                set_mat_a, dynamics.f90:84-96); a coarse coefficient is the
 //                mean of its two children's divided by 4 (twice the spacing),
 //                exact on the reference's uniform grids;
-//   coarsest     one 1024-thread block iterating the smoother in LDS.
+//   coarsest     solved exactly: x = M b with M = (A_c + e e^T)^-1, e the
+//                constant vector (the null space of the periodic/Neumann
+//                operator; for b in the range of A_c this is the zero-mean
+//                solution), M precomputed on the host in FP64 by
+//                Gauss-Jordan elimination and applied as one dense
+//                matrix-vector product by one block; above 512 cells, or
+//                where a pivot vanishes, the coarsest level iterates the
+//                smoother 96 times instead.
+// Launch granularity: a level of a few thousand cells takes as long to
+// launch as to sweep, so every level from the first that fits the LDS of
+// one workgroup (with all the levels below it) down to the coarsest runs in
+// ONE kernel of one 1024-thread block (k_mg_tail): each pass of the cycle
+// over those levels is a loop over the level's cells followed by a
+// workgroup barrier, iterates, right-hand sides and 1-D coefficients held
+// in LDS.  At 256x256x128 that is levels 3-5 in FP32 (levels 4-5 in FP64):
+// 14 launches per V-cycle become one, with the same arithmetic.
 // The kernels and the hierarchy exist in FP32 (the mixed path's inner solve)
 // and FP64 (the FP64 BiCGSTAB with the V-cycle in place of the reference's
 // sweeps, which separates the precision gain from the preconditioner gain).
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "dcm_internal.h"
@@ -39,6 +56,8 @@ namespace {
 
 constexpr int kCoarseSweeps = 96;  // smoother iterations on the coarsest level
 constexpr int kCoarseMax = 1024;   // cells of the coarsest level (one block)
+constexpr int kDenseMax = 512;     // coarsest levels solved by the dense inverse
+                                   // (O(n^3) host work at set-up)
 
 template <typename R> struct Lv {
     int ip, jp, kp, n;
@@ -78,6 +97,24 @@ __device__ __forceinline__ R ax_of(const Lv<R> &L, P x, int k, int j, int i, R a
            L.ad[k] * x[at(L, kd, j, i)] + L.au[k] * x[at(L, ku, j, i)] + ap * x[at(L, k, j, i)];
 }
 
+// One thread per cell of a level on a 3-D grid: x-threads along k (the
+// fastest index), y-threads along j, one i per grid z (no divisions).
+struct Grid3 {
+    dim3 grid, block;
+};
+template <typename R> static Grid3 grid3(const Lv<R> &L) {
+    const int bx = L.kp >= 64 ? 64 : (L.kp >= 32 ? 32 : (L.kp >= 16 ? 16 : 8));
+    const int by = 256 / bx;
+    return {dim3((L.kp + bx - 1) / bx, (L.jp + by - 1) / by, L.ip), dim3(bx, by, 1)};
+}
+template <typename R>
+__device__ __forceinline__ bool cell3(const Lv<R> &L, int &k, int &j, int &i) {
+    k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    i = blockIdx.z + 1;
+    return k <= L.kp && j <= L.jp;
+}
+
 // xo = x + w (b - A x)/a_p; x == nullptr: a sweep from zero, xo = w b/a_p
 template <typename R>
 __global__ __launch_bounds__(256) void k_mg_smooth(const R *__restrict__ b,
@@ -85,10 +122,9 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const R *__restrict__ b,
                                                    R *__restrict__ xo, Lv<R> L, R w,
                                                    const SolverScalars *sc) {
     if (stopped(sc)) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= L.n) return;
     int k, j, i;
-    cell_of(L, c, k, j, i);
+    if (!cell3(L, k, j, i)) return;
+    const int c = at(L, k, j, i);
     const R ap = ap_of(L, k, j, i);
     const R inv = inv_of<R>(ap);
     if (!x) {
@@ -105,10 +141,9 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const R *__restrict__ b,
                                                      R *__restrict__ bc, Lv<R> F, Lv<R> C,
                                                      const SolverScalars *sc) {
     if (stopped(sc)) return;
-    const int cc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cc >= C.n) return;
     int K, J, I;
-    cell_of(C, cc, K, J, I);
+    if (!cell3(C, K, J, I)) return;
+    const int cc = at(C, K, J, I);
     R acc = R(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -126,10 +161,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
                                                     R *__restrict__ x, Lv<R> F, Lv<R> C,
                                                     const SolverScalars *sc) {
     if (stopped(sc)) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= F.n) return;
     int k, j, i;
-    cell_of(F, c, k, j, i);
+    if (!cell3(F, k, j, i)) return;
+    const int c = at(F, k, j, i);
     const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
     // the second coarse cell along each axis: the one on this child's side
     int I2 = (i & 1) ? I - 1 : I + 1;
@@ -147,16 +181,34 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
     x[c] += v;
 }
 
-// the coarsest level: `sweeps` smoother iterations from zero in LDS, one block
+// row c of the dense coarsest solve: sum over r of M[c][r] b[r], M stored
+// transposed (Mt[r n + c]) so that the threads of a block read whole rows
+template <typename R, typename PB>
+__device__ __forceinline__ R dense_row(const R *__restrict__ Mt, int n, PB b, int c) {
+    R acc = R(0);
+#pragma unroll 8
+    for (int r = 0; r < n; ++r) acc += Mt[(size_t)r * n + c] * b[r];
+    return acc;
+}
+
+// the coarsest level, one block: x = M b when Mt is given, else `sweeps`
+// smoother iterations from zero in LDS
 template <typename R>
 __global__ __launch_bounds__(kCoarseMax) void k_mg_coarse(const R *__restrict__ b,
                                                           R *__restrict__ x, Lv<R> L,
                                                           int sweeps, R w,
-                                                          const SolverScalars *sc) {
+                                                          const SolverScalars *sc,
+                                                          const R *__restrict__ Mt) {
     __shared__ R xa[kCoarseMax], xb[kCoarseMax];
     if (stopped(sc)) return;  // the same value for every thread of the block
     const int c = threadIdx.x;
     const bool act = c < L.n;
+    if (Mt) {
+        if (act) xb[c] = b[c];
+        __syncthreads();
+        if (act) x[c] = dense_row(Mt, L.n, (const R *)xb, c);
+        return;
+    }
     int k = 1, j = 1, i = 1;
     R ap = R(-1), bc = R(0), inv = R(0);
     if (act) {
@@ -178,6 +230,218 @@ __global__ __launch_bounds__(kCoarseMax) void k_mg_coarse(const R *__restrict__ 
     if (act) x[c] = cur[c];
 }
 
+// ---------------------------------------------------------------------------
+// The tail of the V-cycle in one block: levels t0..t0+nl-1 of the hierarchy
+// (the last the coarsest).  Level t0's right-hand side comes from global
+// memory (the restriction kernel of the level above writes it) and its
+// result goes back there (for the prolongation into the level above);
+// everything else stays in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kTailThreads = 1024;
+constexpr int kTailMax = 8;  // levels in the tail
+
+template <typename R> struct TailArgs {
+    Lv<R> lv[kTailMax];  // the tail's levels (global coefficient pointers)
+    int nl;              // levels in the tail; the last is the coarsest
+    int coef[kTailMax];  // LDS offset (elements of R) of each level's six 1-D arrays
+    int xo[kTailMax], to[kTailMax], bo[kTailMax];  // LDS offsets of x, t and b (b: levels >= 1)
+    const R *b0;         // right-hand side of the first tail level
+    R *x0;               // its result
+    const R *Mt;         // the coarsest level's dense solve (null: the sweeps)
+};
+
+// level l of the tail with its coefficient arrays in LDS
+template <typename R>
+__device__ __forceinline__ Lv<R> tail_level(const TailArgs<R> &A, R *lds, int l) {
+    Lv<R> L = A.lv[l];
+    R *c = lds + A.coef[l];
+    const int ni = L.ip + 2, nj = L.jp + 2, nk = L.kp + 2;
+    L.ae = c;
+    L.aw = c + ni;
+    L.an = c + 2 * ni;
+    L.as = c + 2 * ni + nj;
+    L.au = c + 2 * ni + 2 * nj;
+    L.ad = c + 2 * ni + 2 * nj + nk;
+    return L;
+}
+
+// t = w b/a_p (a sweep from zero)
+template <typename R, typename PB, typename PO>
+__device__ __forceinline__ void tail_smooth0(const Lv<R> &L, PB b, PO t, R w) {
+    for (int c = threadIdx.x; c < L.n; c += kTailThreads) {
+        int k, j, i;
+        cell_of(L, c, k, j, i);
+        const R inv = inv_of<R>(ap_of(L, k, j, i));
+        t[c] = w * b[c] * inv;
+    }
+}
+// xo = x + w (b - A x)/a_p
+template <typename R, typename PB, typename PX, typename PO>
+__device__ __forceinline__ void tail_smooth(const Lv<R> &L, PB b, PX x, PO xo, R w) {
+    for (int c = threadIdx.x; c < L.n; c += kTailThreads) {
+        int k, j, i;
+        cell_of(L, c, k, j, i);
+        const R ap = ap_of(L, k, j, i);
+        const R inv = inv_of<R>(ap);
+        xo[c] = x[c] + w * (b[c] - ax_of(L, x, k, j, i, ap)) * inv;
+    }
+}
+// bc = mean over the eight children of (b - A x)
+template <typename R, typename PB, typename PX, typename PC>
+__device__ __forceinline__ void tail_restrict(const Lv<R> &F, const Lv<R> &C, PB b, PX x,
+                                              PC bc) {
+    for (int cc = threadIdx.x; cc < C.n; cc += kTailThreads) {
+        int K, J, I;
+        cell_of(C, cc, K, J, I);
+        R acc = R(0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = 2 * K - 1 + (q & 1), j = 2 * J - 1 + ((q >> 1) & 1),
+                      i = 2 * I - 1 + (q >> 2);
+            const R ap = ap_of(F, k, j, i);
+            acc += b[at(F, k, j, i)] - ax_of(F, x, k, j, i, ap);
+        }
+        bc[cc] = R(0.125) * acc;
+    }
+}
+// x += trilinear interpolation of xc
+template <typename R, typename PC, typename PX>
+__device__ __forceinline__ void tail_prolong(const Lv<R> &F, const Lv<R> &C, PC xc, PX x) {
+    for (int c = threadIdx.x; c < F.n; c += kTailThreads) {
+        int k, j, i;
+        cell_of(F, c, k, j, i);
+        const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
+        int I2 = (i & 1) ? I - 1 : I + 1;
+        int J2 = (j & 1) ? J - 1 : J + 1;
+        int K2 = (k & 1) ? K - 1 : K + 1;
+        I2 = I2 < 1 ? C.ip : (I2 > C.ip ? 1 : I2);
+        J2 = J2 < 1 ? C.jp : (J2 > C.jp ? 1 : J2);
+        K2 = K2 < 1 ? 1 : (K2 > C.kp ? C.kp : K2);
+        const R a = R(0.75), q = R(0.25);
+        const R v =
+            a * (a * (a * xc[at(C, K, J, I)] + q * xc[at(C, K2, J, I)]) +
+                 q * (a * xc[at(C, K, J2, I)] + q * xc[at(C, K2, J2, I)])) +
+            q * (a * (a * xc[at(C, K, J, I2)] + q * xc[at(C, K2, J, I2)]) +
+                 q * (a * xc[at(C, K, J2, I2)] + q * xc[at(C, K2, J2, I2)]));
+        x[c] += v;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int sweeps, R w,
+                                                          const SolverScalars *sc) {
+    extern __shared__ __align__(16) unsigned char tail_lds[];
+    R *lds = reinterpret_cast<R *>(tail_lds);
+    if (stopped(sc)) return;  // the same value for every thread of the block
+    const int nl = A.nl;
+    // the 1-D coefficients of every tail level into LDS
+    for (int l = 0; l < nl; ++l) {
+        const Lv<R> &G = A.lv[l];
+        const int ni = G.ip + 2, nj = G.jp + 2, nk = G.kp + 2;
+        R *c = lds + A.coef[l];
+        for (int e = threadIdx.x; e < 2 * (ni + nj + nk); e += kTailThreads) {
+            R v;
+            if (e < ni) v = G.ae[e];
+            else if (e < 2 * ni) v = G.aw[e - ni];
+            else if (e < 2 * ni + nj) v = G.an[e - 2 * ni];
+            else if (e < 2 * ni + 2 * nj) v = G.as[e - 2 * ni - nj];
+            else if (e < 2 * ni + 2 * nj + nk) v = G.au[e - 2 * ni - 2 * nj];
+            else v = G.ad[e - 2 * ni - 2 * nj - nk];
+            c[e] = v;
+        }
+    }
+    __syncthreads();
+    const int last = nl - 1;
+    // down: the first tail level (right-hand side in global memory) ...
+    if (last > 0) {
+        const Lv<R> F = tail_level(A, lds, 0), C = tail_level(A, lds, 1);
+        R *t = lds + A.to[0], *x = lds + A.xo[0];
+        tail_smooth0(F, A.b0, t, w);
+        __syncthreads();
+        tail_smooth(F, A.b0, (const R *)t, x, w);
+        __syncthreads();
+        tail_restrict(F, C, A.b0, (const R *)x, lds + A.bo[1]);
+        __syncthreads();
+    }
+    // ... and the levels below it, all in LDS
+    for (int l = 1; l < last; ++l) {
+        const Lv<R> F = tail_level(A, lds, l), C = tail_level(A, lds, l + 1);
+        const R *b = lds + A.bo[l];
+        R *t = lds + A.to[l], *x = lds + A.xo[l];
+        tail_smooth0(F, b, t, w);
+        __syncthreads();
+        tail_smooth(F, b, (const R *)t, x, w);
+        __syncthreads();
+        tail_restrict(F, C, b, (const R *)x, lds + A.bo[l + 1]);
+        __syncthreads();
+    }
+    // the coarsest level: the smoother from zero, `sweeps` times, ping-pong
+    // between its x and t (at most kCoarseMax cells: one per thread)
+    R *res;
+    {
+        const Lv<R> L = tail_level(A, lds, last);
+        const int c = threadIdx.x;
+        const bool act = c < L.n;
+        int k = 1, j = 1, i = 1;
+        R ap = R(-1), bc = R(0), inv = R(0);
+        R *xa = lds + A.xo[last], *xb = lds + A.to[last];
+        if (A.Mt) {
+            // x = M b, b staged in LDS (the t array of the level)
+            if (act) xb[c] = last > 0 ? lds[A.bo[last] + c] : A.b0[c];
+            __syncthreads();
+            if (act) xa[c] = dense_row(A.Mt, L.n, (const R *)xb, c);
+            __syncthreads();
+            res = xa;
+        } else {
+            if (act) {
+                cell_of(L, c, k, j, i);
+                ap = ap_of(L, k, j, i);
+                inv = inv_of<R>(ap);
+                bc = last > 0 ? lds[A.bo[last] + c] : A.b0[c];
+                xa[c] = w * bc * inv;
+            }
+            __syncthreads();
+            R *cur = xa, *nxt = xb;
+            for (int s = 1; s < sweeps; ++s) {
+                if (act)
+                    nxt[c] = cur[c] + w * (bc - ax_of(L, (const R *)cur, k, j, i, ap)) * inv;
+                __syncthreads();
+                R *tmp = cur;
+                cur = nxt;
+                nxt = tmp;
+            }
+            res = cur;
+        }
+    }
+    // up: prolongation and two sweeps per level, in LDS ...
+    const R *xc = res;
+    for (int l = last - 1; l >= 1; --l) {
+        const Lv<R> F = tail_level(A, lds, l), C = tail_level(A, lds, l + 1);
+        const R *b = lds + A.bo[l];
+        R *t = lds + A.to[l], *x = lds + A.xo[l];
+        tail_prolong(F, C, xc, x);
+        __syncthreads();
+        tail_smooth(F, b, (const R *)x, t, w);
+        __syncthreads();
+        tail_smooth(F, b, (const R *)t, x, w);
+        __syncthreads();
+        xc = x;
+    }
+    // ... then the first tail level, whose result goes to global memory
+    if (last > 0) {
+        const Lv<R> F = tail_level(A, lds, 0), C = tail_level(A, lds, 1);
+        R *t = lds + A.to[0], *x = lds + A.xo[0];
+        tail_prolong(F, C, xc, x);
+        __syncthreads();
+        tail_smooth(F, A.b0, (const R *)x, t, w);
+        __syncthreads();
+        tail_smooth(F, A.b0, (const R *)t, A.x0, w);
+    } else {
+        const int c = threadIdx.x;
+        if (c < A.lv[0].n) A.x0[c] = res[c];
+    }
+}
+
 }  // namespace
 
 // The hierarchy: level 0 is the inner solve's grid (its vectors are passed
@@ -187,6 +451,10 @@ template <typename R> struct MgLevels {
     Lv<R> lv[16];
     R *b[16] = {}, *x[16] = {}, *t[16] = {};
     R *coef = nullptr;  // the coarse levels' 1-D coefficients
+    R *Mt = nullptr;    // the coarsest level's dense solve, transposed (or null)
+    int tail = 0;       // first level of the one-block tail (0: none)
+    size_t tail_lds = 0;
+    TailArgs<R> targs{};
 };
 struct MgHier {
     int nlev = 0;
@@ -204,6 +472,7 @@ template <typename R> static void free_levels(MgLevels<R> &V, int nlev) {
         (void)hipFree(V.t[l]);
     }
     (void)hipFree(V.coef);
+    (void)hipFree(V.Mt);
 }
 
 void mg_release(dcm_ctx *c) {
@@ -234,14 +503,72 @@ bool mg_possible(const Geo &g) {
     return mg_levels(g.ip, g.jp, g.kp, d);
 }
 
+// The first level of the one-block tail: the lowest l >= 1 whose levels
+// l..nlev-1 (x and t each, b below the first, the six 1-D coefficient arrays,
+// every array padded to 16 bytes) fit the LDS of one workgroup, with at most
+// kTailMax levels and at most 2 cells per thread on the first (a level of
+// 16 cells per thread was measured slower in one block than as six
+// launches: each pass is a chain of dependent loads per cell).
+template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
+    V.tail = 0;
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess)
+        return DCM_HIP_ERROR;
+    const size_t pad = 16 / sizeof(R);
+    auto up = [&](size_t n) { return (n + pad - 1) / pad * pad; };
+    for (int t0 = 1; t0 < nlev; ++t0) {
+        const int nl = nlev - t0;
+        if (nl > kTailMax || V.lv[t0].n > 2 * kTailThreads) continue;
+        TailArgs<R> A{};
+        size_t off = 0;
+        for (int l = 0; l < nl; ++l) {
+            const Lv<R> &L = V.lv[t0 + l];
+            A.lv[l] = L;
+            A.coef[l] = (int)off;
+            off += up(2 * (size_t)(L.ip + L.jp + L.kp + 6));
+            A.xo[l] = (int)off;
+            off += up((size_t)L.n);
+            A.to[l] = (int)off;
+            off += up((size_t)L.n);
+            A.bo[l] = (int)off;
+            if (l > 0) off += up((size_t)L.n);
+        }
+        const size_t bytes = off * sizeof(R);
+        if (bytes > (size_t)lds_max) continue;
+        A.nl = nl;
+        A.b0 = V.b[t0];
+        A.x0 = V.x[t0];
+        A.Mt = V.Mt;
+        if (hipFuncSetAttribute((const void *)k_mg_tail<R>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes) != hipSuccess)
+            return DCM_HIP_ERROR;
+        V.tail = t0;
+        V.tail_lds = bytes;
+        V.targs = A;
+        return DCM_OK;
+    }
+    return DCM_OK;  // no level fits: every pass in its own launch
+}
+
 // the device levels of one precision: coarse coefficients from h (the
 // doubles of every coarse level's six 1-D arrays at off), level 0 from the
 // context's own coefficients
 template <typename R>
 static int build_levels(MgLevels<R> &V, MgHier *H, const std::vector<int> &d,
                         const std::vector<double> &h, const std::vector<size_t> &off,
-                        const R *ae, const R *aw, const R *an, const R *as, const R *au,
-                        const R *ad) {
+                        const std::vector<double> &minv, const R *ae, const R *aw,
+                        const R *an, const R *as, const R *au, const R *ad) {
+    if (!minv.empty()) {
+        std::vector<R> mr(minv.begin(), minv.end());
+        if (hipMalloc((void **)&V.Mt, mr.size() * sizeof(R)) != hipSuccess)
+            return DCM_OUT_OF_MEMORY;
+        if (hipMemcpy(V.Mt, mr.data(), mr.size() * sizeof(R), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return DCM_HIP_ERROR;
+    }
     std::vector<R> hr(h.begin(), h.end());
     if (!hr.empty()) {
         if (hipMalloc((void **)&V.coef, hr.size() * sizeof(R)) != hipSuccess)
@@ -263,7 +590,64 @@ static int build_levels(MgLevels<R> &V, MgHier *H, const std::vector<int> &d,
             if (hipMalloc((void **)p, (size_t)L.n * sizeof(R)) != hipSuccess)
                 return DCM_OUT_OF_MEMORY;
     }
-    return DCM_OK;
+    return plan_tail(V, H->nlev);
+}
+
+// M^T for M = (A + e e^T)^-1 of the coarsest level (n cells, coefficients
+// f[0..5] = ae, aw, an, as, au, ad, 1-based), by Gauss-Jordan elimination
+// with partial pivoting in FP64; empty when a pivot vanishes.  A is the
+// operator of ax_of: periodic in i and j, the Neumann copy in k.
+static std::vector<double> coarse_inverse_t(int ip, int jp, int kp,
+                                            const std::vector<double> *f) {
+    const int n = ip * jp * kp;
+    if (n > kDenseMax) return {};
+    auto at_ = [&](int k, int j, int i) { return ((i - 1) * jp + (j - 1)) * kp + (k - 1); };
+    // [A + e e^T | I], row-major
+    std::vector<double> a((size_t)n * 2 * n, 0.0);
+    auto A = [&](int r, int col) -> double & { return a[(size_t)r * 2 * n + col]; };
+    for (int i = 1; i <= ip; ++i)
+        for (int j = 1; j <= jp; ++j)
+            for (int k = 1; k <= kp; ++k) {
+                const int r = at_(k, j, i);
+                const int iw = i == 1 ? ip : i - 1, ie = i == ip ? 1 : i + 1;
+                const int js = j == 1 ? jp : j - 1, jn = j == jp ? 1 : j + 1;
+                const int kd = k == 1 ? 1 : k - 1, ku = k == kp ? kp : k + 1;
+                const double ap = (((((-f[0][i]) - f[1][i]) - f[2][j]) - f[3][j]) - f[4][k]) -
+                                  f[5][k];
+                A(r, at_(k, j, iw)) += f[1][i];
+                A(r, at_(k, j, ie)) += f[0][i];
+                A(r, at_(k, js, i)) += f[3][j];
+                A(r, at_(k, jn, i)) += f[2][j];
+                A(r, at_(kd, j, i)) += f[5][k];
+                A(r, at_(ku, j, i)) += f[4][k];
+                A(r, r) += ap;
+            }
+    double scale = 0.0;
+    for (int r = 0; r < n; ++r) scale = std::max(scale, std::fabs(A(r, r)));
+    // the rank-one term at the operator's scale keeps the matrix well scaled
+    for (int r = 0; r < n; ++r)
+        for (int col = 0; col < n; ++col) A(r, col) += scale / n;
+    for (int r = 0; r < n; ++r) A(r, n + r) = 1.0;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(A(r, col)) > std::fabs(A(piv, col))) piv = r;
+        if (!(std::fabs(A(piv, col)) > 1e-12 * scale)) return {};
+        if (piv != col)
+            for (int q = 0; q < 2 * n; ++q) std::swap(A(piv, q), A(col, q));
+        const double inv = 1.0 / A(col, col);
+        for (int q = 0; q < 2 * n; ++q) A(col, q) *= inv;
+        for (int r = 0; r < n; ++r) {
+            if (r == col) continue;
+            const double m = A(r, col);
+            if (m == 0.0) continue;
+            for (int q = 0; q < 2 * n; ++q) A(r, q) -= m * A(col, q);
+        }
+    }
+    std::vector<double> mt((size_t)n * n);
+    for (int r = 0; r < n; ++r)
+        for (int col = 0; col < n; ++col) mt[(size_t)col * n + r] = A(r, n + col);
+    return mt;
 }
 
 int mg_prepare(dcm_ctx *c) {
@@ -311,10 +695,12 @@ int mg_prepare(dcm_ctx *c) {
             f[a] = nx;
         }
     }
-    int st = build_levels<float>(H->f32, H, d, h, off, c->met32.ae, c->met32.aw, c->met32.an,
-                                 c->met32.as, c->met32.au, c->met32.ad);
+    const int lc = H->nlev - 1;
+    const std::vector<double> minv = coarse_inverse_t(d[3 * lc], d[3 * lc + 1], d[3 * lc + 2], f);
+    int st = build_levels<float>(H->f32, H, d, h, off, minv, c->met32.ae, c->met32.aw,
+                                 c->met32.an, c->met32.as, c->met32.au, c->met32.ad);
     if (st == DCM_OK)
-        st = build_levels<double>(H->f64, H, d, h, off, c->met.ae, c->met.aw, c->met.an,
+        st = build_levels<double>(H->f64, H, d, h, off, minv, c->met.ae, c->met.aw, c->met.an,
                                   c->met.as, c->met.au, c->met.ad);
     c->mg = H;
     if (st != DCM_OK) {
@@ -338,32 +724,43 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
     }
     MgLevels<R> &V = levels_of<R>(H);
     const int L = H->nlev;
-    auto blocks = [](int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); };
     const R w = (R)kMgOmega;
-    hipLaunchKernelGGL(k_mg_restrict<R>, blocks(V.lv[1].n), dim3(kBlock), 0, c->stream, b0,
-                       (const R *)x0, V.b[1], V.lv[0], V.lv[1], sc);
-    for (int l = 1; l < L - 1; ++l) {
+    // levels from `top` down are the tail's (the coarsest alone without it)
+    const bool tail = c->mg_tail && V.tail > 0;
+    const int top = tail ? V.tail : L - 1;
+    Grid3 G = grid3(V.lv[1]);
+    hipLaunchKernelGGL(k_mg_restrict<R>, G.grid, G.block, 0, c->stream, b0, (const R *)x0,
+                       V.b[1], V.lv[0], V.lv[1], sc);
+    for (int l = 1; l < top; ++l) {
         const Lv<R> &F = V.lv[l], &C = V.lv[l + 1];
-        hipLaunchKernelGGL(k_mg_smooth<R>, blocks(F.n), dim3(kBlock), 0, c->stream,
+        const Grid3 GF = grid3(F), GC = grid3(C);
+        hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)nullptr, V.t[l], F, w, sc);
-        hipLaunchKernelGGL(k_mg_smooth<R>, blocks(F.n), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.t[l], V.x[l], F, w, sc);
-        hipLaunchKernelGGL(k_mg_restrict<R>, blocks(C.n), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_mg_restrict<R>, GC.grid, GC.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.x[l], V.b[l + 1], F, C, sc);
     }
-    hipLaunchKernelGGL(k_mg_coarse<R>, dim3(1), dim3(kCoarseMax), 0, c->stream,
-                       (const R *)V.b[L - 1], V.x[L - 1], V.lv[L - 1], kCoarseSweeps, w, sc);
-    for (int l = L - 2; l >= 1; --l) {
+    if (tail)
+        hipLaunchKernelGGL(k_mg_tail<R>, dim3(1), dim3(kTailThreads), V.tail_lds, c->stream,
+                           V.targs, kCoarseSweeps, w, sc);
+    else
+        hipLaunchKernelGGL(k_mg_coarse<R>, dim3(1), dim3(kCoarseMax), 0, c->stream,
+                           (const R *)V.b[L - 1], V.x[L - 1], V.lv[L - 1], kCoarseSweeps, w, sc,
+                           (const R *)V.Mt);
+    for (int l = top - 1; l >= 1; --l) {
         const Lv<R> &F = V.lv[l], &C = V.lv[l + 1];
-        hipLaunchKernelGGL(k_mg_prolong<R>, blocks(F.n), dim3(kBlock), 0, c->stream,
+        const Grid3 GF = grid3(F);
+        hipLaunchKernelGGL(k_mg_prolong<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.x[l + 1], V.x[l], F, C, sc);
-        hipLaunchKernelGGL(k_mg_smooth<R>, blocks(F.n), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.x[l], V.t[l], F, w, sc);
-        hipLaunchKernelGGL(k_mg_smooth<R>, blocks(F.n), dim3(kBlock), 0, c->stream,
+        hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.t[l], V.x[l], F, w, sc);
     }
-    hipLaunchKernelGGL(k_mg_prolong<R>, blocks(V.lv[0].n), dim3(kBlock), 0, c->stream,
-                       (const R *)V.x[1], x0, V.lv[0], V.lv[1], sc);
+    G = grid3(V.lv[0]);
+    hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, (const R *)V.x[1], x0,
+                       V.lv[0], V.lv[1], sc);
     DCM_LAUNCH_CHECK();
     return DCM_OK;
 }
@@ -381,6 +778,7 @@ int dcm_set_precond_multigrid(dcm_ctx *c, int on) {
     if (on) {
         DCM_CHECK(mg_prepare(c));
         c->mg_on = true;
+        c->mg_tail = on != 2;
     } else {
         c->mg_on = false;
     }

@@ -410,9 +410,12 @@ class Context:
         _call("dcm_set_sum_order", self.h, order)
 
     def set_precond_multigrid(self, on=True):
-        """One multigrid V-cycle in place of the Jacobi sweeps in the mixed
-        path's FP32 inner solve (single models and dcm_bicgstab_mixed)."""
-        _call("dcm_set_precond_multigrid", self.h, int(bool(on)))
+        """One multigrid V-cycle in place of the Jacobi sweeps (single models,
+        dcm_bicgstab_mixed, and the FP64 BiCGSTAB).  on=2: the same cycle
+        with every pass of the coarse levels in its own launch (the default,
+        True or 1, runs the levels that fit one workgroup's LDS as one
+        single-block kernel; the bits are the same)."""
+        _call("dcm_set_precond_multigrid", self.h, 2 if on == 2 else int(bool(on)))
 
     def set_mpdata_variant(self, variant):
         """MPDATA_LINKED (pamm/sfvt/advection_3d.F90, the default) or

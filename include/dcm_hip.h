@@ -129,7 +129,11 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
  * (each member is then solved with its own V-cycle, one after another, and
  * evolves bit for bit as when stepped alone) or none has it (one batched
  * solve with the Jacobi/Chebyshev sweeps); dcm_ensemble_step returns
- * DCM_BAD_ARG for a mix.  No counterpart in the reference. */
+ * DCM_BAD_ARG for a mix.  on = 1 runs the levels that fit the LDS of one
+ * workgroup (with every level below them) as one single-block kernel per
+ * V-cycle; on = 2 launches each pass of those levels on its own (the same
+ * arithmetic and bits; kept for comparison).  No counterpart in the
+ * reference. */
 int dcm_set_precond_multigrid(dcm_ctx *ctx, int on);
 #define DCM_MPDATA_LINKED 0
 #define DCM_MPDATA_SFVT 1

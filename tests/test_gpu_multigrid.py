@@ -244,3 +244,30 @@ def test_fp64_solve_with_the_v_cycle_matches_the_reference_sweeps():
         if f == H.F_P:
             a, b = a - a.mean(), b - b.mean()
         assert rel_l2(a, b) < 1e-7, f
+
+
+@pytest.mark.parametrize("precision", [H.MIXED, H.FP64])
+def test_one_block_tail_matches_per_pass_launches(precision):
+    """The levels that fit one workgroup's LDS run as one single-block kernel
+    (at 64x64x32: levels 1-3 in FP32, 2-3 in FP64); with
+    dcm_set_precond_multigrid(ctx, 2) every pass of them is its own launch.
+    Same arithmetic, so the same bits after every step."""
+    case = W.moist_bubble(64, 64, 32, delta=50.0, dt=0.5, steps=4, ntracer=1, seed=41)
+    snaps = {}
+    for mode in (1, 2):
+        with H.Context(case.dom) as c:
+            c.set_precond_multigrid(mode)
+            m = H.Model(c, case, precision)
+            try:
+                out = []
+                for step in range(4):
+                    st = m.step(1)
+                    out.append((st.iters_total, {f: m.get(f) for f in
+                                                 (H.F_U, H.F_W, H.F_TH, H.F_P)}))
+                snaps[mode] = out
+            finally:
+                m.close()
+    for step, ((ia, a), (ib, b)) in enumerate(zip(snaps[1], snaps[2])):
+        assert ia == ib, (step, ia, ib)
+        for f in a:
+            np.testing.assert_array_equal(a[f], b[f], err_msg=f"field {f} after step {step}")

@@ -12,7 +12,8 @@
 // decides convergence either way.
 //
 // Cell-centred hierarchy on the compact (k fastest) layout of the inner
-// solve, every dimension halved per level down to at most 1024 cells:
+// solve, every dimension halved per level down to at most 1024 cells, and on
+// while every dimension stays even, down to 64 cells or fewer:
 //   smoother     damped Jacobi x <- x + w (b - A x)/a_p, two sweeps before
 //                and two after the coarse correction (the first from zero);
 //                on level 0 these are the solver's own sweep kernels, the
@@ -58,6 +59,8 @@ constexpr int kCoarseSweeps = 96;  // smoother iterations on the coarsest level
 constexpr int kCoarseMax = 1024;   // cells of the coarsest level (one block)
 constexpr int kDenseMax = 512;     // coarsest levels solved by the dense inverse
                                    // (O(n^3) host work at set-up)
+constexpr int kDenseTarget = 64;   // the hierarchy halves on toward this size
+constexpr int kFuseMax = 1 << 17;  // largest level whose legs run as one kernel each
 
 template <typename R> struct Lv {
     int ip, jp, kp, n;
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
 // row c of the dense coarsest solve: sum over r of M[c][r] b[r], M stored
 // transposed (Mt[r n + c]) so that the threads of a block read whole rows
 template <typename R, typename PB>
-__device__ __forceinline__ R dense_row(const R *__restrict__ Mt, int n, PB b, int c) {
+__device__ __forceinline__ R dense_row(const R *Mt, int n, PB b, int c) {
     R acc = R(0);
 #pragma unroll 8
     for (int r = 0; r < n; ++r) acc += Mt[(size_t)r * n + c] * b[r];
@@ -231,6 +234,195 @@ __global__ __launch_bounds__(kCoarseMax) void k_mg_coarse(const R *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// A level's down-leg and up-leg in one kernel each (temporal blocking).  A
+// block owns a TI x TJ tile of whole k-columns and keeps the tile with a
+// halo of two columns on each side in LDS (box coordinates ii, jj; global
+// indices wrap periodically), so the passes that follow one another need no
+// grid-wide barrier: the halo cells are computed redundantly.
+//   down: t = w b/a_p on the box, x = t + w (b - A t)/a_p on the box less
+//         one column, x written on the tile, and the restriction of
+//         (b - A x) into the coarse cells of the tile;
+//   up:   x' = x + P xc on the box, t = x' + w (b - A x')/a_p on the box
+//         less one column, x'' = t + w (b - A t)/a_p on the tile, written to
+//         a second array (other blocks still read x in their halos).
+// Threads: x along k (blockDim.x = kp), y over the box's columns.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrap1(int v, int n) {
+    // 1-based periodic index of v (v may lie a few cells outside 1..n)
+    int r = (v - 1) % n;
+    if (r < 0) r += n;
+    return r + 1;
+}
+
+struct Tile2 {
+    int TI, TJ;  // tile size in i and j (each a divisor of the level's ip, jp)
+};
+
+// The box's 1-D coefficients: ae, aw at the BI box columns' i (wrapped), an,
+// as at the BJ box rows' j, in LDS after the three arrays; au, ad of the
+// thread's k in registers.
+template <typename R> struct BoxCoef {
+    const R *ae, *aw, *an, *as;
+    R au, ad;
+};
+template <typename R>
+__device__ __forceinline__ BoxCoef<R> box_coef(const Lv<R> &F, R *tab, int ib, int jb, int BI,
+                                               int BJ, int k) {
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const int nt = (int)(blockDim.x * blockDim.y);
+    for (int e = tid; e < 2 * (BI + BJ); e += nt) {
+        R v;
+        if (e < BI) v = F.ae[wrap1(ib + e, F.ip)];
+        else if (e < 2 * BI) v = F.aw[wrap1(ib + e - BI, F.ip)];
+        else if (e < 2 * BI + BJ) v = F.an[wrap1(jb + e - 2 * BI, F.jp)];
+        else v = F.as[wrap1(jb + e - 2 * BI - BJ, F.jp)];
+        tab[e] = v;
+    }
+    return BoxCoef<R>{tab, tab + BI, tab + 2 * BI, tab + 2 * BI + BJ, F.au[k], F.ad[k]};
+}
+// a_p at box column ii, row jj: the order of ap_of
+template <typename R>
+__device__ __forceinline__ R box_ap(const BoxCoef<R> &B, int ii, int jj) {
+    return (((((-B.ae[ii]) - B.aw[ii]) - B.an[jj]) - B.as[jj]) - B.au) - B.ad;
+}
+// A v at box element e (plane = BJ*kp, row = kp): the operator of ax_of
+template <typename R>
+__device__ __forceinline__ R ax_box(const BoxCoef<R> &B, const R *v, int e, int plane, int kp,
+                                    int k, int ii, int jj, R ap) {
+    const int kd = k == 1 ? 0 : 1, ku = k == kp ? 0 : 1;
+    return B.aw[ii] * v[e - plane] + B.ae[ii] * v[e + plane] + B.as[jj] * v[e - kp] +
+           B.an[jj] * v[e + kp] + B.ad * v[e - kd] + B.au * v[e + ku] + ap * v[e];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_mg_down(const R *__restrict__ b, R *__restrict__ xg,
+                                                 R *__restrict__ bc, Lv<R> F, Lv<R> C,
+                                                 Tile2 T, R w, const SolverScalars *sc) {
+    extern __shared__ __align__(16) unsigned char mg_lds[];
+    if (stopped(sc)) return;
+    const int kp = F.kp, BI = T.TI + 4, BJ = T.TJ + 4;
+    const int plane = BJ * kp, box = BI * plane;
+    R *lb = reinterpret_cast<R *>(mg_lds), *lt = lb + box, *lx = lt + box;
+    const int ntj = F.jp / T.TJ;
+    const int ti = (int)blockIdx.x / ntj, tj = (int)blockIdx.x - ti * ntj;
+    const int ib = ti * T.TI - 1, jb = tj * T.TJ - 1;  // global i, j of box column (0, 0)
+    const int k = (int)threadIdx.x + 1;
+    const BoxCoef<R> B = box_coef(F, lx + box, ib, jb, BI, BJ, k);
+    // b of the box
+#pragma unroll 4
+    for (int col = threadIdx.y; col < BI * BJ; col += blockDim.y) {
+        const int ii = col / BJ, jj = col - ii * BJ;
+        const int i = wrap1(ib + ii, F.ip), j = wrap1(jb + jj, F.jp);
+        lb[col * kp + (k - 1)] = b[at(F, k, j, i)];
+    }
+    __syncthreads();
+    // t = w b/a_p on the whole box
+    for (int col = threadIdx.y; col < BI * BJ; col += blockDim.y) {
+        const int ii = col / BJ, jj = col - ii * BJ;
+        const int e = col * kp + (k - 1);
+        lt[e] = w * lb[e] * inv_of<R>(box_ap(B, ii, jj));
+    }
+    __syncthreads();
+    // x = t + w (b - A t)/a_p on the box less one column
+    for (int col = threadIdx.y; col < (BI - 2) * (BJ - 2); col += blockDim.y) {
+        const int ii = 1 + col / (BJ - 2), jj = 1 + col % (BJ - 2);
+        const int e = (ii * BJ + jj) * kp + (k - 1);
+        const R ap = box_ap(B, ii, jj);
+        const R inv = inv_of<R>(ap);
+        lx[e] = lt[e] + w * (lb[e] - ax_box(B, (const R *)lt, e, plane, kp, k, ii, jj, ap)) * inv;
+    }
+    __syncthreads();
+    // x on the tile
+    for (int col = threadIdx.y; col < T.TI * T.TJ; col += blockDim.y) {
+        const int ii = 2 + col / T.TJ, jj = 2 + col % T.TJ;
+        xg[at(F, k, jb + jj, ib + ii)] = lx[(ii * BJ + jj) * kp + (k - 1)];
+    }
+    // the coarse cells of the tile: mean over the eight children of (b - A x)
+    // (the children's k differ from the thread's: their au, ad from F)
+    if (k <= C.kp) {
+        const int K = k;
+        for (int col = threadIdx.y; col < (T.TI / 2) * (T.TJ / 2); col += blockDim.y) {
+            const int ci = col / (T.TJ / 2), cj = col % (T.TJ / 2);
+            const int I = ti * (T.TI / 2) + ci + 1, J = tj * (T.TJ / 2) + cj + 1;
+            R acc = R(0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int kk = 2 * K - 1 + (q & 1);
+                const int ii = 2 * ci + 2 + ((q >> 2) & 1), jj = 2 * cj + 2 + ((q >> 1) & 1);
+                const int e = (ii * BJ + jj) * kp + (kk - 1);
+                BoxCoef<R> Bq = B;
+                Bq.au = F.au[kk];
+                Bq.ad = F.ad[kk];
+                const R ap = box_ap(Bq, ii, jj);
+                acc += lb[e] - ax_box(Bq, (const R *)lx, e, plane, kp, kk, ii, jj, ap);
+            }
+            bc[at(C, K, J, I)] = R(0.125) * acc;
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_mg_up(const R *__restrict__ xc,
+                                               const R *__restrict__ xg,
+                                               const R *__restrict__ b, R *__restrict__ xo,
+                                               Lv<R> F, Lv<R> C, Tile2 T, R w,
+                                               const SolverScalars *sc) {
+    extern __shared__ __align__(16) unsigned char mg_lds[];
+    if (stopped(sc)) return;
+    const int kp = F.kp, BI = T.TI + 4, BJ = T.TJ + 4;
+    const int plane = BJ * kp, box = BI * plane;
+    R *lb = reinterpret_cast<R *>(mg_lds), *lt = lb + box, *lx = lt + box;
+    const int ntj = F.jp / T.TJ;
+    const int ti = (int)blockIdx.x / ntj, tj = (int)blockIdx.x - ti * ntj;
+    const int ib = ti * T.TI - 1, jb = tj * T.TJ - 1;
+    const int k = (int)threadIdx.x + 1;
+    const BoxCoef<R> B = box_coef(F, lx + box, ib, jb, BI, BJ, k);
+    // x' = x + P xc on the whole box (trilinear, as k_mg_prolong), b staged
+    const int K = (k + 1) >> 1;
+    int K2 = (k & 1) ? K - 1 : K + 1;
+    K2 = K2 < 1 ? 1 : (K2 > C.kp ? C.kp : K2);
+#pragma unroll 2
+    for (int col = threadIdx.y; col < BI * BJ; col += blockDim.y) {
+        const int ii = col / BJ, jj = col - ii * BJ;
+        const int i = wrap1(ib + ii, F.ip), j = wrap1(jb + jj, F.jp);
+        const int e = col * kp + (k - 1);
+        const int I = (i + 1) >> 1, J = (j + 1) >> 1;
+        int I2 = (i & 1) ? I - 1 : I + 1;
+        int J2 = (j & 1) ? J - 1 : J + 1;
+        I2 = I2 < 1 ? C.ip : (I2 > C.ip ? 1 : I2);
+        J2 = J2 < 1 ? C.jp : (J2 > C.jp ? 1 : J2);
+        const R a = R(0.75), q = R(0.25);
+        const R v =
+            a * (a * (a * xc[at(C, K, J, I)] + q * xc[at(C, K2, J, I)]) +
+                 q * (a * xc[at(C, K, J2, I)] + q * xc[at(C, K2, J2, I)])) +
+            q * (a * (a * xc[at(C, K, J, I2)] + q * xc[at(C, K2, J, I2)]) +
+                 q * (a * xc[at(C, K, J2, I2)] + q * xc[at(C, K2, J2, I2)]));
+        const int c = at(F, k, j, i);
+        lx[e] = xg[c] + v;
+        lb[e] = b[c];
+    }
+    __syncthreads();
+    // t = x' + w (b - A x')/a_p on the box less one column
+    for (int col = threadIdx.y; col < (BI - 2) * (BJ - 2); col += blockDim.y) {
+        const int ii = 1 + col / (BJ - 2), jj = 1 + col % (BJ - 2);
+        const int e = (ii * BJ + jj) * kp + (k - 1);
+        const R ap = box_ap(B, ii, jj);
+        const R inv = inv_of<R>(ap);
+        lt[e] = lx[e] + w * (lb[e] - ax_box(B, (const R *)lx, e, plane, kp, k, ii, jj, ap)) * inv;
+    }
+    __syncthreads();
+    // x'' = t + w (b - A t)/a_p on the tile
+    for (int col = threadIdx.y; col < T.TI * T.TJ; col += blockDim.y) {
+        const int ii = 2 + col / T.TJ, jj = 2 + col % T.TJ;
+        const int e = (ii * BJ + jj) * kp + (k - 1);
+        const R ap = box_ap(B, ii, jj);
+        const R inv = inv_of<R>(ap);
+        xo[at(F, k, jb + jj, ib + ii)] =
+            lt[e] + w * (lb[e] - ax_box(B, (const R *)lt, e, plane, kp, k, ii, jj, ap)) * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The tail of the V-cycle in one block: levels t0..t0+nl-1 of the hierarchy
 // (the last the coarsest).  Level t0's right-hand side comes from global
 // memory (the restriction kernel of the level above writes it) and its
@@ -248,6 +440,7 @@ template <typename R> struct TailArgs {
     const R *b0;         // right-hand side of the first tail level
     R *x0;               // its result
     const R *Mt;         // the coarsest level's dense solve (null: the sweeps)
+    int mo;              // its LDS offset
 };
 
 // level l of the tail with its coefficient arrays in LDS
@@ -350,17 +543,24 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
             c[e] = v;
         }
     }
+    // the first level's right-hand side and the dense coarsest solve
+    for (int e = threadIdx.x; e < A.lv[0].n; e += kTailThreads) lds[A.bo[0] + e] = A.b0[e];
+    if (A.Mt) {
+        const int nc = A.lv[nl - 1].n;
+        for (int e = threadIdx.x; e < nc * nc; e += kTailThreads) lds[A.mo + e] = A.Mt[e];
+    }
     __syncthreads();
     const int last = nl - 1;
+    const R *b0 = lds + A.bo[0];
     // down: the first tail level (right-hand side in global memory) ...
     if (last > 0) {
         const Lv<R> F = tail_level(A, lds, 0), C = tail_level(A, lds, 1);
         R *t = lds + A.to[0], *x = lds + A.xo[0];
-        tail_smooth0(F, A.b0, t, w);
+        tail_smooth0(F, b0, t, w);
         __syncthreads();
-        tail_smooth(F, A.b0, (const R *)t, x, w);
+        tail_smooth(F, b0, (const R *)t, x, w);
         __syncthreads();
-        tail_restrict(F, C, A.b0, (const R *)x, lds + A.bo[1]);
+        tail_restrict(F, C, b0, (const R *)x, lds + A.bo[1]);
         __syncthreads();
     }
     // ... and the levels below it, all in LDS
@@ -386,10 +586,8 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
         R ap = R(-1), bc = R(0), inv = R(0);
         R *xa = lds + A.xo[last], *xb = lds + A.to[last];
         if (A.Mt) {
-            // x = M b, b staged in LDS (the t array of the level)
-            if (act) xb[c] = last > 0 ? lds[A.bo[last] + c] : A.b0[c];
-            __syncthreads();
-            if (act) xa[c] = dense_row(A.Mt, L.n, (const R *)xb, c);
+            // x = M b, both in LDS
+            if (act) xa[c] = dense_row((const R *)(lds + A.mo), L.n, (const R *)(lds + A.bo[last]), c);
             __syncthreads();
             res = xa;
         } else {
@@ -397,7 +595,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
                 cell_of(L, c, k, j, i);
                 ap = ap_of(L, k, j, i);
                 inv = inv_of<R>(ap);
-                bc = last > 0 ? lds[A.bo[last] + c] : A.b0[c];
+                bc = lds[A.bo[last] + c];
                 xa[c] = w * bc * inv;
             }
             __syncthreads();
@@ -433,9 +631,9 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
         R *t = lds + A.to[0], *x = lds + A.xo[0];
         tail_prolong(F, C, xc, x);
         __syncthreads();
-        tail_smooth(F, A.b0, (const R *)x, t, w);
+        tail_smooth(F, b0, (const R *)x, t, w);
         __syncthreads();
-        tail_smooth(F, A.b0, (const R *)t, A.x0, w);
+        tail_smooth(F, b0, (const R *)t, A.x0, w);
     } else {
         const int c = threadIdx.x;
         if (c < A.lv[0].n) A.x0[c] = res[c];
@@ -452,6 +650,8 @@ template <typename R> struct MgLevels {
     R *b[16] = {}, *x[16] = {}, *t[16] = {};
     R *coef = nullptr;  // the coarse levels' 1-D coefficients
     R *Mt = nullptr;    // the coarsest level's dense solve, transposed (or null)
+    Tile2 tile[16] = {};  // temporal-blocking tile of each level (TI = 0: none)
+    size_t tile_lds[16] = {};
     int tail = 0;       // first level of the one-block tail (0: none)
     size_t tail_lds = 0;
     TailArgs<R> targs{};
@@ -495,7 +695,15 @@ static bool mg_levels(int ip, int jp, int kp, std::vector<int> &dims) {
         kp /= 2;
         dims.insert(dims.end(), {ip, jp, kp});
     }
-    return dims.size() >= 6;  // at least one coarse level
+    if (dims.size() < 6) return false;  // at least one coarse level
+    // further down while it halves, so that the dense coarsest solve is small
+    while ((long long)ip * jp * kp > kDenseTarget && ip % 2 == 0 && jp % 2 == 0 && kp % 2 == 0) {
+        ip /= 2;
+        jp /= 2;
+        kp /= 2;
+        dims.insert(dims.end(), {ip, jp, kp});
+    }
+    return true;
 }
 
 bool mg_possible(const Geo &g) {
@@ -504,8 +712,9 @@ bool mg_possible(const Geo &g) {
 }
 
 // The first level of the one-block tail: the lowest l >= 1 whose levels
-// l..nlev-1 (x and t each, b below the first, the six 1-D coefficient arrays,
-// every array padded to 16 bytes) fit the LDS of one workgroup, with at most
+// l..nlev-1 (x, t and b each, the six 1-D coefficient arrays, the dense
+// coarsest solve, every array padded to 16 bytes) fit the LDS of one
+// workgroup, with at most
 // kTailMax levels and at most 2 cells per thread on the first (a level of
 // 16 cells per thread was measured slower in one block than as six
 // launches: each pass is a chain of dependent loads per cell).
@@ -533,8 +742,10 @@ template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
             A.to[l] = (int)off;
             off += up((size_t)L.n);
             A.bo[l] = (int)off;
-            if (l > 0) off += up((size_t)L.n);
+            off += up((size_t)L.n);
         }
+        A.mo = (int)off;
+        if (V.Mt) off += up((size_t)V.lv[nlev - 1].n * V.lv[nlev - 1].n);
         const size_t bytes = off * sizeof(R);
         if (bytes > (size_t)lds_max) continue;
         A.nl = nl;
@@ -551,6 +762,40 @@ template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
         return DCM_OK;
     }
     return DCM_OK;  // no level fits: every pass in its own launch
+}
+
+// Tiles of the levels above the tail: 4x4 (else 2x2) columns whose box with
+// its halo (3 arrays of (TI+4)(TJ+4)kp and the box's 1-D coefficients) fits
+// 64 KiB of LDS, kp <= 256, for levels of at most kFuseMax cells (above
+// that the halo work costs more than the launches it saves: 128x128x64 in
+// FP32 took 28 us per fused leg against 19 us for its three kernels).
+template <typename R> static int plan_tiles(MgLevels<R> &V, int nlev) {
+    size_t most = 0;
+    for (int l = 1; l < nlev - 1; ++l) {
+        const Lv<R> &L = V.lv[l];
+        V.tile[l] = Tile2{0, 0};
+        if (L.kp > 256 || L.n > kFuseMax) continue;
+        for (int t : {4, 2}) {
+            const size_t bytes =
+                (3 * (size_t)(t + 4) * (t + 4) * L.kp + 4 * (size_t)(t + 4)) * sizeof(R);
+            if (L.ip % t == 0 && L.jp % t == 0 && bytes <= 64 * 1024) {
+                V.tile[l] = Tile2{t, t};
+                V.tile_lds[l] = bytes;
+                most = std::max(most, bytes);
+                break;
+            }
+        }
+    }
+    if (most > 0) {
+        if (hipFuncSetAttribute((const void *)k_mg_down<R>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)most) != hipSuccess ||
+            hipFuncSetAttribute((const void *)k_mg_up<R>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)most) != hipSuccess)
+            return DCM_HIP_ERROR;
+    }
+    return DCM_OK;
 }
 
 // the device levels of one precision: coarse coefficients from h (the
@@ -590,7 +835,8 @@ static int build_levels(MgLevels<R> &V, MgHier *H, const std::vector<int> &d,
             if (hipMalloc((void **)p, (size_t)L.n * sizeof(R)) != hipSuccess)
                 return DCM_OUT_OF_MEMORY;
     }
-    return plan_tail(V, H->nlev);
+    DCM_CHECK(plan_tail(V, H->nlev));
+    return plan_tiles(V, H->nlev);
 }
 
 // M^T for M = (A + e e^T)^-1 of the coarsest level (n cells, coefficients
@@ -731,8 +977,19 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
     Grid3 G = grid3(V.lv[1]);
     hipLaunchKernelGGL(k_mg_restrict<R>, G.grid, G.block, 0, c->stream, b0, (const R *)x0,
                        V.b[1], V.lv[0], V.lv[1], sc);
+    // a level's legs as one kernel each where it has a tile (mode 1); the
+    // up-leg then leaves the level's result in t
+    auto fused = [&](int l) { return c->mg_tail && V.tile[l].TI > 0; };
     for (int l = 1; l < top; ++l) {
         const Lv<R> &F = V.lv[l], &C = V.lv[l + 1];
+        if (fused(l)) {
+            const Tile2 T = V.tile[l];
+            const dim3 blk((unsigned)F.kp, (unsigned)(256 / F.kp));
+            const unsigned nb = (unsigned)((F.ip / T.TI) * (F.jp / T.TJ));
+            hipLaunchKernelGGL(k_mg_down<R>, dim3(nb), blk, V.tile_lds[l], c->stream,
+                               (const R *)V.b[l], V.x[l], V.b[l + 1], F, C, T, w, sc);
+            continue;
+        }
         const Grid3 GF = grid3(F), GC = grid3(C);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)nullptr, V.t[l], F, w, sc);
@@ -748,19 +1005,30 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
         hipLaunchKernelGGL(k_mg_coarse<R>, dim3(1), dim3(kCoarseMax), 0, c->stream,
                            (const R *)V.b[L - 1], V.x[L - 1], V.lv[L - 1], kCoarseSweeps, w, sc,
                            (const R *)V.Mt);
+    const R *below = V.x[top];  // the result of the level below
     for (int l = top - 1; l >= 1; --l) {
         const Lv<R> &F = V.lv[l], &C = V.lv[l + 1];
+        if (fused(l)) {
+            const Tile2 T = V.tile[l];
+            const dim3 blk((unsigned)F.kp, (unsigned)(256 / F.kp));
+            const unsigned nb = (unsigned)((F.ip / T.TI) * (F.jp / T.TJ));
+            hipLaunchKernelGGL(k_mg_up<R>, dim3(nb), blk, V.tile_lds[l], c->stream, below,
+                               (const R *)V.x[l], (const R *)V.b[l], V.t[l], F, C, T, w, sc);
+            below = V.t[l];
+            continue;
+        }
         const Grid3 GF = grid3(F);
-        hipLaunchKernelGGL(k_mg_prolong<R>, GF.grid, GF.block, 0, c->stream,
-                           (const R *)V.x[l + 1], V.x[l], F, C, sc);
+        hipLaunchKernelGGL(k_mg_prolong<R>, GF.grid, GF.block, 0, c->stream, below, V.x[l], F,
+                           C, sc);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.x[l], V.t[l], F, w, sc);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.t[l], V.x[l], F, w, sc);
+        below = V.x[l];
     }
     G = grid3(V.lv[0]);
-    hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, (const R *)V.x[1], x0,
-                       V.lv[0], V.lv[1], sc);
+    hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, below, x0, V.lv[0],
+                       V.lv[1], sc);
     DCM_LAUNCH_CHECK();
     return DCM_OK;
 }

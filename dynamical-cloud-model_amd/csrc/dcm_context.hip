@@ -457,6 +457,16 @@ int dcm_context_create(int ip, int jp, int kp, const double *dx, const double *d
         return DCM_OUT_OF_MEMORY;
     }
     c->h_solver = (SolverScalars *)p;
+    if (hipMalloc(&p, sizeof(OuterState)) != hipSuccess) {
+        set_error("hipMalloc failed");
+        return DCM_OUT_OF_MEMORY;
+    }
+    c->outer_d = (OuterState *)p;
+    if (hipHostMalloc(&p, sizeof(OuterState), hipHostMallocDefault) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        return DCM_OUT_OF_MEMORY;
+    }
+    c->outer_h = (OuterState *)p;
     if (hipDeviceSynchronize() != hipSuccess) {
         set_error("device initialisation failed");
         return DCM_HIP_ERROR;
@@ -495,12 +505,15 @@ int dcm_context_destroy(dcm_ctx *c) {
     (void)hipHostFree(c->hscal);
     (void)hipFree(c->solver);
     (void)hipHostFree(c->h_solver);
+    (void)hipFree(c->outer_d);
+    if (c->outer_h) (void)hipHostFree(c->outer_h);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->iter_graph) (void)hipGraphExecDestroy(c->iter_graph);
+    if (c->chunk_graph) (void)hipGraphExecDestroy(c->chunk_graph);
     c->solve_tables.release();
     c->flags.release();
     if (c->h_flags) (void)hipHostFree(c->h_flags);

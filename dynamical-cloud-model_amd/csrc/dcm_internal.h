@@ -101,6 +101,21 @@ struct SolverScalars {
     double tol;        // convergence test of this solve (err < tol)
 };
 
+// State of the mixed path's FP64 refinement kept on the device, so that a
+// refinement step (correction, true residual, decision, next inner start)
+// can follow an inner iteration without a host round trip.
+struct OuterState {
+    double sc_err, err_prev, err, rtol;
+    int first;        // no residual evaluated yet
+    int active;       // an inner solve is under way
+    int go;           // this transition runs (the inner solve has finished)
+    int outer;        // inner solves started
+    int status;       // DCM_NONFINITE / DCM_NOT_CONVERGED, else 0
+    int fallback;     // the last correction did not reduce the true residual
+    int total_iters;  // inner iterations of the finished inner solves
+    int pad;
+};
+
 // Per-kernel timing (profiling mode): accumulated over launches.
 struct KernelTimer {
     double total_ms = 0.0;
@@ -221,6 +236,13 @@ struct dcm_ctx {
     // launch arguments it was captured with (the key's bytes) are unchanged
     std::vector<char> iter_key;
     hipGraphExec_t iter_graph = nullptr;
+    // the single-system multigrid mixed solve: one inner iteration and the
+    // predicated refinement step after it, as one graph; the refinement
+    // state on the device and its pinned host copy
+    std::vector<char> chunk_key;
+    hipGraphExec_t chunk_graph = nullptr;
+    dcm::OuterState *outer_d = nullptr;
+    dcm::OuterState *outer_h = nullptr;
     // graphs on (dcm_set_graphs): the single-system mixed solve replays its
     // inner iteration from a graph
     bool graphs = true;

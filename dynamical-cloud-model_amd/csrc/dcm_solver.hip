@@ -43,6 +43,7 @@
 #include <cfloat>
 #include <cstddef>
 #include <cmath>
+#include <functional>
 #include <mutex>
 
 #include <cstring>
@@ -316,7 +317,9 @@ __global__ __launch_bounds__(256) void k_resid0(const double *__restrict__ x,
 __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x,
                                                     const double *__restrict__ b, double *r64,
                                                     Geo pg, CGeo g, CoefPack cp,
-                                                    double *partials, long long stride) {
+                                                    double *partials, long long stride,
+                                                    const OuterState *gate = nullptr) {
+    if (gate && !gate->go) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 1;
@@ -900,8 +903,9 @@ __global__ __launch_bounds__(256) void k_update_b(IterArgs A) {
 // inner-solve start and generic sums
 // ---------------------------------------------------------------------------
 // inner solve start: residuals measured relative to ||r_inner,0||
-__global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *sc,
-                                 double rtol) {
+// the scalars of a fresh inner solve from <r,r> (in the partial sums)
+__device__ __forceinline__ void fin_inner_init_body(const double *partials, int nb,
+                                                    SolverScalars *sc, double rtol) {
     const double rr = reduce_slot_1024(partials, nb);
     if (threadIdx.x == 0) {
         sc->sc_err = rr > 0.0 ? sqrt(rr) : 1.0;
@@ -922,9 +926,15 @@ __global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *
         sc->done = (rr == 0.0) || (sc->err < rtol) || sc->nonfinite;
     }
 }
+__global__ void k_fin_inner_init(const double *partials, int nb, SolverScalars *sc,
+                                 double rtol) {
+    fin_inner_init_body(partials, nb, sc, rtol);
+}
 
 // three generic sums into dst[0..2]
-__global__ void k_finalize_sums_3(const double *partials, int nb, double *dst) {
+__global__ void k_finalize_sums_3(const double *partials, int nb, double *dst,
+                                  const OuterState *gate = nullptr) {
+    if (gate && !gate->go) return;
     const double a = reduce_slot_1024(partials, nb);
     const double b = reduce_slot_1024(partials + nb, nb);
     const double d = reduce_slot_1024(partials + 2 * (long long)nb, nb);
@@ -972,11 +982,102 @@ __global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g) {
     x[pidx(pg, k, j, i)] += (double)d[cidx(g, k, j, i)];
 }
 
+// ---------------------------------------------------------------------------
+// The refinement step on the device (the single-system multigrid mixed solve;
+// the host loop of bicgstab_mixed_batch, one decision at a time)
+// ---------------------------------------------------------------------------
+__global__ void k_outer_start(OuterState *os) {
+    if (threadIdx.x != 0) return;
+    os->sc_err = 0.0;
+    os->err_prev = 0.0;
+    os->err = 0.0;
+    os->rtol = 0.0;
+    os->first = 1;
+    os->active = 1;
+    os->go = 1;
+    os->outer = 0;
+    os->status = 0;
+    os->fallback = 0;
+    os->total_iters = 0;
+}
+// the step runs when an inner solve is under way and has finished
+// (converged, stagnated or broken down), or when the host forces it
+__global__ void k_outer_gate(OuterState *os, const SolverScalars *sc, int force) {
+    if (threadIdx.x != 0) return;
+    os->go = os->active && (force || halted(sc));
+}
+// x += d with the previous x kept (the interior; the halo is not touched
+// until the solve ends)
+__global__ void k_add_correction_save(double *x, double *xsave, const float *d, Geo pg, CGeo g,
+                                      const OuterState *gate) {
+    if (!gate->go) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(pg, k, j, i);
+    const double xv = x[c];
+    xsave[c] = xv;
+    x[c] = xv + (double)d[cidx(g, k, j, i)];
+}
+__global__ void k_restore_interior(double *x, const double *xsave, Geo pg, CGeo g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(pg, k, j, i);
+    x[c] = xsave[c];
+}
+// the decision of the host loop from the true residual's sums
+// (rr = sums[0], bb = sums[2]): stop (converged, fallback, failure) or start
+// the next inner solve with its tolerance
+__global__ void k_outer_decide(const double *sums, OuterState *os, SolverScalars *sc, double tol,
+                               double inner_rtol, int itmax, int max_outer) {
+    if (threadIdx.x != 0 || !os->go) return;
+    const double rr = sums[0], bb = sums[2];
+    if (os->first) {
+        double s = sqrt(bb);
+        if (s < 0.01) s = 0.01;
+        os->sc_err = s;
+    } else {
+        os->total_iters += sc->iters;  // the inner solve that just finished
+    }
+    const double err = sqrt(rr) / os->sc_err;
+    os->err = err;
+    int active = 0;
+    if (os->first && !isfinite(err)) {
+        os->status = DCM_NONFINITE;
+    } else if (!os->first && !(err < os->err_prev)) {
+        os->fallback = 1;
+    } else if (err < tol) {
+        // converged
+    } else if (os->outer == max_outer || os->total_iters >= itmax) {
+        os->status = DCM_NOT_CONVERGED;
+    } else {
+        active = 1;
+        os->err_prev = err;
+        os->outer += 1;
+        double rtol = inner_rtol;
+        const double need_rel = 0.5 * tol * os->sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
+        if (need_rel > rtol) rtol = need_rel;
+        os->rtol = rtol;
+    }
+    os->first = 0;
+    os->active = active;
+    if (!active) sc->done = 1;  // the inner kernels that follow skip
+}
+__global__ void k_fin_inner_init_dev(const double *partials, int nb, SolverScalars *sc,
+                                     const OuterState *gate) {
+    if (!(gate->go && gate->active)) return;
+    fin_inner_init_body(partials, nb, sc, gate->rtol);
+}
+
 // inner start: r = float(r64 - mean) (the mean removed in FP64, so that a
 // large mean does not swamp the rest in FP32), cr = r, p = v1 = d = 0; <r,r>
 __global__ void k_inner_init(const double *r64, float *r, float *cr, float *p, float *v1,
                              float *d, CGeo g, const double *mean_sum, double *partials,
-                             long long stride) {
+                             long long stride, const OuterState *gate = nullptr) {
+    if (gate && !(gate->go && gate->active)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 1;
@@ -1593,10 +1694,169 @@ int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, i
     return st;
 }
 
+// The single-system multigrid mixed solve with the refinement step on the
+// device.  After the start (true residual, decision, first inner start),
+// each chunk is one inner iteration followed by the refinement step
+// (correction, true residual, decision, next inner start), every kernel of
+// the step gated on the inner solve having finished; the chunk is replayed
+// from one graph and the host polls the refinement state once per chunk
+// instead of after the residual, after every iteration and after every
+// inner solve.  The kernels and their order are those of the host loop of
+// bicgstab_mixed_batch, so the bits are the same.
+static int chunk_graph(dcm_ctx *c, const SolvePlan &P, const IterLaunch<float> &L,
+                       const std::vector<const void *> &ptrs, const double *scal, int nscal,
+                       const std::function<void()> &step) {
+    std::vector<char> key(sizeof(IterMember) + ptrs.size() * sizeof(void *) +
+                          (size_t)nscal * sizeof(double) + 2 * sizeof(unsigned));
+    char *k = key.data();
+    std::memcpy(k, &P.mem[0], sizeof(IterMember));
+    k += sizeof(IterMember);
+    std::memcpy(k, ptrs.data(), ptrs.size() * sizeof(void *));
+    k += ptrs.size() * sizeof(void *);
+    std::memcpy(k, scal, (size_t)nscal * sizeof(double));
+    k += (size_t)nscal * sizeof(double);
+    std::memcpy(k, &P.mgrid, sizeof(unsigned));
+    std::memcpy(k + sizeof(unsigned), &P.ugrid, sizeof(unsigned));
+    if (c->chunk_graph && key == c->chunk_key) return DCM_OK;
+    if (c->chunk_graph) (void)hipGraphExecDestroy(c->chunk_graph);
+    c->chunk_graph = nullptr;
+    c->chunk_key.clear();
+    DCM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    L.iteration();
+    step();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (ec != hipSuccess) {
+        set_error(std::string("capturing the inner iteration: ") + hipGetErrorString(ec));
+        return DCM_HIP_ERROR;
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->chunk_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->chunk_graph = nullptr;
+        set_error(std::string("instantiating the inner iteration: ") + hipGetErrorString(ei));
+        return DCM_HIP_ERROR;
+    }
+    c->chunk_key = std::move(key);
+    return DCM_OK;
+}
+
+static int mixed_one_chunked(dcm_ctx *c, const SolveTask &t, double tol, int itmax,
+                             double inner_rtol, SolverOut *out, int *status) {
+    const Geo &pg = c->geo;
+    const CGeo g = cgeo(pg);
+    const Interior IL = interior_launch(pg);
+    const int nb = (int)(IL.grid.x * IL.grid.y * IL.grid.z);
+    const int max_outer = 50;
+    const int inner_cap = 2000;  // FP32 iterations per refinement sweep
+    DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
+    DCM_CHECK(ensure_solver_vectors32(c, 10));
+    const size_t cells = (size_t)pg.ip * pg.jp * pg.kp;
+    float *V[10];
+    for (int v = 0; v < 10; ++v) V[v] = c->sv32 + (size_t)v * cells;
+    double *r64 = nullptr, *xsave = nullptr;
+    DCM_CHECK(work_field64(c, 2, &r64));
+    DCM_CHECK(work_field64(c, 1, &xsave));
+    *out = SolverOut();
+    *status = DCM_OK;
+    SolveSys sy;
+    sy.c = c;
+    for (int v = 0; v < 10; ++v) sy.V[v] = V[v];
+    sy.march_planes = c->march_planes > 0 ? c->march_planes : kMarchPlanes;
+    SolvePlan P;
+    DCM_CHECK(plan_solve(c->stream, &sy, 1, P, c->solve_tables));
+    const IterLaunch<float> L{c, P, 1, c->seq_sums ? 1 : 0};
+    OuterState *os = c->outer_d;
+    const OuterState *gos = os;
+    SolverScalars *sc = c->solver;
+    const CoefPack cp{c->met, c->met32};
+    double *sums = c->dscal + 40;
+    // true residual, decision, inner start (each gated on os->go)
+    auto decide_and_start = [&]() {
+        hipLaunchKernelGGL(k_resid_true, IL.grid, IL.block, 0, c->stream, (const double *)t.x,
+                           (const double *)t.b, r64, pg, g, cp, c->partials, (long long)nb, gos);
+        hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, c->stream,
+                           (const double *)c->partials, nb, sums, gos);
+        hipLaunchKernelGGL(k_outer_decide, dim3(1), dim3(64), 0, c->stream, (const double *)sums,
+                           os, sc, tol, inner_rtol, itmax, max_outer);
+        hipLaunchKernelGGL(k_inner_init, IL.grid, IL.block, 0, c->stream, (const double *)r64,
+                           V[0], V[1], V[2], V[3], V[9], g, (const double *)(sums + 1),
+                           c->partials, (long long)nb, gos);
+        hipLaunchKernelGGL(k_fin_inner_init_dev, dim3(1), dim3(1024), 0, c->stream,
+                           (const double *)c->partials, nb, sc, gos);
+    };
+    // the refinement step after an inner iteration
+    auto step = [&](int force) {
+        hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, os,
+                           (const SolverScalars *)sc, force);
+        hipLaunchKernelGGL(k_add_correction_save, IL.grid, IL.block, 0, c->stream, t.x, xsave,
+                           (const float *)V[9], pg, g, gos);
+        decide_and_start();
+    };
+    hipLaunchKernelGGL(k_outer_start, dim3(1), dim3(64), 0, c->stream, os);
+    decide_and_start();
+    DCM_LAUNCH_CHECK();
+    const std::vector<const void *> ptrs = {P.args.mem, c->mg, c->partials, t.x, t.b,
+                                            r64, xsave, os, sc, c->dscal, c->sv32};
+    const double scal[3] = {tol, inner_rtol, (double)itmax};
+    DCM_CHECK(chunk_graph(c, P, L, ptrs, scal, 3, [&]() { step(0); }));
+    // the start leaves inner solve 1 under way (or nothing to do)
+    int seen_outer = 1, launched = 0, cap = std::min(inner_cap, itmax);
+    OuterState h{};
+    for (;;) {
+        if (launched >= cap) {
+            step(1);  // the inner solve has used its iterations: correct anyway
+            DCM_LAUNCH_CHECK();
+        } else {
+            DCM_HIP(hipGraphLaunch(c->chunk_graph, c->stream));
+            ++launched;
+        }
+        DCM_HIP(hipMemcpyAsync(c->outer_h, os, sizeof(OuterState), hipMemcpyDeviceToHost,
+                               c->stream));
+        DCM_HIP(hipStreamSynchronize(c->stream));
+        h = *c->outer_h;
+        if (!h.active) break;
+        if (h.outer != seen_outer) {  // a new inner solve has started
+            seen_outer = h.outer;
+            launched = 0;
+            cap = std::max(1, std::min(inner_cap, itmax - h.total_iters));
+        }
+    }
+    out->iters = h.total_iters;
+    out->outer = h.outer;
+    out->err = h.err;
+    if (h.status == DCM_NONFINITE) set_error("bicgstab_mixed: non-finite initial residual");
+    *status = h.status;
+    if (h.fallback) {
+        // the last FP32 correction did not reduce the true residual: restore
+        // x and let the reference's FP64 solver finish
+        hipLaunchKernelGGL(k_restore_interior, IL.grid, IL.block, 0, c->stream, t.x,
+                           (const double *)xsave, pg, g);
+        DCM_LAUNCH_CHECK();
+        SolverOut o;
+        int st = DCM_OK;
+        (void)bicgstab64_batch(c, &t, 1, tol, itmax, &o, &st);
+        out->iters += o.iters;
+        out->err = o.err;
+        out->fallback = 1;
+        *status = st;
+        return st;
+    }
+    if (*status == DCM_OK) {
+        DCM_CHECK(fill<double>(c, t.x, FILL_ALONG_DIM_WO, 0.0, 0.0));
+        hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, c->stream, t.x, t.sw, c->geo,
+                           c->h_dzn[0], 1);
+        DCM_LAUNCH_CHECK();
+    }
+    return *status;
+}
+
 // Mixed precision for a batch: FP32 inner BiCGSTAB (all members in one
 // batched loop) inside each member's FP64 iterative refinement.
 int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, int itmax,
                          double inner_rtol, SolverOut *outs, int *status) {
+    if (n == 1 && tasks[0].c == c && c->mg_on && c->mg && c->graphs && !c->profiling)
+        return mixed_one_chunked(c, tasks[0], tol, itmax, inner_rtol, outs, status);
     struct Mem {
         double sc_err = 0.0, err_prev = 0.0;
         double *r64 = nullptr, *xsave = nullptr;

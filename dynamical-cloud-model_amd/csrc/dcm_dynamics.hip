@@ -79,11 +79,12 @@ __global__ __launch_bounds__(256) void k_sources(const double *__restrict__ u,
     // plane i0-1: the values a cell takes from its west neighbour
     double u_w = u[c - SI], un_w = u[c - SI + SJ], uu_w = u[c - SI + 1];
     double v_w = v[c - SI], w_w = w[c - SI];
-    SrcPlane P, N;
+    SrcPlane P, N, NN;
     src_load(P, u, v, w, th, c, SJ);
+    src_load(N, u, v, w, th, c + SI, SJ);  // plane i0+1 <= ip+1: inside the padded array
     for (int i = i0; i <= i1; ++i) {
-        // plane i+1 (always inside the padded array: i + 1 <= ip + 1)
-        src_load(N, u, v, w, th, c + SI, SJ);
+        // the loads of plane i+2 in flight while plane i is computed
+        if (i < i1) src_load(NN, u, v, w, th, c + 2 * SI, SJ);
         {  // su: :743-757
             const double X = (u_w * (P.u + u_w) - N.u * (P.u + N.u)) / (2.0 * (dx[i] + dx[i - 1]));
             const double Y = (P.us * (P.vs + N.vs) - P.un * (P.v + N.v)) / dY_u;
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(256) void k_sources(const double *__restrict__ u,
         v_w = P.v;
         w_w = P.w;
         P = N;
+        N = NN;
         c += SI;
     }
 }

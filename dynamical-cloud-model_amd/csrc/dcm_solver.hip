@@ -339,15 +339,26 @@ __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x
         const int dD = k == 1 ? 0 : -1, dU = k == g.kp ? 0 : 1;
         const double as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
         const int col = j * SJ + k;
-        const int iw = i0 == 1 ? g.ip : i0 - 1;
-        double xw = x[iw * SI + col], xc = x[i0 * SI + col];
         const int CS = g.kp * g.jp, CJ = g.kp;  // compact strides
+        const int iw = i0 == 1 ? g.ip : i0 - 1;
+        int ie = i0 == g.ip ? 1 : i0 + 1;
+        int c = i0 * SI + col;
+        double xw = x[iw * SI + col], xc = x[c], xe = x[ie * SI + col];
+        double xs = x[c + dS], xn = x[c + dN], xd = x[c + dD], xu = x[c + dU], bb = b[c];
         for (int i = i0; i <= i1; ++i) {
-            const int ie = i == g.ip ? 1 : i + 1;
-            const int c = i * SI + col;
-            const double xe = x[ie * SI + col];
-            const double xs = x[c + dS], xn = x[c + dN], xd = x[c + dD], xu = x[c + dU];
-            const double bb = b[c];
+            // the loads of plane i+1 in flight while plane i is computed
+            double nxe = 0.0, nxs = 0.0, nxn = 0.0, nxd = 0.0, nxu = 0.0, nbb = 0.0;
+            const int ce = ie * SI + col;
+            if (i < i1) {
+                const int nie = ie == g.ip ? 1 : ie + 1;
+                nxe = x[nie * SI + col];
+                nxs = x[ce + dS];
+                nxn = x[ce + dN];
+                nxd = x[ce + dD];
+                nxu = x[ce + dU];
+                nbb = b[ce];
+                ie = nie;
+            }
             const double ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
             const double ax =
                 stencil<double>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu, xc);
@@ -358,6 +369,13 @@ __global__ __launch_bounds__(256) void k_resid_true(const double *__restrict__ x
             s[2] += bb * bb;
             xw = xc;
             xc = xe;
+            xe = nxe;
+            xs = nxs;
+            xn = nxn;
+            xd = nxd;
+            xu = nxu;
+            bb = nbb;
+            c = ce;
         }
     }
     block_sum_store<3>(s, partials, stride);
@@ -1041,12 +1059,17 @@ __global__ void k_add_correction_save(double *x, double *xsave, const float *d, 
     if (!gate->go) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 1;
+    const int i0 = blockIdx.z * kResPlanes + 1;
     if (k > g.kp || j > g.jp) return;
-    const long long c = pidx(pg, k, j, i);
-    const double xv = x[c];
-    xsave[c] = xv;
-    x[c] = xv + (double)d[cidx(g, k, j, i)];
+    const int i1 = min(i0 + kResPlanes - 1, g.ip);
+    const int SI = (int)pg.si, CS = g.kp * g.jp;
+    int c = i0 * SI + j * (int)pg.sj + k;
+    int q = (i0 - 1) * CS + (j - 1) * g.kp + (k - 1);
+    for (int i = i0; i <= i1; ++i, c += SI, q += CS) {
+        const double xv = x[c];
+        xsave[c] = xv;
+        x[c] = xv + (double)d[q];
+    }
 }
 __global__ void k_restore_interior(double *x, const double *xsave, Geo pg, CGeo g) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -1108,18 +1131,22 @@ __global__ void k_inner_init(const double *r64, float *r, float *cr, float *p, f
     if (gate && !(gate->go && gate->active)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 1;
+    const int i0 = blockIdx.z * kResPlanes + 1;
     double s[2] = {0.0, 0.0};
     if (k <= g.kp && j <= g.jp) {
-        const long long c = cidx(g, k, j, i);
+        const int i1 = min(i0 + kResPlanes - 1, g.ip);
         const double mean = *mean_sum / (double)g.n;
-        const float rv = (float)(r64[c] - mean);
-        r[c] = rv;
-        cr[c] = rv;
-        p[c] = 0.f;
-        v1[c] = 0.f;
-        d[c] = 0.f;
-        s[0] = (double)rv * (double)rv;
+        const int CS = g.kp * g.jp;
+        int c = (i0 - 1) * CS + (j - 1) * g.kp + (k - 1);
+        for (int i = i0; i <= i1; ++i, c += CS) {
+            const float rv = (float)(r64[c] - mean);
+            r[c] = rv;
+            cr[c] = rv;
+            p[c] = 0.f;
+            v1[c] = 0.f;
+            d[c] = 0.f;
+            s[0] += (double)rv * (double)rv;
+        }
     }
     block_sum_store<2>(s, partials, stride);
 }
@@ -1809,17 +1836,17 @@ static int mixed_one_chunked(dcm_ctx *c, const SolveTask &t, double tol, int itm
                            (const double *)c->partials, nbr, sums, gos);
         hipLaunchKernelGGL(k_outer_decide, dim3(1), dim3(64), 0, c->stream, (const double *)sums,
                            os, sc, tol, inner_rtol, itmax, max_outer);
-        hipLaunchKernelGGL(k_inner_init, IL.grid, IL.block, 0, c->stream, (const double *)r64,
+        hipLaunchKernelGGL(k_inner_init, RL.grid, RL.block, 0, c->stream, (const double *)r64,
                            V[0], V[1], V[2], V[3], V[9], g, (const double *)(sums + 1),
-                           c->partials, (long long)nb, gos);
+                           c->partials, (long long)nbr, gos);
         hipLaunchKernelGGL(k_fin_inner_init_dev, dim3(1), dim3(1024), 0, c->stream,
-                           (const double *)c->partials, nb, sc, gos);
+                           (const double *)c->partials, nbr, sc, gos);
     };
     // the refinement step after an inner iteration
     auto step = [&](int force) {
         hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, os,
                            (const SolverScalars *)sc, force);
-        hipLaunchKernelGGL(k_add_correction_save, IL.grid, IL.block, 0, c->stream, t.x, xsave,
+        hipLaunchKernelGGL(k_add_correction_save, RL.grid, RL.block, 0, c->stream, t.x, xsave,
                            (const float *)V[9], pg, g, gos);
         decide_and_start();
     };
@@ -1977,14 +2004,16 @@ int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, 
             outs[q].outer += 1;
             // inner solve A d = r - mean(r), to a relative reduction
             // inner_rtol, or as soon as the outer target is surely met
-            hipLaunchKernelGGL(k_inner_init, L.grid, L.block, 0, mc->stream, M[q].r64, M[q].V[0],
+            const Interior PL = resid_launch(pg);
+            const int npl = (int)(PL.grid.x * PL.grid.y * PL.grid.z);
+            hipLaunchKernelGGL(k_inner_init, PL.grid, PL.block, 0, mc->stream, M[q].r64, M[q].V[0],
                                M[q].V[1], M[q].V[2], M[q].V[3], M[q].V[9], g, mc->dscal + 41,
-                               mc->partials, (long long)nb);
+                               mc->partials, (long long)npl);
             double rtol = inner_rtol;
             const double need_rel = 0.5 * tol * M[q].sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
             if (need_rel > rtol) rtol = need_rel;
             hipLaunchKernelGGL(k_fin_inner_init, dim3(1), dim3(1024), 0, mc->stream,
-                               mc->partials, nb, mc->solver, rtol);
+                               mc->partials, npl, mc->solver, rtol);
             DCM_LAUNCH_CHECK();
             SolveSys sy;
             sy.c = mc;

@@ -184,6 +184,148 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
     x[c] += v;
 }
 
+// x += P xc on a level with an even ip, marching: a thread owns one (k,j)
+// column and walks kProPlanes fine planes, two per coarse plane, keeping
+// the four coarse values (K or K2) x (J or J2) of coarse planes I-1, I, I+1
+// in registers, so each coarse value is read once per column pair instead of
+// once per fine cell.  The same interpolation expression as k_mg_prolong.
+constexpr int kProPlanes = 8;
+template <typename R> struct Quad {
+    R kj, k2j, kj2, k2j2;
+};
+template <typename R>
+__device__ __forceinline__ Quad<R> quad_at(const R *__restrict__ xc, int plane, int o,
+                                           int dk, int dj) {
+    const R *p = xc + plane + o;
+    return Quad<R>{p[0], p[dk], p[dj], p[dk + dj]};
+}
+template <typename R>
+__device__ __forceinline__ R interp(const Quad<R> &A, const Quad<R> &B) {
+    const R a = R(0.75), q = R(0.25);
+    return a * (a * (a * A.kj + q * A.k2j) + q * (a * A.kj2 + q * A.k2j2)) +
+           q * (a * (a * B.kj + q * B.k2j) + q * (a * B.kj2 + q * B.k2j2));
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_mg_prolong_march(const R *__restrict__ xc,
+                                                          R *__restrict__ x, Lv<R> F, Lv<R> C,
+                                                          const SolverScalars *sc) {
+    if (stopped(sc)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (k > F.kp || j > F.jp) return;
+    const int i0 = blockIdx.z * kProPlanes + 1;  // odd
+    const int i1 = min(i0 + kProPlanes - 1, F.ip);
+    const int K = (k + 1) >> 1, J = (j + 1) >> 1;
+    int K2 = (k & 1) ? K - 1 : K + 1;
+    int J2 = (j & 1) ? J - 1 : J + 1;
+    J2 = J2 < 1 ? C.jp : (J2 > C.jp ? 1 : J2);
+    K2 = K2 < 1 ? 1 : (K2 > C.kp ? C.kp : K2);
+    const int o = (J - 1) * C.kp + (K - 1);  // (K,J) within a coarse plane
+    const int dk = K2 - K, dj = (J2 - J) * C.kp;
+    const int CP = C.jp * C.kp;
+    auto plane = [&](int I) { return ((I < 1 ? C.ip : (I > C.ip ? 1 : I)) - 1) * CP; };
+    int I = (i0 + 1) >> 1;
+    Quad<R> Qm = quad_at(xc, plane(I - 1), o, dk, dj);
+    Quad<R> Q0 = quad_at(xc, plane(I), o, dk, dj);
+    Quad<R> Qp = quad_at(xc, plane(I + 1), o, dk, dj);
+    const int FP = F.jp * F.kp;
+    int c = (i0 - 1) * FP + (j - 1) * F.kp + (k - 1);
+    for (int i = i0; i <= i1; i += 2) {
+        // the next coarse plane in flight while this pair is computed
+        Quad<R> Qn = Qp;
+        if (i + 2 <= i1) Qn = quad_at(xc, plane(I + 2), o, dk, dj);
+        x[c] += interp(Q0, Qm);  // fine plane 2I-1: the second coarse plane is I-1
+        if (i + 1 <= i1) x[c + FP] += interp(Q0, Qp);  // 2I: I+1
+        Qm = Q0;
+        Q0 = Qp;
+        Qp = Qn;
+        ++I;
+        c += 2 * FP;
+    }
+}
+
+// bc = mean over the eight children of (b - A x), marching: a thread owns
+// one coarse (K,J) column, i.e. four fine child columns, and walks
+// kProPlanes/2 coarse planes; x of the child columns at fine planes i-1 and
+// i+1 comes from the previous and next coarse planes (registers), so a
+// coarse cell costs 32 loads instead of 64.  The same arithmetic, in the
+// same child order, as k_mg_restrict.
+template <typename R> struct Kid {
+    R ae, aw, an, as, au, ad;
+};
+template <typename R>
+__device__ __forceinline__ R kid_res(const Kid<R> &q, R xw, R xe, R xs, R xn, R xd, R xu, R xc,
+                                     R bv) {
+    const R ap = (((((-q.ae) - q.aw) - q.an) - q.as) - q.au) - q.ad;
+    return bv - (q.aw * xw + q.ae * xe + q.as * xs + q.an * xn + q.ad * xd + q.au * xu + ap * xc);
+}
+template <typename R> struct Four {
+    R a, b, c, d;  // child columns (ka,ja), (kb,ja), (ka,jb), (kb,jb)
+};
+template <typename R>
+__global__ __launch_bounds__(256) void k_mg_restrict_march(const R *__restrict__ b,
+                                                           const R *__restrict__ x,
+                                                           R *__restrict__ bc, Lv<R> F, Lv<R> C,
+                                                           const SolverScalars *sc) {
+    if (stopped(sc)) return;
+    const int K = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (K > C.kp || J > C.jp) return;
+    const int I0 = blockIdx.z * (kProPlanes / 2) + 1;
+    const int I1 = min(I0 + kProPlanes / 2 - 1, C.ip);
+    const int ka = 2 * K - 1, kb = 2 * K, ja = 2 * J - 1, jb = 2 * J;
+    const int jaS = ja == 1 ? F.jp : ja - 1, jbN = jb == F.jp ? 1 : jb + 1;
+    const int kaD = ka == 1 ? 1 : ka - 1, kbU = kb == F.kp ? F.kp : kb + 1;
+    const int FP = F.jp * F.kp;
+    auto col = [&](int k, int j) { return (j - 1) * F.kp + (k - 1); };
+    const int cA = col(ka, ja), cB = col(kb, ja), cC = col(ka, jb), cD = col(kb, jb);
+    const int cS1 = col(ka, jaS), cS2 = col(kb, jaS), cN1 = col(ka, jbN), cN2 = col(kb, jbN);
+    const int cD1 = col(kaD, ja), cD2 = col(kaD, jb), cU1 = col(kbU, ja), cU2 = col(kbU, jb);
+    const R asa = F.as[ja], ana = F.an[ja], asb = F.as[jb], anb = F.an[jb];
+    const R ada = F.ad[ka], aua = F.au[ka], adb = F.ad[kb], aub = F.au[kb];
+    auto four = [&](int i) {
+        const R *p = x + (i - 1) * FP;
+        return Four<R>{p[cA], p[cB], p[cC], p[cD]};
+    };
+    auto wrapi = [&](int i) { return i < 1 ? F.ip : (i > F.ip ? 1 : i); };
+    // the child columns at fine plane 2*I0-2 (west of the first child plane)
+    Four<R> W = four(wrapi(2 * I0 - 2));
+    Four<R> A = four(2 * I0 - 1);
+    for (int I = I0; I <= I1; ++I) {
+        const int ia = 2 * I - 1, ib = 2 * I;
+        const Four<R> B = four(ib);
+        const Four<R> E = four(wrapi(ib + 1));
+        const R *pa = x + (ia - 1) * FP, *pb = x + (ib - 1) * FP;
+        const R *ba = b + (ia - 1) * FP, *bb = b + (ib - 1) * FP;
+        R acc = R(0);
+        {  // plane ia: west W, east B
+            const R ae = F.ae[ia], aw = F.aw[ia];
+            acc += kid_res(Kid<R>{ae, aw, ana, asa, aua, ada}, W.a, B.a, pa[cS1], A.c, pa[cD1],
+                           A.b, A.a, ba[cA]);
+            acc += kid_res(Kid<R>{ae, aw, ana, asa, aub, adb}, W.b, B.b, pa[cS2], A.d, A.a,
+                           pa[cU1], A.b, ba[cB]);
+            acc += kid_res(Kid<R>{ae, aw, anb, asb, aua, ada}, W.c, B.c, A.a, pa[cN1], pa[cD2],
+                           A.d, A.c, ba[cC]);
+            acc += kid_res(Kid<R>{ae, aw, anb, asb, aub, adb}, W.d, B.d, A.b, pa[cN2], A.c,
+                           pa[cU2], A.d, ba[cD]);
+        }
+        {  // plane ib: west A, east E
+            const R ae = F.ae[ib], aw = F.aw[ib];
+            acc += kid_res(Kid<R>{ae, aw, ana, asa, aua, ada}, A.a, E.a, pb[cS1], B.c, pb[cD1],
+                           B.b, B.a, bb[cA]);
+            acc += kid_res(Kid<R>{ae, aw, ana, asa, aub, adb}, A.b, E.b, pb[cS2], B.d, B.a,
+                           pb[cU1], B.b, bb[cB]);
+            acc += kid_res(Kid<R>{ae, aw, anb, asb, aua, ada}, A.c, E.c, B.a, pb[cN1], pb[cD2],
+                           B.d, B.c, bb[cC]);
+            acc += kid_res(Kid<R>{ae, aw, anb, asb, aub, adb}, A.d, E.d, B.b, pb[cN2], B.c,
+                           pb[cU2], B.d, bb[cD]);
+        }
+        bc[((I - 1) * C.jp + (J - 1)) * C.kp + (K - 1)] = R(0.125) * acc;
+        W = B;
+        A = E;
+    }
+}
+
 // row c of the dense coarsest solve: sum over r of M[c][r] b[r], M stored
 // transposed (Mt[r n + c]) so that the threads of a block read whole rows
 template <typename R, typename PB>
@@ -975,8 +1117,15 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
     const bool tail = c->mg_tail && V.tail > 0;
     const int top = tail ? V.tail : L - 1;
     Grid3 G = grid3(V.lv[1]);
-    hipLaunchKernelGGL(k_mg_restrict<R>, G.grid, G.block, 0, c->stream, b0, (const R *)x0,
-                       V.b[1], V.lv[0], V.lv[1], sc);
+    if (c->mg_tail) {  // (per-pass mode keeps the one-thread-per-cell kernel)
+        Grid3 GM = G;
+        GM.grid.z = (unsigned)((V.lv[1].ip + kProPlanes / 2 - 1) / (kProPlanes / 2));
+        hipLaunchKernelGGL(k_mg_restrict_march<R>, GM.grid, GM.block, 0, c->stream, b0,
+                           (const R *)x0, V.b[1], V.lv[0], V.lv[1], sc);
+    } else {
+        hipLaunchKernelGGL(k_mg_restrict<R>, G.grid, G.block, 0, c->stream, b0, (const R *)x0,
+                           V.b[1], V.lv[0], V.lv[1], sc);
+    }
     // a level's legs as one kernel each where it has a tile (mode 1); the
     // up-leg then leaves the level's result in t
     auto fused = [&](int l) { return c->mg_tail && V.tile[l].TI > 0; };
@@ -1027,8 +1176,14 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
         below = V.x[l];
     }
     G = grid3(V.lv[0]);
-    hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, below, x0, V.lv[0],
-                       V.lv[1], sc);
+    if (c->mg_tail) {  // (per-pass mode keeps the one-thread-per-cell kernel)
+        G.grid.z = (unsigned)((V.lv[0].ip + kProPlanes - 1) / kProPlanes);
+        hipLaunchKernelGGL(k_mg_prolong_march<R>, G.grid, G.block, 0, c->stream, below, x0,
+                           V.lv[0], V.lv[1], sc);
+    } else {
+        hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, below, x0, V.lv[0],
+                           V.lv[1], sc);
+    }
     DCM_LAUNCH_CHECK();
     return DCM_OK;
 }

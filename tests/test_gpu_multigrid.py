@@ -247,12 +247,16 @@ def test_fp64_solve_with_the_v_cycle_matches_the_reference_sweeps():
 
 
 @pytest.mark.parametrize("precision", [H.MIXED, H.FP64])
-def test_one_block_tail_matches_per_pass_launches(precision):
-    """The levels that fit one workgroup's LDS run as one single-block kernel
-    (at 64x64x32: levels 1-3 in FP32, 2-3 in FP64); with
-    dcm_set_precond_multigrid(ctx, 2) every pass of them is its own launch.
-    Same arithmetic, so the same bits after every step."""
-    case = W.moist_bubble(64, 64, 32, delta=50.0, dt=0.5, steps=4, ntracer=1, seed=41)
+@pytest.mark.parametrize("shape,steps", [((64, 64, 32), 4), ((256, 128, 64), 2)])
+def test_one_block_tail_matches_per_pass_launches(precision, shape, steps):
+    """The levels that fit one workgroup's LDS run as one single-block kernel,
+    the levels of at most 2^17 cells as one kernel per leg, and the
+    finest-level transfers as marching kernels; with
+    dcm_set_precond_multigrid(ctx, 2) every pass is its own one-thread-per-
+    cell launch.  64x64x32 has every coarse level fused or in the tail;
+    256x128x64 has a first coarse level of 2^18 cells that keeps its
+    per-pass kernels.  Same arithmetic, so the same bits after every step."""
+    case = W.moist_bubble(*shape, delta=50.0, dt=0.5, steps=steps, ntracer=1, seed=41)
     snaps = {}
     for mode in (1, 2):
         with H.Context(case.dom) as c:
@@ -260,7 +264,7 @@ def test_one_block_tail_matches_per_pass_launches(precision):
             m = H.Model(c, case, precision)
             try:
                 out = []
-                for step in range(4):
+                for step in range(steps):
                     st = m.step(1)
                     out.append((st.iters_total, {f: m.get(f) for f in
                                                  (H.F_U, H.F_W, H.F_TH, H.F_P)}))

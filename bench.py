@@ -173,11 +173,13 @@ def run_path(H, case, precision, warmup, steps, inner_rtol=None, precond="chebys
 
 
 def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
-    """The uneven warm-bubble ensemble two ways: every member's pressure
-    solve batched into one mixed-precision BiCGSTAB per step (after the
-    autotuner picks the solver tile; the stages around the solve replayed
-    from HIP graphs), and the members one after another on the plain FP64
-    path.  Cell-steps/s over the whole ensemble.
+    """The uneven warm-bubble ensemble three ways: every member's pressure
+    solve batched into one mixed-precision BiCGSTAB per step with the
+    Chebyshev-relaxed sweeps (after the autotuner picks the solver tile; the
+    stages around the solve replayed from HIP graphs); every member with its
+    own multigrid-preconditioned mixed solve, the members' solves running
+    concurrently on the ensemble's side streams; and the members one after
+    another on the plain FP64 path.  Cell-steps/s over the whole ensemble.
 
     With ``profile`` the batched Jacobi sweeps are timed with HIP events
     (which switches the graphs off) for the roofline: algorithmic bytes are
@@ -276,6 +278,8 @@ def run_ensemble(H, W, warmup, steps, members=64, seed=404, profile=False):
     if "multigrid_mixed" in out:
         out["speedup_multigrid_mixed_vs_sequential_fp64"] = (out["multigrid_mixed"]["value"] /
                                                              out["sequential_fp64"]["value"])
+        out["best_mixed"] = max(("batched_mixed", "multigrid_mixed"),
+                                key=lambda k: out[k]["value"])
     return out
 
 
@@ -637,27 +641,31 @@ def initialisation(H, W):
 
 
 def ensemble_line(H, W, args):
-    """One JSON line for the 64-member ensemble alone: value = the batched
-    mixed path's cell-steps/s (graphs on), the roofline from a second, event
-    -timed run of the same steps."""
+    """One JSON line for the 64-member ensemble alone: value = the faster
+    mixed path's cell-steps/s (graphs on: the multigrid members' concurrent
+    solves or the batched Chebyshev solve), the roofline of the batched
+    sweep from a second, event-timed run of the same steps."""
     print(f"[bench] ensemble: 64 uneven members, warmup {args.warmup}, timed {args.steps} steps",
           file=sys.stderr, flush=True)
     ens = run_ensemble(H, W, args.warmup, args.steps)
     prof = run_ensemble(H, W, args.warmup, args.steps, profile=True)["batched_mixed"]["kernel"]
-    b = ens["batched_mixed"]
-    print(f"[bench] ensemble: batched mixed {b['value']:.4g}, sequential FP64 "
-          f"{ens['sequential_fp64']['value']:.4g} cell-steps/s", file=sys.stderr, flush=True)
+    best = ens.get("best_mixed", "batched_mixed")
+    b = ens[best]
+    print(f"[bench] ensemble: batched mixed {ens['batched_mixed']['value']:.4g}, multigrid "
+          f"mixed {ens.get('multigrid_mixed', {}).get('value', float('nan')):.4g}, sequential "
+          f"FP64 {ens['sequential_fp64']['value']:.4g} cell-steps/s", file=sys.stderr, flush=True)
     achieved = None
     if prof and prof["avg_ms"]:
         achieved = prof["algorithmic_bytes"] / (prof["avg_ms"] * prof["launches"] * 1e-3) / 1e9
     line = {
-        "metric": "cell-steps/s, 64-member ensemble: batched mixed vs sequential FP64",
+        "metric": "cell-steps/s, 64-member ensemble: mixed vs sequential FP64",
         "value": b["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "ensemble64_uneven_warm_bubbles", "members": ens["members"],
                    "cells": ens["cells"], "delta_m": 50.0, "dt_s": 1.0, "fields": 1, "kord": 2,
-                   "monotone": True, "ptol": 1e-10, "seed": 404, "parallelism": "single GPU"},
+                   "monotone": True, "ptol": 1e-10, "seed": 404, "parallelism": "single GPU",
+                   "path": best},
         "ensemble": ens,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS if achieved else None, "traffic": None,
@@ -1059,6 +1067,7 @@ def main(argv=None):
         line["ensemble"] = run_ensemble(H, W, 1, args.ensemble_steps)
         ens = line["ensemble"]
         print(f"[bench] ensemble: batched mixed {ens['batched_mixed']['value']:.4g}, "
+              f"multigrid mixed {ens['multigrid_mixed']['value']:.4g}, "
               f"sequential FP64 {ens['sequential_fp64']['value']:.4g} cell-steps/s",
               file=sys.stderr, flush=True)
     if args.kh_steps > 0:

@@ -1405,8 +1405,30 @@ static int ensemble_solve(dcm_ensemble *e, std::vector<SolverOut> &outs) {
     const bool all_mg = e->precision == DCM_MIXED && n_mg == n;
     if (all_mg) {
         // every member with the multigrid preconditioner: its own solve, as
-        // when it is stepped alone (the batched iteration has no V-cycle);
-        // the members' contexts all use the ensemble's stream here
+        // when it is stepped alone (the batched iteration has no V-cycle).
+        // With graphs on, the members' solves run concurrently, each on its
+        // side stream (same kernels per member, so the same bits);
+        // otherwise one after another on the ensemble's stream.
+        bool conc = true;
+        for (int q = 0; q < n; ++q)
+            conc = conc && e->mem[q]->c->graphs && !e->mem[q]->c->profiling;
+        if (conc && !e->side.empty()) {
+            std::vector<hipStream_t> streams(n);
+            std::vector<double> rtol(n);
+            for (int q = 0; q < n; ++q) {
+                streams[q] = e->side[e->lane[q]];
+                rtol[q] = e->mem[q]->inner_rtol;
+            }
+            DCM_HIP(hipEventRecord(e->fork, e->stream));
+            for (hipStream_t s : e->side) DCM_HIP(hipStreamWaitEvent(s, e->fork, 0));
+            const int rq = bicgstab_mixed_concurrent(tasks.data(), n, streams.data(), e->ptol,
+                                                     9999, rtol.data(), outs.data(), st.data());
+            for (size_t k = 0; k < e->side.size(); ++k) {
+                DCM_HIP(hipEventRecord(e->join[k], e->side[k]));
+                DCM_HIP(hipStreamWaitEvent(e->stream, e->join[k], 0));
+            }
+            return rq;
+        }
         int worst = DCM_OK;
         for (int q = 0; q < n; ++q) {
             const int rq = bicgstab_mixed_batch(e->mem[q]->c, &tasks[q], 1, e->ptol, 9999,

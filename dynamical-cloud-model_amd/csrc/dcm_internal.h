@@ -432,6 +432,11 @@ int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, 
                          double inner_rtol, SolverOut *outs, int *status);
 int bicgstab_probe(dcm_ctx *c, const SolveTask *tasks, int n, int iters, SolverOut *outs,
                    int *status);
+// single-system multigrid mixed solves of n independent systems, system q on
+// streams[q] while it runs (each context's own graphs and scratch)
+int bicgstab_mixed_concurrent(const SolveTask *tasks, int n, const hipStream_t *streams,
+                              double tol, int itmax, const double *inner_rtol, SolverOut *outs,
+                              int *status);
 int bicgstab64(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,
                SolverOut *out);
 int bicgstab_mixed(dcm_ctx *c, const double *sw, double *x, double *b, double tol, int itmax,

@@ -43,6 +43,7 @@
 #include <cfloat>
 #include <cstddef>
 #include <cmath>
+#include <algorithm>
 #include <functional>
 #include <mutex>
 
@@ -1796,71 +1797,94 @@ static int chunk_graph(dcm_ctx *c, const SolvePlan &P, const IterLaunch<float> &
     return DCM_OK;
 }
 
-static int mixed_one_chunked(dcm_ctx *c, const SolveTask &t, double tol, int itmax,
-                             double inner_rtol, SolverOut *out, int *status) {
-    const Geo &pg = c->geo;
-    const CGeo g = cgeo(pg);
-    const Interior IL = interior_launch(pg);
-    const int nb = (int)(IL.grid.x * IL.grid.y * IL.grid.z);
-    const int max_outer = 50;
-    const int inner_cap = 2000;  // FP32 iterations per refinement sweep
-    DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
-    DCM_CHECK(ensure_solver_vectors32(c, 10));
-    const size_t cells = (size_t)pg.ip * pg.jp * pg.kp;
-    float *V[10];
-    for (int v = 0; v < 10; ++v) V[v] = c->sv32 + (size_t)v * cells;
+// One system's chunked solve as a resumable state: start() queues the start
+// and captures the chunk graph, launch() queues one chunk and the copy of
+// the refinement state to the host, poll() reads it once the stream has
+// drained, finish() handles the outcome.  Everything runs on c->stream.
+struct ChunkedSolve {
+    dcm_ctx *c = nullptr;
+    SolveTask t{};
+    double tol = 0.0, inner_rtol = 0.0;
+    int itmax = 0;
+    static constexpr int kMaxOuter = 50;
+    static constexpr int kInnerCap = 2000;  // FP32 iterations per refinement sweep
+    Geo pg{};
+    CGeo g{};
+    Interior IL{}, RL{};
+    int nbr = 0;
+    float *V[10] = {};
     double *r64 = nullptr, *xsave = nullptr;
-    DCM_CHECK(work_field64(c, 2, &r64));
-    DCM_CHECK(work_field64(c, 1, &xsave));
-    *out = SolverOut();
-    *status = DCM_OK;
-    SolveSys sy;
-    sy.c = c;
-    for (int v = 0; v < 10; ++v) sy.V[v] = V[v];
-    sy.march_planes = c->march_planes > 0 ? c->march_planes : kMarchPlanes;
     SolvePlan P;
-    DCM_CHECK(plan_solve(c->stream, &sy, 1, P, c->solve_tables));
-    const IterLaunch<float> L{c, P, 1, c->seq_sums ? 1 : 0};
-    OuterState *os = c->outer_d;
-    const OuterState *gos = os;
-    SolverScalars *sc = c->solver;
-    const CoefPack cp{c->met, c->met32};
-    double *sums = c->dscal + 40;
+    int seen_outer = 1, launched = 0, cap = 0;
+    OuterState h{};
+
     // true residual, decision, inner start (each gated on os->go)
-    const Interior RL = resid_launch(pg);
-    const int nbr = (int)(RL.grid.x * RL.grid.y * RL.grid.z);
-    auto decide_and_start = [&]() {
+    void decide_and_start() {
+        OuterState *os = c->outer_d;
+        const OuterState *gos = os;
+        SolverScalars *sc = c->solver;
+        const CoefPack cp{c->met, c->met32};
+        double *sums = c->dscal + 40;
         hipLaunchKernelGGL(k_resid_true, RL.grid, RL.block, 0, c->stream, (const double *)t.x,
                            (const double *)t.b, r64, pg, g, cp, c->partials, (long long)nbr, gos);
         hipLaunchKernelGGL(k_finalize_sums_3, dim3(1), dim3(1024), 0, c->stream,
                            (const double *)c->partials, nbr, sums, gos);
         hipLaunchKernelGGL(k_outer_decide, dim3(1), dim3(64), 0, c->stream, (const double *)sums,
-                           os, sc, tol, inner_rtol, itmax, max_outer);
+                           os, sc, tol, inner_rtol, itmax, kMaxOuter);
         hipLaunchKernelGGL(k_inner_init, RL.grid, RL.block, 0, c->stream, (const double *)r64,
                            V[0], V[1], V[2], V[3], V[9], g, (const double *)(sums + 1),
                            c->partials, (long long)nbr, gos);
         hipLaunchKernelGGL(k_fin_inner_init_dev, dim3(1), dim3(1024), 0, c->stream,
                            (const double *)c->partials, nbr, sc, gos);
-    };
+    }
     // the refinement step after an inner iteration
-    auto step = [&](int force) {
-        hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, os,
-                           (const SolverScalars *)sc, force);
+    void step(int force) {
+        hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, c->outer_d,
+                           (const SolverScalars *)c->solver, force);
         hipLaunchKernelGGL(k_add_correction_save, RL.grid, RL.block, 0, c->stream, t.x, xsave,
-                           (const float *)V[9], pg, g, gos);
+                           (const float *)V[9], pg, g, (const OuterState *)c->outer_d);
         decide_and_start();
-    };
-    hipLaunchKernelGGL(k_outer_start, dim3(1), dim3(64), 0, c->stream, os);
-    decide_and_start();
-    DCM_LAUNCH_CHECK();
-    const std::vector<const void *> ptrs = {P.args.mem, c->mg, c->partials, t.x, t.b,
-                                            r64, xsave, os, sc, c->dscal, c->sv32};
-    const double scal[3] = {tol, inner_rtol, (double)itmax};
-    DCM_CHECK(chunk_graph(c, P, L, ptrs, scal, 3, [&]() { step(0); }));
-    // the start leaves inner solve 1 under way (or nothing to do)
-    int seen_outer = 1, launched = 0, cap = std::min(inner_cap, itmax);
-    OuterState h{};
-    for (;;) {
+    }
+
+    int start(dcm_ctx *c_, const SolveTask &t_, double tol_, int itmax_, double inner_rtol_) {
+        c = c_;
+        t = t_;
+        tol = tol_;
+        itmax = itmax_;
+        inner_rtol = inner_rtol_;
+        pg = c->geo;
+        g = cgeo(pg);
+        IL = interior_launch(pg);
+        RL = resid_launch(pg);
+        nbr = (int)(RL.grid.x * RL.grid.y * RL.grid.z);
+        const int nb = (int)(IL.grid.x * IL.grid.y * IL.grid.z);
+        DCM_CHECK(ensure_partials(c, 4 * (size_t)nb + 64));
+        DCM_CHECK(ensure_solver_vectors32(c, 10));
+        const size_t cells = (size_t)pg.ip * pg.jp * pg.kp;
+        for (int v = 0; v < 10; ++v) V[v] = c->sv32 + (size_t)v * cells;
+        DCM_CHECK(work_field64(c, 2, &r64));
+        DCM_CHECK(work_field64(c, 1, &xsave));
+        SolveSys sy;
+        sy.c = c;
+        for (int v = 0; v < 10; ++v) sy.V[v] = V[v];
+        sy.march_planes = c->march_planes > 0 ? c->march_planes : kMarchPlanes;
+        DCM_CHECK(plan_solve(c->stream, &sy, 1, P, c->solve_tables));
+        const IterLaunch<float> L{c, P, 1, c->seq_sums ? 1 : 0};
+        hipLaunchKernelGGL(k_outer_start, dim3(1), dim3(64), 0, c->stream, c->outer_d);
+        decide_and_start();
+        DCM_LAUNCH_CHECK();
+        const std::vector<const void *> ptrs = {P.args.mem, c->mg, c->partials, t.x, t.b,
+                                                r64, xsave, c->outer_d, c->solver, c->dscal,
+                                                c->sv32};
+        const double scal[3] = {tol, inner_rtol, (double)itmax};
+        DCM_CHECK(chunk_graph(c, P, L, ptrs, scal, 3, [&]() { step(0); }));
+        // the start leaves inner solve 1 under way (or nothing to do)
+        seen_outer = 1;
+        launched = 0;
+        cap = std::min(kInnerCap, itmax);
+        return DCM_OK;
+    }
+    int launch() {
         if (launched >= cap) {
             step(1);  // the inner solve has used its iterations: correct anyway
             DCM_LAUNCH_CHECK();
@@ -1868,44 +1892,123 @@ static int mixed_one_chunked(dcm_ctx *c, const SolveTask &t, double tol, int itm
             DCM_HIP(hipGraphLaunch(c->chunk_graph, c->stream));
             ++launched;
         }
-        DCM_HIP(hipMemcpyAsync(c->outer_h, os, sizeof(OuterState), hipMemcpyDeviceToHost,
-                               c->stream));
-        DCM_HIP(hipStreamSynchronize(c->stream));
+        DCM_HIP(hipMemcpyAsync(c->outer_h, c->outer_d, sizeof(OuterState),
+                               hipMemcpyDeviceToHost, c->stream));
+        return DCM_OK;
+    }
+    // after the stream has drained: true while the solve goes on
+    bool poll() {
         h = *c->outer_h;
-        if (!h.active) break;
+        if (!h.active) return false;
         if (h.outer != seen_outer) {  // a new inner solve has started
             seen_outer = h.outer;
             launched = 0;
-            cap = std::max(1, std::min(inner_cap, itmax - h.total_iters));
+            cap = std::max(1, std::min(kInnerCap, itmax - h.total_iters));
         }
+        return true;
     }
-    out->iters = h.total_iters;
-    out->outer = h.outer;
-    out->err = h.err;
-    if (h.status == DCM_NONFINITE) set_error("bicgstab_mixed: non-finite initial residual");
-    *status = h.status;
-    if (h.fallback) {
-        // the last FP32 correction did not reduce the true residual: restore
-        // x and let the reference's FP64 solver finish
-        hipLaunchKernelGGL(k_restore_interior, IL.grid, IL.block, 0, c->stream, t.x,
-                           (const double *)xsave, pg, g);
-        DCM_LAUNCH_CHECK();
-        SolverOut o;
-        int st = DCM_OK;
-        (void)bicgstab64_batch(c, &t, 1, tol, itmax, &o, &st);
-        out->iters += o.iters;
-        out->err = o.err;
-        out->fallback = 1;
-        *status = st;
-        return st;
+    int finish(SolverOut *out, int *status) {
+        *out = SolverOut();
+        out->iters = h.total_iters;
+        out->outer = h.outer;
+        out->err = h.err;
+        if (h.status == DCM_NONFINITE) set_error("bicgstab_mixed: non-finite initial residual");
+        *status = h.status;
+        if (h.fallback) {
+            // the last FP32 correction did not reduce the true residual:
+            // restore x and let the reference's FP64 solver finish
+            hipLaunchKernelGGL(k_restore_interior, IL.grid, IL.block, 0, c->stream, t.x,
+                               (const double *)xsave, pg, g);
+            DCM_LAUNCH_CHECK();
+            SolverOut o;
+            int st = DCM_OK;
+            (void)bicgstab64_batch(c, &t, 1, tol, itmax, &o, &st);
+            out->iters += o.iters;
+            out->err = o.err;
+            out->fallback = 1;
+            *status = st;
+            return st;
+        }
+        if (*status == DCM_OK) {
+            DCM_CHECK(fill<double>(c, t.x, FILL_ALONG_DIM_WO, 0.0, 0.0));
+            hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, c->stream, t.x, t.sw,
+                               c->geo, c->h_dzn[0], 1);
+            DCM_LAUNCH_CHECK();
+        }
+        return *status;
     }
-    if (*status == DCM_OK) {
-        DCM_CHECK(fill<double>(c, t.x, FILL_ALONG_DIM_WO, 0.0, 0.0));
-        hipLaunchKernelGGL(k_x_rows, dim3(256), dim3(kBlock), 0, c->stream, t.x, t.sw, c->geo,
-                           c->h_dzn[0], 1);
-        DCM_LAUNCH_CHECK();
+};
+
+static int mixed_one_chunked(dcm_ctx *c, const SolveTask &t, double tol, int itmax,
+                             double inner_rtol, SolverOut *out, int *status) {
+    ChunkedSolve cs;
+    DCM_CHECK(cs.start(c, t, tol, itmax, inner_rtol));
+    for (;;) {
+        DCM_CHECK(cs.launch());
+        DCM_HIP(hipStreamSynchronize(c->stream));
+        if (!cs.poll()) break;
     }
-    return *status;
+    return cs.finish(out, status);
+}
+
+// Several single-system multigrid mixed solves at once, system q on
+// streams[q] (its context's stream while it runs): every round queues one
+// chunk of each unfinished system, then drains the streams and polls them,
+// so the systems' small kernels overlap on the device.  Each system runs
+// exactly the kernels of its solo solve, so it ends with the same bits.
+int bicgstab_mixed_concurrent(const SolveTask *tasks, int n, const hipStream_t *streams,
+                              double tol, int itmax, const double *inner_rtol, SolverOut *outs,
+                              int *status) {
+    std::vector<ChunkedSolve> cs(n);
+    std::vector<hipStream_t> saved(n);
+    std::vector<char> live(n, 0);
+    int worst = DCM_OK;
+    for (int q = 0; q < n; ++q) {
+        dcm_ctx *mc = tasks[q].c;
+        saved[q] = mc->stream;
+        mc->stream = streams[q];
+    }
+    auto restore = [&]() {
+        for (int q = 0; q < n; ++q) tasks[q].c->stream = saved[q];
+    };
+    for (int q = 0; q < n; ++q) {
+        const int st = cs[q].start(tasks[q].c, tasks[q], tol, itmax, inner_rtol[q]);
+        if (st != DCM_OK) {
+            restore();
+            return st;
+        }
+        live[q] = 1;
+    }
+    std::vector<hipStream_t> uniq(streams, streams + n);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    for (int left = n; left > 0;) {
+        for (int q = 0; q < n; ++q)
+            if (live[q]) {
+                const int st = cs[q].launch();
+                if (st != DCM_OK) {
+                    restore();
+                    return st;
+                }
+            }
+        for (hipStream_t s : uniq) {
+            const hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                restore();
+                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+                return DCM_HIP_ERROR;
+            }
+        }
+        for (int q = 0; q < n; ++q)
+            if (live[q] && !cs[q].poll()) {
+                live[q] = 0;
+                --left;
+                const int st = cs[q].finish(&outs[q], &status[q]);
+                if (st != DCM_OK && worst == DCM_OK) worst = st;
+            }
+    }
+    restore();
+    return worst;
 }
 
 // Mixed precision for a batch: FP32 inner BiCGSTAB (all members in one

@@ -117,6 +117,24 @@ int work_field32(dcm_ctx *c, int slot, float **out) {
     return DCM_OK;
 }
 
+int ensure_partials_adv(dcm_ctx *c, size_t n) {
+    if (c->partials_adv_cap >= n) return DCM_OK;
+    DCM_CHECK(no_alloc_in_capture(c, "ensure_partials_adv"));
+    if (c->partials_adv) {
+        DCM_HIP(hipStreamSynchronize(c->stream));
+        DCM_HIP(hipFree(c->partials_adv));
+        c->partials_adv = nullptr;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) {
+        set_error("hipMalloc of reduction partials failed");
+        return DCM_OUT_OF_MEMORY;
+    }
+    c->partials_adv = (double *)p;
+    c->partials_adv_cap = n;
+    return DCM_OK;
+}
+
 int ensure_partials(dcm_ctx *c, size_t n) {
     if (c->partials_cap >= n) return DCM_OK;
     DCM_CHECK(no_alloc_in_capture(c, "ensure_partials"));
@@ -501,6 +519,7 @@ int dcm_context_destroy(dcm_ctx *c) {
     (void)hipFree(c->dmet32);
     (void)hipFree(c->dtab32);
     (void)hipFree(c->mp32);
+    (void)hipFree(c->partials_adv);
     (void)hipFree(c->dscal);
     (void)hipHostFree(c->hscal);
     (void)hipFree(c->solver);

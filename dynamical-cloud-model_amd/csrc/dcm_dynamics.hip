@@ -465,8 +465,14 @@ struct dcm_model {
     // HIP graphs, each captured once its key has run eagerly (so every
     // scratch buffer exists: nothing is allocated or cleared in a capture)
     bool use_graphs = true;
-    std::vector<std::vector<int>> eager[2];
-    dcm::PhaseGraph pre[3], post[3];
+    std::vector<std::vector<int>> eager[3];
+    dcm::PhaseGraph pre[3], post[3], adv[3];
+    // the mixed path advects theta and q (they do not depend on the
+    // pressure) on a stream of its own while the pressure solve runs:
+    // forked from the main stream after the sources, joined before the
+    // momentum update
+    hipStream_t adv_stream = nullptr;
+    hipEvent_t adv_fork = nullptr, adv_join = nullptr;
 
     double *theta() { return oned; }
     double *thetan() { return oned + (c->geo.kp + 2); }
@@ -775,6 +781,13 @@ int dcm_model_create(dcm_ctx *c, int nq, int ncat, const int *c_s, const int *c_
         dcm_model_destroy(m);
         return DCM_HIP_ERROR;
     }
+    if (hipStreamCreateWithFlags(&m->adv_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->adv_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->adv_join, hipEventDisableTiming) != hipSuccess) {
+        set_error("dcm_model_create: stream or event creation failed");
+        dcm_model_destroy(m);
+        return DCM_HIP_ERROR;
+    }
     *out = m;
     return DCM_OK;
 }
@@ -783,8 +796,11 @@ int dcm_model_destroy(dcm_model *m) {
     if (!m) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(m->c->stream);
     for (int r = 0; r < 3; ++r)
-        for (dcm::PhaseGraph *g : {&m->pre[r], &m->post[r]})
+        for (dcm::PhaseGraph *g : {&m->pre[r], &m->post[r], &m->adv[r]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (m->adv_stream) (void)hipStreamDestroy(m->adv_stream);
+    if (m->adv_fork) (void)hipEventDestroy(m->adv_fork);
+    if (m->adv_join) (void)hipEventDestroy(m->adv_join);
     for (void *p : m->allocs) (void)hipFree(p);
     if (m->shadow) (void)hipFree(m->shadow);
     delete m;
@@ -1047,7 +1063,10 @@ static void add_stats(dcm_step_stats *stats, const SolverOut &so, double true_re
 }
 
 // model_driver after the pressure solve (driver_code.f90:506-674)
-static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
+// The advection of theta and q after the solve (driver_code.f90:506-580):
+// the reference-state advection and MPDATA of every field, from the current
+// velocities; nothing here reads the pressure.
+static int model_advect(dcm_model *m) {
     dcm_ctx *c = m->c;
     const Geo &g = c->geo;
     double *cur[3], *prev[3], *fut[3];
@@ -1119,6 +1138,16 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
             if (st != DCM_OK) return st;
         }
     }
+    return DCM_OK;
+}
+
+// The rest of the step after the advection: momentum, scalar update,
+// filter, rotation and halo rows (driver_code.f90:592-674).
+static int model_after_advect(dcm_model *m, dcm_step_stats *stats) {
+    dcm_ctx *c = m->c;
+    const Geo &g = c->geo;
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
     DCM_CHECK(advance_momentum_dev(c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
                                    prev[2], m->su, m->sv, m->sw, m->p));  // :592-594
     if (m->phys_on) {  // advance_scalar_fields_3d, :601-603
@@ -1145,6 +1174,11 @@ static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
     DCM_LAUNCH_CHECK();
     if (stats) stats->steps += 1;
     return DCM_OK;
+}
+
+static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
+    DCM_CHECK(model_advect(m));
+    return model_after_advect(m, stats);
 }
 
 // advection_scheme 2: the reference leaves thbase and thtop as (bc - m) + m
@@ -1225,12 +1259,15 @@ static std::vector<int> phase_key(const dcm_model *m) {
 // the first time its key is seen, from a graph afterwards (not while kernels
 // are being timed one by one).
 template <typename Body, typename Replay>
-static int model_phase(dcm_model *m, int phase, Body &&body, Replay &&host_replay) {
-    const std::vector<int> key = phase_key(m);
+static int model_phase(dcm_model *m, int phase, Body &&body, Replay &&host_replay,
+                       int extra = 0) {
+    std::vector<int> key = phase_key(m);
+    key.push_back(extra);
     std::vector<std::vector<int>> &seen = m->eager[phase];
     const bool known = std::find(seen.begin(), seen.end(), key) != seen.end();
+    PhaseGraph *slots = phase == 0 ? m->pre : (phase == 1 ? m->post : m->adv);
     if (m->use_graphs && !m->c->profiling && known)
-        return run_phase(m->c->stream, phase ? m->post : m->pre, key, body, host_replay);
+        return run_phase(m->c->stream, slots, key, body, host_replay);
     DCM_CHECK(body());
     if (!known) {
         if (seen.size() >= 48) seen.erase(seen.begin());
@@ -1241,9 +1278,29 @@ static int model_phase(dcm_model *m, int phase, Body &&body, Replay &&host_repla
 
 }  // extern "C++"
 
+// The advection runs beside the solve when it touches no scratch the solve
+// uses: the mixed path's batched FP32 MPDATA (own scratch and partial sums)
+// with advection_scheme 1 or 2, and not while kernels are timed one by one.
+static bool advect_overlaps(const dcm_model *m) {
+    const dcm_ctx *c = m->c;
+    return m->adv_stream && m->precision == DCM_MIXED && m->advection_scheme != 0 &&
+           !c->profiling && mpdata32_fast(c, m->kord);
+}
+
 static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
     DCM_CHECK(model_phase(m, 0, [&] { return model_pre_solve(m); }, [] {}));
+    const bool overlap = advect_overlaps(m);
+    if (overlap) {
+        DCM_HIP(hipEventRecord(m->adv_fork, c->stream));
+        DCM_HIP(hipStreamWaitEvent(m->adv_stream, m->adv_fork, 0));
+        hipStream_t main = c->stream;
+        c->stream = m->adv_stream;
+        const int sa = model_phase(m, 2, [&] { return model_advect(m); }, [] {});
+        c->stream = main;
+        if (sa != DCM_OK) return sa;
+        DCM_HIP(hipEventRecord(m->adv_join, m->adv_stream));
+    }
     SolverOut so;
     int st;
     if (m->precision == DCM_MIXED)
@@ -1259,13 +1316,19 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     else if (m->check_residual)
         DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
     add_stats(stats, so, true_resid);
-    // model_post_solve's host part on a replay: the level rotation and the
+    if (overlap) DCM_HIP(hipStreamWaitEvent(c->stream, m->adv_join, 0));
+    // the host part of the phase on a replay: the level rotation and the
     // step count
-    return model_phase(m, 1, [&] { return model_post_solve(m, stats); },
+    return model_phase(m, 1,
+                       [&] {
+                           return overlap ? model_after_advect(m, stats)
+                                          : model_post_solve(m, stats);
+                       },
                        [&] {
                            m->n += 1;
                            if (stats) stats->steps += 1;
-                       });
+                       },
+                       overlap ? 1 : 0);
 }
 
 int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {

@@ -180,6 +180,8 @@ struct dcm_ctx {
     // reduction scratch
     double *partials = nullptr;      // per-block partial sums (several slots)
     size_t partials_cap = 0;         // doubles
+    double *partials_adv = nullptr;  // the FP32 MPDATA's, which may run beside the solve
+    size_t partials_adv_cap = 0;
     double *dscal = nullptr;         // generic device scalars (64 doubles)
     double *hscal = nullptr;         // pinned host mirror (64 doubles)
     dcm::SolverScalars *solver = nullptr;   // device
@@ -258,6 +260,7 @@ int no_alloc_in_capture(dcm_ctx *c, const char *what);
 int work_field64(dcm_ctx *c, int slot, double **out);   // padded fp64 scratch
 int work_field32(dcm_ctx *c, int slot, float **out);    // padded fp32 scratch
 int ensure_partials(dcm_ctx *c, size_t n);
+int ensure_partials_adv(dcm_ctx *c, size_t n);  // the FP32 MPDATA's own partial sums
 int ensure_solver_vectors64(dcm_ctx *c, size_t count);
 int ensure_solver_vectors32(dcm_ctx *c, size_t count);
 int ensure_host_flags(dcm_ctx *c, size_t n, int **out);  // pinned ints
@@ -379,6 +382,10 @@ struct Mp32Cat {
 // 2, the sfvt/advection_3d.F90 arithmetic) go through mpdata_core<float>
 // with FP32 shadows of u, v, w written to shadow (3 padded fields).  The
 // shift of field f of the batch is left at dscal[16 + f].
+// true when mpdata32_batch takes its batched FP32 kernels (kord 2, the
+// linked copy's arithmetic, 32-bit offsets), whose scratch is its own: the
+// FP32 arrays and partials_adv
+bool mpdata32_fast(const dcm_ctx *c, int kord);
 int mpdata32_batch(dcm_ctx *c, float dt, const double *u, const double *v, const double *w,
                    float *shadow, const Mp32Cat *cats, int ncat, int kord, int monotone,
                    int neumann);

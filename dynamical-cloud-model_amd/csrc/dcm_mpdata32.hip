@@ -1065,12 +1065,12 @@ static int mpdata32_run(dcm_ctx *c, float dt, const double *u, const double *v,
     const size_t ntot4 = (size_t)ntot4_of(m);
     DCM_CHECK(ensure_mp32(c, 3 + (size_t)nf + 6 * (size_t)nlead, ntot4));
     const size_t npart = std::max((size_t)kMinBlocks * nf, (size_t)m.ntiles * nlead);
-    DCM_CHECK(ensure_partials(c, npart + 16));
+    DCM_CHECK(ensure_partials_adv(c, npart + 16));
     A.s32 = c->mp32;
     A.ntot4 = (long long)ntot4;
     A.m = c->dscal + 16;
     A.flag = (int *)(c->dscal + 48);
-    A.partials = c->partials;
+    A.partials = c->partials_adv;
     A.nf = nf;
     A.nlead = nlead;
     A.dt = dt;
@@ -1116,14 +1116,19 @@ static int mpdata32_run(dcm_ctx *c, float dt, const double *u, const double *v,
     return DCM_OK;
 }
 
-int mpdata32_batch(dcm_ctx *c, float dt, const double *u, const double *v, const double *w,
-                   float *shadow, const Mp32Cat *cats, int ncat, int kord, int monotone,
-                   int neumann) {
+bool mpdata32_fast(const dcm_ctx *c, int kord) {
     const Geo &g = c->geo;
     const PM m = pm_geo(g, kMp32Planes);
     // byte offsets of the aligned FP32 arrays and of the FP64 fields fit 32 bits
     const bool fits = (long long)ntot4_of(m) * 4 < (1LL << 32) && g.ntot * 8 < (1LL << 32);
-    const bool fast = kord == 2 && c->mpdata_variant == 0 && fits;
+    return kord == 2 && c->mpdata_variant == 0 && fits;
+}
+
+int mpdata32_batch(dcm_ctx *c, float dt, const double *u, const double *v, const double *w,
+                   float *shadow, const Mp32Cat *cats, int ncat, int kord, int monotone,
+                   int neumann) {
+    const Geo &g = c->geo;
+    const bool fast = mpdata32_fast(c, kord);
     float *uf = shadow, *vf = shadow + g.ntot, *wf = shadow + 2 * g.ntot;
     bool shadowed = false;
     auto generic = [&](const Mp32Cat &q) -> int {

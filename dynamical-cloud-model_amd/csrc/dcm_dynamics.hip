@@ -465,8 +465,8 @@ struct dcm_model {
     // HIP graphs, each captured once its key has run eagerly (so every
     // scratch buffer exists: nothing is allocated or cleared in a capture)
     bool use_graphs = true;
-    std::vector<std::vector<int>> eager[3];
-    dcm::PhaseGraph pre[3], post[3], adv[3];
+    std::vector<std::vector<int>> eager[4];
+    dcm::PhaseGraph pre[3], post[3], adv[3], mom[3];
     // the mixed path advects theta and q (they do not depend on the
     // pressure) on a stream of its own while the pressure solve runs:
     // forked from the main stream after the sources, joined before the
@@ -796,7 +796,7 @@ int dcm_model_destroy(dcm_model *m) {
     if (!m) return DCM_BAD_ARG;
     (void)hipStreamSynchronize(m->c->stream);
     for (int r = 0; r < 3; ++r)
-        for (dcm::PhaseGraph *g : {&m->pre[r], &m->post[r], &m->adv[r]})
+        for (dcm::PhaseGraph *g : {&m->pre[r], &m->post[r], &m->adv[r], &m->mom[r]})
             if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (m->adv_stream) (void)hipStreamDestroy(m->adv_stream);
     if (m->adv_fork) (void)hipEventDestroy(m->adv_fork);
@@ -1141,15 +1141,23 @@ static int model_advect(dcm_model *m) {
     return DCM_OK;
 }
 
-// The rest of the step after the advection: momentum, scalar update,
-// filter, rotation and halo rows (driver_code.f90:592-674).
+// The momentum update (driver_code.f90:592-594): the new level from the
+// previous one, the sources and the pressure; it reads nothing the
+// advection writes.
+static int model_momentum(dcm_model *m) {
+    double *cur[3], *prev[3], *fut[3];
+    m->levels(cur, prev, fut);
+    return advance_momentum_dev(m->c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
+                                prev[2], m->su, m->sv, m->sw, m->p);
+}
+
+// The rest of the step after the advection and the momentum update: scalar
+// update, filter, rotation and halo rows (driver_code.f90:601-674).
 static int model_after_advect(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
     const Geo &g = c->geo;
     double *cur[3], *prev[3], *fut[3];
     m->levels(cur, prev, fut);
-    DCM_CHECK(advance_momentum_dev(c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
-                                   prev[2], m->su, m->sv, m->sw, m->p));  // :592-594
     if (m->phys_on) {  // advance_scalar_fields_3d, :601-603
         const long long n = g.ntot;
         hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->th, m->sth,
@@ -1178,6 +1186,7 @@ static int model_after_advect(dcm_model *m, dcm_step_stats *stats) {
 
 static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
     DCM_CHECK(model_advect(m));
+    DCM_CHECK(model_momentum(m));
     return model_after_advect(m, stats);
 }
 
@@ -1265,7 +1274,7 @@ static int model_phase(dcm_model *m, int phase, Body &&body, Replay &&host_repla
     key.push_back(extra);
     std::vector<std::vector<int>> &seen = m->eager[phase];
     const bool known = std::find(seen.begin(), seen.end(), key) != seen.end();
-    PhaseGraph *slots = phase == 0 ? m->pre : (phase == 1 ? m->post : m->adv);
+    PhaseGraph *slots = phase == 0 ? m->pre : (phase == 1 ? m->post : (phase == 2 ? m->adv : m->mom));
     if (m->use_graphs && !m->c->profiling && known)
         return run_phase(m->c->stream, slots, key, body, host_replay);
     DCM_CHECK(body());
@@ -1316,7 +1325,12 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     else if (m->check_residual)
         DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
     add_stats(stats, so, true_resid);
-    if (overlap) DCM_HIP(hipStreamWaitEvent(c->stream, m->adv_join, 0));
+    if (overlap) {
+        // the momentum update needs only the solve; the rest waits for the
+        // advection
+        DCM_CHECK(model_phase(m, 3, [&] { return model_momentum(m); }, [] {}));
+        DCM_HIP(hipStreamWaitEvent(c->stream, m->adv_join, 0));
+    }
     // the host part of the phase on a replay: the level rotation and the
     // step count
     return model_phase(m, 1,

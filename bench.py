@@ -689,14 +689,14 @@ def kelvin_helmholtz(H, W, steps, warmup=2):
     """The shear layer (512x4x256 cells at 10 m, dt=0.2 s) on both paths from
     the same state: after ``steps`` timed steps, the mixed path's relative L2
     error against FP64 for p (mean removed), u, w and theta, with both paths'
-    iterations, FP64 fallbacks and worst true residual.  The grid is too
-    thin in y for the V-cycle (mg_possible), so both paths use the
-    Chebyshev-relaxed sweeps."""
+    iterations, FP64 fallbacks and worst true residual.  Both paths use the
+    V-cycle, semi-coarsened below 128x1x64 (y stops halving at one cell)."""
     case = W.kelvin_helmholtz(steps=warmup + steps)
     out, snaps = {}, {}
     for name, prec in (("fp64", H.FP64), ("mixed", H.MIXED)):
         with H.Context(case.dom) as ctx:
             ctx.set_precond_chebyshev(CHEB_A, CHEB_DIVISIONS)
+            ctx.set_precond_multigrid(True)
             model = H.Model(ctx, case, prec)
             try:
                 model.set_check_residual(True)

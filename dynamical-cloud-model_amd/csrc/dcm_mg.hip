@@ -12,20 +12,24 @@
 // decides convergence either way.
 //
 // Cell-centred hierarchy on the compact (k fastest) layout of the inner
-// solve, every dimension halved per level down to at most 1024 cells, and on
-// while every dimension stays even, down to 64 cells or fewer:
+// solve, every even dimension halved per level (an odd one, such as a
+// single cell in y, keeps its size: semi-coarsening) down to at most 1024
+// cells, and on while some dimension is even, down to 64 cells or fewer:
 //   smoother     damped Jacobi x <- x + w (b - A x)/a_p, two sweeps before
 //                and two after the coarse correction (the first from zero);
 //                on level 0 these are the solver's own sweep kernels, the
 //                first one fused into the pass that forms t or s, the last
 //                one of the p branch into the update p = x + bet*p;
-//   restriction  the mean of the eight children's residuals;
-//   prolongation trilinear (weights 3/4, 1/4 per axis), periodic in x and y,
-//                the Neumann copy in z, added to the fine iterate;
+//   restriction  the mean of the eight children's residuals (two or four
+//                where an axis keeps its size);
+//   prolongation trilinear (weights 3/4, 1/4 per axis; an axis that keeps
+//                its size is copied), periodic in x and y, the Neumann copy
+//                in z, added to the fine iterate;
 //   operators    level 0 is the solver's own (the 1-D coefficients of
-//                set_mat_a, dynamics.f90:84-96); a coarse coefficient is the
-//                mean of its two children's divided by 4 (twice the spacing),
-//                exact on the reference's uniform grids;
+//                set_mat_a, dynamics.f90:84-96); a coarse coefficient along
+//                a halved axis is the mean of its two children's divided by
+//                4 (twice the spacing), exact on the reference's uniform
+//                grids, and along an axis that keeps its size the fine one;
 //   coarsest     solved exactly: x = M b with M = (A_c + e e^T)^-1, e the
 //                constant vector (the null space of the periodic/Neumann
 //                operator; for b in the range of A_c this is the zero-mean
@@ -100,6 +104,54 @@ __device__ __forceinline__ R ax_of(const Lv<R> &L, P x, int k, int j, int i, R a
            L.ad[k] * x[at(L, kd, j, i)] + L.au[k] * x[at(L, ku, j, i)] + ap * x[at(L, k, j, i)];
 }
 
+// whether C halves every dimension of F (else some axis keeps its size)
+template <typename R> __host__ __device__ __forceinline__ bool halves(const Lv<R> &F,
+                                                                      const Lv<R> &C) {
+    return F.ip == 2 * C.ip && F.jp == 2 * C.jp && F.kp == 2 * C.kp;
+}
+// The transfers of a semi-coarsened pair of levels: along an axis whose
+// size C keeps there is one child (the same index) and no interpolation.
+// The mean of the children's (b - A x), in the child order of the halving
+// kernels (k fastest, then j, then i).
+template <typename R, typename PB, typename PX>
+__device__ __forceinline__ R restrict_semi(const Lv<R> &F, const Lv<R> &C, PB b, PX x, int K,
+                                           int J, int I) {
+    const int ni = F.ip != C.ip ? 2 : 1, nj = F.jp != C.jp ? 2 : 1, nk = F.kp != C.kp ? 2 : 1;
+    R acc = R(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int dk = q & 1, dj = (q >> 1) & 1, di = q >> 2;
+        if (dk >= nk || dj >= nj || di >= ni) continue;
+        const int k = nk * (K - 1) + 1 + dk, j = nj * (J - 1) + 1 + dj, i = ni * (I - 1) + 1 + di;
+        const R ap = ap_of(F, k, j, i);
+        acc += b[at(F, k, j, i)] - ax_of(F, x, k, j, i, ap);
+    }
+    const int halved = ni + nj + nk - 3;
+    const R wt = halved == 3 ? R(0.125) : (halved == 2 ? R(0.25) : (halved == 1 ? R(0.5) : R(1)));
+    return wt * acc;
+}
+// P xc at fine cell (k,j,i): the trilinear weights (3/4, 1/4) along halved
+// axes, (1, 0) along the others
+template <typename R, typename PC>
+__device__ __forceinline__ R prolong_semi(const Lv<R> &F, const Lv<R> &C, PC xc, int k, int j,
+                                          int i) {
+    const bool hi = F.ip != C.ip, hj = F.jp != C.jp, hk = F.kp != C.kp;
+    const int I = hi ? (i + 1) >> 1 : i, J = hj ? (j + 1) >> 1 : j, K = hk ? (k + 1) >> 1 : k;
+    int I2 = hi ? ((i & 1) ? I - 1 : I + 1) : I;
+    int J2 = hj ? ((j & 1) ? J - 1 : J + 1) : J;
+    int K2 = hk ? ((k & 1) ? K - 1 : K + 1) : K;
+    I2 = I2 < 1 ? C.ip : (I2 > C.ip ? 1 : I2);
+    J2 = J2 < 1 ? C.jp : (J2 > C.jp ? 1 : J2);
+    K2 = K2 < 1 ? 1 : (K2 > C.kp ? C.kp : K2);
+    const R ai = hi ? R(0.75) : R(1), qi = hi ? R(0.25) : R(0);
+    const R aj = hj ? R(0.75) : R(1), qj = hj ? R(0.25) : R(0);
+    const R ak = hk ? R(0.75) : R(1), qk = hk ? R(0.25) : R(0);
+    return ai * (aj * (ak * xc[at(C, K, J, I)] + qk * xc[at(C, K2, J, I)]) +
+                 qj * (ak * xc[at(C, K, J2, I)] + qk * xc[at(C, K2, J2, I)])) +
+           qi * (aj * (ak * xc[at(C, K, J, I2)] + qk * xc[at(C, K2, J, I2)]) +
+                 qj * (ak * xc[at(C, K, J2, I2)] + qk * xc[at(C, K2, J2, I2)]));
+}
+
 // One thread per cell of a level on a 3-D grid: x-threads along k (the
 // fastest index), y-threads along j, one i per grid z (no divisions).
 struct Grid3 {
@@ -137,8 +189,9 @@ __global__ __launch_bounds__(256) void k_mg_smooth(const R *__restrict__ b,
     xo[c] = x[c] + w * (b[c] - ax_of(L, x, k, j, i, ap)) * inv;
 }
 
-// bc = mean over the eight children of (b - A x) on the fine level
-template <typename R>
+// bc = mean over the eight children of (b - A x) on the fine level (Semi:
+// C keeps the size of some axis)
+template <typename R, bool Semi>
 __global__ __launch_bounds__(256) void k_mg_restrict(const R *__restrict__ b,
                                                      const R *__restrict__ x,
                                                      R *__restrict__ bc, Lv<R> F, Lv<R> C,
@@ -147,6 +200,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const R *__restrict__ b,
     int K, J, I;
     if (!cell3(C, K, J, I)) return;
     const int cc = at(C, K, J, I);
+    if (Semi) {
+        bc[cc] = restrict_semi(F, C, b, x, K, J, I);
+        return;
+    }
     R acc = R(0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -158,8 +215,9 @@ __global__ __launch_bounds__(256) void k_mg_restrict(const R *__restrict__ b,
     bc[cc] = R(0.125) * acc;
 }
 
-// x += trilinear interpolation of the coarse correction xc
-template <typename R>
+// x += trilinear interpolation of the coarse correction xc (Semi: C keeps
+// the size of some axis)
+template <typename R, bool Semi>
 __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
                                                     R *__restrict__ x, Lv<R> F, Lv<R> C,
                                                     const SolverScalars *sc) {
@@ -167,6 +225,10 @@ __global__ __launch_bounds__(256) void k_mg_prolong(const R *__restrict__ xc,
     int k, j, i;
     if (!cell3(F, k, j, i)) return;
     const int c = at(F, k, j, i);
+    if (Semi) {
+        x[c] += prolong_semi(F, C, xc, k, j, i);
+        return;
+    }
     const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
     // the second coarse cell along each axis: the one on this child's side
     int I2 = (i & 1) ? I - 1 : I + 1;
@@ -625,9 +687,14 @@ __device__ __forceinline__ void tail_smooth(const Lv<R> &L, PB b, PX x, PO xo, R
 template <typename R, typename PB, typename PX, typename PC>
 __device__ __forceinline__ void tail_restrict(const Lv<R> &F, const Lv<R> &C, PB b, PX x,
                                               PC bc) {
+    const bool semi = !halves(F, C);  // the same for every thread
     for (int cc = threadIdx.x; cc < C.n; cc += kTailThreads) {
         int K, J, I;
         cell_of(C, cc, K, J, I);
+        if (semi) {
+            bc[cc] = restrict_semi(F, C, b, x, K, J, I);
+            continue;
+        }
         R acc = R(0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -642,9 +709,14 @@ __device__ __forceinline__ void tail_restrict(const Lv<R> &F, const Lv<R> &C, PB
 // x += trilinear interpolation of xc
 template <typename R, typename PC, typename PX>
 __device__ __forceinline__ void tail_prolong(const Lv<R> &F, const Lv<R> &C, PC xc, PX x) {
+    const bool semi = !halves(F, C);  // the same for every thread
     for (int c = threadIdx.x; c < F.n; c += kTailThreads) {
         int k, j, i;
         cell_of(F, c, k, j, i);
+        if (semi) {
+            x[c] += prolong_semi(F, C, xc, k, j, i);
+            continue;
+        }
         const int I = (i + 1) >> 1, J = (j + 1) >> 1, K = (k + 1) >> 1;
         int I2 = (i & 1) ? I - 1 : I + 1;
         int J2 = (j & 1) ? J - 1 : J + 1;
@@ -826,26 +898,27 @@ void mg_release(dcm_ctx *c) {
     c->mg = nullptr;
 }
 
-// Levels of the grid: every dimension halved while all three are even, down
-// to at most kCoarseMax cells; false when that size is never reached.
+// Levels of the grid: every even dimension halved per level (an odd one
+// keeps its size) down to at most kCoarseMax cells; false when that size is
+// never reached (every dimension odd above it) or the grid has no coarser
+// level.
 static bool mg_levels(int ip, int jp, int kp, std::vector<int> &dims) {
     dims = {ip, jp, kp};
-    while ((long long)ip * jp * kp > kCoarseMax) {
-        if (ip % 2 || jp % 2 || kp % 2) return false;
-        ip /= 2;
-        jp /= 2;
-        kp /= 2;
+    auto coarsen = [&]() {
+        if (ip % 2 && jp % 2 && kp % 2) return false;
+        if (ip % 2 == 0) ip /= 2;
+        if (jp % 2 == 0) jp /= 2;
+        if (kp % 2 == 0) kp /= 2;
         dims.insert(dims.end(), {ip, jp, kp});
-    }
+        return true;
+    };
+    while ((long long)ip * jp * kp > kCoarseMax)
+        if (!coarsen()) return false;
     if (dims.size() < 6) return false;  // at least one coarse level
     // further down while it halves, so that the dense coarsest solve is small
-    while ((long long)ip * jp * kp > kDenseTarget && ip % 2 == 0 && jp % 2 == 0 && kp % 2 == 0) {
-        ip /= 2;
-        jp /= 2;
-        kp /= 2;
-        dims.insert(dims.end(), {ip, jp, kp});
+    while ((long long)ip * jp * kp > kDenseTarget && coarsen()) {
     }
-    return true;
+    return dims.size() / 3 <= 16;  // (the level arrays' size)
 }
 
 bool mg_possible(const Geo &g) {
@@ -916,7 +989,7 @@ template <typename R> static int plan_tiles(MgLevels<R> &V, int nlev) {
     for (int l = 1; l < nlev - 1; ++l) {
         const Lv<R> &L = V.lv[l];
         V.tile[l] = Tile2{0, 0};
-        if (L.kp > 256 || L.n > kFuseMax) continue;
+        if (L.kp > 256 || L.n > kFuseMax || !halves(L, V.lv[l + 1])) continue;
         for (int t : {4, 2}) {
             const size_t bytes =
                 (3 * (size_t)(t + 4) * (t + 4) * L.kp + 4 * (size_t)(t + 4)) * sizeof(R);
@@ -1043,7 +1116,7 @@ int mg_prepare(dcm_ctx *c) {
     const Geo &g = c->geo;
     std::vector<int> d;
     if (!mg_levels(g.ip, g.jp, g.kp, d)) {
-        set_error("multigrid preconditioner: the grid does not halve (all of ip, jp, kp "
+        set_error("multigrid preconditioner: the grid does not coarsen (some of ip, jp, kp "
                   "even at every level) down to 1024 cells or fewer");
         return DCM_BAD_ARG;
     }
@@ -1069,15 +1142,18 @@ int mg_prepare(dcm_ctx *c) {
         f[4][k] = 1.0 / (dz[k - 1] * dzn[k]);
         f[5][k] = 1.0 / (dz[k - 1] * dzn[k - 1]);
     }
-    // coarse levels: the mean of the two children's coefficients over 4
+    // coarse levels: along a halved axis the mean of the two children's
+    // coefficients over 4, along an axis that keeps its size the fine ones
     std::vector<double> h;
     std::vector<size_t> off(6 * H->nlev, 0);
     for (int l = 1; l < H->nlev; ++l) {
         const int n3[3] = {d[3 * l], d[3 * l + 1], d[3 * l + 2]};
         for (int a = 0; a < 6; ++a) {
             const int n = n3[a / 2];
+            const bool halved = d[3 * (l - 1) + a / 2] != n;
             std::vector<double> nx(n + 2, 0.0);
-            for (int m = 1; m <= n; ++m) nx[m] = 0.125 * (f[a][2 * m - 1] + f[a][2 * m]);
+            for (int m = 1; m <= n; ++m)
+                nx[m] = halved ? 0.125 * (f[a][2 * m - 1] + f[a][2 * m]) : f[a][m];
             off[6 * l + a] = h.size();
             for (double v : nx) h.push_back(v);
             f[a] = nx;
@@ -1116,15 +1192,35 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
     // levels from `top` down are the tail's (the coarsest alone without it)
     const bool tail = c->mg_tail && V.tail > 0;
     const int top = tail ? V.tail : L - 1;
-    Grid3 G = grid3(V.lv[1]);
-    if (c->mg_tail) {  // (per-pass mode keeps the one-thread-per-cell kernel)
-        Grid3 GM = G;
+    // the per-cell transfers between levels l and l+1
+    auto restrict_cells = [&](int l, const R *b, const R *x) {
+        const Grid3 GC = grid3(V.lv[l + 1]);
+        if (halves(V.lv[l], V.lv[l + 1]))
+            hipLaunchKernelGGL((k_mg_restrict<R, false>), GC.grid, GC.block, 0, c->stream, b, x,
+                               V.b[l + 1], V.lv[l], V.lv[l + 1], sc);
+        else
+            hipLaunchKernelGGL((k_mg_restrict<R, true>), GC.grid, GC.block, 0, c->stream, b, x,
+                               V.b[l + 1], V.lv[l], V.lv[l + 1], sc);
+    };
+    auto prolong_cells = [&](int l, const R *xc, R *x) {
+        const Grid3 GF = grid3(V.lv[l]);
+        if (halves(V.lv[l], V.lv[l + 1]))
+            hipLaunchKernelGGL((k_mg_prolong<R, false>), GF.grid, GF.block, 0, c->stream, xc, x,
+                               V.lv[l], V.lv[l + 1], sc);
+        else
+            hipLaunchKernelGGL((k_mg_prolong<R, true>), GF.grid, GF.block, 0, c->stream, xc, x,
+                               V.lv[l], V.lv[l + 1], sc);
+    };
+    // the marching finest transfers (mode 1, a halving first level); per-pass
+    // mode keeps the one-thread-per-cell kernels
+    const bool march = c->mg_tail && halves(V.lv[0], V.lv[1]);
+    if (march) {
+        Grid3 GM = grid3(V.lv[1]);
         GM.grid.z = (unsigned)((V.lv[1].ip + kProPlanes / 2 - 1) / (kProPlanes / 2));
         hipLaunchKernelGGL(k_mg_restrict_march<R>, GM.grid, GM.block, 0, c->stream, b0,
                            (const R *)x0, V.b[1], V.lv[0], V.lv[1], sc);
     } else {
-        hipLaunchKernelGGL(k_mg_restrict<R>, G.grid, G.block, 0, c->stream, b0, (const R *)x0,
-                           V.b[1], V.lv[0], V.lv[1], sc);
+        restrict_cells(0, b0, (const R *)x0);
     }
     // a level's legs as one kernel each where it has a tile (mode 1); the
     // up-leg then leaves the level's result in t
@@ -1139,13 +1235,12 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
                                (const R *)V.b[l], V.x[l], V.b[l + 1], F, C, T, w, sc);
             continue;
         }
-        const Grid3 GF = grid3(F), GC = grid3(C);
+        const Grid3 GF = grid3(F);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)nullptr, V.t[l], F, w, sc);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.t[l], V.x[l], F, w, sc);
-        hipLaunchKernelGGL(k_mg_restrict<R>, GC.grid, GC.block, 0, c->stream,
-                           (const R *)V.b[l], (const R *)V.x[l], V.b[l + 1], F, C, sc);
+        restrict_cells(l, (const R *)V.b[l], (const R *)V.x[l]);
     }
     if (tail)
         hipLaunchKernelGGL(k_mg_tail<R>, dim3(1), dim3(kTailThreads), V.tail_lds, c->stream,
@@ -1167,22 +1262,20 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
             continue;
         }
         const Grid3 GF = grid3(F);
-        hipLaunchKernelGGL(k_mg_prolong<R>, GF.grid, GF.block, 0, c->stream, below, V.x[l], F,
-                           C, sc);
+        prolong_cells(l, below, V.x[l]);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.x[l], V.t[l], F, w, sc);
         hipLaunchKernelGGL(k_mg_smooth<R>, GF.grid, GF.block, 0, c->stream,
                            (const R *)V.b[l], (const R *)V.t[l], V.x[l], F, w, sc);
         below = V.x[l];
     }
-    G = grid3(V.lv[0]);
-    if (c->mg_tail) {  // (per-pass mode keeps the one-thread-per-cell kernel)
+    if (march) {
+        Grid3 G = grid3(V.lv[0]);
         G.grid.z = (unsigned)((V.lv[0].ip + kProPlanes - 1) / kProPlanes);
         hipLaunchKernelGGL(k_mg_prolong_march<R>, G.grid, G.block, 0, c->stream, below, x0,
                            V.lv[0], V.lv[1], sc);
     } else {
-        hipLaunchKernelGGL(k_mg_prolong<R>, G.grid, G.block, 0, c->stream, below, x0, V.lv[0],
-                           V.lv[1], sc);
+        prolong_cells(0, below, x0);
     }
     DCM_LAUNCH_CHECK();
     return DCM_OK;

@@ -73,15 +73,41 @@ def test_multigrid_model_tracks_fp64_and_the_jacobi_path():
         assert rel_l2(b, g) < 1e-5, f
 
 
-def test_multigrid_is_refused_where_the_grid_does_not_halve():
+def test_multigrid_is_refused_where_the_grid_does_not_coarsen():
     dom = small_domain(12, 10, 8)   # 960 cells: no coarser level
     with H.Context(dom) as c:
         with pytest.raises(H.DcmError):
             c.set_precond_multigrid(True)
-    dom = small_domain(40, 6, 40)   # 6 -> 3 is odd before 1024 cells
+    dom = small_domain(15, 9, 11)   # 1485 cells, every dimension odd
     with H.Context(dom) as c:
         with pytest.raises(H.DcmError):
             c.set_precond_multigrid(True)
+
+
+@pytest.mark.parametrize("shape", [(40, 6, 40), (64, 2, 32), (128, 4, 64), (96, 8, 16)])
+def test_semi_coarsened_grids_converge_in_few_iterations(shape):
+    """A dimension that turns odd keeps its size on the coarser levels
+    (40x6x40 -> 20x3x20 -> 10x3x10 -> 5x3x5, odd everywhere; 64x2x32 ->
+    32x1x16 -> 16x1x8 -> ...; 96x8x16 -> 48x4x8 -> 24x2x4 -> 12x1x2) and
+    the transfers copy along it: the V-cycle still cuts the inner
+    iterations at least threefold against the Jacobi sweeps, and the solve
+    ends at the FP64 true residual asked for."""
+    dom = small_domain(*shape)
+    b = _rhs(dom, 7)
+    sw = np.zeros(dom.shape)
+    iters, sols = {}, {}
+    for mg in (False, True):
+        with H.Context(dom) as c:
+            if mg:
+                c.set_precond_multigrid(True)
+            x = c.array(np.zeros(dom.shape))
+            it, outer, err = c.bicgstab_mixed(c.array(sw), x, c.array(b), tol=1e-10)
+            assert err < 1e-10, (mg, it, outer, err)
+            iters[mg] = it
+            sols[mg] = x.get()[1:-1, 1:-1, 1:-1]
+    assert iters[True] * 3 <= iters[False], iters
+    a, g = sols[False], sols[True]
+    assert rel_l2(a - a.mean(), g - g.mean()) < 1e-8
 
 
 def test_multigrid_switches_off_and_on():
@@ -108,12 +134,13 @@ def _q_mass(dom, a):
 
 
 @pytest.mark.parametrize("name", ["warm_bubble_64", "moist_bubble_256x256x128",
-                                  "large_domain_512x512x256"])
+                                  "kelvin_helmholtz_512x4x256", "large_domain_512x512x256"])
 def test_multigrid_at_full_size(name):
-    """The workloads' full sizes (the 512x4x256 Kelvin-Helmholtz grid does
-    not halve in y and keeps the sweeps): every solve at FP64 true residual
-    below 1e-10, every q conserved to FP32 flux rounding, within 1e-5 of the
-    FP64 path, and a few inner iterations per solve."""
+    """The workloads' full sizes (the 512x4x256 Kelvin-Helmholtz grid stops
+    halving in y at one cell and is semi-coarsened below that): every solve
+    at FP64 true residual below 1e-10, every q conserved to FP32 flux
+    rounding, within 1e-5 of the FP64 path, and a few inner iterations per
+    solve."""
     case = W.CONFIGS[name](steps=2)
     dom = case.dom
     print(f"multigrid at full size {name}: {dom.ip}x{dom.jp}x{dom.kp}", flush=True)
@@ -247,7 +274,8 @@ def test_fp64_solve_with_the_v_cycle_matches_the_reference_sweeps():
 
 
 @pytest.mark.parametrize("precision", [H.MIXED, H.FP64])
-@pytest.mark.parametrize("shape,steps", [((64, 64, 32), 4), ((256, 128, 64), 2)])
+@pytest.mark.parametrize("shape,steps", [((64, 64, 32), 4), ((256, 128, 64), 2),
+                                         ((128, 4, 64), 3)])
 def test_one_block_tail_matches_per_pass_launches(precision, shape, steps):
     """The levels that fit one workgroup's LDS run as one single-block kernel,
     the levels of at most 2^17 cells as one kernel per leg, and the
@@ -255,7 +283,9 @@ def test_one_block_tail_matches_per_pass_launches(precision, shape, steps):
     dcm_set_precond_multigrid(ctx, 2) every pass is its own one-thread-per-
     cell launch.  64x64x32 has every coarse level fused or in the tail;
     256x128x64 has a first coarse level of 2^18 cells that keeps its
-    per-pass kernels.  Same arithmetic, so the same bits after every step."""
+    per-pass kernels; 128x4x64 reaches one cell in y at level 2, and its
+    tail transfers are semi-coarsened.  Same arithmetic, so the same bits
+    after every step."""
     case = W.moist_bubble(*shape, delta=50.0, dt=0.5, steps=steps, ntracer=1, seed=41)
     snaps = {}
     for mode in (1, 2):

@@ -117,22 +117,26 @@ int dcm_fill_along_z(dcm_ctx *ctx, double *a, double lbc, double ubc);
 /* The preconditioner of the pressure solves of a single model, of
  * dcm_bicgstab_mixed and of dcm_bicgstab: on != 0 applies one geometric
  * multigrid V-cycle (damped-Jacobi smoothing, trilinear transfers, every
- * dimension halved per level) wherever the reference's iteration applies its
+ * even dimension halved per level and an odd one kept) wherever the
+ * reference's iteration applies its
  * Jacobi sweeps (precondition, dynamics.f90:241-270, inside :390-430), so
  * that a solve needs a few iterations at any size.  The mixed path's FP32
  * inner solve runs an FP32 V-cycle (its outer FP64 refinement still ends
  * every solve at the requested true residual); the FP64 BiCGSTAB runs the
  * same cycle in FP64, which is no longer the reference's arithmetic but
  * measures the precision gain at an equal preconditioner.  DCM_BAD_ARG (and
- * no change) when ip, jp and kp do not halve evenly down to 1024 cells.  Off
- * by default.  In a DCM_MIXED ensemble either every member has the setting
- * (each member is then solved with its own V-cycle, one after another, and
- * evolves bit for bit as when stepped alone) or none has it (one batched
+ * no change) when the grid has at most 1024 cells or its dimensions are all
+ * odd above 1024 cells.  Off by default.  In a DCM_MIXED ensemble either
+ * every member has the setting (each member is then solved with its own
+ * V-cycle, the members side by side on the ensemble's streams, and evolves
+ * bit for bit as when stepped alone) or none has it (one batched
  * solve with the Jacobi/Chebyshev sweeps); dcm_ensemble_step returns
  * DCM_BAD_ARG for a mix.  on = 1 runs the levels that fit the LDS of one
  * workgroup (with every level below them) as one single-block kernel per
- * V-cycle; on = 2 launches each pass of those levels on its own (the same
- * arithmetic and bits; kept for comparison).  No counterpart in the
+ * V-cycle, the levels of at most 2^17 cells as one kernel per leg and the
+ * finest-level transfers as marching kernels; on = 2 launches each pass on
+ * its own, one thread per cell (the same arithmetic and bits; kept for
+ * comparison).  No counterpart in the
  * reference. */
 int dcm_set_precond_multigrid(dcm_ctx *ctx, int on);
 #define DCM_MPDATA_LINKED 0

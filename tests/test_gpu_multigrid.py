@@ -220,8 +220,9 @@ def test_ensemble_with_multigrid_members_evolves_them_as_alone():
 
 def test_multigrid_iteration_graph_replays_match_eager_launches():
     """The single-system multigrid inner iteration is replayed from a HIP
-    graph; with per-kernel timing on (profile) the same kernels are launched
-    one by one.  Both give the same bits after every step."""
+    graph and the advection runs on its own stream beside the solve; with
+    per-kernel timing on (profile) the same kernels are launched one by one
+    on one stream.  Both give the same bits after every step."""
     case = W.moist_bubble(32, 32, 16, delta=50.0, dt=0.5, steps=6, ntracer=1, seed=21)
     ctxs = [H.Context(case.dom), H.Context(case.dom)]
     try:

@@ -40,12 +40,16 @@
 //                smoother 96 times instead.
 // Launch granularity: a level of a few thousand cells takes as long to
 // launch as to sweep, so every level from the first that fits the LDS of
-// one workgroup (with all the levels below it) down to the coarsest runs in
-// ONE kernel of one 1024-thread block (k_mg_tail): each pass of the cycle
-// over those levels is a loop over the level's cells followed by a
-// workgroup barrier, iterates, right-hand sides and 1-D coefficients held
-// in LDS.  At 256x256x128 that is levels 3-5 in FP32 (levels 4-5 in FP64):
-// 14 launches per V-cycle become one, with the same arithmetic.
+// one workgroup (with all the levels below it), and has at most two cells
+// per thread, down to the coarsest runs in ONE kernel of one 256-thread
+// block (k_mg_tail): each pass of the cycle over those levels is a loop
+// over the level's cells followed by a workgroup barrier, iterates,
+// right-hand sides and 1-D coefficients held in LDS.  At 256x256x128 that
+// is the levels of 256 and 32 cells: their 7 launches per V-cycle become
+// one, with the same arithmetic.  The block is small on purpose: beside
+// the advection kernels of the model step, which fill the CUs, a 256-thread
+// block finds a free slot sooner than a 1024-thread one (the step 5.90 ->
+// 5.84 ms, with the 2048-cell level moved to the fused legs).
 // The kernels and the hierarchy exist in FP32 (the mixed path's inner solve)
 // and FP64 (the FP64 BiCGSTAB with the V-cycle in place of the reference's
 // sweeps, which separates the precision gain from the preconditioner gain).
@@ -633,7 +637,7 @@ __global__ __launch_bounds__(256) void k_mg_up(const R *__restrict__ xc,
 // result goes back there (for the prolongation into the level above);
 // everything else stays in LDS.
 // ---------------------------------------------------------------------------
-constexpr int kTailThreads = 1024;
+constexpr int kTailThreads = 256;
 constexpr int kTailMax = 8;  // levels in the tail
 
 template <typename R> struct TailArgs {
@@ -932,9 +936,12 @@ bool mg_possible(const Geo &g) {
 // workgroup, with at most
 // kTailMax levels and at most 2 cells per thread on the first (a level of
 // 16 cells per thread was measured slower in one block than as six
-// launches: each pass is a chain of dependent loads per cell).
+// launches: each pass is a chain of dependent loads per cell), and a
+// coarsest level of at most one cell per thread (its solve is one cell per
+// thread); else no tail.
 template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
     V.tail = 0;
+    if (V.lv[nlev - 1].n > kTailThreads) return DCM_OK;
     int dev = 0, lds_max = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=

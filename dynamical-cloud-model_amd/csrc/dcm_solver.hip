@@ -1440,12 +1440,28 @@ struct IterLaunch {
     // inner solve, or the FP64 BiCGSTAB with the V-cycle in place of the
     // reference's sweeps
     bool mg() const { return one() && c->mg_on && c->mg; }
-    // the V-cycle applied to the right-hand side in ti, into oi: the first
-    // sweep from zero is the prep kernel's ti/a_p in VWB; one more sweep, the
-    // coarse-grid correction, two sweeps (the last into oi; P: p = x + bet*p)
-    template <bool P> void vcycle(int ti, int oi) const {
+    // the right-hand side of a V-cycle, ti = r - bet*(omg*v1) (MODE 0) or
+    // r - alf*v1 (MODE 1), and the V-cycle's first two level-0 sweeps into
+    // VWA: the first from zero, ti/a_p, the second damped.  FP32 in the
+    // fused mode: one marching kernel that evaluates ti/a_p at the centre
+    // and the six neighbours (the arithmetic of the two kernels below, which
+    // write ti/a_p to VWB and sweep from it, so the same bits).
+    template <int MODE> void vcycle_start(int ti) const {
         const R w = (R)kMgOmega;
+        if (sizeof(R) == 4 && c->mg_tail) {
+            first_reg<MODE, true>(ti, R(1), w);
+            return;
+        }
+        hipLaunchKernelGGL((k_jac_prep<R, MODE>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
+                           (const R *)v(VR), (const R *)v(VV1), v(ti), v(VWB), M().m, M().cp,
+                           (const SolverScalars *)M().sc);
         jac_launch<false, true>("jac", VWB, ti, VWA, w);
+    }
+    // the rest of the V-cycle on the right-hand side in ti, from the second
+    // sweep's result in VWA: the coarse-grid correction and two sweeps (the
+    // last into oi; P: p = x + bet*p)
+    template <bool P> void vcycle_rest(int ti, int oi) const {
+        const R w = (R)kMgOmega;
         const int tok = prof_begin(c, "mg_correct");
         (void)mg_correct<R>(c, (const R *)v(ti), v(VWA), (const SolverScalars *)M().sc);
         prof_end(c, tok);
@@ -1456,17 +1472,12 @@ struct IterLaunch {
     // the Jacobi sweeps: p = V(t) + bet*p, cs = V(s)
     void iteration_mg() const {
         {
-            const SolverScalars *sc = (const SolverScalars *)M().sc;
-            hipLaunchKernelGGL((k_jac_prep<R, 0>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
-                               (const R *)v(VR), (const R *)v(VV1), v(VT), v(VWB), M().m,
-                               M().cp, sc);  // t = r - bet*(omg*v1), t/a_p
-            vcycle<true>(VT, VP);            // p = V(t) + bet*p
+            vcycle_start<0>(VT);             // t = r - bet*(omg*v1), two sweeps
+            vcycle_rest<true>(VT, VP);       // p = V(t) + bet*p
             matvec<false>(VP, VCR, VV1);     // v1 = A p, <cr,v1>
             fin(0);
-            hipLaunchKernelGGL((k_jac_prep<R, 1>), dim3(P.mgrid), dim3(kBlock), 0, c->stream,
-                               (const R *)v(VR), (const R *)v(VV1), v(VS), v(VWB), M().m,
-                               M().cp, sc);  // s = r - alf*v1, s/a_p
-            vcycle<false>(VS, VCS);          // cs = V(s)
+            vcycle_start<1>(VS);             // s = r - alf*v1, two sweeps
+            vcycle_rest<false>(VS, VCS);     // cs = V(s)
             matvec<true>(VCS, VS, VT);       // t = A cs, <t,t>, <t,s>
             fin(1);
             update();                        // x, r; <r,r>, <r,cr>

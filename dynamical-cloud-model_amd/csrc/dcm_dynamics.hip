@@ -275,25 +275,23 @@ __global__ void k_axpy_dt(double *a, const double *s, double dt, long long n) {
         a[t] = a[t] + s[t] * dt;
 }
 
-// time filter (driver_code.f90:618-626) on whole arrays.  The filtered
-// previous level is written only when WRITE_Z (the model drops it: that
-// buffer becomes the next future level, whose every cell is rewritten before
-// it is read).
-template <bool WRITE_Z>
-__global__ void k_filter(double *u, double *zu, double *v, double *zv, double *w, double *zw,
-                         long long n) {
+// time filter (driver_code.f90:618-626) on whole arrays: the filtered
+// current level 0.99 u + 0.01 zu is written over the previous level's
+// buffer, which the rotation keeps as the next step's previous level (the
+// reference's zu => ut); the current level's buffer, left as it was, becomes
+// the next future level (the reference's tu => zut: every cell of it is
+// rewritten before it is read, so the reference's filtered zu is never
+// needed).  The current level is only read, so the filter may run while the
+// advection still reads it.
+__global__ void k_filter(const double *u, double *zu, const double *v, double *zv,
+                         const double *w, double *zw, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x) {
         const double a = u[t], b = v[t], cc = w[t];
         const double za = zu[t], zb = zv[t], zc = zw[t];
-        u[t] = 0.99 * a + 0.01 * za;
-        v[t] = 0.99 * b + 0.01 * zb;
-        w[t] = 0.99 * cc + 0.01 * zc;
-        if (WRITE_Z) {
-            zu[t] = 0.01 * a + 0.99 * za;
-            zv[t] = 0.01 * b + 0.99 * zb;
-            zw[t] = 0.01 * cc + 0.99 * zc;
-        }
+        zu[t] = 0.99 * a + 0.01 * za;
+        zv[t] = 0.99 * b + 0.01 * zb;
+        zw[t] = 0.99 * cc + 0.01 * zc;
     }
 }
 
@@ -412,9 +410,9 @@ __global__ void k_bc_roundtrip(const double *m, double lbc, double ubc, int *bad
 
 namespace dcm {
 // a captured phase of the step (the kernels before or after the pressure
-// solve), valid while the key it was captured for -- the time-level phase
-// n mod 3, precision, summation order, MPDATA variant and the model's work
-// generation of every model it covers -- is unchanged
+// solve), valid while the key it was captured for -- n mod 2 (the buffers
+// of the time levels), precision, summation order, MPDATA variant and the
+// model's work generation of every model it covers -- is unchanged
 struct PhaseGraph {
     std::vector<int> key;
     hipGraphExec_t exec = nullptr;
@@ -478,9 +476,12 @@ struct dcm_model {
     double *thetan() { return oned + (c->geo.kp + 2); }
     double *thbar() { return oned + 2 * (c->geo.kp + 2); }
     // driver_code.f90:634-646: after n steps
+    // the buffers of the three levels at step n: the previous level stays in
+    // buffer 1 (the filter writes the filtered current level there), the
+    // current and future levels swap buffers 0 and 2 every step
     void levels(double **cur, double **prev, double **fut) {
-        static const int map[3][3] = {{0, 1, 2}, {2, 0, 1}, {1, 2, 0}};  // [n%3][cur,prev,fut]
-        const int r = n % 3;
+        static const int map[2][3] = {{0, 1, 2}, {2, 1, 0}};  // [n%2][cur,prev,fut]
+        const int r = n % 2;
         for (int d = 0; d < 3; ++d) {
             cur[d] = lev[d][map[r][0]];
             prev[d] = lev[d][map[r][1]];
@@ -1151,26 +1152,34 @@ static int model_momentum(dcm_model *m) {
                                 prev[2], m->su, m->sv, m->sw, m->p);
 }
 
-// The rest of the step after the advection and the momentum update: scalar
-// update, filter, rotation and halo rows (driver_code.f90:601-674).
-static int model_after_advect(dcm_model *m, dcm_step_stats *stats) {
+// The scalar update of the sources options (advance_scalar_fields_3d,
+// driver_code.f90:601-603), after the advection.
+static int model_scalars(dcm_model *m) {
+    if (!m->phys_on) return DCM_OK;
+    dcm_ctx *c = m->c;
+    const long long n = c->geo.ntot;
+    hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->th, m->sth, m->dt,
+                       n);
+    if (m->sq && m->nq > 0)
+        hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->q, m->sq,
+                           m->dt, n * m->nq);
+    DCM_LAUNCH_CHECK();
+    return DCM_OK;
+}
+
+// The velocities' end of the step: filter, rotation and halo rows
+// (driver_code.f90:618-674).  It reads the current level and writes the
+// previous and (after the rotation) the new current level's halos, none of
+// which the advection reads.
+static int model_filter_rotate(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
     const Geo &g = c->geo;
     double *cur[3], *prev[3], *fut[3];
     m->levels(cur, prev, fut);
-    if (m->phys_on) {  // advance_scalar_fields_3d, :601-603
-        const long long n = g.ntot;
-        hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->th, m->sth,
-                           m->dt, n);
-        if (m->sq && m->nq > 0)
-            hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(kBlock), 0, c->stream, m->q, m->sq,
-                               m->dt, n * m->nq);
-        DCM_LAUNCH_CHECK();
-    }
     {
         int tok = prof_begin(c, "filter");
-        hipLaunchKernelGGL(k_filter<false>, dim3(4096), dim3(kBlock), 0, c->stream, cur[0],
-                           prev[0], cur[1], prev[1], cur[2], prev[2], g.ntot);  // :618-626
+        hipLaunchKernelGGL(k_filter, dim3(4096), dim3(kBlock), 0, c->stream, cur[0], prev[0],
+                           cur[1], prev[1], cur[2], prev[2], g.ntot);  // :618-626
         DCM_LAUNCH_CHECK();
         prof_end(c, tok);
     }
@@ -1187,7 +1196,8 @@ static int model_after_advect(dcm_model *m, dcm_step_stats *stats) {
 static int model_post_solve(dcm_model *m, dcm_step_stats *stats) {
     DCM_CHECK(model_advect(m));
     DCM_CHECK(model_momentum(m));
-    return model_after_advect(m, stats);
+    DCM_CHECK(model_scalars(m));
+    return model_filter_rotate(m, stats);
 }
 
 // advection_scheme 2: the reference leaves thbase and thtop as (bc - m) + m
@@ -1260,7 +1270,7 @@ static int run_phase(hipStream_t stream, PhaseGraph *slots, const std::vector<in
 
 // what the kernels of a captured phase and their arguments depend on
 static std::vector<int> phase_key(const dcm_model *m) {
-    return {m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
+    return {m->n % 2 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
             12 * m->c->mpdata_variant + 24 * m->gen};
 }
 
@@ -1325,24 +1335,27 @@ static int model_step_once(dcm_model *m, dcm_step_stats *stats) {
     else if (m->check_residual)
         DCM_CHECK(dcm_true_residual(c, m->p, m->psrc, &true_resid));
     add_stats(stats, so, true_resid);
+    // the host part of a phase with the rotation, on a replay: the level
+    // rotation and the step count
+    auto rotate = [&] {
+        m->n += 1;
+        if (stats) stats->steps += 1;
+    };
     if (overlap) {
-        // the momentum update needs only the solve; the rest waits for the
-        // advection
-        DCM_CHECK(model_phase(m, 3, [&] { return model_momentum(m); }, [] {}));
+        // the momentum update, the filter and the rotation need only the
+        // solve; the scalar update (and the next step) wait for the advection
+        DCM_CHECK(model_phase(
+            m, 3,
+            [&] {
+                DCM_CHECK(model_momentum(m));
+                return model_filter_rotate(m, stats);
+            },
+            rotate));
         DCM_HIP(hipStreamWaitEvent(c->stream, m->adv_join, 0));
+        if (!m->phys_on) return DCM_OK;
+        return model_phase(m, 1, [&] { return model_scalars(m); }, [] {}, 1);
     }
-    // the host part of the phase on a replay: the level rotation and the
-    // step count
-    return model_phase(m, 1,
-                       [&] {
-                           return overlap ? model_after_advect(m, stats)
-                                          : model_post_solve(m, stats);
-                       },
-                       [&] {
-                           m->n += 1;
-                           if (stats) stats->steps += 1;
-                       },
-                       overlap ? 1 : 0);
+    return model_phase(m, 1, [&] { return model_post_solve(m, stats); }, rotate);
 }
 
 int dcm_model_step(dcm_model *m, int nsteps, dcm_step_stats *stats) {
@@ -1395,7 +1408,7 @@ int dcm_model_set_check_residual(dcm_model *m, int on) {
 // converged), so the iteration loop costs one set of launches per iteration
 // for the whole ensemble.  The stages before and after the solve (a few
 // dozen small kernels per member) are captured into HIP graphs, one per
-// phase of the three-level time rotation, and replayed from then on.
+// buffer arrangement of the time levels (n mod 2), and replayed from then on.
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
@@ -1691,7 +1704,7 @@ int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats) {
         for (int q = 0; q < n; ++q) {
             // what the captured kernels and their arguments depend on
             const dcm_model *m = e->mem[q];
-            key[q] = m->n % 3 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
+            key[q] = m->n % 2 + 3 * (m->precision == DCM_MIXED) + 6 * (m->c->seq_sums ? 1 : 0) +
                      12 * m->c->mpdata_variant + 24 * m->gen;
             graphs = graphs && !m->c->profiling;
         }

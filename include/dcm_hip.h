@@ -402,8 +402,8 @@ int dcm_ensemble_destroy(dcm_ensemble *e);
 int dcm_ensemble_autotune(dcm_ensemble *e, int *march_planes);
 /* Advance every member nsteps steps; stats (may be NULL) is an array of n.
  * From the second step on, the stages before and after the pressure solve
- * are replayed from HIP graphs captured on first use (one per phase of the
- * three-level time rotation), unless graphs are switched off or a member
+ * are replayed from HIP graphs captured on first use (one per buffer
+ * arrangement of the time levels), unless graphs are switched off or a member
  * is being profiled. */
 int dcm_ensemble_step(dcm_ensemble *e, int nsteps, dcm_step_stats *stats);
 /* Graph capture of the step's stages on (default) or off; off releases the

@@ -248,6 +248,68 @@ __global__ __launch_bounds__(256) void k_advance_momentum(
     w[c] = zw[c] + ((sw[c] - (p[c + 1] - pc) / M.dzn[k]) * dt) / M.rhoa[k];
 }
 
+// advance_momentum and the time filter (driver_code.f90:618-626) of the same
+// cells in one pass: fut = prev + dt (s - grad p)/rho as k_advance_momentum,
+// then prev = 0.99 cur + 0.01 prev from the previous level the momentum
+// update has just read.  The filtered current level goes over the previous
+// level's buffer, which the rotation keeps as the next step's previous level
+// (the reference's zu => ut); the current level's buffer, left as it was,
+// becomes the next future level (the reference's tu => zut: every cell of it
+// is rewritten before it is read, so the reference's filtered zu is never
+// needed).  The current level is only read, so this may run while the
+// advection still reads it.  k_filter_shell filters the padded cells outside
+// k = 0..kp, interior (j,i).
+__global__ __launch_bounds__(256) void k_momentum_filter(
+    double *u, double *v, double *w, double *zu, double *zv, double *zw, const double *cu,
+    const double *cv, const double *cw, const double *su, const double *sv, const double *sw,
+    const double *p, Geo g, Metrics M, double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 1;
+    if (k > g.kp || j > g.jp) return;
+    const long long c = pidx(g, k, j, i);
+    const double pc = p[c];
+    const double za = zu[c], zb = zv[c], zc = zw[c];
+    u[c] = za + ((su[c] - (p[c + g.si] - pc) / M.dxn[i]) * dt) / M.rhoan[k];
+    v[c] = zb + ((sv[c] - (p[c + g.sj] - pc) / M.dyn[j]) * dt) / M.rhoan[k];
+    w[c] = zc + ((sw[c] - (p[c + 1] - pc) / M.dzn[k]) * dt) / M.rhoa[k];
+    zu[c] = 0.99 * cu[c] + 0.01 * za;
+    zv[c] = 0.99 * cv[c] + 0.01 * zb;
+    zw[c] = 0.99 * cw[c] + 0.01 * zc;
+}
+
+// the time filter of k_momentum_filter on the padded cells it does not
+// cover: the plane k = kp+1, the rows j = 0 and jp+1 and the planes i = 0
+// and ip+1 (levels 0..kp)
+__global__ void k_filter_shell(const double *u, double *zu, const double *v, double *zv,
+                               const double *w, double *zw, Geo g) {
+    const long long K = g.kp + 1;
+    const long long A = (long long)g.nj * g.ni, B = 2 * K * g.ni, C = 2 * K * g.jp;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A + B + C;
+         t += (long long)gridDim.x * blockDim.x) {
+        int k, j, i;
+        if (t < A) {
+            k = g.kp + 1;
+            j = (int)(t % g.nj);
+            i = (int)(t / g.nj);
+        } else if (t < A + B) {
+            const long long e = t - A, r = e / K;
+            k = (int)(e - r * K);
+            j = (r & 1) ? g.jp + 1 : 0;
+            i = (int)(r >> 1);
+        } else {
+            const long long e = t - A - B, r = e / K;
+            k = (int)(e - r * K);
+            j = (int)(r >> 1) + 1;
+            i = (r & 1) ? g.ip + 1 : 0;
+        }
+        const long long c = pidx(g, k, j, i);
+        zu[c] = 0.99 * u[c] + 0.01 * zu[c];
+        zv[c] = 0.99 * v[c] + 0.01 * zv[c];
+        zw[c] = 0.99 * w[c] + 0.01 * zw[c];
+    }
+}
+
 // divergence_calc (diagnostics.f90:140-148) on the interior
 __global__ __launch_bounds__(256) void k_divergence(const double *u, const double *v,
                                                     const double *w, double *div, Geo g,
@@ -273,26 +335,6 @@ __global__ void k_axpy_dt(double *a, const double *s, double dt, long long n) {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (long long)gridDim.x * blockDim.x)
         a[t] = a[t] + s[t] * dt;
-}
-
-// time filter (driver_code.f90:618-626) on whole arrays: the filtered
-// current level 0.99 u + 0.01 zu is written over the previous level's
-// buffer, which the rotation keeps as the next step's previous level (the
-// reference's zu => ut); the current level's buffer, left as it was, becomes
-// the next future level (the reference's tu => zut: every cell of it is
-// rewritten before it is read, so the reference's filtered zu is never
-// needed).  The current level is only read, so the filter may run while the
-// advection still reads it.
-__global__ void k_filter(const double *u, double *zu, const double *v, double *zv,
-                         const double *w, double *zw, long long n) {
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const double a = u[t], b = v[t], cc = w[t];
-        const double za = zu[t], zb = zv[t], zc = zw[t];
-        zu[t] = 0.99 * a + 0.01 * za;
-        zv[t] = 0.99 * b + 0.01 * zb;
-        zw[t] = 0.99 * cc + 0.01 * zc;
-    }
 }
 
 // plain copy, two doubles per thread (measurement yardstick)
@@ -381,6 +423,28 @@ int sources_dev(dcm_ctx *c, const double *u, const double *v, const double *w, c
     Interior R = interior_launch(g, 2);
     tok = prof_begin(c, "rhs");
     hipLaunchKernelGGL(k_rhs, R.grid, R.block, 0, c->stream, su, sv, sw, rhs, g, c->met);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    return DCM_OK;
+}
+
+// the model's momentum update with the time filter of the same step
+// (k_momentum_filter, k_filter_shell): fut from prev, then prev = the
+// filtered current level
+int momentum_filter_dev(dcm_ctx *c, double dt, double *const fut[3], double *const prev[3],
+                        const double *const cur[3], const double *su, const double *sv,
+                        const double *sw, const double *p) {
+    const Geo &g = c->geo;
+    Interior L = interior_launch(g, 1);
+    int tok = prof_begin(c, "advance_momentum");
+    hipLaunchKernelGGL(k_momentum_filter, L.grid, L.block, 0, c->stream, fut[0], fut[1], fut[2],
+                       prev[0], prev[1], prev[2], cur[0], cur[1], cur[2], su, sv, sw, p, g,
+                       c->met, dt);
+    DCM_LAUNCH_CHECK();
+    prof_end(c, tok);
+    tok = prof_begin(c, "filter");
+    hipLaunchKernelGGL(k_filter_shell, dim3(256), dim3(kBlock), 0, c->stream, cur[0], prev[0],
+                       cur[1], prev[1], cur[2], prev[2], g);
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
     return DCM_OK;
@@ -1145,11 +1209,13 @@ static int model_advect(dcm_model *m) {
 // The momentum update (driver_code.f90:592-594): the new level from the
 // previous one, the sources and the pressure; it reads nothing the
 // advection writes.
+// advance_momentum (dynamics.f90:1053-1094) and the time filter
+// (driver_code.f90:618-626) in one pass; the filter only reads the current
+// level, which the advection reads too
 static int model_momentum(dcm_model *m) {
     double *cur[3], *prev[3], *fut[3];
     m->levels(cur, prev, fut);
-    return advance_momentum_dev(m->c, 2.0 * m->dt, fut[0], fut[1], fut[2], prev[0], prev[1],
-                                prev[2], m->su, m->sv, m->sw, m->p);
+    return momentum_filter_dev(m->c, 2.0 * m->dt, fut, prev, cur, m->su, m->sv, m->sw, m->p);
 }
 
 // The scalar update of the sources options (advance_scalar_fields_3d,
@@ -1167,22 +1233,13 @@ static int model_scalars(dcm_model *m) {
     return DCM_OK;
 }
 
-// The velocities' end of the step: filter, rotation and halo rows
-// (driver_code.f90:618-674).  It reads the current level and writes the
-// previous and (after the rotation) the new current level's halos, none of
-// which the advection reads.
+// The velocities' end of the step after the momentum update and the filter:
+// the rotation and the halos of the new current level (driver_code.f90:
+// 634-674), which the advection does not read.
 static int model_filter_rotate(dcm_model *m, dcm_step_stats *stats) {
     dcm_ctx *c = m->c;
     const Geo &g = c->geo;
     double *cur[3], *prev[3], *fut[3];
-    m->levels(cur, prev, fut);
-    {
-        int tok = prof_begin(c, "filter");
-        hipLaunchKernelGGL(k_filter, dim3(4096), dim3(kBlock), 0, c->stream, cur[0], prev[0],
-                           cur[1], prev[1], cur[2], prev[2], g.ntot);  // :618-626
-        DCM_LAUNCH_CHECK();
-        prof_end(c, tok);
-    }
     m->n += 1;  // :634-646
     m->levels(cur, prev, fut);
     for (int d = 0; d < 3; ++d) DCM_CHECK(fill<double>(c, cur[d], FILL_FULL, 0.0, 0.0));

@@ -164,9 +164,20 @@ __global__ void k_level_sums(const double *a, Geo g, double *partials) {
     const long long c0 = (long long)blockIdx.y * kHmCols;
     double acc = 0.0;
     if (k < g.nk) {
-        for (long long col = c0 + threadIdx.y; col < c0 + kHmCols && col < ncol; col += 4) {
-            const int j = (int)(col % g.jp) + 1, i = (int)(col / g.jp) + 1;
+        // column col = (i-1)*jp + (j-1): one division for the first, then
+        // steps of 4 columns with the wrap into the next i
+        const long long cfirst = c0 + threadIdx.y;
+        const long long cend = c0 + kHmCols < ncol ? c0 + kHmCols : ncol;
+        int i = (int)(cfirst / g.jp) + 1;
+        int j = (int)(cfirst - (long long)(i - 1) * g.jp) + 1;
+#pragma unroll 8
+        for (long long col = cfirst; col < cend; col += 4) {
             acc += a[pidx(g, k, j, i)];
+            j += 4;
+            while (j > g.jp) {
+                j -= g.jp;
+                ++i;
+            }
         }
     }
     red[threadIdx.y][threadIdx.x] = acc;
@@ -180,6 +191,8 @@ __global__ void k_level_means(const double *partials, int nchunks, Geo g, double
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.nk) return;
     double s = 0.0;
+    // in chunk order; unrolled so that the loads are in flight together
+#pragma unroll 16
     for (int q = 0; q < nchunks; ++q) s += partials[(long long)q * g.nk + k];
     out[k] = s / (double)(g.ip * g.jp);
 }

@@ -1255,7 +1255,11 @@ static int model_filter_rotate(dcm_model *m, dcm_step_stats *stats) {
     double *cur[3], *prev[3], *fut[3];
     m->n += 1;  // :634-646
     m->levels(cur, prev, fut);
-    for (int d = 0; d < 3; ++d) DCM_CHECK(fill<double>(c, cur[d], FILL_FULL, 0.0, 0.0));
+    {  // exchange_full of u, v and w (:651-656), one launch
+        FillSet<double> fs{};
+        for (int d = 0; d < 3; ++d) fs.a[d] = cur[d];
+        DCM_CHECK(fill_set<double>(c, fs, 3, FILL_FULL, 0.0, 0.0));
+    }
     hipLaunchKernelGGL(k_momentum_rows, dim3(256), dim3(kBlock), 0, c->stream, cur[0], cur[1],
                        cur[2], g);  // :663-674
     DCM_LAUNCH_CHECK();

@@ -117,10 +117,9 @@ __global__ __launch_bounds__(256) void k_sources(const double *__restrict__ u,
 
 // floor and lid rows of the sources (dynamics.f90:922-973), whole planes
 __global__ void k_sources_rows(double *su, double *sv, double *sw, Geo g) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;  // (32-bit index arithmetic: a plane is far below 2^31)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         const long long c0 = pidx(g, 0, j, i);
         su[c0] = -su[c0 + 1];
         sv[c0] = -sv[c0 + 1];
@@ -296,24 +295,26 @@ __global__ __launch_bounds__(256) void k_momentum_filter(
 // and ip+1 (levels 0..kp)
 __global__ void k_filter_shell(const double *u, double *zu, const double *v, double *zv,
                                const double *w, double *zw, Geo g) {
-    const long long K = g.kp + 1;
-    const long long A = (long long)g.nj * g.ni, B = 2 * K * g.ni, C = 2 * K * g.jp;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A + B + C;
-         t += (long long)gridDim.x * blockDim.x) {
+    // 32-bit index arithmetic (unsigned divisions): the shell has far fewer
+    // than 2^31 cells
+    const int K = g.kp + 1;
+    const int A = g.nj * g.ni, B = 2 * K * g.ni, C = 2 * K * g.jp;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < A + B + C;
+         t += gridDim.x * blockDim.x) {
         int k, j, i;
         if (t < A) {
             k = g.kp + 1;
-            j = (int)(t % g.nj);
-            i = (int)(t / g.nj);
+            i = (int)((unsigned)t / (unsigned)g.nj);
+            j = t - i * g.nj;
         } else if (t < A + B) {
-            const long long e = t - A, r = e / K;
-            k = (int)(e - r * K);
+            const int e = t - A, r = (int)((unsigned)e / (unsigned)K);
+            k = e - r * K;
             j = (r & 1) ? g.jp + 1 : 0;
-            i = (int)(r >> 1);
+            i = r >> 1;
         } else {
-            const long long e = t - A - B, r = e / K;
-            k = (int)(e - r * K);
-            j = (int)(r >> 1) + 1;
+            const int e = t - A - B, r = (int)((unsigned)e / (unsigned)K);
+            k = e - r * K;
+            j = (r >> 1) + 1;
             i = (r & 1) ? g.ip + 1 : 0;
         }
         const long long c = pidx(g, k, j, i);
@@ -363,10 +364,9 @@ __global__ void k_stream_copy(double *dst, const double *src, long long n) {
 
 // floor / lid rows of the new current level (driver_code.f90:663-674)
 __global__ void k_momentum_rows(double *u, double *v, double *w, Geo g) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;  // (32-bit index arithmetic: a plane is far below 2^31)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         const long long c0 = pidx(g, 0, j, i), ck = pidx(g, g.kp + 1, j, i);
         w[c0] = 0.0;
         u[c0] = -u[c0 + 1];
@@ -379,20 +379,18 @@ __global__ void k_momentum_rows(double *u, double *v, double *w, Geo g) {
 
 // initial previous-level rows (driver_code.f90:287-295)
 __global__ void k_init_rows(double *zu, double *zv, double *zw, Geo g) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;  // (32-bit index arithmetic: a plane is far below 2^31)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         zw[pidx(g, 0, j, i)] = -zw[pidx(g, 1, j, i)];
         zu[pidx(g, 1, j, i)] = 0.0;
         zv[pidx(g, 1, j, i)] = 0.0;
     }
 }
 __global__ void k_init_rows_top(double *zu, double *zv, double *zw, Geo g) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;  // (32-bit index arithmetic: a plane is far below 2^31)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         zw[pidx(g, g.kp, j, i)] = -zw[pidx(g, g.kp - 1, j, i)];
         zw[pidx(g, g.kp + 1, j, i)] = 0.0;
         zu[pidx(g, g.kp, j, i)] = 0.0;
@@ -456,8 +454,9 @@ int momentum_filter_dev(dcm_ctx *c, double dt, double *const fut[3], double *con
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
     tok = prof_begin(c, "filter");
-    hipLaunchKernelGGL(k_filter_shell, dim3(256), dim3(kBlock), 0, c->stream, cur[0], prev[0],
-                       cur[1], prev[1], cur[2], prev[2], g);
+    const long long shell = (long long)g.nj * g.ni + 2LL * (g.kp + 1) * (g.ni + g.jp);
+    hipLaunchKernelGGL(k_filter_shell, dim3(blocks_for(shell)), dim3(kBlock), 0, c->stream,
+                       cur[0], prev[0], cur[1], prev[1], cur[2], prev[2], g);
     DCM_LAUNCH_CHECK();
     prof_end(c, tok);
     return DCM_OK;

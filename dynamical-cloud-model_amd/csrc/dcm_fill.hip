@@ -22,33 +22,34 @@ __global__ void k_fill(FillSet<T> set, Geo g, int kind, double lbc, double ubc,
                        const double *shift, int mode, const int *skip_if) {
     if (skip_if && *skip_if) return;
     T *a = set.a[blockIdx.y];  // one array per grid row
-    const long long P = (long long)g.nj * g.ni;
-    const long long R = 2LL * g.ni + 2LL * g.jp;
-    const long long nz = 2 * P;
-    const long long n = nz + (long long)g.kp * R;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
+    // 32-bit index arithmetic (unsigned divisions): the halo shell of any
+    // grid whose padded planes have fewer than 2^30 cells
+    const int P = g.nj * g.ni;
+    const int R = 2 * g.ni + 2 * g.jp;
+    const int nz = 2 * P;
+    const int n = nz + g.kp * R;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         int k, j, i;
         bool zplane;
         if (t < nz) {
-            const int plane = (int)(t / P);
-            const long long rem = t - plane * P;
-            j = (int)(rem % g.nj);
-            i = (int)(rem / g.nj);
+            const int plane = t >= P ? 1 : 0;
+            const int rem = t - plane * P;
+            i = (int)((unsigned)rem / (unsigned)g.nj);
+            j = rem - i * g.nj;
             k = plane ? g.kp + 1 : 0;
             zplane = true;
         } else {
-            const long long tt = t - nz;
-            const long long r = tt / g.kp;
-            k = (int)(tt - r * g.kp) + 1;
+            const int tt = t - nz;
+            const int r = (int)((unsigned)tt / (unsigned)g.kp);
+            k = tt - r * g.kp + 1;
             if (r < g.ni) {
                 j = 0;
-                i = (int)r;
-            } else if (r < 2LL * g.ni) {
+                i = r;
+            } else if (r < 2 * g.ni) {
                 j = g.jp + 1;
-                i = (int)(r - g.ni);
+                i = r - g.ni;
             } else {
-                const int r2 = (int)(r - 2LL * g.ni);
+                const int r2 = r - 2 * g.ni;
                 if (r2 < g.jp) {
                     i = 0;
                     j = r2 + 1;
@@ -153,10 +154,9 @@ template int fill<float>(dcm_ctx *, float *, FillKind, double, double, const dou
 // x(0,:,:)=x(1,:,:) and x(kp+1,:,:)=x(kp,:,:) over whole planes
 // (mat_ax, dynamics.f90:206-211)
 __global__ void k_neumann_z(double *x, Geo g) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         x[pidx(g, 0, j, i)] = x[pidx(g, 1, j, i)];
         x[pidx(g, g.kp + 1, j, i)] = x[pidx(g, g.kp, j, i)];
     }

@@ -1008,10 +1008,9 @@ __global__ void k_scatter_x(const double *__restrict__ xc, double *__restrict__ 
 // final boundary rows of x (dynamics.f90:437-442), or the Neumann copies left
 // by mat_ax on the early-return path (:383); whole z planes
 __global__ void k_x_rows(double *x, const double *sw, Geo g, double dzn0, int final_rows) {
-    const long long P = (long long)g.nj * g.ni;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < P;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t % g.nj), i = (int)(t / g.nj);
+    const int P = g.nj * g.ni;  // (32-bit index arithmetic: a plane is far below 2^31)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+        const int i = (int)((unsigned)t / (unsigned)g.nj), j = t - i * g.nj;
         x[pidx(g, g.kp + 1, j, i)] = x[pidx(g, g.kp, j, i)];
         if (final_rows)
             x[pidx(g, 0, j, i)] = x[pidx(g, 1, j, i)] - sw[pidx(g, 0, j, i)] * dzn0;

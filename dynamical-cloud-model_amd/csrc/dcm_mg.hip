@@ -49,7 +49,9 @@
 // one, with the same arithmetic.  The block is small on purpose: beside
 // the advection kernels of the model step, which fill the CUs, a 256-thread
 // block finds a free slot sooner than a 1024-thread one (the step 5.90 ->
-// 5.84 ms, with the 2048-cell level moved to the fused legs).
+// 5.84 ms, with the 2048-cell level moved to the fused legs).  A hierarchy
+// whose level of 513..2048 cells cannot run as fused legs (a
+// semi-coarsened pair) takes a 1024-thread block with that level in it.
 // The kernels and the hierarchy exist in FP32 (the mixed path's inner solve)
 // and FP64 (the FP64 BiCGSTAB with the V-cycle in place of the reference's
 // sweeps, which separates the precision gain from the preconditioner gain).
@@ -637,7 +639,8 @@ __global__ __launch_bounds__(256) void k_mg_up(const R *__restrict__ xc,
 // result goes back there (for the prolongation into the level above);
 // everything else stays in LDS.
 // ---------------------------------------------------------------------------
-constexpr int kTailThreads = 256;
+constexpr int kTailThreads = 256;    // the tail's block
+constexpr int kTailThreadsMax = 1024;  // ... where a larger first level pays
 constexpr int kTailMax = 8;  // levels in the tail
 
 template <typename R> struct TailArgs {
@@ -669,7 +672,7 @@ __device__ __forceinline__ Lv<R> tail_level(const TailArgs<R> &A, R *lds, int l)
 // t = w b/a_p (a sweep from zero)
 template <typename R, typename PB, typename PO>
 __device__ __forceinline__ void tail_smooth0(const Lv<R> &L, PB b, PO t, R w) {
-    for (int c = threadIdx.x; c < L.n; c += kTailThreads) {
+    for (int c = threadIdx.x; c < L.n; c += (int)blockDim.x) {
         int k, j, i;
         cell_of(L, c, k, j, i);
         const R inv = inv_of<R>(ap_of(L, k, j, i));
@@ -679,7 +682,7 @@ __device__ __forceinline__ void tail_smooth0(const Lv<R> &L, PB b, PO t, R w) {
 // xo = x + w (b - A x)/a_p
 template <typename R, typename PB, typename PX, typename PO>
 __device__ __forceinline__ void tail_smooth(const Lv<R> &L, PB b, PX x, PO xo, R w) {
-    for (int c = threadIdx.x; c < L.n; c += kTailThreads) {
+    for (int c = threadIdx.x; c < L.n; c += (int)blockDim.x) {
         int k, j, i;
         cell_of(L, c, k, j, i);
         const R ap = ap_of(L, k, j, i);
@@ -692,7 +695,7 @@ template <typename R, typename PB, typename PX, typename PC>
 __device__ __forceinline__ void tail_restrict(const Lv<R> &F, const Lv<R> &C, PB b, PX x,
                                               PC bc) {
     const bool semi = !halves(F, C);  // the same for every thread
-    for (int cc = threadIdx.x; cc < C.n; cc += kTailThreads) {
+    for (int cc = threadIdx.x; cc < C.n; cc += (int)blockDim.x) {
         int K, J, I;
         cell_of(C, cc, K, J, I);
         if (semi) {
@@ -714,7 +717,7 @@ __device__ __forceinline__ void tail_restrict(const Lv<R> &F, const Lv<R> &C, PB
 template <typename R, typename PC, typename PX>
 __device__ __forceinline__ void tail_prolong(const Lv<R> &F, const Lv<R> &C, PC xc, PX x) {
     const bool semi = !halves(F, C);  // the same for every thread
-    for (int c = threadIdx.x; c < F.n; c += kTailThreads) {
+    for (int c = threadIdx.x; c < F.n; c += (int)blockDim.x) {
         int k, j, i;
         cell_of(F, c, k, j, i);
         if (semi) {
@@ -739,7 +742,7 @@ __device__ __forceinline__ void tail_prolong(const Lv<R> &F, const Lv<R> &C, PC 
 }
 
 template <typename R>
-__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int sweeps, R w,
+__global__ __launch_bounds__(kTailThreadsMax) void k_mg_tail(TailArgs<R> A, int sweeps, R w,
                                                           const SolverScalars *sc) {
     extern __shared__ __align__(16) unsigned char tail_lds[];
     R *lds = reinterpret_cast<R *>(tail_lds);
@@ -750,7 +753,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
         const Lv<R> &G = A.lv[l];
         const int ni = G.ip + 2, nj = G.jp + 2, nk = G.kp + 2;
         R *c = lds + A.coef[l];
-        for (int e = threadIdx.x; e < 2 * (ni + nj + nk); e += kTailThreads) {
+        for (int e = threadIdx.x; e < 2 * (ni + nj + nk); e += (int)blockDim.x) {
             R v;
             if (e < ni) v = G.ae[e];
             else if (e < 2 * ni) v = G.aw[e - ni];
@@ -762,10 +765,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs<R> A, int swe
         }
     }
     // the first level's right-hand side and the dense coarsest solve
-    for (int e = threadIdx.x; e < A.lv[0].n; e += kTailThreads) lds[A.bo[0] + e] = A.b0[e];
+    for (int e = threadIdx.x; e < A.lv[0].n; e += (int)blockDim.x)
+        lds[A.bo[0] + e] = A.b0[e];
     if (A.Mt) {
         const int nc = A.lv[nl - 1].n;
-        for (int e = threadIdx.x; e < nc * nc; e += kTailThreads) lds[A.mo + e] = A.Mt[e];
+        for (int e = threadIdx.x; e < nc * nc; e += (int)blockDim.x) lds[A.mo + e] = A.Mt[e];
     }
     __syncthreads();
     const int last = nl - 1;
@@ -871,6 +875,7 @@ template <typename R> struct MgLevels {
     Tile2 tile[16] = {};  // temporal-blocking tile of each level (TI = 0: none)
     size_t tile_lds[16] = {};
     int tail = 0;       // first level of the one-block tail (0: none)
+    int tail_threads = kTailThreads;  // its block
     size_t tail_lds = 0;
     TailArgs<R> targs{};
 };
@@ -938,10 +943,30 @@ bool mg_possible(const Geo &g) {
 // 16 cells per thread was measured slower in one block than as six
 // launches: each pass is a chain of dependent loads per cell), and a
 // coarsest level of at most one cell per thread (its solve is one cell per
-// thread); else no tail.
+// thread); else no tail.  The block has kTailThreads threads, or
+// kTailThreadsMax when that admits a level that fused legs cannot take.
+template <typename R> static bool fusable(const MgLevels<R> &V, int l, int nlev) {
+    if (l < 1 || l >= nlev - 1) return false;
+    const Lv<R> &L = V.lv[l];
+    if (L.kp > 256 || L.n > kFuseMax || !halves(L, V.lv[l + 1])) return false;
+    for (int t : {4, 2}) {
+        const size_t bytes =
+            (3 * (size_t)(t + 4) * (t + 4) * L.kp + 4 * (size_t)(t + 4)) * sizeof(R);
+        if (L.ip % t == 0 && L.jp % t == 0 && bytes <= 64 * 1024) return true;
+    }
+    return false;
+}
 template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
     V.tail = 0;
-    if (V.lv[nlev - 1].n > kTailThreads) return DCM_OK;
+    V.tail_threads = kTailThreads;
+    for (int t0 = 1; t0 < nlev; ++t0)
+        if (V.lv[t0].n > 2 * kTailThreads && V.lv[t0].n <= 2 * kTailThreadsMax &&
+            !fusable(V, t0, nlev)) {
+            V.tail_threads = kTailThreadsMax;
+            break;
+        }
+    const int NT = V.tail_threads;
+    if (V.lv[nlev - 1].n > NT) return DCM_OK;
     int dev = 0, lds_max = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
@@ -951,7 +976,7 @@ template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
     auto up = [&](size_t n) { return (n + pad - 1) / pad * pad; };
     for (int t0 = 1; t0 < nlev; ++t0) {
         const int nl = nlev - t0;
-        if (nl > kTailMax || V.lv[t0].n > 2 * kTailThreads) continue;
+        if (nl > kTailMax || V.lv[t0].n > 2 * NT) continue;
         TailArgs<R> A{};
         size_t off = 0;
         for (int l = 0; l < nl; ++l) {
@@ -974,9 +999,11 @@ template <typename R> static int plan_tail(MgLevels<R> &V, int nlev) {
         A.b0 = V.b[t0];
         A.x0 = V.x[t0];
         A.Mt = V.Mt;
+        // the attribute is the function's, shared by every context: the
+        // device's limit, so that no context lowers another's
         if (hipFuncSetAttribute((const void *)k_mg_tail<R>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes) != hipSuccess)
+                                lds_max) != hipSuccess)
             return DCM_HIP_ERROR;
         V.tail = t0;
         V.tail_lds = bytes;
@@ -996,7 +1023,7 @@ template <typename R> static int plan_tiles(MgLevels<R> &V, int nlev) {
     for (int l = 1; l < nlev - 1; ++l) {
         const Lv<R> &L = V.lv[l];
         V.tile[l] = Tile2{0, 0};
-        if (L.kp > 256 || L.n > kFuseMax || !halves(L, V.lv[l + 1])) continue;
+        if (!fusable(V, l, nlev)) continue;
         for (int t : {4, 2}) {
             const size_t bytes =
                 (3 * (size_t)(t + 4) * (t + 4) * L.kp + 4 * (size_t)(t + 4)) * sizeof(R);
@@ -1009,12 +1036,13 @@ template <typename R> static int plan_tiles(MgLevels<R> &V, int nlev) {
         }
     }
     if (most > 0) {
+        // the bound every tile obeys (the attribute is shared by every context)
         if (hipFuncSetAttribute((const void *)k_mg_down<R>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)most) != hipSuccess ||
+                                64 * 1024) != hipSuccess ||
             hipFuncSetAttribute((const void *)k_mg_up<R>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)most) != hipSuccess)
+                                64 * 1024) != hipSuccess)
             return DCM_HIP_ERROR;
     }
     return DCM_OK;
@@ -1250,8 +1278,8 @@ int mg_correct(dcm_ctx *c, const R *b0, R *x0, const SolverScalars *sc) {
         restrict_cells(l, (const R *)V.b[l], (const R *)V.x[l]);
     }
     if (tail)
-        hipLaunchKernelGGL(k_mg_tail<R>, dim3(1), dim3(kTailThreads), V.tail_lds, c->stream,
-                           V.targs, kCoarseSweeps, w, sc);
+        hipLaunchKernelGGL(k_mg_tail<R>, dim3(1), dim3((unsigned)V.tail_threads), V.tail_lds,
+                           c->stream, V.targs, kCoarseSweeps, w, sc);
     else
         hipLaunchKernelGGL(k_mg_coarse<R>, dim3(1), dim3(kCoarseMax), 0, c->stream,
                            (const R *)V.b[L - 1], V.x[L - 1], V.lv[L - 1], kCoarseSweeps, w, sc,

@@ -276,7 +276,7 @@ def test_fp64_solve_with_the_v_cycle_matches_the_reference_sweeps():
 
 @pytest.mark.parametrize("precision", [H.MIXED, H.FP64])
 @pytest.mark.parametrize("shape,steps", [((64, 64, 32), 4), ((256, 128, 64), 2),
-                                         ((128, 4, 64), 3)])
+                                         ((128, 4, 64), 3), ((256, 4, 64), 2)])
 def test_one_block_tail_matches_per_pass_launches(precision, shape, steps):
     """The levels that fit one workgroup's LDS run as one single-block kernel,
     the levels of at most 2^17 cells as one kernel per leg, and the
@@ -285,8 +285,10 @@ def test_one_block_tail_matches_per_pass_launches(precision, shape, steps):
     cell launch.  64x64x32 has every coarse level fused or in the tail;
     256x128x64 has a first coarse level of 2^18 cells that keeps its
     per-pass kernels; 128x4x64 reaches one cell in y at level 2, and its
-    tail transfers are semi-coarsened.  Same arithmetic, so the same bits
-    after every step."""
+    tail transfers are semi-coarsened; 256x4x64 has a semi-coarsened level
+    of 1024 cells that the fused legs cannot take, so its tail is a
+    1024-thread block.  Same arithmetic, so the same bits after every
+    step."""
     case = W.moist_bubble(*shape, delta=50.0, dt=0.5, steps=steps, ntracer=1, seed=41)
     snaps = {}
     for mode in (1, 2):

@@ -99,6 +99,9 @@ struct SolverScalars {
     int best_it;       // inner solve: iteration of the best residual so far
     double best_err;
     double tol;        // convergence test of this solve (err < tol)
+    int dvalid;        // inner solve: its correction has been updated at least once
+                       // (its start does not store the zero correction)
+    int pad_;
 };
 
 // State of the mixed path's FP64 refinement kept on the device, so that a

@@ -495,6 +495,7 @@ __device__ __forceinline__ void fin_t_apply(SolverScalars *sc, double tt, double
     if (sc->guard && !(tt != 0.0 && isfinite(sc->omg))) sc->breakdown = 1;
 }
 __device__ __forceinline__ void fin_r_apply(SolverScalars *sc, double rr, double rcr) {
+    sc->dvalid = 1;  // the update of this iteration has run
     sc->rr = rr;
     sc->err = sqrt(rr) / sc->sc_err;  // :423
     sc->nrm = sc->rho;                // :425
@@ -605,6 +606,9 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
+    // v1 = 0 in a solve's first iteration (dynamics.f90:380): not read then
+    const bool vz = MODE == 0 && sc->iters == 0;
+    auto V1 = [&](int idx) { return vz ? R(0) : (R)v1[idx]; };
     const int k = T.k, j = T.j;
     const Cols q = cols(m, k, j);
     const int js = j == 1 ? m.jp : j - 1, jn = j == m.jp ? 1 : j + 1;
@@ -619,13 +623,13 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
     int ce = (ie - 1) * m.si + q.base;
     const int cw = (iw - 1) * m.si + q.base;
     auto rw = [&](R x) { return W ? w0 * x : x; };
-    R s0w = rw(div_ap<R>(tv<R, MODE>(r[cw], v1[cw], a, bo), coef_ap(C, k, j, iw)));
-    R tc = tv<R, MODE>(r[cc], v1[cc], a, bo);
+    R s0w = rw(div_ap<R>(tv<R, MODE>(r[cw], V1(cw), a, bo), coef_ap(C, k, j, iw)));
+    R tc = tv<R, MODE>(r[cc], V1(cc), a, bo);
     R s0c = rw(div_ap<R>(tc, coef_ap(C, k, j, T.i0)));
     // r, v1 at the j and k neighbours of plane i, and at the centre of i+1
-    R rs = r[cc + q.ds], vs = v1[cc + q.ds], rn = r[cc + q.dn], vn = v1[cc + q.dn];
-    R rd = r[cc + q.dd], vd = v1[cc + q.dd], ru = r[cc + q.du], vu = v1[cc + q.du];
-    R re = r[ce], ve = v1[ce];
+    R rs = r[cc + q.ds], vs = V1(cc + q.ds), rn = r[cc + q.dn], vn = V1(cc + q.dn);
+    R rd = r[cc + q.dd], vd = V1(cc + q.dd), ru = r[cc + q.du], vu = V1(cc + q.du);
+    R re = r[ce], ve = V1(ce);
     for (int i = T.i0; i <= T.i1; ++i) {
         // loads for plane i+1 in flight while plane i is computed
         R nrs = R(0), nvs = R(0), nrn = R(0), nvn = R(0), nrd = R(0), nvd = R(0), nru = R(0),
@@ -635,15 +639,15 @@ __device__ __forceinline__ void jac_first_reg_body(gp<const R> __restrict__ r,
             nie = ie == m.ip ? 1 : ie + 1;
             nce = (nie - 1) * m.si + q.base;
             nrs = r[ce + q.ds];
-            nvs = v1[ce + q.ds];
+            nvs = V1(ce + q.ds);
             nrn = r[ce + q.dn];
-            nvn = v1[ce + q.dn];
+            nvn = V1(ce + q.dn);
             nrd = r[ce + q.dd];
-            nvd = v1[ce + q.dd];
+            nvd = V1(ce + q.dd);
             nru = r[ce + q.du];
-            nvu = v1[ce + q.du];
+            nvu = V1(ce + q.du);
             nre = r[nce];
-            nve = v1[nce];
+            nve = V1(nce);
         }
         const R te = tv<R, MODE>(re, ve, a, bo);
         const R s0e = rw(div_ap<R>(te, coef_ap(C, k, j, ie)));
@@ -710,11 +714,13 @@ __device__ __forceinline__ void jac_prep_body(gp<const R> __restrict__ r, gp<con
     const Coef<R> C(cp.m64, cp.m32);
     const R a = MODE == 0 ? (R)sc->bet : (R)sc->alf;
     const R bo = (R)sc->omg;
+    // v1 = 0 in a solve's first iteration (dynamics.f90:380): not read then
+    const bool vz = MODE == 0 && sc->iters == 0;
     const int k = T.k, j = T.j;
     const Cols q = cols(m, k, j);
     int cc = (T.i0 - 1) * m.si + q.base;
     for (int i = T.i0; i <= T.i1; ++i, cc += m.si) {
-        const R tc = tval<R, MODE>(r, v1, cc, a, bo);
+        const R tc = vz ? tv<R, MODE>(r[cc], R(0), a, bo) : tval<R, MODE>(r, v1, cc, a, bo);
         t_out[cc] = tc;
         s0_out[cc] = div_ap<R>(tc, coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j));
     }
@@ -753,6 +759,8 @@ __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R>
     const Cols q = cols(m, k, j);
     const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
     const R bet = P_UPDATE ? (R)sc->bet : R(0);
+    // p = 0 in a solve's first iteration (dynamics.f90:379): not read then
+    const bool pread = P_UPDATE && sc->iters != 0;
     const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
     int cc = (T.i0 - 1) * m.si + q.base;
     int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
@@ -761,7 +769,7 @@ __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R>
     R xc = s[cc], xe = s[ce];
     R xs = s[cc + q.ds], xn = s[cc + q.dn], xd = s[cc + q.dd], xu = s[cc + q.du];
     R tc = t[cc];
-    R po = P_UPDATE ? s_out[cc] : R(0);
+    R po = pread ? s_out[cc] : R(0);
     for (int i = T.i0; i <= T.i1; ++i) {
         R nxe = R(0), nxs = R(0), nxn = R(0), nxd = R(0), nxu = R(0), ntc = R(0), npo = R(0);
         int nie = ie, nce = ce;
@@ -774,7 +782,7 @@ __device__ __forceinline__ void jac_body(gp<const R> __restrict__ s, gp<const R>
             nxd = s[ce + q.dd];
             nxu = s[ce + q.du];
             ntc = t[ce];
-            if (P_UPDATE) npo = s_out[ce];
+            if (pread) npo = s_out[ce];
         }
         const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
         const R As = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu, xc);
@@ -913,9 +921,12 @@ __device__ __forceinline__ void update_body(gp<R> __restrict__ x, gp<const R> __
     if (halted(sc)) return;
     const int stride = nbu;
     const R alf = (R)sc->alf, omg = (R)sc->omg;
+    // the correction of a mixed inner solve (guard set) starts at zero: not
+    // read in its first iteration (iters counts it from the <t,t> step on)
+    const bool xz = sc->guard && sc->iters == 1;
     double acc[2] = {0.0, 0.0};
     for (int q = lb * kBlock + threadIdx.x; q < n; q += nbu * kBlock) {
-        x[q] = (x[q] + alf * p[q]) + omg * cs[q];
+        x[q] = ((xz ? R(0) : x[q]) + alf * p[q]) + omg * cs[q];
         const R rv = s[q] - omg * t[q];
         r[q] = rv;
         acc[0] += (double)rv * (double)rv;
@@ -969,6 +980,7 @@ __device__ __forceinline__ void fin_inner_init_body(const double *partials, int 
         sc->tol = rtol;
         sc->best_err = sc->err;
         sc->best_it = 0;
+        sc->dvalid = 0;
         sc->nonfinite = !isfinite(rr);
         sc->done = (rr == 0.0) || (sc->err < rtol) || sc->nonfinite;
     }
@@ -1019,13 +1031,16 @@ __global__ void k_x_rows(double *x, const double *sw, Geo g, double dzn0, int fi
     }
 }
 
-// d (compact FP32) added to the padded FP64 x: x = x + d
-__global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g) {
+// d (compact FP32) added to the padded FP64 x: x = x + d; d = 0 when the
+// inner solve made no update (its start does not store d)
+__global__ void k_add_correction(double *x, const float *d, Geo pg, CGeo g,
+                                 const SolverScalars *sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 1;
     if (k > g.kp || j > g.jp) return;
-    x[pidx(pg, k, j, i)] += (double)d[cidx(g, k, j, i)];
+    const bool none = !sc->dvalid;
+    x[pidx(pg, k, j, i)] += none ? 0.0 : (double)d[cidx(g, k, j, i)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,10 +1068,11 @@ __global__ void k_outer_gate(OuterState *os, const SolverScalars *sc, int force)
     os->go = os->active && (force || halted(sc));
 }
 // x += d with the previous x kept (the interior; the halo is not touched
-// until the solve ends)
+// until the solve ends); d = 0 when the inner solve made no update
 __global__ void k_add_correction_save(double *x, double *xsave, const float *d, Geo pg, CGeo g,
-                                      const OuterState *gate) {
+                                      const OuterState *gate, const SolverScalars *sc) {
     if (!gate->go) return;
+    const bool none = !sc->dvalid;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i0 = blockIdx.z * kResPlanes + 1;
@@ -1068,7 +1084,7 @@ __global__ void k_add_correction_save(double *x, double *xsave, const float *d, 
     for (int i = i0; i <= i1; ++i, c += SI, q += CS) {
         const double xv = x[c];
         xsave[c] = xv;
-        x[c] = xv + (double)d[q];
+        x[c] = xv + (none ? 0.0 : (double)d[q]);
     }
 }
 __global__ void k_restore_interior(double *x, const double *xsave, Geo pg, CGeo g) {
@@ -1124,10 +1140,11 @@ __global__ void k_fin_inner_init_dev(const double *partials, int nb, SolverScala
 }
 
 // inner start: r = float(r64 - mean) (the mean removed in FP64, so that a
-// large mean does not swamp the rest in FP32), cr = r, p = v1 = d = 0; <r,r>
-__global__ void k_inner_init(const double *r64, float *r, float *cr, float *p, float *v1,
-                             float *d, CGeo g, const double *mean_sum, double *partials,
-                             long long stride, const OuterState *gate = nullptr) {
+// large mean does not swamp the rest in FP32), cr = r; <r,r>.  p = v1 = d = 0
+// are not stored: the first iteration's kernels take them as zero (sc->iters)
+__global__ void k_inner_init(const double *r64, float *r, float *cr, CGeo g,
+                             const double *mean_sum, double *partials, long long stride,
+                             const OuterState *gate = nullptr) {
     if (gate && !(gate->go && gate->active)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -1142,9 +1159,6 @@ __global__ void k_inner_init(const double *r64, float *r, float *cr, float *p, f
             const float rv = (float)(r64[c] - mean);
             r[c] = rv;
             cr[c] = rv;
-            p[c] = 0.f;
-            v1[c] = 0.f;
-            d[c] = 0.f;
             s[0] += (double)rv * (double)rv;
         }
     }
@@ -1842,8 +1856,8 @@ struct ChunkedSolve {
         hipLaunchKernelGGL(k_outer_decide, dim3(1), dim3(64), 0, c->stream, (const double *)sums,
                            os, sc, tol, inner_rtol, itmax, kMaxOuter);
         hipLaunchKernelGGL(k_inner_init, RL.grid, RL.block, 0, c->stream, (const double *)r64,
-                           V[0], V[1], V[2], V[3], V[9], g, (const double *)(sums + 1),
-                           c->partials, (long long)nbr, gos);
+                           V[0], V[1], g, (const double *)(sums + 1), c->partials,
+                           (long long)nbr, gos);
         hipLaunchKernelGGL(k_fin_inner_init_dev, dim3(1), dim3(1024), 0, c->stream,
                            (const double *)c->partials, nbr, sc, gos);
     }
@@ -1852,7 +1866,8 @@ struct ChunkedSolve {
         hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, c->outer_d,
                            (const SolverScalars *)c->solver, force);
         hipLaunchKernelGGL(k_add_correction_save, RL.grid, RL.block, 0, c->stream, t.x, xsave,
-                           (const float *)V[9], pg, g, (const OuterState *)c->outer_d);
+                           (const float *)V[9], pg, g, (const OuterState *)c->outer_d,
+                           (const SolverScalars *)c->solver);
         decide_and_start();
     }
 
@@ -2120,8 +2135,7 @@ int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, 
             const Interior PL = resid_launch(pg);
             const int npl = (int)(PL.grid.x * PL.grid.y * PL.grid.z);
             hipLaunchKernelGGL(k_inner_init, PL.grid, PL.block, 0, mc->stream, M[q].r64, M[q].V[0],
-                               M[q].V[1], M[q].V[2], M[q].V[3], M[q].V[9], g, mc->dscal + 41,
-                               mc->partials, (long long)npl);
+                               M[q].V[1], g, mc->dscal + 41, mc->partials, (long long)npl);
             double rtol = inner_rtol;
             const double need_rel = 0.5 * tol * M[q].sc_err / (sqrt(rr) > 0 ? sqrt(rr) : 1.0);
             if (need_rel > rtol) rtol = need_rel;
@@ -2159,7 +2173,7 @@ int bicgstab_mixed_batch(dcm_ctx *c, const SolveTask *tasks, int n, double tol, 
             DCM_HIP(hipMemcpyAsync(M[q].xsave, tasks[q].x, pg.ntot * sizeof(double),
                                    hipMemcpyDeviceToDevice, mc->stream));
             hipLaunchKernelGGL(k_add_correction, L.grid, L.block, 0, mc->stream, tasks[q].x,
-                               M[q].V[9], pg, g);
+                               M[q].V[9], pg, g, (const SolverScalars *)mc->solver);
             DCM_LAUNCH_CHECK();
         }
     }

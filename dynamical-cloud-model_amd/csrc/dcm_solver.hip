@@ -1067,10 +1067,10 @@ __global__ void k_outer_gate(OuterState *os, const SolverScalars *sc, int force)
     if (threadIdx.x != 0) return;
     os->go = os->active && (force || halted(sc));
 }
-// x += d with the previous x kept (the interior; the halo is not touched
-// until the solve ends); d = 0 when the inner solve made no update
-__global__ void k_add_correction_save(double *x, double *xsave, const float *d, Geo pg, CGeo g,
-                                      const OuterState *gate, const SolverScalars *sc) {
+// x += d on the interior (the halo is not touched until the solve ends);
+// d = 0 when the inner solve made no update.  Gated on the refinement step.
+__global__ void k_add_correction_gated(double *x, const float *d, Geo pg, CGeo g,
+                                       const OuterState *gate, const SolverScalars *sc) {
     if (!gate->go) return;
     const bool none = !sc->dvalid;
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -1082,18 +1082,18 @@ __global__ void k_add_correction_save(double *x, double *xsave, const float *d, 
     int c = i0 * SI + j * (int)pg.sj + k;
     int q = (i0 - 1) * CS + (j - 1) * g.kp + (k - 1);
     for (int i = i0; i <= i1; ++i, c += SI, q += CS) {
-        const double xv = x[c];
-        xsave[c] = xv;
-        x[c] = xv + (none ? 0.0 : (double)d[q]);
+        x[c] = x[c] + (none ? 0.0 : (double)d[q]);
     }
 }
-__global__ void k_restore_interior(double *x, const double *xsave, Geo pg, CGeo g) {
+// x -= d on the interior: the iterate before the last correction (to the
+// rounding of x + d - d), when that correction did not reduce the residual
+__global__ void k_undo_correction(double *x, const float *d, Geo pg, CGeo g,
+                                  const SolverScalars *sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 1;
-    if (k > g.kp || j > g.jp) return;
-    const long long c = pidx(pg, k, j, i);
-    x[c] = xsave[c];
+    if (k > g.kp || j > g.jp || !sc->dvalid) return;
+    x[pidx(pg, k, j, i)] -= (double)d[cidx(g, k, j, i)];
 }
 // the decision of the host loop from the true residual's sums
 // (rr = sums[0], bb = sums[2]): stop (converged, fallback, failure) or start
@@ -1837,7 +1837,7 @@ struct ChunkedSolve {
     Interior IL{}, RL{};
     int nbr = 0;
     float *V[10] = {};
-    double *r64 = nullptr, *xsave = nullptr;
+    double *r64 = nullptr;
     SolvePlan P;
     int seen_outer = 1, launched = 0, cap = 0;
     OuterState h{};
@@ -1865,7 +1865,7 @@ struct ChunkedSolve {
     void step(int force) {
         hipLaunchKernelGGL(k_outer_gate, dim3(1), dim3(64), 0, c->stream, c->outer_d,
                            (const SolverScalars *)c->solver, force);
-        hipLaunchKernelGGL(k_add_correction_save, RL.grid, RL.block, 0, c->stream, t.x, xsave,
+        hipLaunchKernelGGL(k_add_correction_gated, RL.grid, RL.block, 0, c->stream, t.x,
                            (const float *)V[9], pg, g, (const OuterState *)c->outer_d,
                            (const SolverScalars *)c->solver);
         decide_and_start();
@@ -1888,7 +1888,6 @@ struct ChunkedSolve {
         const size_t cells = (size_t)pg.ip * pg.jp * pg.kp;
         for (int v = 0; v < 10; ++v) V[v] = c->sv32 + (size_t)v * cells;
         DCM_CHECK(work_field64(c, 2, &r64));
-        DCM_CHECK(work_field64(c, 1, &xsave));
         SolveSys sy;
         sy.c = c;
         for (int v = 0; v < 10; ++v) sy.V[v] = V[v];
@@ -1899,7 +1898,7 @@ struct ChunkedSolve {
         decide_and_start();
         DCM_LAUNCH_CHECK();
         const std::vector<const void *> ptrs = {P.args.mem, c->mg, c->partials, t.x, t.b,
-                                                r64, xsave, c->outer_d, c->solver, c->dscal,
+                                                r64, c->outer_d, c->solver, c->dscal,
                                                 c->sv32};
         const double scal[3] = {tol, inner_rtol, (double)itmax};
         DCM_CHECK(chunk_graph(c, P, L, ptrs, scal, 3, [&]() { step(0); }));
@@ -1941,9 +1940,9 @@ struct ChunkedSolve {
         *status = h.status;
         if (h.fallback) {
             // the last FP32 correction did not reduce the true residual:
-            // restore x and let the reference's FP64 solver finish
-            hipLaunchKernelGGL(k_restore_interior, IL.grid, IL.block, 0, c->stream, t.x,
-                               (const double *)xsave, pg, g);
+            // take it back and let the reference's FP64 solver finish
+            hipLaunchKernelGGL(k_undo_correction, IL.grid, IL.block, 0, c->stream, t.x,
+                               (const float *)V[9], pg, g, (const SolverScalars *)c->solver);
             DCM_LAUNCH_CHECK();
             SolverOut o;
             int st = DCM_OK;

@@ -828,7 +828,7 @@ __global__ __launch_bounds__(256) void k_jac_b(IterArgs A, int si_, int ti, int 
 // that applies the scalar recurrence in place was measured 1.6x slower
 // overall: every block's device-scope release fence writes back its XCD's
 // L2.  One-block kernels reduce instead.)
-template <typename R, bool TWO>
+template <typename R, bool TWO, bool STORE = true>
 __device__ __forceinline__ void matvec_dot_body(gp<const R> __restrict__ x, gp<const R> __restrict__ a,
                                                 gp<R> __restrict__ y, const MG &m,
                                                 const CoefPack &cp,
@@ -868,7 +868,7 @@ __device__ __forceinline__ void matvec_dot_body(gp<const R> __restrict__ x, gp<c
             const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
             const R yv = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu,
                                     xc);
-            y[cc] = yv;
+            if (STORE) y[cc] = yv;
             if (TWO) {
                 acc[0] += (double)yv * (double)yv;  // tt = <t,t>
                 acc[1] += (double)yv * (double)ac;  // ts = <t,s>
@@ -891,14 +891,14 @@ __device__ __forceinline__ void matvec_dot_body(gp<const R> __restrict__ x, gp<c
     block_sum_store_at<2>(acc, partials, stride, T.tile);
 }
 
-template <typename R, bool TWO>
+template <typename R, bool TWO, bool STORE = true>
 __global__ __launch_bounds__(256) void k_matvec_dot(const R *__restrict__ x,
                                                     const R *__restrict__ a, R *__restrict__ y,
                                                     MG m, CoefPack cp,
                                                     const SolverScalars *__restrict__ sc,
                                                     double *partials) {
-    matvec_dot_body<R, TWO>(glob(x), glob(a), glob(y), m, cp, glob(sc), partials,
-                            march_tile_id(m, (int)blockIdx.x));
+    matvec_dot_body<R, TWO, STORE>(glob(x), glob(a), glob(y), m, cp, glob(sc), partials,
+                                   march_tile_id(m, (int)blockIdx.x));
 }
 template <typename R, bool TWO>
 __global__ __launch_bounds__(256) void k_matvec_dot_b(IterArgs A, int xi, int ai, int yi) {
@@ -946,6 +946,72 @@ __global__ __launch_bounds__(256) void k_update(R *__restrict__ x, const R *__re
     update_body<R>(glob(x), glob(p), glob(cs), glob(s), glob(t), glob(cr), glob(r), n, nbu,
                    glob(sc), partials, (int)blockIdx.x);
 }
+// The update of a single system with t = A cs recomputed instead of read
+// (k_matvec_dot<R, true, false> computed <t,t> and <t,s> without storing t):
+// marching tiles like the matvec, the same stencil expression, so the same t;
+// x = x + alf*p + omg*cs, r = s - omg*t, partial sums of <r,r> and <r,cr>
+// per tile (dynamics.f90:409, 419-423)
+template <typename R>
+__global__ __launch_bounds__(256) void k_update_march(R *__restrict__ x, const R *__restrict__ p,
+                                                      const R *__restrict__ cs,
+                                                      const R *__restrict__ s,
+                                                      const R *__restrict__ cr,
+                                                      R *__restrict__ r, MG m, CoefPack cp,
+                                                      const SolverScalars *__restrict__ sc,
+                                                      double *partials) {
+    if (halted(sc)) return;
+    Tile T;
+    if (!march_tile(m, march_tile_id(m, (int)blockIdx.x), T)) return;
+    double acc[2] = {0.0, 0.0};
+    if (T.act) {
+        const R alf = (R)sc->alf, omg = (R)sc->omg;
+        const bool xz = sc->guard && sc->iters == 1;  // as update_body
+        const Coef<R> C(cp.m64, cp.m32);
+        const int k = T.k, j = T.j;
+        const Cols q = cols(m, k, j);
+        const R as = C.as[j], an = C.an[j], ad = C.ad[k], au = C.au[k];
+        const int iw = T.i0 == 1 ? m.ip : T.i0 - 1;
+        int cc = (T.i0 - 1) * m.si + q.base;
+        int ie = T.i0 == m.ip ? 1 : T.i0 + 1;
+        int ce = (ie - 1) * m.si + q.base;
+        R xw = cs[(iw - 1) * m.si + q.base];
+        R xc = cs[cc], xe = cs[ce];
+        R xs = cs[cc + q.ds], xn = cs[cc + q.dn], xd = cs[cc + q.dd], xu = cs[cc + q.du];
+        for (int i = T.i0; i <= T.i1; ++i) {
+            R nxe = R(0), nxs = R(0), nxn = R(0), nxd = R(0), nxu = R(0);
+            int nie = ie, nce = ce;
+            if (i < T.i1) {
+                nie = ie == m.ip ? 1 : ie + 1;
+                nce = (nie - 1) * m.si + q.base;
+                nxe = cs[nce];
+                nxs = cs[ce + q.ds];
+                nxn = cs[ce + q.dn];
+                nxd = cs[ce + q.dd];
+                nxu = cs[ce + q.du];
+            }
+            const R ap = coef_ap_i(C, (-C.ae[i]) - C.aw[i], k, j);
+            const R tv = stencil<R>(C.aw[i], C.ae[i], as, an, ad, au, ap, xw, xe, xs, xn, xd, xu,
+                                    xc);
+            x[cc] = ((xz ? R(0) : x[cc]) + alf * p[cc]) + omg * xc;
+            const R rv = s[cc] - omg * tv;
+            r[cc] = rv;
+            acc[0] += (double)rv * (double)rv;
+            acc[1] += (double)rv * (double)cr[cc];
+            xw = xc;
+            xc = xe;
+            xe = nxe;
+            xs = nxs;
+            xn = nxn;
+            xd = nxd;
+            xu = nxu;
+            cc = ce;
+            ce = nce;
+            ie = nie;
+        }
+    }
+    block_sum_store_at<2>(acc, partials, m.nblocks, T.tile);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_update_b(IterArgs A) {
     int lb;
@@ -1419,8 +1485,30 @@ struct IterLaunch {
         else
             launch_timed(c, "update", k_update_b<R>, dim3(P.ugrid), dim3(kBlock), P.args);
     }
-    // 0: alf (<cr,v1>), 1: omg (<t,t>, <t,s>), 2: the residual (<r,r>, <r,cr>)
-    void fin(int which) const {
+    // the multigrid iteration's t = A cs, <t,t>, <t,s> without storing t, and
+    // its update recomputing t (per-tile partial sums); with sequential sums,
+    // which read t back, the stored t and the streaming update
+    void matvec_t_mg() const {
+        if (seq) {
+            matvec<true>(VCS, VS, VT);
+            return;
+        }
+        launch_timed(c, "matvec_dot", k_matvec_dot<R, true, false>, dim3(P.mgrid), dim3(kBlock),
+                     (const R *)v(VCS), (const R *)v(VS), v(VT), M().m, M().cp,
+                     (const SolverScalars *)M().sc, M().part);
+    }
+    void update_mg() const {
+        if (seq) {
+            update();
+            return;
+        }
+        launch_timed(c, "update", k_update_march<R>, dim3(P.mgrid), dim3(kBlock), v(VX),
+                     (const R *)v(VP), (const R *)v(VCS), (const R *)v(VS), (const R *)v(VCR),
+                     v(VR), M().m, M().cp, (const SolverScalars *)M().sc, M().part);
+    }
+    // 0: alf (<cr,v1>), 1: omg (<t,t>, <t,s>), 2: the residual (<r,r>, <r,cr>);
+    // tiles: the residual's partial sums are per marching tile (update_mg)
+    void fin(int which, bool tiles = false) const {
         if (seq) {
             const int a0 = which == 0 ? VCR : (which == 1 ? VT : VR);
             const int b0 = which == 0 ? VV1 : (which == 1 ? VT : VR);
@@ -1430,7 +1518,7 @@ struct IterLaunch {
                                a1, b1);
         }
         if (one()) {
-            const int nb = seq ? 1 : (which == 2 ? M().nbu : M().m.nblocks);
+            const int nb = seq ? 1 : (which == 2 && !tiles ? M().nbu : M().m.nblocks);
             if (which == 0)
                 hipLaunchKernelGGL(k_fin_v1, dim3(1), dim3(1024), 0, c->stream, M().part, nb,
                                    M().sc);
@@ -1491,10 +1579,10 @@ struct IterLaunch {
             fin(0);
             vcycle_start<1>(VS);             // s = r - alf*v1, two sweeps
             vcycle_rest<false>(VS, VCS);     // cs = V(s)
-            matvec<true>(VCS, VS, VT);       // t = A cs, <t,t>, <t,s>
+            matvec_t_mg();                   // t = A cs, <t,t>, <t,s>
             fin(1);
-            update();                        // x, r; <r,r>, <r,cr>
-            fin(2);
+            update_mg();                     // x, r (t recomputed); <r,r>, <r,cr>
+            fin(2, true);
         }
     }
     // dynamics.f90:391-425

@@ -872,8 +872,8 @@ def main(argv=None):
                          "sequential FP64, --steps timed steps")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
-              "whole domain lives on one MI355X.", file=sys.stderr)
+        print(f"bench.py: the library runs a domain on exactly one GPU (asked for "
+              f"{args.gpus}); the whole domain lives on one MI355X.", file=sys.stderr)
         return 2
 
     from dcm_amd import hip as H

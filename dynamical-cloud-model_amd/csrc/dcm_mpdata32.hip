@@ -23,6 +23,10 @@
 //     planes, keeping the column sums of planes i-1, i, i+1 in registers
 //     (the flux through the lower x face of a cell is the upper x face's
 //     flux of the cell before it);
+//   * in the donor and antidiffusive-velocity passes a lane takes the k+1
+//     and k-1 values of the arrays it has loaded from the adjacent lanes of
+//     the wavefront (DPP shifts), so only the first and last lane of a
+//     column segment load them;
 //   * each pass writes, besides a cell, the halo cells that the reference's
 //     exchange would copy from it: no fill kernel runs;
 //   * the limiter factors never reach HBM: the last kernel computes them one
@@ -174,6 +178,39 @@ __device__ __forceinline__ float min3f(float a, float b, float c) {
     return fminf(fminf(a, b), c);
 }
 __device__ __forceinline__ int wrapi(int x, int n) { return x < 1 ? x + n : (x > n ? x - n : x); }
+
+// The lanes of a marching tile hold consecutive levels of a column (lane l:
+// level k0 + l), so the value at k+1 of an array a lane has loaded at k is
+// the next lane's and the value at k-1 the previous lane's: DPP wave shifts
+// (wave_shl:1 gives lane l the value of lane l+1, wave_shr:1 that of lane
+// l-1) replace those loads.  A lane whose neighbour holds another column or
+// no level (the first and last lane of a column segment, the level kp) gets
+// 0 here and loads the value itself.  Every lane of the tile must execute
+// the shift, outside any branch that depends on the lane.
+__device__ __forceinline__ int dpp_next(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_prev(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(dpp_next(__float_as_int(v)));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(dpp_prev(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_next(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_next((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_prev(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_prev((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // A uniform base pointer read at a 32-bit byte offset: the load takes the
 // base from scalar registers and the offset from one vector register, so a
@@ -375,6 +412,9 @@ __global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T) {
         const float zb = (float)(A.lbc[f] - mm), zt = (float)(A.ubc[f] - mm);
         const int nm = A.neumann;
         const bool jedge = j == 1 || j == m.jp;
+        // lanes that load their k-1 / k+1 values (see lane_next)
+        const int lk = (int)threadIdx.x % m.kt;
+        const bool fix_lo = lk == 0, fix_hi = lk == m.kt - 1 || k == m.kp;
         int c = C.i0 * SI + j * SJ + k;
         int c4 = C.i0 * SI4 + j * SJ4 + k + kK0;
         unsigned o4 = (unsigned)c4 * 4u, o8 = (unsigned)c * 8u;
@@ -384,12 +424,24 @@ __global__ __launch_bounds__(256) void k32_donor1(Mp32Args A, PM m, Tab32 T) {
         const float uw = u[c4 - SI4];
         float fxl = pos0(uw) * pw + neg0(uw) * pc;
         for (int i = C.i0; i <= C.i1; ++i) {
+            double xu = 0.0, xd = 0.0;
+            float xw = 0.f;
+            if (fix_lo) {
+                xd = Pc.at(o8, -1);
+                xw = Wc.at(o4, -1);
+            }
+            if (fix_hi) xu = Pc.at(o8, 1);
             const double de = Pe.at(o8) - mm;
             const float pe = (float)de;
             const float pn = (float)(Pn.at(o8) - mm), ps = (float)(Ps.at(o8) - mm);
-            const float pu = (float)(Pc.at(o8, 1) - mm), pd = (float)(Pc.at(o8, -1) - mm);
-            const float uc = Uc.at(o4), vc = Vc.at(o4), vl = Vs.at(o4), wc = Wc.at(o4),
-                        wl = Wc.at(o4, -1);
+            const float uc = Uc.at(o4), vc = Vc.at(o4), vl = Vs.at(o4), wc = Wc.at(o4);
+            // psi - m at k+1 and k-1 of plane i, w at k-1: the centre values
+            // of the adjacent lanes
+            const double su = lane_next(dc), sd = lane_prev(dc);
+            const float sw = lane_prev(wc);
+            const float pu = (float)(fix_hi ? xu - mm : su);
+            const float pd = (float)(fix_lo ? xd - mm : sd);
+            const float wl = fix_lo ? xw : sw;
             const float fxr = pos0(uc) * pc + neg0(uc) * pe;
             const float fyr = pos0(vc) * pc + neg0(vc) * pn;
             const float fyl = pos0(vl) * ps + neg0(vl) * pc;
@@ -500,27 +552,57 @@ __global__ __launch_bounds__(256) void k32_antidiff(Mp32Args A, PM m, Tab32 T) {
     const float qg1 = q * ig1, qg3 = q * ig3;
     const float wfac = dt * (rak * ig3);
     const bool jedge = j == 1 || j == m.jp;
+    // lanes that load their k-1 / k+1 values (see lane_next)
+    const int lk = (int)threadIdx.x % m.kt;
+    const bool fix_lo = lk == 0, fix_hi = lk == m.kt - 1 || k == m.kp;
     int c = C.i0 * SI + j * SJ + k + kK0;
     unsigned o4 = (unsigned)c * 4u;
     Plane pw, pc, pn;
     plane_at(pw, p1, u, v, w, c - SI, SJ, rnk, rnk1);
     plane_at(pc, p1, u, v, w, c, SJ, rnk, rnk1);
     for (int i = C.i0; i <= C.i1; ++i) {
-        pn.A = nP.at(o4, 1);
-        pn.B = nP.at(o4, -1);
+        // k+1 values (x*) and k-1 values (y*) of the lanes that load them
+        float xa = 0.f, xu = 0.f, xkj = 0.f, xkm = 0.f, xvk = 0.f, xvkm = 0.f, xwk = 0.f;
+        float yb = 0.f, ywm = 0.f, ymj = 0.f, ywmj = 0.f;
+        if (fix_hi) {
+            xa = nP.at(o4, 1);
+            xu = nU.at(o4, 1);
+            xkj = Pj.at(o4, 1);
+            xkm = Pm.at(o4, 1);
+            xvk = Vc.at(o4, 1);
+            xvkm = Vm.at(o4, 1);
+            xwk = Wc.at(o4, 1);
+        }
+        if (fix_lo) {
+            yb = nP.at(o4, -1);
+            ywm = nW.at(o4, -1);
+            ymj = Pj.at(o4, -1);
+            ywmj = Wj.at(o4, -1);
+        }
         pn.Cn = nPn.at(o4);
         pn.Dm = nPs.at(o4);
         pn.O = nP.at(o4);
         pn.U0 = nU.at(o4);
-        pn.Uy = nUj.at(o4) + pn.U0;
-        pn.Uz = rnk1 * nU.at(o4, 1) + rnk * pn.U0;
+        const float uj = nUj.at(o4);
         pn.V0 = nV.at(o4);
         pn.Vm = nVm.at(o4);
         pn.W0 = nW.at(o4);
-        pn.Wm = nW.at(o4, -1);
-        const float pkj = Pj.at(o4, 1), pmj = Pj.at(o4, -1), pkm = Pm.at(o4, 1);
-        const float vk = Vc.at(o4, 1), vkm = Vm.at(o4, 1), vj = Vj.at(o4);
-        const float wj = Wj.at(o4), wmj = Wj.at(o4, -1), wk = Wc.at(o4, 1);
+        const float vj = Vj.at(o4), wj = Wj.at(o4);
+        // the values at k+1 and k-1: plane i+1's from this step's centre
+        // loads, plane i's from the previous step's (pc)
+        const float sa = lane_next(pn.O), sb = lane_prev(pn.O), su = lane_next(pn.U0),
+                    swm = lane_prev(pn.W0);
+        const float skj = lane_next(pc.Cn), smj = lane_prev(pc.Cn), skm = lane_next(pc.Dm);
+        const float svk = lane_next(pc.V0), svkm = lane_next(pc.Vm), swk = lane_next(pc.W0),
+                    swmj = lane_prev(wj);
+        pn.A = fix_hi ? xa : sa;
+        pn.B = fix_lo ? yb : sb;
+        pn.Uy = uj + pn.U0;
+        pn.Uz = rnk1 * (fix_hi ? xu : su) + rnk * pn.U0;
+        pn.Wm = fix_lo ? ywm : swm;
+        const float pkj = fix_hi ? xkj : skj, pmj = fix_lo ? ymj : smj, pkm = fix_hi ? xkm : skm;
+        const float vk = fix_hi ? xvk : svk, vkm = fix_hi ? xvkm : svkm;
+        const float wmj = fix_lo ? ywmj : swmj, wk = fix_hi ? xwk : swk;
         const float4 xt = T.xpk[i];
         const float dxi = xt.x, idxni = xt.y, idxim = xt.z, ihx = xt.w;
         // column sums

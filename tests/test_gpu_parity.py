@@ -434,6 +434,28 @@ def test_model_advection_scheme_mixed(scheme):
         assert rel_l2(a, b) < 1e-5
 
 
+@pytest.mark.parametrize("shape", [(48, 40, 24), (40, 24, 40), (32, 24, 72)])
+def test_mixed_mpdata_column_segments_track_fp64(shape):
+    """The batched FP32 MPDATA kernels take the k+1 and k-1 values of a
+    column from the adjacent lanes of the wavefront and load them where the
+    adjacent lane holds another column or no level: columns of 24 levels
+    (two columns per wavefront), of 40 (a partly filled wavefront) and of 72
+    (a second, partly filled segment).  After 4 steps every advected field of
+    the mixed path is within 1e-5 of the FP64 path's."""
+    case = W.moist_bubble(*shape, ntracer=3, steps=4)
+    out = {}
+    for prec in (H.FP64, H.MIXED):
+        with H.Context(case.dom) as c:
+            gm = H.Model(c, case, prec)
+            st = gm.step(4)
+            assert st.max_true_resid < 1e-9
+            out[prec] = [gm.get(H.F_TH), gm.get(H.F_U), gm.get(H.F_W)]
+            out[prec] += [gm.get(H.F_Q, n) for n in range(case.nq)]
+            gm.close()
+    for n, (a, b) in enumerate(zip(out[H.FP64], out[H.MIXED])):
+        assert rel_l2(a, b) < 1e-5, n
+
+
 @pytest.mark.parametrize("make", [lambda: W.warm_bubble(16, 16, 16, steps=4),
                                   lambda: W.kelvin_helmholtz(24, 4, 16, steps=4)])
 def test_model_tree_sums_within_solver_tolerance(make):

@@ -853,8 +853,9 @@ def main(argv=None):
                     help="IPxJP:STEPS of the CPU baseline (the oracle on the host's cores)")
     ap.add_argument("--no-fp64-multigrid", action="store_true",
                     help="skip the FP64 path with the V-cycle")
-    ap.add_argument("--kh-steps", type=int, default=100,
-                    help="timed steps per path of the Kelvin-Helmholtz leg (0: skip)")
+    ap.add_argument("--kh-steps", type=int, default=998,
+                    help="timed steps per path of the Kelvin-Helmholtz leg after 2 warm-up "
+                         "steps (default: the 1000 steps of the shear-layer run; 0: skip)")
     ap.add_argument("--large-steps", type=int, default=3,
                     help="timed steps per path of the 512x512x256 leg (0: skip)")
     ap.add_argument("--no-stream", action="store_true")
